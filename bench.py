@@ -1,0 +1,220 @@
+"""Throughput of the operator-split reaction-diffusion period on one MI355X.
+
+A step is one operator-split period (SSA + diffusion leaving + entering) over
+every tet of the workload, with inputs already resident in HBM. The default
+workload is the 4-species / 6-reaction model on a 40^3 x 6 Kuhn cube of side
+40 um (384,000 tets) at the diffusion stability bound (2.96 ms per period);
+338 steps are 1 s of simulated time. The JSON line also carries the batched
+ensemble (1,024 realizations of A + B <-> C on cubes of 1,296..48,000 tets,
+0.1 s each) and the same A + B <-> C model run as a single realization, so
+batched and plain throughput are measured on the same GPU in the same run.
+
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from steps_amd import workloads as W  # noqa: E402
+from steps_amd.engine import Engine  # noqa: E402
+
+METRIC = "tet-steps/sec (batched vs plain, same GPU) + achieved HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def interior_faces_per_tet(mt) -> float:
+    return float((mt.nbr >= 0).sum()) / len(mt.vol)
+
+
+def bytes_per_tet_step(S, S_d, S_o, Fbar, T_over_N=0.0, S_s=0) -> float:
+    """Compulsory HBM bytes per tet-step with state round-tripping HBM every
+    period: int32 counts r/w, int32 neighbour + 8-bit face slot, FP64 face
+    factor, int32 outgoing slot written and read, FP64 occupancy r/w for
+    species that react and diffuse, membrane triangle terms."""
+    return (8 * S + 5 * Fbar + 8 * Fbar + 8 * S_d * Fbar + 16 * S_o
+            + T_over_N * (8 * S_s + 12))
+
+
+def model_bytes(w) -> float:
+    fm = w.fm
+    S = len(fm.specs)
+    diff = fm.spec_dcst > 0
+    reacts = (fm.reac_upd != 0).any(axis=0)
+    if len(fm.sreacs):
+        reacts |= (fm.sreac_upd_i != 0).any(axis=0)
+    S_d = int(diff.sum())
+    S_o = int((diff & reacts).sum())
+    N = w.n_tets
+    T = len(w.mt.tri_tet)
+    return bytes_per_tet_step(S, S_d, S_o, interior_faces_per_tet(w.mt), T / N, len(fm.sspecs))
+
+
+def load_engine(w, seed, block=256, graph=False):
+    eng = Engine(w.fm, [w.mt], [seed])
+    for s in range(len(w.fm.specs)):
+        eng.set_tet_counts(s, w.counts[s])
+    for s in range(len(w.fm.sspecs)):
+        eng.set_tri_counts(s, w.tri_counts[s])
+    eng.set_launch(block, graph)
+    return eng
+
+
+def time_plain(w, steps, warmup, seed=1, block=256, graph=False):
+    eng = load_engine(w, seed, block, graph)
+    dt = eng.update_period()
+    if warmup:
+        eng.run(warmup * dt)
+    n0, t0 = eng.nsteps(), eng.time()
+    tic = time.perf_counter()
+    eng.run(t0 + steps * dt)  # returns after its stream has drained
+    wall = time.perf_counter() - tic
+    periods = eng.nsteps() - n0
+    ms_launch, launches = eng.last_run_kernel_ms()
+    return dict(eng=eng, wall=wall, periods=periods, tet_steps=periods * w.n_tets,
+                ms_launch=ms_launch, launches=launches, dt=dt)
+
+
+def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=256, graph=True):
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=n_real, seed=seed,
+                                                             end_time=end_time)
+    eng = Engine(fm, meshes, [seed + 1] * n_real, reac_k=reac_k)
+    for r in range(n_real):
+        for s in range(3):
+            eng.set_tet_counts(s, counts[r][s], real=r)
+    eng.set_launch(block, graph)
+    # warm-up pass (captures and instantiates the graph), then the timed pass
+    eng.run(end_time)
+    eng.reset()
+    for r in range(n_real):
+        for s in range(3):
+            eng.set_tet_counts(s, counts[r][s], real=r)
+    tic = time.perf_counter()
+    eng.run(end_time)
+    wall = time.perf_counter() - tic
+    ms_launch, launches = eng.last_run_kernel_ms()
+    tet_steps = sum(eng.nsteps(r) * len(meshes[r].vol) for r in range(n_real))
+    tot_tets = sum(len(m.vol) for m in meshes)
+    fb = np.mean([interior_faces_per_tet(m) for m in meshes])
+    btet = bytes_per_tet_step(3, 3, 3, fb)
+    # conservation check over the whole batch
+    ok = True
+    for r in range(n_real):
+        tot = [eng.sum_tet_counts(s, r) for s in range(3)]
+        ok &= (tot[0] + tot[2] == counts[r][0].sum() + counts[r][2].sum())
+    return dict(wall=wall, tet_steps=tet_steps, n_tets=tot_tets, ms_launch=ms_launch,
+                launches=launches, bytes_per_tet_step=btet, conserved=bool(ok),
+                max_periods=max(eng.nsteps(r) for r in range(n_real)))
+
+
+def cpu_baseline(w, budget_s=15.0, seed=1):
+    """The CPU oracle on this host, a bounded number of periods of the same workload."""
+    from oracle import bind
+
+    o = bind.OracleSim(w.fm, w.mt, seed=seed)
+    o.counts[:] = w.counts
+    if len(w.fm.sspecs):
+        o.tri_counts[:] = w.tri_counts
+    dt = o.update_period()
+    tic = time.perf_counter()
+    o.run_periods([dt])
+    one = time.perf_counter() - tic
+    n = max(1, int(budget_s / max(one, 1e-6)))
+    tic = time.perf_counter()
+    o.run_periods([dt] * n)
+    wall = time.perf_counter() - tic
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=n * w.n_tets / wall, unit="tet-steps/s", cores=threads, kind="port",
+                sample=f"{n} periods x {w.n_tets} tets of the same workload on the CPU oracle "
+                       f"(OpenMP, {threads} threads), {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=338)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the whole mesh is resident on a single MI355X; "
+              "multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    w = W.four_species_cube(n=40, side=40e-6, seed=0)
+    btet = model_bytes(w)
+    p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph)
+    value = p["tet_steps"] / p["wall"]
+    # roofline of the period kernel: algorithmic bytes per launch / mean launch time
+    achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "tet-steps/s",
+        "n_gpus": 1,
+        "steps": p["periods"],
+        "warmup": args.warmup,
+        "ms_per_step": p["wall"] * 1e3 / max(p["periods"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32 counts, f64 propensities",
+        "data": "synthetic",
+        "config": {
+            "workload": "four_species_cube: 40^3 x 6 Kuhn cube, side 40 um, 384,000 tets, "
+                        "4 species / 6 reactions, Poisson(50) initial, dt at the stability bound",
+            "n_tets": w.n_tets,
+            "period_s": p["dt"],
+            "simulated_s": p["periods"] * p["dt"],
+            "block": args.block,
+            "graph": bool(args.graph),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_tet_step": btet,
+            "kernel_ms_per_launch": p["ms_launch"],
+        },
+    }
+    if not args.no_batched:
+        small = W.ab_binding_cube(n=13, side=10e-6, seed=0)
+        ps = time_plain(small, 9, 1)
+        ens = time_ensemble()
+        out["batched"] = {
+            "workload": "ab_binding_ensemble: 1,024 realizations, Kuhn cubes n=6..20 "
+                        "(1,296..48,000 tets), kf/kb log-uniform +-1 decade, 0.1 s each",
+            "tet_steps_per_s": ens["tet_steps"] / ens["wall"],
+            "n_tets": ens["n_tets"],
+            "kernel_ms_per_launch": ens["ms_launch"],
+            "launches": ens["launches"],
+            "achieved_gbs": ens["bytes_per_tet_step"] * ens["tet_steps"] /
+                            (ens["ms_launch"] * ens["launches"] * 1e-3) / 1e9,
+            "conserved": ens["conserved"],
+            "plain_same_model_tet_steps_per_s": ps["tet_steps"] / ps["wall"],
+            "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
+        }
+        out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
+                                                out["batched"]["plain_same_model_tet_steps_per_s"])
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/*
+ * stg -- C ABI of the MI355X operator-split stochastic reaction-diffusion
+ * engine (drop-in for the STEPS TetOpSplit / DistTetOpSplit solver engines).
+ *
+ * Every entry point returns an int status (STG_OK on success); on failure
+ * stg_last_error(sim) (or stg_last_error(NULL) before a simulation exists)
+ * returns a message. Status codes map one-to-one onto the reference's C++
+ * exception types (src/steps/util/error.hpp:161-246) so a host shim can
+ * rethrow steps::ArgErr, steps::NotImplErr, steps::ProgErr, steps::SysErr.
+ *
+ * All arrays are plain host pointers; the library copies what it needs at
+ * create time and keeps every per-simulation buffer resident on the current
+ * HIP device. One host thread per handle; calls are serialised on the
+ * handle's own HIP stream.
+ */
+#ifndef STG_H
+#define STG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STG_OK 0
+#define STG_ERR_ARG 1     /* steps::ArgErr     */
+#define STG_ERR_NOTIMPL 2 /* steps::NotImplErr */
+#define STG_ERR_PROG 3    /* steps::ProgErr    */
+#define STG_ERR_SYS 4     /* steps::SysErr     */
+#define STG_ERR_DEVICE 5  /* HIP runtime / device failure */
+
+/* Period schedule of stg_run: the last period clipped to the end time as in
+ * TetOpSplitP::_runWithoutEField (src/steps/mpi/tetopsplit/tetopsplit.cpp:1528-1560),
+ * or n-1 standard periods plus one or two remainders as in
+ * OmegaHSimulation::run_rd (src/steps/mpi/dist/tetopsplit/simulation.cpp:1358-1384). */
+#define STG_MODE_TETOPSPLIT 0
+#define STG_MODE_DISTTETOPSPLIT 1
+
+typedef struct stg_sim stg_sim;
+
+/* Tetrahedral mesh of one realization, whole mesh resident on one GPU.
+ * Face f of tet t is the face opposite its f-th vertex. Replaces the
+ * neighbour tables of TetOpSplitP::_setup (tetopsplit.cpp:645-690) and the
+ * DistMesh neighbour CSR (src/steps/geom/dist/distmesh.cpp:344-420). */
+typedef struct stg_mesh {
+    int32_t n_tets;
+    const double *tet_vol;       /* [n_tets] m^3 */
+    const int32_t *tet_nbr;      /* [n_tets*4] neighbour tet, -1 = none */
+    const int32_t *tet_nbr_face; /* [n_tets*4] face slot of this face in the neighbour */
+    const double *tet_face_g;    /* [n_tets*4] A_f / (V d_f) in m^-2, 0 = closed */
+    int32_t n_tris;              /* membrane triangles carrying surface reactions */
+    const int32_t *tri_tet;      /* [n_tris] inner tet */
+    const double *tri_area;      /* [n_tris] m^2 */
+} stg_mesh;
+
+/* Flattened model of one compartment (+ one patch), as the reference's
+ * Compdef/Patchdef tables (src/steps/solver/compdef.hpp:252-258, 420). */
+typedef struct stg_model {
+    int32_t n_specs;             /* volume species */
+    int32_t n_reacs;
+    const int32_t *reac_lhs;     /* [n_reacs*n_specs] reactant multiplicity, order <= 4 each */
+    const int32_t *reac_upd;     /* [n_reacs*n_specs] net change */
+    const double *reac_k;        /* [n_reacs] rate constant, SI (M^(1-order)/s) */
+    const double *spec_dcst;     /* [n_specs] diffusion constant m^2/s, 0 = immobile */
+    int32_t n_sspecs;            /* surface species */
+    int32_t n_sreacs;
+    const int32_t *sreac_lhs_s;  /* [n_sreacs*n_sspecs] */
+    const int32_t *sreac_upd_s;  /* [n_sreacs*n_sspecs] */
+    const int32_t *sreac_lhs_i;  /* [n_sreacs*n_specs] inner-compartment reactants */
+    const int32_t *sreac_upd_i;  /* [n_sreacs*n_specs] */
+    const double *sreac_k;       /* [n_sreacs] */
+} stg_model;
+
+/* Per-realization parameters of a batch; NULL arrays take the model's values. */
+typedef struct stg_realization {
+    uint64_t seed;               /* Philox key */
+    uint32_t stream;             /* Philox stream id (counter word 2) */
+    const double *reac_k;        /* [n_reacs] or NULL */
+    const double *sreac_k;       /* [n_sreacs] or NULL */
+    const double *spec_dcst;     /* [n_specs] or NULL */
+} stg_realization;
+
+/* Engine for one realization: replaces the TetOpSplitP constructor
+ * (src/steps/mpi/tetopsplit/tetopsplit.cpp:83-118, pysteps/cysteps_mpi.pyx:80-109)
+ * and the DistTetOpSplit constructor (pysteps/cysteps_dist.pyx:1581-1633). */
+int stg_create(const stg_model *model, const stg_mesh *mesh, uint64_t seed, int32_t mode,
+               stg_sim **out);
+
+/* n_real independent realizations (own mesh, constants, key) advanced by the
+ * same launches. No reference counterpart: it is the batched ensemble mode. */
+int stg_create_batch(const stg_model *model, int32_t n_real, const stg_mesh *meshes,
+                     const stg_realization *reals, int32_t mode, stg_sim **out);
+
+int stg_destroy(stg_sim *sim);
+
+/* Message of the last failed call on sim (or of the last failed create when
+ * sim is NULL). Never NULL. */
+const char *stg_last_error(const stg_sim *sim);
+
+/* TetOpSplitP::reset (tetopsplit.cpp:1436-1503): counts 0, time 0, steps 0. */
+int stg_reset(stg_sim *sim);
+
+/* TetOpSplitP::run (tetopsplit.cpp:1507-1526) / OmegaHSimulation::run
+ * (simulation.cpp:1418-1453): advance every realization to endtime. */
+int stg_run(stg_sim *sim, double endtime);
+
+/* TetOpSplitP::advance (tetopsplit.cpp:1956-1966). */
+int stg_advance(stg_sim *sim, double adv);
+
+/* Re-key one realization's Philox streams, as re-seeding the solver's RNG
+ * (RNG::initialize, src/steps/rng/rng.hpp) does for the reference. */
+int stg_set_seed(stg_sim *sim, int32_t real, uint64_t seed);
+
+/* API::getTime / setTime. */
+int stg_get_time(const stg_sim *sim, double *t);
+int stg_set_time(stg_sim *sim, double t);
+
+/* API::getNSteps: periods executed since reset, per realization. */
+int stg_get_nsteps(const stg_sim *sim, int32_t real, uint64_t *n);
+
+/* TetOpSplitP::getUpdPeriod (tetopsplit.cpp:5373-5411, getter at :8302-8328). */
+int stg_get_update_period(const stg_sim *sim, int32_t real, double *dt);
+
+/* TetOpSplitP::getReacExtent / getDiffExtent (tetopsplit.cpp:8302-8328):
+ * SSA events and molecules moved by diffusion since reset. */
+int stg_get_extents(const stg_sim *sim, int32_t real, uint64_t *reac_extent,
+                    uint64_t *diff_extent);
+
+/* Whole-species count arrays of one realization (n_tets or n_tris long):
+ * the bulk form of API::setTetSpecCount / getTetSpecCount and of
+ * getBatchTetSpecCountsNP (tetopsplit.cpp:5715-5778). */
+int stg_set_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int32_t *counts);
+int stg_get_tet_counts(stg_sim *sim, int32_t real, int32_t spec, int32_t *counts);
+int stg_set_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, const int32_t *counts);
+int stg_get_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int32_t *counts);
+
+/* getBatchTetSpecCountsNP (tetopsplit.cpp:5715-5778): counts of selected
+ * tets, gathered on the device. */
+int stg_get_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets,
+                             size_t n, double *counts);
+
+/* On-device totals of one species over a realization's tets (or triangles):
+ * the reduction behind getCompSpecCount / getPatchSpecCount
+ * (tetopsplit.cpp:2045-2110). */
+int stg_sum_tet_counts(stg_sim *sim, int32_t real, int32_t spec, int64_t *total);
+int stg_sum_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int64_t *total);
+
+/* setCompReacK / setPatchSReacK / setCompDiffD (api.hpp). A changed
+ * diffusion constant recomputes the update period, as recomputeUpdPeriod
+ * does in TetOpSplitP (tetopsplit.cpp:1515-1517). */
+int stg_set_reac_k(stg_sim *sim, int32_t real, int32_t reac, double k);
+int stg_set_sreac_k(stg_sim *sim, int32_t real, int32_t sreac, double k);
+int stg_set_dcst(stg_sim *sim, int32_t real, int32_t spec, double dcst);
+
+/* Launch shape used by stg_run: threads per block (64..1024, multiple of 64)
+ * and whether the launches of one run are replayed from a captured HIP graph.
+ * No reference counterpart (device tuning). */
+int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
+
+/* Number of realizations and tets / triangles of one realization. */
+int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_tets,
+                  int32_t *n_tris);
+
+/* Average duration (ms) of the period kernel over the launches of the last
+ * stg_run, timed with HIP events on the handle's stream; *n = launches. */
+int stg_last_run_kernel_ms(const stg_sim *sim, double *ms_per_launch, int32_t *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

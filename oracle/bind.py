@@ -1,0 +1,175 @@
+"""ctypes binding of the CPU oracle (oracle/stg_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py for the CPU baseline. The product never imports
+this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "_build" / "libstg_oracle.so"
+KAT_BIN = HERE / "_ref" / "philox_kat"
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class StgoSystem(C.Structure):
+    _fields_ = [
+        ("n_tets", C.c_int32), ("tet_vol", _dp), ("tet_nbr", _ip), ("tet_nbr_face", _ip),
+        ("tet_face_g", _dp), ("n_tris", C.c_int32), ("tri_tet", _ip), ("tri_area", _dp),
+        ("n_specs", C.c_int32), ("n_reacs", C.c_int32), ("n_sspecs", C.c_int32),
+        ("n_sreacs", C.c_int32), ("reac_lhs", _ip), ("reac_upd", _ip), ("reac_k", _dp),
+        ("spec_dcst", _dp), ("sreac_lhs_s", _ip), ("sreac_upd_s", _ip), ("sreac_lhs_i", _ip),
+        ("sreac_upd_i", _ip), ("sreac_k", _dp), ("counts", _ip), ("tri_counts", _ip),
+        ("seed", C.c_uint64), ("stream", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB.exists():
+        build()
+    lib = C.CDLL(str(LIB))
+    u32 = C.c_uint32
+    lib.stgo_philox4x32_8.argtypes = [C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    lib.stgo_philox4x32_8.restype = None
+    lib.stgo_log.argtypes = [C.c_double]
+    lib.stgo_log.restype = C.c_double
+    lib.stgo_sqrt.argtypes = [C.c_double]
+    lib.stgo_sqrt.restype = C.c_double
+    lib.stgo_lfact.argtypes = [C.c_int32]
+    lib.stgo_lfact.restype = C.c_double
+    lib.stgo_binom_stream.argtypes = [C.c_uint64, u32, u32, u32, u32, C.c_int32, C.c_double,
+                                      C.c_int32, _ip]
+    lib.stgo_binom_stream.restype = C.c_int32
+    lib.stgo_uniforms.argtypes = [C.c_uint64, u32, u32, u32, u32, C.c_int32, _dp]
+    lib.stgo_uniforms.restype = None
+    lib.stgo_schedule.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int32, _dp, C.c_int32]
+    lib.stgo_schedule.restype = C.c_int32
+    lib.stgo_update_period.argtypes = [C.POINTER(StgoSystem)]
+    lib.stgo_update_period.restype = C.c_double
+    lib.stgo_run.argtypes = [C.POINTER(StgoSystem), _dp, C.c_int32, u32, _ip,
+                             C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    lib.stgo_run.restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+def philox(ctr, key):
+    lib = load()
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib.stgo_philox4x32_8(c, k, o)
+    return [o[i] for i in range(4)]
+
+
+def binomials(seed, c0, c1, c2, channel, n, p, reps) -> np.ndarray:
+    out = np.empty(reps, dtype=np.int32)
+    load().stgo_binom_stream(seed, c0, c1, c2, channel, n, p, reps,
+                             out.ctypes.data_as(_ip))
+    return out
+
+
+def uniforms(seed, c0, c1, c2, channel, n) -> np.ndarray:
+    out = np.empty(n, dtype=np.float64)
+    load().stgo_uniforms(seed, c0, c1, c2, channel, n, out.ctypes.data_as(_dp))
+    return out
+
+
+def schedule(t0, t_end, dt_std, mode, cap=10_000_000) -> np.ndarray:
+    buf = np.empty(cap, dtype=np.float64)
+    n = load().stgo_schedule(t0, t_end, dt_std, mode, buf.ctypes.data_as(_dp), cap)
+    if n < 0:
+        raise ValueError(f"schedule failed ({n})")
+    return buf[:n].copy()
+
+
+def ref_philox_kat(lines: list[tuple[int, ...]]):
+    """Run the Random123-header generator compiled from the reference sources
+    (oracle/_ref/philox_kat) on (c0..c3, k0, k1) tuples; None if not built."""
+    if not KAT_BIN.exists():
+        return None
+    inp = "\n".join(" ".join(str(int(x)) for x in t) for t in lines) + "\n"
+    out = subprocess.run([str(KAT_BIN)], input=inp, capture_output=True, text=True, check=True)
+    return [tuple(int(x) for x in ln.split()) for ln in out.stdout.strip().splitlines()]
+
+
+class OracleSim:
+    """One realization on the CPU oracle, built from the same tables as the engine."""
+
+    def __init__(self, fm, mt, seed: int, stream: int = 0, reac_k=None, sreac_k=None,
+                 spec_dcst=None):
+        self.lib = load()
+        f64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))  # noqa: E731
+        i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32))  # noqa: E731
+        S, SS = len(fm.specs), len(fm.sspecs)
+        self.n_tets, self.n_tris = len(mt.vol), len(mt.tri_tet)
+        self.a = dict(
+            vol=f64(mt.vol), nbr=i32(mt.nbr), nf=i32(mt.nbr_face), g=f64(mt.face_g),
+            tt=i32(mt.tri_tet), ta=f64(mt.tri_area), rl=i32(fm.reac_lhs), ru=i32(fm.reac_upd),
+            rk=f64(fm.reac_k if reac_k is None else reac_k),
+            dc=f64(fm.spec_dcst if spec_dcst is None else spec_dcst),
+            sls=i32(fm.sreac_lhs_s), sus=i32(fm.sreac_upd_s), sli=i32(fm.sreac_lhs_i),
+            sui=i32(fm.sreac_upd_i), sk=f64(fm.sreac_k if sreac_k is None else sreac_k),
+            counts=np.zeros((S, self.n_tets), dtype=np.int32),
+            tcounts=np.zeros((max(SS, 1), max(self.n_tris, 1)), dtype=np.int32),
+        )
+        a = self.a
+        P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
+        self.sys = StgoSystem(
+            self.n_tets, P(a["vol"], C.c_double), P(a["nbr"], C.c_int32), P(a["nf"], C.c_int32),
+            P(a["g"], C.c_double), self.n_tris, P(a["tt"], C.c_int32), P(a["ta"], C.c_double),
+            S, len(fm.reacs), SS, len(fm.sreacs), P(a["rl"], C.c_int32), P(a["ru"], C.c_int32),
+            P(a["rk"], C.c_double), P(a["dc"], C.c_double), P(a["sls"], C.c_int32),
+            P(a["sus"], C.c_int32), P(a["sli"], C.c_int32), P(a["sui"], C.c_int32),
+            P(a["sk"], C.c_double), P(a["counts"], C.c_int32), P(a["tcounts"], C.c_int32),
+            seed, stream)
+        self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
+        self.nsteps = 0
+        self.time = 0.0
+        self.events = C.c_uint64(0)
+        self.moved = C.c_uint64(0)
+
+    @property
+    def counts(self) -> np.ndarray:
+        return self.a["counts"]
+
+    @property
+    def tri_counts(self) -> np.ndarray:
+        return self.a["tcounts"]
+
+    def update_period(self) -> float:
+        return self.lib.stgo_update_period(C.byref(self.sys))
+
+    def run_periods(self, dts) -> None:
+        d = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
+        if len(d) == 0:
+            return
+        rc = self.lib.stgo_run(C.byref(self.sys), d.ctypes.data_as(_dp), len(d), self.nsteps,
+                               self.scratch.ctypes.data_as(_ip), C.byref(self.events),
+                               C.byref(self.moved))
+        if rc != 0:
+            raise RuntimeError(f"oracle run failed ({rc})")
+        self.nsteps += len(d)
+
+    def run(self, endtime: float, mode: int = 0) -> None:
+        dts = schedule(self.time, endtime, self.update_period(), mode)
+        self.run_periods(dts)
+        self.time = endtime

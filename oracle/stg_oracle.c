@@ -1,0 +1,648 @@
+/*
+ * CPU oracle: operator-split stochastic reaction-diffusion period.
+ *
+ * TEST INFRASTRUCTURE ONLY -- the checker for the HIP path and the CPU
+ * baseline timed by bench.py. Product code never links or calls this file.
+ *
+ * What it restates (paths relative to the STEPS 5.0.4 source tree):
+ *   - per-group Direct SSA with independent kprocs, one group = one tet plus
+ *     its membrane triangles:
+ *       src/steps/mpi/dist/tetopsplit/operator/ssa_operator.cpp:24-47
+ *       src/steps/mpi/dist/tetopsplit/kproc/propensities.hpp:543-586 (drawEvent)
+ *   - propensities h*c with falling factorials and volume/area scaling:
+ *       src/steps/mpi/dist/tetopsplit/kproc/reactions.cpp:66-119
+ *       src/steps/mpi/dist/tetopsplit/kproc/surface_reactions.cpp:176-240
+ *   - time-integrated occupancy of species that react and diffuse:
+ *       src/steps/mpi/dist/tetopsplit/mol_state.hpp:87-200
+ *       src/steps/mpi/dist/tetopsplit/kproc/reactions.cpp:35-40
+ *   - diffusion leaving (stochastic rounding, binomial total, categorical
+ *     split at <= 10 molecules, sequential conditional binomials above) and
+ *     entering (gather of the neighbour's per-face slot):
+ *       src/steps/mpi/dist/tetopsplit/operator/diffusion_operator.cpp:41-238
+ *       src/steps/mpi/dist/tetopsplit/kproc/diffusions.hpp:211-224
+ *       src/steps/rng/rng.hpp:139-156 (stochastic_round)
+ *   - split period = 1 / max summed face rate, face rate D*A/(V*d):
+ *       src/steps/mpi/tetopsplit/tetopsplit.cpp:5373-5411, diff.cpp:74-79
+ *       src/steps/mpi/dist/tetopsplit/simulation.cpp:1288-1337
+ *       src/steps/mpi/dist/tetopsplit/simulation_data.hpp:138-148
+ *   - period schedules: src/steps/mpi/tetopsplit/tetopsplit.cpp:1528-1560
+ *     (TetOpSplit) and src/steps/mpi/dist/tetopsplit/simulation.cpp:1358-1384
+ *     (DistTetOpSplit).
+ *
+ * Random numbers: Philox4x32 with 8 rounds, the generator the reference wraps
+ * (src/steps/rng/r123.hpp:76), but keyed by (seed) with counter
+ * (tet, period, stream, channel<<24 | block) instead of one sequential stream
+ * per rank. Binomials: inversion (Kachitvichyanukul & Schmeiser 1988, BINV)
+ * for n*min(p,1-p) < 10, else BTRS (Hoermann 1993, "The generation of binomial
+ * random variates", J. Stat. Comput. Simul. 46:101-110). log and sqrt are
+ * written out with + - * / only so that the device path reproduces them.
+ *
+ * Build: oracle/Makefile (-ffp-contract=off: no fused multiply-add anywhere).
+ */
+#include "stg_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define AVOGADRO 6.02214179e23 /* src/steps/math/constants.hpp:45 */
+#define DIFF_THRESHOLD 10      /* diffusion_operator.hpp:99 */
+#define MAX_EVENTS_PER_PERIOD (1 << 22)
+
+/* ------------------------------------------------------------------ Philox */
+
+void stgo_philox4x32_8(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 8; ++r) {
+        if (r > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+    }
+    out[0] = c0;
+    out[1] = c1;
+    out[2] = c2;
+    out[3] = c3;
+}
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[4];
+    uint32_t buf[4];
+    int idx;
+} ostream;
+
+static void os_init(ostream *st, uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2,
+                    uint32_t channel) {
+    st->key[0] = (uint32_t)seed;
+    st->key[1] = (uint32_t)(seed >> 32);
+    st->ctr[0] = c0;
+    st->ctr[1] = c1;
+    st->ctr[2] = c2;
+    st->ctr[3] = channel << 24;
+    st->idx = 4;
+}
+
+static uint32_t os_next(ostream *st) {
+    if (st->idx == 4) {
+        stgo_philox4x32_8(st->ctr, st->key, st->buf);
+        st->ctr[3] = (st->ctr[3] & 0xFF000000u) | ((st->ctr[3] + 1u) & 0x00FFFFFFu);
+        st->idx = 0;
+    }
+    return st->buf[st->idx++];
+}
+
+/* uniform on the open interval (0, 1) with 32 random bits */
+static double os_unif(ostream *st) {
+    return ((double)os_next(st) + 0.5) * 0x1p-32;
+}
+
+/* ------------------------------------------------------------ scalar math */
+
+static double d_from_bits(uint64_t b) {
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+static uint64_t d_to_bits(double d) {
+    uint64_t b;
+    memcpy(&b, &d, 8);
+    return b;
+}
+
+/* Natural log of a positive finite double: x = m * 2^e with m in
+ * [sqrt(1/2), sqrt(2)), log m = 2 atanh(f), f = (m-1)/(m+1), atanh by its
+ * Taylor series to f^21 (|f| <= 0.1716 so the tail is below 1e-17). */
+double stgo_log(double x) {
+    if (!(x > 0.0)) return -INFINITY;
+    uint64_t b = d_to_bits(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) { /* subnormal: scale into the normal range */
+        x = x * 0x1p54;
+        b = d_to_bits(x);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = d_from_bits((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    double f = (m - 1.0) / (m + 1.0);
+    double f2 = f * f;
+    double s = 1.0 / 21.0;
+    s = s * f2 + 1.0 / 19.0;
+    s = s * f2 + 1.0 / 17.0;
+    s = s * f2 + 1.0 / 15.0;
+    s = s * f2 + 1.0 / 13.0;
+    s = s * f2 + 1.0 / 11.0;
+    s = s * f2 + 1.0 / 9.0;
+    s = s * f2 + 1.0 / 7.0;
+    s = s * f2 + 1.0 / 5.0;
+    s = s * f2 + 1.0 / 3.0;
+    double tf = f + f;
+    double lm = tf + tf * (f2 * s);
+    double de = (double)e;
+    return de * 0x1.62e42fee00000p-1 + (lm + de * 0x1.a39ef35793c76p-33);
+}
+
+/* Square root of a positive finite normal double by Newton's iteration from
+ * an exponent-halving bit estimate (relative error < 4%), six steps. */
+double stgo_sqrt(double x) {
+    if (!(x > 0.0)) return 0.0;
+    uint64_t b = d_to_bits(x);
+    double y = d_from_bits((b >> 1) + 0x1ff8000000000000ull);
+    for (int i = 0; i < 6; ++i) y = 0.5 * (y + x / y);
+    return y;
+}
+
+static const double LFACT_TABLE[10] = {
+    0.0,
+    0.0,
+    0x1.62e42fefa39efp-1,
+    0x1.cab0bfa2a2002p+0,
+    0x1.96ca77c922cf9p+1,
+    0x1.326643c4479c9p+2,
+    0x1.a51273acf01cap+2,
+    0x1.10ce1f32dcc30p+3,
+    0x1.5358e82fcb70dp+3,
+    0x1.99a8921a7f7cfp+3,
+};
+
+/* log(k!) : exact table below 10, Stirling series of lgamma(k+1) above. */
+double stgo_lfact(int32_t k) {
+    if (k < 10) return LFACT_TABLE[k < 0 ? 0 : k];
+    double x = (double)k + 1.0;
+    double ix = 1.0 / x;
+    double ix2 = ix * ix;
+    double corr = ix * (1.0 / 12.0 - ix2 * (1.0 / 360.0 - ix2 * (1.0 / 1260.0)));
+    return (x - 0.5) * stgo_log(x) - x + 0x1.d67f1c864beb5p-1 + corr;
+}
+
+static double ipow(double base, int32_t n) {
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r = r * base;
+        base = base * base;
+        n >>= 1;
+    }
+    return r;
+}
+
+/* BINV: sequential inversion from k = 0, q <= 1/2, n*q < 10. */
+static int32_t binom_inv(ostream *st, int32_t n, double q) {
+    double r = ipow(1.0 - q, n);
+    double s = q / (1.0 - q);
+    double a = (double)(n + 1) * s;
+    double u = os_unif(st);
+    int32_t x = 0;
+    for (;;) {
+        if (u <= r) return x;
+        if (x >= n) return n;
+        u = u - r;
+        x += 1;
+        r = r * (a / (double)x - s);
+    }
+}
+
+/* BTRS: transformed rejection with squeeze, q <= 1/2, n*q >= 10. */
+static int32_t binom_btrs(ostream *st, int32_t n, double q) {
+    double nd = (double)n;
+    double spq = stgo_sqrt(nd * q * (1.0 - q));
+    double b = 1.15 + 2.53 * spq;
+    double a = -0.0873 + 0.0248 * b + 0.01 * q;
+    double c = nd * q + 0.5;
+    double alpha = (2.83 + 5.1 / b) * spq;
+    double vr = 0.92 - 4.2 / b;
+    double lpq = stgo_log(q / (1.0 - q));
+    int32_t m = (int32_t)floor((nd + 1.0) * q);
+    double h = stgo_lfact(m) + stgo_lfact(n - m);
+    for (int it = 0; it < 1000; ++it) {
+        double u = os_unif(st) - 0.5;
+        double v = os_unif(st);
+        double us = 0.5 - fabs(u);
+        double kd = floor((2.0 * a / us + b) * u + c);
+        if (kd < 0.0 || kd > nd) continue;
+        int32_t k = (int32_t)kd;
+        if (us >= 0.07 && v <= vr) return k;
+        v = stgo_log(v * alpha / (a / (us * us) + b));
+        if (v <= h - stgo_lfact(k) - stgo_lfact(n - k) + (double)(k - m) * lpq) return k;
+    }
+    return m;
+}
+
+/* Binomial(n, p); mirrors the role of RNG::getBinom (src/steps/rng/rng.cpp:516-527). */
+static int32_t binom(ostream *st, int32_t n, double p) {
+    if (n <= 0 || !(p > 0.0)) return 0;
+    if (p >= 1.0) return n;
+    int flip = p > 0.5;
+    double q = flip ? 1.0 - p : p;
+    int32_t k = ((double)n * q < 10.0) ? binom_inv(st, n, q) : binom_btrs(st, n, q);
+    return flip ? n - k : k;
+}
+
+int32_t stgo_binom_stream(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t channel,
+                          int32_t n, double p, int32_t reps, int32_t *out) {
+    ostream st;
+    os_init(&st, seed, c0, c1, c2, channel);
+    for (int32_t i = 0; i < reps; ++i) out[i] = binom(&st, n, p);
+    return 0;
+}
+
+void stgo_uniforms(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t channel,
+                   int32_t n, double *out) {
+    ostream st;
+    os_init(&st, seed, c0, c1, c2, channel);
+    for (int32_t i = 0; i < n; ++i) out[i] = os_unif(&st);
+}
+
+/* ----------------------------------------------------------------- schedule */
+
+/* src/steps/util/collections.hpp:143-149 */
+static int almost_equal(double x, double y) {
+    return fabs(x - y) <= DBL_EPSILON * fabs(x + y) * 4 || fabs(x - y) < DBL_MIN;
+}
+
+int32_t stgo_schedule(double t0, double t_end, double dt_std, int32_t mode, double *dts,
+                      int32_t cap) {
+    int32_t n = 0;
+    double t = t0;
+    if (t_end < t0) return -2;
+    if (mode == 0) {
+        /* tetopsplit.cpp:1508-1560: periods of dt_std, the last clipped. */
+        if (almost_equal(t0, t_end)) return 0;
+        int aligned = 0;
+        while (t < t_end && !aligned) {
+            double per = dt_std;
+            if (t + dt_std > t_end) {
+                per = t_end - t;
+                aligned = 1;
+            }
+            if (n >= cap) return -1;
+            dts[n++] = per;
+            t = t + per;
+        }
+        return n;
+    }
+    /* simulation.cpp:1358-1384 */
+    double nstd_d = floor((t_end - t) / dt_std) - 1.0;
+    long long n_std = nstd_d > 0.0 ? (long long)nstd_d : 0;
+    for (long long i = 0; i < n_std; ++i) {
+        double per = dt_std < t_end - t ? dt_std : t_end - t;
+        if (n >= cap) return -1;
+        dts[n++] = per;
+        t = t + per;
+    }
+    double next = t + dt_std;
+    if (!almost_equal(next, t_end) && next < t_end) {
+        if (n + 2 > cap) return -1;
+        dts[n++] = dt_std;
+        t = t + dt_std;
+        dts[n++] = t_end - t;
+        t = t_end;
+    } else if (!almost_equal(t_end, t)) {
+        if (n >= cap) return -1;
+        dts[n++] = t_end - t;
+    }
+    return n;
+}
+
+/* ------------------------------------------------------------------- model */
+
+static double face_rate_sum(const double *r4) {
+    return ((r4[0] + r4[1]) + r4[2]) + r4[3];
+}
+
+double stgo_update_period(const stgo_system *s) {
+    double mx = 0.0;
+    for (int32_t t = 0; t < s->n_tets; ++t) {
+        for (int32_t sp = 0; sp < s->n_specs; ++sp) {
+            double D = s->spec_dcst[sp];
+            if (!(D > 0.0)) continue;
+            double r4[4];
+            for (int f = 0; f < 4; ++f) r4[f] = D * s->tet_face_g[4 * t + f];
+            double sum = face_rate_sum(r4);
+            if (sum > mx) mx = sum;
+        }
+    }
+    return mx > DBL_EPSILON ? 1.0 / mx : INFINITY;
+}
+
+/* c = k * scale^-(order-1): reactions.cpp:113-119, surface_reactions.cpp:176-203 */
+static double scaled_const(double k, int32_t order, double scale) {
+    int32_t o1 = order - 1;
+    if (o1 == 0) return k;
+    if (o1 < 0) return k * ipow(scale, -o1);
+    return k / ipow(scale, o1);
+}
+
+/* h = prod over reactants of n(n-1)...(n-lhs+1): reactions.cpp:66-109 */
+static double falling(double h, int32_t n, int32_t lhs) {
+    for (int32_t i = lhs - 1; i >= 0; --i) h = h * (double)(n - i);
+    return h;
+}
+
+typedef struct {
+    int32_t *tri_ptr; /* [n_tets+1] */
+    int32_t *tri_idx; /* [n_tris]   */
+    int32_t max_tris;
+    uint8_t *tracked; /* [n_specs]  */
+} topo;
+
+static void topo_build(const stgo_system *s, topo *tp) {
+    tp->tri_ptr = (int32_t *)calloc((size_t)s->n_tets + 1, sizeof(int32_t));
+    tp->tri_idx = (int32_t *)malloc(sizeof(int32_t) * (size_t)(s->n_tris > 0 ? s->n_tris : 1));
+    for (int32_t i = 0; i < s->n_tris; ++i) tp->tri_ptr[s->tri_tet[i] + 1]++;
+    tp->max_tris = 0;
+    for (int32_t t = 0; t < s->n_tets; ++t) {
+        if (tp->tri_ptr[t + 1] > tp->max_tris) tp->max_tris = tp->tri_ptr[t + 1];
+        tp->tri_ptr[t + 1] += tp->tri_ptr[t];
+    }
+    int32_t *fill = (int32_t *)calloc((size_t)s->n_tets, sizeof(int32_t));
+    for (int32_t i = 0; i < s->n_tris; ++i) {
+        int32_t t = s->tri_tet[i];
+        tp->tri_idx[tp->tri_ptr[t] + fill[t]++] = i;
+    }
+    free(fill);
+    /* occupancy is tracked for species that diffuse and are changed by a
+     * reaction (reactions.cpp:35-40) */
+    tp->tracked = (uint8_t *)calloc((size_t)s->n_specs, 1);
+    for (int32_t sp = 0; sp < s->n_specs; ++sp) {
+        if (!(s->spec_dcst[sp] > 0.0)) continue;
+        for (int32_t r = 0; r < s->n_reacs; ++r)
+            if (s->reac_upd[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
+        for (int32_t r = 0; r < s->n_sreacs; ++r)
+            if (s->sreac_upd_i[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
+    }
+}
+
+static void topo_free(topo *tp) {
+    free(tp->tri_ptr);
+    free(tp->tri_idx);
+    free(tp->tracked);
+}
+
+/* SSA of one tet and its triangles over [0, dt): ssa_operator.cpp:24-47 with
+ * the Direct group of propensities.hpp:543-586. Returns events or -1. */
+static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, uint32_t step,
+                    int32_t *cnt, double *corr, double *a, double *ps) {
+    const int32_t S = s->n_specs, SS = s->n_sspecs;
+    const int32_t R = s->n_reacs, RS = s->n_sreacs;
+    const int32_t t0 = tp->tri_ptr[t], nt = tp->tri_ptr[t + 1] - t0;
+    const int32_t nk = R + nt * RS;
+    if (nk == 0) return 0;
+    const double V = s->tet_vol[t];
+    const double vscale = 1.0e3 * V * AVOGADRO;
+    ostream st;
+    os_init(&st, s->seed, (uint32_t)t, step, s->stream, 0u);
+    long nev = 0;
+    double tcur = 0.0;
+    for (;;) {
+        /* propensities: volume reactions, then each triangle's sreacs */
+        int32_t j = 0;
+        for (int32_t r = 0; r < R; ++r, ++j) {
+            const int32_t *lhs = s->reac_lhs + r * S;
+            int32_t order = 0;
+            for (int32_t sp = 0; sp < S; ++sp) order += lhs[sp];
+            double h = 1.0;
+            int zero = 0;
+            for (int32_t sp = 0; sp < S && !zero; ++sp) {
+                if (lhs[sp] == 0) continue;
+                if (lhs[sp] > cnt[sp]) zero = 1;
+                else h = falling(h, cnt[sp], lhs[sp]);
+            }
+            a[j] = zero ? 0.0 : h * scaled_const(s->reac_k[r], order, vscale);
+        }
+        for (int32_t i = 0; i < nt; ++i) {
+            const int32_t tr = tp->tri_idx[t0 + i];
+            for (int32_t r = 0; r < RS; ++r, ++j) {
+                const int32_t *ls = s->sreac_lhs_s + r * SS;
+                const int32_t *li = s->sreac_lhs_i + r * S;
+                int32_t order = 0, nvol = 0;
+                for (int32_t q = 0; q < SS; ++q) order += ls[q];
+                for (int32_t q = 0; q < S; ++q) {
+                    order += li[q];
+                    nvol += li[q];
+                }
+                double h = 1.0;
+                int zero = 0;
+                for (int32_t q = 0; q < SS && !zero; ++q) {
+                    if (ls[q] == 0) continue;
+                    int32_t n = s->tri_counts[(size_t)q * s->n_tris + tr];
+                    if (ls[q] > n) zero = 1;
+                    else h = falling(h, n, ls[q]);
+                }
+                for (int32_t q = 0; q < S && !zero; ++q) {
+                    if (li[q] == 0) continue;
+                    if (li[q] > cnt[q]) zero = 1;
+                    else h = falling(h, cnt[q], li[q]);
+                }
+                if (zero) {
+                    a[j] = 0.0;
+                } else {
+                    double scale = nvol == 0 ? s->tri_area[tr] * AVOGADRO : vscale;
+                    a[j] = h * scaled_const(s->sreac_k[r], order, scale);
+                }
+            }
+        }
+        double acc = 0.0;
+        for (int32_t q = 0; q < nk; ++q) {
+            acc = acc + a[q];
+            ps[q] = acc;
+        }
+        const double a0 = ps[nk - 1];
+        if (a0 < DBL_EPSILON) break;
+        const double u1 = os_unif(&st);
+        const double u2 = os_unif(&st);
+        const double tev = tcur + (-stgo_log(u1) / a0);
+        if (tev > dt) break;
+        tcur = tev;
+        const double x = u2 * a0;
+        int32_t sel = -1;
+        for (int32_t q = 0; q < nk; ++q) {
+            if (ps[q] > x) {
+                sel = q;
+                break;
+            }
+        }
+        if (sel < 0) {
+            for (int32_t q = nk - 1; q >= 0; --q)
+                if (a[q] > 0.0) {
+                    sel = q;
+                    break;
+                }
+        }
+        /* apply: reactions.cpp:124-141 / surface_reactions.cpp:158-172 */
+        if (sel < R) {
+            const int32_t *upd = s->reac_upd + sel * S;
+            for (int32_t sp = 0; sp < S; ++sp) {
+                if (upd[sp] == 0) continue;
+                cnt[sp] += upd[sp];
+                if (tp->tracked[sp]) corr[sp] -= (double)upd[sp] * tcur;
+            }
+        } else {
+            const int32_t i = (sel - R) / RS, r = (sel - R) % RS;
+            const int32_t tr = tp->tri_idx[t0 + i];
+            const int32_t *us_ = s->sreac_upd_s + r * SS;
+            const int32_t *ui = s->sreac_upd_i + r * S;
+            for (int32_t q = 0; q < SS; ++q)
+                if (us_[q] != 0) s->tri_counts[(size_t)q * s->n_tris + tr] += us_[q];
+            for (int32_t sp = 0; sp < S; ++sp) {
+                if (ui[sp] == 0) continue;
+                cnt[sp] += ui[sp];
+                if (tp->tracked[sp]) corr[sp] -= (double)ui[sp] * tcur;
+            }
+        }
+        if (++nev >= MAX_EVENTS_PER_PERIOD) return -1;
+    }
+    return nev;
+}
+
+/* diffusion leaving of one (tet, species): diffusion_operator.cpp:59-130, 222-238 */
+static int32_t leave_tet_species(const stgo_system *s, int32_t t, int32_t sp, double dt,
+                                 uint32_t step, int32_t *cnt, double corr_sp, int tracked,
+                                 int32_t *out4) {
+    out4[0] = out4[1] = out4[2] = out4[3] = 0;
+    const int32_t n = cnt[sp];
+    const double D = s->spec_dcst[sp];
+    double r4[4];
+    for (int f = 0; f < 4; ++f) r4[f] = D * s->tet_face_g[4 * t + f];
+    const double sum = face_rate_sum(r4);
+    if (n <= 0 || !(sum > 0.0)) return 0;
+    ostream st;
+    os_init(&st, s->seed, (uint32_t)t, step, s->stream, 1u + (uint32_t)sp);
+    double occ = (double)n;
+    if (tracked && !almost_equal(dt, 0.0)) occ = (double)n + corr_sp / dt;
+    const double u = os_unif(&st);
+    const double fl = floor(occ);
+    int32_t mp = (int32_t)fl + ((occ - fl) > u ? 1 : 0);
+    if (mp > n) mp = n;
+    if (mp < 0) mp = 0;
+    double p = dt * sum;
+    if (p > 1.0) p = 1.0;
+    if (p < 0.0) p = 0.0;
+    const int32_t nl = binom(&st, mp, p);
+    if (nl <= 0) return 0;
+    cnt[sp] = n - nl;
+    int lastf = -1;
+    for (int f = 0; f < 4; ++f)
+        if (r4[f] > 0.0) lastf = f;
+    if (nl <= DIFF_THRESHOLD) {
+        for (int32_t m = 0; m < nl; ++m) {
+            int chosen = -1;
+            for (int tries = 0; tries < 64 && chosen < 0; ++tries) {
+                const double selv = os_unif(&st) * sum;
+                double cum = 0.0;
+                for (int f = 0; f < 4; ++f) {
+                    cum = cum + r4[f];
+                    if (selv < cum) {
+                        chosen = f;
+                        break;
+                    }
+                }
+            }
+            if (chosen < 0) chosen = lastf;
+            out4[chosen] += 1;
+        }
+    } else {
+        int32_t rem = nl;
+        double rsum = sum;
+        for (int f = 0; f < 4; ++f) {
+            if (!(r4[f] > 0.0)) continue;
+            int32_t k;
+            if (f == lastf) {
+                k = rem;
+            } else {
+                const double pf = r4[f] / rsum;
+                k = pf >= 1.0 ? rem : binom(&st, rem, pf);
+                rsum = rsum - r4[f];
+            }
+            out4[f] = k;
+            rem -= k;
+        }
+    }
+    return nl;
+}
+
+int32_t stgo_period(stgo_system *s, double dt, uint32_t step, int32_t *scratch,
+                    uint64_t *n_events, uint64_t *n_moved) {
+    return stgo_run(s, &dt, 1, step, scratch, n_events, n_moved);
+}
+
+int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t step0,
+                 int32_t *scratch, uint64_t *n_events, uint64_t *n_moved) {
+    const int32_t N = s->n_tets, S = s->n_specs;
+    topo tp;
+    topo_build(s, &tp);
+    const int32_t nk_max = s->n_reacs + tp.max_tris * s->n_sreacs + 1;
+    int32_t err = 0;
+    uint64_t ev_total = 0, mv_total = 0;
+    for (int32_t p = 0; p < n_periods && !err; ++p) {
+        const double dt = dts[p];
+        const uint32_t step = step0 + (uint32_t)p;
+        /* pass 1: SSA + leaving, one tet at a time (independent groups) */
+#pragma omp parallel reduction(+ : ev_total, mv_total) reduction(| : err)
+        {
+            int32_t *cnt = (int32_t *)malloc(sizeof(int32_t) * (size_t)S);
+            double *corr = (double *)malloc(sizeof(double) * (size_t)S);
+            double *a = (double *)malloc(sizeof(double) * (size_t)nk_max);
+            double *ps = (double *)malloc(sizeof(double) * (size_t)nk_max);
+#pragma omp for schedule(dynamic, 256)
+            for (int32_t t = 0; t < N; ++t) {
+                for (int32_t sp = 0; sp < S; ++sp) {
+                    cnt[sp] = s->counts[(size_t)sp * N + t];
+                    corr[sp] = 0.0;
+                }
+                long nev = ssa_tet(s, &tp, t, dt, step, cnt, corr, a, ps);
+                if (nev < 0) {
+                    err |= 1;
+                    nev = 0;
+                }
+                ev_total += (uint64_t)nev;
+                for (int32_t sp = 0; sp < S; ++sp) {
+                    int32_t *o4 = scratch + ((size_t)sp * N + t) * 4;
+                    if (s->spec_dcst[sp] > 0.0)
+                        mv_total += (uint64_t)leave_tet_species(s, t, sp, dt, step, cnt, corr[sp],
+                                                                tp.tracked[sp], o4);
+                    else
+                        o4[0] = o4[1] = o4[2] = o4[3] = 0;
+                    s->counts[(size_t)sp * N + t] = cnt[sp];
+                }
+            }
+            free(cnt);
+            free(corr);
+            free(a);
+            free(ps);
+        }
+        /* pass 2: entering, gather of the neighbour's slot (diffusion_operator.cpp:136-220) */
+#pragma omp parallel for schedule(static)
+        for (int32_t t = 0; t < N; ++t) {
+            for (int32_t sp = 0; sp < S; ++sp) {
+                if (!(s->spec_dcst[sp] > 0.0)) continue;
+                int32_t add = 0;
+                for (int f = 0; f < 4; ++f) {
+                    const int32_t nb = s->tet_nbr[4 * t + f];
+                    if (nb < 0) continue;
+                    add += scratch[((size_t)sp * N + nb) * 4 + s->tet_nbr_face[4 * t + f]];
+                }
+                s->counts[(size_t)sp * N + t] += add;
+            }
+        }
+    }
+    topo_free(&tp);
+    if (n_events) *n_events += ev_total;
+    if (n_moved) *n_moved += mv_total;
+    return err ? -1 : 0;
+}

@@ -1,0 +1,88 @@
+/*
+ * CPU oracle for the operator-split stochastic reaction-diffusion period.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference
+ * algorithm used as the checker for the HIP path (and as the CPU baseline in
+ * bench.py). The product never links, loads or calls it.
+ *
+ * Every function in stg_oracle.c cites the reference file:line it restates.
+ * Random draws come from the same keyed Philox4x32-8 streams as the device
+ * path, so the oracle and the GPU must agree bit-for-bit on integer counts.
+ */
+#ifndef STG_ORACLE_H
+#define STG_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct stgo_system {
+    /* mesh (one realization) */
+    int32_t n_tets;
+    const double  *tet_vol;      /* [n_tets]                 m^3                 */
+    const int32_t *tet_nbr;      /* [n_tets*4] neighbour tet, -1 = none          */
+    const int32_t *tet_nbr_face; /* [n_tets*4] slot of this face in the nbr      */
+    const double  *tet_face_g;   /* [n_tets*4] A_f/(V*d_f) m^-2, 0 = closed face */
+    int32_t n_tris;
+    const int32_t *tri_tet;      /* [n_tris] inner tet of a membrane triangle    */
+    const double  *tri_area;     /* [n_tris] m^2                                 */
+    /* model */
+    int32_t n_specs, n_reacs, n_sspecs, n_sreacs;
+    const int32_t *reac_lhs;     /* [n_reacs*n_specs] reactant multiplicity      */
+    const int32_t *reac_upd;     /* [n_reacs*n_specs] net change                 */
+    const double  *reac_k;       /* [n_reacs] macroscopic constant (SI)          */
+    const double  *spec_dcst;    /* [n_specs] m^2/s, 0 = immobile                */
+    const int32_t *sreac_lhs_s;  /* [n_sreacs*n_sspecs]                          */
+    const int32_t *sreac_upd_s;  /* [n_sreacs*n_sspecs]                          */
+    const int32_t *sreac_lhs_i;  /* [n_sreacs*n_specs] inner-volume reactants    */
+    const int32_t *sreac_upd_i;  /* [n_sreacs*n_specs]                           */
+    const double  *sreac_k;      /* [n_sreacs]                                   */
+    /* state, mutated in place */
+    int32_t *counts;             /* [n_specs][n_tets]   species-major SoA        */
+    int32_t *tri_counts;         /* [n_sspecs][n_tris]                           */
+    /* Philox key and stream id */
+    uint64_t seed;
+    uint32_t stream;
+} stgo_system;
+
+/* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */
+void stgo_philox4x32_8(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Deterministic scalar helpers (exposed for the tests). */
+double stgo_log(double x);
+double stgo_sqrt(double x);
+double stgo_lfact(int32_t k);
+/* Binomial(n, p) drawn from the stream (seed, ctr0..2, channel). */
+int32_t stgo_binom_stream(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t channel,
+                          int32_t n, double p, int32_t reps, int32_t *out);
+/* Uniforms in (0,1) from one stream, for KS checks. */
+void stgo_uniforms(uint64_t seed, uint32_t c0, uint32_t c1, uint32_t c2, uint32_t channel,
+                   int32_t n, double *out);
+
+/* Split period schedule. mode 0 = TetOpSplit (clip the last period),
+ * mode 1 = DistTetOpSplit (n-1 standard periods then one or two remainders).
+ * Returns the number of periods written to dts (at most cap), or -1 if cap is
+ * too small. */
+int32_t stgo_schedule(double t0, double t_end, double dt_std, int32_t mode, double *dts,
+                      int32_t cap);
+
+/* Stability-bound split period 1 / max_{tet,species} sum_f D_s g_f
+ * (0 if nothing diffuses). */
+double stgo_update_period(const stgo_system *s);
+
+/* One operator-split period: per-tet SSA over [0, dt) then multinomial
+ * diffusion (leaving pass, then entering pass). scratch must hold
+ * n_specs*n_tets*4 int32. Returns 0, or a negative code on a model error. */
+int32_t stgo_period(stgo_system *s, double dt, uint32_t step, int32_t *scratch,
+                    uint64_t *n_events, uint64_t *n_moved);
+
+/* Run periods from a precomputed schedule; step0 is the first period index. */
+int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t step0,
+                 int32_t *scratch, uint64_t *n_events, uint64_t *n_moved);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,970 @@
+// Host side of the C ABI declared in include/stg.h: owns the device buffers of
+// one simulation (one or many realizations), builds the period schedule and
+// launches the fused period kernel on the handle's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/stg.h"
+#include "stg_kernels.hpp"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct StgError {
+    int code;
+    std::string msg;
+};
+
+#define HIP_OK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw StgError{STG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+template <typename T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (n == 0) n = 1;
+    HIP_OK(hipMalloc(&p, n * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+// util::almost_equal (src/steps/util/collections.hpp:143-149)
+bool almost_equal(double x, double y) {
+    return std::fabs(x - y) <= DBL_EPSILON * std::fabs(x + y) * 4 ||
+           std::fabs(x - y) < DBL_MIN;
+}
+
+// Period lengths from t0 to t_end. mode TETOPSPLIT follows
+// TetOpSplitP::_runWithoutEField (tetopsplit.cpp:1528-1560); mode
+// DISTTETOPSPLIT follows OmegaHSimulation::run_rd (simulation.cpp:1358-1384).
+void schedule(double t0, double t_end, double dt_std, int mode, std::vector<double> &dts) {
+    dts.clear();
+    double t = t0;
+    if (mode == STG_MODE_TETOPSPLIT) {
+        if (almost_equal(t0, t_end)) return;
+        bool aligned = false;
+        while (t < t_end && !aligned) {
+            double per = dt_std;
+            if (t + dt_std > t_end) {
+                per = t_end - t;
+                aligned = true;
+            }
+            dts.push_back(per);
+            t = t + per;
+        }
+        return;
+    }
+    const double nstd = std::floor((t_end - t) / dt_std) - 1.0;
+    const long long n_std = nstd > 0.0 ? (long long)nstd : 0;
+    for (long long i = 0; i < n_std; ++i) {
+        const double per = std::min(dt_std, t_end - t);
+        dts.push_back(per);
+        t = t + per;
+    }
+    const double next = t + dt_std;
+    if (!almost_equal(next, t_end) && next < t_end) {
+        dts.push_back(dt_std);
+        t = t + dt_std;
+        dts.push_back(t_end - t);
+    } else if (!almost_equal(t_end, t)) {
+        dts.push_back(t_end - t);
+    }
+}
+
+}  // namespace
+
+struct stg_sim {
+    int mode = STG_MODE_TETOPSPLIT;
+    int n_real = 0;
+    int S = 0, R = 0, SS = 0, RS = 0, ND = 0;
+    int block = 256;
+    bool use_graph = false;
+    bool xcd_remap = true;
+    stg::ModelK mk{};
+    std::vector<int32_t> tet_off, tri_off;  // [n_real+1]
+    std::vector<int32_t> block_real, block_tet0;
+    std::vector<std::vector<int32_t>> tri_perm;  // per realization: user tri -> internal tri
+    int32_t NT = 0, NTRI = 0, max_tris = 0;
+    std::vector<double> h_k, h_sk, h_D;  // per-realization tables
+    std::vector<double> h_face_g;         // [NT*4], for update-period recomputation
+    std::vector<double> dt_std;           // [n_real]
+    std::vector<uint64_t> nsteps;         // [n_real]
+    std::vector<uint64_t> seeds;
+    std::vector<uint32_t> streams;
+    double time = 0.0;
+    // device
+    double *d_vol = nullptr;
+    int4 *d_nbr = nullptr;
+    uint32_t *d_nface = nullptr;
+    double2 *d_g = nullptr;
+    int32_t *d_counts = nullptr;
+    int32_t *d_out[2] = {nullptr, nullptr};
+    int32_t *d_tri_ptr = nullptr;
+    int32_t *d_tri_counts = nullptr;
+    double *d_tri_area = nullptr;
+    int32_t *d_block_real = nullptr, *d_block_tet0 = nullptr, *d_real_tet0 = nullptr;
+    double *d_real_k = nullptr, *d_real_sk = nullptr, *d_real_D = nullptr;
+    uint64_t *d_real_seed = nullptr;
+    uint32_t *d_real_stream = nullptr, *d_real_step0 = nullptr;
+    int32_t *d_real_nper = nullptr;
+    double *d_dt_tab = nullptr;
+    size_t dt_tab_cap = 0;
+    unsigned long long *d_ext = nullptr;
+    int32_t *d_err = nullptr;
+    unsigned long long *d_sum = nullptr;
+    int64_t *d_idx = nullptr;
+    size_t idx_cap = 0;
+    double *d_gather = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int graph_launches = -1;
+    int graph_block = -1;
+    double last_ms = 0.0;
+    int32_t last_launches = 0;
+    std::string err;
+
+    size_t smem_bytes(int B) const {
+        const int K = R + max_tris * RS;
+        size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
+        return off + (size_t)S * B * 8 + (size_t)(K > 0 ? K : 1) * B * 8;
+    }
+
+    double compute_dt_std(int r) const {
+        double mx = 0.0;
+        for (int t = tet_off[r]; t < tet_off[r + 1]; ++t) {
+            for (int s = 0; s < S; ++s) {
+                const double D = h_D[(size_t)r * S + s];
+                if (!(D > 0.0)) continue;
+                const double *g = &h_face_g[4 * (size_t)t];
+                const double sum = (((D * g[0]) + (D * g[1])) + (D * g[2])) + (D * g[3]);
+                if (sum > mx) mx = sum;
+            }
+        }
+        return mx > DBL_EPSILON ? 1.0 / mx : INFINITY;
+    }
+
+    void free_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr;
+        graph = nullptr;
+        graph_launches = -1;
+    }
+
+    ~stg_sim() {
+        free_graph();
+        void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
+                        d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
+                        d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
+                        d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
+                        d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
+                        d_gather};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void check_real(const stg_sim *s, int32_t real) {
+    if (real < 0 || real >= s->n_real)
+        throw StgError{STG_ERR_ARG, "realization index " + std::to_string(real) + " out of range"};
+}
+
+void build_model(stg_sim *s, const stg_model *m) {
+    if (!m) throw StgError{STG_ERR_ARG, "null model"};
+    if (m->n_specs < 0 || m->n_specs > stg::kMaxSpecs)
+        throw StgError{STG_ERR_ARG, "n_specs must be in [0, 16]"};
+    if (m->n_reacs < 0 || m->n_reacs > stg::kMaxReacs)
+        throw StgError{STG_ERR_ARG, "n_reacs must be in [0, 32]"};
+    if (m->n_sspecs < 0 || m->n_sspecs > stg::kMaxSSpecs)
+        throw StgError{STG_ERR_ARG, "n_sspecs must be in [0, 8]"};
+    if (m->n_sreacs < 0 || m->n_sreacs > stg::kMaxSReacs)
+        throw StgError{STG_ERR_ARG, "n_sreacs must be in [0, 16]"};
+    s->S = m->n_specs;
+    s->R = m->n_reacs;
+    s->SS = m->n_sspecs;
+    s->RS = m->n_sreacs;
+    stg::ModelK &k = s->mk;
+    std::memset(&k, 0, sizeof(k));
+    k.n_specs = s->S;
+    k.n_reacs = s->R;
+    k.n_sspecs = s->SS;
+    k.n_sreacs = s->RS;
+    int ne = 0;
+    auto push = [&](int spec, int val) {
+        if (val < -128 || val > 127) throw StgError{STG_ERR_ARG, "stoichiometry out of int8 range"};
+        if (ne >= stg::kMaxEntries) throw StgError{STG_ERR_ARG, "model tables too large"};
+        k.ent_spec[ne] = (uint8_t)spec;
+        k.ent_val[ne] = (int8_t)val;
+        ++ne;
+    };
+    for (int r = 0; r < s->R; ++r) {
+        k.lhs_off[r] = (uint16_t)ne;
+        int order = 0;
+        for (int sp = 0; sp < s->S; ++sp) {
+            const int v = m->reac_lhs[r * s->S + sp];
+            if (v < 0 || v > 4)
+                throw StgError{STG_ERR_ARG, "reactant multiplicity must be in [0, 4]"};
+            if (v) push(sp, v);
+            order += v;
+        }
+        k.order[r] = (int8_t)order;
+    }
+    k.lhs_off[s->R] = (uint16_t)ne;
+    for (int r = 0; r < s->R; ++r) {
+        k.upd_off[r] = (uint16_t)ne;
+        for (int sp = 0; sp < s->S; ++sp) {
+            const int v = m->reac_upd[r * s->S + sp];
+            if (v) push(sp, v);
+            if (v < 0 && -v > m->reac_lhs[r * s->S + sp])
+                throw StgError{STG_ERR_ARG, "a reaction consumes a species it does not react with"};
+        }
+    }
+    k.upd_off[s->R] = (uint16_t)ne;
+    for (int r = 0; r < s->RS; ++r) {
+        k.slhs_off[r] = (uint16_t)ne;
+        int order = 0, nvol = 0;
+        for (int q = 0; q < s->SS; ++q) {
+            const int v = m->sreac_lhs_s[r * s->SS + q];
+            if (v < 0 || v > 4) throw StgError{STG_ERR_ARG, "reactant multiplicity must be in [0, 4]"};
+            if (v) push(stg::kSurfFlag | q, v);
+            order += v;
+        }
+        for (int q = 0; q < s->S; ++q) {
+            const int v = m->sreac_lhs_i[r * s->S + q];
+            if (v < 0 || v > 4) throw StgError{STG_ERR_ARG, "reactant multiplicity must be in [0, 4]"};
+            if (v) push(q, v);
+            order += v;
+            nvol += v;
+        }
+        k.sorder[r] = (int8_t)order;
+        k.s_nvol[r] = (int8_t)nvol;
+    }
+    k.slhs_off[s->RS] = (uint16_t)ne;
+    for (int r = 0; r < s->RS; ++r) {
+        k.supd_off[r] = (uint16_t)ne;
+        for (int q = 0; q < s->SS; ++q) {
+            const int v = m->sreac_upd_s[r * s->SS + q];
+            if (v) push(stg::kSurfFlag | q, v);
+            if (v < 0 && -v > m->sreac_lhs_s[r * s->SS + q])
+                throw StgError{STG_ERR_ARG, "a surface reaction consumes a species it does not react with"};
+        }
+        for (int q = 0; q < s->S; ++q) {
+            const int v = m->sreac_upd_i[r * s->S + q];
+            if (v) push(q, v);
+            if (v < 0 && -v > m->sreac_lhs_i[r * s->S + q])
+                throw StgError{STG_ERR_ARG, "a surface reaction consumes a species it does not react with"};
+        }
+    }
+    k.supd_off[s->RS] = (uint16_t)ne;
+}
+
+// Diffusing species and occupancy tracking, from the union of all realizations'
+// diffusion constants (reactions.cpp:35-40: tracked = reacts and diffuses).
+void build_diffusion_sets(stg_sim *s, const stg_model *m) {
+    stg::ModelK &k = s->mk;
+    k.n_diff = 0;
+    k.tracked_mask = 0;
+    for (int sp = 0; sp < s->S; ++sp) {
+        bool diff = false;
+        for (int r = 0; r < s->n_real; ++r) diff = diff || s->h_D[(size_t)r * s->S + sp] > 0.0;
+        if (!diff) continue;
+        k.dspec[k.n_diff++] = sp;
+        bool reacts = false;
+        for (int r = 0; r < s->R; ++r) reacts = reacts || m->reac_upd[r * s->S + sp] != 0;
+        for (int r = 0; r < s->RS; ++r) reacts = reacts || m->sreac_upd_i[r * s->S + sp] != 0;
+        if (reacts) k.tracked_mask |= 1 << sp;
+    }
+    s->ND = k.n_diff;
+}
+
+int fail(stg_sim *s, const StgError &e) {
+    if (s) s->err = e.msg;
+    else g_create_error = e.msg;
+    return e.code;
+}
+
+template <typename F>
+int guard(stg_sim *s, F &&f) {
+    try {
+        f();
+        return STG_OK;
+    } catch (const StgError &e) {
+        return fail(s, e);
+    } catch (const std::bad_alloc &) {
+        return fail(s, StgError{STG_ERR_SYS, "host allocation failed"});
+    } catch (const std::exception &e) {
+        return fail(s, StgError{STG_ERR_PROG, e.what()});
+    }
+}
+
+void upload_real_tables(stg_sim *s) {
+    if (!s->h_k.empty())
+        HIP_OK(hipMemcpyAsync(s->d_real_k, s->h_k.data(), sizeof(double) * s->h_k.size(),
+                              hipMemcpyHostToDevice, s->stream));
+    if (!s->h_sk.empty())
+        HIP_OK(hipMemcpyAsync(s->d_real_sk, s->h_sk.data(), sizeof(double) * s->h_sk.size(),
+                              hipMemcpyHostToDevice, s->stream));
+    if (!s->h_D.empty())
+        HIP_OK(hipMemcpyAsync(s->d_real_D, s->h_D.data(), sizeof(double) * s->h_D.size(),
+                              hipMemcpyHostToDevice, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+}
+
+void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_mesh *meshes,
+                 const stg_realization *reals, int32_t mode) {
+    if (n_real <= 0) throw StgError{STG_ERR_ARG, "need at least one realization"};
+    if (!meshes || !reals) throw StgError{STG_ERR_ARG, "null mesh or realization array"};
+    if (mode != STG_MODE_TETOPSPLIT && mode != STG_MODE_DISTTETOPSPLIT)
+        throw StgError{STG_ERR_ARG, "unknown solver mode"};
+    s->mode = mode;
+    s->n_real = n_real;
+    build_model(s, model);
+    const int S = s->S, R = s->R, RS = s->RS, SS = s->SS;
+    s->tet_off.assign(n_real + 1, 0);
+    s->tri_off.assign(n_real + 1, 0);
+    for (int r = 0; r < n_real; ++r) {
+        const stg_mesh &m = meshes[r];
+        if (m.n_tets <= 0) throw StgError{STG_ERR_ARG, "mesh has no tetrahedra"};
+        if (m.n_tris < 0) throw StgError{STG_ERR_ARG, "negative triangle count"};
+        if (!m.tet_vol || !m.tet_nbr || !m.tet_nbr_face || !m.tet_face_g)
+            throw StgError{STG_ERR_ARG, "null mesh array"};
+        if (m.n_tris > 0 && (!m.tri_tet || !m.tri_area))
+            throw StgError{STG_ERR_ARG, "null triangle array"};
+        const long long nt = (long long)s->tet_off[r] + m.n_tets;
+        const long long ntr = (long long)s->tri_off[r] + m.n_tris;
+        if (nt > INT32_MAX / 4 || ntr > INT32_MAX) throw StgError{STG_ERR_ARG, "mesh too large"};
+        s->tet_off[r + 1] = (int32_t)nt;
+        s->tri_off[r + 1] = (int32_t)ntr;
+    }
+    s->NT = s->tet_off[n_real];
+    s->NTRI = s->tri_off[n_real];
+    const int NT = s->NT, NTRI = s->NTRI;
+
+    // host staging of the concatenated mesh
+    std::vector<double> vol(NT);
+    std::vector<int4> nbr(NT);
+    std::vector<uint32_t> nface(NT);
+    s->h_face_g.assign((size_t)NT * 4, 0.0);
+    std::vector<int32_t> tri_ptr(NT + 1, 0);
+    std::vector<double> tri_area(std::max(NTRI, 1));
+    s->tri_perm.assign(n_real, {});
+    s->max_tris = 0;
+    for (int r = 0; r < n_real; ++r) {
+        const stg_mesh &m = meshes[r];
+        const int32_t t0 = s->tet_off[r];
+        for (int t = 0; t < m.n_tets; ++t) {
+            vol[t0 + t] = m.tet_vol[t];
+            int nb[4];
+            uint32_t nf = 0;
+            for (int f = 0; f < 4; ++f) {
+                const int32_t j = m.tet_nbr[4 * t + f];
+                const int32_t jf = m.tet_nbr_face[4 * t + f];
+                const double g = m.tet_face_g[4 * t + f];
+                if (j >= m.n_tets || j < -1) throw StgError{STG_ERR_ARG, "neighbour index out of range"};
+                if (j >= 0 && (jf < 0 || jf > 3)) throw StgError{STG_ERR_ARG, "neighbour face slot out of range"};
+                if (j >= 0 && m.tet_nbr[4 * j + jf] != t)
+                    throw StgError{STG_ERR_ARG, "neighbour table is not symmetric"};
+                if (!(g >= 0.0) || !std::isfinite(g)) throw StgError{STG_ERR_ARG, "face factor must be finite and >= 0"};
+                if (j < 0 && g != 0.0) throw StgError{STG_ERR_ARG, "boundary face with nonzero face factor"};
+                nb[f] = j >= 0 ? j + t0 : -1;
+                nf |= (uint32_t)(j >= 0 ? jf : 0) << (8 * f);
+                s->h_face_g[4 * (size_t)(t0 + t) + f] = g;
+            }
+            nbr[t0 + t] = make_int4(nb[0], nb[1], nb[2], nb[3]);
+            nface[t0 + t] = nf;
+        }
+        // triangles sorted (stably) by inner tet
+        std::vector<int32_t> order(m.n_tris);
+        for (int i = 0; i < m.n_tris; ++i) {
+            if (m.tri_tet[i] < 0 || m.tri_tet[i] >= m.n_tets)
+                throw StgError{STG_ERR_ARG, "triangle inner tet out of range"};
+            order[i] = i;
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return m.tri_tet[a] < m.tri_tet[b]; });
+        s->tri_perm[r].assign(m.n_tris, 0);
+        for (int i = 0; i < m.n_tris; ++i) {
+            const int32_t ui = order[i];
+            s->tri_perm[r][ui] = i;
+            tri_area[s->tri_off[r] + i] = m.tri_area[ui];
+            tri_ptr[t0 + m.tri_tet[ui] + 1]++;
+        }
+    }
+    for (int t = 0; t < NT; ++t) {
+        s->max_tris = std::max(s->max_tris, tri_ptr[t + 1]);
+        tri_ptr[t + 1] += tri_ptr[t];
+    }
+    if (s->max_tris > 4) throw StgError{STG_ERR_ARG, "a tet has more than four triangles"};
+
+    // per-realization constants
+    s->h_k.assign((size_t)n_real * R, 0.0);
+    s->h_sk.assign((size_t)n_real * RS, 0.0);
+    s->h_D.assign((size_t)n_real * S, 0.0);
+    s->seeds.resize(n_real);
+    s->streams.resize(n_real);
+    for (int r = 0; r < n_real; ++r) {
+        const stg_realization &q = reals[r];
+        for (int j = 0; j < R; ++j) s->h_k[(size_t)r * R + j] = q.reac_k ? q.reac_k[j] : model->reac_k[j];
+        for (int j = 0; j < RS; ++j) s->h_sk[(size_t)r * RS + j] = q.sreac_k ? q.sreac_k[j] : model->sreac_k[j];
+        for (int j = 0; j < S; ++j) {
+            const double D = q.spec_dcst ? q.spec_dcst[j] : model->spec_dcst[j];
+            if (!(D >= 0.0) || !std::isfinite(D)) throw StgError{STG_ERR_ARG, "diffusion constant must be finite and >= 0"};
+            s->h_D[(size_t)r * S + j] = D;
+        }
+        for (int j = 0; j < R; ++j)
+            if (!(s->h_k[(size_t)r * R + j] >= 0.0)) throw StgError{STG_ERR_ARG, "rate constant must be >= 0"};
+        for (int j = 0; j < RS; ++j)
+            if (!(s->h_sk[(size_t)r * RS + j] >= 0.0)) throw StgError{STG_ERR_ARG, "rate constant must be >= 0"};
+        s->seeds[r] = q.seed;
+        s->streams[r] = q.stream;
+    }
+    build_diffusion_sets(s, model);
+    (void)SS;
+    s->dt_std.resize(n_real);
+    for (int r = 0; r < n_real; ++r) s->dt_std[r] = s->compute_dt_std(r);
+    s->nsteps.assign(n_real, 0);
+
+    HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    HIP_OK(hipEventCreate(&s->ev0));
+    HIP_OK(hipEventCreate(&s->ev1));
+    s->d_vol = dalloc<double>(NT);
+    s->d_nbr = dalloc<int4>(NT);
+    s->d_nface = dalloc<uint32_t>(NT);
+    s->d_g = dalloc<double2>((size_t)NT * 2);
+    s->d_counts = dalloc<int32_t>((size_t)std::max(S, 1) * NT);
+    s->d_out[0] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_out[1] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_tri_ptr = dalloc<int32_t>(NT + 1);
+    s->d_tri_counts = dalloc<int32_t>((size_t)std::max(SS, 1) * std::max(NTRI, 1));
+    s->d_tri_area = dalloc<double>(std::max(NTRI, 1));
+    s->d_real_tet0 = dalloc<int32_t>(n_real + 1);
+    s->d_real_k = dalloc<double>(std::max<size_t>(1, s->h_k.size()));
+    s->d_real_sk = dalloc<double>(std::max<size_t>(1, s->h_sk.size()));
+    s->d_real_D = dalloc<double>(std::max<size_t>(1, s->h_D.size()));
+    s->d_real_seed = dalloc<uint64_t>(n_real);
+    s->d_real_stream = dalloc<uint32_t>(n_real);
+    s->d_real_step0 = dalloc<uint32_t>(n_real);
+    s->d_real_nper = dalloc<int32_t>(n_real);
+    s->d_err = dalloc<int32_t>(1);
+    s->d_sum = dalloc<unsigned long long>(1);
+    HIP_OK(hipMemcpy(s->d_vol, vol.data(), sizeof(double) * NT, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_nbr, nbr.data(), sizeof(int4) * NT, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_nface, nface.data(), sizeof(uint32_t) * NT, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_g, s->h_face_g.data(), sizeof(double) * 4 * (size_t)NT, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_tri_ptr, tri_ptr.data(), sizeof(int32_t) * (NT + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_tri_area, tri_area.data(), sizeof(double) * std::max(NTRI, 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_real_tet0, s->tet_off.data(), sizeof(int32_t) * (n_real + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_real_seed, s->seeds.data(), sizeof(uint64_t) * n_real, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_real_stream, s->streams.data(), sizeof(uint32_t) * n_real, hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(s->d_counts, 0, sizeof(int32_t) * (size_t)std::max(S, 1) * NT));
+    HIP_OK(hipMemset(s->d_tri_counts, 0, sizeof(int32_t) * (size_t)std::max(SS, 1) * std::max(NTRI, 1)));
+    HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
+    upload_real_tables(s);
+}
+
+// Block decomposition for a block size: every realization starts on a fresh
+// block so a block never straddles two realizations.
+void build_blocks(stg_sim *s, int B) {
+    s->block_real.clear();
+    s->block_tet0.clear();
+    for (int r = 0; r < s->n_real; ++r) {
+        for (int t = s->tet_off[r]; t < s->tet_off[r + 1]; t += B) {
+            s->block_real.push_back(r);
+            s->block_tet0.push_back(t);
+        }
+    }
+    const size_t nb = s->block_real.size();
+    if (s->d_block_real) (void)hipFree(s->d_block_real);
+    if (s->d_block_tet0) (void)hipFree(s->d_block_tet0);
+    if (s->d_ext) (void)hipFree(s->d_ext);
+    s->d_block_real = dalloc<int32_t>(nb);
+    s->d_block_tet0 = dalloc<int32_t>(nb);
+    s->d_ext = dalloc<unsigned long long>(2 * nb);
+    HIP_OK(hipMemcpy(s->d_block_real, s->block_real.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(s->d_block_tet0, s->block_tet0.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(s->d_ext, 0, sizeof(unsigned long long) * 2 * nb));
+    s->free_graph();
+}
+
+stg::SysK make_sys(const stg_sim *s, int launch_parity) {
+    stg::SysK k{};
+    k.n_tets = s->NT;
+    k.n_tris = s->NTRI;
+    k.n_real = s->n_real;
+    k.xcd_remap = s->xcd_remap ? 1 : 0;
+    k.n_blocks = (int32_t)s->block_real.size();
+    k.vol = s->d_vol;
+    k.nbr = s->d_nbr;
+    k.nface = s->d_nface;
+    k.g = s->d_g;
+    k.counts = s->d_counts;
+    k.out_prev = s->d_out[launch_parity ^ 1];
+    k.out_cur = s->d_out[launch_parity];
+    k.tri_ptr = (s->NTRI > 0 && s->RS > 0) ? s->d_tri_ptr : nullptr;
+    k.tri_counts = s->d_tri_counts;
+    k.tri_area = s->d_tri_area;
+    k.block_real = s->d_block_real;
+    k.block_tet0 = s->d_block_tet0;
+    k.real_tet0 = s->d_real_tet0;
+    k.real_k = s->d_real_k;
+    k.real_sk = s->d_real_sk;
+    k.real_D = s->d_real_D;
+    k.real_seed = s->d_real_seed;
+    k.real_stream = s->d_real_stream;
+    k.real_step0 = s->d_real_step0;
+    k.real_nper = s->d_real_nper;
+    k.dt_tab = s->d_dt_tab;
+    k.ext = s->d_ext;
+    k.err = s->d_err;
+    return k;
+}
+
+void launch_all(stg_sim *s, int L) {
+    const int B = s->block;
+    const int nb = (int)s->block_real.size();
+    const size_t smem = s->smem_bytes(B);
+    for (int l = 0; l <= L; ++l) {
+        const stg::SysK k = make_sys(s, l & 1);
+        if (B <= 256)
+            hipLaunchKernelGGL(stg::k_period<256>, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
+        else
+            hipLaunchKernelGGL(stg::k_period<1024>, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+void run_impl(stg_sim *s, double endtime) {
+    if (!(endtime >= s->time) && !almost_equal(s->time, endtime))
+        throw StgError{STG_ERR_ARG, "Endtime is before current simulation time"};
+    if (almost_equal(s->time, endtime)) {
+        s->time = endtime;
+        s->last_launches = 0;
+        return;
+    }
+    if (s->block_real.empty()) build_blocks(s, s->block);
+    // per-realization schedules
+    std::vector<std::vector<double>> dts(s->n_real);
+    int L = 0;
+    for (int r = 0; r < s->n_real; ++r) {
+        schedule(s->time, endtime, s->dt_std[r], s->mode, dts[r]);
+        if (dts[r].size() > (size_t)INT32_MAX / 2) throw StgError{STG_ERR_ARG, "too many periods"};
+        L = std::max<int>(L, (int)dts[r].size());
+    }
+    if (L > 0) {
+        std::vector<double> tab((size_t)L * s->n_real, 0.0);
+        std::vector<int32_t> nper(s->n_real);
+        std::vector<uint32_t> step0(s->n_real);
+        for (int r = 0; r < s->n_real; ++r) {
+            nper[r] = (int32_t)dts[r].size();
+            step0[r] = (uint32_t)s->nsteps[r];
+            for (size_t i = 0; i < dts[r].size(); ++i) tab[i * s->n_real + r] = dts[r][i];
+        }
+        if (tab.size() > s->dt_tab_cap) {
+            if (s->d_dt_tab) (void)hipFree(s->d_dt_tab);
+            s->d_dt_tab = nullptr;
+            s->dt_tab_cap = tab.size() * 2;
+            s->d_dt_tab = dalloc<double>(s->dt_tab_cap);
+            s->free_graph();
+        }
+        HIP_OK(hipMemcpyAsync(s->d_dt_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(s->d_real_nper, nper.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(s->d_real_step0, step0.data(), sizeof(uint32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
+        const size_t smem = s->smem_bytes(s->block);
+        if (smem > 64 * 1024) {
+            HIP_OK(hipFuncSetAttribute((const void *)stg::k_period<256>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+            HIP_OK(hipFuncSetAttribute((const void *)stg::k_period<1024>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        }
+        HIP_OK(hipEventRecord(s->ev0, s->stream));
+        if (s->use_graph) {
+            if (s->graph_launches != L || s->graph_block != s->block) {
+                s->free_graph();
+                HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+                launch_all(s, L);
+                HIP_OK(hipStreamEndCapture(s->stream, &s->graph));
+                HIP_OK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
+                s->graph_launches = L;
+                s->graph_block = s->block;
+            }
+            HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
+        } else {
+            launch_all(s, L);
+        }
+        HIP_OK(hipEventRecord(s->ev1, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->last_ms = ms;
+        s->last_launches = L + 1;
+        int32_t err = 0;
+        HIP_OK(hipMemcpy(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (err) {
+            HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
+            throw StgError{STG_ERR_PROG, "SSA exceeded 4194304 events in one tet and period"};
+        }
+        for (int r = 0; r < s->n_real; ++r) s->nsteps[r] += dts[r].size();
+    }
+    s->time = endtime;
+}
+
+}  // namespace
+
+extern "C" {
+
+int stg_create(const stg_model *model, const stg_mesh *mesh, uint64_t seed, int32_t mode,
+               stg_sim **out) {
+    stg_realization r{};
+    r.seed = seed;
+    r.stream = 0;
+    return stg_create_batch(model, 1, mesh, &r, mode, out);
+}
+
+int stg_create_batch(const stg_model *model, int32_t n_real, const stg_mesh *meshes,
+                     const stg_realization *reals, int32_t mode, stg_sim **out) {
+    if (!out) {
+        g_create_error = "null output handle";
+        return STG_ERR_ARG;
+    }
+    *out = nullptr;
+    stg_sim *s = new (std::nothrow) stg_sim();
+    if (!s) {
+        g_create_error = "host allocation failed";
+        return STG_ERR_SYS;
+    }
+    const int rc = guard(nullptr, [&] { create_impl(s, model, n_real, meshes, reals, mode); });
+    if (rc != STG_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return STG_OK;
+}
+
+int stg_destroy(stg_sim *sim) {
+    delete sim;
+    return STG_OK;
+}
+
+const char *stg_last_error(const stg_sim *sim) {
+    return sim ? sim->err.c_str() : g_create_error.c_str();
+}
+
+int stg_reset(stg_sim *s) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        HIP_OK(hipMemsetAsync(s->d_counts, 0, sizeof(int32_t) * (size_t)std::max(s->S, 1) * s->NT, s->stream));
+        HIP_OK(hipMemsetAsync(s->d_tri_counts, 0,
+                              sizeof(int32_t) * (size_t)std::max(s->SS, 1) * std::max(s->NTRI, 1), s->stream));
+        if (s->d_ext)
+            HIP_OK(hipMemsetAsync(s->d_ext, 0, sizeof(unsigned long long) * 2 * s->block_real.size(), s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        s->time = 0.0;
+        std::fill(s->nsteps.begin(), s->nsteps.end(), 0);
+    });
+}
+
+int stg_run(stg_sim *s, double endtime) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] { run_impl(s, endtime); });
+}
+
+int stg_advance(stg_sim *s, double adv) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (adv < 0.0) throw StgError{STG_ERR_ARG, "Time to advance cannot be negative"};
+        run_impl(s, s->time + adv);
+    });
+}
+
+int stg_set_seed(stg_sim *s, int32_t real, uint64_t seed) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        s->seeds[real] = seed;
+        HIP_OK(hipMemcpy(s->d_real_seed + real, &seed, sizeof(uint64_t), hipMemcpyHostToDevice));
+    });
+}
+
+int stg_get_time(const stg_sim *s, double *t) {
+    if (!s || !t) return STG_ERR_ARG;
+    *t = s->time;
+    return STG_OK;
+}
+
+int stg_set_time(stg_sim *s, double t) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (!(t >= 0.0)) throw StgError{STG_ERR_ARG, "time must be >= 0"};
+        s->time = t;
+    });
+}
+
+int stg_get_nsteps(const stg_sim *s, int32_t real, uint64_t *n) {
+    if (!s || !n) return STG_ERR_ARG;
+    return guard(const_cast<stg_sim *>(s), [&] {
+        check_real(s, real);
+        *n = s->nsteps[real];
+    });
+}
+
+int stg_get_update_period(const stg_sim *s, int32_t real, double *dt) {
+    if (!s || !dt) return STG_ERR_ARG;
+    return guard(const_cast<stg_sim *>(s), [&] {
+        check_real(s, real);
+        *dt = s->dt_std[real];
+    });
+}
+
+int stg_get_extents(const stg_sim *cs, int32_t real, uint64_t *reac, uint64_t *diff) {
+    stg_sim *s = const_cast<stg_sim *>(cs);
+    if (!s || !reac || !diff) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        *reac = 0;
+        *diff = 0;
+        if (s->block_real.empty()) return;
+        std::vector<unsigned long long> ext(2 * s->block_real.size());
+        HIP_OK(hipMemcpy(ext.data(), s->d_ext, sizeof(unsigned long long) * ext.size(), hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < s->block_real.size(); ++b) {
+            if (s->block_real[b] != real) continue;
+            *reac += ext[2 * b];
+            *diff += ext[2 * b + 1];
+        }
+    });
+}
+
+int stg_set_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int32_t *counts) {
+    if (!s || !counts) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        const int n = s->tet_off[real + 1] - s->tet_off[real];
+        for (int i = 0; i < n; ++i)
+            if (counts[i] < 0) throw StgError{STG_ERR_ARG, "molecule counts must be >= 0"};
+        HIP_OK(hipMemcpy(s->d_counts + (size_t)spec * s->NT + s->tet_off[real], counts, sizeof(int32_t) * n,
+                         hipMemcpyHostToDevice));
+    });
+}
+
+int stg_get_tet_counts(stg_sim *s, int32_t real, int32_t spec, int32_t *counts) {
+    if (!s || !counts) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        const int n = s->tet_off[real + 1] - s->tet_off[real];
+        HIP_OK(hipMemcpy(counts, s->d_counts + (size_t)spec * s->NT + s->tet_off[real], sizeof(int32_t) * n,
+                         hipMemcpyDeviceToHost));
+    });
+}
+
+int stg_set_tri_counts(stg_sim *s, int32_t real, int32_t sspec, const int32_t *counts) {
+    if (!s || !counts) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (sspec < 0 || sspec >= s->SS) throw StgError{STG_ERR_ARG, "surface species index out of range"};
+        const int n = s->tri_off[real + 1] - s->tri_off[real];
+        std::vector<int32_t> internal(std::max(n, 1));
+        for (int i = 0; i < n; ++i) {
+            if (counts[i] < 0) throw StgError{STG_ERR_ARG, "molecule counts must be >= 0"};
+            internal[s->tri_perm[real][i]] = counts[i];
+        }
+        if (n > 0)
+            HIP_OK(hipMemcpy(s->d_tri_counts + (size_t)sspec * s->NTRI + s->tri_off[real], internal.data(),
+                             sizeof(int32_t) * n, hipMemcpyHostToDevice));
+    });
+}
+
+int stg_get_tri_counts(stg_sim *s, int32_t real, int32_t sspec, int32_t *counts) {
+    if (!s || !counts) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (sspec < 0 || sspec >= s->SS) throw StgError{STG_ERR_ARG, "surface species index out of range"};
+        const int n = s->tri_off[real + 1] - s->tri_off[real];
+        if (n == 0) return;
+        std::vector<int32_t> internal(n);
+        HIP_OK(hipMemcpy(internal.data(), s->d_tri_counts + (size_t)sspec * s->NTRI + s->tri_off[real],
+                         sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) counts[i] = internal[s->tri_perm[real][i]];
+    });
+}
+
+int stg_get_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64_t *tets, size_t n,
+                             double *counts) {
+    if (!s || (n > 0 && (!tets || !counts))) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        const int64_t nt = s->tet_off[real + 1] - s->tet_off[real];
+        for (size_t i = 0; i < n; ++i)
+            if (tets[i] < 0 || tets[i] >= nt) throw StgError{STG_ERR_ARG, "tetrahedron index out of range"};
+        if (n == 0) return;
+        if (n > s->idx_cap) {
+            if (s->d_idx) (void)hipFree(s->d_idx);
+            if (s->d_gather) (void)hipFree(s->d_gather);
+            s->d_idx = nullptr;
+            s->d_gather = nullptr;
+            s->idx_cap = n;
+            s->d_idx = dalloc<int64_t>(n);
+            s->d_gather = dalloc<double>(n);
+        }
+        HIP_OK(hipMemcpyAsync(s->d_idx, tets, sizeof(int64_t) * n, hipMemcpyHostToDevice, s->stream));
+        const int B = 256;
+        const int nb = (int)((n + B - 1) / B);
+        hipLaunchKernelGGL(stg::k_gather_counts, dim3(nb), dim3(B), 0, s->stream, s->d_counts + (size_t)spec * s->NT,
+                           s->d_idx, n, s->tet_off[real], s->d_gather);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(counts, s->d_gather, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
+static void sum_range(stg_sim *s, const int32_t *ptr, int32_t n, int64_t *total) {
+    unsigned long long zero = 0, res = 0;
+    HIP_OK(hipMemcpyAsync(s->d_sum, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    if (n > 0) {
+        const int B = 256;
+        int nb = (n + B - 1) / B;
+        nb = std::min(nb, 2048);
+        hipLaunchKernelGGL(stg::k_sum_counts, dim3(nb), dim3(B), 0, s->stream, ptr, n, s->d_sum);
+        HIP_OK(hipGetLastError());
+    }
+    HIP_OK(hipMemcpyAsync(&res, s->d_sum, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+    *total = (int64_t)res;
+}
+
+int stg_sum_tet_counts(stg_sim *s, int32_t real, int32_t spec, int64_t *total) {
+    if (!s || !total) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        sum_range(s, s->d_counts + (size_t)spec * s->NT + s->tet_off[real], s->tet_off[real + 1] - s->tet_off[real],
+                  total);
+    });
+}
+
+int stg_sum_tri_counts(stg_sim *s, int32_t real, int32_t sspec, int64_t *total) {
+    if (!s || !total) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (sspec < 0 || sspec >= s->SS) throw StgError{STG_ERR_ARG, "surface species index out of range"};
+        sum_range(s, s->d_tri_counts + (size_t)sspec * s->NTRI + s->tri_off[real],
+                  s->tri_off[real + 1] - s->tri_off[real], total);
+    });
+}
+
+int stg_set_reac_k(stg_sim *s, int32_t real, int32_t reac, double k) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (reac < 0 || reac >= s->R) throw StgError{STG_ERR_ARG, "reaction index out of range"};
+        if (!(k >= 0.0) || !std::isfinite(k)) throw StgError{STG_ERR_ARG, "rate constant must be finite and >= 0"};
+        s->h_k[(size_t)real * s->R + reac] = k;
+        upload_real_tables(s);
+    });
+}
+
+int stg_set_sreac_k(stg_sim *s, int32_t real, int32_t sreac, double k) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (sreac < 0 || sreac >= s->RS) throw StgError{STG_ERR_ARG, "surface reaction index out of range"};
+        if (!(k >= 0.0) || !std::isfinite(k)) throw StgError{STG_ERR_ARG, "rate constant must be finite and >= 0"};
+        s->h_sk[(size_t)real * s->RS + sreac] = k;
+        upload_real_tables(s);
+    });
+}
+
+int stg_set_dcst(stg_sim *s, int32_t real, int32_t spec, double dcst) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        if (!(dcst >= 0.0) || !std::isfinite(dcst))
+            throw StgError{STG_ERR_ARG, "diffusion constant must be finite and >= 0"};
+        const bool was_diff = (std::find(s->mk.dspec, s->mk.dspec + s->mk.n_diff, spec) != s->mk.dspec + s->mk.n_diff);
+        if (dcst > 0.0 && !was_diff)
+            throw StgError{STG_ERR_NOTIMPL, "enabling diffusion of a species that was immobile at creation"};
+        s->h_D[(size_t)real * s->S + spec] = dcst;
+        s->dt_std[real] = s->compute_dt_std(real);
+        upload_real_tables(s);
+    });
+}
+
+int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (block < 64 || block > 1024 || block % 64 != 0)
+            throw StgError{STG_ERR_ARG, "block must be a multiple of 64 in [64, 1024]"};
+        const size_t smem = s->smem_bytes(block);
+        if (smem > 160 * 1024) throw StgError{STG_ERR_ARG, "block too large for the model's LDS footprint"};
+        // extents are kept per block: fold them into realization totals first
+        std::vector<unsigned long long> keep;
+        std::vector<int32_t> old_real = s->block_real;
+        if (!s->block_real.empty()) {
+            keep.resize(2 * s->block_real.size());
+            HIP_OK(hipMemcpy(keep.data(), s->d_ext, sizeof(unsigned long long) * keep.size(), hipMemcpyDeviceToHost));
+        }
+        if (block != s->block || s->block_real.empty()) {
+            s->block = block;
+            build_blocks(s, block);
+            if (!keep.empty()) {
+                // carry totals into the first block of each realization
+                std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
+                std::vector<unsigned long long> tot(2 * s->n_real, 0);
+                for (size_t b = 0; b < old_real.size(); ++b) {
+                    tot[2 * old_real[b]] += keep[2 * b];
+                    tot[2 * old_real[b] + 1] += keep[2 * b + 1];
+                }
+                std::vector<bool> done(s->n_real, false);
+                for (size_t b = 0; b < s->block_real.size(); ++b) {
+                    const int r = s->block_real[b];
+                    if (done[r]) continue;
+                    done[r] = true;
+                    ext[2 * b] = tot[2 * r];
+                    ext[2 * b + 1] = tot[2 * r + 1];
+                }
+                HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
+            }
+        }
+        s->use_graph = use_graph != 0;
+        s->free_graph();
+    });
+}
+
+int stg_get_sizes(const stg_sim *s, int32_t real, int32_t *n_real, int32_t *n_tets, int32_t *n_tris) {
+    if (!s) return STG_ERR_ARG;
+    return guard(const_cast<stg_sim *>(s), [&] {
+        check_real(s, real);
+        if (n_real) *n_real = s->n_real;
+        if (n_tets) *n_tets = s->tet_off[real + 1] - s->tet_off[real];
+        if (n_tris) *n_tris = s->tri_off[real + 1] - s->tri_off[real];
+    });
+}
+
+int stg_last_run_kernel_ms(const stg_sim *s, double *ms_per_launch, int32_t *n) {
+    if (!s || !ms_per_launch || !n) return STG_ERR_ARG;
+    *n = s->last_launches;
+    *ms_per_launch = s->last_launches > 0 ? s->last_ms / s->last_launches : 0.0;
+    return STG_OK;
+}
+
+}  // extern "C"

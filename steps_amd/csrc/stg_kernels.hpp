@@ -1,0 +1,432 @@
+// Device kernels of the operator-split period (gfx950).
+//
+// One launch = one split period for every realization that still has periods
+// left in the current run, fused as
+//   entering(previous period)  ->  SSA over [0, dt)  ->  leaving(this period)
+// One thread owns one tetrahedron (and the membrane triangles whose inner tet
+// it is). Counts are read once and written once per period; the time-averaged
+// occupancy used by the leaving pass lives in LDS for the whole period, so it
+// never touches HBM. Leaving molecules go to a per-(diffusing species, tet,
+// face) int32 slot; the owner of the neighbouring tet gathers them at the
+// start of the next launch from the other half of a ping-pong buffer, so no
+// atomics are needed and the result is independent of scheduling.
+//
+// Reference semantics:
+//   SSA         src/steps/mpi/dist/tetopsplit/operator/ssa_operator.cpp:24-47,
+//               kproc/propensities.hpp:543-586 (Direct group draw)
+//   propensity  kproc/reactions.cpp:66-119, kproc/surface_reactions.cpp:176-240
+//   occupancy   mol_state.hpp:87-200
+//   diffusion   operator/diffusion_operator.cpp:41-238, kproc/diffusions.hpp:211-224
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "stg_random.hpp"
+
+namespace stg {
+
+constexpr int kMaxSpecs = 16;
+constexpr int kMaxReacs = 32;
+constexpr int kMaxSSpecs = 8;
+constexpr int kMaxSReacs = 16;
+constexpr int kMaxEntries = 384;
+constexpr int kSurfFlag = 0x80;  // entry species >= 0x80: surface species (idx & 0x7f)
+constexpr int kDiffThreshold = 10;  // diffusion_operator.hpp:99
+constexpr long kMaxEventsPerPeriod = 1L << 22;
+constexpr double kAvogadro = 6.02214179e23;  // src/steps/math/constants.hpp:45
+constexpr double kDblEpsilon = 2.220446049250313080847e-16;
+
+// Model tables, passed by value (kernel-argument segment, read with scalar loads).
+struct ModelK {
+    int32_t n_specs, n_reacs, n_sspecs, n_sreacs, n_diff;
+    int32_t tracked_mask;  // bit s: occupancy tracked for species s
+    int32_t dspec[kMaxSpecs];
+    int8_t order[kMaxReacs];
+    int8_t sorder[kMaxSReacs];
+    int8_t s_nvol[kMaxSReacs];  // number of volume reactants of a surface reaction
+    uint16_t lhs_off[kMaxReacs + 1], upd_off[kMaxReacs + 1];
+    uint16_t slhs_off[kMaxSReacs + 1], supd_off[kMaxSReacs + 1];
+    uint8_t ent_spec[kMaxEntries];
+    int8_t ent_val[kMaxEntries];
+};
+
+// Mesh, state and per-realization tables (device pointers).
+struct SysK {
+    int32_t n_tets;        // all realizations
+    int32_t n_tris;
+    int32_t n_real;
+    int32_t xcd_remap;     // nonzero: contiguous block ranges per XCD
+    int32_t n_blocks;
+    const double *vol;           // [NT]
+    const int4 *nbr;             // [NT] global neighbour index, -1 = none
+    const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
+    const double2 *g;            // [NT*2] face factors (f0,f1),(f2,f3)
+    int32_t *counts;             // [S][NT]
+    const int32_t *out_prev;     // [ND][NT][4]
+    int32_t *out_cur;            // [ND][NT][4]
+    const int32_t *tri_ptr;      // [NT+1] or null when there are no triangles
+    int32_t *tri_counts;         // [SS][NTRI]
+    const double *tri_area;      // [NTRI]
+    const int32_t *block_real;   // [n_blocks]
+    const int32_t *block_tet0;   // [n_blocks]
+    const int32_t *real_tet0;    // [R+1]
+    const double *real_k;        // [R][n_reacs]
+    const double *real_sk;       // [R][n_sreacs]
+    const double *real_D;        // [R][n_specs]
+    const uint64_t *real_seed;   // [R]
+    const uint32_t *real_stream; // [R]
+    const uint32_t *real_step0;  // [R] period counter at launch 0
+    const int32_t *real_nper;    // [R] periods in this run
+    const double *dt_tab;        // [L][R]
+    unsigned long long *ext;     // [n_blocks][2] events, molecules moved
+    int32_t *err;                // error flags
+};
+
+__device__ __forceinline__ double scaled_const(double k, int order, double scale) {
+    const int o1 = order - 1;
+    if (o1 == 0) return k;
+    if (o1 < 0) return k * ipow(scale, -o1);
+    return k / ipow(scale, o1);
+}
+
+// h *= n (n-1) ... (n-L+1), multiplied from the smallest factor up
+__device__ __forceinline__ double falling(double h, int32_t n, int32_t L) {
+    for (int32_t i = L - 1; i >= 0; --i) h = h * (double)(n - i);
+    return h;
+}
+
+// Per-thread LDS arrays are laid out [index][blockDim] so that a wavefront
+// touching the same index hits 64 consecutive banks.
+struct Lds {
+    int32_t *cnt;   // [S][B]
+    double *corr;   // [S][B]
+    double *a;      // [K][B]
+    int B, tid;
+    __device__ __forceinline__ int32_t &c(int s) const { return cnt[s * B + tid]; }
+    __device__ __forceinline__ double &o(int s) const { return corr[s * B + tid]; }
+    __device__ __forceinline__ double &p(int j) const { return a[j * B + tid]; }
+};
+
+__device__ __forceinline__ double surf_prop(const ModelK &m, const SysK &sys, const Lds &L,
+                                            int r, int tri, double k, double vscale) {
+    double h = 1.0;
+    for (int e = m.slhs_off[r]; e < m.slhs_off[r + 1]; ++e) {
+        const int sp = m.ent_spec[e];
+        const int mult = m.ent_val[e];
+        int32_t n;
+        if (sp & kSurfFlag) n = sys.tri_counts[(size_t)(sp & 0x7f) * sys.n_tris + tri];
+        else n = L.c(sp);
+        if (mult > n) return 0.0;
+        h = falling(h, n, mult);
+    }
+    const double scale = m.s_nvol[r] == 0 ? sys.tri_area[tri] * kAvogadro : vscale;
+    return h * scaled_const(k, m.sorder[r], scale);
+}
+
+// Direct-method SSA of one tet group over [0, dt). Returns events, or -1 when
+// the per-period event cap is hit.
+__device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
+                          uint32_t ltet, uint32_t period, double dt) {
+    const int R = m.n_reacs, RS = m.n_sreacs;
+    int tp0 = 0, ntri = 0;
+    if (sys.tri_ptr != nullptr && RS > 0) {
+        tp0 = sys.tri_ptr[t];
+        ntri = sys.tri_ptr[t + 1] - tp0;
+    }
+    const int K = R + ntri * RS;
+    if (K == 0) return 0;
+    const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
+    const double *kr = sys.real_k + (size_t)r * R;
+    const double *ksr = sys.real_sk + (size_t)r * RS;
+    Philox g;
+    g.init(sys.real_seed[r], ltet, period, sys.real_stream[r], 0u);
+    long nev = 0;
+    double tcur = 0.0;
+    while (true) {
+        double a0 = 0.0;
+        int j = 0;
+        for (int q = 0; q < R; ++q, ++j) {
+            double h = 1.0;
+            bool zero = false;
+            for (int e = m.lhs_off[q]; e < m.lhs_off[q + 1]; ++e) {
+                const int sp = m.ent_spec[e];
+                const int mult = m.ent_val[e];
+                const int32_t n = L.c(sp);
+                if (mult > n) {
+                    zero = true;
+                    break;
+                }
+                h = falling(h, n, mult);
+            }
+            const double av = zero ? 0.0 : h * scaled_const(kr[q], m.order[q], vscale);
+            L.p(j) = av;
+            a0 = a0 + av;
+        }
+        for (int i = 0; i < ntri; ++i) {
+            for (int q = 0; q < RS; ++q, ++j) {
+                const double av = surf_prop(m, sys, L, q, tp0 + i, ksr[q], vscale);
+                L.p(j) = av;
+                a0 = a0 + av;
+            }
+        }
+        if (a0 < kDblEpsilon) break;
+        const double u1 = g.unif();
+        const double u2 = g.unif();
+        const double tev = tcur + (-dlog(u1) / a0);
+        if (tev > dt) break;
+        tcur = tev;
+        const double x = u2 * a0;
+        int sel = -1;
+        double acc = 0.0;
+        for (int q = 0; q < K; ++q) {
+            acc = acc + L.p(q);
+            if (acc > x) {
+                sel = q;
+                break;
+            }
+        }
+        if (sel < 0) {
+            for (int q = K - 1; q >= 0; --q) {
+                if (L.p(q) > 0.0) {
+                    sel = q;
+                    break;
+                }
+            }
+        }
+        if (sel < R) {
+            for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
+                const int sp = m.ent_spec[e];
+                const int dv = m.ent_val[e];
+                L.c(sp) += dv;
+                if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+            }
+        } else {
+            const int i = (sel - R) / RS;
+            const int q = (sel - R) - i * RS;
+            const int tri = tp0 + i;
+            for (int e = m.supd_off[q]; e < m.supd_off[q + 1]; ++e) {
+                const int sp = m.ent_spec[e];
+                const int dv = m.ent_val[e];
+                if (sp & kSurfFlag) {
+                    sys.tri_counts[(size_t)(sp & 0x7f) * sys.n_tris + tri] += dv;
+                } else {
+                    L.c(sp) += dv;
+                    if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                }
+            }
+        }
+        if (++nev >= kMaxEventsPerPeriod) return -1;
+    }
+    return nev;
+}
+
+__device__ __forceinline__ bool almost_zero_dt(double dt) {
+    // util::almost_equal(dt, 0.0) of src/steps/util/collections.hpp:143-149
+    const double ad = fabs(dt);
+    return ad <= kDblEpsilon * ad * 4 || ad < 2.2250738585072014e-308;
+}
+
+// Leaving pass of one (tet, species): returns molecules moved, writes 4 slots.
+__device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, double D,
+                         const double r4in[4], double dt, uint64_t seed, uint32_t ltet,
+                         uint32_t period, uint32_t stream, int4 &o) {
+    o = make_int4(0, 0, 0, 0);
+    const int32_t n = L.c(sp);
+    double r4[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) r4[f] = D * r4in[f];
+    const double sum = ((r4[0] + r4[1]) + r4[2]) + r4[3];
+    if (n <= 0 || !(sum > 0.0)) return 0;
+    Philox g;
+    g.init(seed, ltet, period, stream, 1u + (uint32_t)sp);
+    double occ = (double)n;
+    if (tracked && !almost_zero_dt(dt)) occ = (double)n + L.o(sp) / dt;
+    const double u = g.unif();
+    const double fl = floor(occ);
+    int32_t mp = (int32_t)fl + ((occ - fl) > u ? 1 : 0);
+    mp = mp > n ? n : mp;
+    mp = mp < 0 ? 0 : mp;
+    double p = dt * sum;
+    p = p > 1.0 ? 1.0 : p;
+    p = p < 0.0 ? 0.0 : p;
+    const int32_t nl = binomial(g, mp, p);
+    if (nl <= 0) return 0;
+    L.c(sp) = n - nl;
+    int lastf = -1;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+        if (r4[f] > 0.0) lastf = f;
+    int32_t k4[4] = {0, 0, 0, 0};
+    if (nl <= kDiffThreshold) {
+        for (int32_t mol = 0; mol < nl; ++mol) {
+            int chosen = -1;
+            for (int tries = 0; tries < 64 && chosen < 0; ++tries) {
+                const double selv = g.unif() * sum;
+                double cum = 0.0;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    cum = cum + r4[f];
+                    if (chosen < 0 && selv < cum) chosen = f;
+                }
+            }
+            if (chosen < 0) chosen = lastf;
+#pragma unroll
+            for (int f = 0; f < 4; ++f) k4[f] += (chosen == f) ? 1 : 0;
+        }
+    } else {
+        int32_t rem = nl;
+        double rsum = sum;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            if (!(r4[f] > 0.0)) continue;
+            int32_t k;
+            if (f == lastf) {
+                k = rem;
+            } else {
+                const double pf = r4[f] / rsum;
+                k = pf >= 1.0 ? rem : binomial(g, rem, pf);
+                rsum = rsum - r4[f];
+            }
+            k4[f] = k;
+            rem -= k;
+        }
+    }
+    o = make_int4(k4[0], k4[1], k4[2], k4[3]);
+    return nl;
+}
+
+template <int MAXB>
+__global__ void __launch_bounds__(MAXB) k_period(const ModelK m, const SysK sys, int launch) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = blockDim.x;
+    const int tid = threadIdx.x;
+    int b = blockIdx.x;
+    if (sys.xcd_remap) {
+        // Hardware dispatches consecutive blocks round-robin over 8 XCDs; give
+        // each XCD a contiguous range of tets so face neighbours share an L2.
+        const int nb = sys.n_blocks;
+        const int per = (nb + 7) / 8;
+        const int x = b & 7, i = b >> 3;
+        const int full = nb - 8 * (per - 1);  // XCDs that get 'per' blocks
+        const int lb = (x < full) ? x * per + i : full * per + (x - full) * (per - 1) + i;
+        b = lb;
+    }
+    const int r = sys.block_real[b];
+    const int np = sys.real_nper[r];
+    const bool do_enter = launch >= 1 && launch <= np;
+    const bool do_step = launch < np;
+    if (!do_enter && !do_step) return;  // uniform over the block
+
+    const int S = m.n_specs;
+    Lds L;
+    L.B = B;
+    L.tid = tid;
+    L.cnt = reinterpret_cast<int32_t *>(smem);
+    L.corr = reinterpret_cast<double *>(smem + (((size_t)S * B * 4 + 15) & ~(size_t)15));
+    L.a = L.corr + (size_t)S * B;
+
+    const int t = sys.block_tet0[b] + tid;
+    const int tend = sys.real_tet0[r + 1];
+    const bool valid = t < tend;
+    long nev = 0;
+    int32_t moved = 0;
+    if (valid) {
+        const int NT = sys.n_tets;
+        for (int s = 0; s < S; ++s) {
+            L.c(s) = sys.counts[(size_t)s * NT + t];
+            L.o(s) = 0.0;
+        }
+        const int4 nb4 = sys.nbr[t];
+        const uint32_t nf = sys.nface[t];
+        if (do_enter) {
+            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            for (int d = 0; d < m.n_diff; ++d) {
+                const int32_t *op = sys.out_prev + (size_t)d * NT * 4;
+                int32_t add = 0;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    if (nbv[f] >= 0) add += op[(size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff)];
+                }
+                L.c(m.dspec[d]) += add;
+            }
+        }
+        if (do_step) {
+            const int tet0 = sys.real_tet0[r];
+            const uint32_t ltet = (uint32_t)(t - tet0);
+            const uint32_t period = sys.real_step0[r] + (uint32_t)launch;
+            const double dt = sys.dt_tab[(size_t)launch * sys.n_real + r];
+            nev = ssa_group(m, sys, L, r, t, ltet, period, dt);
+            if (nev < 0) {
+                atomicOr(sys.err, 1);
+                nev = 0;
+            }
+            if (m.n_diff > 0) {
+                const double2 g01 = sys.g[2 * (size_t)t];
+                const double2 g23 = sys.g[2 * (size_t)t + 1];
+                const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
+                const double *Dr = sys.real_D + (size_t)r * S;
+                const uint64_t seed = sys.real_seed[r];
+                const uint32_t stream = sys.real_stream[r];
+                int4 *oc = reinterpret_cast<int4 *>(sys.out_cur);
+                for (int d = 0; d < m.n_diff; ++d) {
+                    const int sp = m.dspec[d];
+                    int4 o;
+                    moved += leave(sys, L, sp, (m.tracked_mask >> sp) & 1, Dr[sp], r4, dt, seed,
+                                   ltet, period, stream, o);
+                    oc[(size_t)d * NT + t] = o;
+                }
+            }
+        }
+        for (int s = 0; s < S; ++s) sys.counts[(size_t)s * NT + t] = L.c(s);
+    }
+    // per-block extents: wavefront reduction, then one slot per block
+    unsigned long long ev = (unsigned long long)nev, mv = (unsigned long long)moved;
+    for (int off = 32; off > 0; off >>= 1) {
+        ev += __shfl_xor(ev, off, 64);
+        mv += __shfl_xor(mv, off, 64);
+    }
+    __shared__ unsigned long long red[2][16];
+    const int wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        red[0][wave] = ev;
+        red[1][wave] = mv;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long se = 0, sm = 0;
+        for (int w = 0; w < (B + 63) / 64; ++w) {
+            se += red[0][w];
+            sm += red[1][w];
+        }
+        sys.ext[2 * (size_t)b] += se;
+        sys.ext[2 * (size_t)b + 1] += sm;
+    }
+}
+
+// getBatchTetSpecCountsNP gather
+__global__ void k_gather_counts(const int32_t *__restrict__ counts, const int64_t *__restrict__ idx,
+                                size_t n, int32_t base, double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)counts[base + idx[i]];
+}
+
+// Sum of n int32 counts into one int64 (block partials + one atomic per block).
+__global__ void k_sum_counts(const int32_t *__restrict__ v, int32_t n,
+                             unsigned long long *__restrict__ total) {
+    long long acc = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        acc += v[i];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ long long red[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long s = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) s += red[w];
+        atomicAdd(total, (unsigned long long)s);
+    }
+}
+
+}  // namespace stg

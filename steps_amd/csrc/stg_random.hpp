@@ -1,0 +1,208 @@
+// Device random streams and samplers for the operator-split step.
+//
+// Counter-based Philox4x32 with 8 rounds (the round count of the generator
+// STEPS wraps, src/steps/rng/r123.hpp:76). A stream is addressed by
+//   key     = (seed low 32, seed high 32)
+//   counter = (tet, period, stream id, channel << 24 | block)
+// so every (tet, period, channel) owns an independent sequence and results do
+// not depend on launch shape, thread count or batch composition.
+//
+// All floating-point arithmetic here is + - * / with contraction disabled
+// (-ffp-contract=off) so results are reproducible bit-for-bit on any IEEE-754
+// machine; log and sqrt are written out for the same reason.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace stg {
+
+struct Philox {
+    uint32_t k0, k1;
+    uint32_t c0, c1, c2, c3;
+    uint32_t w0, w1, w2, w3;
+    int left;  // words still unread in w0..w3 (consumed from w0)
+
+    __device__ __forceinline__ void init(uint64_t seed, uint32_t tet, uint32_t period,
+                                         uint32_t stream, uint32_t channel) {
+        k0 = (uint32_t)seed;
+        k1 = (uint32_t)(seed >> 32);
+        c0 = tet;
+        c1 = period;
+        c2 = stream;
+        c3 = channel << 24;
+        left = 0;
+    }
+
+    __device__ __forceinline__ void refill() {
+        uint32_t x0 = c0, x1 = c1, x2 = c2, x3 = c3;
+        uint32_t a = k0, b = k1;
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            if (r > 0) {
+                a += 0x9E3779B9u;
+                b += 0xBB67AE85u;
+            }
+            const uint32_t lo0 = 0xD2511F53u * x0;
+            const uint32_t hi0 = __umulhi(0xD2511F53u, x0);
+            const uint32_t lo1 = 0xCD9E8D57u * x2;
+            const uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+            const uint32_t y0 = hi1 ^ x1 ^ a;
+            const uint32_t y2 = hi0 ^ x3 ^ b;
+            x0 = y0;
+            x1 = lo1;
+            x2 = y2;
+            x3 = lo0;
+        }
+        w0 = x0;
+        w1 = x1;
+        w2 = x2;
+        w3 = x3;
+        c3 = (c3 & 0xFF000000u) | ((c3 + 1u) & 0x00FFFFFFu);
+        left = 4;
+    }
+
+    __device__ __forceinline__ uint32_t next() {
+        if (left == 0) refill();
+        const uint32_t r = w0;
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+        --left;
+        return r;
+    }
+
+    // uniform on (0, 1), 32 random bits
+    __device__ __forceinline__ double unif() { return ((double)next() + 0.5) * 0x1p-32; }
+};
+
+__device__ __forceinline__ double dlog(double x) {
+    if (!(x > 0.0)) return -__builtin_inf();
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {
+        x = x * 0x1p54;
+        b = (uint64_t)__double_as_longlong(x);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffull) | 0x3ff0000000000000ull));
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double f = (m - 1.0) / (m + 1.0);
+    const double f2 = f * f;
+    // atanh series 1 + f^2/3 + f^4/5 + ... + f^20/21, Horner from the top
+    double s = 1.0 / 21.0;
+    s = s * f2 + 1.0 / 19.0;
+    s = s * f2 + 1.0 / 17.0;
+    s = s * f2 + 1.0 / 15.0;
+    s = s * f2 + 1.0 / 13.0;
+    s = s * f2 + 1.0 / 11.0;
+    s = s * f2 + 1.0 / 9.0;
+    s = s * f2 + 1.0 / 7.0;
+    s = s * f2 + 1.0 / 5.0;
+    s = s * f2 + 1.0 / 3.0;
+    const double tf = f + f;
+    const double lm = tf + tf * (f2 * s);
+    const double de = (double)e;
+    // ln 2 split so that de * hi is exact
+    return de * 0x1.62e42fee00000p-1 + (lm + de * 0x1.a39ef35793c76p-33);
+}
+
+__device__ __forceinline__ double dsqrt(double x) {
+    if (!(x > 0.0)) return 0.0;
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    double y = __longlong_as_double((long long)((b >> 1) + 0x1ff8000000000000ull));
+#pragma unroll
+    for (int i = 0; i < 6; ++i) y = 0.5 * (y + x / y);
+    return y;
+}
+
+// log(k!) exact to double for k < 10, Stirling series of lgamma(k + 1) above
+__device__ __forceinline__ double lfact(int32_t k) {
+    switch (k < 0 ? 0 : k) {
+        case 0:
+        case 1: return 0.0;
+        case 2: return 0x1.62e42fefa39efp-1;
+        case 3: return 0x1.cab0bfa2a2002p+0;
+        case 4: return 0x1.96ca77c922cf9p+1;
+        case 5: return 0x1.326643c4479c9p+2;
+        case 6: return 0x1.a51273acf01cap+2;
+        case 7: return 0x1.10ce1f32dcc30p+3;
+        case 8: return 0x1.5358e82fcb70dp+3;
+        case 9: return 0x1.99a8921a7f7cfp+3;
+        default: break;
+    }
+    const double x = (double)k + 1.0;
+    const double ix = 1.0 / x;
+    const double ix2 = ix * ix;
+    const double corr = ix * (1.0 / 12.0 - ix2 * (1.0 / 360.0 - ix2 * (1.0 / 1260.0)));
+    return (x - 0.5) * dlog(x) - x + 0x1.d67f1c864beb5p-1 + corr;
+}
+
+__device__ __forceinline__ double ipow(double base, int32_t n) {
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r = r * base;
+        base = base * base;
+        n >>= 1;
+    }
+    return r;
+}
+
+// Binomial by sequential inversion (Kachitvichyanukul & Schmeiser 1988, BINV);
+// q <= 1/2 and n*q < 10 so the expected search length is below 11.
+__device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double q) {
+    double r = ipow(1.0 - q, n);
+    const double s = q / (1.0 - q);
+    const double a = (double)(n + 1) * s;
+    double u = g.unif();
+    int32_t x = 0;
+    while (true) {
+        if (u <= r) return x;
+        if (x >= n) return n;
+        u = u - r;
+        x += 1;
+        r = r * (a / (double)x - s);
+    }
+}
+
+// Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
+// q <= 1/2 and n*q >= 10. Bounded at 1000 trials (acceptance is > 0.8 per trial).
+__device__ __noinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
+    const double nd = (double)n;
+    const double spq = dsqrt(nd * q * (1.0 - q));
+    const double b = 1.15 + 2.53 * spq;
+    const double a = -0.0873 + 0.0248 * b + 0.01 * q;
+    const double c = nd * q + 0.5;
+    const double alpha = (2.83 + 5.1 / b) * spq;
+    const double vr = 0.92 - 4.2 / b;
+    const double lpq = dlog(q / (1.0 - q));
+    const int32_t m = (int32_t)floor((nd + 1.0) * q);
+    const double h = lfact(m) + lfact(n - m);
+    for (int it = 0; it < 1000; ++it) {
+        const double u = g.unif() - 0.5;
+        double v = g.unif();
+        const double us = 0.5 - fabs(u);
+        const double kd = floor((2.0 * a / us + b) * u + c);
+        if (kd < 0.0 || kd > nd) continue;
+        const int32_t k = (int32_t)kd;
+        if (us >= 0.07 && v <= vr) return k;
+        v = dlog(v * alpha / (a / (us * us) + b));
+        if (v <= h - lfact(k) - lfact(n - k) + (double)(k - m) * lpq) return k;
+    }
+    return m;
+}
+
+__device__ __forceinline__ int32_t binomial(Philox& g, int32_t n, double p) {
+    if (n <= 0 || !(p > 0.0)) return 0;
+    if (p >= 1.0) return n;
+    const bool flip = p > 0.5;
+    const double q = flip ? 1.0 - p : p;
+    const int32_t k = ((double)n * q < 10.0) ? binom_inversion(g, n, q) : binom_btrs(g, n, q);
+    return flip ? n - k : k;
+}
+
+}  // namespace stg

@@ -1,0 +1,177 @@
+"""Thin object wrapper over the C ABI: one ``Engine`` owns one ``stg_sim`` handle.
+
+Used by the reference-shaped solver classes (steps_amd.solver), the batched
+ensemble (steps_amd.ensemble), the tests and bench.py. All arrays crossing the
+boundary are numpy arrays; the engine copies them at create time.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+from .geom import MeshTables
+from .model import FlatModel
+
+
+def _mesh_struct(mt: MeshTables, keep: list) -> N.StgMesh:
+    arrs = [N.as_f64(mt.vol), N.as_i32(mt.nbr), N.as_i32(mt.nbr_face), N.as_f64(mt.face_g),
+            N.as_i32(mt.tri_tet), N.as_f64(mt.tri_area)]
+    keep.extend(arrs)
+    vol, nbr, nf, g, tt, ta = arrs
+    return N.StgMesh(len(vol), N.ptr(vol, C.c_double), N.ptr(nbr, C.c_int32), N.ptr(nf, C.c_int32),
+                     N.ptr(g, C.c_double), len(tt), N.ptr(tt, C.c_int32), N.ptr(ta, C.c_double))
+
+
+def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
+    arrs = [N.as_i32(fm.reac_lhs), N.as_i32(fm.reac_upd), N.as_f64(fm.reac_k), N.as_f64(fm.spec_dcst),
+            N.as_i32(fm.sreac_lhs_s), N.as_i32(fm.sreac_upd_s), N.as_i32(fm.sreac_lhs_i),
+            N.as_i32(fm.sreac_upd_i), N.as_f64(fm.sreac_k)]
+    keep.extend(arrs)
+    rl, ru, rk, dc, sls, sus, sli, sui, sk = arrs
+    return N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
+                      N.ptr(rk, C.c_double), N.ptr(dc, C.c_double), len(fm.sspecs), len(fm.sreacs),
+                      N.ptr(sls, C.c_int32), N.ptr(sus, C.c_int32), N.ptr(sli, C.c_int32),
+                      N.ptr(sui, C.c_int32), N.ptr(sk, C.c_double))
+
+
+class Engine:
+    """One simulation handle: a single realization or a batch of them."""
+
+    def __init__(self, fm: FlatModel, meshes, seeds, streams=None, reac_k=None, sreac_k=None,
+                 spec_dcst=None, mode: int = N.MODE_TETOPSPLIT):
+        self.lib = N.load()
+        self.fm = fm
+        if isinstance(meshes, MeshTables):
+            meshes = [meshes]
+        n = len(meshes)
+        seeds = np.broadcast_to(np.asarray(seeds, dtype=np.uint64), (n,))
+        streams = np.arange(n, dtype=np.uint32) if streams is None else np.asarray(streams, dtype=np.uint32)
+        keep: list = []
+        self._keep = keep
+        model = _model_struct(fm, keep)
+        mesh_arr = (N.StgMesh * n)(*[_mesh_struct(m, keep) for m in meshes])
+        reals = (N.StgRealization * n)()
+        for r in range(n):
+            reals[r].seed = int(seeds[r])
+            reals[r].stream = int(streams[r])
+            for name, src in (("reac_k", reac_k), ("sreac_k", sreac_k), ("spec_dcst", spec_dcst)):
+                if src is not None:
+                    a = N.as_f64(src[r])
+                    keep.append(a)
+                    setattr(reals[r], name, N.ptr(a, C.c_double))
+        h = C.c_void_p()
+        rc = self.lib.stg_create_batch(C.byref(model), n, mesh_arr, reals, mode, C.byref(h))
+        N.check(rc, None)
+        self.h = h
+        self.n_real = n
+        self.n_tets = [len(m.vol) for m in meshes]
+        self.n_tris = [len(m.tri_tet) for m in meshes]
+        self._keep = []  # the engine copied everything
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.stg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        N.check(rc, self.h)
+
+    # --- state ---
+    def reset(self):
+        self._chk(self.lib.stg_reset(self.h))
+
+    def run(self, endtime: float):
+        self._chk(self.lib.stg_run(self.h, float(endtime)))
+
+    def advance(self, adv: float):
+        self._chk(self.lib.stg_advance(self.h, float(adv)))
+
+    def set_seed(self, seed: int, real: int = 0):
+        self._chk(self.lib.stg_set_seed(self.h, real, int(seed)))
+
+    def time(self) -> float:
+        t = C.c_double()
+        self._chk(self.lib.stg_get_time(self.h, C.byref(t)))
+        return t.value
+
+    def set_time(self, t: float):
+        self._chk(self.lib.stg_set_time(self.h, float(t)))
+
+    def nsteps(self, real: int = 0) -> int:
+        n = C.c_uint64()
+        self._chk(self.lib.stg_get_nsteps(self.h, real, C.byref(n)))
+        return n.value
+
+    def update_period(self, real: int = 0) -> float:
+        d = C.c_double()
+        self._chk(self.lib.stg_get_update_period(self.h, real, C.byref(d)))
+        return d.value
+
+    def extents(self, real: int = 0):
+        a, b = C.c_uint64(), C.c_uint64()
+        self._chk(self.lib.stg_get_extents(self.h, real, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def set_tet_counts(self, spec: int, counts, real: int = 0):
+        a = N.as_i32(counts)
+        if a.shape != (self.n_tets[real],):
+            raise ValueError(f"expected {self.n_tets[real]} counts")
+        self._chk(self.lib.stg_set_tet_counts(self.h, real, spec, N.ptr(a, C.c_int32)))
+
+    def get_tet_counts(self, spec: int, real: int = 0) -> np.ndarray:
+        a = np.empty(self.n_tets[real], dtype=np.int32)
+        self._chk(self.lib.stg_get_tet_counts(self.h, real, spec, N.ptr(a, C.c_int32)))
+        return a
+
+    def set_tri_counts(self, sspec: int, counts, real: int = 0):
+        a = N.as_i32(counts)
+        if a.shape != (self.n_tris[real],):
+            raise ValueError(f"expected {self.n_tris[real]} counts")
+        self._chk(self.lib.stg_set_tri_counts(self.h, real, sspec, N.ptr(a, C.c_int32)))
+
+    def get_tri_counts(self, sspec: int, real: int = 0) -> np.ndarray:
+        a = np.empty(self.n_tris[real], dtype=np.int32)
+        self._chk(self.lib.stg_get_tri_counts(self.h, real, sspec, N.ptr(a, C.c_int32)))
+        return a
+
+    def batch_tet_counts(self, spec: int, tets, real: int = 0) -> np.ndarray:
+        idx = np.ascontiguousarray(np.asarray(tets, dtype=np.int64))
+        out = np.empty(len(idx), dtype=np.float64)
+        self._chk(self.lib.stg_get_batch_tet_counts(self.h, real, spec, N.ptr(idx, C.c_int64), len(idx),
+                                                    N.ptr(out, C.c_double)))
+        return out
+
+    def sum_tet_counts(self, spec: int, real: int = 0) -> int:
+        t = C.c_int64()
+        self._chk(self.lib.stg_sum_tet_counts(self.h, real, spec, C.byref(t)))
+        return t.value
+
+    def sum_tri_counts(self, sspec: int, real: int = 0) -> int:
+        t = C.c_int64()
+        self._chk(self.lib.stg_sum_tri_counts(self.h, real, sspec, C.byref(t)))
+        return t.value
+
+    def set_reac_k(self, reac: int, k: float, real: int = 0):
+        self._chk(self.lib.stg_set_reac_k(self.h, real, reac, float(k)))
+
+    def set_sreac_k(self, sreac: int, k: float, real: int = 0):
+        self._chk(self.lib.stg_set_sreac_k(self.h, real, sreac, float(k)))
+
+    def set_dcst(self, spec: int, d: float, real: int = 0):
+        self._chk(self.lib.stg_set_dcst(self.h, real, spec, float(d)))
+
+    def set_launch(self, block: int = 256, use_graph: bool = False):
+        self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
+
+    def last_run_kernel_ms(self):
+        ms, n = C.c_double(), C.c_int32()
+        self._chk(self.lib.stg_last_run_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,256 @@
+"""Tetrahedral mesh, compartments and patches, and the flat tables the engine reads.
+
+Mirrors the parts of the reference's geometry layer the operator-split solver
+consumes:
+  - ``Tetmesh(verts, tets)`` with tet/triangle neighbour relations, volumes,
+    areas and barycentres (src/steps/geom/tetmesh.hpp, used by
+    TetOpSplitP::_setup at src/steps/mpi/tetopsplit/tetopsplit.cpp:645-690);
+  - ``TmComp`` / ``TmPatch`` (src/steps/geom/tmcomp.hpp, tmpatch.hpp);
+  - the per-face diffusion factor A_f / (V d_f) with d_f the barycentre
+    distance (src/steps/mpi/tetopsplit/diff.cpp:74-79,
+    src/steps/mpi/dist/tetopsplit/simulation.cpp:1306-1331).
+
+Face f of a tetrahedron is the face opposite its f-th vertex.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def _face_keys(faces: np.ndarray, nverts: int) -> np.ndarray:
+    f = np.sort(faces, axis=1).astype(np.int64)
+    if nverts < (1 << 21):
+        return (f[:, 0] * nverts + f[:, 1]) * nverts + f[:, 2]
+    # wide meshes: rank the sorted triples
+    order = np.lexsort((f[:, 2], f[:, 1], f[:, 0]))
+    fs = f[order]
+    new = np.ones(len(fs), dtype=bool)
+    new[1:] = np.any(fs[1:] != fs[:-1], axis=1)
+    rank = np.cumsum(new) - 1
+    keys = np.empty(len(f), dtype=np.int64)
+    keys[order] = rank
+    return keys
+
+
+class Tetmesh:
+    """Unstructured tetrahedral mesh with its face topology and measures (SI units)."""
+
+    def __init__(self, verts, tets):
+        self.verts = np.ascontiguousarray(np.asarray(verts, dtype=np.float64))
+        self.tets = np.ascontiguousarray(np.asarray(tets, dtype=np.int64))
+        if self.verts.ndim != 2 or self.verts.shape[1] != 3:
+            raise ValueError("verts must be (n, 3)")
+        if self.tets.ndim != 2 or self.tets.shape[1] != 4:
+            raise ValueError("tets must be (n, 4)")
+        nt = len(self.tets)
+        nv = len(self.verts)
+        if nt == 0:
+            raise ValueError("mesh has no tetrahedra")
+        if self.tets.min() < 0 or self.tets.max() >= nv:
+            raise ValueError("tet vertex index out of range")
+        p = self.verts[self.tets]  # (nt, 4, 3)
+        e1, e2, e3 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]
+        self.tet_vol = np.abs(np.einsum("ij,ij->i", e1, np.cross(e2, e3))) / 6.0
+        if np.any(self.tet_vol <= 0):
+            raise ValueError("degenerate tetrahedron")
+        self.tet_bary = p.mean(axis=1)
+        # faces opposite vertex f
+        opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
+        faces = self.tets[:, opp].reshape(-1, 3)  # (nt*4, 3)
+        keys = _face_keys(faces, nv)
+        uniq, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+        inv = inv.reshape(-1)
+        self.tris = np.sort(faces[first], axis=1)  # unique triangles
+        ntri = len(self.tris)
+        self.tet_tri = inv.reshape(nt, 4)
+        # the (tet, face) occurrences of each triangle
+        occ = np.argsort(inv, kind="stable")
+        counts = np.bincount(inv, minlength=ntri)
+        if counts.max() > 2:
+            raise ValueError("non-manifold mesh: a triangle is shared by more than two tets")
+        starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+        a = occ[starts]
+        b = np.where(counts == 2, occ[np.minimum(starts + 1, len(occ) - 1)], -1)
+        self.tri_tets = np.stack([a // 4, np.where(b >= 0, b // 4, -1)], axis=1)
+        nbr = -np.ones(nt * 4, dtype=np.int64)
+        nbr_face = -np.ones(nt * 4, dtype=np.int64)
+        two = b >= 0
+        nbr[a[two]] = b[two] // 4
+        nbr_face[a[two]] = b[two] % 4
+        nbr[b[two]] = a[two] // 4
+        nbr_face[b[two]] = a[two] % 4
+        self.tet_nbr = nbr.reshape(nt, 4)
+        self.tet_nbr_face = nbr_face.reshape(nt, 4)
+        tp = self.verts[self.tris]
+        self.tri_area = 0.5 * np.linalg.norm(np.cross(tp[:, 1] - tp[:, 0], tp[:, 2] - tp[:, 0]), axis=1)
+        self.tri_bary = tp.mean(axis=1)
+        self.tet_face_area = self.tri_area[self.tet_tri]
+        d = np.zeros((nt, 4))
+        has = self.tet_nbr >= 0
+        ti = np.nonzero(has)
+        d[ti] = np.linalg.norm(self.tet_bary[ti[0]] - self.tet_bary[self.tet_nbr[ti]], axis=1)
+        self.tet_face_dist = d
+        self.comps: dict[str, TmComp] = {}
+        self.patches: dict[str, TmPatch] = {}
+
+    # --- reference-style accessors (src/steps/geom/tetmesh.hpp) ---
+    def countTets(self) -> int:
+        return len(self.tets)
+
+    def countTris(self) -> int:
+        return len(self.tris)
+
+    def countVertices(self) -> int:
+        return len(self.verts)
+
+    def getTetVol(self, t: int) -> float:
+        return float(self.tet_vol[t])
+
+    def getTriArea(self, t: int) -> float:
+        return float(self.tri_area[t])
+
+    def getTetTetNeighb(self, t: int):
+        return [int(x) if x >= 0 else -1 for x in self.tet_nbr[t]]
+
+    def getTetTriNeighb(self, t: int):
+        return [int(x) for x in self.tet_tri[t]]
+
+    def getTriTetNeighb(self, tri: int):
+        return [int(x) for x in self.tri_tets[tri]]
+
+    def getSurfTris(self) -> np.ndarray:
+        return np.nonzero(self.tri_tets[:, 1] < 0)[0]
+
+    def getTetBarycenter(self, t: int):
+        return self.tet_bary[t].tolist()
+
+    def getMeshVolume(self) -> float:
+        return float(self.tet_vol.sum())
+
+    def findTetByPoint(self, pt) -> int:
+        """Index of a tet containing pt (barycentric test), or -1."""
+        pt = np.asarray(pt, dtype=np.float64)
+        p = self.verts[self.tets]
+        T = np.stack([p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]], axis=2)
+        lam = np.linalg.solve(T, (pt - p[:, 0])[:, :, None])[:, :, 0]
+        inside = np.all(lam >= -1e-12, axis=1) & (lam.sum(axis=1) <= 1 + 1e-12)
+        idx = np.nonzero(inside)[0]
+        return int(idx[0]) if len(idx) else -1
+
+
+class TmComp:
+    """Compartment: a set of tets with volume systems (src/steps/geom/tmcomp.hpp)."""
+
+    def __init__(self, name: str, mesh: Tetmesh, tets=None):
+        self.name = name
+        self.mesh = mesh
+        self.tets = (np.arange(mesh.countTets()) if tets is None
+                     else np.unique(np.asarray(tets, dtype=np.int64)))
+        self.volsys: list[str] = []
+        mesh.comps[name] = self
+
+    def addVolsys(self, name: str) -> None:
+        self.volsys.append(name)
+
+    def getVol(self) -> float:
+        return float(self.mesh.tet_vol[self.tets].sum())
+
+
+class TmPatch:
+    """Membrane patch of triangles on the surface of ``icomp`` (src/steps/geom/tmpatch.hpp)."""
+
+    def __init__(self, name: str, mesh: Tetmesh, tris, icomp: TmComp, ocomp: TmComp | None = None):
+        self.name = name
+        self.mesh = mesh
+        self.tris = np.asarray(tris, dtype=np.int64)
+        self.icomp = icomp
+        self.ocomp = ocomp
+        self.surfsys: list[str] = []
+        mesh.patches[name] = self
+
+    def addSurfsys(self, name: str) -> None:
+        self.surfsys.append(name)
+
+    def getArea(self) -> float:
+        return float(self.mesh.tri_area[self.tris].sum())
+
+
+def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0)) -> Tetmesh:
+    """Cube of edge ``side`` (m) cut into n^3 sub-cubes, each split into the six
+    Kuhn (Freudenthal) tetrahedra sharing the main diagonal. The split is
+    conforming across sub-cubes, every tet has volume (side/n)^3/6, and the six
+    tets of a sub-cube are consecutive (cube-major, x fastest)."""
+    if n < 1:
+        raise ValueError("n must be >= 1")
+    h = side / n
+    g = np.arange(n + 1)
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    verts = np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1) * h + np.asarray(origin)
+
+    def vid(i, j, k):
+        return (k * (n + 1) + j) * (n + 1) + i
+
+    c = np.arange(n)
+    ck, cj, ci = np.meshgrid(c, c, c, indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    tets = np.empty((len(ci), 6, 4), dtype=np.int64)
+    for pi, perm in enumerate(perms):
+        cur = [ci.copy(), cj.copy(), ck.copy()]
+        tets[:, pi, 0] = vid(*cur)
+        for s, ax in enumerate(perm):
+            cur[ax] = cur[ax] + 1
+            tets[:, pi, s + 1] = vid(*cur)
+    return Tetmesh(verts, tets.reshape(-1, 4))
+
+
+@dataclass
+class MeshTables:
+    """The engine's per-realization mesh arrays (stg_mesh in include/stg.h)."""
+
+    vol: np.ndarray        # (n,) float64
+    nbr: np.ndarray        # (n,4) int32
+    nbr_face: np.ndarray   # (n,4) int32
+    face_g: np.ndarray     # (n,4) float64
+    tri_tet: np.ndarray    # (m,) int32
+    tri_area: np.ndarray   # (m,) float64
+
+
+def mesh_tables(mesh: Tetmesh, comp: TmComp, patch_tris=None) -> MeshTables:
+    """Tables for the tets of ``comp`` (renumbered 0..n-1 in mesh order).
+    Faces to tets outside the compartment are closed (no diffusion), as in
+    Diff::Diff (src/steps/mpi/tetopsplit/diff.cpp:66-79)."""
+    tets = comp.tets
+    n = len(tets)
+    local = -np.ones(mesh.countTets(), dtype=np.int64)
+    local[tets] = np.arange(n)
+    gnbr = mesh.tet_nbr[tets]
+    lnbr = np.where(gnbr >= 0, local[np.maximum(gnbr, 0)], -1)
+    nface = np.where(lnbr >= 0, mesh.tet_nbr_face[tets], -1)
+    area = mesh.tet_face_area[tets]
+    dist = mesh.tet_face_dist[tets]
+    vol = mesh.tet_vol[tets]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        g = np.where(lnbr >= 0, area / (vol[:, None] * dist), 0.0)
+    g = np.where(np.isfinite(g), g, 0.0)
+    if patch_tris is None:
+        tri_tet = np.zeros(0, dtype=np.int32)
+        tri_area = np.zeros(0)
+    else:
+        tris = np.asarray(patch_tris, dtype=np.int64)
+        tt = mesh.tri_tets[tris]
+        inner = np.where(local[np.maximum(tt[:, 0], 0)] >= 0, tt[:, 0], tt[:, 1])
+        if np.any(inner < 0) or np.any(local[inner] < 0):
+            raise ValueError("patch triangle without an inner tet in the compartment")
+        tri_tet = local[inner].astype(np.int32)
+        tri_area = mesh.tri_area[tris]
+    return MeshTables(
+        vol=np.ascontiguousarray(vol, dtype=np.float64),
+        nbr=np.ascontiguousarray(lnbr, dtype=np.int32),
+        nbr_face=np.ascontiguousarray(nface, dtype=np.int32),
+        face_g=np.ascontiguousarray(g, dtype=np.float64),
+        tri_tet=np.ascontiguousarray(tri_tet, dtype=np.int32),
+        tri_area=np.ascontiguousarray(tri_area, dtype=np.float64),
+    )

@@ -1,0 +1,171 @@
+"""Synthetic reaction-diffusion workloads on Kuhn-split cubes.
+
+Each builder returns a ``Workload``: the flattened model, the mesh tables of
+one realization and its initial counts. The same workloads feed the GPU
+engine, the CPU oracle, the tests and bench.py.
+
+  ab_binding_cube          A + B <-> C on an 8^3 x 6 cube of side 10 um
+  four_species_cube        4 species, 6 reactions on a 40^3 x 6 cube of side 40 um
+  calcium_membrane_cube    Ca buffering with membrane pumps and influx, 100^3 x 6 cube, 20 um
+  ab_binding_ensemble      1024 realizations of A + B <-> C on cubes n = 6..20
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import geom, model as M
+
+AVOGADRO = 6.02214179e23
+
+
+@dataclass
+class Workload:
+    name: str
+    fm: M.FlatModel
+    mt: geom.MeshTables
+    counts: np.ndarray                 # (S, n_tets) int32
+    tri_counts: np.ndarray             # (SS, n_tris) int32
+    end_time: float
+    params: dict = field(default_factory=dict)
+    mesh: geom.Tetmesh | None = None
+
+    @property
+    def n_tets(self) -> int:
+        return len(self.mt.vol)
+
+
+def _cube(n, side, with_patch):
+    mesh = geom.kuhn_cube(n, side)
+    comp = geom.TmComp("cyto", mesh)
+    comp.addVolsys("vsys")
+    tris = mesh.getSurfTris() if with_patch else None
+    if with_patch:
+        patch = geom.TmPatch("memb", mesh, tris, comp)
+        patch.addSurfsys("ssys")
+    return mesh, comp, tris
+
+
+def place_uniform(n_mol: int, vol: np.ndarray, rng: np.random.Generator) -> np.ndarray:
+    """Molecules placed independently, each tet chosen with probability V_t / V."""
+    return rng.multinomial(n_mol, vol / vol.sum()).astype(np.int32)
+
+
+def ab_binding_model(kf=1e6, kb=1.0, dA=1e-12, dB=1e-12, dC=5e-13):
+    mdl = M.Model()
+    A, B, C = M.Spec("A", mdl), M.Spec("B", mdl), M.Spec("C", mdl)
+    vs = M.Volsys("vsys", mdl)
+    M.Reac("fwd", vs, lhs=[A, B], rhs=[C], kcst=kf)
+    M.Reac("bwd", vs, lhs=[C], rhs=[A, B], kcst=kb)
+    M.Diff("dA", vs, A, dA)
+    M.Diff("dB", vs, B, dB)
+    M.Diff("dC", vs, C, dC)
+    return mdl
+
+
+def ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=0, end_time=0.1,
+                    kf=1e6, kb=1.0) -> Workload:
+    mdl = ab_binding_model(kf, kb)
+    mesh, comp, _ = _cube(n, side, False)
+    fm = M.flatten(mdl, ["vsys"])
+    mt = geom.mesh_tables(mesh, comp)
+    rng = np.random.default_rng(seed)
+    counts = np.zeros((3, len(mt.vol)), dtype=np.int32)
+    counts[0] = place_uniform(n_a, mt.vol, rng)
+    counts[1] = place_uniform(n_b, mt.vol, rng)
+    return Workload("ab_binding_cube", fm, mt, counts, np.zeros((0, 0), np.int32), end_time,
+                    dict(n=n, side=side, n_a=n_a, n_b=n_b, seed=seed, kf=kf, kb=kb), mesh)
+
+
+def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> Workload:
+    """A + B <-> C, C + B <-> D (kf 1e6 /M/s, kb 1 /s), 0 -> A at 10 /s per tet,
+    D -> 0 at 0.5 /s; D log-uniform in [1e-13, 1e-11] m^2/s per species;
+    initial counts Poisson(mean_count) per (tet, species)."""
+    rng = np.random.default_rng(seed)
+    dcst = 10.0 ** rng.uniform(-13, -11, size=4)
+    dcst[int(np.argmax(dcst))] = 1e-11  # the fastest species sets dt at the stability bound
+    mesh, comp, _ = _cube(n, side, False)
+    vol_t = float(mesh.tet_vol[0])
+    mdl = M.Model()
+    A, B, Cs, D = (M.Spec(x, mdl) for x in "ABCD")
+    vs = M.Volsys("vsys", mdl)
+    M.Reac("bind1_f", vs, lhs=[A, B], rhs=[Cs], kcst=1e6)
+    M.Reac("bind1_b", vs, lhs=[Cs], rhs=[A, B], kcst=1.0)
+    M.Reac("bind2_f", vs, lhs=[Cs, B], rhs=[D], kcst=1e6)
+    M.Reac("bind2_b", vs, lhs=[D], rhs=[Cs, B], kcst=1.0)
+    M.Reac("prod", vs, lhs=[], rhs=[A], kcst=10.0 / (1e3 * vol_t * AVOGADRO))
+    M.Reac("deg", vs, lhs=[D], rhs=[], kcst=0.5)
+    for sp, d in zip((A, B, Cs, D), dcst):
+        M.Diff("d" + sp.id, vs, sp, float(d))
+    fm = M.flatten(mdl, ["vsys"])
+    mt = geom.mesh_tables(mesh, comp)
+    counts = rng.poisson(mean_count, size=(4, len(mt.vol))).astype(np.int32)
+    return Workload("four_species_cube", fm, mt, counts, np.zeros((0, 0), np.int32), end_time,
+                    dict(n=n, side=side, seed=seed, dcst=dcst.tolist()), mesh)
+
+
+def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0,
+                          ca_um=0.1, pump_density=10.0, influx=1e5) -> Workload:
+    """Ca + Buf <-> CaB (kf 1e8 /M/s, kb 10 /s), D_Ca 2e-10 m^2/s (Buf, CaB
+    immobile); every boundary triangle is membrane. Pumps P (pump_density per
+    um^2) run P + Ca_in -> P + Ca_out with k = kcat/Km = 50 /s / 1 uM; Ca_out
+    is a surface species counting extruded ions. The x = 0 face carries one
+    influx channel per triangle, Chan -> Chan + Ca_in, with k set so the face
+    admits ``influx`` ions/s in total."""
+    rng = np.random.default_rng(seed)
+    mesh, comp, tris = _cube(n, side, True)
+    mdl = M.Model()
+    Ca, Buf, CaB = M.Spec("Ca", mdl), M.Spec("Buf", mdl), M.Spec("CaB", mdl)
+    P, CaOut, Chan = M.Spec("Pump", mdl), M.Spec("CaOut", mdl), M.Spec("Chan", mdl)
+    vs = M.Volsys("vsys", mdl)
+    M.Reac("bind", vs, lhs=[Ca, Buf], rhs=[CaB], kcst=1e8)
+    M.Reac("unbind", vs, lhs=[CaB], rhs=[Ca, Buf], kcst=10.0)
+    M.Diff("dCa", vs, Ca, 2e-10)
+    ss = M.Surfsys("ssys", mdl)
+    M.SReac("pump", ss, ilhs=[Ca], slhs=[P], srhs=[P, CaOut], kcst=50.0 / 1e-6)
+    face_tris = tris[np.all(np.abs(mesh.verts[mesh.tris[tris]][:, :, 0]) < 1e-3 * side / n, axis=1)]
+    n_face = len(face_tris)
+    M.SReac("influx", ss, slhs=[Chan], srhs=[Chan], irhs=[Ca], kcst=influx / max(n_face, 1))
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    mt = geom.mesh_tables(mesh, comp, tris)
+    vol_l = mt.vol * 1e3
+    counts = np.zeros((len(fm.specs), len(mt.vol)), dtype=np.int32)
+    counts[fm.spec_index("Buf")] = rng.poisson(buf_um * 1e-6 * AVOGADRO * vol_l)
+    counts[fm.spec_index("Ca")] = rng.poisson(ca_um * 1e-6 * AVOGADRO * vol_l)
+    tri_counts = np.zeros((len(fm.sspecs), len(tris)), dtype=np.int32)
+    area_um2 = mt.tri_area * 1e12
+    tri_counts[fm.sspec_index("Pump")] = place_uniform(int(round(pump_density * area_um2.sum())),
+                                                       mt.tri_area, rng)
+    is_face = np.isin(tris, face_tris)
+    tri_counts[fm.sspec_index("Chan")] = is_face.astype(np.int32)
+    return Workload("calcium_membrane_cube", fm, mt, counts, tri_counts, end_time,
+                    dict(n=n, side=side, seed=seed, n_influx_tris=int(n_face)), mesh)
+
+
+def ab_binding_ensemble(n_real=1024, n_min=6, n_max=20, side=10e-6, seed=0, end_time=0.1,
+                        n_a=10_000, n_b=10_000, kf=1e6, kb=1.0):
+    """Independent A + B <-> C realizations: cube resolution n ~ U{n_min..n_max},
+    kf and kb log-uniform over +-1 decade, molecules placed by the realization's
+    own seed. Returns (flat model, [mesh tables], [counts], reac_k (R, 2))."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, size=n_real)
+    kfs = kf * 10.0 ** rng.uniform(-1, 1, size=n_real)
+    kbs = kb * 10.0 ** rng.uniform(-1, 1, size=n_real)
+    fm = M.flatten(ab_binding_model(kf, kb), ["vsys"])
+    cache = {}
+    meshes, counts = [], []
+    for r in range(n_real):
+        n = int(sizes[r])
+        if n not in cache:
+            mesh = geom.kuhn_cube(n, side)
+            cache[n] = geom.mesh_tables(mesh, geom.TmComp("cyto", mesh))
+        mt = cache[n]
+        meshes.append(mt)
+        c = np.zeros((3, len(mt.vol)), dtype=np.int32)
+        rr = np.random.default_rng([seed, r])
+        c[0] = place_uniform(n_a, mt.vol, rr)
+        c[1] = place_uniform(n_b, mt.vol, rr)
+        counts.append(c)
+    reac_k = np.stack([kfs, kbs], axis=1)
+    return fm, meshes, counts, reac_k, dict(sizes=sizes.tolist(), seed=seed, end_time=end_time)

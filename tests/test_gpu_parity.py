@@ -1,0 +1,143 @@
+"""GPU engine vs CPU oracle: integer counts must agree bit-for-bit.
+
+Both sides draw from the same keyed Philox4x32-8 streams with the same
+floating-point operation sequence, so any difference in a molecule count is a
+bug. Runs through the C ABI (steps_amd.engine -> include/stg.h).
+"""
+import numpy as np
+import pytest
+
+from oracle import bind
+from steps_amd import workloads as W
+from steps_amd.engine import Engine
+from steps_amd import _native as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(w, seed, mode=0, **kw):
+    eng = Engine(w.fm, [w.mt], [seed], mode=mode, **kw)
+    for s in range(len(w.fm.specs)):
+        eng.set_tet_counts(s, w.counts[s])
+    for s in range(len(w.fm.sspecs)):
+        eng.set_tri_counts(s, w.tri_counts[s])
+    return eng
+
+
+def _oracle(w, seed, **kw):
+    o = bind.OracleSim(w.fm, w.mt, seed=seed, **kw)
+    o.counts[:] = w.counts
+    if len(w.fm.sspecs):
+        o.tri_counts[:] = w.tri_counts
+    return o
+
+
+def _gpu_counts(eng, w, real=0):
+    return np.stack([eng.get_tet_counts(s, real) for s in range(len(w.fm.specs))])
+
+
+def _gpu_tri_counts(eng, w, real=0):
+    return np.stack([eng.get_tri_counts(s, real) for s in range(len(w.fm.sspecs))])
+
+
+@pytest.mark.parametrize("seed", [1, 2, 7])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ab_binding_bit_exact(seed, mode):
+    w = W.ab_binding_cube(n=4, side=5e-6, n_a=3000, n_b=2000, seed=seed, end_time=0.2)
+    print(f"case ab_binding seed={seed} mode={mode}", flush=True)
+    eng = _engine(w, seed, mode)
+    o = _oracle(w, seed)
+    for t in (0.013, 0.05, 0.2):  # several run() calls, as result saving does
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+    assert eng.nsteps() == o.nsteps
+    ev, mv = eng.extents()
+    assert ev == o.events.value and mv == o.moved.value
+    tot = o.counts.sum(axis=1)
+    assert tot[0] + tot[2] == 3000 and tot[1] + tot[2] == 2000
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_four_species_bit_exact(mode):
+    w = W.four_species_cube(n=6, side=6e-6, seed=5, end_time=0.05)
+    print(f"case four_species n=6 mode={mode}", flush=True)
+    eng = _engine(w, 99, mode)
+    o = _oracle(w, 99)
+    eng.run(w.end_time)
+    o.run(w.end_time, mode)
+    assert np.array_equal(_gpu_counts(eng, w), o.counts)
+    assert eng.extents() == (o.events.value, o.moved.value)
+
+
+def test_calcium_membrane_bit_exact():
+    w = W.calcium_membrane_cube(n=6, side=2e-6, seed=4, end_time=2e-4, buf_um=100.0, ca_um=50.0)
+    print("case calcium_membrane n=6", flush=True)
+    eng = _engine(w, 123)
+    o = _oracle(w, 123)
+    eng.run(w.end_time)
+    o.run(w.end_time, 0)
+    assert np.array_equal(_gpu_counts(eng, w), o.counts)
+    assert np.array_equal(_gpu_tri_counts(eng, w), o.tri_counts[: len(w.fm.sspecs)])
+    # calcium is conserved: free + bound + extruded - admitted through the channels
+    assert o.events.value > 0
+
+
+def test_batch_matches_independent_runs():
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=5, n_min=3, n_max=6,
+                                                             side=4e-6, n_a=800, n_b=600, seed=2)
+    print("case batch of 5", flush=True)
+    eng = Engine(fm, meshes, [17] * 5, reac_k=reac_k)
+    for r in range(5):
+        for s in range(3):
+            eng.set_tet_counts(s, counts[r][s], real=r)
+    eng.run(0.08)
+    for r in range(5):
+        o = bind.OracleSim(fm, meshes[r], seed=17, stream=r, reac_k=reac_k[r])
+        o.counts[:] = counts[r]
+        o.run(0.08, 0)
+        got = np.stack([eng.get_tet_counts(s, r) for s in range(3)])
+        assert np.array_equal(got, o.counts), f"realization {r}"
+        assert eng.nsteps(r) == o.nsteps
+
+
+@pytest.mark.parametrize("block,graph", [(64, False), (1024, False), (256, True)])
+def test_launch_shape_invariance(block, graph):
+    w = W.four_species_cube(n=5, side=5e-6, seed=8, end_time=0.03)
+    print(f"case launch block={block} graph={graph}", flush=True)
+    ref = _engine(w, 3)
+    ref.run(w.end_time / 2)
+    ref.run(w.end_time)
+    eng = _engine(w, 3)
+    eng.set_launch(block, graph)
+    eng.run(w.end_time / 2)
+    eng.run(w.end_time)
+    assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
+
+
+def test_conservation_full_size_diffusion_only():
+    """Exact conservation at the 384,000-tet size with reactions switched off."""
+    w = W.four_species_cube(n=40, side=40e-6, seed=0)
+    print("case conservation n=40", flush=True)
+    eng = _engine(w, 5)
+    for j in range(len(w.fm.reacs)):
+        eng.set_reac_k(j, 0.0)
+    before = w.counts.sum(axis=1)
+    eng.run(20 * eng.update_period())
+    after = np.array([eng.sum_tet_counts(s) for s in range(4)])
+    assert np.array_equal(before, after)
+    assert all(np.all(eng.get_tet_counts(s) >= 0) for s in range(4))
+    idx = np.array([0, 5, 383999, 1234])
+    assert np.array_equal(eng.batch_tet_counts(2, idx), eng.get_tet_counts(2)[idx].astype(float))
+
+
+def test_errors_are_reported():
+    w = W.ab_binding_cube(n=2, side=2e-6, n_a=10, n_b=10)
+    eng = _engine(w, 1)
+    eng.run(0.01)
+    with pytest.raises(N.ArgErr):
+        eng.run(0.005)  # end time before current time
+    with pytest.raises(N.ArgErr):
+        eng.set_tet_counts(0, -np.ones(w.n_tets))
+    with pytest.raises(N.ArgErr):
+        eng.set_launch(100, False)
