@@ -158,6 +158,13 @@ int stg_set_dcst(stg_sim *sim, int32_t real, int32_t spec, double dcst);
  * No reference counterpart (device tuning). */
 int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
 
+/* TetOpSplitP::setDiffApplyThreshold (tetopsplit.hpp:995,
+ * pysteps/cysteps_mpi.pyx:1288; DiffusionOperator default 10,
+ * diffusion_operator.hpp:99): leaving counts up to the threshold are split
+ * over faces one molecule at a time, larger ones by conditional binomials.
+ * Both give the same multinomial distribution. */
+int stg_set_diff_apply_threshold(stg_sim *sim, int32_t threshold);
+
 /* Number of realizations and tets / triangles of one realization. */
 int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_tets,
                   int32_t *n_tris);

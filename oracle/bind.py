@@ -28,7 +28,7 @@ class StgoSystem(C.Structure):
         ("n_sreacs", C.c_int32), ("reac_lhs", _ip), ("reac_upd", _ip), ("reac_k", _dp),
         ("spec_dcst", _dp), ("sreac_lhs_s", _ip), ("sreac_upd_s", _ip), ("sreac_lhs_i", _ip),
         ("sreac_upd_i", _ip), ("sreac_k", _dp), ("counts", _ip), ("tri_counts", _ip),
-        ("seed", C.c_uint64), ("stream", C.c_uint32),
+        ("seed", C.c_uint64), ("stream", C.c_uint32), ("diff_threshold", C.c_int32),
     ]
 
 
@@ -115,7 +115,7 @@ class OracleSim:
     """One realization on the CPU oracle, built from the same tables as the engine."""
 
     def __init__(self, fm, mt, seed: int, stream: int = 0, reac_k=None, sreac_k=None,
-                 spec_dcst=None):
+                 spec_dcst=None, diff_threshold: int = 10):
         self.lib = load()
         f64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))  # noqa: E731
         i32 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int32))  # noqa: E731
@@ -140,7 +140,7 @@ class OracleSim:
             P(a["rk"], C.c_double), P(a["dc"], C.c_double), P(a["sls"], C.c_int32),
             P(a["sus"], C.c_int32), P(a["sli"], C.c_int32), P(a["sui"], C.c_int32),
             P(a["sk"], C.c_double), P(a["counts"], C.c_int32), P(a["tcounts"], C.c_int32),
-            seed, stream)
+            seed, stream, diff_threshold)
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         self.time = 0.0

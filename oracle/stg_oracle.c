@@ -47,7 +47,6 @@
 #include <string.h>
 
 #define AVOGADRO 6.02214179e23 /* src/steps/math/constants.hpp:45 */
-#define DIFF_THRESHOLD 10      /* diffusion_operator.hpp:99 */
 #define MAX_EVENTS_PER_PERIOD (1 << 22)
 
 /* ------------------------------------------------------------------ Philox */
@@ -158,32 +157,20 @@ double stgo_log(double x) {
     return de * 0x1.62e42fee00000p-1 + (lm + de * 0x1.a39ef35793c76p-33);
 }
 
-/* Square root of a positive finite normal double by Newton's iteration from
- * an exponent-halving bit estimate (relative error < 4%), six steps. */
+/* IEEE-754 square root (correctly rounded, so every conforming machine
+ * returns the same double). */
 double stgo_sqrt(double x) {
-    if (!(x > 0.0)) return 0.0;
-    uint64_t b = d_to_bits(x);
-    double y = d_from_bits((b >> 1) + 0x1ff8000000000000ull);
-    for (int i = 0; i < 6; ++i) y = 0.5 * (y + x / y);
-    return y;
+    return x > 0.0 ? sqrt(x) : 0.0;
 }
 
-static const double LFACT_TABLE[10] = {
-    0.0,
-    0.0,
-    0x1.62e42fefa39efp-1,
-    0x1.cab0bfa2a2002p+0,
-    0x1.96ca77c922cf9p+1,
-    0x1.326643c4479c9p+2,
-    0x1.a51273acf01cap+2,
-    0x1.10ce1f32dcc30p+3,
-    0x1.5358e82fcb70dp+3,
-    0x1.99a8921a7f7cfp+3,
+static const double LFACT_TABLE[1024] = {
+#include "lfact_table.inc"
 };
 
-/* log(k!) : exact table below 10, Stirling series of lgamma(k+1) above. */
+/* log(k!) : correctly rounded table below 1024, Stirling series of
+ * lgamma(k+1) above. */
 double stgo_lfact(int32_t k) {
-    if (k < 10) return LFACT_TABLE[k < 0 ? 0 : k];
+    if (k < 1024) return LFACT_TABLE[k < 0 ? 0 : k];
     double x = (double)k + 1.0;
     double ix = 1.0 / x;
     double ix2 = ix * ix;
@@ -201,19 +188,22 @@ static double ipow(double base, int32_t n) {
     return r;
 }
 
-/* BINV: sequential inversion from k = 0, q <= 1/2, n*q < 10. */
+/* BINV: sequential inversion from k = 0, q <= 1/2, n*q < 10, written in
+ * factorial-scaled form so the search loop has no division: with
+ * U_x = u_x x! and R_x = f(x) x!, U_{x+1} = (U_x - R_x)(x+1) and
+ * R_{x+1} = R_x (n-x) q/(1-q). */
 static int32_t binom_inv(ostream *st, int32_t n, double q) {
     double r = ipow(1.0 - q, n);
     double s = q / (1.0 - q);
-    double a = (double)(n + 1) * s;
     double u = os_unif(st);
     int32_t x = 0;
     for (;;) {
         if (u <= r) return x;
         if (x >= n) return n;
-        u = u - r;
+        if (x >= 150) return x;
         x += 1;
-        r = r * (a / (double)x - s);
+        u = (u - r) * (double)x;
+        r = r * ((double)(n - x + 1) * s);
     }
 }
 
@@ -222,10 +212,11 @@ static int32_t binom_btrs(ostream *st, int32_t n, double q) {
     double nd = (double)n;
     double spq = stgo_sqrt(nd * q * (1.0 - q));
     double b = 1.15 + 2.53 * spq;
+    double ib = 1.0 / b;
     double a = -0.0873 + 0.0248 * b + 0.01 * q;
     double c = nd * q + 0.5;
-    double alpha = (2.83 + 5.1 / b) * spq;
-    double vr = 0.92 - 4.2 / b;
+    double alpha = (2.83 + 5.1 * ib) * spq;
+    double vr = 0.92 - 4.2 * ib;
     double lpq = stgo_log(q / (1.0 - q));
     int32_t m = (int32_t)floor((nd + 1.0) * q);
     double h = stgo_lfact(m) + stgo_lfact(n - m);
@@ -465,6 +456,9 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
         const double a0 = ps[nk - 1];
         if (a0 < DBL_EPSILON) break;
         const double u1 = os_unif(&st);
+        /* -log(u) >= 1 - u: a waiting time certainly past the period end
+         * needs no logarithm */
+        if (1.0 - u1 > a0 * (dt - tcur) * 1.0000001) break;
         const double u2 = os_unif(&st);
         const double tev = tcur + (-stgo_log(u1) / a0);
         if (tev > dt) break;
@@ -539,7 +533,7 @@ static int32_t leave_tet_species(const stgo_system *s, int32_t t, int32_t sp, do
     int lastf = -1;
     for (int f = 0; f < 4; ++f)
         if (r4[f] > 0.0) lastf = f;
-    if (nl <= DIFF_THRESHOLD) {
+    if (nl <= s->diff_threshold) {
         for (int32_t m = 0; m < nl; ++m) {
             int chosen = -1;
             for (int tries = 0; tries < 64 && chosen < 0; ++tries) {

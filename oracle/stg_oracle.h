@@ -45,6 +45,9 @@ typedef struct stgo_system {
     /* Philox key and stream id */
     uint64_t seed;
     uint32_t stream;
+    /* leaving counts up to this are split one molecule at a time
+     * (diffusion_operator.hpp:99 default 10, setDiffApplyThreshold) */
+    int32_t diff_threshold;
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */

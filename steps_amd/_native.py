@@ -100,6 +100,7 @@ SIGNATURES = {
     "stg_set_sreac_k": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "stg_set_diff_apply_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),
 }

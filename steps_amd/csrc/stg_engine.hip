@@ -90,6 +90,7 @@ struct stg_sim {
     int block = 256;
     bool use_graph = false;
     bool xcd_remap = true;
+    int diff_threshold = 10;  // diffusion_operator.hpp:99, tetopsplit.hpp:995
     stg::ModelK mk{};
     std::vector<int32_t> tet_off, tri_off;  // [n_real+1]
     std::vector<int32_t> block_real, block_tet0;
@@ -510,6 +511,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.n_real = s->n_real;
     k.xcd_remap = s->xcd_remap ? 1 : 0;
     k.n_blocks = (int32_t)s->block_real.size();
+    k.diff_threshold = s->diff_threshold;
     k.vol = s->d_vol;
     k.nbr = s->d_nbr;
     k.nface = s->d_nface;
@@ -946,6 +948,15 @@ int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
             }
         }
         s->use_graph = use_graph != 0;
+        s->free_graph();
+    });
+}
+
+int stg_set_diff_apply_threshold(stg_sim *s, int32_t threshold) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (threshold < 0) throw StgError{STG_ERR_ARG, "threshold must be >= 0"};
+        s->diff_threshold = threshold;
         s->free_graph();
     });
 }

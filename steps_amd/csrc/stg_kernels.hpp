@@ -32,7 +32,6 @@ constexpr int kMaxSSpecs = 8;
 constexpr int kMaxSReacs = 16;
 constexpr int kMaxEntries = 384;
 constexpr int kSurfFlag = 0x80;  // entry species >= 0x80: surface species (idx & 0x7f)
-constexpr int kDiffThreshold = 10;  // diffusion_operator.hpp:99
 constexpr long kMaxEventsPerPeriod = 1L << 22;
 constexpr double kAvogadro = 6.02214179e23;  // src/steps/math/constants.hpp:45
 constexpr double kDblEpsilon = 2.220446049250313080847e-16;
@@ -58,6 +57,7 @@ struct SysK {
     int32_t n_real;
     int32_t xcd_remap;     // nonzero: contiguous block ranges per XCD
     int32_t n_blocks;
+    int32_t diff_threshold;      // leaving counts up to this are split per molecule
     const double *vol;           // [NT]
     const int4 *nbr;             // [NT] global neighbour index, -1 = none
     const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
@@ -172,6 +172,9 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
         }
         if (a0 < kDblEpsilon) break;
         const double u1 = g.unif();
+        // -log(u) >= 1 - u: a waiting time certainly past the period end
+        // needs no logarithm
+        if (1.0 - u1 > a0 * (dt - tcur) * 1.0000001) break;
         const double u2 = g.unif();
         const double tev = tcur + (-dlog(u1) / a0);
         if (tev > dt) break;
@@ -258,7 +261,7 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     for (int f = 0; f < 4; ++f)
         if (r4[f] > 0.0) lastf = f;
     int32_t k4[4] = {0, 0, 0, 0};
-    if (nl <= kDiffThreshold) {
+    if (nl <= sys.diff_threshold) {
         for (int32_t mol = 0; mol < nl; ++mol) {
             int chosen = -1;
             for (int tries = 0; tries < 64 && chosen < 0; ++tries) {

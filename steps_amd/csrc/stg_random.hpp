@@ -7,9 +7,10 @@
 // so every (tet, period, channel) owns an independent sequence and results do
 // not depend on launch shape, thread count or batch composition.
 //
-// All floating-point arithmetic here is + - * / with contraction disabled
-// (-ffp-contract=off) so results are reproducible bit-for-bit on any IEEE-754
-// machine; log and sqrt are written out for the same reason.
+// All floating-point arithmetic here is + - * / and sqrt (all correctly
+// rounded) with contraction disabled (-ffp-contract=off), so results are
+// reproducible bit-for-bit on any IEEE-754 machine; log is written out for the
+// same reason.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -43,10 +44,10 @@ struct Philox {
                 a += 0x9E3779B9u;
                 b += 0xBB67AE85u;
             }
-            const uint32_t lo0 = 0xD2511F53u * x0;
-            const uint32_t hi0 = __umulhi(0xD2511F53u, x0);
-            const uint32_t lo1 = 0xCD9E8D57u * x2;
-            const uint32_t hi1 = __umulhi(0xCD9E8D57u, x2);
+            const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)x0;
+            const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)x2;
+            const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+            const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
             const uint32_t y0 = hi1 ^ x1 ^ a;
             const uint32_t y2 = hi0 ^ x3 ^ b;
             x0 = y0;
@@ -111,30 +112,17 @@ __device__ __forceinline__ double dlog(double x) {
     return de * 0x1.62e42fee00000p-1 + (lm + de * 0x1.a39ef35793c76p-33);
 }
 
-__device__ __forceinline__ double dsqrt(double x) {
-    if (!(x > 0.0)) return 0.0;
-    const uint64_t b = (uint64_t)__double_as_longlong(x);
-    double y = __longlong_as_double((long long)((b >> 1) + 0x1ff8000000000000ull));
-#pragma unroll
-    for (int i = 0; i < 6; ++i) y = 0.5 * (y + x / y);
-    return y;
-}
+// IEEE-754 square root: correctly rounded on gfx950 as on any conforming host.
+__device__ __forceinline__ double dsqrt(double x) { return x > 0.0 ? __builtin_sqrt(x) : 0.0; }
 
-// log(k!) exact to double for k < 10, Stirling series of lgamma(k + 1) above
+// log(k!) for k < 1024, each entry the correctly rounded double
+__device__ __constant__ const double kLfactTable[1024] = {
+#include "stg_lfact_table.inc"
+};
+
+// log(k!): table below 1024, Stirling series of lgamma(k + 1) above
 __device__ __forceinline__ double lfact(int32_t k) {
-    switch (k < 0 ? 0 : k) {
-        case 0:
-        case 1: return 0.0;
-        case 2: return 0x1.62e42fefa39efp-1;
-        case 3: return 0x1.cab0bfa2a2002p+0;
-        case 4: return 0x1.96ca77c922cf9p+1;
-        case 5: return 0x1.326643c4479c9p+2;
-        case 6: return 0x1.a51273acf01cap+2;
-        case 7: return 0x1.10ce1f32dcc30p+3;
-        case 8: return 0x1.5358e82fcb70dp+3;
-        case 9: return 0x1.99a8921a7f7cfp+3;
-        default: break;
-    }
+    if (k < 1024) return kLfactTable[k < 0 ? 0 : k];
     const double x = (double)k + 1.0;
     const double ix = 1.0 / x;
     const double ix2 = ix * ix;
@@ -153,35 +141,40 @@ __device__ __forceinline__ double ipow(double base, int32_t n) {
 }
 
 // Binomial by sequential inversion (Kachitvichyanukul & Schmeiser 1988, BINV);
-// q <= 1/2 and n*q < 10 so the expected search length is below 11.
+// q <= 1/2 and n*q < 10 so the expected search length is below 11. Written in
+// factorial-scaled form (U_x = u_x x!, R_x = f(x) x!) so the search loop needs
+// no division: U_{x+1} = (U_x - R_x)(x+1), R_{x+1} = R_x (n-x) q/(1-q).
 __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double q) {
     double r = ipow(1.0 - q, n);
     const double s = q / (1.0 - q);
-    const double a = (double)(n + 1) * s;
     double u = g.unif();
     int32_t x = 0;
     while (true) {
         if (u <= r) return x;
         if (x >= n) return n;
-        u = u - r;
+        if (x >= 150) return x;
         x += 1;
-        r = r * (a / (double)x - s);
+        u = (u - r) * (double)x;
+        r = r * ((double)(n - x + 1) * s);
     }
 }
 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
-// q <= 1/2 and n*q >= 10. Bounded at 1000 trials (acceptance is > 0.8 per trial).
-__device__ __noinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
+// q <= 1/2 and n*q >= 10. The log-factorial bound is only formed when the
+// squeeze fails (about one trial in seven). Bounded at 1000 trials
+// (acceptance is > 0.8 per trial).
+__device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
     const double nd = (double)n;
     const double spq = dsqrt(nd * q * (1.0 - q));
     const double b = 1.15 + 2.53 * spq;
+    const double ib = 1.0 / b;
     const double a = -0.0873 + 0.0248 * b + 0.01 * q;
     const double c = nd * q + 0.5;
-    const double alpha = (2.83 + 5.1 / b) * spq;
-    const double vr = 0.92 - 4.2 / b;
-    const double lpq = dlog(q / (1.0 - q));
+    const double alpha = (2.83 + 5.1 * ib) * spq;
+    const double vr = 0.92 - 4.2 * ib;
     const int32_t m = (int32_t)floor((nd + 1.0) * q);
-    const double h = lfact(m) + lfact(n - m);
+    bool tail = false;
+    double lpq = 0.0, h = 0.0;
     for (int it = 0; it < 1000; ++it) {
         const double u = g.unif() - 0.5;
         double v = g.unif();
@@ -190,6 +183,11 @@ __device__ __noinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
         if (kd < 0.0 || kd > nd) continue;
         const int32_t k = (int32_t)kd;
         if (us >= 0.07 && v <= vr) return k;
+        if (!tail) {
+            lpq = dlog(q / (1.0 - q));
+            h = lfact(m) + lfact(n - m);
+            tail = true;
+        }
         v = dlog(v * alpha / (a / (us * us) + b));
         if (v <= h - lfact(k) - lfact(n - k) + (double)(k - m) * lpq) return k;
     }

@@ -171,6 +171,9 @@ class Engine:
     def set_launch(self, block: int = 256, use_graph: bool = False):
         self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
 
+    def set_diff_apply_threshold(self, threshold: int):
+        self._chk(self.lib.stg_set_diff_apply_threshold(self.h, int(threshold)))
+
     def last_run_kernel_ms(self):
         ms, n = C.c_double(), C.c_int32()
         self._chk(self.lib.stg_last_run_kernel_ms(self.h, C.byref(ms), C.byref(n)))

@@ -206,6 +206,56 @@ def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0)) -> Tetmesh:
     return Tetmesh(verts, tets.reshape(-1, 4))
 
 
+def load_gmsh(path, scale: float = 1.0):
+    """Read a Gmsh 2.2 ASCII mesh (the format of the reference's test/mesh fixtures,
+    read there by TetMesh.LoadGmsh, pysteps/steps/API_2/geom.py). Returns the
+    Tetmesh and {physical name: (dim, element indices)}: for dim 3 the indices
+    are tets, for dim 2 they are the mesh's triangle indices."""
+    lines = iter(open(path).read().splitlines())
+    names, nodes, tets, tet_tag, tris, tri_tag = {}, {}, [], [], [], []
+    for ln in lines:
+        ln = ln.strip()
+        if ln == "$PhysicalNames":
+            for _ in range(int(next(lines))):
+                d, tag, name = next(lines).split(maxsplit=2)
+                names[int(tag)] = (int(d), name.strip('"'))
+        elif ln == "$Nodes":
+            for _ in range(int(next(lines))):
+                p = next(lines).split()
+                nodes[int(p[0])] = [float(x) * scale for x in p[1:4]]
+        elif ln == "$Elements":
+            for _ in range(int(next(lines))):
+                p = [int(x) for x in next(lines).split()]
+                etype, ntag = p[1], p[2]
+                tag = p[3] if ntag > 0 else 0
+                vs = p[3 + ntag:]
+                if etype == 4:
+                    tets.append(vs)
+                    tet_tag.append(tag)
+                elif etype == 2:
+                    tris.append(vs)
+                    tri_tag.append(tag)
+    ids = sorted(nodes)
+    index = {v: i for i, v in enumerate(ids)}
+    verts = np.array([nodes[v] for v in ids])
+    mesh = Tetmesh(verts, np.array([[index[v] for v in t] for t in tets]))
+    groups: dict[str, tuple[int, np.ndarray]] = {}
+    tet_tag = np.array(tet_tag)
+    for tag, (dim, name) in names.items():
+        if dim == 3:
+            groups[name] = (3, np.nonzero(tet_tag == tag)[0])
+    if tris:
+        keys = _face_keys(np.array([[index[v] for v in t] for t in tris]), len(verts))
+        mesh_keys = _face_keys(mesh.tris, len(verts))
+        order = np.argsort(mesh_keys)
+        pos = order[np.searchsorted(mesh_keys[order], keys)]
+        tri_tag = np.array(tri_tag)
+        for tag, (dim, name) in names.items():
+            if dim == 2:
+                groups[name] = (2, pos[tri_tag == tag])
+    return mesh, groups
+
+
 @dataclass
 class MeshTables:
     """The engine's per-realization mesh arrays (stg_mesh in include/stg.h)."""

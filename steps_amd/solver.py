@@ -1,0 +1,380 @@
+"""Solver classes with the interface of the reference's operator-split solvers.
+
+``TetOpSplit`` mirrors ``_py_TetOpSplitP`` (pysteps/cysteps_mpi.pyx:74-1546,
+C++ TetOpSplitP in src/steps/mpi/tetopsplit/tetopsplit.hpp) and
+``DistTetOpSplit`` mirrors ``_py_DistTetOpSplitP`` (pysteps/cysteps_dist.pyx:
+1574-2932). Both run on the HIP engine behind include/stg.h with the whole mesh
+resident on one GPU; the host partition arguments are accepted and ignored.
+Method names, argument meaning and error types follow the reference: invalid
+arguments raise ``ArgErr`` (a ValueError), unsupported features ``NotImplErr``.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as N
+from .engine import Engine
+from .geom import Tetmesh, mesh_tables
+from .model import Model, flatten
+
+EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC = 0, 1, 2, 3  # src/steps/solver/api.hpp:61-66
+AVOGADRO = 6.02214179e23
+
+
+def _argerr(msg):
+    return N.ArgErr(N.STG_ERR_ARG, msg)
+
+
+def _notimpl(msg):
+    return N.NotImplErr(N.STG_ERR_NOTIMPL, msg)
+
+
+def distribute_counts(n: float, vol: np.ndarray, rng: np.random.Generator) -> np.ndarray:
+    """Volume-weighted fair distribution of n molecules over tets, the scheme of
+    util::distribute_quantity (src/steps/util/distribute.hpp): stochastic
+    rounding of n, floor of each tet's share, and the remainder placed on
+    distinct tets sampled with probability proportional to the fractional
+    parts (Pareto pi-ps sampling)."""
+    if n < 0:
+        raise _argerr("negative number of molecules")
+    x = np.floor(n) + (1.0 if rng.random() < n - np.floor(n) else 0.0)
+    share = x * vol / vol.sum()
+    base = np.floor(share)
+    frac = share - base
+    out = base.astype(np.int64)
+    rem = int(round(x - base.sum()))
+    if rem > 0:
+        pi = frac * rem / frac.sum()
+        pi = np.clip(pi, 1e-300, 1 - 1e-12)
+        u = rng.random(len(vol))
+        q = u * (1 - pi) / ((1 - u) * pi)
+        out[np.argsort(q, kind="stable")[:rem]] += 1
+    if np.any(out > np.iinfo(np.int32).max):
+        raise _argerr("count exceeds the int32 range of a tet")
+    return out.astype(np.int32)
+
+
+class TetOpSplit:
+    """Operator-split stochastic reaction-diffusion on one GPU (STEPS 'TetOpSplit')."""
+
+    _mode = N.MODE_TETOPSPLIT
+    _name = "TetOpSplitP"
+
+    def __init__(self, model: Model, geom: Tetmesh, rng, calcMembPot: int = 0, tet_hosts=(),
+                 tri_hosts=None, wm_hosts=()):
+        if calcMembPot != EF_NONE:
+            raise _notimpl("the membrane potential solve is selected through the EField "
+                           "interface of steps_amd.efield")
+        self._setup(model, geom, rng)
+
+    # --- construction ---
+    def _setup(self, model, geom, rng):
+        if not isinstance(geom, Tetmesh):
+            raise _argerr("geometry must be a Tetmesh")
+        if len(geom.comps) != 1:
+            raise _notimpl("exactly one compartment is supported (got %d)" % len(geom.comps))
+        self.model, self.geom, self.rng = model, geom, rng
+        self.comp = next(iter(geom.comps.values()))
+        patches = [p for p in geom.patches.values() if p.surfsys]
+        if len(patches) > 1:
+            raise _notimpl("at most one patch with surface systems is supported")
+        self.patch = patches[0] if patches else None
+        if self.patch is not None and self.patch.icomp is not self.comp:
+            raise _argerr("the patch's inner compartment must be the solver's compartment")
+        self.fm = flatten(model, self.comp.volsys, self.patch.surfsys if self.patch else ())
+        self.mt = mesh_tables(geom, self.comp, self.patch.tris if self.patch is not None else None)
+        self._tet_local = -np.ones(geom.countTets(), dtype=np.int64)
+        self._tet_local[self.comp.tets] = np.arange(len(self.comp.tets))
+        self._tri_local = -np.ones(geom.countTris(), dtype=np.int64)
+        if self.patch is not None:
+            self._tri_local[self.patch.tris] = np.arange(len(self.patch.tris))
+        self._seed = int(getattr(rng, "seed", 0))
+        self.eng = Engine(self.fm, [self.mt], [self._seed], mode=self._mode)
+        self._host_rng = np.random.default_rng([self._seed, 0x5EED])
+
+    # --- identity ---
+    def getSolverName(self):
+        return self._name
+
+    def getSolverDesc(self):
+        return "Operator-split spatial SSA and multinomial diffusion on one MI355X"
+
+    def getSolverAuthors(self):
+        return "steps_amd"
+
+    def getSolverEmail(self):
+        return ""
+
+    # --- time stepping ---
+    def reset(self):
+        seed = int(getattr(self.rng, "seed", self._seed))
+        if seed != self._seed:
+            self._seed = seed
+            self.eng.set_seed(seed)
+        self._host_rng = np.random.default_rng([self._seed, 0x5EED])
+        self.eng.reset()
+
+    def run(self, endtime: float):
+        self.eng.run(endtime)
+
+    def advance(self, adv: float):
+        self.eng.advance(adv)
+
+    def step(self):
+        raise _notimpl("step() is not implemented for operator splitting")  # tetopsplit.cpp:1969
+
+    def getTime(self) -> float:
+        return self.eng.time()
+
+    def setTime(self, t: float):
+        self.eng.set_time(t)
+
+    def getNSteps(self) -> int:
+        return self.eng.nsteps()
+
+    getNIteration = getNSteps
+
+    def getUpdPeriod(self) -> float:
+        return self.eng.update_period()
+
+    def getReacExtent(self, local: bool = False) -> int:
+        return self.eng.extents()[0]
+
+    def getDiffExtent(self, local: bool = False) -> int:
+        return self.eng.extents()[1]
+
+    def setDiffApplyThreshold(self, threshold: int):
+        self.eng.set_diff_apply_threshold(threshold)
+
+    # --- lookups ---
+    def _spec(self, s: str) -> int:
+        try:
+            return self.fm.spec_index(s)
+        except KeyError as e:
+            raise _argerr(str(e)) from None
+
+    def _sspec(self, s: str) -> int:
+        try:
+            return self.fm.sspec_index(s)
+        except KeyError as e:
+            raise _argerr(str(e)) from None
+
+    def _check_comp(self, c: str):
+        if c != self.comp.name:
+            raise _argerr(f"unknown compartment {c!r}")
+
+    def _check_patch(self, p: str):
+        if self.patch is None or p != self.patch.name:
+            raise _argerr(f"unknown patch {p!r}")
+
+    def _tet(self, t: int) -> int:
+        if t < 0 or t >= len(self._tet_local) or self._tet_local[t] < 0:
+            raise _argerr(f"tetrahedron {t} is not in the solver's compartment")
+        return int(self._tet_local[t])
+
+    def _tri(self, t: int) -> int:
+        if t < 0 or t >= len(self._tri_local) or self._tri_local[t] < 0:
+            raise _argerr(f"triangle {t} is not in the patch")
+        return int(self._tri_local[t])
+
+    # --- compartments ---
+    def getCompVol(self, c: str) -> float:
+        self._check_comp(c)
+        return float(self.mt.vol.sum())
+
+    def getCompSpecCount(self, c: str, s: str) -> float:
+        self._check_comp(c)
+        return float(self.eng.sum_tet_counts(self._spec(s)))
+
+    def setCompSpecCount(self, c: str, s: str, n: float):
+        self._check_comp(c)
+        self.eng.set_tet_counts(self._spec(s), distribute_counts(n, self.mt.vol, self._host_rng))
+
+    def getCompSpecAmount(self, c: str, s: str) -> float:
+        return self.getCompSpecCount(c, s) / AVOGADRO
+
+    def setCompSpecAmount(self, c: str, s: str, a: float):
+        self.setCompSpecCount(c, s, a * AVOGADRO)
+
+    def getCompSpecConc(self, c: str, s: str) -> float:
+        return self.getCompSpecCount(c, s) / (1e3 * self.getCompVol(c) * AVOGADRO)
+
+    def setCompSpecConc(self, c: str, s: str, conc: float):
+        self.setCompSpecCount(c, s, conc * 1e3 * self.getCompVol(c) * AVOGADRO)
+
+    def getCompReacK(self, c: str, r: str) -> float:
+        self._check_comp(c)
+        return float(self._reac_k()[self._reac(r)])
+
+    def setCompReacK(self, c: str, r: str, k: float):
+        self._check_comp(c)
+        j = self._reac(r)
+        self.eng.set_reac_k(j, k)
+        self._kcache[j] = k
+
+    def getCompDiffD(self, c: str, d: str) -> float:
+        self._check_comp(c)
+        return float(self._dcst()[self._diff(d)])
+
+    def setCompDiffD(self, c: str, d: str, dcst: float, direction_comp=None):
+        self._check_comp(c)
+        s = self._diff(d)
+        self.eng.set_dcst(s, dcst)
+        self._dcache[s] = dcst
+
+    def _reac(self, r):
+        try:
+            return self.fm.reacs.index(r)
+        except ValueError:
+            raise _argerr(f"unknown reaction {r!r}") from None
+
+    def _diff(self, d):
+        if d not in self.fm.diffs:
+            raise _argerr(f"unknown diffusion rule {d!r}")
+        return self.fm.diffs[d]
+
+    @property
+    def _kcache(self):
+        if not hasattr(self, "_kc"):
+            self._kc = self.fm.reac_k.copy()
+        return self._kc
+
+    def _reac_k(self):
+        return self._kcache
+
+    @property
+    def _dcache(self):
+        if not hasattr(self, "_dc"):
+            self._dc = self.fm.spec_dcst.copy()
+        return self._dc
+
+    def _dcst(self):
+        return self._dcache
+
+    # --- tetrahedra ---
+    def getTetVol(self, t: int) -> float:
+        return float(self.mt.vol[self._tet(t)])
+
+    def getTetSpecCount(self, t: int, s: str) -> float:
+        return float(self.eng.batch_tet_counts(self._spec(s), [self._tet(t)])[0])
+
+    def setTetSpecCount(self, t: int, s: str, n: float):
+        if n < 0:
+            raise _argerr("negative number of molecules")
+        sp = self._spec(s)
+        c = self.eng.get_tet_counts(sp)
+        c[self._tet(t)] = int(np.floor(n) + (1 if self._host_rng.random() < n - np.floor(n) else 0))
+        self.eng.set_tet_counts(sp, c)
+
+    def getTetSpecConc(self, t: int, s: str) -> float:
+        return self.getTetSpecCount(t, s) / (1e3 * self.getTetVol(t) * AVOGADRO)
+
+    def setTetSpecConc(self, t: int, s: str, conc: float):
+        self.setTetSpecCount(t, s, conc * 1e3 * self.getTetVol(t) * AVOGADRO)
+
+    def getBatchTetSpecCounts(self, tets, s: str):
+        return self.eng.batch_tet_counts(self._spec(s), [self._tet(int(t)) for t in tets]).tolist()
+
+    def getBatchTetSpecCountsNP(self, index_array, s: str, counts):
+        idx = np.asarray(index_array, dtype=np.int64)
+        if len(counts) != len(idx):
+            raise _argerr("counts and index arrays differ in length")
+        loc = self._tet_local[idx] if len(idx) else idx
+        if np.any(idx < 0) or np.any(idx >= len(self._tet_local)) or np.any(loc < 0):
+            raise _argerr("tetrahedron index out of range")
+        counts[:] = self.eng.batch_tet_counts(self._spec(s), loc)
+
+    def getBatchTetSpecConcsNP(self, index_array, s: str, concs):
+        self.getBatchTetSpecCountsNP(index_array, s, concs)
+        idx = np.asarray(index_array, dtype=np.int64)
+        concs[:] = concs / (1e3 * self.mt.vol[self._tet_local[idx]] * AVOGADRO)
+
+    def sumBatchTetCountsNP(self, tet_array, s: str) -> float:
+        c = np.zeros(len(tet_array))
+        self.getBatchTetSpecCountsNP(tet_array, s, c)
+        return float(c.sum())
+
+    # --- patches and triangles ---
+    def getPatchSpecCount(self, p: str, s: str) -> float:
+        self._check_patch(p)
+        return float(self.eng.sum_tri_counts(self._sspec(s)))
+
+    def setPatchSpecCount(self, p: str, s: str, n: float):
+        self._check_patch(p)
+        self.eng.set_tri_counts(self._sspec(s),
+                                distribute_counts(n, self.mt.tri_area, self._host_rng))
+
+    def getPatchSReacK(self, p: str, r: str) -> float:
+        self._check_patch(p)
+        return float(self.fm.sreac_k[self._sreac(r)])
+
+    def setPatchSReacK(self, p: str, r: str, k: float):
+        self._check_patch(p)
+        j = self._sreac(r)
+        self.eng.set_sreac_k(j, k)
+        self.fm.sreac_k[j] = k
+
+    def _sreac(self, r):
+        try:
+            return self.fm.sreacs.index(r)
+        except ValueError:
+            raise _argerr(f"unknown surface reaction {r!r}") from None
+
+    def getTriSpecCount(self, tri: int, s: str) -> float:
+        return float(self.eng.get_tri_counts(self._sspec(s))[self._tri(tri)])
+
+    def setTriSpecCount(self, tri: int, s: str, n: float):
+        if n < 0:
+            raise _argerr("negative number of molecules")
+        sp = self._sspec(s)
+        c = self.eng.get_tri_counts(sp)
+        c[self._tri(tri)] = int(round(n))
+        self.eng.set_tri_counts(sp, c)
+
+    def getBatchTriSpecCountsNP(self, index_array, s: str, counts):
+        c = self.eng.get_tri_counts(self._sspec(s))
+        counts[:] = c[[self._tri(int(t)) for t in index_array]]
+
+    # --- checkpointing (own format: counts, time and period counter) ---
+    def checkpoint(self, file_name: str):
+        st = {f"v_{i}": self.eng.get_tet_counts(i) for i in range(len(self.fm.specs))}
+        st.update({f"s_{i}": self.eng.get_tri_counts(i) for i in range(len(self.fm.sspecs))})
+        np.savez(file_name, time=self.getTime(), nsteps=self.getNSteps(), **st)
+
+    def restore(self, file_name: str):
+        raise _notimpl("restore of the period counter needs an engine setter; "
+                       "reload counts with setTetSpecCount instead")
+
+
+class DistTetOpSplit(TetOpSplit):
+    """Distributed-mesh operator splitting (STEPS 'DistTetOpSplit'), whole mesh on one GPU.
+    The per-tet SSA always uses independent groups; the SSA method and search
+    method arguments are accepted for compatibility (cysteps_dist.pyx:1581-1633)."""
+
+    _mode = N.MODE_DISTTETOPSPLIT
+    _name = "DistTetOpSplitP"
+    UNIFORM, MULTINOMIAL = 0, 1
+
+    def __init__(self, model: Model, mesh: Tetmesh, rng, SSAMethod=0, searchMethod=0,
+                 indepKProcs: bool = False, isEfield: bool = True):
+        self._setup(model, mesh, rng)
+
+    def setCompSpecCount(self, c: str, s: str, n: float, distribution: int = 0):
+        if distribution == self.MULTINOMIAL:
+            self._check_comp(c)
+            cnt = self._host_rng.multinomial(int(round(n)), self.mt.vol / self.mt.vol.sum())
+            self.eng.set_tet_counts(self._spec(s), cnt.astype(np.int32))
+        else:
+            super().setCompSpecCount(c, s, n)
+
+
+_SOLVERS = {"TetOpSplit": TetOpSplit, "DistTetOpSplit": DistTetOpSplit}
+
+
+def get_solver(name: str):
+    """Solver class for a reference solver name, as MPI._getSolver does for
+    _solverNameMapping (pysteps/steps/API_2/sim.py:1502-1505, 1564-1579)."""
+    try:
+        return _SOLVERS[name]
+    except KeyError:
+        raise _argerr(f"solver {name!r} is not provided by this engine") from None

@@ -141,3 +141,16 @@ def test_errors_are_reported():
         eng.set_tet_counts(0, -np.ones(w.n_tets))
     with pytest.raises(N.ArgErr):
         eng.set_launch(100, False)
+
+
+@pytest.mark.parametrize("threshold", [0, 3, 1000])
+def test_diff_apply_threshold_bit_exact(threshold):
+    """Per-molecule vs conditional-binomial face split, at any threshold."""
+    w = W.four_species_cube(n=6, side=6e-6, seed=11, end_time=0.02)
+    print(f"case threshold={threshold}", flush=True)
+    eng = _engine(w, 5)
+    eng.set_diff_apply_threshold(threshold)
+    o = _oracle(w, 5, diff_threshold=threshold)
+    eng.run(w.end_time)
+    o.run(w.end_time, 0)
+    assert np.array_equal(_gpu_counts(eng, w), o.counts)
