@@ -173,6 +173,76 @@ int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_
  * stg_run, timed with HIP events on the handle's stream; *n = launches. */
 int stg_last_run_kernel_ms(const stg_sim *sim, double *ms_per_launch, int32_t *n);
 
+/* ------------------------------------------------------------------ EField
+ * Membrane potential on the vertices of a tet mesh, one membrane. Replaces
+ * EField / dVSolverBase (src/steps/solver/efield/efield.cpp:48-110,
+ * dVsolver.hpp:126-171) with the BDSystem / PETSc solvers
+ * (bdsystem.cpp:40-157, dVsolver_petsc.cpp:163-268) and the DistTetOpSplit
+ * EFieldOperator (src/steps/mpi/dist/tetopsplit/operator/efield_operator.cpp:
+ * 387-540). Arguments are SI (m, s, V, A, F/m^2, ohm.m, ohm.m^2) as in the
+ * reference's setters. */
+typedef struct stg_efield stg_efield;
+
+typedef struct stg_efield_mesh {
+    int32_t n_verts;
+    const double *verts;   /* [n_verts*3] m */
+    int32_t n_tets;
+    const int32_t *tets;   /* [n_tets*4] */
+    int32_t n_tris;
+    const int32_t *tris;   /* [n_tris*3] membrane triangles, in the order of the
+                              simulation's patch triangles when one is attached */
+} stg_efield_mesh;
+
+/* EField::initMesh (efield.cpp:48-110); potential -65 mV, 1 uF/cm^2, 1 ohm.m. */
+int stg_efield_create(const stg_efield_mesh *mesh, stg_efield **out);
+int stg_efield_destroy(stg_efield *ef);
+const char *stg_efield_last_error(const stg_efield *ef);
+
+/* EField::setMembPotential / setMembCapac / setMembVolRes /
+ * setSurfaceResistivity (efield.cpp:163-195). */
+int stg_efield_set_potential(stg_efield *ef, double v);
+int stg_efield_set_capac(stg_efield *ef, double cm);
+int stg_efield_set_volres(stg_efield *ef, double ro);
+int stg_efield_set_memb_res(stg_efield *ef, double rspec, double vrev);
+
+/* EField::setVertIClamp / setTriIClamp (efield.cpp:213-225, 376-381):
+ * positive current depolarises. setVertVClamped (efield.cpp:255-264). */
+int stg_efield_set_vert_iclamp(stg_efield *ef, int32_t vert, double amps);
+int stg_efield_set_tri_iclamp(stg_efield *ef, int32_t tri, double amps);
+int stg_efield_set_vert_clamped(stg_efield *ef, int32_t vert, int32_t clamped);
+
+/* EField::getVertV / setVertV / getTriV (efield.cpp:238-300), all vertices or
+ * triangles at once, in volts. */
+int stg_efield_set_verts_v(stg_efield *ef, const double *v);
+int stg_efield_get_verts_v(stg_efield *ef, double *v);
+int stg_efield_get_tris_v(stg_efield *ef, double *v);
+
+/* Ohmic currents (OhmicCurr, Tri::computeI src/steps/mpi/tetopsplit/tri.cpp:
+ * 664-693): for each k, channels of surface species sspec[k] of the attached
+ * simulation conduct g[k] siemens each with reversal potential erev[k]. */
+int stg_efield_set_ohmic(stg_efield *ef, int32_t n, const int32_t *sspec, const double *g,
+                         const double *erev);
+
+/* Couple to realization real of a simulation whose patch triangles are the
+ * EField's membrane triangles (NULL detaches). */
+int stg_efield_attach(stg_efield *ef, stg_sim *sim, int32_t real);
+
+/* EField::advance (efield.cpp:229-233): one implicit step of dt seconds with
+ * the current membrane currents. */
+int stg_efield_advance(stg_efield *ef, double dt);
+
+/* TetOpSplitP::_runWithEField (tetopsplit.cpp:1851-1954): alternate
+ * reaction-diffusion to t+efdt (stg_run) and one EField step, to endtime. */
+int stg_efield_run(stg_efield *ef, stg_sim *sim, double endtime, double efdt);
+
+/* Solver choice (no reference counterpart): precision 0 = FP32 conjugate
+ * gradients with FP64 dot products inside FP64 iterative refinement, 1 = FP64
+ * conjugate gradients; tol = relative FP64 residual ||b - A dV|| / ||b||. */
+int stg_efield_set_solver(stg_efield *ef, int32_t precision, double tol, int32_t max_iter);
+int stg_efield_stats(const stg_efield *ef, int32_t *iters, int32_t *refinements,
+                     double *rel_residual);
+int stg_efield_sizes(const stg_efield *ef, int32_t *n_verts, int32_t *n_tris, int32_t *nnz);
+
 #ifdef __cplusplus
 }
 #endif

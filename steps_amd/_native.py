@@ -70,6 +70,17 @@ class StgRealization(C.Structure):
     ]
 
 
+class StgEfieldMesh(C.Structure):
+    _fields_ = [
+        ("n_verts", C.c_int32),
+        ("verts", _dp),
+        ("n_tets", C.c_int32),
+        ("tets", _ip),
+        ("n_tris", C.c_int32),
+        ("tris", _ip),
+    ]
+
+
 # name -> (restype, argtypes); the list is the header's entry points, in order.
 SIGNATURES = {
     "stg_create": (C.c_int, [C.POINTER(StgModel), C.POINTER(StgMesh), C.c_uint64, C.c_int32,
@@ -103,6 +114,26 @@ SIGNATURES = {
     "stg_set_diff_apply_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "stg_efield_create": (C.c_int, [C.POINTER(StgEfieldMesh), C.POINTER(C.c_void_p)]),
+    "stg_efield_destroy": (C.c_int, [C.c_void_p]),
+    "stg_efield_last_error": (C.c_char_p, [C.c_void_p]),
+    "stg_efield_set_potential": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_efield_set_capac": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_efield_set_volres": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_efield_set_memb_res": (C.c_int, [C.c_void_p, C.c_double, C.c_double]),
+    "stg_efield_set_vert_iclamp": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "stg_efield_set_tri_iclamp": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "stg_efield_set_vert_clamped": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "stg_efield_set_verts_v": (C.c_int, [C.c_void_p, _dp]),
+    "stg_efield_get_verts_v": (C.c_int, [C.c_void_p, _dp]),
+    "stg_efield_get_tris_v": (C.c_int, [C.c_void_p, _dp]),
+    "stg_efield_set_ohmic": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
+    "stg_efield_attach": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
+    "stg_efield_advance": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_efield_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
+    "stg_efield_set_solver": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
+    "stg_efield_stats": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
+    "stg_efield_sizes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
 }
 
 
@@ -151,11 +182,11 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
     return lib
 
 
-def check(rc: int, handle=None) -> None:
+def check(rc: int, handle=None, efield: bool = False) -> None:
     if rc == STG_OK:
         return
     lib = load()
-    msg = lib.stg_last_error(handle)
+    msg = (lib.stg_efield_last_error if efield else lib.stg_last_error)(handle)
     msg = msg.decode() if msg else "unknown error"
     raise _ERR_CLASS.get(rc, StgError)(rc, msg)
 

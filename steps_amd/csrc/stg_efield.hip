@@ -1,0 +1,944 @@
+// Membrane-potential (EField) step on the GPU: vertex system assembly and a
+// mixed-precision conjugate-gradient solve.
+//
+// The system of one step (dVSolverBase::_advance,
+// src/steps/solver/efield/dVsolver.hpp:126-171) in microns / ms / mV / pF / nS / pA:
+//   (C_i/dt + G_i + sum_j c_ij) dV_i - sum_j c_ij dV_j
+//       = I_i + G_i (V_ext - V_i) + sum_j c_ij (V_j - V_i)
+// with c_ij = sigma * w_ij, w_ij the coupling of a piecewise-linear potential
+// over the barycentric dual cells (TetCoupler, tetcoupler.cpp:48-290), which is
+// the P1 stiffness -sum_tets V grad(phi_i).grad(phi_j) assembled here directly.
+// The matrix is symmetric positive definite; the reference solves it by banded
+// LU (bdsystem.cpp) or PETSc CG (dVsolver_petsc.cpp:163-268). Here: Jacobi-
+// preconditioned CG with the matrix and vectors in FP32 and every dot product
+// accumulated in FP64, wrapped in FP64 iterative refinement (residual b - A x
+// in FP64) until the FP64 residual meets the tolerance. An all-FP64 CG mode
+// gives the reference solution on the device.
+//
+// Reductions are deterministic: each block writes its partial, the last block
+// to finish (device-scope counter) sums the partials in block order and
+// resets the counter, so no memset is needed between launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <cmath>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/stg.h"
+
+namespace {
+
+struct EfError {
+    int code;
+    std::string msg;
+};
+
+#define EF_HIP(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw EfError{STG_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+template <typename T>
+T *ef_alloc(size_t n) {
+    void *p = nullptr;
+    EF_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    return static_cast<T *>(p);
+}
+
+template <typename T>
+void ef_put(T *d, const std::vector<T> &h) {
+    if (!h.empty()) EF_HIP(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+constexpr int kB = 256;
+
+// ------------------------------------------------------------- reductions
+
+// Sum of one double per thread into partial[blockIdx]; the last block sums the
+// partials in order into out[0] and resets the counter.
+__device__ __forceinline__ void block_sum_last(double v, double *partial, unsigned *counter,
+                                               double *out) {
+    __shared__ double red[kB / 64];
+    __shared__ bool last;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) s += red[i];
+        partial[blockIdx.x] = s;
+        __threadfence();
+        const unsigned done = atomicAdd(counter, 1u);
+        last = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double s = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += partial[i];
+        // fixed-order combine: strided per-thread sums, then lanes, then waves
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (l == 0) red[w] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) t += red[i];
+            out[0] = t;
+            *counter = 0u;
+        }
+    }
+}
+
+// ------------------------------------------------------------- kernels
+
+struct EfDev {
+    int n;                 // vertices
+    const int *rp;         // [n+1]
+    const int *col;        // [nnz] off-diagonal columns
+    const double *w;       // [nnz] geometric coupling (um)
+    const double *wsum;    // [n]   row sums of w
+    const float *a32;      // [nnz] -c_ij in FP32 (0 for clamped rows/cols)
+    const float *d32;      // [n]   diagonal in FP32
+    const double *d64;     // [n]   diagonal in FP64
+    const unsigned char *clamped;
+};
+
+// membrane current of one triangle (pA, outward positive) from ohmic channels
+// (Tri::computeI, src/steps/mpi/tetopsplit/tri.cpp:664-693)
+struct OhmicK {
+    int n;
+    int sspec[8];
+    double g[8];      // S per channel
+    double erev[8];   // V
+};
+
+__global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, const int *tri_v,
+                                 const double *V, const double *tri_clamp, const double *vert_clamp,
+                                 const int *tri_counts, const int *tri_map, int n_tris_sim, OhmicK oc,
+                                 double *I) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double cur = vert_clamp[i];
+    for (int e = vt_ptr[i]; e < vt_ptr[i + 1]; ++e) {
+        const int t = vt_tri[e];
+        double c = tri_clamp[t];
+        if (oc.n > 0 && tri_counts != nullptr) {
+            const double vt = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0;
+            double im = 0.0;
+            for (int k = 0; k < oc.n; ++k) {
+                const double nopen = (double)tri_counts[(size_t)oc.sspec[k] * n_tris_sim + tri_map[t]];
+                im += nopen * oc.g[k] * (vt * 1e-3 - oc.erev[k]);
+            }
+            c -= im * 1e12;
+        }
+        cur += c / 3.0;
+    }
+    I[i] = cur;
+}
+
+// b = I + G (Vext - V) + sum_j c_ij (V_j - V_i); diagonal C/dt + G + sigma*wsum
+__global__ void k_assemble(EfDev m, double sigma, double oodt, const double *cap, const double *gext,
+                           double vext, const double *V, const double *I, double *b, double *d64,
+                           float *d32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    if (m.clamped[i]) {
+        b[i] = 0.0;
+        d64[i] = 1.0;
+        d32[i] = 1.0f;
+        return;
+    }
+    const double vi = V[i];
+    double rhs = I[i] + gext[i] * (vext - vi);
+    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) rhs += sigma * m.w[e] * (V[m.col[e]] - vi);
+    b[i] = rhs;
+    const double d = cap[i] * oodt + gext[i] + sigma * m.wsum[i];
+    d64[i] = d;
+    d32[i] = (float)d;
+}
+
+__global__ void k_offdiag32(EfDev m, double sigma, float *a32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+        const bool off = m.clamped[i] || m.clamped[m.col[e]];
+        a32[e] = off ? 0.0f : (float)(-sigma * m.w[e]);
+    }
+}
+
+// r = b - A x in FP64, and ||r||^2, ||b||^2
+__global__ void k_residual64(EfDev m, double sigma, const double *b, const double *x, double *r,
+                             double *part_r, unsigned *cnt_r, double *out_r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rr = 0.0;
+    if (i < m.n) {
+        double ax = m.d64[i] * x[i];
+        if (!m.clamped[i]) {
+            for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+                const int j = m.col[e];
+                if (!m.clamped[j]) ax -= sigma * m.w[e] * x[j];
+            }
+        }
+        const double ri = b[i] - ax;
+        r[i] = ri;
+        rr = ri * ri;
+    }
+    block_sum_last(rr, part_r, cnt_r, out_r);
+}
+
+// PCG iteration, kernel A: p_new = z + beta p_old; q = A p_new; partial (p_new, q).
+// Neighbour values of p_new are formed on the fly from z and p_old, so p_old
+// is only read and p_new only written (ping-pong buffers).
+template <typename T>
+__global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *q, const double *scal,
+                       int first, double *part, unsigned *cnt, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double pq = 0.0;
+    if (i < m.n) {
+        // the first iteration never reads p_old (it is uninitialised memory)
+        const double beta = first ? 0.0 : scal[1];
+        const T pi = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
+        T qi;
+        if constexpr (sizeof(T) == 4) {
+            float acc = m.d32[i] * (float)pi;
+            for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+                const int j = m.col[e];
+                const float pj = first ? (float)z[j] : (float)((double)z[j] + beta * (double)p[j]);
+                acc += m.a32[e] * pj;
+            }
+            qi = acc;
+        } else {
+            double acc = m.d64[i] * (double)pi;
+            if (!m.clamped[i]) {
+                for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+                    const int j = m.col[e];
+                    if (m.clamped[j]) continue;
+                    const double pj = first ? (double)z[j] : (double)z[j] + beta * (double)p[j];
+                    acc -= sigma * m.w[e] * pj;
+                }
+            }
+            qi = acc;
+        }
+        q[i] = qi;
+        pn[i] = pi;
+        pq = (double)pi * (double)qi;
+    }
+    block_sum_last(pq, part, cnt, out);
+}
+
+// kernel B: alpha = rz / pq; x += alpha p; r -= alpha q; z = r / diag; partial (r, z)
+template <typename T>
+__global__ void k_cg_b(EfDev m, const double *scal, const double *pq_out, T *x, T *r, T *z,
+                       const T *pnew, const T *q, double *part, unsigned *cnt, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0;
+    if (i < m.n) {
+        const double alpha = scal[0] / pq_out[0];
+        const double xi = (double)x[i] + alpha * (double)pnew[i];
+        const double ri = (double)r[i] - alpha * (double)q[i];
+        const double dg = sizeof(T) == 4 ? (double)m.d32[i] : m.d64[i];
+        const T zi = (T)(ri / dg);
+        x[i] = (T)xi;
+        r[i] = (T)ri;
+        z[i] = zi;
+        rz = (double)r[i] * (double)zi;
+    }
+    block_sum_last(rz, part, cnt, out);
+}
+
+// scal[0] = rz_new, scal[1] = beta = rz_new / rz_old; scal[2] keeps rz_old
+__global__ void k_cg_scalars(double *scal, const double *rz_out, int first) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const double rz = rz_out[0];
+        scal[1] = first ? 0.0 : (scal[0] > 0.0 ? rz / scal[0] : 0.0);
+        scal[0] = rz;
+    }
+}
+
+// initial: r = residual (as T), z = r / diag, partial (r, z)
+template <typename T>
+__global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *part, unsigned *cnt,
+                          double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0;
+    if (i < m.n) {
+        const T ri = (T)r64[i];
+        const double dg = sizeof(T) == 4 ? (double)m.d32[i] : m.d64[i];
+        const T zi = (T)((double)ri / dg);
+        x[i] = (T)0;
+        r[i] = ri;
+        z[i] = zi;
+        rz = (double)ri * (double)zi;
+    }
+    block_sum_last(rz, part, cnt, out);
+}
+
+// Sum over the block of one double per thread; every thread gets the total.
+__device__ __forceinline__ double block_allsum(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) s += red[i];
+        red[16] = s;
+    }
+    __syncthreads();
+    return red[16];
+}
+
+// Whole Jacobi-PCG solve of a small system in one workgroup: no launches or
+// host round trips per iteration. Adds the solution to x64; writes the
+// iteration count. The loop condition depends only on block-wide sums, so
+// every thread leaves the loop together.
+template <typename T>
+__global__ void __launch_bounds__(1024) k_cg_small(EfDev m, double sigma, const double *r64, T *x, T *r,
+                                                   T *z, T *p, T *q, double rel_tol, int max_iter,
+                                                   double *x64, int *iters) {
+    __shared__ double red[17];
+    const int n = m.n, B = blockDim.x, tid = threadIdx.x;
+    const bool f32 = sizeof(T) == 4;
+    double loc = 0.0;
+    for (int i = tid; i < n; i += B) {
+        const T ri = (T)r64[i];
+        const T zi = (T)((double)ri / (f32 ? (double)m.d32[i] : m.d64[i]));
+        x[i] = (T)0;
+        r[i] = ri;
+        z[i] = zi;
+        p[i] = zi;
+        loc += (double)ri * (double)zi;
+    }
+    double rz = block_allsum(loc, red);
+    const double stop = rel_tol * rel_tol * rz;
+    int it = 0;
+    while (it < max_iter && rz > stop && rz > 0.0) {
+        loc = 0.0;
+        for (int i = tid; i < n; i += B) {
+            double qi;
+            if (f32) {
+                float acc = m.d32[i] * (float)p[i];
+                for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) acc += m.a32[e] * (float)p[m.col[e]];
+                qi = acc;
+            } else {
+                double acc = m.d64[i] * (double)p[i];
+                if (!m.clamped[i])
+                    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e)
+                        if (!m.clamped[m.col[e]]) acc -= sigma * m.w[e] * (double)p[m.col[e]];
+                qi = acc;
+            }
+            q[i] = (T)qi;
+            loc += (double)p[i] * (double)(T)qi;
+        }
+        const double pq = block_allsum(loc, red);
+        const double alpha = rz / pq;
+        loc = 0.0;
+        for (int i = tid; i < n; i += B) {
+            const double ri = (double)r[i] - alpha * (double)q[i];
+            x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+            r[i] = (T)ri;
+            const T zi = (T)((double)(T)ri / (f32 ? (double)m.d32[i] : m.d64[i]));
+            z[i] = zi;
+            loc += (double)(T)ri * (double)zi;
+        }
+        const double rzn = block_allsum(loc, red);
+        const double beta = rzn / rz;
+        rz = rzn;
+        for (int i = tid; i < n; i += B) p[i] = (T)((double)z[i] + beta * (double)p[i]);
+        __syncthreads();
+        ++it;
+    }
+    for (int i = tid; i < n; i += B) x64[i] += (double)x[i];
+    if (tid == 0) *iters = it;
+}
+
+template <typename T>
+__global__ void k_accumulate(int n, double *x64, const T *d) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x64[i] += (double)d[i];
+}
+
+__global__ void k_apply_dv(int n, const unsigned char *clamped, const double *dv, double *V) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !clamped[i]) V[i] += dv[i];
+}
+
+__global__ void k_sumsq(int n, const double *v, double *part, unsigned *cnt, double *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double x = i < n ? v[i] : 0.0;
+    block_sum_last(x * x, part, cnt, out);
+}
+
+}  // namespace
+
+struct stg_efield {
+    int n = 0, n_tris = 0, nnz = 0;
+    std::vector<int> tri_v;                 // [n_tris*3]
+    std::vector<double> vert_area, tri_area, cap_h, gext_h;
+    std::vector<double> vert_clamp_h, tri_clamp_h;
+    std::vector<unsigned char> clamped_h;
+    double sigma = 1.0 / 1.0e-3;            // nS/um, 1 ohm.m
+    double vext = 0.0;
+    double cm = 0.01;
+    int precision = 0;                      // 0 = FP32 CG + FP64 refinement, 1 = FP64 CG
+    double tol = 1e-11;                     // relative FP64 residual target
+    double floor_tol = 1e-9;                // accepted when refinement stagnates above tol
+    double inner_tol = 1e-5;                // relative preconditioned residual per FP32 solve
+    int inner_max = 4000;                   // FP32 iterations per refinement
+    int max_iter = 20000;
+    int last_iters = 0, last_refine = 0;
+    double last_rel = 0.0;
+    OhmicK oc{};
+    stg_sim *sim = nullptr;
+    int sim_real = 0;
+    std::string err;
+    hipStream_t stream = nullptr;
+    // device
+    int *rp = nullptr, *col = nullptr, *vt_ptr = nullptr, *vt_tri = nullptr, *tri_vd = nullptr;
+    int *tri_map = nullptr;  // EField triangle -> the attached simulation's triangle slot
+    double *w = nullptr, *wsum = nullptr, *d64 = nullptr, *V = nullptr, *I = nullptr, *b = nullptr;
+    double *cap = nullptr, *gext = nullptr, *vclamp = nullptr, *tclamp = nullptr;
+    double *x64 = nullptr, *r64 = nullptr, *dv_prev = nullptr;
+    float *a32 = nullptr, *d32 = nullptr;
+    unsigned char *clamped = nullptr;
+    void *xs = nullptr, *rs = nullptr, *zs = nullptr, *ps = nullptr, *qs = nullptr;  // T-typed work
+    double *part = nullptr, *scal = nullptr, *outs = nullptr;
+    unsigned *cnt = nullptr;
+    bool a32_dirty = true;
+    int nb = 1;
+
+    EfDev dev() const {
+        return EfDev{n, rp, col, w, wsum, a32, d32, d64, clamped};
+    }
+
+    ~stg_efield() {
+        void *ptrs[] = {rp, col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
+                        tclamp, x64, r64, dv_prev, a32, d32, clamped, xs, rs, zs, ps, qs, part,
+                        scal, outs, cnt};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// stg_sim accessors defined in stg_engine.hip
+extern "C" int stg_internal_tri_counts_device(stg_sim *sim, int32_t real, const int32_t **ptr,
+                                              int32_t *n_tris_total, int32_t *tri0);
+extern "C" int stg_internal_tri_perm(const stg_sim *sim, int32_t real, const int32_t **perm,
+                                     int32_t *n);
+
+namespace {
+
+thread_local std::string g_ef_error;
+
+template <typename F>
+int ef_guard(stg_efield *e, F &&f) {
+    try {
+        f();
+        return STG_OK;
+    } catch (const EfError &x) {
+        if (e) e->err = x.msg;
+        else g_ef_error = x.msg;
+        return x.code;
+    } catch (const std::bad_alloc &) {
+        if (e) e->err = "host allocation failed";
+        else g_ef_error = "host allocation failed";
+        return STG_ERR_SYS;
+    } catch (const std::exception &x) {
+        if (e) e->err = x.what();
+        else g_ef_error = x.what();
+        return STG_ERR_PROG;
+    }
+}
+
+void upload_params(stg_efield *e) {
+    ef_put(e->cap, e->cap_h);
+    ef_put(e->gext, e->gext_h);
+    ef_put(e->vclamp, e->vert_clamp_h);
+    ef_put(e->tclamp, e->tri_clamp_h);
+    ef_put(e->clamped, e->clamped_h);
+    e->a32_dirty = true;
+}
+
+void build(stg_efield *e, const stg_efield_mesh *m) {
+    if (!m || m->n_verts <= 0 || m->n_tets <= 0 || !m->verts || !m->tets)
+        throw EfError{STG_ERR_ARG, "empty or null EField mesh"};
+    if (m->n_tris < 0 || (m->n_tris > 0 && !m->tris)) throw EfError{STG_ERR_ARG, "bad membrane triangles"};
+    const int n = m->n_verts;
+    e->n = n;
+    e->n_tris = m->n_tris;
+    // P1 stiffness: w_ij = -sum_tets V grad(phi_i).grad(phi_j), coordinates in microns
+    std::vector<std::map<int, double>> rows(n);
+    for (int t = 0; t < m->n_tets; ++t) {
+        int v[4];
+        double P[4][3];
+        for (int a = 0; a < 4; ++a) {
+            v[a] = m->tets[4 * t + a];
+            if (v[a] < 0 || v[a] >= n) throw EfError{STG_ERR_ARG, "tet vertex index out of range"};
+            for (int k = 0; k < 3; ++k) P[a][k] = m->verts[3 * v[a] + k] * 1e6;
+        }
+        double E[3][3];
+        for (int a = 0; a < 3; ++a)
+            for (int k = 0; k < 3; ++k) E[a][k] = P[a + 1][k] - P[0][k];
+        const double det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) -
+                           E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
+                           E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
+        if (!(std::fabs(det) > 0.0)) throw EfError{STG_ERR_ARG, "degenerate tetrahedron"};
+        // inverse of E; gradients of phi_1..3 are the columns of inv(E)
+        double inv[3][3];
+        inv[0][0] = (E[1][1] * E[2][2] - E[1][2] * E[2][1]) / det;
+        inv[0][1] = (E[0][2] * E[2][1] - E[0][1] * E[2][2]) / det;
+        inv[0][2] = (E[0][1] * E[1][2] - E[0][2] * E[1][1]) / det;
+        inv[1][0] = (E[1][2] * E[2][0] - E[1][0] * E[2][2]) / det;
+        inv[1][1] = (E[0][0] * E[2][2] - E[0][2] * E[2][0]) / det;
+        inv[1][2] = (E[0][2] * E[1][0] - E[0][0] * E[1][2]) / det;
+        inv[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
+        inv[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
+        inv[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
+        double g[4][3];
+        for (int k = 0; k < 3; ++k) {
+            for (int a = 1; a < 4; ++a) g[a][k] = inv[k][a - 1];
+            g[0][k] = -(g[1][k] + g[2][k] + g[3][k]);
+        }
+        const double vol = std::fabs(det) / 6.0;
+        for (int a = 0; a < 4; ++a)
+            for (int c = 0; c < 4; ++c) {
+                if (a == c) continue;
+                const double kij = vol * (g[a][0] * g[c][0] + g[a][1] * g[c][1] + g[a][2] * g[c][2]);
+                rows[v[a]][v[c]] -= kij;
+            }
+    }
+    std::vector<int> rp(n + 1, 0), col;
+    std::vector<double> w, wsum(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (auto &kv : rows[i]) {
+            col.push_back(kv.first);
+            w.push_back(kv.second);
+            wsum[i] += kv.second;
+        }
+        rp[i + 1] = (int)col.size();
+        rows[i].clear();
+    }
+    e->nnz = (int)col.size();
+    // membrane: triangle areas, one third to each vertex (tetmesh.cpp:231-260)
+    e->tri_v.assign(m->tris, m->tris + 3 * (size_t)m->n_tris);
+    e->tri_area.resize(m->n_tris);
+    e->vert_area.assign(n, 0.0);
+    std::vector<std::vector<int>> vt(n);
+    for (int t = 0; t < m->n_tris; ++t) {
+        double P[3][3];
+        for (int a = 0; a < 3; ++a) {
+            const int vi = e->tri_v[3 * t + a];
+            if (vi < 0 || vi >= n) throw EfError{STG_ERR_ARG, "triangle vertex index out of range"};
+            for (int k = 0; k < 3; ++k) P[a][k] = m->verts[3 * vi + k] * 1e6;
+            vt[vi].push_back(t);
+        }
+        double u[3], v[3];
+        for (int k = 0; k < 3; ++k) {
+            u[k] = P[1][k] - P[0][k];
+            v[k] = P[2][k] - P[0][k];
+        }
+        const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2],
+                     cz = u[0] * v[1] - u[1] * v[0];
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        e->tri_area[t] = area;
+        for (int a = 0; a < 3; ++a) e->vert_area[e->tri_v[3 * t + a]] += area / 3.0;
+    }
+    std::vector<int> vt_ptr(n + 1, 0), vt_tri;
+    for (int i = 0; i < n; ++i) {
+        for (int t : vt[i]) vt_tri.push_back(t);
+        vt_ptr[i + 1] = (int)vt_tri.size();
+    }
+    e->cap_h.resize(n);
+    for (int i = 0; i < n; ++i) e->cap_h[i] = e->cm * e->vert_area[i];
+    e->gext_h.assign(n, 0.0);
+    e->vert_clamp_h.assign(n, 0.0);
+    e->tri_clamp_h.assign(std::max(m->n_tris, 1), 0.0);
+    e->clamped_h.assign(n, 0);
+
+    EF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->nb = (n + kB - 1) / kB;
+    e->rp = ef_alloc<int>(n + 1);
+    e->col = ef_alloc<int>(e->nnz);
+    e->w = ef_alloc<double>(e->nnz);
+    e->wsum = ef_alloc<double>(n);
+    e->a32 = ef_alloc<float>(e->nnz);
+    e->d32 = ef_alloc<float>(n);
+    e->d64 = ef_alloc<double>(n);
+    e->V = ef_alloc<double>(n);
+    e->I = ef_alloc<double>(n);
+    e->b = ef_alloc<double>(n);
+    e->cap = ef_alloc<double>(n);
+    e->gext = ef_alloc<double>(n);
+    e->vclamp = ef_alloc<double>(n);
+    e->tclamp = ef_alloc<double>(std::max(m->n_tris, 1));
+    e->clamped = ef_alloc<unsigned char>(n);
+    e->x64 = ef_alloc<double>(n);
+    e->r64 = ef_alloc<double>(n);
+    e->dv_prev = ef_alloc<double>(n);
+    e->xs = ef_alloc<double>(n);
+    e->rs = ef_alloc<double>(n);
+    e->zs = ef_alloc<double>(n);
+    e->ps = ef_alloc<double>(2 * (size_t)n);
+    e->qs = ef_alloc<double>(n);
+    e->part = ef_alloc<double>(e->nb);
+    e->scal = ef_alloc<double>(4);
+    e->outs = ef_alloc<double>(8);
+    e->cnt = ef_alloc<unsigned>(8);
+    e->vt_ptr = ef_alloc<int>(n + 1);
+    e->vt_tri = ef_alloc<int>(std::max<size_t>(vt_tri.size(), 1));
+    e->tri_vd = ef_alloc<int>(std::max<size_t>(e->tri_v.size(), 1));
+    ef_put(e->rp, rp);
+    ef_put(e->col, col);
+    ef_put(e->w, w);
+    ef_put(e->wsum, wsum);
+    ef_put(e->vt_ptr, vt_ptr);
+    ef_put(e->vt_tri, vt_tri);
+    ef_put(e->tri_vd, e->tri_v);
+    EF_HIP(hipMemset(e->cnt, 0, 8 * sizeof(unsigned)));
+    EF_HIP(hipMemset(e->dv_prev, 0, n * sizeof(double)));
+    std::vector<double> v0(n, -65.0);
+    ef_put(e->V, v0);
+    upload_params(e);
+}
+
+// Jacobi PCG on A d = r (r given in r64), result added to x64. Returns iterations.
+constexpr int kSmallSystem = 16384;  // solved by one persistent workgroup
+
+template <typename T>
+int pcg(stg_efield *e, double rel_tol, int max_iter) {
+    const EfDev m = e->dev();
+    const int nb = e->nb, n = e->n;
+    T *x = (T *)e->xs, *r = (T *)e->rs, *z = (T *)e->zs, *p = (T *)e->ps, *q = (T *)e->qs;
+    if (n <= kSmallSystem) {
+        hipLaunchKernelGGL(k_cg_small<T>, dim3(1), dim3(1024), 0, e->stream, m, e->sigma, e->r64, x, r, z,
+                           p, q, rel_tol, max_iter, e->x64, (int *)e->cnt + 7);
+        int it = 0;
+        EF_HIP(hipMemcpyAsync(&it, (int *)e->cnt + 7, sizeof(int), hipMemcpyDeviceToHost, e->stream));
+        EF_HIP(hipStreamSynchronize(e->stream));
+        return it;
+    }
+    double *out_rz = e->outs + 0, *out_pq = e->outs + 1;
+    hipLaunchKernelGGL(k_cg_init<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->r64, x, r, z, e->part,
+                       e->cnt + 0, out_rz);
+    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, e->stream, e->scal, out_rz, 1);
+    double rz0 = 0.0;
+    EF_HIP(hipMemcpyAsync(&rz0, out_rz, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    EF_HIP(hipStreamSynchronize(e->stream));
+    int it = 0;
+    if (!(rz0 > 0.0)) return 0;
+    const int check = 16;
+    while (it < max_iter) {
+        for (int k = 0; k < check && it < max_iter; ++k, ++it) {
+            T *pold = p + (size_t)(it & 1) * n, *pnew = p + (size_t)((it + 1) & 1) * n;
+            hipLaunchKernelGGL(k_cg_a<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, z, pold, pnew, q,
+                               e->scal, it == 0 ? 1 : 0, e->part, e->cnt + 1, out_pq);
+            hipLaunchKernelGGL(k_cg_b<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->scal, out_pq, x, r, z,
+                               pnew, q, e->part, e->cnt + 2, out_rz);
+            hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, e->stream, e->scal, out_rz, 0);
+        }
+        double rz = 0.0;
+        EF_HIP(hipMemcpyAsync(&rz, e->scal, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+        EF_HIP(hipStreamSynchronize(e->stream));
+        if (!(rz > rel_tol * rel_tol * rz0)) break;
+    }
+    EF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->x64, x);
+    return it;
+}
+
+double fetch(stg_efield *e, const double *d) {
+    double h = 0.0;
+    EF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    EF_HIP(hipStreamSynchronize(e->stream));
+    return h;
+}
+
+void advance_impl(stg_efield *e, double dt_s) {
+    if (!(dt_s > 0.0)) throw EfError{STG_ERR_ARG, "EField time step must be positive"};
+    const int n = e->n, nb = e->nb;
+    const EfDev m = e->dev();
+    const int32_t *tri_counts = nullptr;
+    int32_t ntot = 0, tri0 = 0;
+    if (e->sim && e->oc.n > 0) {
+        const int rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tri_counts, &ntot, &tri0);
+        if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
+        tri_counts += tri0;
+    }
+    hipLaunchKernelGGL(k_vertex_current, dim3(nb), dim3(kB), 0, e->stream, n, e->vt_ptr, e->vt_tri,
+                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, e->tri_map, ntot, e->oc, e->I);
+    const double oodt = 1.0 / (dt_s * 1e3);
+    hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, oodt, e->cap, e->gext,
+                       e->vext, e->V, e->I, e->b, e->d64, e->d32);
+    if (e->a32_dirty) {
+        hipLaunchKernelGGL(k_offdiag32, dim3(nb), dim3(kB), 0, e->stream, e->dev(), e->sigma, e->a32);
+        e->a32_dirty = false;
+    }
+    // initial guess: the previous step's increment
+    EF_HIP(hipMemcpyAsync(e->x64, e->dv_prev, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kB), 0, e->stream, n, e->b, e->part, e->cnt + 3, e->outs + 3);
+    const double bb = fetch(e, e->outs + 3);
+    int iters = 0, refine = 0;
+    double rel = 0.0, prev = INFINITY;
+    bool ok = true;
+    if (bb > 0.0) {
+        for (refine = 0;; ++refine) {
+            hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, e->b, e->x64,
+                               e->r64, e->part, e->cnt + 4, e->outs + 4);
+            const double rr = fetch(e, e->outs + 4);
+            rel = std::sqrt(rr / bb);
+            if (rel <= e->tol) break;
+            // the FP64 residual has a rounding floor near eps * cond(A); stop
+            // once a refinement no longer halves it
+            if (refine >= 2 && rel > 0.5 * prev) {
+                ok = rel <= e->floor_tol;
+                break;
+            }
+            if (refine >= 12) {
+                ok = false;
+                break;
+            }
+            prev = rel;
+            if (e->precision == 1) iters += pcg<double>(e, std::max(e->tol / rel, 1e-15), e->max_iter);
+            else iters += pcg<float>(e, e->inner_tol, std::min(e->inner_max, e->max_iter));
+        }
+    } else {
+        EF_HIP(hipMemsetAsync(e->x64, 0, n * sizeof(double), e->stream));
+    }
+    e->last_iters = iters;
+    e->last_refine = refine;
+    e->last_rel = rel;
+    if (!ok) throw EfError{STG_ERR_PROG, "EField solve did not reach the residual tolerance"};
+    hipLaunchKernelGGL(k_apply_dv, dim3(nb), dim3(kB), 0, e->stream, n, e->clamped, e->x64, e->V);
+    EF_HIP(hipMemcpyAsync(e->dv_prev, e->x64, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    EF_HIP(hipGetLastError());
+    EF_HIP(hipStreamSynchronize(e->stream));
+}
+
+void check_v(const stg_efield *e, int32_t v) {
+    if (v < 0 || v >= e->n) throw EfError{STG_ERR_ARG, "vertex index out of range"};
+}
+
+void check_t(const stg_efield *e, int32_t t) {
+    if (t < 0 || t >= e->n_tris) throw EfError{STG_ERR_ARG, "triangle index out of range"};
+}
+
+}  // namespace
+
+extern "C" {
+
+int stg_efield_create(const stg_efield_mesh *mesh, stg_efield **out) {
+    if (!out) {
+        g_ef_error = "null output handle";
+        return STG_ERR_ARG;
+    }
+    *out = nullptr;
+    stg_efield *e = new (std::nothrow) stg_efield();
+    if (!e) return STG_ERR_SYS;
+    const int rc = ef_guard(nullptr, [&] { build(e, mesh); });
+    if (rc != STG_OK) {
+        delete e;
+        return rc;
+    }
+    *out = e;
+    return STG_OK;
+}
+
+int stg_efield_destroy(stg_efield *e) {
+    delete e;
+    return STG_OK;
+}
+
+const char *stg_efield_last_error(const stg_efield *e) {
+    return e ? e->err.c_str() : g_ef_error.c_str();
+}
+
+int stg_efield_set_potential(stg_efield *e, double v) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        std::vector<double> h(e->n, v * 1e3);
+        ef_put(e->V, h);
+        EF_HIP(hipMemset(e->dv_prev, 0, e->n * sizeof(double)));
+    });
+}
+
+int stg_efield_set_capac(stg_efield *e, double cm) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (!(cm >= 0.0)) throw EfError{STG_ERR_ARG, "capacitance must be >= 0"};
+        e->cm = cm;  // F/m^2 = pF/um^2 (efield.cpp:151-160)
+        for (int i = 0; i < e->n; ++i) e->cap_h[i] = cm * e->vert_area[i];
+        upload_params(e);
+    });
+}
+
+int stg_efield_set_volres(stg_efield *e, double ro) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (!(ro > 0.0)) throw EfError{STG_ERR_ARG, "volume resistivity must be > 0"};
+        e->sigma = 1.0 / (ro * 1.0e-3);  // efield.cpp:183-186
+        e->a32_dirty = true;
+    });
+}
+
+int stg_efield_set_memb_res(stg_efield *e, double rspec, double vrev) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (!(rspec > 0.0)) throw EfError{STG_ERR_ARG, "membrane resistivity must be > 0"};
+        const double rs = 1.0e3 * rspec;  // efield.cpp:188-195
+        for (int i = 0; i < e->n; ++i) e->gext_h[i] = e->vert_area[i] / rs;
+        e->vext = vrev * 1.0e3;
+        upload_params(e);
+    });
+}
+
+int stg_efield_set_vert_iclamp(stg_efield *e, int32_t v, double amps) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_v(e, v);
+        e->vert_clamp_h[v] = amps * 1e12;
+        upload_params(e);
+    });
+}
+
+int stg_efield_set_tri_iclamp(stg_efield *e, int32_t t, double amps) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_t(e, t);
+        e->tri_clamp_h[t] = amps * 1e12;
+        upload_params(e);
+    });
+}
+
+int stg_efield_set_vert_clamped(stg_efield *e, int32_t v, int32_t clamped) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_v(e, v);
+        e->clamped_h[v] = clamped ? 1 : 0;
+        upload_params(e);
+    });
+}
+
+int stg_efield_set_verts_v(stg_efield *e, const double *v) {
+    if (!e || !v) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        std::vector<double> h(v, v + e->n);
+        for (auto &x : h) x *= 1e3;
+        ef_put(e->V, h);
+    });
+}
+
+int stg_efield_get_verts_v(stg_efield *e, double *v) {
+    if (!e || !v) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        EF_HIP(hipMemcpy(v, e->V, e->n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int i = 0; i < e->n; ++i) v[i] *= 1e-3;
+    });
+}
+
+int stg_efield_get_tris_v(stg_efield *e, double *v) {
+    if (!e || !v) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        std::vector<double> h(e->n);
+        EF_HIP(hipMemcpy(h.data(), e->V, e->n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int t = 0; t < e->n_tris; ++t)
+            v[t] = (h[e->tri_v[3 * t]] + h[e->tri_v[3 * t + 1]] + h[e->tri_v[3 * t + 2]]) / 3.0 * 1e-3;
+    });
+}
+
+int stg_efield_set_ohmic(stg_efield *e, int32_t n, const int32_t *sspec, const double *g,
+                         const double *erev) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (n < 0 || n > 8) throw EfError{STG_ERR_ARG, "between 0 and 8 ohmic currents"};
+        e->oc.n = n;
+        for (int k = 0; k < n; ++k) {
+            if (sspec[k] < 0) throw EfError{STG_ERR_ARG, "bad channel species index"};
+            e->oc.sspec[k] = sspec[k];
+            e->oc.g[k] = g[k];
+            e->oc.erev[k] = erev[k];
+        }
+    });
+}
+
+int stg_efield_attach(stg_efield *e, stg_sim *sim, int32_t real) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (sim) {
+            const int32_t *perm = nullptr;
+            int32_t nt = 0;
+            const int rc = stg_internal_tri_perm(sim, real, &perm, &nt);
+            if (rc != STG_OK) throw EfError{rc, "bad realization"};
+            if (nt != e->n_tris)
+                throw EfError{STG_ERR_ARG, "simulation and EField membranes differ in triangle count"};
+            std::vector<int> map(perm, perm + nt);
+            if (!e->tri_map) e->tri_map = ef_alloc<int>(std::max(nt, 1));
+            ef_put(e->tri_map, map);
+        }
+        e->sim = sim;
+        e->sim_real = real;
+    });
+}
+
+int stg_efield_advance(stg_efield *e, double dt) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] { advance_impl(e, dt); });
+}
+
+int stg_efield_run(stg_efield *e, stg_sim *sim, double endtime, double efdt) {
+    if (!e || !sim) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (!(efdt > 0.0)) throw EfError{STG_ERR_ARG, "EField dt must be positive"};
+        double t = 0.0;
+        int rc = stg_get_time(sim, &t);
+        if (rc != STG_OK) throw EfError{rc, stg_last_error(sim)};
+        if (endtime < t) throw EfError{STG_ERR_ARG, "Endtime is before current simulation time"};
+        // TetOpSplitP::_runWithEField (tetopsplit.cpp:1851-1954)
+        while (t < endtime) {
+            const double t0 = t;
+            double t1 = std::min(t0 + efdt, endtime);
+            if (std::fabs(t1 - endtime) <= DBL_EPSILON * std::fabs(t1 + endtime) * 4) t1 = endtime;
+            rc = stg_run(sim, t1);
+            if (rc != STG_OK) throw EfError{rc, stg_last_error(sim)};
+            advance_impl(e, t1 - t0);
+            t = t1;
+        }
+    });
+}
+
+int stg_efield_set_solver(stg_efield *e, int32_t precision, double tol, int32_t max_iter) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (precision != 0 && precision != 1) throw EfError{STG_ERR_ARG, "precision is 0 (mixed) or 1 (FP64)"};
+        if (!(tol > 0.0) || max_iter <= 0) throw EfError{STG_ERR_ARG, "bad tolerance or iteration cap"};
+        e->precision = precision;
+        e->tol = tol;
+        e->max_iter = max_iter;
+    });
+}
+
+int stg_efield_stats(const stg_efield *e, int32_t *iters, int32_t *refinements, double *rel_residual) {
+    if (!e || !iters || !refinements || !rel_residual) return STG_ERR_ARG;
+    *iters = e->last_iters;
+    *refinements = e->last_refine;
+    *rel_residual = e->last_rel;
+    return STG_OK;
+}
+
+int stg_efield_sizes(const stg_efield *e, int32_t *n_verts, int32_t *n_tris, int32_t *nnz) {
+    if (!e) return STG_ERR_ARG;
+    if (n_verts) *n_verts = e->n;
+    if (n_tris) *n_tris = e->n_tris;
+    if (nnz) *nnz = e->nnz;
+    return STG_OK;
+}
+
+}  // extern "C"

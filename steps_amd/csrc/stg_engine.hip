@@ -961,6 +961,29 @@ int stg_set_diff_apply_threshold(stg_sim *s, int32_t threshold) {
     });
 }
 
+// Device pointer of the triangle counts and the realization's triangle
+// offset, for the EField's membrane currents (not part of the public header).
+int stg_internal_tri_counts_device(stg_sim *s, int32_t real, const int32_t **ptr,
+                                   int32_t *n_tris_total, int32_t *tri0) {
+    if (!s || !ptr || !n_tris_total || !tri0) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        *ptr = s->d_tri_counts;
+        *n_tris_total = s->NTRI;
+        *tri0 = s->tri_off[real];
+    });
+}
+
+// User triangle order -> internal (inner-tet sorted) slot of one realization.
+int stg_internal_tri_perm(const stg_sim *s, int32_t real, const int32_t **perm, int32_t *n) {
+    if (!s || !perm || !n) return STG_ERR_ARG;
+    return guard(const_cast<stg_sim *>(s), [&] {
+        check_real(s, real);
+        *perm = s->tri_perm[real].data();
+        *n = (int32_t)s->tri_perm[real].size();
+    });
+}
+
 int stg_get_sizes(const stg_sim *s, int32_t real, int32_t *n_real, int32_t *n_tets, int32_t *n_tris) {
     if (!s) return STG_ERR_ARG;
     return guard(const_cast<stg_sim *>(s), [&] {

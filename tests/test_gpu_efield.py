@@ -1,0 +1,120 @@
+"""GPU EField against the FP64 oracle (oracle/efield_ref.py) and Rallpack1.
+
+Tolerances: the mixed-precision solve (FP32 CG, FP64 dots and refinement)
+must match the FP64 sparse-LU solution to 1e-6 relative in every step's dV;
+the FP64 CG mode to 1e-9. Rallpack1: RMS error below 1 mV against the golden
+traces, as test/validation/API_2/validation_efield/test_rallpack1.py requires.
+"""
+import numpy as np
+import pytest
+
+from oracle.efield_ref import EFieldOracle
+from steps_amd import geom, model as M
+from steps_amd.efield import EField, FP64, MIXED
+from steps_amd.engine import Engine
+from test_efield_oracle import RALL, rallpack1_setup, read_trace, sample_verts
+
+pytestmark = pytest.mark.gpu
+
+
+def _cube_case(n=6, side=3e-6):
+    mesh = geom.kuhn_cube(n, side)
+    tris = mesh.tris[mesh.getSurfTris()]
+    return mesh, tris
+
+
+@pytest.mark.parametrize("precision,tol", [(MIXED, 1e-6), (FP64, 1e-9)])
+def test_steps_match_fp64_oracle(precision, tol):
+    mesh, tris = _cube_case()
+    print(f"case efield cube n=6 precision={precision}", flush=True)
+    ef = EField(mesh.verts, mesh.tets, tris)
+    ef.set_solver(precision, 1e-11)
+    ora = EFieldOracle(mesh.verts, mesh.tets, tris)
+    for obj in (ef, ora):
+        obj.set_potential(-0.065)
+        obj.set_capac(0.01)
+        obj.set_volres(1.0)
+        obj.set_memb_res(1.0, -0.065)
+    face = np.nonzero(np.all(mesh.verts[tris][:, :, 0] < 1e-12, axis=1))[0]
+    for t in face:
+        ef.set_tri_iclamp(int(t), 1e-9 / len(face))
+        ora.set_tri_iclamp(int(t), 1e-9 / len(face))
+    v_prev = ora.V.copy() * 1e-3
+    for step in range(5):
+        ef.advance(1e-5)
+        ora.advance(1e-5)
+        gpu = ef.verts_v()
+        ref = ora.V * 1e-3
+        dv_ref = ref - v_prev
+        err = np.abs((gpu - v_prev) - dv_ref).max() / np.abs(dv_ref).max()
+        assert err < tol, (step, err, ef.stats())
+        v_prev = ref
+        ef.set_verts_v(ref)      # compare step by step from the same state
+    st = ef.stats()
+    assert st["rel_residual"] <= 1e-9
+
+
+def test_clamped_vertices_hold():
+    mesh, tris = _cube_case(n=4, side=2e-6)
+    ef = EField(mesh.verts, mesh.tets, tris)
+    ora = EFieldOracle(mesh.verts, mesh.tets, tris)
+    for v in (0, 5):
+        ef.set_vert_clamped(v)
+        ora.clamped[v] = True
+    for obj in (ef, ora):
+        obj.set_vert_iclamp(10, 5e-10)
+    for _ in range(3):
+        ef.advance(2e-5)
+        ora.advance(2e-5)
+    assert np.allclose(ef.verts_v()[[0, 5]], -0.065, atol=0, rtol=0)
+    assert np.allclose(ef.verts_v(), ora.V * 1e-3, rtol=1e-9, atol=1e-12)
+
+
+def test_rallpack1_gpu_coupled_with_ohmic_leak():
+    """Rallpack1 through the coupled loop: the leak is an ohmic current of one
+    'Leak' channel per membrane triangle held by the reaction-diffusion engine."""
+    p = RALL
+    mesh, memb, zmin_tris, zmax_tris = rallpack1_setup()
+    comp = geom.TmComp("cyto", mesh)
+    comp.addVolsys("vsys")
+    patch = geom.TmPatch("memb", mesh, memb, comp)
+    patch.addSurfsys("ssys")
+    mdl = M.Model()
+    leak = M.Spec("Leak", mdl)
+    M.Volsys("vsys", mdl)
+    ss = M.Surfsys("ssys", mdl)
+    M.SReac("hold", ss, slhs=[leak], srhs=[leak], kcst=0.0)
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    mt = geom.mesh_tables(mesh, comp, memb)
+    eng = Engine(fm, [mt], [7])
+    eng.set_tri_counts(0, np.ones(len(memb), dtype=np.int32))
+    ef = EField(mesh.verts, mesh.tets, mesh.tris[memb])
+    area_cyl = np.pi * p["diameter"] * p["length"]
+    tri_area = mt.tri_area.sum()
+    ef.set_ohmic([0], [area_cyl / p["R_M"] / len(memb)], [p["E_M"]])
+    ef.attach(eng)
+    ef.set_potential(p["E_M"])
+    ef.set_volres(p["R_A"])
+    ef.set_capac(p["C_M"] / (tri_area / area_cyl))
+    v_zmin = np.unique(mesh.tris[zmin_tris])
+    for v in v_zmin:
+        ef.set_vert_iclamp(int(v), p["Iinj"] / len(v_zmin))
+    s0, s1 = sample_verts(mesh, zmin_tris), sample_verts(mesh, zmax_tris)
+    out0, out1 = [], []
+    n_samples = 1001          # the first 50 ms of the 250 ms trace (2,000 EField steps)
+    for k in range(n_samples):
+        t = k * 5e-5
+        if k > 0:
+            ef.run(eng, t, p["EF_dt"])
+        v = ef.verts_v()
+        out0.append(v[s0].mean())
+        out1.append(v[s1].mean())
+        if k % 100 == 0:
+            print(f"rallpack1 t={t:.4f} s v0={out0[-1]:.6f} vx={out1[-1]:.6f} {ef.stats()}", flush=True)
+    v0, vx = read_trace("v0"), read_trace("vx")
+    sim0, sim1 = np.array(out0)[1:], np.array(out1)[1:]
+    n = min(len(sim0), len(v0))
+    rms0 = np.linalg.norm(v0[:n] - sim0[:n]) / np.sqrt(n)
+    rms1 = np.linalg.norm(vx[:n] - sim1[:n]) / np.sqrt(n)
+    print(f"rallpack1 rms {rms0:.3e} {rms1:.3e}", flush=True)
+    assert rms0 < 1e-3 and rms1 < 1e-3
