@@ -115,6 +115,71 @@ def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=256, graph=True):
                 max_periods=max(eng.nsteps(r) for r in range(n_real)))
 
 
+def time_membrane(w, steps=300, warmup=5):
+    """Calcium buffering with membrane pumps and influx on the 100^3 x 6 cube."""
+    p = time_plain(w, steps, warmup)
+    btet = model_bytes(w)
+    return dict(
+        workload="calcium_membrane_cube: 100^3 x 6 Kuhn cube, side 20 um, 6,000,000 tets, "
+                 "120,000 membrane triangles, Ca + Buf <-> CaB, pumps, influx on one face",
+        tet_steps_per_s=p["tet_steps"] / p["wall"], periods=p["periods"], period_s=p["dt"],
+        kernel_ms_per_launch=p["ms_launch"], bytes_per_tet_step=btet,
+        achieved_gbs=btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9,
+        frac=btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+
+
+def time_efield(w, ef_steps=10, ef_dt=1e-5):
+    """EField on the membrane cube's 1,030,301 vertices, coupled to the reaction-diffusion
+    step: passive leak 1 S/m^2 at -65 mV, 1e-2 F/m^2, 1 ohm.m, 1 nA on the x = 0 face.
+    Mixed-precision solve timed; its dV compared with the FP64 CG solution of the same step."""
+    from steps_amd.efield import EField, FP64, MIXED
+
+    mesh = w.mesh
+    surf = mesh.getSurfTris()
+    tic = time.perf_counter()
+    ef = EField(mesh.verts, mesh.tets, mesh.tris[surf])
+    setup_s = time.perf_counter() - tic
+    ef.set_potential(-0.065)
+    ef.set_capac(1e-2)
+    ef.set_volres(1.0)
+    ef.set_memb_res(1.0, -0.065)
+    vx = mesh.verts[mesh.tris[surf]][:, :, 0]
+    face = np.nonzero(np.all(np.abs(vx) < 1e-12, axis=1))[0]
+    area = w.mt.tri_area[face]
+    for t, a in zip(face, area):
+        ef.set_tri_iclamp(int(t), 1e-9 * a / area.sum())
+    eng = load_engine(w, 3)
+    ef.run(eng, ef_dt, ef_dt)                      # warm-up step
+    t0 = eng.time()
+    tic = time.perf_counter()
+    ef.run(eng, t0 + ef_steps * ef_dt, ef_dt)
+    wall = time.perf_counter() - tic
+    stats = ef.stats()
+    # precision: one more step from the same state, mixed vs FP64 CG
+    v0 = ef.verts_v()
+    ef.advance(ef_dt)
+    dv_mixed = ef.verts_v() - v0
+    it_mixed = ef.stats()
+    ef.set_verts_v(v0)
+    ef.set_solver(FP64, 1e-11, 50000)
+    tic = time.perf_counter()
+    ef.advance(ef_dt)
+    fp64_s = time.perf_counter() - tic
+    dv_64 = ef.verts_v() - v0
+    it_64 = ef.stats()
+    ef.set_solver(MIXED)
+    err = float(np.abs(dv_mixed - dv_64).max() / np.abs(dv_64).max())
+    nv, nt, nnz = ef.sizes()
+    return dict(
+        workload="EField on calcium_membrane_cube: 1,030,301 vertices, 120,000 membrane "
+                 "triangles, 10 us steps coupled to the reaction-diffusion period",
+        n_verts=nv, nnz=nnz, setup_s=setup_s, ms_per_ef_step=wall * 1e3 / ef_steps,
+        ef_steps=ef_steps, cg_iterations_last=stats["iterations"],
+        refinements_last=stats["refinements"], rel_residual_last=stats["rel_residual"],
+        mixed_vs_fp64_dv_rel_err=err, mixed_iterations=it_mixed["iterations"],
+        fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3)
+
+
 def cpu_baseline(w, budget_s=15.0, seed=1):
     """The CPU oracle on this host, a bounded number of periods of the same workload."""
     from oracle import bind
@@ -147,6 +212,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--membrane", action="store_true",
+                    help="also time the 6,000,000-tet membrane cube and its EField")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -224,6 +291,10 @@ def main():
         }
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])
+    if args.membrane:
+        wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0)
+        out["membrane"] = time_membrane(wm)
+        out["efield"] = time_efield(wm)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
     print(json.dumps(out))

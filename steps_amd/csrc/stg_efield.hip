@@ -15,16 +15,15 @@
 // in FP64) until the FP64 residual meets the tolerance. An all-FP64 CG mode
 // gives the reference solution on the device.
 //
-// Reductions are deterministic: each block writes its partial, the last block
-// to finish (device-scope counter) sums the partials in block order and
-// resets the counter, so no memset is needed between launches.
+// Reductions are deterministic and atomic-free: each block writes its partial,
+// then one workgroup sums the partials in a fixed order and updates the CG
+// scalars on the device, so an iteration needs no host round trip.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <array>
 #include <cfloat>
 #include <cmath>
-#include <map>
 #include <string>
 #include <vector>
 
@@ -60,12 +59,9 @@ constexpr int kB = 256;
 
 // ------------------------------------------------------------- reductions
 
-// Sum of one double per thread into partial[blockIdx]; the last block sums the
-// partials in order into out[0] and resets the counter.
-__device__ __forceinline__ void block_sum_last(double v, double *partial, unsigned *counter,
-                                               double *out) {
+// Sum of one double per thread of the block into partial[blockIdx.x].
+__device__ __forceinline__ void block_partial(double v, double *partial) {
     __shared__ double red[kB / 64];
-    __shared__ bool last;
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     if (l == 0) red[w] = v;
@@ -74,37 +70,52 @@ __device__ __forceinline__ void block_sum_last(double v, double *partial, unsign
         double s = 0.0;
         for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) s += red[i];
         partial[blockIdx.x] = s;
-        __threadfence();
-        const unsigned done = atomicAdd(counter, 1u);
-        last = (done == gridDim.x - 1);
     }
+}
+
+// One workgroup sums the nb partials in a fixed order and applies `op`:
+//   0: scal[0] = sum, scal[1] = 0            (CG start: rz, beta)
+//   1: scal[3] = sum                          (p.q of this iteration)
+//   2: scal[1] = sum / scal[0], scal[0] = sum (new rz and beta)
+//   3: out[0] = sum                           (a plain reduction)
+__global__ void __launch_bounds__(1024) k_reduce(const double *partial, int nb, int op, double *scal,
+                                                 double *out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += partial[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) red[w] = s;
     __syncthreads();
-    if (last) {
-        __threadfence();
-        double s = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += blockDim.x) s += partial[i];
-        // fixed-order combine: strided per-thread sums, then lanes, then waves
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (l == 0) red[w] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) t += red[i];
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int i = 0; i < (int)(blockDim.x + 63) / 64; ++i) t += red[i];
+        if (op == 0) {
+            scal[0] = t;
+            scal[1] = 0.0;
+        } else if (op == 1) {
+            scal[3] = t;
+        } else if (op == 2) {
+            scal[1] = scal[0] > 0.0 ? t / scal[0] : 0.0;
+            scal[0] = t;
+        } else {
             out[0] = t;
-            *counter = 0u;
         }
     }
 }
 
 // ------------------------------------------------------------- kernels
 
+// The off-diagonal couplings are stored ELLPACK-style, slot-major
+// ([width][n], rows padded with column i and coupling 0), so the 64 rows of a
+// wavefront read each slot with one coalesced load.
 struct EfDev {
     int n;                 // vertices
-    const int *rp;         // [n+1]
-    const int *col;        // [nnz] off-diagonal columns
-    const double *w;       // [nnz] geometric coupling (um)
+    int width;             // slots per row
+    const int *col;        // [width][n] off-diagonal columns
+    const double *w;       // [width][n] geometric coupling (um)
     const double *wsum;    // [n]   row sums of w
-    const float *a32;      // [nnz] -c_ij in FP32 (0 for clamped rows/cols)
+    const float *a32;      // [width][n] -c_ij in FP32 (0 for clamped rows/cols)
     const float *d32;      // [n]   diagonal in FP32
     const double *d64;     // [n]   diagonal in FP64
     const unsigned char *clamped;
@@ -157,7 +168,10 @@ __global__ void k_assemble(EfDev m, double sigma, double oodt, const double *cap
     }
     const double vi = V[i];
     double rhs = I[i] + gext[i] * (vext - vi);
-    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) rhs += sigma * m.w[e] * (V[m.col[e]] - vi);
+    for (int k = 0; k < m.width; ++k) {
+        const int e = k * m.n + i;
+        rhs += sigma * m.w[e] * (V[m.col[e]] - vi);
+    }
     b[i] = rhs;
     const double d = cap[i] * oodt + gext[i] + sigma * m.wsum[i];
     d64[i] = d;
@@ -167,7 +181,8 @@ __global__ void k_assemble(EfDev m, double sigma, double oodt, const double *cap
 __global__ void k_offdiag32(EfDev m, double sigma, float *a32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
-    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+    for (int k = 0; k < m.width; ++k) {
+        const int e = k * m.n + i;
         const bool off = m.clamped[i] || m.clamped[m.col[e]];
         a32[e] = off ? 0.0f : (float)(-sigma * m.w[e]);
     }
@@ -175,13 +190,14 @@ __global__ void k_offdiag32(EfDev m, double sigma, float *a32) {
 
 // r = b - A x in FP64, and ||r||^2, ||b||^2
 __global__ void k_residual64(EfDev m, double sigma, const double *b, const double *x, double *r,
-                             double *part_r, unsigned *cnt_r, double *out_r) {
+                             double *part_r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0;
     if (i < m.n) {
         double ax = m.d64[i] * x[i];
         if (!m.clamped[i]) {
-            for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+            for (int k = 0; k < m.width; ++k) {
+                const int e = k * m.n + i;
                 const int j = m.col[e];
                 if (!m.clamped[j]) ax -= sigma * m.w[e] * x[j];
             }
@@ -190,7 +206,7 @@ __global__ void k_residual64(EfDev m, double sigma, const double *b, const doubl
         r[i] = ri;
         rr = ri * ri;
     }
-    block_sum_last(rr, part_r, cnt_r, out_r);
+    block_partial(rr, part_r);
 }
 
 // PCG iteration, kernel A: p_new = z + beta p_old; q = A p_new; partial (p_new, q).
@@ -198,7 +214,7 @@ __global__ void k_residual64(EfDev m, double sigma, const double *b, const doubl
 // is only read and p_new only written (ping-pong buffers).
 template <typename T>
 __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *q, const double *scal,
-                       int first, double *part, unsigned *cnt, double *out) {
+                       int first, double *part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double pq = 0.0;
     if (i < m.n) {
@@ -208,7 +224,8 @@ __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *
         T qi;
         if constexpr (sizeof(T) == 4) {
             float acc = m.d32[i] * (float)pi;
-            for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+            for (int k = 0; k < m.width; ++k) {
+                const int e = k * m.n + i;
                 const int j = m.col[e];
                 const float pj = first ? (float)z[j] : (float)((double)z[j] + beta * (double)p[j]);
                 acc += m.a32[e] * pj;
@@ -217,7 +234,8 @@ __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *
         } else {
             double acc = m.d64[i] * (double)pi;
             if (!m.clamped[i]) {
-                for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) {
+                for (int k = 0; k < m.width; ++k) {
+                    const int e = k * m.n + i;
                     const int j = m.col[e];
                     if (m.clamped[j]) continue;
                     const double pj = first ? (double)z[j] : (double)z[j] + beta * (double)p[j];
@@ -230,17 +248,17 @@ __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *
         pn[i] = pi;
         pq = (double)pi * (double)qi;
     }
-    block_sum_last(pq, part, cnt, out);
+    block_partial(pq, part);
 }
 
 // kernel B: alpha = rz / pq; x += alpha p; r -= alpha q; z = r / diag; partial (r, z)
 template <typename T>
-__global__ void k_cg_b(EfDev m, const double *scal, const double *pq_out, T *x, T *r, T *z,
-                       const T *pnew, const T *q, double *part, unsigned *cnt, double *out) {
+__global__ void k_cg_b(EfDev m, const double *scal, T *x, T *r, T *z, const T *pnew, const T *q,
+                       double *part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0;
     if (i < m.n) {
-        const double alpha = scal[0] / pq_out[0];
+        const double alpha = scal[0] / scal[3];
         const double xi = (double)x[i] + alpha * (double)pnew[i];
         const double ri = (double)r[i] - alpha * (double)q[i];
         const double dg = sizeof(T) == 4 ? (double)m.d32[i] : m.d64[i];
@@ -250,22 +268,12 @@ __global__ void k_cg_b(EfDev m, const double *scal, const double *pq_out, T *x, 
         z[i] = zi;
         rz = (double)r[i] * (double)zi;
     }
-    block_sum_last(rz, part, cnt, out);
-}
-
-// scal[0] = rz_new, scal[1] = beta = rz_new / rz_old; scal[2] keeps rz_old
-__global__ void k_cg_scalars(double *scal, const double *rz_out, int first) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
-        const double rz = rz_out[0];
-        scal[1] = first ? 0.0 : (scal[0] > 0.0 ? rz / scal[0] : 0.0);
-        scal[0] = rz;
-    }
+    block_partial(rz, part);
 }
 
 // initial: r = residual (as T), z = r / diag, partial (r, z)
 template <typename T>
-__global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *part, unsigned *cnt,
-                          double *out) {
+__global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rz = 0.0;
     if (i < m.n) {
@@ -277,7 +285,7 @@ __global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *
         z[i] = zi;
         rz = (double)ri * (double)zi;
     }
-    block_sum_last(rz, part, cnt, out);
+    block_partial(rz, part);
 }
 
 // Sum over the block of one double per thread; every thread gets the total.
@@ -326,13 +334,18 @@ __global__ void __launch_bounds__(1024) k_cg_small(EfDev m, double sigma, const 
             double qi;
             if (f32) {
                 float acc = m.d32[i] * (float)p[i];
-                for (int e = m.rp[i]; e < m.rp[i + 1]; ++e) acc += m.a32[e] * (float)p[m.col[e]];
+                for (int k = 0; k < m.width; ++k) {
+                    const int e = k * n + i;
+                    acc += m.a32[e] * (float)p[m.col[e]];
+                }
                 qi = acc;
             } else {
                 double acc = m.d64[i] * (double)p[i];
                 if (!m.clamped[i])
-                    for (int e = m.rp[i]; e < m.rp[i + 1]; ++e)
+                    for (int k = 0; k < m.width; ++k) {
+                        const int e = k * n + i;
                         if (!m.clamped[m.col[e]]) acc -= sigma * m.w[e] * (double)p[m.col[e]];
+                    }
                 qi = acc;
             }
             q[i] = (T)qi;
@@ -371,16 +384,16 @@ __global__ void k_apply_dv(int n, const unsigned char *clamped, const double *dv
     if (i < n && !clamped[i]) V[i] += dv[i];
 }
 
-__global__ void k_sumsq(int n, const double *v, double *part, unsigned *cnt, double *out) {
+__global__ void k_sumsq(int n, const double *v, double *part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const double x = i < n ? v[i] : 0.0;
-    block_sum_last(x * x, part, cnt, out);
+    block_partial(x * x, part);
 }
 
 }  // namespace
 
 struct stg_efield {
-    int n = 0, n_tris = 0, nnz = 0;
+    int n = 0, n_tris = 0, nnz = 0, width = 0;
     std::vector<int> tri_v;                 // [n_tris*3]
     std::vector<double> vert_area, tri_area, cap_h, gext_h;
     std::vector<double> vert_clamp_h, tri_clamp_h;
@@ -389,9 +402,9 @@ struct stg_efield {
     double vext = 0.0;
     double cm = 0.01;
     int precision = 0;                      // 0 = FP32 CG + FP64 refinement, 1 = FP64 CG
-    double tol = 1e-11;                     // relative FP64 residual target
+    double tol = 1e-8;                      // relative FP64 residual target
     double floor_tol = 1e-9;                // accepted when refinement stagnates above tol
-    double inner_tol = 1e-5;                // relative preconditioned residual per FP32 solve
+    double inner_tol = 1e-6;                // relative preconditioned residual per FP32 solve
     int inner_max = 4000;                   // FP32 iterations per refinement
     int max_iter = 20000;
     int last_iters = 0, last_refine = 0;
@@ -402,7 +415,7 @@ struct stg_efield {
     std::string err;
     hipStream_t stream = nullptr;
     // device
-    int *rp = nullptr, *col = nullptr, *vt_ptr = nullptr, *vt_tri = nullptr, *tri_vd = nullptr;
+    int *col = nullptr, *vt_ptr = nullptr, *vt_tri = nullptr, *tri_vd = nullptr;
     int *tri_map = nullptr;  // EField triangle -> the attached simulation's triangle slot
     double *w = nullptr, *wsum = nullptr, *d64 = nullptr, *V = nullptr, *I = nullptr, *b = nullptr;
     double *cap = nullptr, *gext = nullptr, *vclamp = nullptr, *tclamp = nullptr;
@@ -416,11 +429,11 @@ struct stg_efield {
     int nb = 1;
 
     EfDev dev() const {
-        return EfDev{n, rp, col, w, wsum, a32, d32, d64, clamped};
+        return EfDev{n, width, col, w, wsum, a32, d32, d64, clamped};
     }
 
     ~stg_efield() {
-        void *ptrs[] = {rp, col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
+        void *ptrs[] = {col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
                         tclamp, x64, r64, dv_prev, a32, d32, clamped, xs, rs, zs, ps, qs, part,
                         scal, outs, cnt};
         for (void *p : ptrs)
@@ -475,8 +488,18 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     const int n = m->n_verts;
     e->n = n;
     e->n_tris = m->n_tris;
-    // P1 stiffness: w_ij = -sum_tets V grad(phi_i).grad(phi_j), coordinates in microns
-    std::vector<std::map<int, double>> rows(n);
+    // P1 stiffness: w_ij = -sum_tets V grad(phi_i).grad(phi_j), coordinates in microns.
+    // Rows are short (a vertex touches a few tens of others): linear search.
+    std::vector<std::vector<std::pair<int, double>>> rows(n);
+    auto add = [&rows](int i, int j, double v) {
+        auto &row = rows[i];
+        for (auto &kv : row)
+            if (kv.first == j) {
+                kv.second += v;
+                return;
+            }
+        row.emplace_back(j, v);
+    };
     for (int t = 0; t < m->n_tets; ++t) {
         int v[4];
         double P[4][3];
@@ -513,21 +536,28 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
             for (int c = 0; c < 4; ++c) {
                 if (a == c) continue;
                 const double kij = vol * (g[a][0] * g[c][0] + g[a][1] * g[c][1] + g[a][2] * g[c][2]);
-                rows[v[a]][v[c]] -= kij;
+                add(v[a], v[c], -kij);
             }
     }
-    std::vector<int> rp(n + 1, 0), col;
-    std::vector<double> w, wsum(n, 0.0);
+    size_t width = 1, nnz = 0;
     for (int i = 0; i < n; ++i) {
-        for (auto &kv : rows[i]) {
-            col.push_back(kv.first);
-            w.push_back(kv.second);
-            wsum[i] += kv.second;
-        }
-        rp[i + 1] = (int)col.size();
-        rows[i].clear();
+        width = std::max(width, rows[i].size());
+        nnz += rows[i].size();
     }
-    e->nnz = (int)col.size();
+    std::vector<int> col(width * (size_t)n);
+    std::vector<double> w(width * (size_t)n, 0.0), wsum(n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        std::sort(rows[i].begin(), rows[i].end());
+        for (size_t k = 0; k < width; ++k) {
+            const bool have = k < rows[i].size();
+            col[k * n + i] = have ? rows[i][k].first : i;
+            w[k * n + i] = have ? rows[i][k].second : 0.0;
+            if (have) wsum[i] += rows[i][k].second;
+        }
+        std::vector<std::pair<int, double>>().swap(rows[i]);
+    }
+    e->nnz = (int)nnz;
+    e->width = (int)width;
     // membrane: triangle areas, one third to each vertex (tetmesh.cpp:231-260)
     e->tri_v.assign(m->tris, m->tris + 3 * (size_t)m->n_tris);
     e->tri_area.resize(m->n_tris);
@@ -566,11 +596,10 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
 
     EF_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     e->nb = (n + kB - 1) / kB;
-    e->rp = ef_alloc<int>(n + 1);
-    e->col = ef_alloc<int>(e->nnz);
-    e->w = ef_alloc<double>(e->nnz);
+    e->col = ef_alloc<int>(col.size());
+    e->w = ef_alloc<double>(w.size());
     e->wsum = ef_alloc<double>(n);
-    e->a32 = ef_alloc<float>(e->nnz);
+    e->a32 = ef_alloc<float>(w.size());
     e->d32 = ef_alloc<float>(n);
     e->d64 = ef_alloc<double>(n);
     e->V = ef_alloc<double>(n);
@@ -596,7 +625,6 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     e->vt_ptr = ef_alloc<int>(n + 1);
     e->vt_tri = ef_alloc<int>(std::max<size_t>(vt_tri.size(), 1));
     e->tri_vd = ef_alloc<int>(std::max<size_t>(e->tri_v.size(), 1));
-    ef_put(e->rp, rp);
     ef_put(e->col, col);
     ef_put(e->w, w);
     ef_put(e->wsum, wsum);
@@ -611,6 +639,13 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
 }
 
 // Jacobi PCG on A d = r (r given in r64), result added to x64. Returns iterations.
+double fetch(stg_efield *e, const double *d) {
+    double h = 0.0;
+    EF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    EF_HIP(hipStreamSynchronize(e->stream));
+    return h;
+}
+
 constexpr int kSmallSystem = 16384;  // solved by one persistent workgroup
 
 template <typename T>
@@ -626,13 +661,9 @@ int pcg(stg_efield *e, double rel_tol, int max_iter) {
         EF_HIP(hipStreamSynchronize(e->stream));
         return it;
     }
-    double *out_rz = e->outs + 0, *out_pq = e->outs + 1;
-    hipLaunchKernelGGL(k_cg_init<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->r64, x, r, z, e->part,
-                       e->cnt + 0, out_rz);
-    hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, e->stream, e->scal, out_rz, 1);
-    double rz0 = 0.0;
-    EF_HIP(hipMemcpyAsync(&rz0, out_rz, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    EF_HIP(hipStreamSynchronize(e->stream));
+    hipLaunchKernelGGL(k_cg_init<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->r64, x, r, z, e->part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 0, e->scal, e->outs);
+    const double rz0 = fetch(e, e->scal);
     int it = 0;
     if (!(rz0 > 0.0)) return 0;
     const int check = 16;
@@ -640,26 +671,18 @@ int pcg(stg_efield *e, double rel_tol, int max_iter) {
         for (int k = 0; k < check && it < max_iter; ++k, ++it) {
             T *pold = p + (size_t)(it & 1) * n, *pnew = p + (size_t)((it + 1) & 1) * n;
             hipLaunchKernelGGL(k_cg_a<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, z, pold, pnew, q,
-                               e->scal, it == 0 ? 1 : 0, e->part, e->cnt + 1, out_pq);
-            hipLaunchKernelGGL(k_cg_b<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->scal, out_pq, x, r, z,
-                               pnew, q, e->part, e->cnt + 2, out_rz);
-            hipLaunchKernelGGL(k_cg_scalars, dim3(1), dim3(64), 0, e->stream, e->scal, out_rz, 0);
+                               e->scal, it == 0 ? 1 : 0, e->part);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 1, e->scal, e->outs);
+            hipLaunchKernelGGL(k_cg_b<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->scal, x, r, z, pnew, q,
+                               e->part);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 2, e->scal, e->outs);
         }
-        double rz = 0.0;
-        EF_HIP(hipMemcpyAsync(&rz, e->scal, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-        EF_HIP(hipStreamSynchronize(e->stream));
+        const double rz = fetch(e, e->scal);
         if (!(rz > rel_tol * rel_tol * rz0)) break;
     }
     EF_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_accumulate<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->x64, x);
     return it;
-}
-
-double fetch(stg_efield *e, const double *d) {
-    double h = 0.0;
-    EF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-    EF_HIP(hipStreamSynchronize(e->stream));
-    return h;
 }
 
 void advance_impl(stg_efield *e, double dt_s) {
@@ -684,7 +707,8 @@ void advance_impl(stg_efield *e, double dt_s) {
     }
     // initial guess: the previous step's increment
     EF_HIP(hipMemcpyAsync(e->x64, e->dv_prev, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kB), 0, e->stream, n, e->b, e->part, e->cnt + 3, e->outs + 3);
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kB), 0, e->stream, n, e->b, e->part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal, e->outs + 3);
     const double bb = fetch(e, e->outs + 3);
     int iters = 0, refine = 0;
     double rel = 0.0, prev = INFINITY;
@@ -692,7 +716,9 @@ void advance_impl(stg_efield *e, double dt_s) {
     if (bb > 0.0) {
         for (refine = 0;; ++refine) {
             hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, e->b, e->x64,
-                               e->r64, e->part, e->cnt + 4, e->outs + 4);
+                               e->r64, e->part);
+            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal,
+                               e->outs + 4);
             const double rr = fetch(e, e->outs + 4);
             rel = std::sqrt(rr / bb);
             if (rel <= e->tol) break;
@@ -708,7 +734,9 @@ void advance_impl(stg_efield *e, double dt_s) {
             }
             prev = rel;
             if (e->precision == 1) iters += pcg<double>(e, std::max(e->tol / rel, 1e-15), e->max_iter);
-            else iters += pcg<float>(e, e->inner_tol, std::min(e->inner_max, e->max_iter));
+            else  // the inner solve only needs to close the remaining gap to tol
+                iters += pcg<float>(e, std::max(e->inner_tol, 0.1 * e->tol / rel),
+                                    std::min(e->inner_max, e->max_iter));
         }
     } else {
         EF_HIP(hipMemsetAsync(e->x64, 0, n * sizeof(double), e->stream));

@@ -94,7 +94,7 @@ class EField:
     def run(self, engine, endtime, efdt):
         self._chk(self.lib.stg_efield_run(self.h, engine.h, float(endtime), float(efdt)))
 
-    def set_solver(self, precision=MIXED, tol=1e-11, max_iter=20000):
+    def set_solver(self, precision=MIXED, tol=1e-8, max_iter=20000):
         self._chk(self.lib.stg_efield_set_solver(self.h, int(precision), float(tol), int(max_iter)))
 
     def stats(self):
