@@ -58,6 +58,21 @@ def model_bytes(w) -> float:
     return bytes_per_tet_step(S, S_d, S_o, interior_faces_per_tet(w.mt), T / N, len(fm.sspecs))
 
 
+TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_period_four_species_cube_384k_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the period kernel on the default workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration)."""
+    try:
+        for line in open(TRAFFIC_SUMMARY):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def load_engine(w, seed, block=256, graph=False):
     eng = Engine(w.fm, [w.mt], [seed])
     for s in range(len(w.fm.specs)):
@@ -267,7 +282,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+            "algorithmic_bytes_per_launch": btet * w.n_tets,
             "bytes_per_tet_step": btet,
             "kernel_ms_per_launch": p["ms_launch"],
         },

@@ -139,7 +139,7 @@ struct stg_sim {
     size_t smem_bytes(int B) const {
         const int K = R + max_tris * RS;
         size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
-        return off + (size_t)S * B * 8 + (size_t)(K > 0 ? K : 1) * B * 8;
+        return off + (size_t)S * B * 8 + 2 * (size_t)(K > 0 ? K : 1) * B * 8;
     }
 
     double compute_dt_std(int r) const {
@@ -512,6 +512,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.xcd_remap = s->xcd_remap ? 1 : 0;
     k.n_blocks = (int32_t)s->block_real.size();
     k.diff_threshold = s->diff_threshold;
+    k.k_max = std::max(1, s->R + s->max_tris * s->RS);
     k.vol = s->d_vol;
     k.nbr = s->d_nbr;
     k.nface = s->d_nface;

@@ -58,6 +58,7 @@ struct SysK {
     int32_t xcd_remap;     // nonzero: contiguous block ranges per XCD
     int32_t n_blocks;
     int32_t diff_threshold;      // leaving counts up to this are split per molecule
+    int32_t k_max;               // most reactions of one tet group (LDS rows per thread)
     const double *vol;           // [NT]
     const int4 *nbr;             // [NT] global neighbour index, -1 = none
     const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
@@ -101,15 +102,18 @@ __device__ __forceinline__ double falling(double h, int32_t n, int32_t L) {
 struct Lds {
     int32_t *cnt;   // [S][B]
     double *corr;   // [S][B]
-    double *a;      // [K][B]
+    double *a;      // [K][B] propensities
+    double *kc;     // [K][B] scaled rate constants of this tet's reactions
     int B, tid;
     __device__ __forceinline__ int32_t &c(int s) const { return cnt[s * B + tid]; }
     __device__ __forceinline__ double &o(int s) const { return corr[s * B + tid]; }
     __device__ __forceinline__ double &p(int j) const { return a[j * B + tid]; }
+    __device__ __forceinline__ double &c(int j, int) const { return kc[j * B + tid]; }
 };
 
-__device__ __forceinline__ double surf_prop(const ModelK &m, const SysK &sys, const Lds &L,
-                                            int r, int tri, double k, double vscale) {
+// combinatorial factor h of a surface reaction on triangle tri (0 if a
+// reactant is short)
+__device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const Lds &L, int r, int tri) {
     double h = 1.0;
     for (int e = m.slhs_off[r]; e < m.slhs_off[r + 1]; ++e) {
         const int sp = m.ent_spec[e];
@@ -120,8 +124,7 @@ __device__ __forceinline__ double surf_prop(const ModelK &m, const SysK &sys, co
         if (mult > n) return 0.0;
         h = falling(h, n, mult);
     }
-    const double scale = m.s_nvol[r] == 0 ? sys.tri_area[tri] * kAvogadro : vscale;
-    return h * scaled_const(k, m.sorder[r], scale);
+    return h;
 }
 
 // Direct-method SSA of one tet group over [0, dt). Returns events, or -1 when
@@ -139,6 +142,16 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
     const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
     const double *kr = sys.real_k + (size_t)r * R;
     const double *ksr = sys.real_sk + (size_t)r * RS;
+    // rate constants scaled by volume / area, formed once per period
+    {
+        int j = 0;
+        for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale);
+        for (int i = 0; i < ntri; ++i) {
+            const double ascale = sys.tri_area[tp0 + i] * kAvogadro;
+            for (int q = 0; q < RS; ++q, ++j)
+                L.c(j, 0) = scaled_const(ksr[q], m.sorder[q], m.s_nvol[q] == 0 ? ascale : vscale);
+        }
+    }
     Philox g;
     g.init(sys.real_seed[r], ltet, period, sys.real_stream[r], 0u);
     long nev = 0;
@@ -159,13 +172,14 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                 }
                 h = falling(h, n, mult);
             }
-            const double av = zero ? 0.0 : h * scaled_const(kr[q], m.order[q], vscale);
+            const double av = zero ? 0.0 : h * L.c(j, 0);
             L.p(j) = av;
             a0 = a0 + av;
         }
         for (int i = 0; i < ntri; ++i) {
             for (int q = 0; q < RS; ++q, ++j) {
-                const double av = surf_prop(m, sys, L, q, tp0 + i, ksr[q], vscale);
+                const double h = surf_h(m, sys, L, q, tp0 + i);
+                const double av = h == 0.0 ? 0.0 : h * L.c(j, 0);
                 L.p(j) = av;
                 a0 = a0 + av;
             }
@@ -299,8 +313,13 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     return nl;
 }
 
+#ifndef STG_PERIOD_WAVES_PER_EU
+#define STG_PERIOD_WAVES_PER_EU 1
+#endif
+
 template <int MAXB>
-__global__ void __launch_bounds__(MAXB) k_period(const ModelK m, const SysK sys, int launch) {
+__global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
+    k_period(const ModelK m, const SysK sys, int launch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int B = blockDim.x;
     const int tid = threadIdx.x;
@@ -328,6 +347,7 @@ __global__ void __launch_bounds__(MAXB) k_period(const ModelK m, const SysK sys,
     L.cnt = reinterpret_cast<int32_t *>(smem);
     L.corr = reinterpret_cast<double *>(smem + (((size_t)S * B * 4 + 15) & ~(size_t)15));
     L.a = L.corr + (size_t)S * B;
+    L.kc = L.a + (size_t)sys.k_max * B;
 
     const int t = sys.block_tet0[b] + tid;
     const int tend = sys.real_tet0[r + 1];
