@@ -70,6 +70,17 @@ typedef struct stg_model {
     const int32_t *sreac_lhs_i;  /* [n_sreacs*n_specs] inner-compartment reactants */
     const int32_t *sreac_upd_i;  /* [n_sreacs*n_specs] */
     const double *sreac_k;       /* [n_sreacs] */
+    /* Voltage-dependent surface reactions (VDepSReac, src/steps/mpi/tetopsplit/
+     * vdepsreac.cpp:217-320; VDepSReacdef::getVDepK, src/steps/solver/
+     * vdepsreacdef.cpp:194-217): surface reaction r takes its rate constant from
+     * table sreac_vdep[r] (-1: the constant sreac_k[r]), linearly interpolated
+     * at the triangle's potential. NULL / 0 when the model has none. */
+    const int32_t *sreac_vdep;   /* [n_sreacs] */
+    int32_t n_vdep;
+    const double *vdep_vmin;     /* [n_vdep] V */
+    const double *vdep_dv;       /* [n_vdep] V */
+    const int32_t *vdep_off;     /* [n_vdep+1] offsets into vdep_k */
+    const double *vdep_k;        /* tables of k(vmin + i dv), SI */
 } stg_model;
 
 /* Per-realization parameters of a batch; NULL arrays take the model's values. */
@@ -164,6 +175,21 @@ int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
  * over faces one molecule at a time, larger ones by conditional binomials.
  * Both give the same multinomial distribution. */
 int stg_set_diff_apply_threshold(stg_sim *sim, int32_t threshold);
+
+/* TetOpSplitP::checkpoint / restore (tetopsplit.cpp:217-453), own format:
+ * time, per-realization period counters, seeds, extents and constants, and all
+ * molecule counts. Because every random draw is keyed by (seed, tet, period),
+ * a restored simulation continues bit-for-bit as the original would have.
+ * stg_restore requires a simulation of the same shape. */
+int stg_state_size(const stg_sim *sim, size_t *bytes);
+int stg_checkpoint(stg_sim *sim, void *buf, size_t bytes);
+int stg_restore(stg_sim *sim, const void *buf, size_t bytes);
+
+/* Membrane potential of every triangle of one realization (volts), used by
+ * voltage-dependent surface reactions; TetOpSplitP::setTriV. An attached
+ * EField (stg_efield_attach) writes it after every step. Default -65 mV. */
+int stg_set_tri_v(stg_sim *sim, int32_t real, const double *v);
+int stg_get_tri_v(stg_sim *sim, int32_t real, double *v);
 
 /* Number of realizations and tets / triangles of one realization. */
 int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_tets,

@@ -29,6 +29,8 @@ class StgoSystem(C.Structure):
         ("spec_dcst", _dp), ("sreac_lhs_s", _ip), ("sreac_upd_s", _ip), ("sreac_lhs_i", _ip),
         ("sreac_upd_i", _ip), ("sreac_k", _dp), ("counts", _ip), ("tri_counts", _ip),
         ("seed", C.c_uint64), ("stream", C.c_uint32), ("diff_threshold", C.c_int32),
+        ("sreac_vdep", _ip), ("vdep_vmin", _dp), ("vdep_dv", _dp), ("vdep_off", _ip),
+        ("vdep_k", _dp), ("tri_v", _dp),
     ]
 
 
@@ -130,6 +132,12 @@ class OracleSim:
             sui=i32(fm.sreac_upd_i), sk=f64(fm.sreac_k if sreac_k is None else sreac_k),
             counts=np.zeros((S, self.n_tets), dtype=np.int32),
             tcounts=np.zeros((max(SS, 1), max(self.n_tris, 1)), dtype=np.int32),
+            svd=i32(fm.sreac_vdep if fm.sreac_vdep is not None else np.full(len(fm.sreacs), -1)),
+            vmin=f64(fm.vdep_vmin if fm.vdep_vmin is not None else []),
+            vdv=f64(fm.vdep_dv if fm.vdep_dv is not None else []),
+            voff=i32(fm.vdep_off if fm.vdep_off is not None else [0]),
+            vk=f64(fm.vdep_k if fm.vdep_k is not None else []),
+            triv=np.full(max(self.n_tris, 1), -65.0e-3),
         )
         a = self.a
         P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
@@ -140,7 +148,9 @@ class OracleSim:
             P(a["rk"], C.c_double), P(a["dc"], C.c_double), P(a["sls"], C.c_int32),
             P(a["sus"], C.c_int32), P(a["sli"], C.c_int32), P(a["sui"], C.c_int32),
             P(a["sk"], C.c_double), P(a["counts"], C.c_int32), P(a["tcounts"], C.c_int32),
-            seed, stream, diff_threshold)
+            seed, stream, diff_threshold, P(a["svd"], C.c_int32), P(a["vmin"], C.c_double),
+            P(a["vdv"], C.c_double), P(a["voff"], C.c_int32), P(a["vk"], C.c_double),
+            P(a["triv"], C.c_double))
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         self.time = 0.0
@@ -150,6 +160,11 @@ class OracleSim:
     @property
     def counts(self) -> np.ndarray:
         return self.a["counts"]
+
+    @property
+    def tri_v(self) -> np.ndarray:
+        """Membrane potential of each triangle (V), read by k(V) reactions."""
+        return self.a["triv"]
 
     @property
     def tri_counts(self) -> np.ndarray:

@@ -339,6 +339,21 @@ static double scaled_const(double k, int32_t order, double scale) {
     return k / ipow(scale, o1);
 }
 
+/* k(V) by linear interpolation in the tabulated rate constant:
+ * vdepsreacdef.cpp:179-205 (getVDepK). Returns -1 when V lies outside the
+ * table, which the reference reports as a ProgErr. */
+static double vdep_const(const stgo_system *s, int32_t tab, double v) {
+    const double vmin = s->vdep_vmin[tab], dv = s->vdep_dv[tab];
+    const int32_t n = s->vdep_off[tab + 1] - s->vdep_off[tab];
+    const double *K = s->vdep_k + s->vdep_off[tab];
+    const double v2 = (v - vmin) / dv;
+    if (!(v2 >= 0.0) || !(v2 <= (double)(n - 1))) return -1.0;
+    const double lv = floor(v2);
+    const int32_t l = (int32_t)lv, u = (int32_t)ceil(v2);
+    const double r = v2 - lv;
+    return (1.0 - r) * K[l] + r * K[u];
+}
+
 /* h = prod over reactants of n(n-1)...(n-lhs+1): reactions.cpp:66-109 */
 static double falling(double h, int32_t n, int32_t lhs) {
     for (int32_t i = lhs - 1; i >= 0; --i) h = h * (double)(n - i);
@@ -444,7 +459,12 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                     a[j] = 0.0;
                 } else {
                     double scale = nvol == 0 ? s->tri_area[tr] * AVOGADRO : vscale;
-                    a[j] = h * scaled_const(s->sreac_k[r], order, scale);
+                    double k = s->sreac_k[r];
+                    if (s->sreac_vdep && s->sreac_vdep[r] >= 0) {
+                        k = vdep_const(s, s->sreac_vdep[r], s->tri_v[tr]);
+                        if (k < 0.0) return -2;
+                    }
+                    a[j] = h * scaled_const(k, order, scale);
                 }
             }
         }
@@ -601,7 +621,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                 }
                 long nev = ssa_tet(s, &tp, t, dt, step, cnt, corr, a, ps);
                 if (nev < 0) {
-                    err |= 1;
+                    err |= nev == -2 ? 2 : 1;
                     nev = 0;
                 }
                 ev_total += (uint64_t)nev;
@@ -638,5 +658,6 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
     topo_free(&tp);
     if (n_events) *n_events += ev_total;
     if (n_moved) *n_moved += mv_total;
-    return err ? -1 : 0;
+    /* -1: event cap hit; -2: a triangle potential outside a k(V) table */
+    return (err & 1) ? -1 : (err & 2) ? -2 : 0;
 }

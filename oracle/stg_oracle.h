@@ -48,6 +48,15 @@ typedef struct stgo_system {
     /* leaving counts up to this are split one molecule at a time
      * (diffusion_operator.hpp:99 default 10, setDiffApplyThreshold) */
     int32_t diff_threshold;
+    /* voltage-dependent surface reactions (may all be null / 0): sreac_vdep[r]
+     * is the k(V) table of sreac r or -1; table t spans vdep_k[vdep_off[t] ..
+     * vdep_off[t+1]) at V = vdep_vmin[t] + i*vdep_dv[t]; tri_v [n_tris] volts */
+    const int32_t *sreac_vdep;
+    const double  *vdep_vmin;
+    const double  *vdep_dv;
+    const int32_t *vdep_off;
+    const double  *vdep_k;
+    const double  *tri_v;
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */

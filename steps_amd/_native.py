@@ -57,6 +57,12 @@ class StgModel(C.Structure):
         ("sreac_lhs_i", _ip),
         ("sreac_upd_i", _ip),
         ("sreac_k", _dp),
+        ("sreac_vdep", _ip),
+        ("n_vdep", C.c_int32),
+        ("vdep_vmin", _dp),
+        ("vdep_dv", _dp),
+        ("vdep_off", _ip),
+        ("vdep_k", _dp),
     ]
 
 
@@ -112,6 +118,11 @@ SIGNATURES = {
     "stg_set_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "stg_set_diff_apply_threshold": (C.c_int, [C.c_void_p, C.c_int32]),
+    "stg_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "stg_checkpoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "stg_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "stg_set_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_get_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),
     "stg_efield_create": (C.c_int, [C.POINTER(StgEfieldMesh), C.POINTER(C.c_void_p)]),

@@ -379,6 +379,13 @@ __global__ void k_accumulate(int n, double *x64, const T *d) {
     if (i < n) x64[i] += (double)d[i];
 }
 
+// potential of each membrane triangle (mean of its vertices, volts) into the
+// attached simulation's slot for that triangle
+__global__ void k_tri_v(int nt, const int *tri_v, const double *V, const int *tri_map, double *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) out[tri_map[t]] = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0 * 1e-3;
+}
+
 __global__ void k_apply_dv(int n, const unsigned char *clamped, const double *dv, double *V) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && !clamped[i]) V[i] += dv[i];
@@ -447,6 +454,7 @@ extern "C" int stg_internal_tri_counts_device(stg_sim *sim, int32_t real, const 
                                               int32_t *n_tris_total, int32_t *tri0);
 extern "C" int stg_internal_tri_perm(const stg_sim *sim, int32_t real, const int32_t **perm,
                                      int32_t *n);
+extern "C" int stg_internal_tri_v_device(stg_sim *sim, int32_t real, double **ptr);
 
 namespace {
 
@@ -685,6 +693,18 @@ int pcg(stg_efield *e, double rel_tol, int max_iter) {
     return it;
 }
 
+// refresh the attached simulation's triangle potentials (TetOpSplitP::_refreshEFTrisV,
+// tetopsplit.cpp:1931)
+void publish_tri_v(stg_efield *e) {
+    if (!e->sim || e->n_tris == 0) return;
+    double *dst = nullptr;
+    const int rc = stg_internal_tri_v_device(e->sim, e->sim_real, &dst);
+    if (rc != STG_OK) throw EfError{rc, "attached simulation rejected the triangle potentials"};
+    hipLaunchKernelGGL(k_tri_v, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
+                       e->tri_vd, e->V, e->tri_map, dst);
+    EF_HIP(hipGetLastError());
+}
+
 void advance_impl(stg_efield *e, double dt_s) {
     if (!(dt_s > 0.0)) throw EfError{STG_ERR_ARG, "EField time step must be positive"};
     const int n = e->n, nb = e->nb;
@@ -746,6 +766,7 @@ void advance_impl(stg_efield *e, double dt_s) {
     e->last_rel = rel;
     if (!ok) throw EfError{STG_ERR_PROG, "EField solve did not reach the residual tolerance"};
     hipLaunchKernelGGL(k_apply_dv, dim3(nb), dim3(kB), 0, e->stream, n, e->clamped, e->x64, e->V);
+    publish_tri_v(e);
     EF_HIP(hipMemcpyAsync(e->dv_prev, e->x64, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
     EF_HIP(hipGetLastError());
     EF_HIP(hipStreamSynchronize(e->stream));
@@ -913,6 +934,8 @@ int stg_efield_attach(stg_efield *e, stg_sim *sim, int32_t real) {
         }
         e->sim = sim;
         e->sim_real = real;
+        publish_tri_v(e);
+        EF_HIP(hipStreamSynchronize(e->stream));
     });
 }
 

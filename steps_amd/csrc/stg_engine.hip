@@ -98,6 +98,8 @@ struct stg_sim {
     int32_t NT = 0, NTRI = 0, max_tris = 0;
     std::vector<double> h_k, h_sk, h_D;  // per-realization tables
     std::vector<double> h_face_g;         // [NT*4], for update-period recomputation
+    std::vector<double> vdep_vmin, vdep_dv, vdep_k;
+    std::vector<int32_t> vdep_off;
     std::vector<double> dt_std;           // [n_real]
     std::vector<uint64_t> nsteps;         // [n_real]
     std::vector<uint64_t> seeds;
@@ -121,6 +123,9 @@ struct stg_sim {
     double *d_dt_tab = nullptr;
     size_t dt_tab_cap = 0;
     unsigned long long *d_ext = nullptr;
+    double *d_tri_v = nullptr;
+    double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
+    int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
     unsigned long long *d_sum = nullptr;
     int64_t *d_idx = nullptr;
@@ -171,7 +176,8 @@ struct stg_sim {
                         d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
-                        d_gather};
+                        d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
+                        d_vdep_off};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -274,6 +280,29 @@ void build_model(stg_sim *s, const stg_model *m) {
         }
     }
     k.supd_off[s->RS] = (uint16_t)ne;
+    // voltage-dependent rate tables
+    s->vdep_vmin.clear();
+    s->vdep_dv.clear();
+    s->vdep_off.assign(1, 0);
+    s->vdep_k.clear();
+    const int nv = (m->n_vdep > 0 && m->sreac_vdep) ? m->n_vdep : 0;
+    for (int t = 0; t < nv; ++t) {
+        const int a = m->vdep_off[t], b = m->vdep_off[t + 1];
+        if (b - a < 2 || !(m->vdep_dv[t] > 0.0))
+            throw StgError{STG_ERR_ARG, "a k(V) table needs two or more points and dv > 0"};
+        s->vdep_vmin.push_back(m->vdep_vmin[t]);
+        s->vdep_dv.push_back(m->vdep_dv[t]);
+        for (int i = a; i < b; ++i) {
+            if (!(m->vdep_k[i] >= 0.0)) throw StgError{STG_ERR_ARG, "k(V) must be >= 0"};
+            s->vdep_k.push_back(m->vdep_k[i]);
+        }
+        s->vdep_off.push_back((int32_t)s->vdep_k.size());
+    }
+    for (int r = 0; r < s->RS; ++r) {
+        const int t = (nv > 0) ? m->sreac_vdep[r] : -1;
+        if (t < -1 || t >= nv) throw StgError{STG_ERR_ARG, "k(V) table index out of range"};
+        k.s_vdep[r] = (int8_t)t;
+    }
 }
 
 // Diffusing species and occupancy tracking, from the union of all realizations'
@@ -477,6 +506,21 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     HIP_OK(hipMemset(s->d_counts, 0, sizeof(int32_t) * (size_t)std::max(S, 1) * NT));
     HIP_OK(hipMemset(s->d_tri_counts, 0, sizeof(int32_t) * (size_t)std::max(SS, 1) * std::max(NTRI, 1)));
     HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
+    s->d_tri_v = dalloc<double>(std::max(NTRI, 1));
+    {
+        std::vector<double> v0(std::max(NTRI, 1), -65.0e-3);  // DEFAULT_MEMB_POT
+        HIP_OK(hipMemcpy(s->d_tri_v, v0.data(), sizeof(double) * v0.size(), hipMemcpyHostToDevice));
+    }
+    s->d_vdep_vmin = dalloc<double>(std::max<size_t>(1, s->vdep_vmin.size()));
+    s->d_vdep_dv = dalloc<double>(std::max<size_t>(1, s->vdep_dv.size()));
+    s->d_vdep_off = dalloc<int32_t>(std::max<size_t>(1, s->vdep_off.size()));
+    s->d_vdep_k = dalloc<double>(std::max<size_t>(1, s->vdep_k.size()));
+    if (!s->vdep_k.empty()) {
+        HIP_OK(hipMemcpy(s->d_vdep_vmin, s->vdep_vmin.data(), sizeof(double) * s->vdep_vmin.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_vdep_dv, s->vdep_dv.data(), sizeof(double) * s->vdep_dv.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_vdep_off, s->vdep_off.data(), sizeof(int32_t) * s->vdep_off.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_vdep_k, s->vdep_k.data(), sizeof(double) * s->vdep_k.size(), hipMemcpyHostToDevice));
+    }
     upload_real_tables(s);
 }
 
@@ -536,6 +580,11 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.dt_tab = s->d_dt_tab;
     k.ext = s->d_ext;
     k.err = s->d_err;
+    k.tri_v = s->d_tri_v;
+    k.vdep_vmin = s->d_vdep_vmin;
+    k.vdep_dv = s->d_vdep_dv;
+    k.vdep_off = s->d_vdep_off;
+    k.vdep_k = s->d_vdep_k;
     return k;
 }
 
@@ -621,7 +670,8 @@ void run_impl(stg_sim *s, double endtime) {
         HIP_OK(hipMemcpy(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) {
             HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
-            throw StgError{STG_ERR_PROG, "SSA exceeded 4194304 events in one tet and period"};
+            if (err & 1) throw StgError{STG_ERR_PROG, "SSA exceeded 4194304 events in one tet and period"};
+            throw StgError{STG_ERR_PROG, "triangle potential outside a voltage-dependent rate table"};
         }
         for (int r = 0; r < s->n_real; ++r) s->nsteps[r] += dts[r].size();
     }
@@ -953,12 +1003,158 @@ int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
     });
 }
 
+// ------------------------------------------------------------ checkpoint
+// Layout (all little-endian, host byte order): header, per-realization time
+// counters and constants, then the count arrays as stored on the device.
+namespace {
+constexpr uint64_t kStateMagic = 0x32534154534754ull;  // "TGSTATS2": counts, constants, triangle V
+struct StateHeader {
+    uint64_t magic;
+    int32_t n_real, n_tets, n_tris, n_specs, n_sspecs, n_reacs, n_sreacs, diff_threshold;
+    double time;
+};
+
+size_t state_bytes(const stg_sim *s) {
+    return sizeof(StateHeader) + s->n_real * (sizeof(uint64_t) * 4) +
+           sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size()) +
+           sizeof(int32_t) * ((size_t)s->S * s->NT + (size_t)s->SS * s->NTRI) +
+           sizeof(double) * (size_t)s->NTRI;
+}
+}  // namespace
+
+int stg_state_size(const stg_sim *s, size_t *bytes) {
+    if (!s || !bytes) return STG_ERR_ARG;
+    *bytes = state_bytes(s);
+    return STG_OK;
+}
+
+int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
+    if (!s || !buf) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (bytes < state_bytes(s)) throw StgError{STG_ERR_ARG, "checkpoint buffer too small"};
+        char *p = static_cast<char *>(buf);
+        StateHeader h{kStateMagic, s->n_real, s->NT, s->NTRI, s->S, s->SS, s->R, s->RS, s->diff_threshold,
+                      s->time};
+        std::memcpy(p, &h, sizeof(h));
+        p += sizeof(h);
+        std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
+        if (!ext.empty())
+            HIP_OK(hipMemcpy(ext.data(), s->d_ext, sizeof(unsigned long long) * ext.size(), hipMemcpyDeviceToHost));
+        for (int r = 0; r < s->n_real; ++r) {
+            uint64_t ev = 0, mv = 0;
+            for (size_t b = 0; b < s->block_real.size(); ++b)
+                if (s->block_real[b] == r) {
+                    ev += ext[2 * b];
+                    mv += ext[2 * b + 1];
+                }
+            const uint64_t v[4] = {s->nsteps[r], s->seeds[r], ev, mv};
+            std::memcpy(p, v, sizeof(v));
+            p += sizeof(v);
+        }
+        for (const auto *vec : {&s->h_k, &s->h_sk, &s->h_D}) {
+            std::memcpy(p, vec->data(), vec->size() * sizeof(double));
+            p += vec->size() * sizeof(double);
+        }
+        const size_t nc = (size_t)s->S * s->NT, nt = (size_t)s->SS * s->NTRI;
+        if (nc) HIP_OK(hipMemcpy(p, s->d_counts, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+        p += nc * sizeof(int32_t);
+        if (nt) HIP_OK(hipMemcpy(p, s->d_tri_counts, nt * sizeof(int32_t), hipMemcpyDeviceToHost));
+        p += nt * sizeof(int32_t);
+        if (s->NTRI) HIP_OK(hipMemcpy(p, s->d_tri_v, (size_t)s->NTRI * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
+    if (!s || !buf) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (bytes < sizeof(StateHeader)) throw StgError{STG_ERR_ARG, "checkpoint too short"};
+        const char *p = static_cast<const char *>(buf);
+        StateHeader h;
+        std::memcpy(&h, p, sizeof(h));
+        if (h.magic != kStateMagic) throw StgError{STG_ERR_ARG, "not an stg checkpoint"};
+        if (h.n_real != s->n_real || h.n_tets != s->NT || h.n_tris != s->NTRI || h.n_specs != s->S ||
+            h.n_sspecs != s->SS || h.n_reacs != s->R || h.n_sreacs != s->RS)
+            throw StgError{STG_ERR_ARG, "checkpoint was written by a simulation of another shape"};
+        if (bytes < state_bytes(s)) throw StgError{STG_ERR_ARG, "checkpoint too short"};
+        p += sizeof(h);
+        std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
+        std::vector<bool> placed(s->n_real, false);
+        for (int r = 0; r < s->n_real; ++r) {
+            uint64_t v[4];
+            std::memcpy(v, p, sizeof(v));
+            p += sizeof(v);
+            s->nsteps[r] = v[0];
+            s->seeds[r] = v[1];
+            for (size_t b = 0; b < s->block_real.size(); ++b)
+                if (s->block_real[b] == r && !placed[r]) {
+                    ext[2 * b] = v[2];
+                    ext[2 * b + 1] = v[3];
+                    placed[r] = true;
+                }
+        }
+        for (auto *vec : {&s->h_k, &s->h_sk, &s->h_D}) {
+            std::memcpy(vec->data(), p, vec->size() * sizeof(double));
+            p += vec->size() * sizeof(double);
+        }
+        const size_t nc = (size_t)s->S * s->NT, nt = (size_t)s->SS * s->NTRI;
+        if (nc) HIP_OK(hipMemcpy(s->d_counts, p, nc * sizeof(int32_t), hipMemcpyHostToDevice));
+        p += nc * sizeof(int32_t);
+        if (nt) HIP_OK(hipMemcpy(s->d_tri_counts, p, nt * sizeof(int32_t), hipMemcpyHostToDevice));
+        p += nt * sizeof(int32_t);
+        if (s->NTRI) HIP_OK(hipMemcpy(s->d_tri_v, p, (size_t)s->NTRI * sizeof(double), hipMemcpyHostToDevice));
+        if (!ext.empty())
+            HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_real_seed, s->seeds.data(), sizeof(uint64_t) * s->n_real, hipMemcpyHostToDevice));
+        s->time = h.time;
+        s->diff_threshold = h.diff_threshold;
+        for (int r = 0; r < s->n_real; ++r) s->dt_std[r] = s->compute_dt_std(r);
+        upload_real_tables(s);
+        s->free_graph();
+    });
+}
+
 int stg_set_diff_apply_threshold(stg_sim *s, int32_t threshold) {
     if (!s) return STG_ERR_ARG;
     return guard(s, [&] {
         if (threshold < 0) throw StgError{STG_ERR_ARG, "threshold must be >= 0"};
         s->diff_threshold = threshold;
         s->free_graph();
+    });
+}
+
+int stg_set_tri_v(stg_sim *s, int32_t real, const double *v) {
+    if (!s || !v) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int n = s->tri_off[real + 1] - s->tri_off[real];
+        std::vector<double> internal(std::max(n, 1));
+        for (int i = 0; i < n; ++i) internal[s->tri_perm[real][i]] = v[i];
+        if (n > 0)
+            HIP_OK(hipMemcpy(s->d_tri_v + s->tri_off[real], internal.data(), sizeof(double) * n,
+                             hipMemcpyHostToDevice));
+    });
+}
+
+int stg_get_tri_v(stg_sim *s, int32_t real, double *v) {
+    if (!s || !v) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int n = s->tri_off[real + 1] - s->tri_off[real];
+        if (n == 0) return;
+        std::vector<double> internal(n);
+        HIP_OK(hipMemcpy(internal.data(), s->d_tri_v + s->tri_off[real], sizeof(double) * n,
+                         hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) v[i] = internal[s->tri_perm[real][i]];
+    });
+}
+
+// Device pointer of the triangle potentials of one realization (internal
+// triangle order), for the EField (not part of the public header).
+int stg_internal_tri_v_device(stg_sim *s, int32_t real, double **ptr) {
+    if (!s || !ptr) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        *ptr = s->d_tri_v + s->tri_off[real];
     });
 }
 

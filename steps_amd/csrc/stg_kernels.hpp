@@ -44,6 +44,7 @@ struct ModelK {
     int8_t order[kMaxReacs];
     int8_t sorder[kMaxSReacs];
     int8_t s_nvol[kMaxSReacs];  // number of volume reactants of a surface reaction
+    int8_t s_vdep[kMaxSReacs];  // k(V) table of a surface reaction, -1 = constant k
     uint16_t lhs_off[kMaxReacs + 1], upd_off[kMaxReacs + 1];
     uint16_t slhs_off[kMaxSReacs + 1], supd_off[kMaxSReacs + 1];
     uint8_t ent_spec[kMaxEntries];
@@ -81,7 +82,12 @@ struct SysK {
     const int32_t *real_nper;    // [R] periods in this run
     const double *dt_tab;        // [L][R]
     unsigned long long *ext;     // [n_blocks][2] events, molecules moved
-    int32_t *err;                // error flags
+    int32_t *err;                // error flags: 1 event cap, 2 potential outside a k(V) table
+    const double *tri_v;         // [NTRI] membrane potential of each triangle (V)
+    const double *vdep_vmin;     // [n_vdep]
+    const double *vdep_dv;       // [n_vdep]
+    const int32_t *vdep_off;     // [n_vdep+1]
+    const double *vdep_k;        // concatenated k(V) tables
 };
 
 __device__ __forceinline__ double scaled_const(double k, int order, double scale) {
@@ -110,6 +116,23 @@ struct Lds {
     __device__ __forceinline__ double &p(int j) const { return a[j * B + tid]; }
     __device__ __forceinline__ double &c(int j, int) const { return kc[j * B + tid]; }
 };
+
+// k(V) by linear interpolation in a table (VDepSReacdef::getVDepK,
+// src/steps/solver/vdepsreacdef.cpp:194-217); sets error bit 2 outside it.
+__device__ __forceinline__ double vdep_k(const SysK &sys, int tab, double v) {
+    const double vmin = sys.vdep_vmin[tab], dv = sys.vdep_dv[tab];
+    const int n = sys.vdep_off[tab + 1] - sys.vdep_off[tab];
+    const double *K = sys.vdep_k + sys.vdep_off[tab];
+    const double v2 = (v - vmin) / dv;
+    if (!(v2 >= 0.0) || !(v2 <= (double)(n - 1))) {
+        atomicOr(sys.err, 2);
+        return 0.0;
+    }
+    const double lv = floor(v2);
+    const int l = (int)lv, u = (int)ceil(v2);
+    const double r = v2 - lv;
+    return (1.0 - r) * K[l] + r * K[u];
+}
 
 // combinatorial factor h of a surface reaction on triangle tri (0 if a
 // reactant is short)
@@ -148,8 +171,10 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
         for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale);
         for (int i = 0; i < ntri; ++i) {
             const double ascale = sys.tri_area[tp0 + i] * kAvogadro;
-            for (int q = 0; q < RS; ++q, ++j)
-                L.c(j, 0) = scaled_const(ksr[q], m.sorder[q], m.s_nvol[q] == 0 ? ascale : vscale);
+            for (int q = 0; q < RS; ++q, ++j) {
+                const double k = m.s_vdep[q] < 0 ? ksr[q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tp0 + i]);
+                L.c(j, 0) = scaled_const(k, m.sorder[q], m.s_nvol[q] == 0 ? ascale : vscale);
+            }
         }
     }
     Philox g;

@@ -27,13 +27,16 @@ def _mesh_struct(mt: MeshTables, keep: list) -> N.StgMesh:
 def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
     arrs = [N.as_i32(fm.reac_lhs), N.as_i32(fm.reac_upd), N.as_f64(fm.reac_k), N.as_f64(fm.spec_dcst),
             N.as_i32(fm.sreac_lhs_s), N.as_i32(fm.sreac_upd_s), N.as_i32(fm.sreac_lhs_i),
-            N.as_i32(fm.sreac_upd_i), N.as_f64(fm.sreac_k)]
+            N.as_i32(fm.sreac_upd_i), N.as_f64(fm.sreac_k), N.as_i32(fm.sreac_vdep),
+            N.as_f64(fm.vdep_vmin), N.as_f64(fm.vdep_dv), N.as_i32(fm.vdep_off), N.as_f64(fm.vdep_k)]
     keep.extend(arrs)
-    rl, ru, rk, dc, sls, sus, sli, sui, sk = arrs
+    rl, ru, rk, dc, sls, sus, sli, sui, sk, svd, vmin, vdv, voff, vk = arrs
     return N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
                       N.ptr(rk, C.c_double), N.ptr(dc, C.c_double), len(fm.sspecs), len(fm.sreacs),
                       N.ptr(sls, C.c_int32), N.ptr(sus, C.c_int32), N.ptr(sli, C.c_int32),
-                      N.ptr(sui, C.c_int32), N.ptr(sk, C.c_double))
+                      N.ptr(sui, C.c_int32), N.ptr(sk, C.c_double), N.ptr(svd, C.c_int32),
+                      len(fm.vdep_vmin), N.ptr(vmin, C.c_double), N.ptr(vdv, C.c_double),
+                      N.ptr(voff, C.c_int32), N.ptr(vk, C.c_double))
 
 
 class Engine:
@@ -171,8 +174,28 @@ class Engine:
     def set_launch(self, block: int = 256, use_graph: bool = False):
         self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
 
+    def set_tri_v(self, v, real: int = 0):
+        a = N.as_f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n_tris[real],)))
+        self._chk(self.lib.stg_set_tri_v(self.h, real, N.ptr(a, C.c_double)))
+
+    def get_tri_v(self, real: int = 0) -> np.ndarray:
+        a = np.empty(self.n_tris[real])
+        self._chk(self.lib.stg_get_tri_v(self.h, real, N.ptr(a, C.c_double)))
+        return a
+
     def set_diff_apply_threshold(self, threshold: int):
         self._chk(self.lib.stg_set_diff_apply_threshold(self.h, int(threshold)))
+
+    def checkpoint(self) -> bytes:
+        n = C.c_size_t()
+        self._chk(self.lib.stg_state_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._chk(self.lib.stg_checkpoint(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, state: bytes):
+        buf = C.create_string_buffer(state, len(state))
+        self._chk(self.lib.stg_restore(self.h, buf, len(state)))
 
     def last_run_kernel_ms(self):
         ms, n = C.c_double(), C.c_int32()

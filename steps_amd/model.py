@@ -100,6 +100,33 @@ class SReac:
         return len(self.ilhs) + len(self.slhs) + len(self.olhs)
 
 
+class VDepSReac(SReac):
+    """Voltage-dependent surface reaction: the rate constant is k(V) of the
+    triangle's membrane potential, tabulated over vrange = [vmin, vmax, dv] and
+    linearly interpolated. The table holds int((vmax-vmin)/dv)+1 values of k at
+    V = vmin, vmin+dv, ... with V accumulated by repeated addition, as
+    _VoltageTable builds it (pysteps/steps/API_1/model.py:496-521); lookup is
+    VDepSReacdef::getVDepK (src/steps/solver/vdepsreacdef.cpp:194-216)."""
+
+    def __init__(self, id: str, surfsys: Surfsys, ilhs=(), slhs=(), olhs=(), irhs=(), srhs=(),
+                 orhs=(), k=None, vrange=(-150.0e-3, 100.0e-3, 1.0e-4)):
+        if k is None or not callable(k):
+            raise ValueError("VDepSReac needs a callable k(V)")
+        vmin, vmax, dv = (float(x) for x in vrange)
+        if not (vmax > vmin and dv > 0):
+            raise ValueError("vrange must be [vmin, vmax, dv] with vmax > vmin and dv > 0")
+        n = int((vmax - vmin) / dv) + 1
+        self.vmin, self.vmax, self.dv = vmin, vmax, dv
+        ktab, v = [], vmin
+        for _ in range(n):
+            ktab.append(float(k(v)))
+            v += dv
+        self.ktab = np.array(ktab)
+        if np.any(self.ktab < 0) or not np.all(np.isfinite(self.ktab)):
+            raise ValueError("k(V) must be finite and >= 0 over vrange")
+        super().__init__(id, surfsys, ilhs, slhs, olhs, irhs, srhs, orhs, kcst=0.0)
+
+
 class Diff:
     """Diffusion rule of ``lig`` with constant dcst (m^2/s)."""
 
@@ -129,6 +156,11 @@ class FlatModel:
     sreac_lhs_i: np.ndarray = None
     sreac_upd_i: np.ndarray = None
     sreac_k: np.ndarray = None
+    sreac_vdep: np.ndarray = None     # [n_sreacs] k(V) table index, -1 = constant k
+    vdep_vmin: np.ndarray = None
+    vdep_dv: np.ndarray = None
+    vdep_off: np.ndarray = None
+    vdep_k: np.ndarray = None
 
     def spec_index(self, name: str) -> int:
         try:
@@ -205,4 +237,11 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
         fm.sreac_upd_s[i] = _count(r.srhs, sspecs) - ls
         fm.sreac_upd_i[i] = _count(r.irhs, specs) - li
     fm.sreac_k = np.array([r.kcst for r in sreacs], dtype=np.float64)
+    vd = [r for r in sreacs if isinstance(r, VDepSReac)]
+    fm.sreac_vdep = np.array([vd.index(r) if isinstance(r, VDepSReac) else -1 for r in sreacs],
+                             dtype=np.int32)
+    fm.vdep_vmin = np.array([r.vmin for r in vd], dtype=np.float64)
+    fm.vdep_dv = np.array([r.dv for r in vd], dtype=np.float64)
+    fm.vdep_off = np.concatenate([[0], np.cumsum([len(r.ktab) for r in vd])]).astype(np.int32)
+    fm.vdep_k = (np.concatenate([r.ktab for r in vd]) if vd else np.zeros(0)).astype(np.float64)
     return fm

@@ -335,15 +335,34 @@ class TetOpSplit:
         c = self.eng.get_tri_counts(self._sspec(s))
         counts[:] = c[[self._tri(int(t)) for t in index_array]]
 
-    # --- checkpointing (own format: counts, time and period counter) ---
+    # --- triangle membrane potential read by voltage-dependent surface reactions ---
+    def getTriV(self, tri: int) -> float:
+        return float(self.eng.get_tri_v()[self._tri(tri)])
+
+    def setTriV(self, tri: int, v: float):
+        """Potential (V) of one membrane triangle. An attached EField overwrites
+        it after every step (tetopsplit.cpp:6600-6626, _setTriV)."""
+        vv = self.eng.get_tri_v()
+        vv[self._tri(tri)] = v
+        self.eng.set_tri_v(vv)
+
+    def getBatchTriVsNP(self, index_array, vs):
+        vv = self.eng.get_tri_v()
+        vs[:] = vv[[self._tri(int(t)) for t in index_array]]
+
+    def setBatchTriVsNP(self, index_array, vs):
+        vv = self.eng.get_tri_v()
+        vv[[self._tri(int(t)) for t in index_array]] = vs
+        self.eng.set_tri_v(vv)
+
+    # --- checkpointing (own binary format, stg_checkpoint in include/stg.h) ---
     def checkpoint(self, file_name: str):
-        st = {f"v_{i}": self.eng.get_tet_counts(i) for i in range(len(self.fm.specs))}
-        st.update({f"s_{i}": self.eng.get_tri_counts(i) for i in range(len(self.fm.sspecs))})
-        np.savez(file_name, time=self.getTime(), nsteps=self.getNSteps(), **st)
+        with open(file_name, "wb") as f:
+            f.write(self.eng.checkpoint())
 
     def restore(self, file_name: str):
-        raise _notimpl("restore of the period counter needs an engine setter; "
-                       "reload counts with setTetSpecCount instead")
+        with open(file_name, "rb") as f:
+            self.eng.restore(f.read())
 
 
 class DistTetOpSplit(TetOpSplit):

@@ -106,13 +106,15 @@ def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> 
 
 
 def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0,
-                          ca_um=0.1, pump_density=10.0, influx=1e5) -> Workload:
+                          ca_um=0.1, pump_density=10.0, influx=1e5,
+                          vdep_influx=False) -> Workload:
     """Ca + Buf <-> CaB (kf 1e8 /M/s, kb 10 /s), D_Ca 2e-10 m^2/s (Buf, CaB
     immobile); every boundary triangle is membrane. Pumps P (pump_density per
     um^2) run P + Ca_in -> P + Ca_out with k = kcat/Km = 50 /s / 1 uM; Ca_out
     is a surface species counting extruded ions. The x = 0 face carries one
     influx channel per triangle, Chan -> Chan + Ca_in, with k set so the face
-    admits ``influx`` ions/s in total."""
+    admits ``influx`` ions/s in total. With ``vdep_influx`` the channel is
+    voltage gated: k(V) = k_max / (1 + exp(-(V + 20 mV) / 6 mV)) over -150..100 mV."""
     rng = np.random.default_rng(seed)
     mesh, comp, tris = _cube(n, side, True)
     mdl = M.Model()
@@ -126,7 +128,12 @@ def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0
     M.SReac("pump", ss, ilhs=[Ca], slhs=[P], srhs=[P, CaOut], kcst=50.0 / 1e-6)
     face_tris = tris[np.all(np.abs(mesh.verts[mesh.tris[tris]][:, :, 0]) < 1e-3 * side / n, axis=1)]
     n_face = len(face_tris)
-    M.SReac("influx", ss, slhs=[Chan], srhs=[Chan], irhs=[Ca], kcst=influx / max(n_face, 1))
+    k_in = influx / max(n_face, 1)
+    if vdep_influx:
+        M.VDepSReac("influx", ss, slhs=[Chan], srhs=[Chan], irhs=[Ca],
+                    k=lambda v: k_in / (1.0 + np.exp(-(v + 20e-3) / 6e-3)))
+    else:
+        M.SReac("influx", ss, slhs=[Chan], srhs=[Chan], irhs=[Ca], kcst=k_in)
     fm = M.flatten(mdl, ["vsys"], ["ssys"])
     mt = geom.mesh_tables(mesh, comp, tris)
     vol_l = mt.vol * 1e3

@@ -154,3 +154,74 @@ def test_diff_apply_threshold_bit_exact(threshold):
     eng.run(w.end_time)
     o.run(w.end_time, 0)
     assert np.array_equal(_gpu_counts(eng, w), o.counts)
+
+
+def test_checkpoint_restore_continues_bit_exact(tmp_path):
+    w = W.four_species_cube(n=5, side=5e-6, seed=21, end_time=0.04)
+    print("case checkpoint/restore", flush=True)
+    eng = _engine(w, 9)
+    eng.run(0.013)
+    state = eng.checkpoint()
+    eng.run(0.04)
+    ref = _gpu_counts(eng, w)
+    ev = eng.extents()
+    eng.restore(state)
+    assert eng.time() == 0.013
+    eng.run(0.04)
+    assert np.array_equal(_gpu_counts(eng, w), ref)
+    assert eng.extents() == ev
+    # through the solver-level file interface of a fresh engine
+    eng2 = _engine(w, 9)
+    p = tmp_path / "cp.bin"
+    p.write_bytes(state)
+    eng2.restore(p.read_bytes())
+    eng2.run(0.04)
+    assert np.array_equal(_gpu_counts(eng2, w), ref)
+
+
+def test_autotune_keeps_results():
+    from steps_amd.autotune import autotune
+
+    w = W.four_species_cube(n=6, side=6e-6, seed=4, end_time=0.03)
+    print("case autotune", flush=True)
+    ref = _engine(w, 2)
+    ref.run(w.end_time)
+    eng = _engine(w, 2)
+    res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True))
+    assert res["block"] in (64, 256) and len(res["timings"]) == 4
+    assert eng.time() == 0.0
+    eng.run(w.end_time)
+    assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_vdep_membrane_bit_exact(mode):
+    """Voltage-gated influx channels at per-triangle potentials spread over
+    -90..+30 mV (most between table points), GPU vs oracle."""
+    w = W.calcium_membrane_cube(n=6, side=2e-6, seed=8, end_time=3e-4, buf_um=100.0,
+                                ca_um=20.0, influx=3e6, vdep_influx=True)
+    print(f"case vdep membrane mode={mode}", flush=True)
+    assert (w.fm.sreac_vdep >= 0).sum() == 1
+    v = np.random.default_rng(3).uniform(-90e-3, 30e-3, len(w.mt.tri_tet))
+    eng = _engine(w, 77, mode)
+    eng.set_tri_v(v)
+    assert np.array_equal(eng.get_tri_v(), v)
+    o = _oracle(w, 77)
+    o.tri_v[:] = v
+    for t in (1e-4, w.end_time):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+        assert np.array_equal(_gpu_tri_counts(eng, w), o.tri_counts[:len(w.fm.sspecs)])
+    assert eng.extents() == (o.events.value, o.moved.value)
+
+
+def test_vdep_voltage_outside_table_is_reported():
+    w = W.calcium_membrane_cube(n=3, side=1e-6, seed=1, end_time=1e-4, vdep_influx=True)
+    print("case vdep out of range", flush=True)
+    eng = _engine(w, 1)
+    v = np.full(len(w.mt.tri_tet), -65e-3)
+    v[0] = 0.2  # above the 100 mV end of the table
+    eng.set_tri_v(v)
+    with pytest.raises(N.StgError, match="outside"):
+        eng.run(w.end_time)

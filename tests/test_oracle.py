@@ -14,6 +14,10 @@
 - Diffusion: exact conservation, leaving counts ~ Binomial, face split ~
   r_f / sum r, time-averaged occupancy, point-source profile vs the 3-D
   Green's function, equilibrium uniformity.
+- Voltage-dependent surface reactions: the k(V) table as the reference fills
+  it, bit-exact agreement with a constant-rate reaction at exactly
+  representable interpolation points, first-order decay at a nonlinear k(V),
+  and the out-of-range error.
 """
 import decimal
 import json
@@ -401,6 +405,102 @@ def test_surface_volume_reaction_uses_inner_volume_scale():
     o.run(0.2, 0)
     rate = k / (1e3 * VOL * AV) * 3
     assert tolerable(o.counts[fm.spec_index("Ca")].mean(), 200 * math.exp(-rate * 0.2), 0.01)
+
+
+# ------------------------------------------------ voltage-dependent surface reactions
+
+def _gate_model(kfun=None, vrange=None, kcst=None):
+    """C.s -> O.s, either voltage dependent (kfun over vrange) or constant kcst."""
+    mdl = M.Model()
+    c, o_, vol = M.Spec("C", mdl), M.Spec("O", mdl), M.Spec("V", mdl)
+    vs = M.Volsys("vsys", mdl)
+    M.Diff("dV", vs, vol, 0.0)
+    ss = M.Surfsys("ssys", mdl)
+    if kfun is None:
+        M.SReac("open", ss, slhs=[c], srhs=[o_], kcst=kcst)
+    else:
+        M.VDepSReac("open", ss, slhs=[c], srhs=[o_], k=kfun, vrange=vrange)
+    return M.flatten(mdl, ["vsys"], ["ssys"])
+
+
+# Exactly representable grid: V = -0.125 + i * 2^-10, k linear in V
+VR_EXACT = (-0.125, 0.125, 2.0 ** -10)
+
+
+def _k_lin(v):
+    return 1000.0 + 8000.0 * (v + 0.125)
+
+
+def test_vdep_table_matches_reference_fill():
+    # int((vmax - vmin)/dv) + 1 entries at V accumulated by repeated addition
+    # (pysteps/steps/API_1/model.py:510-517); default range -150..100 mV step 0.1 mV
+    fm = _gate_model(lambda v: v + 1.0, (-150.0e-3, 100.0e-3, 1.0e-4))
+    assert fm.vdep_off.tolist() == [0, 2501]   # 2500 intervals of 0.1 mV plus the end point
+    v, want = -150.0e-3, []
+    for _ in range(2501):
+        want.append(v + 1.0)
+        v += 1.0e-4
+    assert np.array_equal(fm.vdep_k, np.array(want))
+    assert fm.sreac_vdep.tolist() == [0]
+
+
+@pytest.mark.parametrize("v,k_expect", [
+    (-0.0625, 1500.0),                        # on grid point 64: r = 0 picks K[64] alone
+    (-0.0625 + 2.0 ** -12, 1501.953125),      # v2 = 64.25: 0.75 K[64] + 0.25 K[65]
+])
+def test_vdep_equals_constant_rate_at_interpolated_k(v, k_expect):
+    # every value here is exact in binary, so k(V) must reproduce the constant
+    # surface reaction with the interpolated k bit for bit; a swapped weight
+    # (0.25 K[64] + 0.75 K[65] = 1505.859375) or an off-by-one index fails
+    area = 1e-13
+    mt = _isolated(400, VOL, with_tris=True, area=area)
+    fv = _gate_model(_k_lin, VR_EXACT)
+    fc = _gate_model(kcst=k_expect)
+    ov, oc = bind.OracleSim(fv, mt, seed=9), bind.OracleSim(fc, mt, seed=9)
+    for o, fm in ((ov, fv), (oc, fc)):
+        o.tri_counts[fm.sspec_index("C")] = 50
+    ov.tri_v[:] = v
+    for t in (1e-4, 3e-4):
+        ov.run(t, 0)
+        oc.run(t, 0)
+        assert np.array_equal(ov.tri_counts, oc.tri_counts)
+    assert 0 < ov.tri_counts[fv.sspec_index("O")].sum() < 400 * 50
+
+
+def test_vdep_first_order_follows_k_of_v():
+    # a nonlinear k(V) (Hodgkin-Huxley alpha_n, per second), triangles at several
+    # potentials: surviving fraction exp(-k(V) t) with k read off the table by
+    # numpy's piecewise-linear interpolation
+    def alpha_n(v):
+        vm = v * 1e3
+        return 1e3 * 0.01 * (10 - (vm + 65)) / (math.exp((10 - (vm + 65)) / 10) - 1)
+
+    vr = (-100.0e-3, 50.0e-3, 1.0e-4)
+    fm = _gate_model(alpha_n, vr)
+    vs = np.array([-80.0e-3, -65.0e-3, -40.33e-3, 12.345e-3])
+    n_per = 1500
+    mt = _isolated(n_per * len(vs), VOL, with_tris=True)
+    o = bind.OracleSim(fm, mt, seed=11)
+    o.tri_counts[fm.sspec_index("C")] = 20
+    o.tri_v[:] = np.repeat(vs, n_per)
+    t = 2e-3
+    o.run(t, 0)
+    grid = vr[0] + np.arange(len(fm.vdep_k)) * vr[2]
+    left = o.tri_counts[fm.sspec_index("C")].reshape(len(vs), n_per).mean(axis=1)
+    for v, m in zip(vs, left):
+        k = np.interp(v, grid, fm.vdep_k)
+        assert tolerable(m, 20 * math.exp(-k * t), 0.03), (v, m, k)
+
+
+def test_vdep_voltage_outside_table_is_an_error():
+    # getVDepK raises for V above vmax or below vmin (vdepsreacdef.cpp:196-207)
+    mt = _isolated(4, VOL, with_tris=True)
+    fm = _gate_model(_k_lin, VR_EXACT)
+    o = bind.OracleSim(fm, mt, seed=1)
+    o.tri_counts[fm.sspec_index("C")] = 5
+    o.tri_v[:] = 0.2
+    with pytest.raises(RuntimeError, match="-2"):
+        o.run(1e-4, 0)
 
 
 # ---------------------------------------------------------------- diffusion
