@@ -30,7 +30,7 @@ class StgoSystem(C.Structure):
         ("sreac_upd_i", _ip), ("sreac_k", _dp), ("counts", _ip), ("tri_counts", _ip),
         ("seed", C.c_uint64), ("stream", C.c_uint32), ("diff_threshold", C.c_int32),
         ("sreac_vdep", _ip), ("vdep_vmin", _dp), ("vdep_dv", _dp), ("vdep_off", _ip),
-        ("vdep_k", _dp), ("tri_v", _dp),
+        ("vdep_k", _dp), ("tri_v", _dp), ("tri_intg_mask", C.c_uint32), ("tri_intg", _dp),
     ]
 
 
@@ -138,6 +138,7 @@ class OracleSim:
             voff=i32(fm.vdep_off if fm.vdep_off is not None else [0]),
             vk=f64(fm.vdep_k if fm.vdep_k is not None else []),
             triv=np.full(max(self.n_tris, 1), -65.0e-3),
+            intg=np.zeros((max(SS, 1), max(self.n_tris, 1))),
         )
         a = self.a
         P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
@@ -150,7 +151,7 @@ class OracleSim:
             P(a["sk"], C.c_double), P(a["counts"], C.c_int32), P(a["tcounts"], C.c_int32),
             seed, stream, diff_threshold, P(a["svd"], C.c_int32), P(a["vmin"], C.c_double),
             P(a["vdv"], C.c_double), P(a["voff"], C.c_int32), P(a["vk"], C.c_double),
-            P(a["triv"], C.c_double))
+            P(a["triv"], C.c_double), 0, P(a["intg"], C.c_double))
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         self.time = 0.0
@@ -160,6 +161,16 @@ class OracleSim:
     @property
     def counts(self) -> np.ndarray:
         return self.a["counts"]
+
+    def integrate_tri_species(self, sspecs) -> None:
+        """Accumulate the time integral of these surface species' counts into
+        tri_intg from now on (the ohmic-current channel average)."""
+        for q in sspecs:
+            self.sys.tri_intg_mask |= 1 << int(q)
+
+    @property
+    def tri_intg(self) -> np.ndarray:
+        return self.a["intg"]
 
     @property
     def tri_v(self) -> np.ndarray:

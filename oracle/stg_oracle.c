@@ -511,8 +511,14 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
             const int32_t tr = tp->tri_idx[t0 + i];
             const int32_t *us_ = s->sreac_upd_s + r * SS;
             const int32_t *ui = s->sreac_upd_i + r * S;
-            for (int32_t q = 0; q < SS; ++q)
-                if (us_[q] != 0) s->tri_counts[(size_t)q * s->n_tris + tr] += us_[q];
+            for (int32_t q = 0; q < SS; ++q) {
+                if (us_[q] == 0) continue;
+                s->tri_counts[(size_t)q * s->n_tris + tr] += us_[q];
+                /* channel-count integral: n_end*dt - sum dn*t_event, the same
+                 * value as the reference's piecewise sum (tri.cpp:544-561) */
+                if ((s->tri_intg_mask >> q) & 1u)
+                    s->tri_intg[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
+            }
             for (int32_t sp = 0; sp < S; ++sp) {
                 if (ui[sp] == 0) continue;
                 cnt[sp] += ui[sp];
@@ -623,6 +629,15 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                 if (nev < 0) {
                     err |= nev == -2 ? 2 : 1;
                     nev = 0;
+                }
+                if (s->tri_intg_mask != 0) {
+                    for (int32_t q = 0; q < s->n_sspecs; ++q) {
+                        if (!((s->tri_intg_mask >> q) & 1u)) continue;
+                        for (int32_t k = tp.tri_ptr[t]; k < tp.tri_ptr[t + 1]; ++k) {
+                            const size_t ix = (size_t)q * s->n_tris + tp.tri_idx[k];
+                            s->tri_intg[ix] += (double)s->tri_counts[ix] * dt;
+                        }
+                    }
                 }
                 ev_total += (uint64_t)nev;
                 for (int32_t sp = 0; sp < S; ++sp) {

@@ -57,6 +57,11 @@ typedef struct stgo_system {
     const int32_t *vdep_off;
     const double  *vdep_k;
     const double  *tri_v;
+    /* time integral of the counts of the surface species in tri_intg_mask
+     * (bit q = species q), [n_sspecs][n_tris], for time-averaged ohmic
+     * channel counts (tri.cpp:544-561, 664-693); may be null when the mask is 0 */
+    uint32_t tri_intg_mask;
+    double  *tri_intg;
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */

@@ -130,10 +130,12 @@ struct OhmicK {
     double erev[8];   // V
 };
 
+// The open-channel count is the time average over the last span seconds
+// (integral / span, Tri::computeI) when intg is given, else the present count.
 __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, const int *tri_v,
                                  const double *V, const double *tri_clamp, const double *vert_clamp,
-                                 const int *tri_counts, const int *tri_map, int n_tris_sim, OhmicK oc,
-                                 double *I) {
+                                 const int *tri_counts, const double *intg, double span,
+                                 const int *tri_map, int n_tris_sim, OhmicK oc, double *I) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double cur = vert_clamp[i];
@@ -144,7 +146,8 @@ __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, co
             const double vt = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0;
             double im = 0.0;
             for (int k = 0; k < oc.n; ++k) {
-                const double nopen = (double)tri_counts[(size_t)oc.sspec[k] * n_tris_sim + tri_map[t]];
+                const size_t ix = (size_t)oc.sspec[k] * n_tris_sim + tri_map[t];
+                const double nopen = intg ? intg[ix] / span : (double)tri_counts[ix];
                 im += nopen * oc.g[k] * (vt * 1e-3 - oc.erev[k]);
             }
             c -= im * 1e12;
@@ -386,6 +389,14 @@ __global__ void k_tri_v(int nt, const int *tri_v, const double *V, const int *tr
     if (t < nt) out[tri_map[t]] = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0 * 1e-3;
 }
 
+// restart the time integrals of the channel species on this EField's triangles
+__global__ void k_zero_intg(int nt, const int *tri_map, int n_tris_sim, uint32_t mask, double *intg) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nt) return;
+    for (int q = 0; q < 32; ++q)
+        if ((mask >> q) & 1u) intg[(size_t)q * n_tris_sim + tri_map[t]] = 0.0;
+}
+
 __global__ void k_apply_dv(int n, const unsigned char *clamped, const double *dv, double *V) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n && !clamped[i]) V[i] += dv[i];
@@ -419,6 +430,9 @@ struct stg_efield {
     OhmicK oc{};
     stg_sim *sim = nullptr;
     int sim_real = 0;
+    double *intg = nullptr;                 // attached simulation's channel-count integrals
+    uint32_t intg_mask = 0;
+    double intg_t0 = 0.0;                   // simulation time the integrals started from
     std::string err;
     hipStream_t stream = nullptr;
     // device
@@ -455,6 +469,7 @@ extern "C" int stg_internal_tri_counts_device(stg_sim *sim, int32_t real, const 
 extern "C" int stg_internal_tri_perm(const stg_sim *sim, int32_t real, const int32_t **perm,
                                      int32_t *n);
 extern "C" int stg_internal_tri_v_device(stg_sim *sim, int32_t real, double **ptr);
+extern "C" int stg_internal_tri_intg_device(stg_sim *sim, uint32_t mask, double **ptr);
 
 namespace {
 
@@ -705,19 +720,57 @@ void publish_tri_v(stg_efield *e) {
     EF_HIP(hipGetLastError());
 }
 
+// Ohmic currents average the channel counts over the time the simulation
+// advanced since the previous EField step (Tri::computeI, tri.cpp:664-693):
+// start the integrals of the channel species from the present time.
+void start_integrals(stg_efield *e) {
+    e->intg = nullptr;
+    e->intg_mask = 0;
+    if (!e->sim || e->oc.n == 0) return;
+    uint32_t mask = 0;
+    for (int k = 0; k < e->oc.n; ++k) mask |= 1u << e->oc.sspec[k];
+    int rc = stg_internal_tri_intg_device(e->sim, mask, &e->intg);
+    if (rc != STG_OK) throw EfError{rc, stg_last_error(e->sim)};
+    const int32_t *tc = nullptr;
+    int32_t ntot = 0, tri0 = 0;
+    rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tc, &ntot, &tri0);
+    if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
+    e->intg_mask = mask;
+    rc = stg_get_time(e->sim, &e->intg_t0);
+    if (rc != STG_OK) throw EfError{rc, stg_last_error(e->sim)};
+    hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
+                       e->tri_map, ntot, mask, e->intg + tri0);
+    EF_HIP(hipGetLastError());
+    EF_HIP(hipStreamSynchronize(e->stream));
+}
+
 void advance_impl(stg_efield *e, double dt_s) {
     if (!(dt_s > 0.0)) throw EfError{STG_ERR_ARG, "EField time step must be positive"};
     const int n = e->n, nb = e->nb;
     const EfDev m = e->dev();
     const int32_t *tri_counts = nullptr;
+    const double *intg = nullptr;
+    double span = 0.0, now = 0.0;
     int32_t ntot = 0, tri0 = 0;
     if (e->sim && e->oc.n > 0) {
-        const int rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tri_counts, &ntot, &tri0);
+        int rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tri_counts, &ntot, &tri0);
         if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
         tri_counts += tri0;
+        rc = stg_get_time(e->sim, &now);
+        if (rc != STG_OK) throw EfError{rc, stg_last_error(e->sim)};
+        span = now - e->intg_t0;
+        // no simulated time since the last step (or the clock was reset): the
+        // present counts stand in for the average
+        if (e->intg && span > 0.0) intg = e->intg + tri0;
     }
     hipLaunchKernelGGL(k_vertex_current, dim3(nb), dim3(kB), 0, e->stream, n, e->vt_ptr, e->vt_tri,
-                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, e->tri_map, ntot, e->oc, e->I);
+                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, intg, span, e->tri_map, ntot,
+                       e->oc, e->I);
+    if (e->intg) {
+        hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
+                           e->tri_map, ntot, e->intg_mask, e->intg + tri0);
+        e->intg_t0 = now;
+    }
     const double oodt = 1.0 / (dt_s * 1e3);
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, oodt, e->cap, e->gext,
                        e->vext, e->V, e->I, e->b, e->d64, e->d32);
@@ -816,6 +869,8 @@ int stg_efield_set_potential(stg_efield *e, double v) {
         std::vector<double> h(e->n, v * 1e3);
         ef_put(e->V, h);
         EF_HIP(hipMemset(e->dv_prev, 0, e->n * sizeof(double)));
+        publish_tri_v(e);
+        EF_HIP(hipStreamSynchronize(e->stream));
     });
 }
 
@@ -882,6 +937,8 @@ int stg_efield_set_verts_v(stg_efield *e, const double *v) {
         std::vector<double> h(v, v + e->n);
         for (auto &x : h) x *= 1e3;
         ef_put(e->V, h);
+        publish_tri_v(e);
+        EF_HIP(hipStreamSynchronize(e->stream));
     });
 }
 
@@ -910,11 +967,12 @@ int stg_efield_set_ohmic(stg_efield *e, int32_t n, const int32_t *sspec, const d
         if (n < 0 || n > 8) throw EfError{STG_ERR_ARG, "between 0 and 8 ohmic currents"};
         e->oc.n = n;
         for (int k = 0; k < n; ++k) {
-            if (sspec[k] < 0) throw EfError{STG_ERR_ARG, "bad channel species index"};
+            if (sspec[k] < 0 || sspec[k] >= 8) throw EfError{STG_ERR_ARG, "bad channel species index"};
             e->oc.sspec[k] = sspec[k];
             e->oc.g[k] = g[k];
             e->oc.erev[k] = erev[k];
         }
+        start_integrals(e);
     });
 }
 
@@ -934,6 +992,7 @@ int stg_efield_attach(stg_efield *e, stg_sim *sim, int32_t real) {
         }
         e->sim = sim;
         e->sim_real = real;
+        start_integrals(e);
         publish_tri_v(e);
         EF_HIP(hipStreamSynchronize(e->stream));
     });

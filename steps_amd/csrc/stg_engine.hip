@@ -124,6 +124,7 @@ struct stg_sim {
     size_t dt_tab_cap = 0;
     unsigned long long *d_ext = nullptr;
     double *d_tri_v = nullptr;
+    double *d_tri_intg = nullptr;  // [SS][NTRI] time integrals of channel counts, lazily allocated
     double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
     int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
@@ -177,7 +178,7 @@ struct stg_sim {
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
-                        d_vdep_off};
+                        d_vdep_off,  d_tri_intg};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -585,6 +586,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.vdep_dv = s->d_vdep_dv;
     k.vdep_off = s->d_vdep_off;
     k.vdep_k = s->d_vdep_k;
+    k.tri_intg = s->d_tri_intg;
     return k;
 }
 
@@ -728,6 +730,8 @@ int stg_reset(stg_sim *s) {
                               sizeof(int32_t) * (size_t)std::max(s->SS, 1) * std::max(s->NTRI, 1), s->stream));
         if (s->d_ext)
             HIP_OK(hipMemsetAsync(s->d_ext, 0, sizeof(unsigned long long) * 2 * s->block_real.size(), s->stream));
+        if (s->d_tri_intg)  // Tri::reset clears the ohmic-current integrals (tri.cpp:441)
+            HIP_OK(hipMemsetAsync(s->d_tri_intg, 0, sizeof(double) * (size_t)s->SS * s->NTRI, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         s->time = 0.0;
         std::fill(s->nsteps.begin(), s->nsteps.end(), 0);
@@ -1155,6 +1159,30 @@ int stg_internal_tri_v_device(stg_sim *s, int32_t real, double **ptr) {
     return guard(s, [&] {
         check_real(s, real);
         *ptr = s->d_tri_v + s->tri_off[real];
+    });
+}
+
+// Start accumulating the time integral of the counts of the surface species
+// in mask (bit q = species q) on every triangle, and return the [SS][NTRI]
+// integral array (internal triangle order, all realizations). For the
+// EField's time-averaged channel counts (not part of the public header).
+int stg_internal_tri_intg_device(stg_sim *s, uint32_t mask, double **ptr) {
+    if (!s || !ptr) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (s->SS == 0 || s->NTRI == 0) throw StgError{STG_ERR_ARG, "the simulation has no membrane"};
+        if (mask == 0 || (s->SS < 32 && (mask >> s->SS) != 0))
+            throw StgError{STG_ERR_ARG, "surface species mask out of range"};
+        if (!s->d_tri_intg) {
+            const size_t n = (size_t)s->SS * s->NTRI;
+            s->d_tri_intg = dalloc<double>(n);
+            HIP_OK(hipMemsetAsync(s->d_tri_intg, 0, n * sizeof(double), s->stream));
+            HIP_OK(hipStreamSynchronize(s->stream));
+        }
+        if ((s->mk.tri_intg_mask | (int32_t)mask) != s->mk.tri_intg_mask) {
+            s->mk.tri_intg_mask |= (int32_t)mask;
+            s->free_graph();  // the model tables are baked into a captured graph
+        }
+        *ptr = s->d_tri_intg;
     });
 }
 

@@ -40,6 +40,7 @@ constexpr double kDblEpsilon = 2.220446049250313080847e-16;
 struct ModelK {
     int32_t n_specs, n_reacs, n_sspecs, n_sreacs, n_diff;
     int32_t tracked_mask;  // bit s: occupancy tracked for species s
+    int32_t tri_intg_mask; // bit q: time integral of surface species q is accumulated
     int32_t dspec[kMaxSpecs];
     int8_t order[kMaxReacs];
     int8_t sorder[kMaxSReacs];
@@ -88,6 +89,7 @@ struct SysK {
     const double *vdep_dv;       // [n_vdep]
     const int32_t *vdep_off;     // [n_vdep+1]
     const double *vdep_k;        // concatenated k(V) tables
+    double *tri_intg;            // [SS][NTRI] time integral of counts (tri_intg_mask species)
 };
 
 __device__ __forceinline__ double scaled_const(double k, int order, double scale) {
@@ -251,7 +253,10 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                 const int sp = m.ent_spec[e];
                 const int dv = m.ent_val[e];
                 if (sp & kSurfFlag) {
-                    sys.tri_counts[(size_t)(sp & 0x7f) * sys.n_tris + tri] += dv;
+                    const size_t ix = (size_t)(sp & 0x7f) * sys.n_tris + tri;
+                    sys.tri_counts[ix] += dv;
+                    // integral of n over the period = n_end * dt - sum dn * t_event
+                    if ((m.tri_intg_mask >> (sp & 0x7f)) & 1) sys.tri_intg[ix] -= (double)dv * tcur;
                 } else {
                     L.c(sp) += dv;
                     if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
@@ -408,6 +413,16 @@ __global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
+            }
+            if (m.tri_intg_mask != 0 && sys.tri_ptr != nullptr) {
+                const int tp1 = sys.tri_ptr[t + 1];
+                for (int q = 0; q < m.n_sspecs; ++q) {
+                    if (!((m.tri_intg_mask >> q) & 1)) continue;
+                    for (int tri = sys.tri_ptr[t]; tri < tp1; ++tri) {
+                        const size_t ix = (size_t)q * sys.n_tris + tri;
+                        sys.tri_intg[ix] += (double)sys.tri_counts[ix] * dt;
+                    }
+                }
             }
             if (m.n_diff > 0) {
                 const double2 g01 = sys.g[2 * (size_t)t];

@@ -7,7 +7,9 @@ traces, as test/validation/API_2/validation_efield/test_rallpack1.py requires.
 """
 import numpy as np
 import pytest
+from scipy.sparse.linalg import spsolve
 
+from oracle import bind
 from oracle.efield_ref import EFieldOracle
 from steps_amd import geom, model as M
 from steps_amd.efield import EField, FP64, MIXED
@@ -118,3 +120,69 @@ def test_rallpack1_gpu_coupled_with_ohmic_leak():
     rms1 = np.linalg.norm(vx[:n] - sim1[:n]) / np.sqrt(n)
     print(f"rallpack1 rms {rms0:.3e} {rms1:.3e}", flush=True)
     assert rms0 < 1e-3 and rms1 < 1e-3
+
+
+def test_ohmic_current_uses_time_averaged_channel_count():
+    """Channels C.s <-> O.s gate many times within one EField step; the ohmic
+    current must use the open count averaged over the step (Tri::computeI,
+    src/steps/mpi/tetopsplit/tri.cpp:664-693), not the count at its end. The
+    opening rate is voltage dependent, so the potentials the EField publishes
+    to the triangles feed back into the next step. GPU (FP64 CG) vs oracle
+    (FP64 LU) with bit-exact counts: dV within 1e-9 relative per step."""
+    mesh, tris_v = _cube_case(n=4, side=2e-6)
+    comp = geom.TmComp("cyto", mesh)
+    memb = mesh.getSurfTris()
+    mdl = M.Model()
+    c, o_ = M.Spec("C", mdl), M.Spec("O", mdl)
+    M.Volsys("vsys", mdl)
+    ss = M.Surfsys("ssys", mdl)
+    M.VDepSReac("open", ss, slhs=[c], srhs=[o_], k=lambda v: 2e4 * np.exp((v + 0.065) / 0.02))
+    M.SReac("close", ss, slhs=[o_], srhs=[c], kcst=3e4)
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    mt = geom.mesh_tables(mesh, comp, memb)
+    qc, qo = fm.sspec_index("C"), fm.sspec_index("O")
+    n0 = np.full(len(memb), 5, dtype=np.int32)
+    g, erev, T = 1e-12, -0.077, 1e-4
+    print("case ohmic time average", flush=True)
+    eng = Engine(fm, [mt], [5])
+    eng.set_tri_counts(qc, n0)
+    ef = EField(mesh.verts, mesh.tets, mesh.tris[memb])
+    ef.set_solver(FP64, 1e-12)
+    ora = EFieldOracle(mesh.verts, mesh.tets, mesh.tris[memb])
+    for obj in (ef, ora):
+        obj.set_potential(-0.065)
+        obj.set_capac(0.01)
+        obj.set_volres(1.0)
+    ef.set_ohmic([qo], [g], [erev])
+    ef.attach(eng)
+    sim = bind.OracleSim(fm, mt, seed=5)
+    sim.tri_counts[qc] = n0
+    sim.integrate_tri_species([qo])
+    for step in range(1, 4):
+        # the potentials the EField published are the ones k(V) reads
+        sim.tri_v[:] = eng.get_tri_v()
+        assert np.allclose(sim.tri_v, ora.tri_v(), rtol=1e-12, atol=0)
+        v_prev = ora.V * 1e-3
+        ef.run(eng, step * T, T)
+        sim.run(step * T, 0)
+        assert np.array_equal(eng.get_tri_counts(qo), sim.tri_counts[qo]), step
+        vt = ora.tri_v()
+        i_avg = g * (sim.tri_intg[qo] / T) * (vt - erev)
+        i_end = g * sim.tri_counts[qo] * (vt - erev)
+        sim.tri_intg[:] = 0.0
+        dv_end = _oracle_dv(ora, T, i_end)
+        dv = ora.advance(T, i_avg) * 1e-3
+        got = ef.verts_v() - v_prev
+        err = np.abs(got - dv).max() / np.abs(dv).max()
+        assert err < 1e-9, (step, err, ef.stats())
+        # the end-of-step count gives a visibly different answer
+        assert np.abs(dv_end - dv).max() / np.abs(dv).max() > 1e-4
+        ef.set_verts_v(ora.V * 1e-3)
+
+
+def _oracle_dv(ora, dt, i_amps):
+    """dV (volts) of one oracle step, leaving the oracle's state unchanged."""
+    A, b = ora.system(dt, i_amps)
+    dv = spsolve(A.tocsc(), b)
+    dv[ora.clamped] = 0.0
+    return dv * 1e-3

@@ -492,6 +492,38 @@ def test_vdep_first_order_follows_k_of_v():
         assert tolerable(m, 20 * math.exp(-k * t), 0.03), (v, m, k)
 
 
+def test_channel_count_time_integral():
+    # open channels O.s -> C.s at k from n0: E[int_0^T n dt] = n0 (1 - e^{-kT}) / k;
+    # T spans five periods (one per run call, nothing diffuses), so a correction
+    # taken from the absolute time instead of the period start, or with the
+    # wrong sign, misses by far more than the 1% tolerance
+    k, n0, T = 400.0, 30, 5e-3
+    mdl = M.Model()
+    c, o_, vol = M.Spec("C", mdl), M.Spec("O", mdl), M.Spec("V", mdl)
+    vs = M.Volsys("vsys", mdl)
+    M.Diff("dV", vs, vol, 0.0)
+    ss = M.Surfsys("ssys", mdl)
+    M.SReac("close", ss, slhs=[o_], srhs=[c], kcst=k)
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    mt = _isolated(4000, VOL, with_tris=True)
+    o = bind.OracleSim(fm, mt, seed=21)
+    qo, qc = fm.sspec_index("O"), fm.sspec_index("C")
+    o.tri_counts[qo] = n0
+    o.integrate_tri_species([qo, qc])
+    for i in range(1, 6):
+        o.run(i * T / 5, 0)
+    assert tolerable(o.tri_intg[qo].mean(), n0 * (1 - math.exp(-k * T)) / k, 0.01)
+    # O + C is constant per triangle, so the two integrals add up to n0 * T
+    assert np.allclose(o.tri_intg[qo] + o.tri_intg[qc], n0 * T, rtol=1e-12)
+    # a species no event touches integrates to n * T exactly up to rounding
+    o2 = bind.OracleSim(fm, mt, seed=21)
+    o2.tri_counts[qc] = 7
+    o2.integrate_tri_species([qc])
+    for i in range(1, 6):
+        o2.run(i * T / 5, 0)
+    assert np.allclose(o2.tri_intg[qc], 7 * T, rtol=1e-14)
+
+
 def test_vdep_voltage_outside_table_is_an_error():
     # getVDepK raises for V above vmax or below vmin (vdepsreacdef.cpp:196-207)
     mt = _isolated(4, VOL, with_tris=True)
