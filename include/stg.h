@@ -81,6 +81,20 @@ typedef struct stg_model {
     const double *vdep_dv;       /* [n_vdep] V */
     const int32_t *vdep_off;     /* [n_vdep+1] offsets into vdep_k */
     const double *vdep_k;        /* tables of k(vmin + i dv), SI */
+    /* GHK currents (GHKcurr::rate/apply, src/steps/mpi/tetopsplit/
+     * ghkcurr.cpp:168-272; math::GHKcurrent, src/steps/math/ghk.cpp:62-81):
+     * on every membrane triangle, ions of volume species ghk_ion cross at the
+     * GHK flux of each channel in surface species ghk_chan. The outer side is
+     * a virtual concentration ghk_voconc (M); ghk_realflux = 0 moves charge
+     * but no ions. Up to 4 currents. */
+    int32_t n_ghk;
+    const int32_t *ghk_chan;     /* [n_ghk] surface species of the open channel */
+    const int32_t *ghk_ion;      /* [n_ghk] volume species of the ion */
+    const int32_t *ghk_valence;  /* [n_ghk] nonzero */
+    const double *ghk_perm;      /* [n_ghk] single-channel permeability, m^3/s */
+    const double *ghk_voconc;    /* [n_ghk] virtual outer concentration, M, >= 0 */
+    const double *ghk_vshift;    /* [n_ghk] V added to the membrane potential */
+    const int32_t *ghk_realflux; /* [n_ghk] 1: the ion count changes per event */
 } stg_model;
 
 /* Per-realization parameters of a batch; NULL arrays take the model's values. */
@@ -190,6 +204,11 @@ int stg_restore(stg_sim *sim, const void *buf, size_t bytes);
  * EField (stg_efield_attach) writes it after every step. Default -65 mV. */
 int stg_set_tri_v(stg_sim *sim, int32_t real, const double *v);
 int stg_get_tri_v(stg_sim *sim, int32_t real, double *v);
+
+/* Temperature of the GHK flux equation, K (TetOpSplitP::setTemp/getTemp,
+ * tetopsplit.cpp:2001-2010; default 293.15, tetopsplit.cpp:1119). */
+int stg_set_temp(stg_sim *sim, double kelvin);
+int stg_get_temp(const stg_sim *sim, double *kelvin);
 
 /* Number of realizations and tets / triangles of one realization. */
 int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_tets,

@@ -31,6 +31,9 @@ class StgoSystem(C.Structure):
         ("seed", C.c_uint64), ("stream", C.c_uint32), ("diff_threshold", C.c_int32),
         ("sreac_vdep", _ip), ("vdep_vmin", _dp), ("vdep_dv", _dp), ("vdep_off", _ip),
         ("vdep_k", _dp), ("tri_v", _dp), ("tri_intg_mask", C.c_uint32), ("tri_intg", _dp),
+        ("n_ghk", C.c_int32), ("ghk_chan", _ip), ("ghk_ion", _ip), ("ghk_valence", _ip),
+        ("ghk_perm", _dp), ("ghk_voconc", _dp), ("ghk_vshift", _dp), ("ghk_realflux", _ip),
+        ("temp", C.c_double), ("tri_charge", _ip),
     ]
 
 
@@ -53,6 +56,8 @@ def load() -> C.CDLL:
     lib.stgo_philox4x32_8.restype = None
     lib.stgo_log.argtypes = [C.c_double]
     lib.stgo_log.restype = C.c_double
+    lib.stgo_exp.argtypes = [C.c_double]
+    lib.stgo_exp.restype = C.c_double
     lib.stgo_sqrt.argtypes = [C.c_double]
     lib.stgo_sqrt.restype = C.c_double
     lib.stgo_lfact.argtypes = [C.c_int32]
@@ -113,6 +118,11 @@ def ref_philox_kat(lines: list[tuple[int, ...]]):
     return [tuple(int(x) for x in ln.split()) for ln in out.stdout.strip().splitlines()]
 
 
+def _or(obj, name, default):
+    v = getattr(obj, name, None)
+    return default if v is None else v
+
+
 class OracleSim:
     """One realization on the CPU oracle, built from the same tables as the engine."""
 
@@ -139,6 +149,11 @@ class OracleSim:
             vk=f64(fm.vdep_k if fm.vdep_k is not None else []),
             triv=np.full(max(self.n_tris, 1), -65.0e-3),
             intg=np.zeros((max(SS, 1), max(self.n_tris, 1))),
+            gc=i32(_or(fm, "ghk_chan", [])), gi=i32(_or(fm, "ghk_ion", [])),
+            gz=i32(_or(fm, "ghk_valence", [])), gp=f64(_or(fm, "ghk_perm", [])),
+            go=f64(_or(fm, "ghk_voconc", [])), gv=f64(_or(fm, "ghk_vshift", [])),
+            gr=i32(_or(fm, "ghk_realflux", [])),
+            charge=np.zeros(max(self.n_tris, 1), dtype=np.int32),
         )
         a = self.a
         P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
@@ -151,7 +166,10 @@ class OracleSim:
             P(a["sk"], C.c_double), P(a["counts"], C.c_int32), P(a["tcounts"], C.c_int32),
             seed, stream, diff_threshold, P(a["svd"], C.c_int32), P(a["vmin"], C.c_double),
             P(a["vdv"], C.c_double), P(a["voff"], C.c_int32), P(a["vk"], C.c_double),
-            P(a["triv"], C.c_double), 0, P(a["intg"], C.c_double))
+            P(a["triv"], C.c_double), 0, P(a["intg"], C.c_double), len(a["gc"]),
+            P(a["gc"], C.c_int32), P(a["gi"], C.c_int32), P(a["gz"], C.c_int32),
+            P(a["gp"], C.c_double), P(a["go"], C.c_double), P(a["gv"], C.c_double),
+            P(a["gr"], C.c_int32), 293.15, P(a["charge"], C.c_int32))
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         self.time = 0.0
@@ -167,6 +185,14 @@ class OracleSim:
         tri_intg from now on (the ohmic-current channel average)."""
         for q in sspecs:
             self.sys.tri_intg_mask |= 1 << int(q)
+
+    @property
+    def tri_charge(self) -> np.ndarray:
+        """GHK charge moved out through each triangle (elementary charges)."""
+        return self.a["charge"]
+
+    def set_temp(self, kelvin: float) -> None:
+        self.sys.temp = kelvin
 
     @property
     def tri_intg(self) -> np.ndarray:

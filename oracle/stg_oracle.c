@@ -47,6 +47,9 @@
 #include <string.h>
 
 #define AVOGADRO 6.02214179e23 /* src/steps/math/constants.hpp:45 */
+#define FARADAY 96485.3399     /* src/steps/math/constants.hpp:49 */
+#define GAS_CONSTANT 8.314472  /* src/steps/math/constants.hpp:53 */
+#define E_CHARGE 1.602176487e-19 /* src/steps/math/constants.hpp:57 */
 #define MAX_EVENTS_PER_PERIOD (1 << 22)
 
 /* ------------------------------------------------------------------ Philox */
@@ -176,6 +179,30 @@ double stgo_lfact(int32_t k) {
     double ix2 = ix * ix;
     double corr = ix * (1.0 / 12.0 - ix2 * (1.0 / 360.0 - ix2 * (1.0 / 1260.0)));
     return (x - 0.5) * stgo_log(x) - x + 0x1.d67f1c864beb5p-1 + corr;
+}
+
+/* exp(x): x = k ln2 + r, |r| <= ln2/2, e^r by Taylor to r^13, times 2^k;
+ * the device evaluates the same operations in the same order */
+double stgo_exp(double x) {
+    if (!(x < 709.782712893384)) return x != x ? x : INFINITY;
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = floor(x * 0x1.71547652b82fep+0 + 0.5);
+    const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
+    double p = 0x1.6124613a86d09p-33;
+    p = p * r + 0x1.1eed8eff8d898p-29;
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return ldexp(p, (int)kd);
 }
 
 static double ipow(double base, int32_t n) {
@@ -354,6 +381,37 @@ static double vdep_const(const stgo_system *s, int32_t tab, double v) {
     return (1.0 - r) * K[l] + r * K[u];
 }
 
+/* GHK flux of one open channel split for a fixed potential into A = P z F
+ * nuFoRT, Bo = c_o e^-nuFoRT, Dn = 1 - e^-nuFoRT (math/ghk.cpp:62-81), so that
+ * the flux is (A (c_i - Bo)) / Dn as the reference evaluates it. Returns 0
+ * when nuFoRT is outside the range the reference accepts. */
+static int ghk_consts(const stgo_system *s, int32_t g, double v, double *A, double *Bo, double *Dn) {
+    const double z = (double)s->ghk_valence[g];
+    const double nuFoRT = z * (v + s->ghk_vshift[g]) * FARADAY / (GAS_CONSTANT * s->temp);
+    if (nuFoRT >= 308 * 2.30 || nuFoRT <= -307 * 2.30) return 0;
+    const double e = stgo_exp(-nuFoRT);
+    const double oconc = s->ghk_voconc[g] * 1.0e3;
+    if (fabs(nuFoRT) > DBL_EPSILON) {
+        *A = s->ghk_perm[g] * z * FARADAY * nuFoRT;
+        *Bo = oconc * e;
+        *Dn = 1.0 - e;
+    } else {
+        *A = s->ghk_perm[g] * z * FARADAY;
+        *Bo = oconc;
+        *Dn = 1.0;
+    }
+    return 1;
+}
+
+/* signed ions per second through one channel, positive = efflux
+ * (ghkcurr.cpp:168-208); n_ion / vscale is the inner concentration in M */
+static double ghk_rt(const stgo_system *s, int32_t g, double A, double Bo, double Dn, int32_t n_ion,
+                     double vscale) {
+    const double iconc = (double)n_ion / vscale * 1.0e3;
+    const double flux = A * (iconc - Bo) / Dn;
+    return flux / (E_CHARGE * (double)s->ghk_valence[g]);
+}
+
 /* h = prod over reactants of n(n-1)...(n-lhs+1): reactions.cpp:66-109 */
 static double falling(double h, int32_t n, int32_t lhs) {
     for (int32_t i = lhs - 1; i >= 0; --i) h = h * (double)(n - i);
@@ -391,6 +449,8 @@ static void topo_build(const stgo_system *s, topo *tp) {
             if (s->reac_upd[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
         for (int32_t r = 0; r < s->n_sreacs; ++r)
             if (s->sreac_upd_i[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
+        for (int32_t g = 0; g < s->n_ghk; ++g)
+            if (s->ghk_ion[g] == sp && s->ghk_realflux[g]) tp->tracked[sp] = 1;
     }
 }
 
@@ -407,7 +467,8 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
     const int32_t S = s->n_specs, SS = s->n_sspecs;
     const int32_t R = s->n_reacs, RS = s->n_sreacs;
     const int32_t t0 = tp->tri_ptr[t], nt = tp->tri_ptr[t + 1] - t0;
-    const int32_t nk = R + nt * RS;
+    const int32_t NG = s->n_ghk, RT = RS + NG; /* entries per triangle */
+    const int32_t nk = R + nt * RT;
     if (nk == 0) return 0;
     const double V = s->tet_vol[t];
     const double vscale = 1.0e3 * V * AVOGADRO;
@@ -467,6 +528,14 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                     a[j] = h * scaled_const(k, order, scale);
                 }
             }
+            /* GHK currents: |rt| * open channels (ghkcurr.cpp:168-208) */
+            for (int32_t g = 0; g < NG; ++g, ++j) {
+                double A, Bo, Dn;
+                if (!ghk_consts(s, g, s->tri_v[tr], &A, &Bo, &Dn)) return -3;
+                const int32_t nch = s->tri_counts[(size_t)s->ghk_chan[g] * s->n_tris + tr];
+                const double rt = ghk_rt(s, g, A, Bo, Dn, cnt[s->ghk_ion[g]], vscale);
+                a[j] = fabs(rt) * (double)nch;
+            }
         }
         double acc = 0.0;
         for (int32_t q = 0; q < nk; ++q) {
@@ -506,8 +575,24 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 cnt[sp] += upd[sp];
                 if (tp->tracked[sp]) corr[sp] -= (double)upd[sp] * tcur;
             }
+        } else if ((sel - R) % RT >= RS) {
+            /* GHK: one ion in the direction of the flux, charge counted
+             * outward positive (ghkcurr.cpp:212-272) */
+            const int32_t i = (sel - R) / RT, g = (sel - R) % RT - RS;
+            const int32_t tr = tp->tri_idx[t0 + i];
+            const int32_t ion = s->ghk_ion[g];
+            double A, Bo, Dn;
+            ghk_consts(s, g, s->tri_v[tr], &A, &Bo, &Dn);
+            const double rt = ghk_rt(s, g, A, Bo, Dn, cnt[ion], vscale);
+            const int efflux = rt >= 0.0;
+            if (s->ghk_realflux[g]) {
+                const int32_t dv = efflux ? -1 : 1;
+                cnt[ion] += dv;
+                if (tp->tracked[ion]) corr[ion] -= (double)dv * tcur;
+            }
+            s->tri_charge[tr] += efflux ? s->ghk_valence[g] : -s->ghk_valence[g];
         } else {
-            const int32_t i = (sel - R) / RS, r = (sel - R) % RS;
+            const int32_t i = (sel - R) / RT, r = (sel - R) % RT;
             const int32_t tr = tp->tri_idx[t0 + i];
             const int32_t *us_ = s->sreac_upd_s + r * SS;
             const int32_t *ui = s->sreac_upd_i + r * S;
@@ -606,7 +691,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
     const int32_t N = s->n_tets, S = s->n_specs;
     topo tp;
     topo_build(s, &tp);
-    const int32_t nk_max = s->n_reacs + tp.max_tris * s->n_sreacs + 1;
+    const int32_t nk_max = s->n_reacs + tp.max_tris * (s->n_sreacs + s->n_ghk) + 1;
     int32_t err = 0;
     uint64_t ev_total = 0, mv_total = 0;
     for (int32_t p = 0; p < n_periods && !err; ++p) {
@@ -627,7 +712,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                 }
                 long nev = ssa_tet(s, &tp, t, dt, step, cnt, corr, a, ps);
                 if (nev < 0) {
-                    err |= nev == -2 ? 2 : 1;
+                    err |= nev == -2 ? 2 : nev == -3 ? 4 : 1;
                     nev = 0;
                 }
                 if (s->tri_intg_mask != 0) {
@@ -673,6 +758,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
     topo_free(&tp);
     if (n_events) *n_events += ev_total;
     if (n_moved) *n_moved += mv_total;
-    /* -1: event cap hit; -2: a triangle potential outside a k(V) table */
-    return (err & 1) ? -1 : (err & 2) ? -2 : 0;
+    /* -1: event cap hit; -2: a triangle potential outside a k(V) table;
+     * -3: GHK exponent out of range */
+    return (err & 1) ? -1 : (err & 2) ? -2 : (err & 4) ? -3 : 0;
 }

@@ -62,6 +62,19 @@ typedef struct stgo_system {
      * channel counts (tri.cpp:544-561, 664-693); may be null when the mask is 0 */
     uint32_t tri_intg_mask;
     double  *tri_intg;
+    /* GHK currents (ghkcurr.cpp:168-272) with a virtual outer concentration;
+     * tri_charge [n_tris] accumulates the charge moved out (elementary
+     * charges); may be null when n_ghk is 0 */
+    int32_t n_ghk;
+    const int32_t *ghk_chan;     /* [n_ghk] surface species */
+    const int32_t *ghk_ion;      /* [n_ghk] volume species */
+    const int32_t *ghk_valence;
+    const double  *ghk_perm;     /* m^3/s */
+    const double  *ghk_voconc;   /* M */
+    const double  *ghk_vshift;   /* V */
+    const int32_t *ghk_realflux;
+    double temp;                 /* K */
+    int32_t *tri_charge;
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */
@@ -69,6 +82,7 @@ void stgo_philox4x32_8(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 
 /* Deterministic scalar helpers (exposed for the tests). */
 double stgo_log(double x);
+double stgo_exp(double x);
 double stgo_sqrt(double x);
 double stgo_lfact(int32_t k);
 /* Binomial(n, p) drawn from the stream (seed, ctr0..2, channel). */

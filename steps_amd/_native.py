@@ -63,6 +63,14 @@ class StgModel(C.Structure):
         ("vdep_dv", _dp),
         ("vdep_off", _ip),
         ("vdep_k", _dp),
+        ("n_ghk", C.c_int32),
+        ("ghk_chan", _ip),
+        ("ghk_ion", _ip),
+        ("ghk_valence", _ip),
+        ("ghk_perm", _dp),
+        ("ghk_voconc", _dp),
+        ("ghk_vshift", _dp),
+        ("ghk_realflux", _ip),
     ]
 
 
@@ -123,6 +131,8 @@ SIGNATURES = {
     "stg_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "stg_set_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_get_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_set_temp": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_get_temp": (C.c_int, [C.c_void_p, _dp]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),
     "stg_efield_create": (C.c_int, [C.POINTER(StgEfieldMesh), C.POINTER(C.c_void_p)]),

@@ -135,13 +135,16 @@ struct OhmicK {
 __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, const int *tri_v,
                                  const double *V, const double *tri_clamp, const double *vert_clamp,
                                  const int *tri_counts, const double *intg, double span,
-                                 const int *tri_map, int n_tris_sim, OhmicK oc, double *I) {
+                                 const int *charge, const int *tri_map, int n_tris_sim, OhmicK oc,
+                                 double *I) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double cur = vert_clamp[i];
     for (int e = vt_ptr[i]; e < vt_ptr[i + 1]; ++e) {
         const int t = vt_tri[e];
         double c = tri_clamp[t];
+        // GHK charge moved out over the span, as a mean current (Tri::computeI)
+        if (charge != nullptr) c -= (double)charge[tri_map[t]] * 1.602176487e-19 / span * 1e12;
         if (oc.n > 0 && tri_counts != nullptr) {
             const double vt = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0;
             double im = 0.0;
@@ -389,6 +392,11 @@ __global__ void k_tri_v(int nt, const int *tri_v, const double *V, const int *tr
     if (t < nt) out[tri_map[t]] = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0 * 1e-3;
 }
 
+__global__ void k_zero_charge(int nt, const int *tri_map, int *charge) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nt) charge[tri_map[t]] = 0;
+}
+
 // restart the time integrals of the channel species on this EField's triangles
 __global__ void k_zero_intg(int nt, const int *tri_map, int n_tris_sim, uint32_t mask, double *intg) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -432,6 +440,7 @@ struct stg_efield {
     int sim_real = 0;
     double *intg = nullptr;                 // attached simulation's channel-count integrals
     uint32_t intg_mask = 0;
+    int32_t *charge = nullptr;              // attached simulation's GHK charges (this realization)
     double intg_t0 = 0.0;                   // simulation time the integrals started from
     std::string err;
     hipStream_t stream = nullptr;
@@ -470,6 +479,7 @@ extern "C" int stg_internal_tri_perm(const stg_sim *sim, int32_t real, const int
                                      int32_t *n);
 extern "C" int stg_internal_tri_v_device(stg_sim *sim, int32_t real, double **ptr);
 extern "C" int stg_internal_tri_intg_device(stg_sim *sim, uint32_t mask, double **ptr);
+extern "C" int stg_internal_tri_charge_device(stg_sim *sim, int32_t real, int32_t **ptr);
 
 namespace {
 
@@ -726,7 +736,19 @@ void publish_tri_v(stg_efield *e) {
 void start_integrals(stg_efield *e) {
     e->intg = nullptr;
     e->intg_mask = 0;
-    if (!e->sim || e->oc.n == 0) return;
+    e->charge = nullptr;
+    if (!e->sim) return;
+    int rc0 = stg_get_time(e->sim, &e->intg_t0);
+    if (rc0 != STG_OK) throw EfError{rc0, stg_last_error(e->sim)};
+    rc0 = stg_internal_tri_charge_device(e->sim, e->sim_real, &e->charge);
+    if (rc0 != STG_OK) throw EfError{rc0, stg_last_error(e->sim)};
+    if (e->charge && e->n_tris > 0) {
+        hipLaunchKernelGGL(k_zero_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
+                           e->tri_map, e->charge);
+        EF_HIP(hipGetLastError());
+        EF_HIP(hipStreamSynchronize(e->stream));
+    }
+    if (e->oc.n == 0) return;
     uint32_t mask = 0;
     for (int k = 0; k < e->oc.n; ++k) mask |= 1u << e->oc.sspec[k];
     int rc = stg_internal_tri_intg_device(e->sim, mask, &e->intg);
@@ -736,8 +758,6 @@ void start_integrals(stg_efield *e) {
     rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tc, &ntot, &tri0);
     if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
     e->intg_mask = mask;
-    rc = stg_get_time(e->sim, &e->intg_t0);
-    if (rc != STG_OK) throw EfError{rc, stg_last_error(e->sim)};
     hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
                        e->tri_map, ntot, mask, e->intg + tri0);
     EF_HIP(hipGetLastError());
@@ -750,25 +770,34 @@ void advance_impl(stg_efield *e, double dt_s) {
     const EfDev m = e->dev();
     const int32_t *tri_counts = nullptr;
     const double *intg = nullptr;
+    const int32_t *charge = nullptr;
     double span = 0.0, now = 0.0;
     int32_t ntot = 0, tri0 = 0;
-    if (e->sim && e->oc.n > 0) {
-        int rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tri_counts, &ntot, &tri0);
-        if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
-        tri_counts += tri0;
-        rc = stg_get_time(e->sim, &now);
+    if (e->sim) {
+        int rc = stg_get_time(e->sim, &now);
         if (rc != STG_OK) throw EfError{rc, stg_last_error(e->sim)};
         span = now - e->intg_t0;
-        // no simulated time since the last step (or the clock was reset): the
-        // present counts stand in for the average
-        if (e->intg && span > 0.0) intg = e->intg + tri0;
+        // GHK charge is turned into a current only over simulated time
+        if (e->charge && span > 0.0) charge = e->charge;
+        if (e->oc.n > 0) {
+            rc = stg_internal_tri_counts_device(e->sim, e->sim_real, &tri_counts, &ntot, &tri0);
+            if (rc != STG_OK) throw EfError{rc, "attached simulation has no triangle counts"};
+            tri_counts += tri0;
+            // no simulated time since the last step (or the clock was reset):
+            // the present counts stand in for the average
+            if (e->intg && span > 0.0) intg = e->intg + tri0;
+        }
     }
     hipLaunchKernelGGL(k_vertex_current, dim3(nb), dim3(kB), 0, e->stream, n, e->vt_ptr, e->vt_tri,
-                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, intg, span, e->tri_map, ntot,
-                       e->oc, e->I);
-    if (e->intg) {
-        hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
-                           e->tri_map, ntot, e->intg_mask, e->intg + tri0);
+                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, intg, span, charge, e->tri_map,
+                       ntot, e->oc, e->I);
+    if (e->sim) {
+        if (e->intg)
+            hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream,
+                               e->n_tris, e->tri_map, ntot, e->intg_mask, e->intg + tri0);
+        if (e->charge)
+            hipLaunchKernelGGL(k_zero_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream,
+                               e->n_tris, e->tri_map, e->charge);
         e->intg_t0 = now;
     }
     const double oodt = 1.0 / (dt_s * 1e3);

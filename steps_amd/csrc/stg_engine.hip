@@ -96,6 +96,7 @@ struct stg_sim {
     std::vector<int32_t> block_real, block_tet0;
     std::vector<std::vector<int32_t>> tri_perm;  // per realization: user tri -> internal tri
     int32_t NT = 0, NTRI = 0, max_tris = 0;
+    int32_t NG = 0;  // GHK currents
     std::vector<double> h_k, h_sk, h_D;  // per-realization tables
     std::vector<double> h_face_g;         // [NT*4], for update-period recomputation
     std::vector<double> vdep_vmin, vdep_dv, vdep_k;
@@ -125,6 +126,7 @@ struct stg_sim {
     unsigned long long *d_ext = nullptr;
     double *d_tri_v = nullptr;
     double *d_tri_intg = nullptr;  // [SS][NTRI] time integrals of channel counts, lazily allocated
+    int32_t *d_tri_charge = nullptr;  // [NTRI] GHK charge, when the model has GHK currents
     double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
     int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
@@ -143,9 +145,10 @@ struct stg_sim {
     std::string err;
 
     size_t smem_bytes(int B) const {
-        const int K = R + max_tris * RS;
+        const int K = R + max_tris * (RS + NG);
         size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
-        return off + (size_t)S * B * 8 + 2 * (size_t)(K > 0 ? K : 1) * B * 8;
+        return off + (size_t)S * B * 8 + 2 * (size_t)(K > 0 ? K : 1) * B * 8 +
+               2 * (size_t)max_tris * NG * B * 8;
     }
 
     double compute_dt_std(int r) const {
@@ -178,7 +181,7 @@ struct stg_sim {
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
-                        d_vdep_off,  d_tri_intg};
+                        d_vdep_off,  d_tri_intg,    d_tri_charge};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -304,6 +307,29 @@ void build_model(stg_sim *s, const stg_model *m) {
         if (t < -1 || t >= nv) throw StgError{STG_ERR_ARG, "k(V) table index out of range"};
         k.s_vdep[r] = (int8_t)t;
     }
+    // GHK currents
+    if (m->n_ghk < 0 || m->n_ghk > stg::kMaxGhk) throw StgError{STG_ERR_ARG, "n_ghk must be in [0, 4]"};
+    k.n_ghk = m->n_ghk;
+    s->NG = m->n_ghk;
+    for (int g = 0; g < m->n_ghk; ++g) {
+        if (m->ghk_chan[g] < 0 || m->ghk_chan[g] >= s->SS)
+            throw StgError{STG_ERR_ARG, "GHK channel state must be a surface species"};
+        if (m->ghk_ion[g] < 0 || m->ghk_ion[g] >= s->S)
+            throw StgError{STG_ERR_ARG, "GHK ion must be a volume species"};
+        if (m->ghk_valence[g] == 0 || m->ghk_valence[g] < -127 || m->ghk_valence[g] > 127)
+            throw StgError{STG_ERR_ARG, "GHK ion valence must be nonzero"};
+        if (!(m->ghk_perm[g] > 0.0)) throw StgError{STG_ERR_ARG, "GHK permeability must be > 0"};
+        if (!(m->ghk_voconc[g] >= 0.0))
+            throw StgError{STG_ERR_NOTIMPL, "GHK currents need a virtual outer concentration >= 0"};
+        k.g_chan[g] = (int8_t)m->ghk_chan[g];
+        k.g_ion[g] = (int8_t)m->ghk_ion[g];
+        k.g_z[g] = (int8_t)m->ghk_valence[g];
+        k.g_real[g] = m->ghk_realflux[g] ? 1 : 0;
+        k.g_perm[g] = m->ghk_perm[g];
+        k.g_oconc[g] = m->ghk_voconc[g];
+        k.g_vshift[g] = m->ghk_vshift[g];
+    }
+    k.temp = 293.15;
 }
 
 // Diffusing species and occupancy tracking, from the union of all realizations'
@@ -320,6 +346,7 @@ void build_diffusion_sets(stg_sim *s, const stg_model *m) {
         bool reacts = false;
         for (int r = 0; r < s->R; ++r) reacts = reacts || m->reac_upd[r * s->S + sp] != 0;
         for (int r = 0; r < s->RS; ++r) reacts = reacts || m->sreac_upd_i[r * s->S + sp] != 0;
+        for (int g = 0; g < m->n_ghk; ++g) reacts = reacts || (m->ghk_ion[g] == sp && m->ghk_realflux[g]);
         if (reacts) k.tracked_mask |= 1 << sp;
     }
     s->ND = k.n_diff;
@@ -507,6 +534,10 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     HIP_OK(hipMemset(s->d_counts, 0, sizeof(int32_t) * (size_t)std::max(S, 1) * NT));
     HIP_OK(hipMemset(s->d_tri_counts, 0, sizeof(int32_t) * (size_t)std::max(SS, 1) * std::max(NTRI, 1)));
     HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
+    if (s->NG > 0) {
+        s->d_tri_charge = dalloc<int32_t>(std::max(NTRI, 1));
+        HIP_OK(hipMemset(s->d_tri_charge, 0, sizeof(int32_t) * std::max(NTRI, 1)));
+    }
     s->d_tri_v = dalloc<double>(std::max(NTRI, 1));
     {
         std::vector<double> v0(std::max(NTRI, 1), -65.0e-3);  // DEFAULT_MEMB_POT
@@ -557,7 +588,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.xcd_remap = s->xcd_remap ? 1 : 0;
     k.n_blocks = (int32_t)s->block_real.size();
     k.diff_threshold = s->diff_threshold;
-    k.k_max = std::max(1, s->R + s->max_tris * s->RS);
+    k.k_max = std::max(1, s->R + s->max_tris * (s->RS + s->NG));
     k.vol = s->d_vol;
     k.nbr = s->d_nbr;
     k.nface = s->d_nface;
@@ -565,7 +596,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.counts = s->d_counts;
     k.out_prev = s->d_out[launch_parity ^ 1];
     k.out_cur = s->d_out[launch_parity];
-    k.tri_ptr = (s->NTRI > 0 && s->RS > 0) ? s->d_tri_ptr : nullptr;
+    k.tri_ptr = s->NTRI > 0 ? s->d_tri_ptr : nullptr;
     k.tri_counts = s->d_tri_counts;
     k.tri_area = s->d_tri_area;
     k.block_real = s->d_block_real;
@@ -587,6 +618,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.vdep_off = s->d_vdep_off;
     k.vdep_k = s->d_vdep_k;
     k.tri_intg = s->d_tri_intg;
+    k.tri_charge = s->d_tri_charge;
     return k;
 }
 
@@ -673,7 +705,8 @@ void run_impl(stg_sim *s, double endtime) {
         if (err) {
             HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
             if (err & 1) throw StgError{STG_ERR_PROG, "SSA exceeded 4194304 events in one tet and period"};
-            throw StgError{STG_ERR_PROG, "triangle potential outside a voltage-dependent rate table"};
+            if (err & 2) throw StgError{STG_ERR_PROG, "triangle potential outside a voltage-dependent rate table"};
+            throw StgError{STG_ERR_PROG, "GHK flux overflow: |z V F / (R T)| too large"};
         }
         for (int r = 0; r < s->n_real; ++r) s->nsteps[r] += dts[r].size();
     }
@@ -732,6 +765,8 @@ int stg_reset(stg_sim *s) {
             HIP_OK(hipMemsetAsync(s->d_ext, 0, sizeof(unsigned long long) * 2 * s->block_real.size(), s->stream));
         if (s->d_tri_intg)  // Tri::reset clears the ohmic-current integrals (tri.cpp:441)
             HIP_OK(hipMemsetAsync(s->d_tri_intg, 0, sizeof(double) * (size_t)s->SS * s->NTRI, s->stream));
+        if (s->d_tri_charge)  // ... and the GHK charges (tri.cpp:435)
+            HIP_OK(hipMemsetAsync(s->d_tri_charge, 0, sizeof(int32_t) * (size_t)s->NTRI, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         s->time = 0.0;
         std::fill(s->nsteps.begin(), s->nsteps.end(), 0);
@@ -1159,6 +1194,34 @@ int stg_internal_tri_v_device(stg_sim *s, int32_t real, double **ptr) {
     return guard(s, [&] {
         check_real(s, real);
         *ptr = s->d_tri_v + s->tri_off[real];
+    });
+}
+
+int stg_set_temp(stg_sim *s, double kelvin) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (!(kelvin >= 0.0)) throw StgError{STG_ERR_ARG, "temperature must be >= 0 K"};
+        if (s->mk.temp != kelvin) {
+            s->mk.temp = kelvin;
+            s->free_graph();
+        }
+    });
+}
+
+int stg_get_temp(const stg_sim *s, double *kelvin) {
+    if (!s || !kelvin) return STG_ERR_ARG;
+    *kelvin = s->mk.temp;
+    return STG_OK;
+}
+
+// Device pointer of the GHK charge of each triangle of one realization
+// (internal order), null when the model has no GHK currents; for the EField
+// (not part of the public header).
+int stg_internal_tri_charge_device(stg_sim *s, int32_t real, int32_t **ptr) {
+    if (!s || !ptr) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        *ptr = s->d_tri_charge ? s->d_tri_charge + s->tri_off[real] : nullptr;
     });
 }
 

@@ -31,10 +31,14 @@ constexpr int kMaxReacs = 32;
 constexpr int kMaxSSpecs = 8;
 constexpr int kMaxSReacs = 16;
 constexpr int kMaxEntries = 384;
+constexpr int kMaxGhk = 4;
 constexpr int kSurfFlag = 0x80;  // entry species >= 0x80: surface species (idx & 0x7f)
 constexpr long kMaxEventsPerPeriod = 1L << 22;
 constexpr double kAvogadro = 6.02214179e23;  // src/steps/math/constants.hpp:45
 constexpr double kDblEpsilon = 2.220446049250313080847e-16;
+constexpr double kFaraday = 96485.3399;        // src/steps/math/constants.hpp:49
+constexpr double kGasConstant = 8.314472;      // src/steps/math/constants.hpp:53
+constexpr double kECharge = 1.602176487e-19;   // src/steps/math/constants.hpp:57
 
 // Model tables, passed by value (kernel-argument segment, read with scalar loads).
 struct ModelK {
@@ -50,6 +54,12 @@ struct ModelK {
     uint16_t slhs_off[kMaxSReacs + 1], supd_off[kMaxSReacs + 1];
     uint8_t ent_spec[kMaxEntries];
     int8_t ent_val[kMaxEntries];
+    // GHK currents: channel-state surface species, inner ion, valence, whether
+    // ions move; permeability (m/s), virtual outer concentration (M), V shift (V)
+    int32_t n_ghk;
+    int8_t g_chan[kMaxGhk], g_ion[kMaxGhk], g_z[kMaxGhk], g_real[kMaxGhk];
+    double g_perm[kMaxGhk], g_oconc[kMaxGhk], g_vshift[kMaxGhk];
+    double temp;  // K
 };
 
 // Mesh, state and per-realization tables (device pointers).
@@ -83,13 +93,15 @@ struct SysK {
     const int32_t *real_nper;    // [R] periods in this run
     const double *dt_tab;        // [L][R]
     unsigned long long *ext;     // [n_blocks][2] events, molecules moved
-    int32_t *err;                // error flags: 1 event cap, 2 potential outside a k(V) table
+    int32_t *err;                // error flags: 1 event cap, 2 potential outside a k(V) table,
+                                 // 4 GHK exponent out of range
     const double *tri_v;         // [NTRI] membrane potential of each triangle (V)
     const double *vdep_vmin;     // [n_vdep]
     const double *vdep_dv;       // [n_vdep]
     const int32_t *vdep_off;     // [n_vdep+1]
     const double *vdep_k;        // concatenated k(V) tables
     double *tri_intg;            // [SS][NTRI] time integral of counts (tri_intg_mask species)
+    int32_t *tri_charge;         // [NTRI] GHK charge moved out (elementary charges), or null
 };
 
 __device__ __forceinline__ double scaled_const(double k, int order, double scale) {
@@ -112,12 +124,46 @@ struct Lds {
     double *corr;   // [S][B]
     double *a;      // [K][B] propensities
     double *kc;     // [K][B] scaled rate constants of this tet's reactions
+    double *gx;     // [2 * triangles * GHK currents][B] GHK flux constants
     int B, tid;
     __device__ __forceinline__ int32_t &c(int s) const { return cnt[s * B + tid]; }
     __device__ __forceinline__ double &o(int s) const { return corr[s * B + tid]; }
     __device__ __forceinline__ double &p(int j) const { return a[j * B + tid]; }
     __device__ __forceinline__ double &c(int j, int) const { return kc[j * B + tid]; }
+    __device__ __forceinline__ double &g(int j, int w) const { return gx[(2 * j + w) * B + tid]; }
 };
+
+// GHK flux of one open channel (math/ghk.cpp:62-81), split for a fixed
+// potential into P z F nuFoRT, c_o e^-nuFoRT and 1 - e^-nuFoRT so that the
+// flux is (A (c_i - Bo)) / Dn, the reference's left-to-right evaluation.
+// Returns false when nuFoRT is outside the range the reference accepts.
+__device__ __forceinline__ bool ghk_consts(const ModelK &m, int g, double v, double &A, double &Bo,
+                                           double &Dn) {
+    const double z = (double)m.g_z[g];
+    const double nuFoRT = z * (v + m.g_vshift[g]) * kFaraday / (kGasConstant * m.temp);
+    if (nuFoRT >= 308 * 2.30 || nuFoRT <= -307 * 2.30) return false;
+    const double e = dexp(-nuFoRT);
+    const double oconc = m.g_oconc[g] * 1.0e3;  // mol/m^3
+    if (fabs(nuFoRT) > kDblEpsilon) {
+        A = m.g_perm[g] * z * kFaraday * nuFoRT;
+        Bo = oconc * e;
+        Dn = 1.0 - e;
+    } else {  // the limit nuFoRT -> 0
+        A = m.g_perm[g] * z * kFaraday;
+        Bo = oconc;
+        Dn = 1.0;
+    }
+    return true;
+}
+
+// ions per second through one channel, signed: positive is an efflux
+// (ghkcurr.cpp:168-208); n_ion / vscale is the inner concentration in M
+__device__ __forceinline__ double ghk_rt(const ModelK &m, int g, double A, double Bo, double Dn,
+                                         int32_t n_ion, double vscale) {
+    const double iconc = (double)n_ion / vscale * 1.0e3;
+    const double flux = A * (iconc - Bo) / Dn;
+    return flux / (kECharge * (double)m.g_z[g]);
+}
 
 // k(V) by linear interpolation in a table (VDepSReacdef::getVDepK,
 // src/steps/solver/vdepsreacdef.cpp:194-217); sets error bit 2 outside it.
@@ -156,13 +202,14 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 // the per-period event cap is hit.
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
                           uint32_t ltet, uint32_t period, double dt) {
-    const int R = m.n_reacs, RS = m.n_sreacs;
+    const int R = m.n_reacs, RS = m.n_sreacs, NG = m.n_ghk;
+    const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
-    if (sys.tri_ptr != nullptr && RS > 0) {
+    if (sys.tri_ptr != nullptr && RT > 0) {
         tp0 = sys.tri_ptr[t];
         ntri = sys.tri_ptr[t + 1] - tp0;
     }
-    const int K = R + ntri * RS;
+    const int K = R + ntri * RT;
     if (K == 0) return 0;
     const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
     const double *kr = sys.real_k + (size_t)r * R;
@@ -176,6 +223,14 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             for (int q = 0; q < RS; ++q, ++j) {
                 const double k = m.s_vdep[q] < 0 ? ksr[q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tp0 + i]);
                 L.c(j, 0) = scaled_const(k, m.sorder[q], m.s_nvol[q] == 0 ? ascale : vscale);
+            }
+            for (int q = 0; q < NG; ++q, ++j) {
+                if (!ghk_consts(m, q, sys.tri_v[tp0 + i], L.c(j, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1))) {
+                    atomicOr(sys.err, 4);
+                    L.c(j, 0) = 0.0;
+                    L.g(i * NG + q, 0) = 0.0;
+                    L.g(i * NG + q, 1) = 1.0;
+                }
             }
         }
     }
@@ -207,6 +262,14 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             for (int q = 0; q < RS; ++q, ++j) {
                 const double h = surf_h(m, sys, L, q, tp0 + i);
                 const double av = h == 0.0 ? 0.0 : h * L.c(j, 0);
+                L.p(j) = av;
+                a0 = a0 + av;
+            }
+            for (int q = 0; q < NG; ++q, ++j) {
+                const int32_t nch = sys.tri_counts[(size_t)m.g_chan[q] * sys.n_tris + tp0 + i];
+                const double rt = ghk_rt(m, q, L.c(j, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1),
+                                         L.c(m.g_ion[q]), vscale);
+                const double av = fabs(rt) * (double)nch;
                 L.p(j) = av;
                 a0 = a0 + av;
             }
@@ -245,9 +308,24 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                 L.c(sp) += dv;
                 if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
             }
+        } else if ((sel - R) % RT >= RS) {
+            // GHK current: one ion crosses in the direction of the flux
+            // (ghkcurr.cpp:212-272); the triangle's charge counts it outward positive
+            const int i = (sel - R) / RT;
+            const int q = (sel - R) - i * RT - RS;
+            const int ion = m.g_ion[q];
+            const double rt = ghk_rt(m, q, L.c(sel, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1), L.c(ion),
+                                     vscale);
+            const bool efflux = rt >= 0.0;
+            if (m.g_real[q]) {
+                const int dv = efflux ? -1 : 1;
+                L.c(ion) += dv;
+                if ((m.tracked_mask >> ion) & 1) L.o(ion) -= (double)dv * tcur;
+            }
+            sys.tri_charge[tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
         } else {
-            const int i = (sel - R) / RS;
-            const int q = (sel - R) - i * RS;
+            const int i = (sel - R) / RT;
+            const int q = (sel - R) - i * RT;
             const int tri = tp0 + i;
             for (int e = m.supd_off[q]; e < m.supd_off[q + 1]; ++e) {
                 const int sp = m.ent_spec[e];
@@ -378,6 +456,7 @@ __global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
     L.corr = reinterpret_cast<double *>(smem + (((size_t)S * B * 4 + 15) & ~(size_t)15));
     L.a = L.corr + (size_t)S * B;
     L.kc = L.a + (size_t)sys.k_max * B;
+    L.gx = L.kc + (size_t)sys.k_max * B;
 
     const int t = sys.block_tet0[b] + tid;
     const int tend = sys.real_tet0[r + 1];

@@ -112,6 +112,31 @@ __device__ __forceinline__ double dlog(double x) {
     return de * 0x1.62e42fee00000p-1 + (lm + de * 0x1.a39ef35793c76p-33);
 }
 
+// exp(x): x = k ln2 + r with |r| <= ln2/2 (ln2 split so that k * hi is exact),
+// e^r by its Taylor series to r^13 (truncation below 5e-18 relative), then
+// scaled by 2^k. The oracle evaluates the same operations in the same order.
+__device__ __forceinline__ double dexp(double x) {
+    if (!(x < 709.782712893384)) return x != x ? x : __builtin_inf();
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = floor(x * 0x1.71547652b82fep+0 + 0.5);
+    const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
+    double p = 0x1.6124613a86d09p-33;
+    p = p * r + 0x1.1eed8eff8d898p-29;
+    p = p * r + 0x1.ae64567f544e4p-26;
+    p = p * r + 0x1.27e4fb7789f5cp-22;
+    p = p * r + 0x1.71de3a556c734p-19;
+    p = p * r + 0x1.a01a01a01a01ap-16;
+    p = p * r + 0x1.a01a01a01a01ap-13;
+    p = p * r + 0x1.6c16c16c16c17p-10;
+    p = p * r + 0x1.1111111111111p-7;
+    p = p * r + 0x1.5555555555555p-5;
+    p = p * r + 0x1.5555555555555p-3;
+    p = p * r + 0.5;
+    p = p * r + 1.0;
+    p = p * r + 1.0;
+    return __builtin_ldexp(p, (int)kd);
+}
+
 // IEEE-754 square root: correctly rounded on gfx950 as on any conforming host.
 __device__ __forceinline__ double dsqrt(double x) { return x > 0.0 ? __builtin_sqrt(x) : 0.0; }
 

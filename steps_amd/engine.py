@@ -28,15 +28,22 @@ def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
     arrs = [N.as_i32(fm.reac_lhs), N.as_i32(fm.reac_upd), N.as_f64(fm.reac_k), N.as_f64(fm.spec_dcst),
             N.as_i32(fm.sreac_lhs_s), N.as_i32(fm.sreac_upd_s), N.as_i32(fm.sreac_lhs_i),
             N.as_i32(fm.sreac_upd_i), N.as_f64(fm.sreac_k), N.as_i32(fm.sreac_vdep),
-            N.as_f64(fm.vdep_vmin), N.as_f64(fm.vdep_dv), N.as_i32(fm.vdep_off), N.as_f64(fm.vdep_k)]
+            N.as_f64(fm.vdep_vmin), N.as_f64(fm.vdep_dv), N.as_i32(fm.vdep_off), N.as_f64(fm.vdep_k),
+            N.as_i32(fm.ghk_chan), N.as_i32(fm.ghk_ion), N.as_i32(fm.ghk_valence),
+            N.as_f64(fm.ghk_perm), N.as_f64(fm.ghk_voconc), N.as_f64(fm.ghk_vshift),
+            N.as_i32(fm.ghk_realflux)]
     keep.extend(arrs)
-    rl, ru, rk, dc, sls, sus, sli, sui, sk, svd, vmin, vdv, voff, vk = arrs
+    (rl, ru, rk, dc, sls, sus, sli, sui, sk, svd, vmin, vdv, voff, vk,
+     gc, gi, gz, gp, go, gv, gr) = arrs
     return N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
                       N.ptr(rk, C.c_double), N.ptr(dc, C.c_double), len(fm.sspecs), len(fm.sreacs),
                       N.ptr(sls, C.c_int32), N.ptr(sus, C.c_int32), N.ptr(sli, C.c_int32),
                       N.ptr(sui, C.c_int32), N.ptr(sk, C.c_double), N.ptr(svd, C.c_int32),
                       len(fm.vdep_vmin), N.ptr(vmin, C.c_double), N.ptr(vdv, C.c_double),
-                      N.ptr(voff, C.c_int32), N.ptr(vk, C.c_double))
+                      N.ptr(voff, C.c_int32), N.ptr(vk, C.c_double), len(fm.ghk_chan),
+                      N.ptr(gc, C.c_int32), N.ptr(gi, C.c_int32), N.ptr(gz, C.c_int32),
+                      N.ptr(gp, C.c_double), N.ptr(go, C.c_double), N.ptr(gv, C.c_double),
+                      N.ptr(gr, C.c_int32))
 
 
 class Engine:
@@ -182,6 +189,14 @@ class Engine:
         a = np.empty(self.n_tris[real])
         self._chk(self.lib.stg_get_tri_v(self.h, real, N.ptr(a, C.c_double)))
         return a
+
+    def set_temp(self, kelvin: float):
+        self._chk(self.lib.stg_set_temp(self.h, float(kelvin)))
+
+    def temp(self) -> float:
+        x = C.c_double()
+        self._chk(self.lib.stg_get_temp(self.h, C.byref(x)))
+        return x.value
 
     def set_diff_apply_threshold(self, threshold: int):
         self._chk(self.lib.stg_set_diff_apply_threshold(self.h, int(threshold)))

@@ -1,8 +1,9 @@
 """Kinetic model objects with the reference's legacy (API_1) constructors.
 
-``Model``, ``Spec``, ``Volsys``, ``Surfsys``, ``Reac``, ``SReac`` and ``Diff``
-take the same arguments as steps.model (pysteps/steps/API_1/model.py, backed by
-src/steps/model/*.cpp). ``flatten`` turns a model bound to one compartment and
+``Model``, ``Spec``, ``Volsys``, ``Surfsys``, ``Reac``, ``SReac``, ``VDepSReac``,
+``Diff``, ``Chan``, ``ChanState``, ``OhmicCurr`` and ``GHKcurr`` take the same
+arguments as steps.model (pysteps/steps/API_1/model.py, pysteps/cysteps_model.pyx,
+backed by src/steps/model/*.cpp). ``flatten`` turns a model bound to one compartment and
 at most one patch into the stoichiometry tables the engine reads, the role of
 Statedef/Compdef/Patchdef (src/steps/solver/compdef.hpp:252-258, 420).
 """
@@ -10,7 +11,12 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import math
+
 import numpy as np
+
+FARADAY = 96485.3399        # src/steps/math/constants.hpp:49
+GAS_CONSTANT = 8.314472     # src/steps/math/constants.hpp:53
 
 
 class Model:
@@ -51,12 +57,35 @@ class Volsys:
         model.volsys[id] = self
 
 
+class Chan:
+    """A channel: the parent of its channel states (cysteps_model.pyx:500-520)."""
+
+    def __init__(self, id: str, model: Model):
+        self.id = id
+        self.model = model
+        self.states: list[ChanState] = []
+
+
+class ChanState(Spec):
+    """A state of a channel; a surface species (cysteps_model.pyx:638-655)."""
+
+    def __init__(self, id: str, model: Model, chan: Chan):
+        super().__init__(id, model)
+        self.chan = chan
+        chan.states.append(self)
+
+    def getChan(self) -> Chan:
+        return self.chan
+
+
 class Surfsys:
     def __init__(self, id: str, model: Model):
         self.id = id
         self.model = model
         self.sreacs: dict[str, SReac] = {}
         self.diffs: dict[str, Diff] = {}
+        self.ohmiccurrs: dict[str, OhmicCurr] = {}
+        self.ghkcurrs: dict[str, GHKcurr] = {}
         model.surfsys[id] = self
 
 
@@ -127,6 +156,92 @@ class VDepSReac(SReac):
         super().__init__(id, surfsys, ilhs, slhs, olhs, irhs, srhs, orhs, kcst=0.0)
 
 
+class OhmicCurr:
+    """Ohmic current I = g (V - erev) per channel in state chanstate
+    (src/steps/model/ohmiccurr.cpp; cysteps_model.pyx:3115-3141)."""
+
+    def __init__(self, id: str, surfsys: Surfsys, chanstate: Spec, erev: float, g: float):
+        if g < 0:
+            raise ValueError("conductance must be >= 0")
+        self.id = id
+        self.surfsys = surfsys
+        self.chanstate = chanstate
+        self.erev = float(erev)
+        self.g = float(g)
+        surfsys.ohmiccurrs[id] = self
+
+    def getERev(self) -> float:
+        return self.erev
+
+    def setERev(self, erev: float) -> None:
+        self.erev = float(erev)
+
+    def getG(self) -> float:
+        return self.g
+
+    def setG(self, g: float) -> None:
+        if g < 0:
+            raise ValueError("conductance must be >= 0")
+        self.g = float(g)
+
+
+def ghk_permeability(g, v, z, temp, iconc, oconc):
+    """Single-channel permeability (m^3/s) giving conductance g (S) at
+    potential v (V), temperature temp (K) and concentrations in M: the
+    inversion of the GHK flux equation's slope, math::permeability
+    (src/steps/math/ghk.cpp:34-60)."""
+    b, c = iconc * 1000.0, oconc * 1000.0
+    d = ((0.0 - z) * FARADAY) / (GAS_CONSTANT * temp)
+    den = c * math.exp(2.0 * d * v) + ((b - c) * d * v - c - b) * math.exp(d * v) + b
+    num = math.exp(2.0 * d * v) - 2.0 * math.exp(d * v) + 1
+    eps = np.finfo(float).eps
+    a = g * (num / den) if (abs(num) > eps or abs(den) > eps) else 2.0 * g / (c + b)
+    p = (a * GAS_CONSTANT * temp) / (z ** 2.0 * FARADAY ** 2.0)
+    if math.isnan(p):
+        raise ValueError("Failed to find permeability, check parameters")
+    return p
+
+
+class GHKcurr:
+    """GHK current of ``ion`` through channels in state chanstate
+    (src/steps/model/ghkcurr.cpp:41-170; cysteps_model.pyx:3336-3562). The
+    permeability comes from setP or setPInfo. computeflux=False moves charge
+    but no ions. virtual_oconc (M) stands in for the outer compartment."""
+
+    def __init__(self, id: str, surfsys: Surfsys, chanstate: Spec, ion: Spec,
+                 computeflux: bool = True, virtual_oconc: float = -1.0, vshift: float = 0.0):
+        if ion.valence == 0:
+            raise ValueError("Ion provided to GHKcurr initializer function has valence zero")
+        self.id = id
+        self.surfsys = surfsys
+        self.chanstate = chanstate
+        self.ion = ion
+        self.computeflux = bool(computeflux)
+        self.virtual_oconc = float(virtual_oconc)
+        self.vshift = float(vshift)
+        self.P = 0.0
+        surfsys.ghkcurrs[id] = self
+
+    def setP(self, p: float) -> None:
+        if not p > 0.0:
+            raise ValueError("Permeability provided to GHKcurr::setP function can't be "
+                             "negative or zero")
+        self.P = float(p)
+
+    def getP(self) -> float:
+        return self.P
+
+    def setPInfo(self, g: float, V: float, T: float, oconc: float, iconc: float) -> None:
+        if not g > 0.0:
+            raise ValueError("Conductance provided to GHKcurr::setPInfo function can't be "
+                             "negative or zero")
+        if V == 0.0:
+            raise ValueError("Potential provided to GHKcurr::setPInfo function can't be zero.")
+        if T < 0.0 or oconc < 0.0 or iconc < 0.0:
+            raise ValueError("temperature and concentrations must be >= 0")
+        self.P = ghk_permeability(g, V, self.ion.valence, T, iconc, oconc)
+
+
 class Diff:
     """Diffusion rule of ``lig`` with constant dcst (m^2/s)."""
 
@@ -161,6 +276,18 @@ class FlatModel:
     vdep_dv: np.ndarray = None
     vdep_off: np.ndarray = None
     vdep_k: np.ndarray = None
+    ghk: list = field(default_factory=list)          # GHKcurr ids
+    ghk_chan: np.ndarray = None
+    ghk_ion: np.ndarray = None
+    ghk_valence: np.ndarray = None
+    ghk_perm: np.ndarray = None
+    ghk_voconc: np.ndarray = None
+    ghk_vshift: np.ndarray = None
+    ghk_realflux: np.ndarray = None
+    ohmic: list = field(default_factory=list)        # OhmicCurr ids
+    ohmic_chan: np.ndarray = None                    # surface species index
+    ohmic_g: np.ndarray = None                       # S per channel
+    ohmic_erev: np.ndarray = None                    # V
 
     def spec_index(self, name: str) -> int:
         try:
@@ -202,6 +329,17 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
                     "tet on both sides of the membrane")
             used_v.update(s.id for s in r.ilhs + r.irhs)
             used_s.update(s.id for s in r.slhs + r.srhs)
+        for c in ss.ohmiccurrs.values():
+            used_s.add(c.chanstate.id)
+        for c in ss.ghkcurrs.values():
+            if c.virtual_oconc < 0:
+                raise NotImplementedError(
+                    f"GHK current {c.id!r} needs an outer compartment; give it a "
+                    "virtual_oconc >= 0")
+            if not c.P > 0.0:
+                raise ValueError(f"Permeability not defined for GHK current object {c.id!r}")
+            used_s.add(c.chanstate.id)
+            used_v.add(c.ion.id)
         if ss.diffs:
             raise NotImplementedError("surface diffusion is not supported by this solver")
     order = list(model.specs)
@@ -244,4 +382,20 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.vdep_dv = np.array([r.dv for r in vd], dtype=np.float64)
     fm.vdep_off = np.concatenate([[0], np.cumsum([len(r.ktab) for r in vd])]).astype(np.int32)
     fm.vdep_k = (np.concatenate([r.ktab for r in vd]) if vd else np.zeros(0)).astype(np.float64)
+    ghk = [c for ss in ssys for c in ss.ghkcurrs.values()]
+    if len(ghk) > 4:
+        raise NotImplementedError("at most 4 GHK currents are supported")
+    fm.ghk = [c.id for c in ghk]
+    fm.ghk_chan = np.array([sspecs.index(c.chanstate.id) for c in ghk], dtype=np.int32)
+    fm.ghk_ion = np.array([specs.index(c.ion.id) for c in ghk], dtype=np.int32)
+    fm.ghk_valence = np.array([c.ion.valence for c in ghk], dtype=np.int32)
+    fm.ghk_perm = np.array([c.P for c in ghk], dtype=np.float64)
+    fm.ghk_voconc = np.array([c.virtual_oconc for c in ghk], dtype=np.float64)
+    fm.ghk_vshift = np.array([c.vshift for c in ghk], dtype=np.float64)
+    fm.ghk_realflux = np.array([int(c.computeflux) for c in ghk], dtype=np.int32)
+    ohm = [c for ss in ssys for c in ss.ohmiccurrs.values()]
+    fm.ohmic = [c.id for c in ohm]
+    fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
+    fm.ohmic_g = np.array([c.g for c in ohm], dtype=np.float64)
+    fm.ohmic_erev = np.array([c.erev for c in ohm], dtype=np.float64)
     return fm

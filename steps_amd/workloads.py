@@ -107,18 +107,20 @@ def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> 
 
 def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0,
                           ca_um=0.1, pump_density=10.0, influx=1e5,
-                          vdep_influx=False) -> Workload:
+                          vdep_influx=False, ghk_influx=False) -> Workload:
     """Ca + Buf <-> CaB (kf 1e8 /M/s, kb 10 /s), D_Ca 2e-10 m^2/s (Buf, CaB
     immobile); every boundary triangle is membrane. Pumps P (pump_density per
     um^2) run P + Ca_in -> P + Ca_out with k = kcat/Km = 50 /s / 1 uM; Ca_out
     is a surface species counting extruded ions. The x = 0 face carries one
     influx channel per triangle, Chan -> Chan + Ca_in, with k set so the face
     admits ``influx`` ions/s in total. With ``vdep_influx`` the channel is
-    voltage gated: k(V) = k_max / (1 + exp(-(V + 20 mV) / 6 mV)) over -150..100 mV."""
+    voltage gated: k(V) = k_max / (1 + exp(-(V + 20 mV) / 6 mV)) over -150..100 mV.
+    With ``ghk_influx`` the channel instead passes a GHK current of Ca (valence
+    2, virtual outer concentration 2 mM, 20 pS slope conductance at -65 mV)."""
     rng = np.random.default_rng(seed)
     mesh, comp, tris = _cube(n, side, True)
     mdl = M.Model()
-    Ca, Buf, CaB = M.Spec("Ca", mdl), M.Spec("Buf", mdl), M.Spec("CaB", mdl)
+    Ca, Buf, CaB = M.Spec("Ca", mdl, valence=2), M.Spec("Buf", mdl), M.Spec("CaB", mdl)
     P, CaOut, Chan = M.Spec("Pump", mdl), M.Spec("CaOut", mdl), M.Spec("Chan", mdl)
     vs = M.Volsys("vsys", mdl)
     M.Reac("bind", vs, lhs=[Ca, Buf], rhs=[CaB], kcst=1e8)
@@ -129,7 +131,10 @@ def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0
     face_tris = tris[np.all(np.abs(mesh.verts[mesh.tris[tris]][:, :, 0]) < 1e-3 * side / n, axis=1)]
     n_face = len(face_tris)
     k_in = influx / max(n_face, 1)
-    if vdep_influx:
+    if ghk_influx:
+        cur = M.GHKcurr("influx", ss, Chan, Ca, computeflux=True, virtual_oconc=2e-3)
+        cur.setPInfo(20e-12, -65e-3, 293.15, 2e-3, 1e-7)
+    elif vdep_influx:
         M.VDepSReac("influx", ss, slhs=[Chan], srhs=[Chan], irhs=[Ca],
                     k=lambda v: k_in / (1.0 + np.exp(-(v + 20e-3) / 6e-3)))
     else:

@@ -186,3 +186,58 @@ def _oracle_dv(ora, dt, i_amps):
     dv = spsolve(A.tocsc(), b)
     dv[ora.clamped] = 0.0
     return dv * 1e-3
+
+
+def test_ghk_charge_drives_the_potential():
+    """GHK calcium channels on the membrane: the charge each triangle moved
+    over an EField step enters as its mean current (Tri::computeI); the GHK
+    rate depends on the published triangle potentials. GPU (FP64 CG) vs
+    oracle (FP64 LU): bit-exact counts, dV within 1e-9 relative per step."""
+    mesh, _ = _cube_case(n=4, side=2e-6)
+    comp = geom.TmComp("cyto", mesh)
+    memb = mesh.getSurfTris()
+    mdl = M.Model()
+    ch = M.Chan("CaChan", mdl)
+    op = M.ChanState("Open", mdl, ch)
+    ca = M.Spec("Ca", mdl, valence=2)
+    vs = M.Volsys("vsys", mdl)
+    M.Diff("dCa", vs, ca, 2e-10)
+    ss = M.Surfsys("ssys", mdl)
+    cur = M.GHKcurr("ghk", ss, op, ca, computeflux=True, virtual_oconc=2e-3)
+    cur.setPInfo(2e-12, -65e-3, 293.15, 2e-3, 1e-7)
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    mt = geom.mesh_tables(mesh, comp, memb)
+    qo, sca = fm.sspec_index("Open"), fm.spec_index("Ca")
+    n_open = np.full(len(memb), 3, dtype=np.int32)
+    ca0 = np.full(len(mt.vol), 40, dtype=np.int32)
+    T = 5e-5
+    print("case ghk charge", flush=True)
+    eng = Engine(fm, [mt], [9])
+    eng.set_tri_counts(qo, n_open)
+    eng.set_tet_counts(sca, ca0)
+    ef = EField(mesh.verts, mesh.tets, mesh.tris[memb])
+    ef.set_solver(FP64, 1e-12)
+    ora = EFieldOracle(mesh.verts, mesh.tets, mesh.tris[memb])
+    for obj in (ef, ora):
+        obj.set_potential(-0.065)
+        obj.set_capac(0.01)
+        obj.set_volres(1.0)
+    ef.attach(eng)
+    sim = bind.OracleSim(fm, mt, seed=9)
+    sim.tri_counts[qo] = n_open
+    sim.counts[sca] = ca0
+    for step in range(1, 4):
+        sim.tri_v[:] = eng.get_tri_v()
+        assert np.allclose(sim.tri_v, ora.tri_v(), rtol=1e-12, atol=0)
+        v_prev = ora.V * 1e-3
+        ef.run(eng, step * T, T)
+        sim.run(step * T, 0)
+        assert np.array_equal(eng.get_tet_counts(sca), sim.counts[sca]), step
+        i_tri = sim.tri_charge[:len(memb)] * 1.602176487e-19 / T
+        sim.tri_charge[:] = 0
+        assert i_tri.sum() < 0   # calcium flows in
+        dv = ora.advance(T, i_tri) * 1e-3
+        got = ef.verts_v() - v_prev
+        err = np.abs(got - dv).max() / np.abs(dv).max()
+        assert err < 1e-9, (step, err, ef.stats())
+        ef.set_verts_v(ora.V * 1e-3)

@@ -225,3 +225,28 @@ def test_vdep_voltage_outside_table_is_reported():
     eng.set_tri_v(v)
     with pytest.raises(N.StgError, match="outside"):
         eng.run(w.end_time)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_ghk_membrane_bit_exact(mode):
+    """GHK calcium influx (virtual outer 2 mM) at per-triangle potentials,
+    GPU vs oracle, with the temperature changed from its default."""
+    w = W.calcium_membrane_cube(n=6, side=2e-6, seed=8, end_time=3e-4, buf_um=100.0,
+                                ca_um=20.0, ghk_influx=True)
+    print(f"case ghk membrane mode={mode}", flush=True)
+    assert len(w.fm.ghk) == 1
+    v = np.random.default_rng(4).uniform(-90e-3, 30e-3, len(w.mt.tri_tet))
+    eng = _engine(w, 55, mode)
+    eng.set_tri_v(v)
+    eng.set_temp(308.15)
+    assert eng.temp() == 308.15
+    o = _oracle(w, 55)
+    o.tri_v[:] = v
+    o.set_temp(308.15)
+    for t in (1e-4, w.end_time):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+        assert np.array_equal(_gpu_tri_counts(eng, w), o.tri_counts[:len(w.fm.sspecs)])
+    assert eng.extents() == (o.events.value, o.moved.value)
+    assert o.tri_charge.sum() < 0   # net calcium influx

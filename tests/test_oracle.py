@@ -18,6 +18,11 @@
   it, bit-exact agreement with a constant-rate reaction at exactly
   representable interpolation points, first-order decay at a nonlinear k(V),
   and the out-of-range error.
+- Channel-count time integral against the first-order closed form.
+- GHK currents: exp against exact decimal arithmetic, setPInfo's permeability
+  against the slope conductance it must reproduce, event rate and charge
+  against the textbook GHK current for cations, anions and both directions,
+  the computeflux switch, vshift and temperature.
 """
 import decimal
 import json
@@ -90,6 +95,20 @@ def test_log_within_two_ulp_of_exact():
             continue
         worst = max(worst, _ulps(got, exact))
     assert worst <= 2.0, worst
+
+
+def test_exp_within_one_ulp_of_exact():
+    lib = bind.load()
+    decimal.getcontext().prec = 40
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(-700, 700, 3000), rng.uniform(-1, 1, 2000),
+                         [0.0, 1e-300, -1e-300, 0.34657359027997264, -0.34657359027997264, 709.7]])
+    worst = 0.0
+    for x in xs:
+        exact = float(decimal.Decimal(float(x)).exp())
+        worst = max(worst, _ulps(lib.stgo_exp(float(x)), exact))
+    assert worst <= 1.0, worst
+    assert lib.stgo_exp(710.0) == math.inf and lib.stgo_exp(-746.0) == 0.0
 
 
 def test_sqrt_is_correctly_rounded():
@@ -522,6 +541,102 @@ def test_channel_count_time_integral():
     for i in range(1, 6):
         o2.run(i * T / 5, 0)
     assert np.allclose(o2.tri_intg[qc], 7 * T, rtol=1e-14)
+
+
+FARADAY, GAS_R = 96485.3399, 8.314472
+
+
+def _ghk_current(P, V, z, T, ci, co):
+    """Textbook GHK current (A) of one channel, concentrations in M."""
+    ci, co = ci * 1e3, co * 1e3
+    x = z * V * FARADAY / (GAS_R * T)
+    return P * z * z * FARADAY ** 2 * V / (GAS_R * T) * (ci - co * math.exp(-x)) / (1 - math.exp(-x))
+
+
+def test_ghk_permeability_gives_slope_conductance():
+    # setPInfo(g, V, T, oconc, iconc) picks P so that dI/dV = g at V
+    for g, V, z, ci, co in [(20e-12, -0.05, 2, 1e-7, 2e-3), (5e-12, 0.03, 1, 0.14, 0.005),
+                            (1e-12, -0.08, -1, 0.01, 0.12)]:
+        P = M.ghk_permeability(g, V, z, 293.15, ci, co)
+        h = 1e-7
+        slope = (_ghk_current(P, V + h, z, 293.15, ci, co)
+                 - _ghk_current(P, V - h, z, 293.15, ci, co)) / (2 * h)
+        assert abs(slope / g - 1) < 1e-6
+
+
+def _ghk_model(z, perm, voconc, realflux=True, vshift=0.0):
+    mdl = M.Model()
+    chan = M.Chan("ch", mdl)
+    op = M.ChanState("Open", mdl, chan)
+    ion = M.Spec("X", mdl, valence=z)
+    vs = M.Volsys("vsys", mdl)
+    M.Diff("dX", vs, ion, 0.0)
+    ss = M.Surfsys("ssys", mdl)
+    cur = M.GHKcurr("ghk", ss, op, ion, computeflux=realflux, virtual_oconc=voconc, vshift=vshift)
+    cur.setP(perm)
+    return M.flatten(mdl, ["vsys"], ["ssys"])
+
+
+@pytest.mark.parametrize("z,V,ci_n,voconc", [
+    (2, -0.065, 30, 2e-3),     # Ca influx against a large outward concentration ratio
+    (1, 0.02, 5000, 0.005),    # K efflux at a depolarised potential
+    (-1, -0.07, 3000, 0.12),   # Cl
+])
+def test_ghk_event_rate_and_charge(z, V, ci_n, voconc):
+    # over a short run (inner count nearly constant) the mean charge moved out
+    # per triangle is z * rt * channels * t with rt the textbook current / (z e)
+    perm, nch, t = 2e-19, 3, 2e-4
+    fm = _ghk_model(z, perm, voconc)
+    mt = _isolated(3000, VOL, with_tris=True)
+    o = bind.OracleSim(fm, mt, seed=31)
+    o.counts[fm.spec_index("X")] = ci_n
+    o.tri_counts[fm.sspec_index("Open")] = nch
+    o.tri_v[:] = V
+    o.run(t, 0)
+    ci = ci_n / (1e3 * VOL * AV)
+    rt = _ghk_current(perm, V, z, 293.15, ci, voconc) / (z * 1.602176487e-19)
+    q = o.tri_charge.astype(float)
+    assert np.sign(q.sum()) == np.sign(z * rt)   # charge counted outward positive
+    assert tolerable(q.mean(), z * rt * nch * t, 0.03), (q.mean(), z * rt * nch * t)
+    # every event moves one ion: ion change = -charge / z exactly, per triangle
+    assert np.array_equal(o.counts[fm.spec_index("X")] - ci_n, -(o.tri_charge[:3000] // z))
+
+
+def test_ghk_without_flux_moves_charge_only():
+    fm = _ghk_model(1, 2e-19, 0.005, realflux=False)
+    mt = _isolated(500, VOL, with_tris=True)
+    o = bind.OracleSim(fm, mt, seed=3)
+    o.counts[fm.spec_index("X")] = 4000
+    o.tri_counts[fm.sspec_index("Open")] = 3
+    o.tri_v[:] = 0.02
+    o.run(2e-4, 0)
+    assert np.all(o.counts[fm.spec_index("X")] == 4000) and o.tri_charge.sum() != 0
+
+
+def test_ghk_vshift_and_temperature_enter_the_exponent():
+    # vshift adds to V; T scales the exponent: (V=-0.0625, shift 0.015625)
+    # must match (V=-0.046875, no shift) bit for bit (all exact in binary)
+    mt = _isolated(200, VOL, with_tris=True)
+    runs = []
+    for V, shift in ((-0.0625, 0.015625), (-0.046875, 0.0)):
+        fm = _ghk_model(2, 2e-19, 2e-3, vshift=shift)
+        o = bind.OracleSim(fm, mt, seed=5)
+        o.counts[fm.spec_index("X")] = 100
+        o.tri_counts[fm.sspec_index("Open")] = 2
+        o.tri_v[:] = V
+        o.run(1e-4, 0)
+        runs.append(o.tri_charge.copy())
+    assert np.array_equal(runs[0], runs[1]) and runs[0].sum() != 0
+    fm = _ghk_model(2, 2e-19, 2e-3)
+    hot, cold = (bind.OracleSim(fm, mt, seed=5) for _ in range(2))
+    for o, T in ((hot, 310.0), (cold, 273.0)):
+        o.set_temp(T)
+        o.counts[fm.spec_index("X")] = 100
+        o.tri_counts[fm.sspec_index("Open")] = 2
+        o.tri_v[:] = -0.05
+        o.run(1e-4, 0)
+    # influx grows in magnitude as |zFV/RT| grows
+    assert hot.tri_charge.sum() > cold.tri_charge.sum()
 
 
 def test_vdep_voltage_outside_table_is_an_error():
