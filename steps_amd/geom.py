@@ -94,6 +94,7 @@ class Tetmesh:
         self.tet_face_dist = d
         self.comps: dict[str, TmComp] = {}
         self.patches: dict[str, TmPatch] = {}
+        self.membs: dict[str, Memb] = {}
 
     # --- reference-style accessors (src/steps/geom/tetmesh.hpp) ---
     def countTets(self) -> int:
@@ -175,6 +176,44 @@ class TmPatch:
 
     def getArea(self) -> float:
         return float(self.mesh.tri_area[self.tris].sum())
+
+
+class Memb:
+    """Membrane for the potential solve: the triangles of ``patches`` and the
+    conduction volume of their inner compartments plus ``supplementary_comps``
+    (src/steps/geom/memb.cpp; cysteps_geom.pyx:2905-2952). The bandwidth
+    ordering options are accepted and ignored: the GPU solve is a sparse CG."""
+
+    def __init__(self, id: str, container: Tetmesh, patches, verify: bool = False,
+                 opt_method: int = 1, search_percent: float = 100.0, opt_file_name: str = "",
+                 supplementary_comps=()):
+        if not patches:
+            raise ValueError("a membrane needs at least one patch")
+        self.id = id
+        self.mesh = container
+        self.patches = list(patches)
+        self.tris = np.concatenate([np.asarray(p.tris, dtype=np.int64) for p in self.patches])
+        if len(np.unique(self.tris)) != len(self.tris):
+            raise ValueError("a triangle belongs to more than one patch of the membrane")
+        comps = []
+        for c in [p.icomp for p in self.patches] + list(supplementary_comps):
+            if all(c is not x for x in comps):
+                comps.append(c)
+        self.comps = comps
+        self.tets = np.unique(np.concatenate([c.tets for c in comps]))
+        container.membs[id] = self
+
+    def getID(self) -> str:
+        return self.id
+
+    def getAllTriIndices(self) -> np.ndarray:
+        return self.tris.copy()
+
+    def getAllVolTetIndices(self) -> np.ndarray:
+        return self.tets.copy()
+
+    def countVertices(self) -> int:
+        return len(np.unique(self.mesh.tets[self.tets]))
 
 
 def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0)) -> Tetmesh:

@@ -62,10 +62,12 @@ class TetOpSplit:
 
     def __init__(self, model: Model, geom: Tetmesh, rng, calcMembPot: int = 0, tet_hosts=(),
                  tri_hosts=None, wm_hosts=()):
-        if calcMembPot != EF_NONE:
-            raise _notimpl("the membrane potential solve is selected through the EField "
-                           "interface of steps_amd.efield")
+        if calcMembPot not in (EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC):
+            raise _argerr(f"unknown membrane potential solver {calcMembPot}")
         self._setup(model, geom, rng)
+        if calcMembPot != EF_NONE:
+            # every EField solver choice maps to the GPU preconditioned CG
+            self._setup_efield(geom)
 
     # --- construction ---
     def _setup(self, model, geom, rng):
@@ -91,6 +93,44 @@ class TetOpSplit:
         self._seed = int(getattr(rng, "seed", 0))
         self.eng = Engine(self.fm, [self.mt], [self._seed], mode=self._mode)
         self._host_rng = np.random.default_rng([self._seed, 0x5EED])
+        self.ef = None
+
+    def _setup_efield(self, geom):
+        """Membrane potential on the membrane's conduction volume
+        (TetOpSplitP::_setupEField, tetopsplit.cpp:1080-1170)."""
+        from .efield import EField
+
+        membs = list(geom.membs.values())
+        if len(membs) != 1:
+            raise _argerr("the membrane potential solve needs exactly one Memb (got %d)"
+                          % len(membs))
+        memb = self.memb = membs[0]
+        ptris = self.patch.tris if self.patch is not None else np.zeros(0, np.int64)
+        if len(ptris) and set(ptris.tolist()) != set(memb.tris.tolist()):
+            raise _notimpl("the membrane must consist of the patch that holds the surface systems")
+        # EField triangle k is the solver's patch triangle k
+        tris = ptris if len(ptris) else memb.tris
+        tets = geom.tets[memb.tets]
+        vids, local = np.unique(tets, return_inverse=True)
+        self._ef_vert_local = -np.ones(geom.countVertices(), dtype=np.int64)
+        self._ef_vert_local[vids] = np.arange(len(vids))
+        tri_v = self._ef_vert_local[geom.tris[tris]]
+        if np.any(tri_v < 0):
+            raise _argerr("a membrane triangle is not on the conduction volume")
+        self._ef_tris = tris
+        self._ef_tri_local = -np.ones(geom.countTris(), dtype=np.int64)
+        self._ef_tri_local[tris] = np.arange(len(tris))
+        self.ef = EField(geom.verts[vids], local.reshape(-1, 4), tri_v)
+        self._efdt = 1.0e-5                                  # tetopsplit.hpp:938
+        self._vert_iclamp = np.zeros(len(vids))
+        self._tri_iclamp = np.zeros(len(tris))
+        self._vert_clamped = np.zeros(len(vids), dtype=bool)
+        if len(self.fm.ohmic):
+            if len(self.fm.ohmic) > 8:
+                raise _notimpl("at most 8 ohmic currents are supported")
+            self.ef.set_ohmic(self.fm.ohmic_chan, self.fm.ohmic_g, self.fm.ohmic_erev)
+        if self.patch is not None:
+            self.ef.attach(self.eng)
 
     # --- identity ---
     def getSolverName(self):
@@ -115,10 +155,15 @@ class TetOpSplit:
         self.eng.reset()
 
     def run(self, endtime: float):
-        self.eng.run(endtime)
+        if self.ef is not None:
+            self.ef.run(self.eng, endtime, self._efdt)   # _runWithEField
+        else:
+            self.eng.run(endtime)
 
     def advance(self, adv: float):
-        self.eng.advance(adv)
+        if adv < 0.0:
+            raise _argerr("Time to advance cannot be negative")
+        self.run(self.getTime() + adv)
 
     def step(self):
         raise _notimpl("step() is not implemented for operator splitting")  # tetopsplit.cpp:1969
@@ -331,29 +376,205 @@ class TetOpSplit:
         c[self._tri(tri)] = int(round(n))
         self.eng.set_tri_counts(sp, c)
 
+    def getBatchTriSpecCounts(self, tris, s: str):
+        c = self.eng.get_tri_counts(self._sspec(s))
+        return [float(c[self._tri(int(t))]) for t in tris]
+
+    def sumBatchTriCountsNP(self, tri_array, s: str) -> float:
+        c = self.eng.get_tri_counts(self._sspec(s))
+        return float(c[[self._tri(int(t)) for t in tri_array]].sum())
+
     def getBatchTriSpecCountsNP(self, index_array, s: str, counts):
         c = self.eng.get_tri_counts(self._sspec(s))
         counts[:] = c[[self._tri(int(t)) for t in index_array]]
 
-    # --- triangle membrane potential read by voltage-dependent surface reactions ---
+    # --- membrane potential (TetOpSplitP EField methods, tetopsplit.cpp:4751-5000) ---
+    def _need_ef(self):
+        if self.ef is None:
+            raise _argerr("Method not available: EField calculation not included in simulation.")
+
+    def _vert(self, v: int) -> int:
+        self._need_ef()
+        if not 0 <= v < len(self._ef_vert_local) or self._ef_vert_local[v] < 0:
+            raise _argerr(f"Vertex index {v} not assigned to the conduction volume.")
+        return int(self._ef_vert_local[v])
+
+    def _eftri(self, t: int) -> int:
+        self._need_ef()
+        if not 0 <= t < len(self._ef_tri_local) or self._ef_tri_local[t] < 0:
+            raise _argerr(f"Triangle index {t} not assigned to a membrane.")
+        return int(self._ef_tri_local[t])
+
+    def _memb(self, m: str):
+        self._need_ef()
+        if m != self.memb.id:
+            raise _argerr(f"unknown membrane {m!r}")
+
+    def setEfieldDT(self, efdt: float):
+        if not efdt > 0.0:
+            raise _argerr("EField dt must be positive")
+        self._need_ef()
+        self._efdt = float(efdt)
+
+    def getEfieldDT(self) -> float:
+        self._need_ef()
+        return self._efdt
+
+    def setTemp(self, t: float):
+        if t < 0.0:
+            raise _argerr("temperature must be >= 0 K")
+        self.eng.set_temp(t)
+
+    def getTemp(self) -> float:
+        return self.eng.temp()
+
+    def setMembPotential(self, m: str, v: float):
+        self._memb(m)
+        self.ef.set_potential(v)
+
+    def setMembCapac(self, m: str, cm: float):
+        self._memb(m)
+        if cm < 0.0:
+            raise _argerr("Capacitance must be non-negative")
+        self.ef.set_capac(cm)
+
+    def setMembVolRes(self, m: str, ro: float):
+        self._memb(m)
+        if ro < 0.0:
+            raise _argerr("Resistivity must be non-negative")
+        self.ef.set_volres(ro)
+
+    def setMembRes(self, m: str, ro: float, vrev: float):
+        self._memb(m)
+        if ro <= 0.0:
+            raise _argerr("Resistivity must be greater than zero")
+        self.ef.set_memb_res(ro, vrev)
+
+    def getVertV(self, v: int) -> float:
+        return float(self.ef.verts_v()[self._vert(v)])
+
+    def setVertV(self, v: int, val: float):
+        i = self._vert(v)
+        vv = self.ef.verts_v()
+        vv[i] = val
+        self.ef.set_verts_v(vv)
+
+    def getVertVClamped(self, v: int) -> bool:
+        return bool(self._vert_clamped[self._vert(v)])
+
+    def setVertVClamped(self, v: int, cl: bool):
+        i = self._vert(v)
+        self.ef.set_vert_clamped(i, cl)
+        self._vert_clamped[i] = bool(cl)
+
+    def getVertIClamp(self, v: int) -> float:
+        return float(self._vert_iclamp[self._vert(v)])
+
+    def setVertIClamp(self, v: int, i: float):
+        k = self._vert(v)
+        self.ef.set_vert_iclamp(k, i)
+        self._vert_iclamp[k] = i
+
+    def getTetV(self, t: int) -> float:
+        self._need_ef()
+        vv = self.ef.verts_v()
+        return float(vv[[self._vert(int(x)) for x in self.geom.tets[t]]].mean())
+
+    def setTetV(self, t: int, v: float):
+        self._need_ef()
+        vv = self.ef.verts_v()
+        vv[[self._vert(int(x)) for x in self.geom.tets[t]]] = v
+        self.ef.set_verts_v(vv)
+
+    def getTetVClamped(self, t: int) -> bool:
+        self._need_ef()
+        return bool(all(self._vert_clamped[self._vert(int(x))] for x in self.geom.tets[t]))
+
+    def setTetVClamped(self, t: int, cl: bool):
+        self._need_ef()
+        for x in self.geom.tets[t]:
+            self.setVertVClamped(int(x), cl)
+
     def getTriV(self, tri: int) -> float:
-        return float(self.eng.get_tri_v()[self._tri(tri)])
+        """Potential (V) of a membrane triangle, the mean of its vertices (EField::getTriV,
+        efield.cpp:290-302). Without an EField, the value voltage-dependent reactions read."""
+        if self.ef is None:
+            return float(self.eng.get_tri_v()[self._tri(tri)])
+        return float(self.ef.tris_v()[self._eftri(tri)])
 
     def setTriV(self, tri: int, v: float):
-        """Potential (V) of one membrane triangle. An attached EField overwrites
-        it after every step (tetopsplit.cpp:6600-6626, _setTriV)."""
-        vv = self.eng.get_tri_v()
-        vv[self._tri(tri)] = v
-        self.eng.set_tri_v(vv)
+        """Set the potential of a membrane triangle: with an EField, of its three
+        vertices (EField::setTriV, efield.cpp:306-317; TetOpSplitP::_setTriV,
+        tetopsplit.cpp:4871-4893). Without one, only the value voltage-dependent
+        reactions read (no counterpart in the reference, which raises)."""
+        if self.ef is None:
+            vv = self.eng.get_tri_v()
+            vv[self._tri(tri)] = v
+            self.eng.set_tri_v(vv)
+            return
+        self._eftri(tri)
+        vv = self.ef.verts_v()
+        vv[[self._vert(int(x)) for x in self.geom.tris[tri]]] = v
+        self.ef.set_verts_v(vv)
+
+    def getTriVClamped(self, tri: int) -> bool:
+        self._eftri(tri)
+        return bool(all(self._vert_clamped[self._vert(int(x))] for x in self.geom.tris[tri]))
+
+    def setTriVClamped(self, tri: int, cl: bool):
+        self._eftri(tri)
+        for x in self.geom.tris[tri]:
+            self.setVertVClamped(int(x), cl)
+
+    def getTriIClamp(self, tri: int) -> float:
+        return float(self._tri_iclamp[self._eftri(tri)])
+
+    def setTriIClamp(self, tri: int, i: float):
+        k = self._eftri(tri)
+        self.ef.set_tri_iclamp(k, i)
+        self._tri_iclamp[k] = i
+
+    def getTriOhmicI(self, tri: int, oc: str) -> float:
+        """Present ohmic current (A) of one current on a triangle: open channels
+        times g times (V - erev) (Tri::getOhmicI, tri.cpp:709-718)."""
+        k = self._eftri(tri)
+        try:
+            j = self.fm.ohmic.index(oc)
+        except ValueError:
+            raise _argerr(f"unknown ohmic current {oc!r}") from None
+        n = self.eng.get_tri_counts(int(self.fm.ohmic_chan[j]))[self._tri(tri)]
+        v = self.ef.tris_v()[k]
+        return float(n * self.fm.ohmic_g[j] * (v - self.fm.ohmic_erev[j]))
 
     def getBatchTriVsNP(self, index_array, vs):
-        vv = self.eng.get_tri_v()
-        vs[:] = vv[[self._tri(int(t)) for t in index_array]]
+        if self.ef is None:
+            vv = self.eng.get_tri_v()
+            vs[:] = vv[[self._tri(int(t)) for t in index_array]]
+        else:
+            vv = self.ef.tris_v()
+            vs[:] = vv[[self._eftri(int(t)) for t in index_array]]
+
+    def getBatchTetVsNP(self, index_array, vs):
+        self._need_ef()
+        vv = self.ef.verts_v()
+        for k, t in enumerate(index_array):
+            vs[k] = vv[[self._vert(int(x)) for x in self.geom.tets[int(t)]]].mean()
+
+    def getBatchTriOhmicIsNP(self, index_array, oc: str, cur):
+        for k, t in enumerate(index_array):
+            cur[k] = self.getTriOhmicI(int(t), oc)
+
+    def sumBatchTriOhmicIsNP(self, tri_array, oc: str) -> float:
+        return float(sum(self.getTriOhmicI(int(t), oc) for t in tri_array))
+
+    def getBatchVertVsNP(self, index_array, vs):
+        self._need_ef()
+        vv = self.ef.verts_v()
+        vs[:] = vv[[self._vert(int(v)) for v in index_array]]
 
     def setBatchTriVsNP(self, index_array, vs):
-        vv = self.eng.get_tri_v()
-        vv[[self._tri(int(t)) for t in index_array]] = vs
-        self.eng.set_tri_v(vv)
+        for t, v in zip(index_array, vs):
+            self.setTriV(int(t), float(v))
 
     # --- checkpointing (own binary format, stg_checkpoint in include/stg.h) ---
     def checkpoint(self, file_name: str):
@@ -377,6 +598,8 @@ class DistTetOpSplit(TetOpSplit):
     def __init__(self, model: Model, mesh: Tetmesh, rng, SSAMethod=0, searchMethod=0,
                  indepKProcs: bool = False, isEfield: bool = True):
         self._setup(model, mesh, rng)
+        if isEfield and mesh.membs:
+            self._setup_efield(mesh)
 
     def setCompSpecCount(self, c: str, s: str, n: float, distribution: int = 0):
         if distribution == self.MULTINOMIAL:

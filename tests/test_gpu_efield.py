@@ -241,3 +241,58 @@ def test_ghk_charge_drives_the_potential():
         err = np.abs(got - dv).max() / np.abs(dv).max()
         assert err < 1e-9, (step, err, ef.stats())
         ef.set_verts_v(ora.V * 1e-3)
+
+
+def test_rallpack1_through_the_solver_interface():
+    """Rallpack1 as the reference script sets it up (rallpack1.py:39-110):
+    a Memb over the cable's patch, one 'Leak' channel state per triangle with
+    an OhmicCurr, TetOpSplit(calcMembPot=EF_DEFAULT), membrane setters and a
+    vertex current clamp. First 20 ms against the golden traces, RMS < 1 mV."""
+    from steps_amd import rng as R
+    from steps_amd.solver import EF_DEFAULT, TetOpSplit
+
+    p = RALL
+    mesh, memb_tris, zmin_tris, zmax_tris = rallpack1_setup()
+    cyto = geom.TmComp("cyto", mesh)
+    patch = geom.TmPatch("memb", mesh, memb_tris, cyto)
+    patch.addSurfsys("ssys")
+    geom.Memb("membrane", mesh, [patch])
+    mdl = M.Model()
+    ssys = M.Surfsys("ssys", mdl)
+    chan = M.Chan("L", mdl)
+    leak = M.ChanState("Leak", mdl, chan)
+    area_cyl = np.pi * p["diameter"] * p["length"]
+    M.OhmicCurr("OC_L", ssys, leak, p["E_M"], area_cyl / p["R_M"] / len(memb_tris))
+    r = R.create("mt19937", 512)
+    r.initialize(7)
+    print("case rallpack1 solver interface", flush=True)
+    sim = TetOpSplit(mdl, mesh, r, EF_DEFAULT)
+    sim.setEfieldDT(p["EF_dt"])
+    for t in memb_tris:
+        sim.setTriSpecCount(int(t), "Leak", 1)
+    area_factor = patch.getArea() / area_cyl
+    sim.setMembPotential("membrane", p["E_M"])
+    sim.setMembVolRes("membrane", p["R_A"])
+    sim.setMembCapac("membrane", p["C_M"] / area_factor)
+    v_zmin = np.unique(mesh.tris[zmin_tris])
+    for v in v_zmin:
+        sim.setVertIClamp(int(v), p["Iinj"] / len(v_zmin))
+    s0, s1 = sample_verts(mesh, zmin_tris), sample_verts(mesh, zmax_tris)
+    out0, out1 = [], []
+    buf = np.zeros(2)
+    for k in range(1, 401):
+        sim.run(k * 5e-5)
+        sim.getBatchVertVsNP(np.array(s0), buf)
+        out0.append(buf.mean())
+        sim.getBatchVertVsNP(np.array(s1), buf)
+        out1.append(buf.mean())
+        if k % 100 == 0:
+            print(f"rallpack1 solver t={k * 5e-5:.4f} v0={out0[-1]:.6f}", flush=True)
+    v0, vx = read_trace("v0"), read_trace("vx")
+    n = len(out0)
+    rms0 = np.linalg.norm(v0[:n] - np.array(out0)) / np.sqrt(n)
+    rms1 = np.linalg.norm(vx[:n] - np.array(out1)) / np.sqrt(n)
+    print(f"rallpack1 solver rms {rms0:.3e} {rms1:.3e}", flush=True)
+    assert rms0 < 1e-3 and rms1 < 1e-3
+    assert abs(sim.getTriV(int(memb_tris[0])) - p["E_M"]) < 0.05
+    assert sim.getEfieldDT() == p["EF_dt"]

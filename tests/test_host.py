@@ -95,3 +95,57 @@ def test_workloads_shapes():
     fm, meshes, counts, k, info = W.ab_binding_ensemble(n_real=6, n_min=2, n_max=3, n_a=10, n_b=10)
     assert len(meshes) == 6 and k.shape == (6, 2)
     assert all(c[0].sum() == 10 for c in counts)
+
+
+def test_memb_collects_patch_triangles_and_conduction_volume():
+    mesh = geom.kuhn_cube(3, 3e-6)
+    surf = mesh.getSurfTris()
+    inner = geom.TmComp("inner", mesh, np.arange(60))
+    other = geom.TmComp("other", mesh, np.arange(60, mesh.countTets()))
+    p1 = geom.TmPatch("a", mesh, surf[:10], inner)
+    p2 = geom.TmPatch("b", mesh, surf[10:25], inner)
+    m = geom.Memb("memb", mesh, [p1, p2])
+    assert np.array_equal(m.getAllTriIndices(), surf[:25])
+    assert np.array_equal(m.getAllVolTetIndices(), np.arange(60))
+    m2 = geom.Memb("memb2", mesh, [p1], supplementary_comps=[other])
+    assert len(m2.getAllVolTetIndices()) == mesh.countTets()
+    assert mesh.membs["memb"] is m
+    with pytest.raises(ValueError):
+        geom.Memb("dup", mesh, [p1, p1])
+
+
+def test_solver_rejects_unknown_membrane_solver():
+    mesh = geom.kuhn_cube(2, 1e-6)
+    geom.TmComp("c", mesh)
+    mdl = M.Model()
+    M.Volsys("vsys", mdl)
+    with pytest.raises(N.ArgErr):
+        S.TetOpSplit(mdl, mesh, R.create("r123", 16), calcMembPot=9)
+
+
+def test_channel_and_current_tables():
+    mdl = M.Model()
+    ch = M.Chan("K", mdl)
+    n0, n1 = M.ChanState("n0", mdl, ch), M.ChanState("n1", mdl, ch)
+    k = M.Spec("Kion", mdl, valence=1)
+    vs = M.Volsys("vsys", mdl)
+    M.Diff("dK", vs, k, 1e-10)
+    ss = M.Surfsys("ssys", mdl)
+    M.VDepSReac("open", ss, slhs=[n0], srhs=[n1], k=lambda v: 100.0)
+    M.OhmicCurr("leak", ss, n0, -0.07, 1e-12)
+    g = M.GHKcurr("ghk", ss, n1, k, virtual_oconc=4e-3, vshift=0.001)
+    with pytest.raises(ValueError):
+        M.flatten(mdl, ["vsys"], ["ssys"])          # no permeability yet
+    g.setP(1e-19)
+    fm = M.flatten(mdl, ["vsys"], ["ssys"])
+    assert fm.sspecs == ["n0", "n1"] and fm.specs == ["Kion"]
+    assert fm.ohmic_chan.tolist() == [0] and fm.ohmic_g.tolist() == [1e-12]
+    assert fm.ghk_chan.tolist() == [1] and fm.ghk_ion.tolist() == [0]
+    assert fm.ghk_valence.tolist() == [1] and fm.ghk_vshift.tolist() == [0.001]
+    assert ch.states == [n0, n1] and n1.getChan() is ch
+    with pytest.raises(ValueError):
+        M.GHKcurr("bad", ss, n1, M.Spec("neutral", mdl))   # valence zero
+    g2 = M.GHKcurr("nooc", ss, n1, k)
+    g2.setP(1e-19)
+    with pytest.raises(NotImplementedError):
+        M.flatten(mdl, ["vsys"], ["ssys"])          # needs a virtual outer concentration
