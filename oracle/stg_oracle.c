@@ -358,12 +358,14 @@ double stgo_update_period(const stgo_system *s) {
     return mx > DBL_EPSILON ? 1.0 / mx : INFINITY;
 }
 
-/* c = k * scale^-(order-1): reactions.cpp:113-119, surface_reactions.cpp:176-203 */
+/* c = k * scale^-(order-1): comp_ccst, reac.cpp:44-55 (kcst * pow(vscale, -o1));
+ * reactions.cpp:113-119, surface_reactions.cpp:176-203. Negative powers are
+ * powers of inv = 1 / scale. */
 static double scaled_const(double k, int32_t order, double scale) {
     int32_t o1 = order - 1;
     if (o1 == 0) return k;
     if (o1 < 0) return k * ipow(scale, -o1);
-    return k / ipow(scale, o1);
+    return k * ipow(1.0 / scale, o1);
 }
 
 /* k(V) by linear interpolation in the tabulated rate constant:

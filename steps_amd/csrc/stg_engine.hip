@@ -144,11 +144,15 @@ struct stg_sim {
     int32_t last_launches = 0;
     std::string err;
 
+    bool needs_memb() const { return NTRI > 0 && (RS > 0 || NG > 0 || mk.tri_intg_mask != 0); }
+
+    // per-thread LDS rows of the period kernel: counts, occupancy corrections,
+    // rate constants, propensities and (membrane variant) GHK constants
     size_t smem_bytes(int B) const {
-        const int K = R + max_tris * (RS + NG);
+        const int K = std::max(1, R + max_tris * (RS + NG));
         size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
-        return off + (size_t)S * B * 8 + 2 * (size_t)(K > 0 ? K : 1) * B * 8 +
-               2 * (size_t)max_tris * NG * B * 8;
+        const size_t rows = 2 * (size_t)K + (needs_memb() ? 2 * (size_t)max_tris * NG : 0);
+        return off + (size_t)S * B * 8 + rows * B * 8;
     }
 
     double compute_dt_std(int r) const {
@@ -221,12 +225,11 @@ void build_model(stg_sim *s, const stg_model *m) {
     auto push = [&](int spec, int val) {
         if (val < -128 || val > 127) throw StgError{STG_ERR_ARG, "stoichiometry out of int8 range"};
         if (ne >= stg::kMaxEntries) throw StgError{STG_ERR_ARG, "model tables too large"};
-        k.ent_spec[ne] = (uint8_t)spec;
-        k.ent_val[ne] = (int8_t)val;
+        k.ent[ne] = (int32_t)(((uint32_t)val << 16) | (uint32_t)spec);
         ++ne;
     };
     for (int r = 0; r < s->R; ++r) {
-        k.lhs_off[r] = (uint16_t)ne;
+        k.lhs_off[r] = ne;
         int order = 0;
         for (int sp = 0; sp < s->S; ++sp) {
             const int v = m->reac_lhs[r * s->S + sp];
@@ -235,11 +238,11 @@ void build_model(stg_sim *s, const stg_model *m) {
             if (v) push(sp, v);
             order += v;
         }
-        k.order[r] = (int8_t)order;
+        k.order[r] = order;
     }
-    k.lhs_off[s->R] = (uint16_t)ne;
+    k.lhs_off[s->R] = ne;
     for (int r = 0; r < s->R; ++r) {
-        k.upd_off[r] = (uint16_t)ne;
+        k.upd_off[r] = ne;
         for (int sp = 0; sp < s->S; ++sp) {
             const int v = m->reac_upd[r * s->S + sp];
             if (v) push(sp, v);
@@ -247,9 +250,9 @@ void build_model(stg_sim *s, const stg_model *m) {
                 throw StgError{STG_ERR_ARG, "a reaction consumes a species it does not react with"};
         }
     }
-    k.upd_off[s->R] = (uint16_t)ne;
+    k.upd_off[s->R] = ne;
     for (int r = 0; r < s->RS; ++r) {
-        k.slhs_off[r] = (uint16_t)ne;
+        k.slhs_off[r] = ne;
         int order = 0, nvol = 0;
         for (int q = 0; q < s->SS; ++q) {
             const int v = m->sreac_lhs_s[r * s->SS + q];
@@ -264,12 +267,12 @@ void build_model(stg_sim *s, const stg_model *m) {
             order += v;
             nvol += v;
         }
-        k.sorder[r] = (int8_t)order;
-        k.s_nvol[r] = (int8_t)nvol;
+        k.sorder[r] = order;
+        k.s_nvol[r] = nvol;
     }
-    k.slhs_off[s->RS] = (uint16_t)ne;
+    k.slhs_off[s->RS] = ne;
     for (int r = 0; r < s->RS; ++r) {
-        k.supd_off[r] = (uint16_t)ne;
+        k.supd_off[r] = ne;
         for (int q = 0; q < s->SS; ++q) {
             const int v = m->sreac_upd_s[r * s->SS + q];
             if (v) push(stg::kSurfFlag | q, v);
@@ -283,7 +286,7 @@ void build_model(stg_sim *s, const stg_model *m) {
                 throw StgError{STG_ERR_ARG, "a surface reaction consumes a species it does not react with"};
         }
     }
-    k.supd_off[s->RS] = (uint16_t)ne;
+    k.supd_off[s->RS] = ne;
     // voltage-dependent rate tables
     s->vdep_vmin.clear();
     s->vdep_dv.clear();
@@ -305,7 +308,7 @@ void build_model(stg_sim *s, const stg_model *m) {
     for (int r = 0; r < s->RS; ++r) {
         const int t = (nv > 0) ? m->sreac_vdep[r] : -1;
         if (t < -1 || t >= nv) throw StgError{STG_ERR_ARG, "k(V) table index out of range"};
-        k.s_vdep[r] = (int8_t)t;
+        k.s_vdep[r] = t;
     }
     // GHK currents
     if (m->n_ghk < 0 || m->n_ghk > stg::kMaxGhk) throw StgError{STG_ERR_ARG, "n_ghk must be in [0, 4]"};
@@ -321,9 +324,9 @@ void build_model(stg_sim *s, const stg_model *m) {
         if (!(m->ghk_perm[g] > 0.0)) throw StgError{STG_ERR_ARG, "GHK permeability must be > 0"};
         if (!(m->ghk_voconc[g] >= 0.0))
             throw StgError{STG_ERR_NOTIMPL, "GHK currents need a virtual outer concentration >= 0"};
-        k.g_chan[g] = (int8_t)m->ghk_chan[g];
-        k.g_ion[g] = (int8_t)m->ghk_ion[g];
-        k.g_z[g] = (int8_t)m->ghk_valence[g];
+        k.g_chan[g] = m->ghk_chan[g];
+        k.g_ion[g] = m->ghk_ion[g];
+        k.g_z[g] = m->ghk_valence[g];
         k.g_real[g] = m->ghk_realflux[g] ? 1 : 0;
         k.g_perm[g] = m->ghk_perm[g];
         k.g_oconc[g] = m->ghk_voconc[g];
@@ -622,16 +625,26 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     return k;
 }
 
+// the membrane variant of the period kernel is needed when a triangle can
+// hold a surface reaction, a GHK current or an integrated channel count
+bool needs_memb(const stg_sim *s) { return s->needs_memb(); }
+
+using PeriodKernel = void (*)(stg::ModelK, stg::SysK, int);
+
+PeriodKernel period_kernel(const stg_sim *s, int B) {
+    const bool memb = needs_memb(s);
+    if (B <= 256) return memb ? stg::k_period<256, true> : stg::k_period<256, false>;
+    return memb ? stg::k_period<1024, true> : stg::k_period<1024, false>;
+}
+
 void launch_all(stg_sim *s, int L) {
     const int B = s->block;
     const int nb = (int)s->block_real.size();
     const size_t smem = s->smem_bytes(B);
+    const PeriodKernel kern = period_kernel(s, B);
     for (int l = 0; l <= L; ++l) {
         const stg::SysK k = make_sys(s, l & 1);
-        if (B <= 256)
-            hipLaunchKernelGGL(stg::k_period<256>, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
-        else
-            hipLaunchKernelGGL(stg::k_period<1024>, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
+        hipLaunchKernelGGL(kern, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
     }
     HIP_OK(hipGetLastError());
 }
@@ -673,12 +686,9 @@ void run_impl(stg_sim *s, double endtime) {
         HIP_OK(hipMemcpyAsync(s->d_real_nper, nper.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
         HIP_OK(hipMemcpyAsync(s->d_real_step0, step0.data(), sizeof(uint32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
         const size_t smem = s->smem_bytes(s->block);
-        if (smem > 64 * 1024) {
-            HIP_OK(hipFuncSetAttribute((const void *)stg::k_period<256>,
+        if (smem > 64 * 1024)
+            HIP_OK(hipFuncSetAttribute((const void *)period_kernel(s, s->block),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-            HIP_OK(hipFuncSetAttribute((const void *)stg::k_period<1024>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        }
         HIP_OK(hipEventRecord(s->ev0, s->stream));
         if (s->use_graph) {
             if (s->graph_launches != L || s->graph_block != s->block) {

@@ -40,24 +40,25 @@ constexpr double kFaraday = 96485.3399;        // src/steps/math/constants.hpp:4
 constexpr double kGasConstant = 8.314472;      // src/steps/math/constants.hpp:53
 constexpr double kECharge = 1.602176487e-19;   // src/steps/math/constants.hpp:57
 
-// Model tables, passed by value (kernel-argument segment, read with scalar loads).
+// Model tables, passed by value in the kernel-argument segment. Every field is
+// a 32-bit word so that wave-uniform reads are scalar loads (sub-dword fields
+// would be fetched with per-lane vector loads).
 struct ModelK {
     int32_t n_specs, n_reacs, n_sspecs, n_sreacs, n_diff;
     int32_t tracked_mask;  // bit s: occupancy tracked for species s
     int32_t tri_intg_mask; // bit q: time integral of surface species q is accumulated
     int32_t dspec[kMaxSpecs];
-    int8_t order[kMaxReacs];
-    int8_t sorder[kMaxSReacs];
-    int8_t s_nvol[kMaxSReacs];  // number of volume reactants of a surface reaction
-    int8_t s_vdep[kMaxSReacs];  // k(V) table of a surface reaction, -1 = constant k
-    uint16_t lhs_off[kMaxReacs + 1], upd_off[kMaxReacs + 1];
-    uint16_t slhs_off[kMaxSReacs + 1], supd_off[kMaxSReacs + 1];
-    uint8_t ent_spec[kMaxEntries];
-    int8_t ent_val[kMaxEntries];
+    int32_t order[kMaxReacs];
+    int32_t sorder[kMaxSReacs];
+    int32_t s_nvol[kMaxSReacs];  // number of volume reactants of a surface reaction
+    int32_t s_vdep[kMaxSReacs];  // k(V) table of a surface reaction, -1 = constant k
+    int32_t lhs_off[kMaxReacs + 1], upd_off[kMaxReacs + 1];
+    int32_t slhs_off[kMaxSReacs + 1], supd_off[kMaxSReacs + 1];
+    int32_t ent[kMaxEntries];  // species (low 16 bits) | signed value << 16
     // GHK currents: channel-state surface species, inner ion, valence, whether
     // ions move; permeability (m/s), virtual outer concentration (M), V shift (V)
     int32_t n_ghk;
-    int8_t g_chan[kMaxGhk], g_ion[kMaxGhk], g_z[kMaxGhk], g_real[kMaxGhk];
+    int32_t g_chan[kMaxGhk], g_ion[kMaxGhk], g_z[kMaxGhk], g_real[kMaxGhk];
     double g_perm[kMaxGhk], g_oconc[kMaxGhk], g_vshift[kMaxGhk];
     double temp;  // K
 };
@@ -104,11 +105,16 @@ struct SysK {
     int32_t *tri_charge;         // [NTRI] GHK charge moved out (elementary charges), or null
 };
 
-__device__ __forceinline__ double scaled_const(double k, int order, double scale) {
+// both structures travel in the 4 KiB kernel-argument segment
+static_assert(sizeof(ModelK) + sizeof(SysK) + 16 <= 4096, "kernel arguments exceed 4 KiB");
+
+// c = k scale^-(order-1) (comp_ccst, src/steps/mpi/tetopsplit/reac.cpp:44-55),
+// as a product with powers of scale or of inv = 1 / scale: no division per call
+__device__ __forceinline__ double scaled_const(double k, int order, double scale, double inv) {
     const int o1 = order - 1;
     if (o1 == 0) return k;
     if (o1 < 0) return k * ipow(scale, -o1);
-    return k / ipow(scale, o1);
+    return k * ipow(inv, o1);
 }
 
 // h *= n (n-1) ... (n-L+1), multiplied from the smallest factor up
@@ -187,8 +193,8 @@ __device__ __forceinline__ double vdep_k(const SysK &sys, int tab, double v) {
 __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const Lds &L, int r, int tri) {
     double h = 1.0;
     for (int e = m.slhs_off[r]; e < m.slhs_off[r + 1]; ++e) {
-        const int sp = m.ent_spec[e];
-        const int mult = m.ent_val[e];
+        const int sp = m.ent[e] & 0xffff;
+        const int mult = m.ent[e] >> 16;
         int32_t n;
         if (sp & kSurfFlag) n = sys.tri_counts[(size_t)(sp & 0x7f) * sys.n_tris + tri];
         else n = L.c(sp);
@@ -200,29 +206,35 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 
 // Direct-method SSA of one tet group over [0, dt). Returns events, or -1 when
 // the per-period event cap is hit.
+// MEMB = false compiles a variant without triangles (no surface reactions,
+// GHK currents, k(V) tables or channel integrals) for volume-only models.
+template <bool MEMB>
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
                           uint32_t ltet, uint32_t period, double dt) {
     const int R = m.n_reacs, RS = m.n_sreacs, NG = m.n_ghk;
     const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
-    if (sys.tri_ptr != nullptr && RT > 0) {
+    if (MEMB && sys.tri_ptr != nullptr && RT > 0) {
         tp0 = sys.tri_ptr[t];
         ntri = sys.tri_ptr[t + 1] - tp0;
     }
     const int K = R + ntri * RT;
     if (K == 0) return 0;
     const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
+    const double ivscale = 1.0 / vscale;
     const double *kr = sys.real_k + (size_t)r * R;
     const double *ksr = sys.real_sk + (size_t)r * RS;
     // rate constants scaled by volume / area, formed once per period
     {
         int j = 0;
-        for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale);
+        for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale, ivscale);
         for (int i = 0; i < ntri; ++i) {
             const double ascale = sys.tri_area[tp0 + i] * kAvogadro;
+            const double iascale = 1.0 / ascale;
             for (int q = 0; q < RS; ++q, ++j) {
                 const double k = m.s_vdep[q] < 0 ? ksr[q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tp0 + i]);
-                L.c(j, 0) = scaled_const(k, m.sorder[q], m.s_nvol[q] == 0 ? ascale : vscale);
+                L.c(j, 0) = m.s_nvol[q] == 0 ? scaled_const(k, m.sorder[q], ascale, iascale)
+                                             : scaled_const(k, m.sorder[q], vscale, ivscale);
             }
             for (int q = 0; q < NG; ++q, ++j) {
                 if (!ghk_consts(m, q, sys.tri_v[tp0 + i], L.c(j, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1))) {
@@ -245,8 +257,8 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             double h = 1.0;
             bool zero = false;
             for (int e = m.lhs_off[q]; e < m.lhs_off[q + 1]; ++e) {
-                const int sp = m.ent_spec[e];
-                const int mult = m.ent_val[e];
+                const int sp = m.ent[e] & 0xffff;
+                const int mult = m.ent[e] >> 16;
                 const int32_t n = L.c(sp);
                 if (mult > n) {
                     zero = true;
@@ -283,28 +295,21 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
         const double tev = tcur + (-dlog(u1) / a0);
         if (tev > dt) break;
         tcur = tev;
+        // first q whose running sum exceeds x, else the last nonzero propensity
         const double x = u2 * a0;
-        int sel = -1;
+        int sel = -1, last = -1;
         double acc = 0.0;
         for (int q = 0; q < K; ++q) {
-            acc = acc + L.p(q);
-            if (acc > x) {
-                sel = q;
-                break;
-            }
+            const double pq = L.p(q);
+            acc = acc + pq;
+            if (sel < 0 && acc > x) sel = q;
+            if (pq > 0.0) last = q;
         }
-        if (sel < 0) {
-            for (int q = K - 1; q >= 0; --q) {
-                if (L.p(q) > 0.0) {
-                    sel = q;
-                    break;
-                }
-            }
-        }
-        if (sel < R) {
+        if (sel < 0) sel = last;
+        if (!MEMB || sel < R) {
             for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
-                const int sp = m.ent_spec[e];
-                const int dv = m.ent_val[e];
+                const int sp = m.ent[e] & 0xffff;
+                const int dv = m.ent[e] >> 16;
                 L.c(sp) += dv;
                 if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
             }
@@ -325,11 +330,11 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             sys.tri_charge[tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
         } else {
             const int i = (sel - R) / RT;
-            const int q = (sel - R) - i * RT;
+            const int qs = (sel - R) - i * RT;
             const int tri = tp0 + i;
-            for (int e = m.supd_off[q]; e < m.supd_off[q + 1]; ++e) {
-                const int sp = m.ent_spec[e];
-                const int dv = m.ent_val[e];
+            for (int e = m.supd_off[qs]; e < m.supd_off[qs + 1]; ++e) {
+                const int sp = m.ent[e] & 0xffff;
+                const int dv = m.ent[e] >> 16;
                 if (sp & kSurfFlag) {
                     const size_t ix = (size_t)(sp & 0x7f) * sys.n_tris + tri;
                     sys.tri_counts[ix] += dv;
@@ -424,9 +429,12 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
 #ifndef STG_PERIOD_WAVES_PER_EU
 #define STG_PERIOD_WAVES_PER_EU 1
 #endif
+#ifndef STG_VOL_WAVES_PER_EU
+#define STG_VOL_WAVES_PER_EU 1
+#endif
 
-template <int MAXB>
-__global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
+template <int MAXB, bool MEMB>
+__global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL_WAVES_PER_EU)
     k_period(const ModelK m, const SysK sys, int launch) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int B = blockDim.x;
@@ -454,9 +462,9 @@ __global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
     L.tid = tid;
     L.cnt = reinterpret_cast<int32_t *>(smem);
     L.corr = reinterpret_cast<double *>(smem + (((size_t)S * B * 4 + 15) & ~(size_t)15));
-    L.a = L.corr + (size_t)S * B;
-    L.kc = L.a + (size_t)sys.k_max * B;
-    L.gx = L.kc + (size_t)sys.k_max * B;
+    L.kc = L.corr + (size_t)S * B;
+    L.a = L.kc + (size_t)sys.k_max * B;
+    L.gx = MEMB ? L.a + (size_t)sys.k_max * B : nullptr;
 
     const int t = sys.block_tet0[b] + tid;
     const int tend = sys.real_tet0[r + 1];
@@ -472,14 +480,21 @@ __global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
         const int4 nb4 = sys.nbr[t];
         const uint32_t nf = sys.nface[t];
         if (do_enter) {
+            // every slot is read unconditionally (a missing neighbour reads
+            // slot 0 and is masked) so the four loads are in flight together
             const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            size_t off[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
             for (int d = 0; d < m.n_diff; ++d) {
                 const int32_t *op = sys.out_prev + (size_t)d * NT * 4;
+                int32_t v[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) v[f] = op[off[f]];
                 int32_t add = 0;
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    if (nbv[f] >= 0) add += op[(size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff)];
-                }
+                for (int f = 0; f < 4; ++f) add += nbv[f] >= 0 ? v[f] : 0;
                 L.c(m.dspec[d]) += add;
             }
         }
@@ -488,12 +503,12 @@ __global__ void __launch_bounds__(MAXB, STG_PERIOD_WAVES_PER_EU)
             const uint32_t ltet = (uint32_t)(t - tet0);
             const uint32_t period = sys.real_step0[r] + (uint32_t)launch;
             const double dt = sys.dt_tab[(size_t)launch * sys.n_real + r];
-            nev = ssa_group(m, sys, L, r, t, ltet, period, dt);
+            nev = ssa_group<MEMB>(m, sys, L, r, t, ltet, period, dt);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
             }
-            if (m.tri_intg_mask != 0 && sys.tri_ptr != nullptr) {
+            if (MEMB && m.tri_intg_mask != 0 && sys.tri_ptr != nullptr) {
                 const int tp1 = sys.tri_ptr[t + 1];
                 for (int q = 0; q < m.n_sspecs; ++q) {
                     if (!((m.tri_intg_mask >> q) & 1)) continue;
