@@ -52,6 +52,11 @@ typedef struct stg_mesh {
     int32_t n_tris;              /* membrane triangles carrying surface reactions */
     const int32_t *tri_tet;      /* [n_tris] inner tet */
     const double *tri_area;      /* [n_tris] m^2 */
+    /* Compartments created with stg_create_comps only (NULL otherwise): */
+    const int32_t *tet_nbr_comp; /* [n_tets*4] compartment of the neighbour; tet_nbr indexes
+                                    that compartment's mesh */
+    const int32_t *tet_face_bdir;/* [n_tets*4] -1, or the diffusion-boundary direction d whose
+                                    per-species constant (stg_set_boundary_dcst) opens the face */
 } stg_mesh;
 
 /* Flattened model of one compartment (+ one patch), as the reference's
@@ -116,6 +121,22 @@ int stg_create(const stg_model *model, const stg_mesh *mesh, uint64_t seed, int3
  * same launches. No reference counterpart: it is the batched ensemble mode. */
 int stg_create_batch(const stg_model *model, int32_t n_real, const stg_mesh *meshes,
                      const stg_realization *reals, int32_t mode, stg_sim **out);
+
+/* Several compartments of one simulation (TetOpSplitP over a mesh with
+ * several TmComps, tetopsplit.cpp:83-118; Compdef per compartment,
+ * src/steps/solver/compdef.hpp). Compartment c is realization c: its mesh
+ * (local tet numbering), reaction constants and diffusion constants (zero for
+ * reactions and species it does not hold), Philox stream comp_params[c].stream.
+ * All compartments advance on one period schedule. Faces between compartments
+ * are closed unless they lie on a diffusion boundary (tet_face_bdir). */
+int stg_create_comps(const stg_model *model, int32_t n_comps, const stg_mesh *comp_meshes,
+                     const stg_realization *comp_params, int32_t mode, stg_sim **out);
+
+/* Diffusion constant of species spec across boundary direction bdir (m^2/s;
+ * 0 closes it), as DiffBoundary activation and setDiffBoundarySpecDcst set it
+ * per tet direction (tetopsplit.cpp:2745-2931, diff.cpp:307-460). */
+int stg_set_boundary_dcst(stg_sim *sim, int32_t bdir, int32_t spec, double dcst);
+int stg_get_boundary_dcst(const stg_sim *sim, int32_t bdir, int32_t spec, double *dcst);
 
 int stg_destroy(stg_sim *sim);
 

@@ -34,6 +34,9 @@ class StgoSystem(C.Structure):
         ("n_ghk", C.c_int32), ("ghk_chan", _ip), ("ghk_ion", _ip), ("ghk_valence", _ip),
         ("ghk_perm", _dp), ("ghk_voconc", _dp), ("ghk_vshift", _dp), ("ghk_realflux", _ip),
         ("temp", C.c_double), ("tri_charge", _ip),
+        ("n_comps", C.c_int32), ("tet_comp", _ip), ("tet_ltet", _ip),
+        ("comp_stream", C.POINTER(C.c_uint32)), ("comp_reac_k", _dp), ("comp_dcst", _dp),
+        ("tet_bdir", _ip), ("bnd_dcst", _dp),
     ]
 
 
@@ -185,6 +188,54 @@ class OracleSim:
         tri_intg from now on (the ohmic-current channel average)."""
         for q in sspecs:
             self.sys.tri_intg_mask |= 1 << int(q)
+
+    @classmethod
+    def comps(cls, fm, comp_meshes, seed: int, comp_k, comp_dcst, streams=None, n_bdir: int = 0,
+              **kw):
+        """Several compartments of one simulation, laid out as the engine lays
+        out stg_create_comps: compartment-major tets and triangles, compartment c
+        on Philox stream streams[c] (default c) with its local tet numbering."""
+        from steps_amd.geom import MeshTables
+
+        nc = len(comp_meshes)
+        off = np.concatenate([[0], np.cumsum([len(m.vol) for m in comp_meshes])])
+        nbr, nf, bdir = [], [], []
+        for c, m in enumerate(comp_meshes):
+            ncmp = m.nbr_comp if m.nbr_comp is not None else np.full_like(m.nbr, c)
+            nbr.append(np.where(m.nbr >= 0, off[ncmp] + m.nbr, -1))
+            nf.append(m.nbr_face)
+            bdir.append(m.bdir if m.bdir is not None else -np.ones_like(m.nbr))
+        tri_tet = np.concatenate([m.tri_tet + off[c] for c, m in enumerate(comp_meshes)])
+        mt = MeshTables(vol=np.concatenate([m.vol for m in comp_meshes]), nbr=np.concatenate(nbr),
+                        nbr_face=np.concatenate(nf), face_g=np.concatenate([m.face_g for m in comp_meshes]),
+                        tri_tet=tri_tet.astype(np.int32),
+                        tri_area=np.concatenate([m.tri_area for m in comp_meshes]))
+        o = cls(fm, mt, seed, **kw)
+        a = o.a
+        a["tcomp"] = np.ascontiguousarray(np.repeat(np.arange(nc), np.diff(off)), dtype=np.int32)
+        a["tlt"] = np.ascontiguousarray(np.concatenate([np.arange(len(m.vol)) for m in comp_meshes]),
+                                        dtype=np.int32)
+        a["cstr"] = np.ascontiguousarray(np.arange(nc) if streams is None else streams, dtype=np.uint32)
+        a["ck"] = np.ascontiguousarray(np.asarray(comp_k, dtype=np.float64).reshape(nc, -1))
+        a["cd"] = np.ascontiguousarray(np.asarray(comp_dcst, dtype=np.float64).reshape(nc, -1))
+        a["bdir"] = np.ascontiguousarray(np.concatenate(bdir), dtype=np.int32)
+        a["bnd"] = np.zeros((max(n_bdir, 1), len(fm.specs)))
+        P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
+        o.sys.n_comps = nc
+        o.sys.tet_comp = P(a["tcomp"], C.c_int32)
+        o.sys.tet_ltet = P(a["tlt"], C.c_int32)
+        o.sys.comp_stream = P(a["cstr"], C.c_uint32)
+        o.sys.comp_reac_k = P(a["ck"], C.c_double)
+        o.sys.comp_dcst = P(a["cd"], C.c_double)
+        o.sys.tet_bdir = P(a["bdir"], C.c_int32)
+        o.sys.bnd_dcst = P(a["bnd"], C.c_double)
+        o.comp_off = off
+        return o
+
+    @property
+    def bnd_dcst(self) -> np.ndarray:
+        """[direction][species] diffusion constant across each boundary direction."""
+        return self.a["bnd"]
 
     @property
     def tri_charge(self) -> np.ndarray:

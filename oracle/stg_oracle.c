@@ -343,14 +343,37 @@ static double face_rate_sum(const double *r4) {
     return ((r4[0] + r4[1]) + r4[2]) + r4[3];
 }
 
+/* diffusion constant of species sp in tet t's compartment */
+static double tet_dcst(const stgo_system *s, int32_t t, int32_t sp) {
+    if (s->tet_comp) return s->comp_dcst[(size_t)s->tet_comp[t] * s->n_specs + sp];
+    return s->spec_dcst[sp];
+}
+
+/* D_f g_f per face: faces on a diffusion boundary use the boundary's
+ * constant (Diff::setDcst / setDirectionDcst, diff.cpp:340-460) */
+static void face_rates(const stgo_system *s, int32_t t, int32_t sp, double r4[4]) {
+    const double D = tet_dcst(s, t, sp);
+    for (int f = 0; f < 4; ++f) {
+        const int32_t d = s->tet_bdir ? s->tet_bdir[4 * t + f] : -1;
+        const double Df = d < 0 ? D : s->bnd_dcst[(size_t)d * s->n_specs + sp];
+        r4[f] = Df * s->tet_face_g[4 * t + f];
+    }
+}
+
+/* the species diffuses in some compartment */
+static int spec_diffuses(const stgo_system *s, int32_t sp) {
+    if (!s->tet_comp) return s->spec_dcst[sp] > 0.0;
+    for (int32_t c = 0; c < s->n_comps; ++c)
+        if (s->comp_dcst[(size_t)c * s->n_specs + sp] > 0.0) return 1;
+    return 0;
+}
+
 double stgo_update_period(const stgo_system *s) {
     double mx = 0.0;
     for (int32_t t = 0; t < s->n_tets; ++t) {
         for (int32_t sp = 0; sp < s->n_specs; ++sp) {
-            double D = s->spec_dcst[sp];
-            if (!(D > 0.0)) continue;
             double r4[4];
-            for (int f = 0; f < 4; ++f) r4[f] = D * s->tet_face_g[4 * t + f];
+            face_rates(s, t, sp, r4);
             double sum = face_rate_sum(r4);
             if (sum > mx) mx = sum;
         }
@@ -446,7 +469,7 @@ static void topo_build(const stgo_system *s, topo *tp) {
      * reaction (reactions.cpp:35-40) */
     tp->tracked = (uint8_t *)calloc((size_t)s->n_specs, 1);
     for (int32_t sp = 0; sp < s->n_specs; ++sp) {
-        if (!(s->spec_dcst[sp] > 0.0)) continue;
+        if (!spec_diffuses(s, sp)) continue;
         for (int32_t r = 0; r < s->n_reacs; ++r)
             if (s->reac_upd[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
         for (int32_t r = 0; r < s->n_sreacs; ++r)
@@ -475,7 +498,10 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
     const double V = s->tet_vol[t];
     const double vscale = 1.0e3 * V * AVOGADRO;
     ostream st;
-    os_init(&st, s->seed, (uint32_t)t, step, s->stream, 0u);
+    const uint32_t ltet = s->tet_comp ? (uint32_t)s->tet_ltet[t] : (uint32_t)t;
+    const uint32_t stream = s->tet_comp ? s->comp_stream[s->tet_comp[t]] : s->stream;
+    const double *reac_k = s->tet_comp ? s->comp_reac_k + (size_t)s->tet_comp[t] * R : s->reac_k;
+    os_init(&st, s->seed, ltet, step, stream, 0u);
     long nev = 0;
     double tcur = 0.0;
     for (;;) {
@@ -492,7 +518,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 if (lhs[sp] > cnt[sp]) zero = 1;
                 else h = falling(h, cnt[sp], lhs[sp]);
             }
-            a[j] = zero ? 0.0 : h * scaled_const(s->reac_k[r], order, vscale);
+            a[j] = zero ? 0.0 : h * scaled_const(reac_k[r], order, vscale);
         }
         for (int32_t i = 0; i < nt; ++i) {
             const int32_t tr = tp->tri_idx[t0 + i];
@@ -623,13 +649,14 @@ static int32_t leave_tet_species(const stgo_system *s, int32_t t, int32_t sp, do
                                  int32_t *out4) {
     out4[0] = out4[1] = out4[2] = out4[3] = 0;
     const int32_t n = cnt[sp];
-    const double D = s->spec_dcst[sp];
     double r4[4];
-    for (int f = 0; f < 4; ++f) r4[f] = D * s->tet_face_g[4 * t + f];
+    face_rates(s, t, sp, r4);
     const double sum = face_rate_sum(r4);
     if (n <= 0 || !(sum > 0.0)) return 0;
     ostream st;
-    os_init(&st, s->seed, (uint32_t)t, step, s->stream, 1u + (uint32_t)sp);
+    const uint32_t ltet = s->tet_comp ? (uint32_t)s->tet_ltet[t] : (uint32_t)t;
+    const uint32_t stream = s->tet_comp ? s->comp_stream[s->tet_comp[t]] : s->stream;
+    os_init(&st, s->seed, ltet, step, stream, 1u + (uint32_t)sp);
     double occ = (double)n;
     if (tracked && !almost_equal(dt, 0.0)) occ = (double)n + corr_sp / dt;
     const double u = os_unif(&st);
@@ -729,7 +756,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                 ev_total += (uint64_t)nev;
                 for (int32_t sp = 0; sp < S; ++sp) {
                     int32_t *o4 = scratch + ((size_t)sp * N + t) * 4;
-                    if (s->spec_dcst[sp] > 0.0)
+                    if (spec_diffuses(s, sp))
                         mv_total += (uint64_t)leave_tet_species(s, t, sp, dt, step, cnt, corr[sp],
                                                                 tp.tracked[sp], o4);
                     else
@@ -746,7 +773,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
 #pragma omp parallel for schedule(static)
         for (int32_t t = 0; t < N; ++t) {
             for (int32_t sp = 0; sp < S; ++sp) {
-                if (!(s->spec_dcst[sp] > 0.0)) continue;
+                if (!spec_diffuses(s, sp)) continue;
                 int32_t add = 0;
                 for (int f = 0; f < 4; ++f) {
                     const int32_t nb = s->tet_nbr[4 * t + f];

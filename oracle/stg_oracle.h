@@ -75,6 +75,20 @@ typedef struct stgo_system {
     const int32_t *ghk_realflux;
     double temp;                 /* K */
     int32_t *tri_charge;
+    /* compartments (all null / 0 for one compartment): tet t is in
+     * compartment tet_comp[t], draws from stream comp_stream[c] with local
+     * index tet_ltet[t], reacts with comp_reac_k[c][r] and diffuses with
+     * comp_dcst[c][s]; a face f with tet_bdir[4t+f] = d >= 0 lies on a
+     * diffusion boundary and uses bnd_dcst[d][s] instead (tetopsplit.cpp:
+     * 2745-2931, diff.cpp:340-390) */
+    int32_t n_comps;
+    const int32_t *tet_comp;     /* [n_tets] */
+    const int32_t *tet_ltet;     /* [n_tets] */
+    const uint32_t *comp_stream; /* [n_comps] */
+    const double *comp_reac_k;   /* [n_comps*n_reacs] */
+    const double *comp_dcst;     /* [n_comps*n_specs] */
+    const int32_t *tet_bdir;     /* [n_tets*4], or null */
+    const double *bnd_dcst;      /* [directions*n_specs] */
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */

@@ -39,6 +39,8 @@ class StgMesh(C.Structure):
         ("n_tris", C.c_int32),
         ("tri_tet", _ip),
         ("tri_area", _dp),
+        ("tet_nbr_comp", _ip),
+        ("tet_face_bdir", _ip),
     ]
 
 
@@ -132,6 +134,10 @@ SIGNATURES = {
     "stg_set_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_get_tri_v": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_set_temp": (C.c_int, [C.c_void_p, C.c_double]),
+    "stg_create_comps": (C.c_int, [C.POINTER(StgModel), C.c_int32, C.POINTER(StgMesh),
+                                   C.POINTER(StgRealization), C.c_int32, C.POINTER(C.c_void_p)]),
+    "stg_set_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
+    "stg_get_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "stg_get_temp": (C.c_int, [C.c_void_p, _dp]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),

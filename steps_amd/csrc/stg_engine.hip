@@ -97,6 +97,10 @@ struct stg_sim {
     std::vector<std::vector<int32_t>> tri_perm;  // per realization: user tri -> internal tri
     int32_t NT = 0, NTRI = 0, max_tris = 0;
     int32_t NG = 0;  // GHK currents
+    bool linked = false;                   // compartments: one period schedule for all
+    int32_t NBD = 0;                       // diffusion-boundary directions
+    std::vector<double> h_bnd_D;           // [NBD][S]
+    std::vector<uint32_t> h_tet_bdir;      // [NT] 4 x 8-bit direction per face, 0xff = none
     std::vector<double> h_k, h_sk, h_D;  // per-realization tables
     std::vector<double> h_face_g;         // [NT*4], for update-period recomputation
     std::vector<double> vdep_vmin, vdep_dv, vdep_k;
@@ -127,6 +131,8 @@ struct stg_sim {
     double *d_tri_v = nullptr;
     double *d_tri_intg = nullptr;  // [SS][NTRI] time integrals of channel counts, lazily allocated
     int32_t *d_tri_charge = nullptr;  // [NTRI] GHK charge, when the model has GHK currents
+    uint32_t *d_tet_bdir = nullptr;   // [NT] when there are diffusion boundaries
+    double *d_bnd_D = nullptr;        // [NBD][S]
     double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
     int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
@@ -144,7 +150,12 @@ struct stg_sim {
     int32_t last_launches = 0;
     std::string err;
 
-    bool needs_memb() const { return NTRI > 0 && (RS > 0 || NG > 0 || mk.tri_intg_mask != 0); }
+    // the general variant of the period kernel: triangles that can hold a
+    // surface reaction, a GHK current or an integrated channel count, or
+    // faces on a diffusion boundary
+    bool needs_memb() const {
+        return (NTRI > 0 && (RS > 0 || NG > 0 || mk.tri_intg_mask != 0)) || NBD > 0;
+    }
 
     // per-thread LDS rows of the period kernel: counts, occupancy corrections,
     // rate constants, propensities and (membrane variant) GHK constants
@@ -160,13 +171,27 @@ struct stg_sim {
         for (int t = tet_off[r]; t < tet_off[r + 1]; ++t) {
             for (int s = 0; s < S; ++s) {
                 const double D = h_D[(size_t)r * S + s];
-                if (!(D > 0.0)) continue;
                 const double *g = &h_face_g[4 * (size_t)t];
-                const double sum = (((D * g[0]) + (D * g[1])) + (D * g[2])) + (D * g[3]);
+                double Df[4];
+                for (int f = 0; f < 4; ++f) {
+                    const uint32_t k = h_tet_bdir.empty() ? 0xffu : (h_tet_bdir[t] >> (8 * f)) & 0xffu;
+                    Df[f] = k == 0xffu ? D : h_bnd_D[(size_t)k * S + s];
+                }
+                const double sum = (((Df[0] * g[0]) + (Df[1] * g[1])) + (Df[2] * g[2])) + (Df[3] * g[3]);
                 if (sum > mx) mx = sum;
             }
         }
         return mx > DBL_EPSILON ? 1.0 / mx : INFINITY;
+    }
+
+    // split periods of every realization; compartments share the smallest
+    void refresh_dt() {
+        dt_std.resize(n_real);
+        for (int r = 0; r < n_real; ++r) dt_std[r] = compute_dt_std(r);
+        if (linked) {
+            const double m = *std::min_element(dt_std.begin(), dt_std.end());
+            std::fill(dt_std.begin(), dt_std.end(), m);
+        }
     }
 
     void free_graph() {
@@ -185,7 +210,7 @@ struct stg_sim {
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
-                        d_vdep_off,  d_tri_intg,    d_tri_charge};
+                        d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -376,6 +401,9 @@ int guard(stg_sim *s, F &&f) {
 }
 
 void upload_real_tables(stg_sim *s) {
+    if (!s->h_bnd_D.empty())
+        HIP_OK(hipMemcpyAsync(s->d_bnd_D, s->h_bnd_D.data(), sizeof(double) * s->h_bnd_D.size(),
+                              hipMemcpyHostToDevice, s->stream));
     if (!s->h_k.empty())
         HIP_OK(hipMemcpyAsync(s->d_real_k, s->h_k.data(), sizeof(double) * s->h_k.size(),
                               hipMemcpyHostToDevice, s->stream));
@@ -389,13 +417,14 @@ void upload_real_tables(stg_sim *s) {
 }
 
 void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_mesh *meshes,
-                 const stg_realization *reals, int32_t mode) {
+                 const stg_realization *reals, int32_t mode, bool linked = false) {
     if (n_real <= 0) throw StgError{STG_ERR_ARG, "need at least one realization"};
     if (!meshes || !reals) throw StgError{STG_ERR_ARG, "null mesh or realization array"};
     if (mode != STG_MODE_TETOPSPLIT && mode != STG_MODE_DISTTETOPSPLIT)
         throw StgError{STG_ERR_ARG, "unknown solver mode"};
     s->mode = mode;
     s->n_real = n_real;
+    s->linked = linked;
     build_model(s, model);
     const int S = s->S, R = s->R, RS = s->RS, SS = s->SS;
     s->tet_off.assign(n_real + 1, 0);
@@ -408,6 +437,8 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
             throw StgError{STG_ERR_ARG, "null mesh array"};
         if (m.n_tris > 0 && (!m.tri_tet || !m.tri_area))
             throw StgError{STG_ERR_ARG, "null triangle array"};
+        if (!linked && (m.tet_nbr_comp || m.tet_face_bdir))
+            throw StgError{STG_ERR_ARG, "compartment links need stg_create_comps"};
         const long long nt = (long long)s->tet_off[r] + m.n_tets;
         const long long ntr = (long long)s->tri_off[r] + m.n_tris;
         if (nt > INT32_MAX / 4 || ntr > INT32_MAX) throw StgError{STG_ERR_ARG, "mesh too large"};
@@ -427,6 +458,13 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     std::vector<double> tri_area(std::max(NTRI, 1));
     s->tri_perm.assign(n_real, {});
     s->max_tris = 0;
+    int32_t nbd = 0;
+    for (int r = 0; r < n_real; ++r)
+        if (meshes[r].tet_face_bdir)
+            for (int i = 0; i < 4 * meshes[r].n_tets; ++i) nbd = std::max(nbd, meshes[r].tet_face_bdir[i] + 1);
+    if (nbd > 255) throw StgError{STG_ERR_ARG, "at most 255 diffusion-boundary directions"};
+    s->NBD = nbd;
+    if (nbd > 0) s->h_tet_bdir.assign(NT, 0xffffffffu);
     for (int r = 0; r < n_real; ++r) {
         const stg_mesh &m = meshes[r];
         const int32_t t0 = s->tet_off[r];
@@ -438,13 +476,26 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
                 const int32_t j = m.tet_nbr[4 * t + f];
                 const int32_t jf = m.tet_nbr_face[4 * t + f];
                 const double g = m.tet_face_g[4 * t + f];
-                if (j >= m.n_tets || j < -1) throw StgError{STG_ERR_ARG, "neighbour index out of range"};
+                const int32_t jr = m.tet_nbr_comp ? m.tet_nbr_comp[4 * t + f] : r;
+                if (jr < 0 || jr >= n_real) throw StgError{STG_ERR_ARG, "neighbour compartment out of range"};
+                const stg_mesh &mj = meshes[jr];
+                if (j >= mj.n_tets || j < -1) throw StgError{STG_ERR_ARG, "neighbour index out of range"};
                 if (j >= 0 && (jf < 0 || jf > 3)) throw StgError{STG_ERR_ARG, "neighbour face slot out of range"};
-                if (j >= 0 && m.tet_nbr[4 * j + jf] != t)
+                if (j >= 0 && (mj.tet_nbr[4 * j + jf] != t ||
+                               (mj.tet_nbr_comp ? mj.tet_nbr_comp[4 * j + jf] : jr) != r))
                     throw StgError{STG_ERR_ARG, "neighbour table is not symmetric"};
                 if (!(g >= 0.0) || !std::isfinite(g)) throw StgError{STG_ERR_ARG, "face factor must be finite and >= 0"};
                 if (j < 0 && g != 0.0) throw StgError{STG_ERR_ARG, "boundary face with nonzero face factor"};
-                nb[f] = j >= 0 ? j + t0 : -1;
+                const int32_t bd = m.tet_face_bdir ? m.tet_face_bdir[4 * t + f] : -1;
+                if (bd < -1 || (bd >= 0 && j < 0))
+                    throw StgError{STG_ERR_ARG, "a diffusion-boundary face needs a neighbour"};
+                if (jr != r && j >= 0 && bd < 0)
+                    throw StgError{STG_ERR_ARG, "a face between compartments must be closed or on a diffusion boundary"};
+                if (bd >= 0) {
+                    s->h_tet_bdir[t0 + t] &= ~(0xffu << (8 * f));
+                    s->h_tet_bdir[t0 + t] |= (uint32_t)bd << (8 * f);
+                }
+                nb[f] = j >= 0 ? j + s->tet_off[jr] : -1;
                 nf |= (uint32_t)(j >= 0 ? jf : 0) << (8 * f);
                 s->h_face_g[4 * (size_t)(t0 + t) + f] = g;
             }
@@ -496,10 +547,10 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
         s->seeds[r] = q.seed;
         s->streams[r] = q.stream;
     }
+    s->h_bnd_D.assign((size_t)s->NBD * S, 0.0);
     build_diffusion_sets(s, model);
     (void)SS;
-    s->dt_std.resize(n_real);
-    for (int r = 0; r < n_real; ++r) s->dt_std[r] = s->compute_dt_std(r);
+    s->refresh_dt();
     s->nsteps.assign(n_real, 0);
 
     HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
@@ -519,6 +570,11 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     s->d_real_k = dalloc<double>(std::max<size_t>(1, s->h_k.size()));
     s->d_real_sk = dalloc<double>(std::max<size_t>(1, s->h_sk.size()));
     s->d_real_D = dalloc<double>(std::max<size_t>(1, s->h_D.size()));
+    if (s->NBD > 0) {
+        s->d_tet_bdir = dalloc<uint32_t>(NT);
+        HIP_OK(hipMemcpy(s->d_tet_bdir, s->h_tet_bdir.data(), sizeof(uint32_t) * NT, hipMemcpyHostToDevice));
+        s->d_bnd_D = dalloc<double>((size_t)s->NBD * S);
+    }
     s->d_real_seed = dalloc<uint64_t>(n_real);
     s->d_real_stream = dalloc<uint32_t>(n_real);
     s->d_real_step0 = dalloc<uint32_t>(n_real);
@@ -622,6 +678,8 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.vdep_k = s->d_vdep_k;
     k.tri_intg = s->d_tri_intg;
     k.tri_charge = s->d_tri_charge;
+    k.tet_bdir = s->d_tet_bdir;
+    k.bnd_D = s->d_bnd_D;
     return k;
 }
 
@@ -753,6 +811,50 @@ int stg_create_batch(const stg_model *model, int32_t n_real, const stg_mesh *mes
         return rc;
     }
     *out = s;
+    return STG_OK;
+}
+
+int stg_create_comps(const stg_model *model, int32_t n_comps, const stg_mesh *comp_meshes,
+                     const stg_realization *comp_params, int32_t mode, stg_sim **out) {
+    if (!out) {
+        g_create_error = "null output handle";
+        return STG_ERR_ARG;
+    }
+    *out = nullptr;
+    stg_sim *s = new (std::nothrow) stg_sim();
+    if (!s) {
+        g_create_error = "host allocation failed";
+        return STG_ERR_SYS;
+    }
+    const int rc = guard(nullptr, [&] { create_impl(s, model, n_comps, comp_meshes, comp_params, mode, true); });
+    if (rc != STG_OK) {
+        delete s;
+        return rc;
+    }
+    *out = s;
+    return STG_OK;
+}
+
+int stg_set_boundary_dcst(stg_sim *s, int32_t bdir, int32_t spec, double dcst) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (bdir < 0 || bdir >= s->NBD) throw StgError{STG_ERR_ARG, "boundary direction out of range"};
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        if (!(dcst >= 0.0) || !std::isfinite(dcst))
+            throw StgError{STG_ERR_ARG, "diffusion constant must be finite and >= 0"};
+        const bool diffuses = std::find(s->mk.dspec, s->mk.dspec + s->mk.n_diff, spec) != s->mk.dspec + s->mk.n_diff;
+        if (dcst > 0.0 && !diffuses)
+            throw StgError{STG_ERR_NOTIMPL, "species has no diffusion rule in any compartment"};
+        s->h_bnd_D[(size_t)bdir * s->S + spec] = dcst;
+        s->refresh_dt();
+        upload_real_tables(s);
+        HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
+int stg_get_boundary_dcst(const stg_sim *s, int32_t bdir, int32_t spec, double *dcst) {
+    if (!s || !dcst || bdir < 0 || bdir >= s->NBD || spec < 0 || spec >= s->S) return STG_ERR_ARG;
+    *dcst = s->h_bnd_D[(size_t)bdir * s->S + spec];
     return STG_OK;
 }
 
@@ -1006,7 +1108,7 @@ int stg_set_dcst(stg_sim *s, int32_t real, int32_t spec, double dcst) {
         if (dcst > 0.0 && !was_diff)
             throw StgError{STG_ERR_NOTIMPL, "enabling diffusion of a species that was immobile at creation"};
         s->h_D[(size_t)real * s->S + spec] = dcst;
-        s->dt_std[real] = s->compute_dt_std(real);
+        s->refresh_dt();
         upload_real_tables(s);
     });
 }
@@ -1065,7 +1167,7 @@ struct StateHeader {
 
 size_t state_bytes(const stg_sim *s) {
     return sizeof(StateHeader) + s->n_real * (sizeof(uint64_t) * 4) +
-           sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size()) +
+           sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size() + s->h_bnd_D.size()) +
            sizeof(int32_t) * ((size_t)s->S * s->NT + (size_t)s->SS * s->NTRI) +
            sizeof(double) * (size_t)s->NTRI;
 }
@@ -1100,7 +1202,7 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
             std::memcpy(p, v, sizeof(v));
             p += sizeof(v);
         }
-        for (const auto *vec : {&s->h_k, &s->h_sk, &s->h_D}) {
+        for (const auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D}) {
             std::memcpy(p, vec->data(), vec->size() * sizeof(double));
             p += vec->size() * sizeof(double);
         }
@@ -1141,7 +1243,7 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
                     placed[r] = true;
                 }
         }
-        for (auto *vec : {&s->h_k, &s->h_sk, &s->h_D}) {
+        for (auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D}) {
             std::memcpy(vec->data(), p, vec->size() * sizeof(double));
             p += vec->size() * sizeof(double);
         }
@@ -1156,7 +1258,7 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
         HIP_OK(hipMemcpy(s->d_real_seed, s->seeds.data(), sizeof(uint64_t) * s->n_real, hipMemcpyHostToDevice));
         s->time = h.time;
         s->diff_threshold = h.diff_threshold;
-        for (int r = 0; r < s->n_real; ++r) s->dt_std[r] = s->compute_dt_std(r);
+        s->refresh_dt();
         upload_real_tables(s);
         s->free_graph();
     });

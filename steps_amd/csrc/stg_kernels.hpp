@@ -103,6 +103,9 @@ struct SysK {
     const double *vdep_k;        // concatenated k(V) tables
     double *tri_intg;            // [SS][NTRI] time integral of counts (tri_intg_mask species)
     int32_t *tri_charge;         // [NTRI] GHK charge moved out (elementary charges), or null
+    const uint32_t *tet_bdir;    // [NT] 4 x 8-bit diffusion-boundary direction per face
+                                 // (0xff = none), or null
+    const double *bnd_D;         // [directions][n_specs] diffusion constant across a boundary
 };
 
 // both structures travel in the 4 KiB kernel-argument segment
@@ -358,14 +361,20 @@ __device__ __forceinline__ bool almost_zero_dt(double dt) {
 }
 
 // Leaving pass of one (tet, species): returns molecules moved, writes 4 slots.
+// Faces on a diffusion boundary (bdir byte != 0xff) take the boundary's
+// constant for this species instead of D (0 when it is closed).
 __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, double D,
-                         const double r4in[4], double dt, uint64_t seed, uint32_t ltet,
-                         uint32_t period, uint32_t stream, int4 &o) {
+                         const double r4in[4], uint32_t bdir, int S, double dt, uint64_t seed,
+                         uint32_t ltet, uint32_t period, uint32_t stream, int4 &o) {
     o = make_int4(0, 0, 0, 0);
     const int32_t n = L.c(sp);
     double r4[4];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) r4[f] = D * r4in[f];
+    for (int f = 0; f < 4; ++f) {
+        const uint32_t k = (bdir >> (8 * f)) & 0xffu;
+        const double Df = k == 0xffu ? D : sys.bnd_D[(size_t)k * S + sp];
+        r4[f] = Df * r4in[f];
+    }
     const double sum = ((r4[0] + r4[1]) + r4[2]) + r4[3];
     if (n <= 0 || !(sum > 0.0)) return 0;
     Philox g;
@@ -522,6 +531,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 const double2 g01 = sys.g[2 * (size_t)t];
                 const double2 g23 = sys.g[2 * (size_t)t + 1];
                 const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
+                const uint32_t bdir = (MEMB && sys.tet_bdir != nullptr) ? sys.tet_bdir[t] : 0xffffffffu;
                 const double *Dr = sys.real_D + (size_t)r * S;
                 const uint64_t seed = sys.real_seed[r];
                 const uint32_t stream = sys.real_stream[r];
@@ -529,8 +539,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 for (int d = 0; d < m.n_diff; ++d) {
                     const int sp = m.dspec[d];
                     int4 o;
-                    moved += leave(sys, L, sp, (m.tracked_mask >> sp) & 1, Dr[sp], r4, dt, seed,
-                                   ltet, period, stream, o);
+                    moved += leave(sys, L, sp, (m.tracked_mask >> sp) & 1, Dr[sp], r4, bdir, S, dt,
+                                   seed, ltet, period, stream, o);
                     oc[(size_t)d * NT + t] = o;
                 }
             }

@@ -20,8 +20,14 @@ def _mesh_struct(mt: MeshTables, keep: list) -> N.StgMesh:
             N.as_i32(mt.tri_tet), N.as_f64(mt.tri_area)]
     keep.extend(arrs)
     vol, nbr, nf, g, tt, ta = arrs
-    return N.StgMesh(len(vol), N.ptr(vol, C.c_double), N.ptr(nbr, C.c_int32), N.ptr(nf, C.c_int32),
-                     N.ptr(g, C.c_double), len(tt), N.ptr(tt, C.c_int32), N.ptr(ta, C.c_double))
+    m = N.StgMesh(len(vol), N.ptr(vol, C.c_double), N.ptr(nbr, C.c_int32), N.ptr(nf, C.c_int32),
+                  N.ptr(g, C.c_double), len(tt), N.ptr(tt, C.c_int32), N.ptr(ta, C.c_double))
+    if mt.nbr_comp is not None:
+        nc, bd = N.as_i32(mt.nbr_comp), N.as_i32(mt.bdir)
+        keep.extend([nc, bd])
+        m.tet_nbr_comp = N.ptr(nc, C.c_int32)
+        m.tet_face_bdir = N.ptr(bd, C.c_int32)
+    return m
 
 
 def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
@@ -50,7 +56,9 @@ class Engine:
     """One simulation handle: a single realization or a batch of them."""
 
     def __init__(self, fm: FlatModel, meshes, seeds, streams=None, reac_k=None, sreac_k=None,
-                 spec_dcst=None, mode: int = N.MODE_TETOPSPLIT):
+                 spec_dcst=None, mode: int = N.MODE_TETOPSPLIT, comps: bool = False):
+        """comps=True: the meshes are the compartments of one simulation
+        (stg_create_comps), on one period schedule."""
         self.lib = N.load()
         self.fm = fm
         if isinstance(meshes, MeshTables):
@@ -72,7 +80,8 @@ class Engine:
                     keep.append(a)
                     setattr(reals[r], name, N.ptr(a, C.c_double))
         h = C.c_void_p()
-        rc = self.lib.stg_create_batch(C.byref(model), n, mesh_arr, reals, mode, C.byref(h))
+        create = self.lib.stg_create_comps if comps else self.lib.stg_create_batch
+        rc = create(C.byref(model), n, mesh_arr, reals, mode, C.byref(h))
         N.check(rc, None)
         self.h = h
         self.n_real = n
@@ -189,6 +198,14 @@ class Engine:
         a = np.empty(self.n_tris[real])
         self._chk(self.lib.stg_get_tri_v(self.h, real, N.ptr(a, C.c_double)))
         return a
+
+    def set_boundary_dcst(self, bdir: int, spec: int, dcst: float):
+        self._chk(self.lib.stg_set_boundary_dcst(self.h, int(bdir), int(spec), float(dcst)))
+
+    def boundary_dcst(self, bdir: int, spec: int) -> float:
+        x = C.c_double()
+        self._chk(self.lib.stg_get_boundary_dcst(self.h, int(bdir), int(spec), C.byref(x)))
+        return x.value
 
     def set_temp(self, kelvin: float):
         self._chk(self.lib.stg_set_temp(self.h, float(kelvin)))

@@ -95,6 +95,7 @@ class Tetmesh:
         self.comps: dict[str, TmComp] = {}
         self.patches: dict[str, TmPatch] = {}
         self.membs: dict[str, Memb] = {}
+        self.diffbs: dict[str, DiffBoundary] = {}
 
     # --- reference-style accessors (src/steps/geom/tetmesh.hpp) ---
     def countTets(self) -> int:
@@ -150,6 +151,7 @@ class TmComp:
         self.tets = (np.arange(mesh.countTets()) if tets is None
                      else np.unique(np.asarray(tets, dtype=np.int64)))
         self.volsys: list[str] = []
+        self._tetset = set(self.tets.tolist())
         mesh.comps[name] = self
 
     def addVolsys(self, name: str) -> None:
@@ -176,6 +178,41 @@ class TmPatch:
 
     def getArea(self) -> float:
         return float(self.mesh.tri_area[self.tris].sum())
+
+
+class DiffBoundary:
+    """Triangles between two compartments across which chosen species may
+    diffuse once activated (src/steps/geom/diffboundary.cpp; API_1
+    steps.geom.DiffBoundary(id, container, tris))."""
+
+    def __init__(self, id: str, container: Tetmesh, tris):
+        self.id = id
+        self.mesh = container
+        self.tris = np.asarray(tris, dtype=np.int64)
+        tt = container.tri_tets[self.tris]
+        if np.any(tt[:, 1] < 0):
+            raise ValueError("a diffusion-boundary triangle must lie between two tetrahedra")
+        pairs = set()
+        for a, b in tt:
+            ca = [c for c in container.comps.values() if a in c._tetset]
+            cb = [c for c in container.comps.values() if b in c._tetset]
+            if len(ca) != 1 or len(cb) != 1 or ca[0] is cb[0]:
+                raise ValueError("a diffusion-boundary triangle must separate two compartments")
+            pairs.add(tuple(sorted((ca[0].name, cb[0].name))))
+        if len(pairs) != 1:
+            raise ValueError("a diffusion boundary must lie between exactly two compartments")
+        na, nb_ = pairs.pop()
+        self.compA, self.compB = container.comps[na], container.comps[nb_]
+        container.diffbs[id] = self
+
+    def getID(self) -> str:
+        return self.id
+
+    def getAllTriIndices(self) -> np.ndarray:
+        return self.tris.copy()
+
+    def getComps(self):
+        return [self.compA, self.compB]
 
 
 class Memb:
@@ -305,6 +342,75 @@ class MeshTables:
     face_g: np.ndarray     # (n,4) float64
     tri_tet: np.ndarray    # (m,) int32
     tri_area: np.ndarray   # (m,) float64
+    nbr_comp: np.ndarray | None = None   # (n,4) int32 compartment of the neighbour
+    bdir: np.ndarray | None = None       # (n,4) int32 diffusion-boundary direction, -1 none
+
+
+def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
+    """Tables of several compartments of one mesh (one MeshTables per
+    compartment, local tet numbering in mesh order). A face between two
+    compartments stays open only on a diffusion boundary: it then keeps its
+    neighbour (nbr_comp, nbr) and carries direction 2 b + side, side 0 when
+    the tet is in boundary b's compA (tetopsplit.cpp:2745-2931). Returns the
+    tables and, per direction, (boundary index, source comp, destination comp)."""
+    ncomp = len(comps)
+    comp_of = -np.ones(mesh.countTets(), dtype=np.int64)
+    local = -np.ones(mesh.countTets(), dtype=np.int64)
+    for ci, c in enumerate(comps):
+        if np.any(comp_of[c.tets] >= 0):
+            raise ValueError("a tetrahedron belongs to more than one compartment")
+        comp_of[c.tets] = ci
+        local[c.tets] = np.arange(len(c.tets))
+    # (tet, face) -> boundary direction
+    face_dir = {}
+    dirs = []
+    for bi, db in enumerate(diffbs):
+        ia = comps.index(db.compA)
+        for side in (0, 1):
+            dirs.append((bi, [ia, comps.index(db.compB)][side], [comps.index(db.compB), ia][side]))
+        for tri in db.tris:
+            for tet in mesh.tri_tets[tri]:
+                f = int(np.nonzero(mesh.tet_tri[tet] == tri)[0][0])
+                side = 0 if comp_of[tet] == ia else 1
+                face_dir[(int(tet), f)] = 2 * bi + side
+    out = []
+    for ci, c in enumerate(comps):
+        tets = c.tets
+        gnbr = mesh.tet_nbr[tets]
+        ncomp_arr = np.where(gnbr >= 0, comp_of[np.maximum(gnbr, 0)], -1)
+        bdir = -np.ones((len(tets), 4), dtype=np.int64)
+        for (tet, f), d in face_dir.items():
+            if comp_of[tet] == ci:
+                bdir[local[tet], f] = d
+        keep = (gnbr >= 0) & (ncomp_arr >= 0) & ((ncomp_arr == ci) | (bdir >= 0))
+        lnbr = np.where(keep, local[np.maximum(gnbr, 0)], -1)
+        nface = np.where(keep, mesh.tet_nbr_face[tets], -1)
+        area = mesh.tet_face_area[tets]
+        dist = mesh.tet_face_dist[tets]
+        vol = mesh.tet_vol[tets]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            g = np.where(keep, area / (vol[:, None] * dist), 0.0)
+        g = np.where(np.isfinite(g), g, 0.0)
+        ptris = patch_tris_by_comp.get(ci)
+        if ptris is None:
+            tri_tet, tri_area = np.zeros(0, dtype=np.int32), np.zeros(0)
+        else:
+            tris = np.asarray(ptris, dtype=np.int64)
+            tt = mesh.tri_tets[tris]
+            inner = np.where(comp_of[np.maximum(tt[:, 0], 0)] == ci, tt[:, 0], tt[:, 1])
+            if np.any(inner < 0) or np.any(comp_of[inner] != ci):
+                raise ValueError("patch triangle without an inner tet in its compartment")
+            tri_tet, tri_area = local[inner].astype(np.int32), mesh.tri_area[tris]
+        out.append(MeshTables(
+            vol=np.ascontiguousarray(vol, dtype=np.float64),
+            nbr=np.ascontiguousarray(lnbr, dtype=np.int32),
+            nbr_face=np.ascontiguousarray(nface, dtype=np.int32),
+            face_g=np.ascontiguousarray(g, dtype=np.float64),
+            tri_tet=np.ascontiguousarray(tri_tet, dtype=np.int32),
+            tri_area=np.ascontiguousarray(tri_area, dtype=np.float64),
+            nbr_comp=np.ascontiguousarray(np.where(keep, ncomp_arr, ci), dtype=np.int32),
+            bdir=np.ascontiguousarray(bdir, dtype=np.int32)))
+    return out, dirs
 
 
 def mesh_tables(mesh: Tetmesh, comp: TmComp, patch_tris=None) -> MeshTables:

@@ -399,3 +399,91 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.ohmic_g = np.array([c.g for c in ohm], dtype=np.float64)
     fm.ohmic_erev = np.array([c.erev for c in ohm], dtype=np.float64)
     return fm
+
+
+@dataclass
+class CompTables:
+    """Several compartments of one simulation: one flat model over the union of
+    their species and reactions (reaction i belongs to compartment
+    reac_comp[i]), with per-compartment constants (zero where a compartment
+    lacks a reaction or a diffusion rule), as Statedef holds one Compdef per
+    compartment (src/steps/solver/statedef.cpp, compdef.cpp)."""
+
+    fm: FlatModel
+    comp_reacs: list          # [comp] {reaction id: global index}
+    comp_diffs: list          # [comp] {diff id: species index}
+    comp_specs: list          # [comp] set of species names defined there
+    comp_k: np.ndarray        # [comp][reaction]
+    comp_dcst: np.ndarray     # [comp][species]
+
+
+def flatten_comps(model: Model, comps_volsys, surfsys_ids=()) -> CompTables:
+    """Compartments with the given volume systems (one list per compartment)
+    and at most one patch (surface systems) on the inner side of one of them."""
+    per = [flatten(model, vs, ()) for vs in comps_volsys]
+    patch = flatten(model, [], surfsys_ids) if surfsys_ids else None
+    used = set()
+    for f in per:
+        used.update(f.specs)
+    if patch is not None:
+        used.update(patch.specs)
+    order = list(model.specs)
+    specs = [s for s in order if s in used]
+    S = len(specs)
+    reacs = [(ci, model.volsys[v].reacs[r]) for ci, vs in enumerate(comps_volsys)
+             for v in vs for r in model.volsys[v].reacs]
+    R = len(reacs)
+    fm = FlatModel(specs=specs, sspecs=patch.sspecs if patch else [],
+                   reacs=[f"{ci}:{r.id}" for ci, r in reacs],
+                   sreacs=patch.sreacs if patch else [])
+    fm.reac_lhs = np.zeros((R, S), dtype=np.int32)
+    fm.reac_upd = np.zeros((R, S), dtype=np.int32)
+    for i, (_, r) in enumerate(reacs):
+        lhs = _count(r.lhs, specs)
+        fm.reac_lhs[i] = lhs
+        fm.reac_upd[i] = _count(r.rhs, specs) - lhs
+    fm.reac_k = np.zeros(R)
+    comp_k = np.zeros((len(per), R))
+    comp_reacs = [dict() for _ in per]
+    for i, (ci, r) in enumerate(reacs):
+        comp_k[ci, i] = r.kcst
+        comp_reacs[ci][r.id] = i
+    comp_dcst = np.zeros((len(per), S))
+    comp_diffs = []
+    for ci, f in enumerate(per):
+        d = {}
+        for did, sj in f.diffs.items():
+            sg = specs.index(f.specs[sj])
+            comp_dcst[ci, sg] = f.spec_dcst[sj]
+            d[did] = sg
+        comp_diffs.append(d)
+    fm.spec_dcst = comp_dcst.max(axis=0)
+    for did in set().union(*[d.keys() for d in comp_diffs]) if comp_diffs else ():
+        fm.diffs[did] = next(d[did] for d in comp_diffs if did in d)
+    if patch is not None:
+        SS = len(patch.sspecs)
+        n = len(patch.sreacs)
+        fm.sreac_lhs_s, fm.sreac_upd_s = patch.sreac_lhs_s, patch.sreac_upd_s
+        fm.sreac_lhs_i = np.zeros((n, S), dtype=np.int32)
+        fm.sreac_upd_i = np.zeros((n, S), dtype=np.int32)
+        for j, sp in enumerate(patch.specs):
+            g = specs.index(sp)
+            fm.sreac_lhs_i[:, g] = patch.sreac_lhs_i[:, j]
+            fm.sreac_upd_i[:, g] = patch.sreac_upd_i[:, j]
+        for name in ("sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv", "vdep_off", "vdep_k",
+                     "ghk", "ghk_chan", "ghk_valence", "ghk_perm", "ghk_voconc", "ghk_vshift",
+                     "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g", "ohmic_erev"):
+            setattr(fm, name, getattr(patch, name))
+        fm.ghk_ion = np.array([specs.index(patch.specs[i]) for i in patch.ghk_ion], dtype=np.int32)
+        del SS
+    else:
+        empty = flatten(Model(), [], ())
+        for name in ("sreac_lhs_s", "sreac_upd_s", "sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv",
+                     "vdep_off", "vdep_k", "ghk", "ghk_chan", "ghk_ion", "ghk_valence", "ghk_perm",
+                     "ghk_voconc", "ghk_vshift", "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g",
+                     "ohmic_erev"):
+            setattr(fm, name, getattr(empty, name))
+        fm.sreac_lhs_i = np.zeros((0, S), dtype=np.int32)
+        fm.sreac_upd_i = np.zeros((0, S), dtype=np.int32)
+    comp_specs = [set(f.specs) for f in per]
+    return CompTables(fm, comp_reacs, comp_diffs, comp_specs, comp_k, comp_dcst)

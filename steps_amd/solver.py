@@ -14,8 +14,8 @@ import numpy as np
 
 from . import _native as N
 from .engine import Engine
-from .geom import Tetmesh, mesh_tables
-from .model import Model, flatten
+from .geom import Tetmesh, comps_mesh_tables, mesh_tables
+from .model import Model, flatten, flatten_comps
 
 EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC = 0, 1, 2, 3  # src/steps/solver/api.hpp:61-66
 AVOGADRO = 6.02214179e23
@@ -73,25 +73,63 @@ class TetOpSplit:
     def _setup(self, model, geom, rng):
         if not isinstance(geom, Tetmesh):
             raise _argerr("geometry must be a Tetmesh")
-        if len(geom.comps) != 1:
-            raise _notimpl("exactly one compartment is supported (got %d)" % len(geom.comps))
         self.model, self.geom, self.rng = model, geom, rng
-        self.comp = next(iter(geom.comps.values()))
+        self.comps = list(geom.comps.values())
+        if not self.comps:
+            raise _argerr("the geometry has no compartment")
+        self._ci = {c.name: i for i, c in enumerate(self.comps)}
         patches = [p for p in geom.patches.values() if p.surfsys]
         if len(patches) > 1:
             raise _notimpl("at most one patch with surface systems is supported")
         self.patch = patches[0] if patches else None
-        if self.patch is not None and self.patch.icomp is not self.comp:
-            raise _argerr("the patch's inner compartment must be the solver's compartment")
-        self.fm = flatten(model, self.comp.volsys, self.patch.surfsys if self.patch else ())
-        self.mt = mesh_tables(geom, self.comp, self.patch.tris if self.patch is not None else None)
+        if self.patch is not None and self.patch.icomp.name not in self._ci:
+            raise _argerr("the patch's inner compartment is not in the geometry")
+        self._preal = self._ci[self.patch.icomp.name] if self.patch is not None else 0
+        ptris = self.patch.tris if self.patch is not None else None
+        ssys = self.patch.surfsys if self.patch else ()
+        nc = len(self.comps)
+        self.diffbs = list(geom.diffbs.values())
+        if nc == 1:
+            self.fm = flatten(model, self.comps[0].volsys, ssys)
+            self.mts = [mesh_tables(geom, self.comps[0], ptris)]
+            self._bdirs = []
+            self._comp_reacs = [{r: i for i, r in enumerate(self.fm.reacs)}]
+            self._comp_diffs = [dict(self.fm.diffs)]
+            self._comp_specs = [set(self.fm.specs)]
+            self._ck = self.fm.reac_k.copy()[None, :]
+            self._cd = self.fm.spec_dcst.copy()[None, :]
+        else:
+            ct = flatten_comps(model, [c.volsys for c in self.comps], ssys)
+            self.fm = ct.fm
+            self.mts, self._bdirs = comps_mesh_tables(
+                geom, self.comps, {self._preal: ptris} if ptris is not None else {}, self.diffbs)
+            self._comp_reacs, self._comp_diffs = ct.comp_reacs, ct.comp_diffs
+            self._comp_specs = ct.comp_specs
+            self._ck, self._cd = ct.comp_k.copy(), ct.comp_dcst.copy()
+        if self.patch is not None:
+            self._comp_specs[self._preal] |= {self.fm.specs[j] for j in
+                                              np.nonzero(np.abs(self.fm.sreac_lhs_i).sum(0)
+                                                         + np.abs(self.fm.sreac_upd_i).sum(0))[0]}
+        self.comp = self.comps[0]
+        self.mt = self.mts[self._preal]
+        # global tet -> (compartment, local index)
+        self._tet_real = -np.ones(geom.countTets(), dtype=np.int64)
         self._tet_local = -np.ones(geom.countTets(), dtype=np.int64)
-        self._tet_local[self.comp.tets] = np.arange(len(self.comp.tets))
+        for ci, c in enumerate(self.comps):
+            self._tet_real[c.tets] = ci
+            self._tet_local[c.tets] = np.arange(len(c.tets))
         self._tri_local = -np.ones(geom.countTris(), dtype=np.int64)
         if self.patch is not None:
             self._tri_local[self.patch.tris] = np.arange(len(self.patch.tris))
         self._seed = int(getattr(rng, "seed", 0))
-        self.eng = Engine(self.fm, [self.mt], [self._seed], mode=self._mode)
+        if nc == 1:
+            self.eng = Engine(self.fm, [self.mt], [self._seed], mode=self._mode)
+        else:
+            self.eng = Engine(self.fm, self.mts, [self._seed] * nc, streams=np.arange(nc),
+                              reac_k=self._ck, spec_dcst=self._cd, mode=self._mode, comps=True)
+        # diffusion boundaries: open directions and directional constants
+        self._db_active = set()        # (boundary, species, source comp)
+        self._db_dcst = {}             # (boundary, species, source comp) -> constant
         self._host_rng = np.random.default_rng([self._seed, 0x5EED])
         self.ef = None
 
@@ -130,7 +168,7 @@ class TetOpSplit:
                 raise _notimpl("at most 8 ohmic currents are supported")
             self.ef.set_ohmic(self.fm.ohmic_chan, self.fm.ohmic_g, self.fm.ohmic_erev)
         if self.patch is not None:
-            self.ef.attach(self.eng)
+            self.ef.attach(self.eng, self._preal)
 
     # --- identity ---
     def getSolverName(self):
@@ -183,10 +221,10 @@ class TetOpSplit:
         return self.eng.update_period()
 
     def getReacExtent(self, local: bool = False) -> int:
-        return self.eng.extents()[0]
+        return sum(self.eng.extents(r)[0] for r in range(self.eng.n_real))
 
     def getDiffExtent(self, local: bool = False) -> int:
-        return self.eng.extents()[1]
+        return sum(self.eng.extents(r)[1] for r in range(self.eng.n_real))
 
     def setDiffApplyThreshold(self, threshold: int):
         self.eng.set_diff_apply_threshold(threshold)
@@ -204,18 +242,26 @@ class TetOpSplit:
         except KeyError as e:
             raise _argerr(str(e)) from None
 
-    def _check_comp(self, c: str):
-        if c != self.comp.name:
+    def _check_comp(self, c: str) -> int:
+        if c not in self._ci:
             raise _argerr(f"unknown compartment {c!r}")
+        return self._ci[c]
+
+    def _comp_spec(self, ci: int, s: str) -> int:
+        sp = self._spec(s)
+        if s not in self._comp_specs[ci]:
+            raise _argerr(f"species {s!r} is undefined in compartment {self.comps[ci].name!r}")
+        return sp
 
     def _check_patch(self, p: str):
         if self.patch is None or p != self.patch.name:
             raise _argerr(f"unknown patch {p!r}")
 
-    def _tet(self, t: int) -> int:
+    def _tet(self, t: int):
+        """(compartment, local index) of a mesh tetrahedron."""
         if t < 0 or t >= len(self._tet_local) or self._tet_local[t] < 0:
-            raise _argerr(f"tetrahedron {t} is not in the solver's compartment")
-        return int(self._tet_local[t])
+            raise _argerr(f"tetrahedron {t} is not in a compartment")
+        return int(self._tet_real[t]), int(self._tet_local[t])
 
     def _tri(self, t: int) -> int:
         if t < 0 or t >= len(self._tri_local) or self._tri_local[t] < 0:
@@ -224,16 +270,16 @@ class TetOpSplit:
 
     # --- compartments ---
     def getCompVol(self, c: str) -> float:
-        self._check_comp(c)
-        return float(self.mt.vol.sum())
+        return float(self.mts[self._check_comp(c)].vol.sum())
 
     def getCompSpecCount(self, c: str, s: str) -> float:
-        self._check_comp(c)
-        return float(self.eng.sum_tet_counts(self._spec(s)))
+        ci = self._check_comp(c)
+        return float(self.eng.sum_tet_counts(self._comp_spec(ci, s), real=ci))
 
     def setCompSpecCount(self, c: str, s: str, n: float):
-        self._check_comp(c)
-        self.eng.set_tet_counts(self._spec(s), distribute_counts(n, self.mt.vol, self._host_rng))
+        ci = self._check_comp(c)
+        self.eng.set_tet_counts(self._comp_spec(ci, s),
+                                distribute_counts(n, self.mts[ci].vol, self._host_rng), real=ci)
 
     def getCompSpecAmount(self, c: str, s: str) -> float:
         return self.getCompSpecCount(c, s) / AVOGADRO
@@ -247,69 +293,127 @@ class TetOpSplit:
     def setCompSpecConc(self, c: str, s: str, conc: float):
         self.setCompSpecCount(c, s, conc * 1e3 * self.getCompVol(c) * AVOGADRO)
 
+    def _reac(self, ci: int, r: str) -> int:
+        if r not in self._comp_reacs[ci]:
+            raise _argerr(f"unknown reaction {r!r} in compartment {self.comps[ci].name!r}")
+        return self._comp_reacs[ci][r]
+
     def getCompReacK(self, c: str, r: str) -> float:
-        self._check_comp(c)
-        return float(self._reac_k()[self._reac(r)])
+        ci = self._check_comp(c)
+        return float(self._ck[ci, self._reac(ci, r)])
 
     def setCompReacK(self, c: str, r: str, k: float):
-        self._check_comp(c)
-        j = self._reac(r)
-        self.eng.set_reac_k(j, k)
-        self._kcache[j] = k
+        ci = self._check_comp(c)
+        j = self._reac(ci, r)
+        self.eng.set_reac_k(j, k, real=ci)
+        self._ck[ci, j] = k
+
+    def _diff(self, ci: int, d: str) -> int:
+        if d not in self._comp_diffs[ci]:
+            raise _argerr(f"unknown diffusion rule {d!r} in compartment {self.comps[ci].name!r}")
+        return self._comp_diffs[ci][d]
 
     def getCompDiffD(self, c: str, d: str) -> float:
-        self._check_comp(c)
-        return float(self._dcst()[self._diff(d)])
+        ci = self._check_comp(c)
+        return float(self._cd[ci, self._diff(ci, d)])
 
     def setCompDiffD(self, c: str, d: str, dcst: float, direction_comp=None):
-        self._check_comp(c)
-        s = self._diff(d)
-        self.eng.set_dcst(s, dcst)
-        self._dcache[s] = dcst
+        ci = self._check_comp(c)
+        s = self._diff(ci, d)
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        self.eng.set_dcst(s, dcst, real=ci)
+        self._cd[ci, s] = dcst
+        # Diff::setDcst also resets the rule's boundary directions to the new
+        # constant (diff.cpp:340-390)
+        for key in [k for k in self._db_dcst if k[1] == s and k[2] == ci]:
+            del self._db_dcst[key]
+        self._push_boundaries(s)
 
-    def _reac(self, r):
-        try:
-            return self.fm.reacs.index(r)
-        except ValueError:
-            raise _argerr(f"unknown reaction {r!r}") from None
+    # --- diffusion boundaries (tetopsplit.cpp:2745-2931) ---
+    def _db(self, db: str) -> int:
+        for i, b in enumerate(self.diffbs):
+            if b.id == db:
+                return i
+        raise _argerr(f"unknown diffusion boundary {db!r}")
 
-    def _diff(self, d):
-        if d not in self.fm.diffs:
-            raise _argerr(f"unknown diffusion rule {d!r}")
-        return self.fm.diffs[d]
+    def _db_spec(self, bi: int, s: str) -> int:
+        sp = self._spec(s)
+        b = self.diffbs[bi]
+        for c in (b.compA, b.compB):
+            if s not in self._comp_specs[self._ci[c.name]]:
+                raise _argerr("Species undefined in compartments connected by diffusion boundary.")
+        return sp
 
-    @property
-    def _kcache(self):
-        if not hasattr(self, "_kc"):
-            self._kc = self.fm.reac_k.copy()
-        return self._kc
+    def _push_boundaries(self, sp: int):
+        """Boundary constants of species sp from the activation state: a
+        direction from compartment c uses c's diffusion constant for the
+        species, or the directional constant set for it; 0 when inactive or
+        when c has no diffusion rule for it (no Diff kproc there)."""
+        for d, (bi, src, _dst) in enumerate(self._bdirs):
+            has_rule = sp in self._comp_diffs[src].values()
+            on = (bi, sp, src) in self._db_active and has_rule
+            val = self._db_dcst.get((bi, sp, src), self._cd[src, sp]) if on else 0.0
+            self.eng.set_boundary_dcst(d, sp, val)
 
-    def _reac_k(self):
-        return self._kcache
+    def setDiffBoundarySpecDiffusionActive(self, db: str, s: str, act: bool):
+        bi = self._db(db)
+        sp = self._db_spec(bi, s)
+        b = self.diffbs[bi]
+        for c in (b.compA, b.compB):
+            key = (bi, sp, self._ci[c.name])
+            if act:
+                self._db_active.add(key)
+            else:
+                self._db_active.discard(key)
+            self._db_dcst.pop(key, None)    # setDiffBndActive -> setDcst clears overrides
+        self._push_boundaries(sp)
 
-    @property
-    def _dcache(self):
-        if not hasattr(self, "_dc"):
-            self._dc = self.fm.spec_dcst.copy()
-        return self._dc
+    def getDiffBoundarySpecDiffusionActive(self, db: str, s: str) -> bool:
+        """True when the species crosses the boundary in both directions."""
+        bi = self._db(db)
+        sp = self._db_spec(bi, s)
+        b = self.diffbs[bi]
+        return all((bi, sp, self._ci[c.name]) in self._db_active for c in (b.compA, b.compB))
 
-    def _dcst(self):
-        return self._dcache
+    def setDiffBoundarySpecDcst(self, db: str, s: str, dcst: float, direction_comp=None):
+        """Activate the boundary for s with constant dcst from both sides, or
+        only into direction_comp (tetopsplit.cpp:2860-2917)."""
+        bi = self._db(db)
+        sp = self._db_spec(bi, s)
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        b = self.diffbs[bi]
+        dest = None
+        if direction_comp is not None:
+            name = direction_comp if isinstance(direction_comp, str) else direction_comp.name
+            if name not in (b.compA.name, b.compB.name):
+                raise _argerr("direction compartment is not on the diffusion boundary")
+            dest = self._ci[name]
+        for c in (b.compA, b.compB):
+            src = self._ci[c.name]
+            if src != dest:
+                self._db_active.add((bi, sp, src))
+                self._db_dcst[(bi, sp, src)] = float(dcst)
+        self._push_boundaries(sp)
 
     # --- tetrahedra ---
     def getTetVol(self, t: int) -> float:
-        return float(self.mt.vol[self._tet(t)])
+        ci, i = self._tet(t)
+        return float(self.mts[ci].vol[i])
 
     def getTetSpecCount(self, t: int, s: str) -> float:
-        return float(self.eng.batch_tet_counts(self._spec(s), [self._tet(t)])[0])
+        ci, i = self._tet(t)
+        return float(self.eng.batch_tet_counts(self._comp_spec(ci, s), [i], real=ci)[0])
 
     def setTetSpecCount(self, t: int, s: str, n: float):
         if n < 0:
             raise _argerr("negative number of molecules")
-        sp = self._spec(s)
-        c = self.eng.get_tet_counts(sp)
-        c[self._tet(t)] = int(np.floor(n) + (1 if self._host_rng.random() < n - np.floor(n) else 0))
-        self.eng.set_tet_counts(sp, c)
+        ci, i = self._tet(t)
+        sp = self._comp_spec(ci, s)
+        c = self.eng.get_tet_counts(sp, real=ci)
+        c[i] = int(np.floor(n) + (1 if self._host_rng.random() < n - np.floor(n) else 0))
+        self.eng.set_tet_counts(sp, c, real=ci)
 
     def getTetSpecConc(self, t: int, s: str) -> float:
         return self.getTetSpecCount(t, s) / (1e3 * self.getTetVol(t) * AVOGADRO)
@@ -318,21 +422,28 @@ class TetOpSplit:
         self.setTetSpecCount(t, s, conc * 1e3 * self.getTetVol(t) * AVOGADRO)
 
     def getBatchTetSpecCounts(self, tets, s: str):
-        return self.eng.batch_tet_counts(self._spec(s), [self._tet(int(t)) for t in tets]).tolist()
+        c = np.zeros(len(tets))
+        self.getBatchTetSpecCountsNP(np.asarray(tets, dtype=np.int64), s, c)
+        return c.tolist()
 
     def getBatchTetSpecCountsNP(self, index_array, s: str, counts):
         idx = np.asarray(index_array, dtype=np.int64)
         if len(counts) != len(idx):
             raise _argerr("counts and index arrays differ in length")
-        loc = self._tet_local[idx] if len(idx) else idx
-        if np.any(idx < 0) or np.any(idx >= len(self._tet_local)) or np.any(loc < 0):
+        if len(idx) and (np.any(idx < 0) or np.any(idx >= len(self._tet_local))
+                         or np.any(self._tet_local[idx] < 0)):
             raise _argerr("tetrahedron index out of range")
-        counts[:] = self.eng.batch_tet_counts(self._spec(s), loc)
+        sp = self._spec(s)
+        real, loc = self._tet_real[idx], self._tet_local[idx]
+        for ci in np.unique(real):
+            m = real == ci
+            counts[m] = self.eng.batch_tet_counts(sp, loc[m], real=int(ci))
 
     def getBatchTetSpecConcsNP(self, index_array, s: str, concs):
         self.getBatchTetSpecCountsNP(index_array, s, concs)
         idx = np.asarray(index_array, dtype=np.int64)
-        concs[:] = concs / (1e3 * self.mt.vol[self._tet_local[idx]] * AVOGADRO)
+        vol = np.array([self.mts[r].vol[i] for r, i in zip(self._tet_real[idx], self._tet_local[idx])])
+        concs[:] = concs / (1e3 * vol * AVOGADRO)
 
     def sumBatchTetCountsNP(self, tet_array, s: str) -> float:
         c = np.zeros(len(tet_array))
@@ -342,12 +453,12 @@ class TetOpSplit:
     # --- patches and triangles ---
     def getPatchSpecCount(self, p: str, s: str) -> float:
         self._check_patch(p)
-        return float(self.eng.sum_tri_counts(self._sspec(s)))
+        return float(self.eng.sum_tri_counts(self._sspec(s), real=self._preal))
 
     def setPatchSpecCount(self, p: str, s: str, n: float):
         self._check_patch(p)
         self.eng.set_tri_counts(self._sspec(s),
-                                distribute_counts(n, self.mt.tri_area, self._host_rng))
+                                distribute_counts(n, self.mt.tri_area, self._host_rng), real=self._preal)
 
     def getPatchSReacK(self, p: str, r: str) -> float:
         self._check_patch(p)
@@ -356,7 +467,7 @@ class TetOpSplit:
     def setPatchSReacK(self, p: str, r: str, k: float):
         self._check_patch(p)
         j = self._sreac(r)
-        self.eng.set_sreac_k(j, k)
+        self.eng.set_sreac_k(j, k, real=self._preal)
         self.fm.sreac_k[j] = k
 
     def _sreac(self, r):
@@ -366,26 +477,26 @@ class TetOpSplit:
             raise _argerr(f"unknown surface reaction {r!r}") from None
 
     def getTriSpecCount(self, tri: int, s: str) -> float:
-        return float(self.eng.get_tri_counts(self._sspec(s))[self._tri(tri)])
+        return float(self.eng.get_tri_counts(self._sspec(s), real=self._preal)[self._tri(tri)])
 
     def setTriSpecCount(self, tri: int, s: str, n: float):
         if n < 0:
             raise _argerr("negative number of molecules")
         sp = self._sspec(s)
-        c = self.eng.get_tri_counts(sp)
+        c = self.eng.get_tri_counts(sp, real=self._preal)
         c[self._tri(tri)] = int(round(n))
-        self.eng.set_tri_counts(sp, c)
+        self.eng.set_tri_counts(sp, c, real=self._preal)
 
     def getBatchTriSpecCounts(self, tris, s: str):
-        c = self.eng.get_tri_counts(self._sspec(s))
+        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
         return [float(c[self._tri(int(t))]) for t in tris]
 
     def sumBatchTriCountsNP(self, tri_array, s: str) -> float:
-        c = self.eng.get_tri_counts(self._sspec(s))
+        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
         return float(c[[self._tri(int(t)) for t in tri_array]].sum())
 
     def getBatchTriSpecCountsNP(self, index_array, s: str, counts):
-        c = self.eng.get_tri_counts(self._sspec(s))
+        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
         counts[:] = c[[self._tri(int(t)) for t in index_array]]
 
     # --- membrane potential (TetOpSplitP EField methods, tetopsplit.cpp:4751-5000) ---
@@ -499,7 +610,7 @@ class TetOpSplit:
         """Potential (V) of a membrane triangle, the mean of its vertices (EField::getTriV,
         efield.cpp:290-302). Without an EField, the value voltage-dependent reactions read."""
         if self.ef is None:
-            return float(self.eng.get_tri_v()[self._tri(tri)])
+            return float(self.eng.get_tri_v(self._preal)[self._tri(tri)])
         return float(self.ef.tris_v()[self._eftri(tri)])
 
     def setTriV(self, tri: int, v: float):
@@ -508,9 +619,9 @@ class TetOpSplit:
         tetopsplit.cpp:4871-4893). Without one, only the value voltage-dependent
         reactions read (no counterpart in the reference, which raises)."""
         if self.ef is None:
-            vv = self.eng.get_tri_v()
+            vv = self.eng.get_tri_v(self._preal)
             vv[self._tri(tri)] = v
-            self.eng.set_tri_v(vv)
+            self.eng.set_tri_v(vv, self._preal)
             return
         self._eftri(tri)
         vv = self.ef.verts_v()
@@ -542,13 +653,13 @@ class TetOpSplit:
             j = self.fm.ohmic.index(oc)
         except ValueError:
             raise _argerr(f"unknown ohmic current {oc!r}") from None
-        n = self.eng.get_tri_counts(int(self.fm.ohmic_chan[j]))[self._tri(tri)]
+        n = self.eng.get_tri_counts(int(self.fm.ohmic_chan[j]), real=self._preal)[self._tri(tri)]
         v = self.ef.tris_v()[k]
         return float(n * self.fm.ohmic_g[j] * (v - self.fm.ohmic_erev[j]))
 
     def getBatchTriVsNP(self, index_array, vs):
         if self.ef is None:
-            vv = self.eng.get_tri_v()
+            vv = self.eng.get_tri_v(self._preal)
             vs[:] = vv[[self._tri(int(t)) for t in index_array]]
         else:
             vv = self.ef.tris_v()
@@ -603,9 +714,10 @@ class DistTetOpSplit(TetOpSplit):
 
     def setCompSpecCount(self, c: str, s: str, n: float, distribution: int = 0):
         if distribution == self.MULTINOMIAL:
-            self._check_comp(c)
-            cnt = self._host_rng.multinomial(int(round(n)), self.mt.vol / self.mt.vol.sum())
-            self.eng.set_tet_counts(self._spec(s), cnt.astype(np.int32))
+            ci = self._check_comp(c)
+            vol = self.mts[ci].vol
+            cnt = self._host_rng.multinomial(int(round(n)), vol / vol.sum())
+            self.eng.set_tet_counts(self._comp_spec(ci, s), cnt.astype(np.int32), real=ci)
         else:
             super().setCompSpecCount(c, s, n)
 

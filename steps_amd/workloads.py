@@ -181,3 +181,36 @@ def ab_binding_ensemble(n_real=1024, n_min=6, n_max=20, side=10e-6, seed=0, end_
         counts.append(c)
     reac_k = np.stack([kfs, kbs], axis=1)
     return fm, meshes, counts, reac_k, dict(sizes=sizes.tolist(), seed=seed, end_time=end_time)
+
+
+def two_comp_cube(n=6, side=6e-6, dA=1e-12, dC_left=5e-13, dC_right=2e-12):
+    """Kuhn cube cut at x = side/2 into compartments 'left' and 'right' with a
+    diffusion boundary 'mid' on the cut. left: A + B -> C (1e6 /M/s), A and C
+    diffuse (B is immobile there); right: C -> A + B (2 /s), A, B and C
+    diffuse, and species R exists only on the right. Returns (model, mesh,
+    {'left', 'right'} comps, boundary)."""
+    mesh = geom.kuhn_cube(n, side)
+    left = np.nonzero(mesh.tet_bary[:, 0] < side / 2)[0]
+    right = np.nonzero(mesh.tet_bary[:, 0] >= side / 2)[0]
+    cl = geom.TmComp("left", mesh, left)
+    cr = geom.TmComp("right", mesh, right)
+    cl.addVolsys("vl")
+    cr.addVolsys("vr")
+    tt = mesh.tri_tets
+    both = (tt[:, 1] >= 0)
+    side_of = np.zeros(mesh.countTets(), dtype=np.int64)
+    side_of[right] = 1
+    cut = np.nonzero(both & (side_of[tt[:, 0]] != side_of[np.maximum(tt[:, 1], 0)]))[0]
+    db = geom.DiffBoundary("mid", mesh, cut)
+    mdl = M.Model()
+    A, B, C = M.Spec("A", mdl), M.Spec("B", mdl), M.Spec("C", mdl)
+    vl, vr = M.Volsys("vl", mdl), M.Volsys("vr", mdl)
+    M.Reac("bind", vl, lhs=[A, B], rhs=[C], kcst=1e6)
+    M.Diff("dA", vl, A, dA)
+    M.Diff("dC", vl, C, dC_left)
+    M.Reac("split", vr, lhs=[C], rhs=[A, B], kcst=2.0)
+    M.Diff("dA", vr, A, dA)
+    M.Diff("dB", vr, B, dA / 2)
+    M.Diff("dC", vr, C, dC_right)
+    M.Diff("dR", vr, M.Spec("R", mdl), dA)
+    return mdl, mesh, {"left": cl, "right": cr}, db

@@ -1,0 +1,123 @@
+"""Compartments and diffusion boundaries on the GPU: the engine created with
+stg_create_comps against the oracle's compartment layout, bit-for-bit, and
+the reference's diffusion-boundary calls through the solver interface."""
+import numpy as np
+import pytest
+
+from oracle import bind
+from steps_amd import _native as N, geom, model as M, workloads as W
+from steps_amd import rng as R
+from steps_amd.engine import Engine
+from steps_amd.solver import TetOpSplit, DistTetOpSplit
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(n=4, side=4e-6):
+    mdl, mesh, comps, db = W.two_comp_cube(n=n, side=side)
+    ct = M.flatten_comps(mdl, [["vl"], ["vr"]])
+    mts, dirs = geom.comps_mesh_tables(mesh, [comps["left"], comps["right"]], {}, [db])
+    return ct, mts, dirs
+
+
+def _gpu_counts(eng, S, n_comp):
+    return np.concatenate([np.stack([eng.get_tet_counts(s, real=c) for s in range(S)], axis=0)
+                           for c in range(n_comp)], axis=1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_two_compartments_bit_exact(mode):
+    ct, mts, dirs = _setup()
+    print(f"case two compartments mode={mode}", flush=True)
+    S = len(ct.fm.specs)
+    eng = Engine(ct.fm, mts, [21, 21], streams=[0, 1], reac_k=ct.comp_k, spec_dcst=ct.comp_dcst,
+                 mode=mode, comps=True)
+    o = bind.OracleSim.comps(ct.fm, mts, 21, ct.comp_k, ct.comp_dcst, n_bdir=len(dirs))
+    a, b, c = (ct.fm.specs.index(x) for x in "ABC")
+    # A open both ways at its own constant; C one-way right -> left at 1e-12
+    for d, (_, src, dst) in enumerate(dirs):
+        eng.set_boundary_dcst(d, a, ct.comp_dcst[src, a])
+        o.bnd_dcst[d, a] = ct.comp_dcst[src, a]
+        if src == 1:
+            eng.set_boundary_dcst(d, c, 1e-12)
+            o.bnd_dcst[d, c] = 1e-12
+    rng = np.random.default_rng(2)
+    init = rng.poisson(25, o.counts.shape).astype(np.int32)
+    init[b, :o.comp_off[1]] = 0
+    o.counts[:] = init
+    for ci in range(2):
+        for s in range(S):
+            eng.set_tet_counts(s, init[s, o.comp_off[ci]:o.comp_off[ci + 1]], real=ci)
+    assert abs(eng.update_period() - o.update_period()) == 0
+    for t in (0.02, 0.07, 0.2):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, S, 2), o.counts), t
+    assert eng.nsteps(0) == eng.nsteps(1) == o.nsteps
+    ev = sum(eng.extents(r)[0] for r in range(2))
+    mv = sum(eng.extents(r)[1] for r in range(2))
+    assert (ev, mv) == (o.events.value, o.moved.value)
+    assert eng.boundary_dcst(0, a) == ct.comp_dcst[0, a]
+
+
+def test_diffusion_boundary_through_the_solver():
+    mdl, mesh, comps, db = W.two_comp_cube(n=4, side=4e-6)
+    r = R.create("r123", 512)
+    r.initialize(5)
+    print("case solver diffusion boundary", flush=True)
+    sim = TetOpSplit(mdl, mesh, r)
+    with pytest.raises(N.ArgErr):
+        sim.setCompSpecCount("left", "D_unknown", 10)
+    sim.setCompSpecCount("left", "A", 4000)
+    sim.setCompSpecCount("right", "C", 0)
+    assert sim.getCompSpecCount("left", "A") == 4000
+    assert not sim.getDiffBoundarySpecDiffusionActive("mid", "A")
+    sim.run(0.5)
+    assert sim.getCompSpecCount("right", "A") == 0          # closed boundary
+    sim.setDiffBoundarySpecDiffusionActive("mid", "A", True)
+    assert sim.getDiffBoundarySpecDiffusionActive("mid", "A")
+    sim.run(1.0)
+    left, right = sim.getCompSpecCount("left", "A"), sim.getCompSpecCount("right", "A")
+    assert right > 0 and left + right + 2 * sim.getCompSpecCount("left", "C") <= 4000
+    # one-way: only into the left compartment
+    sim.setDiffBoundarySpecDiffusionActive("mid", "A", False)
+    sim.setDiffBoundarySpecDcst("mid", "A", 1e-12, direction_comp="left")
+    before = sim.getCompSpecCount("right", "A")
+    sim.run(1.5)
+    assert sim.getCompSpecCount("right", "A") <= before
+    # a species not defined on both sides cannot cross (R is only on the right)
+    with pytest.raises(N.ArgErr):
+        sim.setDiffBoundarySpecDiffusionActive("mid", "R", True)
+    with pytest.raises(N.ArgErr):
+        sim.setCompSpecCount("left", "R", 5)
+    # per-compartment reaction constants
+    sim.setCompReacK("right", "split", 4.0)
+    assert sim.getCompReacK("right", "split") == 4.0
+    with pytest.raises(N.ArgErr):
+        sim.getCompReacK("left", "split")
+    assert sim.getTetVol(0) > 0
+
+
+def test_dist_solver_with_compartments_matches_oracle():
+    mdl, mesh, comps, db = W.two_comp_cube(n=4, side=4e-6)
+    r = R.create("r123", 512)
+    r.initialize(17)
+    print("case dist solver compartments vs oracle", flush=True)
+    sim = DistTetOpSplit(mdl, mesh, r)
+    sim.setDiffBoundarySpecDiffusionActive("mid", "A", True)
+    sim.setDiffBoundarySpecDcst("mid", "C", 3e-13, direction_comp="right")
+    o = bind.OracleSim.comps(sim.fm, sim.mts, 17, sim._ck, sim._cd, n_bdir=len(sim._bdirs))
+    for d in range(len(sim._bdirs)):
+        for s in range(len(sim.fm.specs)):
+            o.bnd_dcst[d, s] = sim.eng.boundary_dcst(d, s)
+    sim.setCompSpecCount("left", "A", 3000)
+    sim.setCompSpecCount("right", "C", 2000)
+    for ci, name in enumerate(["left", "right"]):
+        for s in range(len(sim.fm.specs)):
+            o.counts[s, o.comp_off[ci]:o.comp_off[ci + 1]] = sim.eng.get_tet_counts(s, real=ci)
+    sim.run(0.3)
+    o.run(0.3, 1)
+    S = len(sim.fm.specs)
+    got = np.concatenate([np.stack([sim.eng.get_tet_counts(s, real=c) for s in range(S)])
+                          for c in range(2)], axis=1)
+    assert np.array_equal(got, o.counts)
