@@ -57,6 +57,12 @@ typedef struct stg_mesh {
                                     that compartment's mesh */
     const int32_t *tet_face_bdir;/* [n_tets*4] -1, or the diffusion-boundary direction d whose
                                     per-species constant (stg_set_boundary_dcst) opens the face */
+    /* Surface diffusion (SDiff, src/steps/mpi/tetopsplit/sdiff.cpp:42-90); NULL when no
+     * surface species diffuses. Edge e of triangle i joins it to tri_nbr[3i+e] (-1 none),
+     * whose slot for that edge is tri_nbr_edge[3i+e]; tri_edge_g = l_e / (A d_e), m^-2. */
+    const int32_t *tri_nbr;      /* [n_tris*3] */
+    const int32_t *tri_nbr_edge; /* [n_tris*3] */
+    const double *tri_edge_g;    /* [n_tris*3] */
 } stg_mesh;
 
 /* Flattened model of one compartment (+ one patch), as the reference's
@@ -100,6 +106,7 @@ typedef struct stg_model {
     const double *ghk_voconc;    /* [n_ghk] virtual outer concentration, M, >= 0 */
     const double *ghk_vshift;    /* [n_ghk] V added to the membrane potential */
     const int32_t *ghk_realflux; /* [n_ghk] 1: the ion count changes per event */
+    const double *sspec_dcst;    /* [n_sspecs] surface diffusion constant m^2/s, or NULL */
 } stg_model;
 
 /* Per-realization parameters of a batch; NULL arrays take the model's values. */
@@ -198,6 +205,8 @@ int stg_sum_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int64_t *total
 int stg_set_reac_k(stg_sim *sim, int32_t real, int32_t reac, double k);
 int stg_set_sreac_k(stg_sim *sim, int32_t real, int32_t sreac, double k);
 int stg_set_dcst(stg_sim *sim, int32_t real, int32_t spec, double dcst);
+/* Surface diffusion constant of surface species sspec (setPatchSDiffD). */
+int stg_set_sdcst(stg_sim *sim, int32_t real, int32_t sspec, double dcst);
 
 /* Launch shape used by stg_run: threads per block (64..1024, multiple of 64)
  * and whether the launches of one run are replayed from a captured HIP graph.

@@ -37,6 +37,8 @@ class StgoSystem(C.Structure):
         ("n_comps", C.c_int32), ("tet_comp", _ip), ("tet_ltet", _ip),
         ("comp_stream", C.POINTER(C.c_uint32)), ("comp_reac_k", _dp), ("comp_dcst", _dp),
         ("tet_bdir", _ip), ("bnd_dcst", _dp),
+        ("tri_nbr", _ip), ("tri_nbr_edge", _ip), ("tri_edge_g", _dp), ("sspec_dcst", _dp),
+        ("tri_ltri", _ip),
     ]
 
 
@@ -157,7 +159,10 @@ class OracleSim:
             go=f64(_or(fm, "ghk_voconc", [])), gv=f64(_or(fm, "ghk_vshift", [])),
             gr=i32(_or(fm, "ghk_realflux", [])),
             charge=np.zeros(max(self.n_tris, 1), dtype=np.int32),
+            sdc=f64(_or(fm, "sspec_dcst", np.zeros(SS))),
         )
+        if getattr(mt, "tri_nbr", None) is not None and self.n_tris:
+            self.a.update(tn=i32(mt.tri_nbr), te=i32(mt.tri_nbr_edge), tg=f64(mt.tri_edge_g))
         a = self.a
         P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
         self.sys = StgoSystem(
@@ -173,6 +178,11 @@ class OracleSim:
             P(a["gc"], C.c_int32), P(a["gi"], C.c_int32), P(a["gz"], C.c_int32),
             P(a["gp"], C.c_double), P(a["go"], C.c_double), P(a["gv"], C.c_double),
             P(a["gr"], C.c_int32), 293.15, P(a["charge"], C.c_int32))
+        if "tn" in a:
+            self.sys.tri_nbr = P(a["tn"], C.c_int32)
+            self.sys.tri_nbr_edge = P(a["te"], C.c_int32)
+            self.sys.tri_edge_g = P(a["tg"], C.c_double)
+        self.sys.sspec_dcst = P(a["sdc"], C.c_double)
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         self.time = 0.0
@@ -206,10 +216,19 @@ class OracleSim:
             nf.append(m.nbr_face)
             bdir.append(m.bdir if m.bdir is not None else -np.ones_like(m.nbr))
         tri_tet = np.concatenate([m.tri_tet + off[c] for c, m in enumerate(comp_meshes)])
+        toff = np.concatenate([[0], np.cumsum([len(m.tri_tet) for m in comp_meshes])])
+        tn = [m for m in comp_meshes if m.tri_nbr is not None and len(m.tri_tet)]
+        extra = {}
+        if tn:
+            extra = dict(
+                tri_nbr=np.concatenate([np.where(m.tri_nbr >= 0, m.tri_nbr + toff[c], -1)
+                                        for c, m in enumerate(comp_meshes) if len(m.tri_tet)]),
+                tri_nbr_edge=np.concatenate([m.tri_nbr_edge for m in comp_meshes if len(m.tri_tet)]),
+                tri_edge_g=np.concatenate([m.tri_edge_g for m in comp_meshes if len(m.tri_tet)]))
         mt = MeshTables(vol=np.concatenate([m.vol for m in comp_meshes]), nbr=np.concatenate(nbr),
                         nbr_face=np.concatenate(nf), face_g=np.concatenate([m.face_g for m in comp_meshes]),
                         tri_tet=tri_tet.astype(np.int32),
-                        tri_area=np.concatenate([m.tri_area for m in comp_meshes]))
+                        tri_area=np.concatenate([m.tri_area for m in comp_meshes]), **extra)
         o = cls(fm, mt, seed, **kw)
         a = o.a
         a["tcomp"] = np.ascontiguousarray(np.repeat(np.arange(nc), np.diff(off)), dtype=np.int32)
@@ -220,6 +239,8 @@ class OracleSim:
         a["cd"] = np.ascontiguousarray(np.asarray(comp_dcst, dtype=np.float64).reshape(nc, -1))
         a["bdir"] = np.ascontiguousarray(np.concatenate(bdir), dtype=np.int32)
         a["bnd"] = np.zeros((max(n_bdir, 1), len(fm.specs)))
+        a["trl"] = np.ascontiguousarray(np.concatenate(
+            [np.arange(len(m.tri_tet)) for m in comp_meshes] + [np.zeros(1)]), dtype=np.int32)
         P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
         o.sys.n_comps = nc
         o.sys.tet_comp = P(a["tcomp"], C.c_int32)
@@ -229,6 +250,7 @@ class OracleSim:
         o.sys.comp_dcst = P(a["cd"], C.c_double)
         o.sys.tet_bdir = P(a["bdir"], C.c_int32)
         o.sys.bnd_dcst = P(a["bnd"], C.c_double)
+        o.sys.tri_ltri = P(a["trl"], C.c_int32)
         o.comp_off = off
         return o
 

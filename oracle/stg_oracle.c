@@ -378,6 +378,14 @@ double stgo_update_period(const stgo_system *s) {
             if (sum > mx) mx = sum;
         }
     }
+    for (int32_t tr = 0; tr < s->n_tris && s->tri_nbr && s->sspec_dcst; ++tr) {
+        for (int32_t q = 0; q < s->n_sspecs; ++q) {
+            const double D = s->sspec_dcst[q];
+            const double *g = s->tri_edge_g + 3 * tr;
+            const double sum = ((D * g[0]) + (D * g[1])) + (D * g[2]);
+            if (sum > mx) mx = sum;
+        }
+    }
     return mx > DBL_EPSILON ? 1.0 / mx : INFINITY;
 }
 
@@ -448,7 +456,13 @@ typedef struct {
     int32_t *tri_idx; /* [n_tris]   */
     int32_t max_tris;
     uint8_t *tracked; /* [n_specs]  */
+    uint8_t *tri_tracked; /* [n_sspecs] surface species that react and diffuse */
+    double *tri_corr;     /* [n_sspecs][n_tris] occupancy correction in the period */
 } topo;
+
+static int sspec_diffuses(const stgo_system *s, int32_t q) {
+    return s->sspec_dcst && s->tri_nbr && s->sspec_dcst[q] > 0.0;
+}
 
 static void topo_build(const stgo_system *s, topo *tp) {
     tp->tri_ptr = (int32_t *)calloc((size_t)s->n_tets + 1, sizeof(int32_t));
@@ -477,12 +491,22 @@ static void topo_build(const stgo_system *s, topo *tp) {
         for (int32_t g = 0; g < s->n_ghk; ++g)
             if (s->ghk_ion[g] == sp && s->ghk_realflux[g]) tp->tracked[sp] = 1;
     }
+    tp->tri_tracked = (uint8_t *)calloc((size_t)(s->n_sspecs > 0 ? s->n_sspecs : 1), 1);
+    tp->tri_corr = (double *)calloc((size_t)(s->n_sspecs > 0 ? s->n_sspecs : 1) *
+                                        (size_t)(s->n_tris > 0 ? s->n_tris : 1), sizeof(double));
+    for (int32_t q = 0; q < s->n_sspecs; ++q) {
+        if (!sspec_diffuses(s, q)) continue;
+        for (int32_t r = 0; r < s->n_sreacs; ++r)
+            if (s->sreac_upd_s[r * s->n_sspecs + q] != 0) tp->tri_tracked[q] = 1;
+    }
 }
 
 static void topo_free(topo *tp) {
     free(tp->tri_ptr);
     free(tp->tri_idx);
     free(tp->tracked);
+    free(tp->tri_tracked);
+    free(tp->tri_corr);
 }
 
 /* SSA of one tet and its triangles over [0, dt): ssa_operator.cpp:24-47 with
@@ -631,6 +655,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                  * value as the reference's piecewise sum (tri.cpp:544-561) */
                 if ((s->tri_intg_mask >> q) & 1u)
                     s->tri_intg[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
+                if (tp->tri_tracked[q]) tp->tri_corr[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
             }
             for (int32_t sp = 0; sp < S; ++sp) {
                 if (ui[sp] == 0) continue;
@@ -710,6 +735,76 @@ static int32_t leave_tet_species(const stgo_system *s, int32_t t, int32_t sp, do
     return nl;
 }
 
+/* surface diffusion out of one (triangle, surface species): the volume's
+ * leaving pass with three edges (SDiff block of tetopsplit.cpp:1716-1788) */
+static int32_t leave_tri_species(const stgo_system *s, const topo *tp, int32_t tr, int32_t q,
+                                 double dt, uint32_t step, int32_t *out3) {
+    out3[0] = out3[1] = out3[2] = 0;
+    const size_t ix = (size_t)q * s->n_tris + tr;
+    const int32_t n = s->tri_counts[ix];
+    const double D = s->sspec_dcst[q];
+    double r3[3];
+    for (int e = 0; e < 3; ++e) r3[e] = D * s->tri_edge_g[3 * tr + e];
+    const double sum = (r3[0] + r3[1]) + r3[2];
+    if (n <= 0 || !(sum > 0.0)) return 0;
+    const int32_t t = s->tri_tet[tr];
+    const uint32_t utri = s->tri_ltri ? (uint32_t)s->tri_ltri[tr] : (uint32_t)tr;
+    const uint32_t stream = s->tet_comp ? s->comp_stream[s->tet_comp[t]] : s->stream;
+    ostream st;
+    os_init(&st, s->seed, utri, step, stream, 0x40u + (uint32_t)q);
+    double occ = (double)n;
+    if (tp->tri_tracked[q] && !almost_equal(dt, 0.0)) occ = (double)n + tp->tri_corr[ix] / dt;
+    const double u = os_unif(&st);
+    const double fl = floor(occ);
+    int32_t mp = (int32_t)fl + ((occ - fl) > u ? 1 : 0);
+    if (mp > n) mp = n;
+    if (mp < 0) mp = 0;
+    double p = dt * sum;
+    if (p > 1.0) p = 1.0;
+    if (p < 0.0) p = 0.0;
+    const int32_t nl = binom(&st, mp, p);
+    if (nl <= 0) return 0;
+    s->tri_counts[ix] = n - nl;
+    int lastf = -1;
+    for (int e = 0; e < 3; ++e)
+        if (r3[e] > 0.0) lastf = e;
+    if (nl <= s->diff_threshold) {
+        for (int32_t m = 0; m < nl; ++m) {
+            int chosen = -1;
+            for (int tries = 0; tries < 64 && chosen < 0; ++tries) {
+                const double selv = os_unif(&st) * sum;
+                double cum = 0.0;
+                for (int e = 0; e < 3; ++e) {
+                    cum = cum + r3[e];
+                    if (selv < cum) {
+                        chosen = e;
+                        break;
+                    }
+                }
+            }
+            if (chosen < 0) chosen = lastf;
+            out3[chosen] += 1;
+        }
+    } else {
+        int32_t rem = nl;
+        double rsum = sum;
+        for (int e = 0; e < 3; ++e) {
+            if (!(r3[e] > 0.0)) continue;
+            int32_t k;
+            if (e == lastf) {
+                k = rem;
+            } else {
+                const double pf = r3[e] / rsum;
+                k = pf >= 1.0 ? rem : binom(&st, rem, pf);
+                rsum = rsum - r3[e];
+            }
+            out3[e] = k;
+            rem -= k;
+        }
+    }
+    return nl;
+}
+
 int32_t stgo_period(stgo_system *s, double dt, uint32_t step, int32_t *scratch,
                     uint64_t *n_events, uint64_t *n_moved) {
     return stgo_run(s, &dt, 1, step, scratch, n_events, n_moved);
@@ -723,6 +818,9 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
     const int32_t nk_max = s->n_reacs + tp.max_tris * (s->n_sreacs + s->n_ghk) + 1;
     int32_t err = 0;
     uint64_t ev_total = 0, mv_total = 0;
+    const int32_t NTR = s->n_tris, SS = s->n_sspecs;
+    int32_t *tout = (int32_t *)calloc((size_t)(SS > 0 ? SS : 1) * (size_t)(NTR > 0 ? NTR : 1) * 3,
+                                      sizeof(int32_t));
     for (int32_t p = 0; p < n_periods && !err; ++p) {
         const double dt = dts[p];
         const uint32_t step = step0 + (uint32_t)p;
@@ -739,6 +837,10 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                     cnt[sp] = s->counts[(size_t)sp * N + t];
                     corr[sp] = 0.0;
                 }
+                for (int32_t q = 0; q < SS; ++q)
+                    if (tp.tri_tracked[q])
+                        for (int32_t k = tp.tri_ptr[t]; k < tp.tri_ptr[t + 1]; ++k)
+                            tp.tri_corr[(size_t)q * NTR + tp.tri_idx[k]] = 0.0;
                 long nev = ssa_tet(s, &tp, t, dt, step, cnt, corr, a, ps);
                 if (nev < 0) {
                     err |= nev == -2 ? 2 : nev == -3 ? 4 : 1;
@@ -763,6 +865,16 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                         o4[0] = o4[1] = o4[2] = o4[3] = 0;
                     s->counts[(size_t)sp * N + t] = cnt[sp];
                 }
+                for (int32_t k = tp.tri_ptr[t]; k < tp.tri_ptr[t + 1]; ++k) {
+                    const int32_t tr = tp.tri_idx[k];
+                    for (int32_t q = 0; q < SS; ++q) {
+                        int32_t *o3 = tout + ((size_t)q * NTR + tr) * 3;
+                        if (sspec_diffuses(s, q))
+                            mv_total += (uint64_t)leave_tri_species(s, &tp, tr, q, dt, step, o3);
+                        else
+                            o3[0] = o3[1] = o3[2] = 0;
+                    }
+                }
             }
             free(cnt);
             free(corr);
@@ -783,7 +895,22 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                 s->counts[(size_t)sp * N + t] += add;
             }
         }
+        /* surface molecules entering across triangle edges */
+#pragma omp parallel for schedule(static)
+        for (int32_t tr = 0; tr < NTR; ++tr) {
+            for (int32_t q = 0; q < SS; ++q) {
+                if (!sspec_diffuses(s, q)) continue;
+                int32_t add = 0;
+                for (int e = 0; e < 3; ++e) {
+                    const int32_t nb = s->tri_nbr[3 * tr + e];
+                    if (nb < 0) continue;
+                    add += tout[((size_t)q * NTR + nb) * 3 + s->tri_nbr_edge[3 * tr + e]];
+                }
+                s->tri_counts[(size_t)q * NTR + tr] += add;
+            }
+        }
     }
+    free(tout);
     topo_free(&tp);
     if (n_events) *n_events += ev_total;
     if (n_moved) *n_moved += mv_total;

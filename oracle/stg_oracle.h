@@ -89,6 +89,15 @@ typedef struct stgo_system {
     const double *comp_dcst;     /* [n_comps*n_specs] */
     const int32_t *tet_bdir;     /* [n_tets*4], or null */
     const double *bnd_dcst;      /* [directions*n_specs] */
+    /* surface diffusion (sdiff.cpp:42-90, tetopsplit.cpp:1716-1788), null /
+     * zero when no surface species diffuses: edge e of triangle i leads to
+     * tri_nbr[3i+e] (its slot tri_nbr_edge[3i+e]) with l_e/(A d_e) =
+     * tri_edge_g[3i+e]; triangle i draws with local index tri_ltri[i] (or i) */
+    const int32_t *tri_nbr;
+    const int32_t *tri_nbr_edge;
+    const double  *tri_edge_g;
+    const double  *sspec_dcst;   /* [n_sspecs] */
+    const int32_t *tri_ltri;
 } stgo_system;
 
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */

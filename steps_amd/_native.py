@@ -41,6 +41,9 @@ class StgMesh(C.Structure):
         ("tri_area", _dp),
         ("tet_nbr_comp", _ip),
         ("tet_face_bdir", _ip),
+        ("tri_nbr", _ip),
+        ("tri_nbr_edge", _ip),
+        ("tri_edge_g", _dp),
     ]
 
 
@@ -73,6 +76,7 @@ class StgModel(C.Structure):
         ("ghk_voconc", _dp),
         ("ghk_vshift", _dp),
         ("ghk_realflux", _ip),
+        ("sspec_dcst", _dp),
     ]
 
 
@@ -136,6 +140,7 @@ SIGNATURES = {
     "stg_set_temp": (C.c_int, [C.c_void_p, C.c_double]),
     "stg_create_comps": (C.c_int, [C.POINTER(StgModel), C.c_int32, C.POINTER(StgMesh),
                                    C.POINTER(StgRealization), C.c_int32, C.POINTER(C.c_void_p)]),
+    "stg_set_sdcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_get_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
     "stg_get_temp": (C.c_int, [C.c_void_p, _dp]),

@@ -100,6 +100,9 @@ struct stg_sim {
     bool linked = false;                   // compartments: one period schedule for all
     int32_t NBD = 0;                       // diffusion-boundary directions
     std::vector<double> h_bnd_D;           // [NBD][S]
+    int32_t NSD = 0;                       // diffusing surface species
+    std::vector<double> h_sD;              // [n_real][SS] surface diffusion constants
+    std::vector<double> h_tri_g;           // [NTRI*3] internal triangle order
     std::vector<uint32_t> h_tet_bdir;      // [NT] 4 x 8-bit direction per face, 0xff = none
     std::vector<double> h_k, h_sk, h_D;  // per-realization tables
     std::vector<double> h_face_g;         // [NT*4], for update-period recomputation
@@ -133,6 +136,13 @@ struct stg_sim {
     int32_t *d_tri_charge = nullptr;  // [NTRI] GHK charge, when the model has GHK currents
     uint32_t *d_tet_bdir = nullptr;   // [NT] when there are diffusion boundaries
     double *d_bnd_D = nullptr;        // [NBD][S]
+    int4 *d_tri_nbr = nullptr;        // surface diffusion tables
+    uint32_t *d_tri_nface = nullptr;
+    double *d_tri_g = nullptr;
+    int32_t *d_tri_user = nullptr;
+    double *d_real_sD = nullptr;
+    int32_t *d_tout[2] = {nullptr, nullptr};
+    double *d_tri_corr = nullptr;
     double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
     int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
@@ -154,7 +164,7 @@ struct stg_sim {
     // surface reaction, a GHK current or an integrated channel count, or
     // faces on a diffusion boundary
     bool needs_memb() const {
-        return (NTRI > 0 && (RS > 0 || NG > 0 || mk.tri_intg_mask != 0)) || NBD > 0;
+        return (NTRI > 0 && (RS > 0 || NG > 0 || NSD > 0 || mk.tri_intg_mask != 0)) || NBD > 0;
     }
 
     // per-thread LDS rows of the period kernel: counts, occupancy corrections,
@@ -178,6 +188,15 @@ struct stg_sim {
                     Df[f] = k == 0xffu ? D : h_bnd_D[(size_t)k * S + s];
                 }
                 const double sum = (((Df[0] * g[0]) + (Df[1] * g[1])) + (Df[2] * g[2])) + (Df[3] * g[3]);
+                if (sum > mx) mx = sum;
+            }
+        }
+        // surface diffusion (TetOpSplitP::_computeUpdPeriod, tetopsplit.cpp:5373-5411)
+        for (int i = tri_off[r]; i < tri_off[r + 1] && NSD > 0; ++i) {
+            for (int q = 0; q < SS; ++q) {
+                const double D = h_sD[(size_t)r * SS + q];
+                const double *g = &h_tri_g[3 * (size_t)i];
+                const double sum = ((D * g[0]) + (D * g[1])) + (D * g[2]);
                 if (sum > mx) mx = sum;
             }
         }
@@ -210,7 +229,9 @@ struct stg_sim {
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
-                        d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D};
+                        d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D,
+                        d_tri_nbr,   d_tri_nface,   d_tri_g,       d_tri_user,    d_real_sD,
+                        d_tout[0],   d_tout[1],     d_tri_corr};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -401,6 +422,9 @@ int guard(stg_sim *s, F &&f) {
 }
 
 void upload_real_tables(stg_sim *s) {
+    if (s->d_real_sD && !s->h_sD.empty())
+        HIP_OK(hipMemcpyAsync(s->d_real_sD, s->h_sD.data(), sizeof(double) * s->h_sD.size(),
+                              hipMemcpyHostToDevice, s->stream));
     if (!s->h_bnd_D.empty())
         HIP_OK(hipMemcpyAsync(s->d_bnd_D, s->h_bnd_D.data(), sizeof(double) * s->h_bnd_D.size(),
                               hipMemcpyHostToDevice, s->stream));
@@ -525,6 +549,36 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     }
     if (s->max_tris > 4) throw StgError{STG_ERR_ARG, "a tet has more than four triangles"};
 
+    // surface diffusion: edge neighbours in internal triangle order
+    std::vector<int4> tri_nbr(std::max(NTRI, 1), make_int4(-1, -1, -1, -1));
+    std::vector<uint32_t> tri_nface(std::max(NTRI, 1), 0);
+    std::vector<int32_t> tri_user(std::max(NTRI, 1), 0);
+    s->h_tri_g.assign(3 * (size_t)std::max(NTRI, 1), 0.0);
+    for (int r = 0; r < n_real; ++r) {
+        const stg_mesh &m = meshes[r];
+        const int32_t i0 = s->tri_off[r];
+        for (int u = 0; u < m.n_tris; ++u) {
+            const int32_t i = i0 + s->tri_perm[r][u];
+            tri_user[i] = u;
+            if (!m.tri_nbr) continue;
+            int nb[3] = {-1, -1, -1};
+            for (int e = 0; e < 3; ++e) {
+                const int32_t j = m.tri_nbr[3 * u + e];
+                const int32_t je = m.tri_nbr_edge ? m.tri_nbr_edge[3 * u + e] : -1;
+                const double g = m.tri_edge_g ? m.tri_edge_g[3 * u + e] : 0.0;
+                if (j < -1 || j >= m.n_tris) throw StgError{STG_ERR_ARG, "triangle neighbour out of range"};
+                if (j >= 0 && (je < 0 || je > 2 || m.tri_nbr[3 * j + je] != u))
+                    throw StgError{STG_ERR_ARG, "triangle neighbour table is not symmetric"};
+                if (!(g >= 0.0) || !std::isfinite(g) || (j < 0 && g != 0.0))
+                    throw StgError{STG_ERR_ARG, "edge factor must be finite, >= 0 and 0 on open edges"};
+                nb[e] = j >= 0 ? i0 + s->tri_perm[r][j] : -1;
+                tri_nface[i] |= (uint32_t)(j >= 0 ? je : 0) << (8 * e);
+                s->h_tri_g[3 * (size_t)i + e] = g;
+            }
+            tri_nbr[i] = make_int4(nb[0], nb[1], nb[2], -1);
+        }
+    }
+
     // per-realization constants
     s->h_k.assign((size_t)n_real * R, 0.0);
     s->h_sk.assign((size_t)n_real * RS, 0.0);
@@ -548,6 +602,25 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
         s->streams[r] = q.stream;
     }
     s->h_bnd_D.assign((size_t)s->NBD * S, 0.0);
+    s->h_sD.assign((size_t)n_real * SS, 0.0);
+    for (int r = 0; r < n_real; ++r)
+        for (int q = 0; q < SS; ++q) {
+            const double D = model->sspec_dcst ? model->sspec_dcst[q] : 0.0;
+            if (!(D >= 0.0) || !std::isfinite(D)) throw StgError{STG_ERR_ARG, "surface diffusion constant must be finite and >= 0"};
+            s->h_sD[(size_t)r * SS + q] = D;
+        }
+    {
+        stg::ModelK &k = s->mk;
+        k.n_sdiff = 0;
+        k.tri_tracked_mask = 0;
+        for (int q = 0; q < SS; ++q) {
+            if (!(model->sspec_dcst && model->sspec_dcst[q] > 0.0)) continue;
+            k.sdspec[k.n_sdiff++] = q;
+            for (int j = 0; j < RS; ++j)
+                if (model->sreac_upd_s[j * SS + q] != 0) k.tri_tracked_mask |= 1 << q;
+        }
+        s->NSD = k.n_sdiff;
+    }
     build_diffusion_sets(s, model);
     (void)SS;
     s->refresh_dt();
@@ -570,6 +643,23 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     s->d_real_k = dalloc<double>(std::max<size_t>(1, s->h_k.size()));
     s->d_real_sk = dalloc<double>(std::max<size_t>(1, s->h_sk.size()));
     s->d_real_D = dalloc<double>(std::max<size_t>(1, s->h_D.size()));
+    if (s->NSD > 0) {
+        const size_t nt = std::max(NTRI, 1);
+        s->d_tri_nbr = dalloc<int4>(nt);
+        s->d_tri_nface = dalloc<uint32_t>(nt);
+        s->d_tri_g = dalloc<double>(3 * nt);
+        s->d_tri_user = dalloc<int32_t>(nt);
+        s->d_real_sD = dalloc<double>(std::max<size_t>(1, s->h_sD.size()));
+        s->d_tri_corr = dalloc<double>((size_t)std::max(SS, 1) * nt);
+        for (int b = 0; b < 2; ++b) {
+            s->d_tout[b] = dalloc<int32_t>((size_t)s->NSD * nt * 4);
+            HIP_OK(hipMemset(s->d_tout[b], 0, sizeof(int32_t) * (size_t)s->NSD * nt * 4));
+        }
+        HIP_OK(hipMemcpy(s->d_tri_nbr, tri_nbr.data(), sizeof(int4) * nt, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_tri_nface, tri_nface.data(), sizeof(uint32_t) * nt, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_tri_g, s->h_tri_g.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_tri_user, tri_user.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice));
+    }
     if (s->NBD > 0) {
         s->d_tet_bdir = dalloc<uint32_t>(NT);
         HIP_OK(hipMemcpy(s->d_tet_bdir, s->h_tet_bdir.data(), sizeof(uint32_t) * NT, hipMemcpyHostToDevice));
@@ -680,6 +770,14 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.tri_charge = s->d_tri_charge;
     k.tet_bdir = s->d_tet_bdir;
     k.bnd_D = s->d_bnd_D;
+    k.tri_nbr = s->d_tri_nbr;
+    k.tri_nface = s->d_tri_nface;
+    k.tri_g = s->d_tri_g;
+    k.tri_user = s->d_tri_user;
+    k.real_sD = s->d_real_sD;
+    k.tout_prev = s->d_tout[launch_parity ^ 1];
+    k.tout_cur = s->d_tout[launch_parity];
+    k.tri_corr = s->d_tri_corr;
     return k;
 }
 
@@ -1113,6 +1211,23 @@ int stg_set_dcst(stg_sim *s, int32_t real, int32_t spec, double dcst) {
     });
 }
 
+int stg_set_sdcst(stg_sim *s, int32_t real, int32_t sspec, double dcst) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (sspec < 0 || sspec >= s->SS) throw StgError{STG_ERR_ARG, "surface species index out of range"};
+        if (!(dcst >= 0.0) || !std::isfinite(dcst))
+            throw StgError{STG_ERR_ARG, "diffusion constant must be finite and >= 0"};
+        const bool was = std::find(s->mk.sdspec, s->mk.sdspec + s->mk.n_sdiff, sspec) != s->mk.sdspec + s->mk.n_sdiff;
+        if (dcst > 0.0 && !was)
+            throw StgError{STG_ERR_NOTIMPL, "enabling surface diffusion of a species that was immobile at creation"};
+        s->h_sD[(size_t)real * s->SS + sspec] = dcst;
+        s->refresh_dt();
+        upload_real_tables(s);
+        HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
 int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
     if (!s) return STG_ERR_ARG;
     return guard(s, [&] {
@@ -1167,7 +1282,8 @@ struct StateHeader {
 
 size_t state_bytes(const stg_sim *s) {
     return sizeof(StateHeader) + s->n_real * (sizeof(uint64_t) * 4) +
-           sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size() + s->h_bnd_D.size()) +
+           sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size() + s->h_bnd_D.size() +
+                             s->h_sD.size()) +
            sizeof(int32_t) * ((size_t)s->S * s->NT + (size_t)s->SS * s->NTRI) +
            sizeof(double) * (size_t)s->NTRI;
 }
@@ -1202,7 +1318,7 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
             std::memcpy(p, v, sizeof(v));
             p += sizeof(v);
         }
-        for (const auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D}) {
+        for (const auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D, &s->h_sD}) {
             std::memcpy(p, vec->data(), vec->size() * sizeof(double));
             p += vec->size() * sizeof(double);
         }
@@ -1243,7 +1359,7 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
                     placed[r] = true;
                 }
         }
-        for (auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D}) {
+        for (auto *vec : {&s->h_k, &s->h_sk, &s->h_D, &s->h_bnd_D, &s->h_sD}) {
             std::memcpy(vec->data(), p, vec->size() * sizeof(double));
             p += vec->size() * sizeof(double);
         }

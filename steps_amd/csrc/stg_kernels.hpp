@@ -57,6 +57,9 @@ struct ModelK {
     int32_t ent[kMaxEntries];  // species (low 16 bits) | signed value << 16
     // GHK currents: channel-state surface species, inner ion, valence, whether
     // ions move; permeability (m/s), virtual outer concentration (M), V shift (V)
+    int32_t n_sdiff;            // surface species that diffuse
+    int32_t sdspec[kMaxSSpecs];
+    int32_t tri_tracked_mask;   // bit q: occupancy tracked for surface species q
     int32_t n_ghk;
     int32_t g_chan[kMaxGhk], g_ion[kMaxGhk], g_z[kMaxGhk], g_real[kMaxGhk];
     double g_perm[kMaxGhk], g_oconc[kMaxGhk], g_vshift[kMaxGhk];
@@ -106,6 +109,15 @@ struct SysK {
     const uint32_t *tet_bdir;    // [NT] 4 x 8-bit diffusion-boundary direction per face
                                  // (0xff = none), or null
     const double *bnd_D;         // [directions][n_specs] diffusion constant across a boundary
+    // surface diffusion
+    const int4 *tri_nbr;         // [NTRI] neighbour triangle per edge (global), -1 none
+    const uint32_t *tri_nface;   // [NTRI] 3 x 8-bit edge slot in the neighbour
+    const double *tri_g;         // [NTRI*3] l_e / (A d_e)
+    const int32_t *tri_user;     // [NTRI] triangle index in its realization's input order
+    const double *real_sD;       // [R][n_sspecs]
+    const int32_t *tout_prev;    // [n_sdiff][NTRI][4]
+    int32_t *tout_cur;           // [n_sdiff][NTRI][4]
+    double *tri_corr;            // [n_sspecs][NTRI] occupancy correction within the period
 };
 
 // both structures travel in the 4 KiB kernel-argument segment
@@ -343,6 +355,7 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                     sys.tri_counts[ix] += dv;
                     // integral of n over the period = n_end * dt - sum dn * t_event
                     if ((m.tri_intg_mask >> (sp & 0x7f)) & 1) sys.tri_intg[ix] -= (double)dv * tcur;
+                    if ((m.tri_tracked_mask >> (sp & 0x7f)) & 1) sys.tri_corr[ix] -= (double)dv * tcur;
                 } else {
                     L.c(sp) += dv;
                     if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
@@ -435,6 +448,77 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     return nl;
 }
 
+// Surface diffusion out of one (triangle, surface species): the leaving pass
+// of the volume with three edge directions (SDiff, sdiff.cpp:42-90, and the
+// SDiff block of tetopsplit.cpp:1716-1788). Writes three edge slots.
+__device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int tri, double dt,
+                             uint64_t seed, uint32_t period, uint32_t stream, int4 &o) {
+    o = make_int4(0, 0, 0, 0);
+    const size_t ix = (size_t)q * sys.n_tris + tri;
+    const int32_t n = sys.tri_counts[ix];
+    double r3[3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) r3[e] = D * sys.tri_g[3 * (size_t)tri + e];
+    const double sum = (r3[0] + r3[1]) + r3[2];
+    if (n <= 0 || !(sum > 0.0)) return 0;
+    Philox g;
+    g.init(seed, (uint32_t)sys.tri_user[tri], period, stream, 0x40u + (uint32_t)q);
+    double occ = (double)n;
+    if (tracked && !almost_zero_dt(dt)) occ = (double)n + sys.tri_corr[ix] / dt;
+    const double u = g.unif();
+    const double fl = floor(occ);
+    int32_t mp = (int32_t)fl + ((occ - fl) > u ? 1 : 0);
+    mp = mp > n ? n : mp;
+    mp = mp < 0 ? 0 : mp;
+    double p = dt * sum;
+    p = p > 1.0 ? 1.0 : p;
+    p = p < 0.0 ? 0.0 : p;
+    const int32_t nl = binomial(g, mp, p);
+    if (nl <= 0) return 0;
+    sys.tri_counts[ix] = n - nl;
+    int lastf = -1;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+        if (r3[e] > 0.0) lastf = e;
+    int32_t k3[3] = {0, 0, 0};
+    if (nl <= sys.diff_threshold) {
+        for (int32_t mol = 0; mol < nl; ++mol) {
+            int chosen = -1;
+            for (int tries = 0; tries < 64 && chosen < 0; ++tries) {
+                const double selv = g.unif() * sum;
+                double cum = 0.0;
+#pragma unroll
+                for (int e = 0; e < 3; ++e) {
+                    cum = cum + r3[e];
+                    if (chosen < 0 && selv < cum) chosen = e;
+                }
+            }
+            if (chosen < 0) chosen = lastf;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) k3[e] += (chosen == e) ? 1 : 0;
+        }
+    } else {
+        int32_t rem = nl;
+        double rsum = sum;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            if (!(r3[e] > 0.0)) continue;
+            int32_t k;
+            if (e == lastf) {
+                k = rem;
+            } else {
+                const double pf = r3[e] / rsum;
+                k = pf >= 1.0 ? rem : binomial(g, rem, pf);
+                rsum = rsum - r3[e];
+            }
+            k3[e] = k;
+            rem -= k;
+        }
+    }
+    o = make_int4(k3[0], k3[1], k3[2], 0);
+    return nl;
+}
+
 #ifndef STG_PERIOD_WAVES_PER_EU
 #define STG_PERIOD_WAVES_PER_EU 1
 #endif
@@ -507,21 +591,45 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 L.c(m.dspec[d]) += add;
             }
         }
+        int tp0 = 0, tp1 = 0;
+        if (MEMB && sys.tri_ptr != nullptr) {
+            tp0 = sys.tri_ptr[t];
+            tp1 = sys.tri_ptr[t + 1];
+        }
+        if (MEMB && do_enter && m.n_sdiff > 0) {
+            // surface molecules entering this tet's triangles across their edges
+            for (int tri = tp0; tri < tp1; ++tri) {
+                const int4 nb = sys.tri_nbr[tri];
+                const uint32_t ne = sys.tri_nface[tri];
+                const int nbv[3] = {nb.x, nb.y, nb.z};
+                for (int d = 0; d < m.n_sdiff; ++d) {
+                    const int32_t *op = sys.tout_prev + (size_t)d * sys.n_tris * 4;
+                    int32_t add = 0;
+#pragma unroll
+                    for (int e = 0; e < 3; ++e)
+                        if (nbv[e] >= 0) add += op[(size_t)nbv[e] * 4 + ((ne >> (8 * e)) & 0xff)];
+                    sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
+                }
+            }
+        }
         if (do_step) {
             const int tet0 = sys.real_tet0[r];
             const uint32_t ltet = (uint32_t)(t - tet0);
             const uint32_t period = sys.real_step0[r] + (uint32_t)launch;
             const double dt = sys.dt_tab[(size_t)launch * sys.n_real + r];
+            if (MEMB && m.tri_tracked_mask != 0)
+                for (int q = 0; q < m.n_sspecs; ++q)
+                    if ((m.tri_tracked_mask >> q) & 1)
+                        for (int tri = tp0; tri < tp1; ++tri) sys.tri_corr[(size_t)q * sys.n_tris + tri] = 0.0;
             nev = ssa_group<MEMB>(m, sys, L, r, t, ltet, period, dt);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
             }
-            if (MEMB && m.tri_intg_mask != 0 && sys.tri_ptr != nullptr) {
-                const int tp1 = sys.tri_ptr[t + 1];
+            if (MEMB && m.tri_intg_mask != 0) {
                 for (int q = 0; q < m.n_sspecs; ++q) {
                     if (!((m.tri_intg_mask >> q) & 1)) continue;
-                    for (int tri = sys.tri_ptr[t]; tri < tp1; ++tri) {
+                    for (int tri = tp0; tri < tp1; ++tri) {
                         const size_t ix = (size_t)q * sys.n_tris + tri;
                         sys.tri_intg[ix] += (double)sys.tri_counts[ix] * dt;
                     }
@@ -542,6 +650,21 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     moved += leave(sys, L, sp, (m.tracked_mask >> sp) & 1, Dr[sp], r4, bdir, S, dt,
                                    seed, ltet, period, stream, o);
                     oc[(size_t)d * NT + t] = o;
+                }
+            }
+            if (MEMB && m.n_sdiff > 0) {
+                const double *sDr = sys.real_sD + (size_t)r * m.n_sspecs;
+                const uint64_t seed = sys.real_seed[r];
+                const uint32_t stream = sys.real_stream[r];
+                int4 *toc = reinterpret_cast<int4 *>(sys.tout_cur);
+                for (int tri = tp0; tri < tp1; ++tri) {
+                    for (int d = 0; d < m.n_sdiff; ++d) {
+                        const int q = m.sdspec[d];
+                        int4 o;
+                        moved += tri_leave(sys, q, (m.tri_tracked_mask >> q) & 1, sDr[q], tri, dt, seed,
+                                           period, stream, o);
+                        toc[(size_t)d * sys.n_tris + tri] = o;
+                    }
                 }
             }
         }

@@ -27,6 +27,12 @@ def _mesh_struct(mt: MeshTables, keep: list) -> N.StgMesh:
         keep.extend([nc, bd])
         m.tet_nbr_comp = N.ptr(nc, C.c_int32)
         m.tet_face_bdir = N.ptr(bd, C.c_int32)
+    if mt.tri_nbr is not None and len(mt.tri_tet):
+        tn, te, tg = N.as_i32(mt.tri_nbr), N.as_i32(mt.tri_nbr_edge), N.as_f64(mt.tri_edge_g)
+        keep.extend([tn, te, tg])
+        m.tri_nbr = N.ptr(tn, C.c_int32)
+        m.tri_nbr_edge = N.ptr(te, C.c_int32)
+        m.tri_edge_g = N.ptr(tg, C.c_double)
     return m
 
 
@@ -37,10 +43,11 @@ def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
             N.as_f64(fm.vdep_vmin), N.as_f64(fm.vdep_dv), N.as_i32(fm.vdep_off), N.as_f64(fm.vdep_k),
             N.as_i32(fm.ghk_chan), N.as_i32(fm.ghk_ion), N.as_i32(fm.ghk_valence),
             N.as_f64(fm.ghk_perm), N.as_f64(fm.ghk_voconc), N.as_f64(fm.ghk_vshift),
-            N.as_i32(fm.ghk_realflux)]
+            N.as_i32(fm.ghk_realflux),
+            N.as_f64(fm.sspec_dcst if fm.sspec_dcst is not None else np.zeros(len(fm.sspecs)))]
     keep.extend(arrs)
     (rl, ru, rk, dc, sls, sus, sli, sui, sk, svd, vmin, vdv, voff, vk,
-     gc, gi, gz, gp, go, gv, gr) = arrs
+     gc, gi, gz, gp, go, gv, gr, sdc) = arrs
     return N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
                       N.ptr(rk, C.c_double), N.ptr(dc, C.c_double), len(fm.sspecs), len(fm.sreacs),
                       N.ptr(sls, C.c_int32), N.ptr(sus, C.c_int32), N.ptr(sli, C.c_int32),
@@ -49,7 +56,7 @@ def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
                       N.ptr(voff, C.c_int32), N.ptr(vk, C.c_double), len(fm.ghk_chan),
                       N.ptr(gc, C.c_int32), N.ptr(gi, C.c_int32), N.ptr(gz, C.c_int32),
                       N.ptr(gp, C.c_double), N.ptr(go, C.c_double), N.ptr(gv, C.c_double),
-                      N.ptr(gr, C.c_int32))
+                      N.ptr(gr, C.c_int32), N.ptr(sdc, C.c_double))
 
 
 class Engine:
@@ -198,6 +205,9 @@ class Engine:
         a = np.empty(self.n_tris[real])
         self._chk(self.lib.stg_get_tri_v(self.h, real, N.ptr(a, C.c_double)))
         return a
+
+    def set_sdcst(self, sspec: int, d: float, real: int = 0):
+        self._chk(self.lib.stg_set_sdcst(self.h, real, sspec, float(d)))
 
     def set_boundary_dcst(self, bdir: int, spec: int, dcst: float):
         self._chk(self.lib.stg_set_boundary_dcst(self.h, int(bdir), int(spec), float(dcst)))

@@ -344,6 +344,41 @@ class MeshTables:
     tri_area: np.ndarray   # (m,) float64
     nbr_comp: np.ndarray | None = None   # (n,4) int32 compartment of the neighbour
     bdir: np.ndarray | None = None       # (n,4) int32 diffusion-boundary direction, -1 none
+    tri_nbr: np.ndarray | None = None    # (m,3) int32 neighbour triangle across each edge
+    tri_nbr_edge: np.ndarray | None = None   # (m,3) int32 edge slot in the neighbour
+    tri_edge_g: np.ndarray | None = None     # (m,3) float64 l_e / (A d_e), 0 = closed
+
+
+def patch_tri_tables(mesh: Tetmesh, tris):
+    """Edge neighbours of the patch triangles ``tris`` (local numbering) for
+    surface diffusion: across edge e = (v_e, v_(e+1)) the other patch triangle
+    holding that edge, its slot for the edge, and l_e / (A d_e) with l_e the
+    edge length, A the area and d_e the distance between the barycentres
+    (SDiff::SDiff, src/steps/mpi/tetopsplit/sdiff.cpp:42-90)."""
+    tris = np.asarray(tris, dtype=np.int64)
+    m = len(tris)
+    tv = mesh.tris[tris]
+    nbr = -np.ones((m, 3), dtype=np.int32)
+    nbe = -np.ones((m, 3), dtype=np.int32)
+    g = np.zeros((m, 3))
+    owner = {}
+    for i in range(m):
+        for e in range(3):
+            key = tuple(sorted((int(tv[i, e]), int(tv[i, (e + 1) % 3]))))
+            owner.setdefault(key, []).append((i, e))
+    bary = mesh.tri_bary[tris]
+    area = mesh.tri_area[tris]
+    for key, occ in owner.items():
+        if len(occ) != 2:
+            continue            # a patch border, or an edge shared by more than two
+        (i, ei), (j, ej) = occ
+        length = float(np.linalg.norm(mesh.verts[key[0]] - mesh.verts[key[1]]))
+        d = float(np.linalg.norm(bary[i] - bary[j]))
+        if d <= 0.0:
+            continue
+        nbr[i, ei], nbe[i, ei], g[i, ei] = j, ej, length / (area[i] * d)
+        nbr[j, ej], nbe[j, ej], g[j, ej] = i, ei, length / (area[j] * d)
+    return nbr, nbe, g
 
 
 def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
@@ -401,6 +436,7 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
             if np.any(inner < 0) or np.any(comp_of[inner] != ci):
                 raise ValueError("patch triangle without an inner tet in its compartment")
             tri_tet, tri_area = local[inner].astype(np.int32), mesh.tri_area[tris]
+        tn = patch_tri_tables(mesh, ptris) if ptris is not None else (None, None, None)
         out.append(MeshTables(
             vol=np.ascontiguousarray(vol, dtype=np.float64),
             nbr=np.ascontiguousarray(lnbr, dtype=np.int32),
@@ -409,7 +445,8 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
             tri_tet=np.ascontiguousarray(tri_tet, dtype=np.int32),
             tri_area=np.ascontiguousarray(tri_area, dtype=np.float64),
             nbr_comp=np.ascontiguousarray(np.where(keep, ncomp_arr, ci), dtype=np.int32),
-            bdir=np.ascontiguousarray(bdir, dtype=np.int32)))
+            bdir=np.ascontiguousarray(bdir, dtype=np.int32),
+            tri_nbr=tn[0], tri_nbr_edge=tn[1], tri_edge_g=tn[2]))
     return out, dirs
 
 
@@ -441,6 +478,7 @@ def mesh_tables(mesh: Tetmesh, comp: TmComp, patch_tris=None) -> MeshTables:
             raise ValueError("patch triangle without an inner tet in the compartment")
         tri_tet = local[inner].astype(np.int32)
         tri_area = mesh.tri_area[tris]
+    tn = patch_tri_tables(mesh, patch_tris) if patch_tris is not None else (None, None, None)
     return MeshTables(
         vol=np.ascontiguousarray(vol, dtype=np.float64),
         nbr=np.ascontiguousarray(lnbr, dtype=np.int32),
@@ -448,4 +486,5 @@ def mesh_tables(mesh: Tetmesh, comp: TmComp, patch_tris=None) -> MeshTables:
         face_g=np.ascontiguousarray(g, dtype=np.float64),
         tri_tet=np.ascontiguousarray(tri_tet, dtype=np.int32),
         tri_area=np.ascontiguousarray(tri_area, dtype=np.float64),
+        tri_nbr=tn[0], tri_nbr_edge=tn[1], tri_edge_g=tn[2],
     )

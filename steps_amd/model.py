@@ -288,6 +288,8 @@ class FlatModel:
     ohmic_chan: np.ndarray = None                    # surface species index
     ohmic_g: np.ndarray = None                       # S per channel
     ohmic_erev: np.ndarray = None                    # V
+    sspec_dcst: np.ndarray = None                    # [n_sspecs] surface diffusion m^2/s
+    sdiffs: dict = field(default_factory=dict)       # surface diff id -> surface species index
 
     def spec_index(self, name: str) -> int:
         try:
@@ -340,8 +342,8 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
                 raise ValueError(f"Permeability not defined for GHK current object {c.id!r}")
             used_s.add(c.chanstate.id)
             used_v.add(c.ion.id)
-        if ss.diffs:
-            raise NotImplementedError("surface diffusion is not supported by this solver")
+        for d in ss.diffs.values():
+            used_s.add(d.lig.id)
     order = list(model.specs)
     specs = [s for s in order if s in used_v]
     sspecs = [s for s in order if s in used_s]
@@ -393,6 +395,11 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.ghk_voconc = np.array([c.virtual_oconc for c in ghk], dtype=np.float64)
     fm.ghk_vshift = np.array([c.vshift for c in ghk], dtype=np.float64)
     fm.ghk_realflux = np.array([int(c.computeflux) for c in ghk], dtype=np.int32)
+    fm.sspec_dcst = np.zeros(SS)
+    for ss in ssys:
+        for d in ss.diffs.values():
+            fm.sspec_dcst[sspecs.index(d.lig.id)] = d.dcst
+            fm.sdiffs[d.id] = sspecs.index(d.lig.id)
     ohm = [c for ss in ssys for c in ss.ohmiccurrs.values()]
     fm.ohmic = [c.id for c in ohm]
     fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
@@ -472,7 +479,8 @@ def flatten_comps(model: Model, comps_volsys, surfsys_ids=()) -> CompTables:
             fm.sreac_upd_i[:, g] = patch.sreac_upd_i[:, j]
         for name in ("sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv", "vdep_off", "vdep_k",
                      "ghk", "ghk_chan", "ghk_valence", "ghk_perm", "ghk_voconc", "ghk_vshift",
-                     "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g", "ohmic_erev"):
+                     "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g", "ohmic_erev", "sspec_dcst",
+                     "sdiffs"):
             setattr(fm, name, getattr(patch, name))
         fm.ghk_ion = np.array([specs.index(patch.specs[i]) for i in patch.ghk_ion], dtype=np.int32)
         del SS
@@ -481,7 +489,7 @@ def flatten_comps(model: Model, comps_volsys, surfsys_ids=()) -> CompTables:
         for name in ("sreac_lhs_s", "sreac_upd_s", "sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv",
                      "vdep_off", "vdep_k", "ghk", "ghk_chan", "ghk_ion", "ghk_valence", "ghk_perm",
                      "ghk_voconc", "ghk_vshift", "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g",
-                     "ohmic_erev"):
+                     "ohmic_erev", "sspec_dcst", "sdiffs"):
             setattr(fm, name, getattr(empty, name))
         fm.sreac_lhs_i = np.zeros((0, S), dtype=np.int32)
         fm.sreac_upd_i = np.zeros((0, S), dtype=np.int32)

@@ -470,6 +470,28 @@ class TetOpSplit:
         self.eng.set_sreac_k(j, k, real=self._preal)
         self.fm.sreac_k[j] = k
 
+    def _sdiff(self, d: str) -> int:
+        if d not in self.fm.sdiffs:
+            raise _argerr(f"unknown surface diffusion rule {d!r}")
+        return self.fm.sdiffs[d]
+
+    def getPatchSDiffD(self, p: str, d: str) -> float:
+        self._check_patch(p)
+        return float(self.fm.sspec_dcst[self._sdiff(d)])
+
+    def setPatchSDiffD(self, p: str, d: str, dcst: float, direction_patch=None):
+        """Surface diffusion constant of a rule on the patch (setPatchSDiffD;
+        directional constants need surface diffusion boundaries, which this
+        solver does not model)."""
+        self._check_patch(p)
+        if direction_patch is not None:
+            raise _notimpl("surface diffusion boundaries are not supported")
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        q = self._sdiff(d)
+        self.eng.set_sdcst(q, dcst, real=self._preal)
+        self.fm.sspec_dcst[q] = dcst
+
     def _sreac(self, r):
         try:
             return self.fm.sreacs.index(r)

@@ -107,7 +107,7 @@ def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> 
 
 def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0,
                           ca_um=0.1, pump_density=10.0, influx=1e5,
-                          vdep_influx=False, ghk_influx=False) -> Workload:
+                          vdep_influx=False, ghk_influx=False, pump_dcst=0.0) -> Workload:
     """Ca + Buf <-> CaB (kf 1e8 /M/s, kb 10 /s), D_Ca 2e-10 m^2/s (Buf, CaB
     immobile); every boundary triangle is membrane. Pumps P (pump_density per
     um^2) run P + Ca_in -> P + Ca_out with k = kcat/Km = 50 /s / 1 uM; Ca_out
@@ -116,7 +116,8 @@ def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0
     admits ``influx`` ions/s in total. With ``vdep_influx`` the channel is
     voltage gated: k(V) = k_max / (1 + exp(-(V + 20 mV) / 6 mV)) over -150..100 mV.
     With ``ghk_influx`` the channel instead passes a GHK current of Ca (valence
-    2, virtual outer concentration 2 mM, 20 pS slope conductance at -65 mV)."""
+    2, virtual outer concentration 2 mM, 20 pS slope conductance at -65 mV).
+    pump_dcst > 0 lets the pumps diffuse on the membrane (surface diffusion)."""
     rng = np.random.default_rng(seed)
     mesh, comp, tris = _cube(n, side, True)
     mdl = M.Model()
@@ -128,6 +129,9 @@ def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0
     M.Diff("dCa", vs, Ca, 2e-10)
     ss = M.Surfsys("ssys", mdl)
     M.SReac("pump", ss, ilhs=[Ca], slhs=[P], srhs=[P, CaOut], kcst=50.0 / 1e-6)
+    if pump_dcst > 0:
+        M.Diff("dPump", ss, P, pump_dcst)
+        M.Diff("dCaOut", ss, CaOut, pump_dcst)
     face_tris = tris[np.all(np.abs(mesh.verts[mesh.tris[tris]][:, :, 0]) < 1e-3 * side / n, axis=1)]
     n_face = len(face_tris)
     k_in = influx / max(n_face, 1)

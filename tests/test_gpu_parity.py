@@ -250,3 +250,26 @@ def test_ghk_membrane_bit_exact(mode):
         assert np.array_equal(_gpu_tri_counts(eng, w), o.tri_counts[:len(w.fm.sspecs)])
     assert eng.extents() == (o.events.value, o.moved.value)
     assert o.tri_charge.sum() < 0   # net calcium influx
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_surface_diffusion_bit_exact(mode):
+    """Pumps and extruded ions diffusing on the membrane (CaOut is produced by
+    the pump reaction, so its occupancy is tracked), GPU vs oracle."""
+    w = W.calcium_membrane_cube(n=6, side=2e-6, seed=12, end_time=3e-4, buf_um=100.0,
+                                ca_um=30.0, pump_density=300.0, pump_dcst=1e-12)
+    print(f"case surface diffusion mode={mode}", flush=True)
+    assert w.fm.sspec_dcst.max() > 0
+    eng = _engine(w, 31, mode)
+    o = _oracle(w, 31)
+    assert eng.update_period() == o.update_period()
+    tot0 = w.tri_counts.sum(axis=1)
+    for t in (1e-4, w.end_time):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+        assert np.array_equal(_gpu_tri_counts(eng, w), o.tri_counts[:len(w.fm.sspecs)]), f"t={t}"
+    assert eng.extents() == (o.events.value, o.moved.value)
+    p = w.fm.sspec_index("Pump")
+    assert _gpu_tri_counts(eng, w)[p].sum() == tot0[p]          # pumps are conserved
+    assert not np.array_equal(_gpu_tri_counts(eng, w)[p], w.tri_counts[p])   # and moved

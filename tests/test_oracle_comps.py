@@ -141,3 +141,74 @@ def test_boundary_leaving_count_is_binomial():
     e = np.append(exp[keep], exp[~keep].sum())
     ob = np.append(obs[keep], obs[~keep].sum())
     assert stats.chisquare(ob, e * ob.sum() / e.sum()).pvalue > 1e-3
+
+
+# ---------------------------------------------------------- surface diffusion
+
+def _sdiff_model(D_fast, D_slow=None):
+    mdl = M.Model()
+    X = M.Spec("X", mdl)
+    ss = M.Surfsys("ssys", mdl)
+    M.Diff("sdX", ss, X, D_fast)
+    if D_slow is not None:
+        M.Diff("sdY", ss, M.Spec("Y", mdl), D_slow)
+    M.Volsys("vsys", mdl)
+    return mdl
+
+
+def test_surface_leaving_count_is_binomial_and_conserved():
+    # isolated triangle pairs joined across edge 0 (SDiff with one open edge);
+    # Y diffuses at a quarter of the stability rate: Binomial(n, 1/4) crosses
+    n_pairs, g = 3000, 2e12
+    n_tri = 2 * n_pairs
+    tri_nbr = -np.ones((n_tri, 3), dtype=np.int32)
+    tri_nbr[0::2, 0] = np.arange(1, n_tri, 2)
+    tri_nbr[1::2, 0] = np.arange(0, n_tri, 2)
+    tri_edge = np.where(tri_nbr >= 0, 0, -1).astype(np.int32)
+    tg = np.where(tri_nbr >= 0, g, 0.0)
+    z4 = -np.ones((n_tri, 4), dtype=np.int32)
+    mt = geom.MeshTables(vol=np.full(n_tri, 1e-19), nbr=z4, nbr_face=z4.copy(),
+                         face_g=np.zeros((n_tri, 4)), tri_tet=np.arange(n_tri, dtype=np.int32),
+                         tri_area=np.full(n_tri, 1e-13), tri_nbr=tri_nbr, tri_nbr_edge=tri_edge,
+                         tri_edge_g=tg)
+    fm = M.flatten(_sdiff_model(1e-12, 0.25e-12), ["vsys"], ["ssys"])
+    o = bind.OracleSim(fm, mt, seed=19)
+    assert abs(o.update_period() - 1.0 / (1e-12 * g)) < 1e-12 * o.update_period()
+    y = fm.sspec_index("Y")
+    o.tri_counts[y, 0::2] = 40
+    o.run_periods([o.update_period()])
+    crossed = 40 - o.tri_counts[y, 0::2]
+    assert np.array_equal(o.tri_counts[y, 1::2], crossed)
+    k = np.arange(41)
+    exp = stats.binom.pmf(k, 40, 0.25) * n_pairs
+    obs = np.bincount(crossed, minlength=41)
+    keep = exp >= 5
+    e = np.append(exp[keep], exp[~keep].sum())
+    ob = np.append(obs[keep], obs[~keep].sum())
+    assert stats.chisquare(ob, e * ob.sum() / e.sum()).pvalue > 1e-3
+
+
+def test_surface_point_source_spreads_as_2d_diffusion():
+    # molecules released at the centre of the flat z = 0 face of a cube: the
+    # mean squared displacement on the plane grows as 4 D t (2-D diffusion),
+    # before the spread reaches the face's edges
+    n, side, D = 24, 24e-6, 1e-12
+    mesh = geom.kuhn_cube(n, side)
+    comp = geom.TmComp("c", mesh)
+    surf = mesh.getSurfTris()
+    z = mesh.verts[mesh.tris[surf]][:, :, 2]
+    face = surf[np.all(np.abs(z) < 1e-12, axis=1)]
+    mt = geom.mesh_tables(mesh, comp, face)
+    fm = M.flatten(_sdiff_model(D), ["vsys"], ["ssys"])
+    o = bind.OracleSim(fm, mt, seed=23)
+    bary = mesh.tri_bary[face][:, :2]
+    centre = np.array([side / 2, side / 2])
+    src = int(np.argmin(np.linalg.norm(bary - centre, axis=1)))
+    o.tri_counts[0, src] = 20000
+    t = 8.0
+    dt = o.update_period()
+    o.run_periods([dt] * int(round(t / dt)))
+    c = o.tri_counts[0, :len(face)]
+    assert c.sum() == 20000
+    r2 = (np.linalg.norm(bary - bary[src], axis=1) ** 2 * c).sum() / c.sum()
+    assert abs(r2 / (4 * D * t) - 1) < 0.1, r2 / (4 * D * t)
