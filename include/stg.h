@@ -193,6 +193,11 @@ int stg_get_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int32_t *count
 int stg_get_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets,
                              size_t n, double *counts);
 
+/* Total count of species spec over the listed tets of a realization, reduced
+ * on the device (sumBatchTetCountsNP / getROISpecCount, tetopsplit.cpp:5715-5778). */
+int stg_sum_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets, size_t n,
+                             int64_t *total);
+
 /* On-device totals of one species over a realization's tets (or triangles):
  * the reduction behind getCompSpecCount / getPatchSpecCount
  * (tetopsplit.cpp:2045-2110). */

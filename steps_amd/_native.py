@@ -140,6 +140,8 @@ SIGNATURES = {
     "stg_set_temp": (C.c_int, [C.c_void_p, C.c_double]),
     "stg_create_comps": (C.c_int, [C.POINTER(StgModel), C.c_int32, C.POINTER(StgMesh),
                                    C.POINTER(StgRealization), C.c_int32, C.POINTER(C.c_void_p)]),
+    "stg_sum_batch_tet_counts": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
+                                           C.c_size_t, C.POINTER(C.c_int64)]),
     "stg_set_sdcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_get_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),

@@ -1108,26 +1108,31 @@ int stg_get_tri_counts(stg_sim *s, int32_t real, int32_t sspec, int32_t *counts)
     });
 }
 
+// device copy of a tet index list, checked against the realization's size
+static void stage_indices(stg_sim *s, int32_t real, const int64_t *tets, size_t n) {
+    const int64_t nt = s->tet_off[real + 1] - s->tet_off[real];
+    for (size_t i = 0; i < n; ++i)
+        if (tets[i] < 0 || tets[i] >= nt) throw StgError{STG_ERR_ARG, "tetrahedron index out of range"};
+    if (n > s->idx_cap) {
+        if (s->d_idx) (void)hipFree(s->d_idx);
+        if (s->d_gather) (void)hipFree(s->d_gather);
+        s->d_idx = nullptr;
+        s->d_gather = nullptr;
+        s->idx_cap = n;
+        s->d_idx = dalloc<int64_t>(n);
+        s->d_gather = dalloc<double>(n);
+    }
+    if (n) HIP_OK(hipMemcpyAsync(s->d_idx, tets, sizeof(int64_t) * n, hipMemcpyHostToDevice, s->stream));
+}
+
 int stg_get_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64_t *tets, size_t n,
                              double *counts) {
     if (!s || (n > 0 && (!tets || !counts))) return STG_ERR_ARG;
     return guard(s, [&] {
         check_real(s, real);
         if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
-        const int64_t nt = s->tet_off[real + 1] - s->tet_off[real];
-        for (size_t i = 0; i < n; ++i)
-            if (tets[i] < 0 || tets[i] >= nt) throw StgError{STG_ERR_ARG, "tetrahedron index out of range"};
+        stage_indices(s, real, tets, n);
         if (n == 0) return;
-        if (n > s->idx_cap) {
-            if (s->d_idx) (void)hipFree(s->d_idx);
-            if (s->d_gather) (void)hipFree(s->d_gather);
-            s->d_idx = nullptr;
-            s->d_gather = nullptr;
-            s->idx_cap = n;
-            s->d_idx = dalloc<int64_t>(n);
-            s->d_gather = dalloc<double>(n);
-        }
-        HIP_OK(hipMemcpyAsync(s->d_idx, tets, sizeof(int64_t) * n, hipMemcpyHostToDevice, s->stream));
         const int B = 256;
         const int nb = (int)((n + B - 1) / B);
         hipLaunchKernelGGL(stg::k_gather_counts, dim3(nb), dim3(B), 0, s->stream, s->d_counts + (size_t)spec * s->NT,
@@ -1151,6 +1156,29 @@ static void sum_range(stg_sim *s, const int32_t *ptr, int32_t n, int64_t *total)
     HIP_OK(hipMemcpyAsync(&res, s->d_sum, sizeof(res), hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
     *total = (int64_t)res;
+}
+
+int stg_sum_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64_t *tets, size_t n,
+                             int64_t *total) {
+    if (!s || !total || (n > 0 && !tets)) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        stage_indices(s, real, tets, n);
+        unsigned long long zero = 0, res = 0;
+        HIP_OK(hipMemcpyAsync(s->d_sum, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+        if (n > 0) {
+            const int B = 256;
+            const int nb = (int)std::min<size_t>((n + B - 1) / B, 2048);
+            hipLaunchKernelGGL(stg::k_sum_indexed, dim3(nb), dim3(B), 0, s->stream,
+                               s->d_counts + (size_t)spec * s->NT, s->d_idx, (int64_t)n, s->tet_off[real],
+                               s->d_sum);
+            HIP_OK(hipGetLastError());
+        }
+        HIP_OK(hipMemcpyAsync(&res, s->d_sum, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+        *total = (int64_t)res;
+    });
 }
 
 int stg_sum_tet_counts(stg_sim *s, int32_t real, int32_t spec, int64_t *total) {

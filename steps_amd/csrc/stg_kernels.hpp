@@ -719,4 +719,22 @@ __global__ void k_sum_counts(const int32_t *__restrict__ v, int32_t n,
     }
 }
 
+// Sum of counts[base + idx[i]] over i < n into one int64 (an ROI total).
+__global__ void k_sum_indexed(const int32_t *__restrict__ counts, const int64_t *__restrict__ idx,
+                              int64_t n, int32_t base, unsigned long long *__restrict__ total) {
+    long long acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += counts[base + idx[i]];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    __shared__ long long red[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long s = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) s += red[w];
+        atomicAdd(total, (unsigned long long)s);
+    }
+}
+
 }  // namespace stg

@@ -175,6 +175,13 @@ class Engine:
                                                     N.ptr(out, C.c_double)))
         return out
 
+    def sum_batch_tet_counts(self, spec: int, tets, real: int = 0) -> int:
+        idx = np.ascontiguousarray(np.asarray(tets, dtype=np.int64))
+        t = C.c_int64()
+        self._chk(self.lib.stg_sum_batch_tet_counts(self.h, real, spec, N.ptr(idx, C.c_int64), len(idx),
+                                                    C.byref(t)))
+        return t.value
+
     def sum_tet_counts(self, spec: int, real: int = 0) -> int:
         t = C.c_int64()
         self._chk(self.lib.stg_sum_tet_counts(self.h, real, spec, C.byref(t)))

@@ -34,6 +34,9 @@ def _face_keys(faces: np.ndarray, nverts: int) -> np.ndarray:
     return keys
 
 
+ELEM_VERTEX, ELEM_TRI, ELEM_TET, ELEM_UNDEFINED = 0, 1, 2, 99   # src/steps/geom/tetmesh.hpp:46
+
+
 class Tetmesh:
     """Unstructured tetrahedral mesh with its face topology and measures (SI units)."""
 
@@ -96,6 +99,40 @@ class Tetmesh:
         self.patches: dict[str, TmPatch] = {}
         self.membs: dict[str, Memb] = {}
         self.diffbs: dict[str, DiffBoundary] = {}
+        self.rois: dict[str, tuple[int, np.ndarray]] = {}
+
+    # --- regions of interest (Tetmesh::addROI, src/steps/geom/tetmesh.cpp) ---
+    def addROI(self, id: str, type: int, indices) -> None:
+        if id in self.rois:
+            raise ValueError(f"ROI {id!r} already exists")
+        if type not in (ELEM_VERTEX, ELEM_TRI, ELEM_TET):
+            raise ValueError("unknown ROI element type")
+        idx = np.unique(np.asarray(list(indices), dtype=np.int64))
+        limit = {ELEM_VERTEX: len(self.verts), ELEM_TRI: len(self.tris), ELEM_TET: len(self.tets)}[type]
+        if len(idx) and (idx[0] < 0 or idx[-1] >= limit):
+            raise ValueError("ROI element index out of range")
+        self.rois[id] = (type, idx)
+
+    def removeROI(self, id: str) -> None:
+        self._roi(id)
+        del self.rois[id]
+
+    def _roi(self, id: str):
+        if id not in self.rois:
+            raise ValueError(f"ROI {id!r} does not exist")
+        return self.rois[id]
+
+    def getROIType(self, id: str) -> int:
+        return self._roi(id)[0]
+
+    def getROIData(self, id: str):
+        return self._roi(id)[1].tolist()
+
+    def getROIDataSize(self, id: str) -> int:
+        return len(self._roi(id)[1])
+
+    def getAllROINames(self):
+        return list(self.rois)
 
     # --- reference-style accessors (src/steps/geom/tetmesh.hpp) ---
     def countTets(self) -> int:

@@ -450,6 +450,88 @@ class TetOpSplit:
         self.getBatchTetSpecCountsNP(tet_array, s, c)
         return float(c.sum())
 
+    # --- regions of interest (tetopsplit.cpp getROI*/setROI*) ---
+    def _roi(self, roi: str, want: int):
+        from .geom import ELEM_TET
+        if roi not in self.geom.rois:
+            raise _argerr(f"ROI {roi!r} does not exist")
+        typ, idx = self.geom.rois[roi]
+        if typ != want:
+            raise _argerr(f"ROI {roi!r} holds {'tetrahedra' if typ == ELEM_TET else 'other elements'}, "
+                          f"not {'tetrahedra' if want == ELEM_TET else 'triangles'}")
+        return idx
+
+    def _roi_tets(self, roi: str):
+        from .geom import ELEM_TET
+        idx = self._roi(roi, ELEM_TET)
+        if np.any(self._tet_local[idx] < 0):
+            raise _argerr(f"ROI {roi!r} holds a tetrahedron outside every compartment")
+        return idx
+
+    def getROIVol(self, roi: str) -> float:
+        return float(self.geom.tet_vol[self._roi_tets(roi)].sum())
+
+    def getROIArea(self, roi: str) -> float:
+        from .geom import ELEM_TRI
+        return float(self.geom.tri_area[self._roi(roi, ELEM_TRI)].sum())
+
+    def getROITetSpecCounts(self, roi: str, s: str):
+        idx = self._roi_tets(roi)
+        c = np.zeros(len(idx))
+        self.getBatchTetSpecCountsNP(idx, s, c)
+        return c.tolist()
+
+    def getROITetSpecCountsNP(self, roi: str, s: str, counts):
+        self.getBatchTetSpecCountsNP(self._roi_tets(roi), s, counts)
+
+    def getROITriSpecCounts(self, roi: str, s: str):
+        from .geom import ELEM_TRI
+        idx = self._roi(roi, ELEM_TRI)
+        c = np.zeros(len(idx))
+        self.getBatchTriSpecCountsNP(idx, s, c)
+        return c.tolist()
+
+    def getROITriSpecCountsNP(self, roi: str, s: str, counts):
+        from .geom import ELEM_TRI
+        self.getBatchTriSpecCountsNP(self._roi(roi, ELEM_TRI), s, counts)
+
+    def getROISpecCount(self, roi: str, s: str) -> float:
+        """Total over the ROI, reduced on the device per compartment."""
+        from .geom import ELEM_TET, ELEM_TRI
+        typ = self.geom.rois[roi][0] if roi in self.geom.rois else None
+        if typ == ELEM_TRI:
+            return float(sum(self.getROITriSpecCounts(roi, s)))
+        idx = self._roi_tets(roi)
+        sp = self._spec(s)
+        real, loc = self._tet_real[idx], self._tet_local[idx]
+        return float(sum(self.eng.sum_batch_tet_counts(sp, loc[real == ci], real=int(ci))
+                         for ci in np.unique(real)))
+
+    def setROISpecCount(self, roi: str, s: str, n: float):
+        """Distribute n molecules over the ROI's tets by volume (setROISpecCount)."""
+        idx = self._roi_tets(roi)
+        sp = self._spec(s)
+        real, loc = self._tet_real[idx], self._tet_local[idx]
+        vol = self.geom.tet_vol[idx]
+        add = distribute_counts(n, vol, self._host_rng)
+        for ci in np.unique(real):
+            m = real == ci
+            c = self.eng.get_tet_counts(sp, real=int(ci))
+            c[loc[m]] = add[m]
+            self.eng.set_tet_counts(sp, c, real=int(ci))
+
+    def getROISpecAmount(self, roi: str, s: str) -> float:
+        return self.getROISpecCount(roi, s) / AVOGADRO
+
+    def setROISpecAmount(self, roi: str, s: str, a: float):
+        self.setROISpecCount(roi, s, a * AVOGADRO)
+
+    def getROISpecConc(self, roi: str, s: str) -> float:
+        return self.getROISpecCount(roi, s) / (1e3 * self.getROIVol(roi) * AVOGADRO)
+
+    def setROISpecConc(self, roi: str, s: str, conc: float):
+        self.setROISpecCount(roi, s, conc * 1e3 * self.getROIVol(roi) * AVOGADRO)
+
     # --- patches and triangles ---
     def getPatchSpecCount(self, p: str, s: str) -> float:
         self._check_patch(p)

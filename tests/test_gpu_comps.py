@@ -121,3 +121,34 @@ def test_dist_solver_with_compartments_matches_oracle():
     got = np.concatenate([np.stack([sim.eng.get_tet_counts(s, real=c) for s in range(S)])
                           for c in range(2)], axis=1)
     assert np.array_equal(got, o.counts)
+
+
+def test_roi_totals_on_device():
+    """ROIs across both compartments: device-reduced totals equal the sum of
+    the per-tet counts; setROISpecCount places exactly n molecules in the ROI."""
+    from steps_amd.geom import ELEM_TET, ELEM_TRI
+
+    mdl, mesh, comps, db = W.two_comp_cube(n=4, side=4e-6)
+    rng = np.random.default_rng(0)
+    roi_tets = rng.choice(mesh.countTets(), 100, replace=False)
+    mesh.addROI("blob", ELEM_TET, roi_tets)
+    mesh.addROI("faces", ELEM_TRI, mesh.getSurfTris()[:10])
+    r = R.create("r123", 512)
+    r.initialize(3)
+    print("case roi", flush=True)
+    sim = TetOpSplit(mdl, mesh, r)
+    sim.setCompSpecCount("left", "A", 5000)
+    sim.setCompSpecCount("right", "A", 3000)
+    counts = np.array([sim.getTetSpecCount(int(t), "A") for t in sorted(roi_tets)])
+    assert sim.getROISpecCount("blob", "A") == counts.sum()
+    c = np.zeros(100)
+    sim.getROITetSpecCountsNP("blob", "A", c)
+    assert np.array_equal(c, counts)
+    assert abs(sim.getROIVol("blob") - mesh.tet_vol[roi_tets].sum()) < 1e-30
+    sim.setROISpecCount("blob", "C", 777)
+    assert sim.getROISpecCount("blob", "C") == 777
+    with pytest.raises(N.ArgErr):
+        sim.getROISpecCount("faces", "A")      # a triangle ROI has no volume species
+    with pytest.raises(N.ArgErr):
+        sim.getROIVol("nope")
+    assert abs(sim.getROIArea("faces") - mesh.tri_area[mesh.getSurfTris()[:10]].sum()) < 1e-30
