@@ -33,7 +33,7 @@
  * (src/steps/rng/r123.hpp:76), but keyed by (seed) with counter
  * (tet, period, stream, channel<<24 | block) instead of one sequential stream
  * per rank. Binomials: inversion (Kachitvichyanukul & Schmeiser 1988, BINV)
- * for n*min(p,1-p) < 10, else BTRS (Hoermann 1993, "The generation of binomial
+ * for n*min(p,1-p) < 30 (that paper's switch point), else BTRS (Hoermann 1993, "The generation of binomial
  * random variates", J. Stat. Comput. Simul. 46:101-110). log and sqrt are
  * written out with + - * / only so that the device path reproduces them.
  *
@@ -215,7 +215,7 @@ static double ipow(double base, int32_t n) {
     return r;
 }
 
-/* BINV: sequential inversion from k = 0, q <= 1/2, n*q < 10, written in
+/* BINV: sequential inversion from k = 0, q <= 1/2, n*q < 30, written in
  * factorial-scaled form so the search loop has no division: with
  * U_x = u_x x! and R_x = f(x) x!, U_{x+1} = (U_x - R_x)(x+1) and
  * R_{x+1} = R_x (n-x) q/(1-q). */
@@ -234,7 +234,7 @@ static int32_t binom_inv(ostream *st, int32_t n, double q) {
     }
 }
 
-/* BTRS: transformed rejection with squeeze, q <= 1/2, n*q >= 10. */
+/* BTRS: transformed rejection with squeeze, q <= 1/2, n*q >= 30. */
 static int32_t binom_btrs(ostream *st, int32_t n, double q) {
     double nd = (double)n;
     double spq = stgo_sqrt(nd * q * (1.0 - q));
@@ -267,7 +267,7 @@ static int32_t binom(ostream *st, int32_t n, double p) {
     if (p >= 1.0) return n;
     int flip = p > 0.5;
     double q = flip ? 1.0 - p : p;
-    int32_t k = ((double)n * q < 10.0) ? binom_inv(st, n, q) : binom_btrs(st, n, q);
+    int32_t k = ((double)n * q < 30.0) ? binom_inv(st, n, q) : binom_btrs(st, n, q);
     return flip ? n - k : k;
 }
 

@@ -166,9 +166,14 @@ __device__ __forceinline__ double ipow(double base, int32_t n) {
 }
 
 // Binomial by sequential inversion (Kachitvichyanukul & Schmeiser 1988, BINV);
-// q <= 1/2 and n*q < 10 so the expected search length is below 11. Written in
-// factorial-scaled form (U_x = u_x x!, R_x = f(x) x!) so the search loop needs
-// no division: U_{x+1} = (U_x - R_x)(x+1), R_{x+1} = R_x (n-x) q/(1-q).
+// q <= 1/2 and n*q < 30, the switch point of that paper, so the search length
+// is n*q + 1 on average and the cap of 150 steps lies more than 20 standard
+// deviations above the mean. Written in factorial-scaled form (U_x = u_x x!,
+// R_x = f(x) x!, both below 150! < 1e263) so the search loop needs no
+// division: U_{x+1} = (U_x - R_x)(x+1), R_{x+1} = R_x (n-x) q/(1-q).
+// On a wavefront the loop runs to the largest draw among the lanes; below
+// n*q = 30 that costs fewer instructions than the rejection sampler, whose
+// wavefront runs to its slowest lane's trial count and exact test.
 __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double q) {
     double r = ipow(1.0 - q, n);
     const double s = q / (1.0 - q);
@@ -185,7 +190,7 @@ __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double 
 }
 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
-// q <= 1/2 and n*q >= 10. The log-factorial bound is only formed when the
+// q <= 1/2 and n*q >= 30. The log-factorial bound is only formed when the
 // squeeze fails (about one trial in seven). Bounded at 1000 trials
 // (acceptance is > 0.8 per trial).
 __device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
@@ -224,7 +229,7 @@ __device__ __forceinline__ int32_t binomial(Philox& g, int32_t n, double p) {
     if (p >= 1.0) return n;
     const bool flip = p > 0.5;
     const double q = flip ? 1.0 - p : p;
-    const int32_t k = ((double)n * q < 10.0) ? binom_inversion(g, n, q) : binom_btrs(g, n, q);
+    const int32_t k = ((double)n * q < 30.0) ? binom_inversion(g, n, q) : binom_btrs(g, n, q);
     return flip ? n - k : k;
 }
 

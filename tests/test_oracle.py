@@ -159,6 +159,9 @@ def _chi2_pvalue(sample, n, p):
 @pytest.mark.parametrize("n,p", [
     (1, 0.3), (10, 0.5), (20, 0.04), (50, 0.19), (50, 0.21), (100, 0.5), (1000, 0.003),
     (1000, 0.7), (30, 0.97), (200, 0.999), (100_000, 0.3), (5000, 0.0015), (11, 0.95),
+    # either side of the inversion / rejection switch at n*min(p, 1-p) = 30, and
+    # inversion with a long search from a large n
+    (100, 0.29), (100, 0.31), (60, 0.51), (200_000, 0.000149),
 ])
 def test_binomial_matches_exact_pmf(n, p):
     x = bind.binomials(42, n, 7, 3, 9, n, p, 200_000)
