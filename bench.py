@@ -25,6 +25,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from steps_amd import workloads as W  # noqa: E402
+from steps_amd.autotune import autotune  # noqa: E402
 from steps_amd.engine import Engine  # noqa: E402
 
 METRIC = "tet-steps/sec (batched vs plain, same GPU) + achieved HBM GB/s vs peak"
@@ -83,8 +84,10 @@ def load_engine(w, seed, block=256, graph=False):
     return eng
 
 
-def time_plain(w, steps, warmup, seed=1, block=256, graph=False):
-    eng = load_engine(w, seed, block, graph)
+def time_plain(w, steps, warmup, seed=1, block=0, graph=False):
+    """block = 0: launch shape chosen by the autotuner on this state."""
+    eng = load_engine(w, seed, block or 256, graph)
+    tuned = autotune(eng, trial_periods=4) if not block else None
     dt = eng.update_period()
     if warmup:
         eng.run(warmup * dt)
@@ -95,17 +98,20 @@ def time_plain(w, steps, warmup, seed=1, block=256, graph=False):
     periods = eng.nsteps() - n0
     ms_launch, launches = eng.last_run_kernel_ms()
     return dict(eng=eng, wall=wall, periods=periods, tet_steps=periods * w.n_tets,
-                ms_launch=ms_launch, launches=launches, dt=dt)
+                ms_launch=ms_launch, launches=launches, dt=dt, tuned=tuned,
+                block=tuned["block"] if tuned else block, graph=tuned["graph"] if tuned else graph)
 
 
-def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=256, graph=True):
+def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True):
     fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=n_real, seed=seed,
                                                              end_time=end_time)
     eng = Engine(fm, meshes, [seed + 1] * n_real, reac_k=reac_k)
     for r in range(n_real):
         for s in range(3):
             eng.set_tet_counts(s, counts[r][s], real=r)
-    eng.set_launch(block, graph)
+    eng.set_launch(block or 256, graph)
+    # launch shape from the autotuner (block = 0), timed on this ensemble's own state
+    tuned = autotune(eng, trial_periods=3, blocks=(64, 128, 256)) if not block else None
     # warm-up pass (captures and instantiates the graph), then the timed pass
     eng.run(end_time)
     eng.reset()
@@ -126,7 +132,7 @@ def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=256, graph=True):
         tot = [eng.sum_tet_counts(s, r) for s in range(3)]
         ok &= (tot[0] + tot[2] == counts[r][0].sum() + counts[r][2].sum())
     return dict(wall=wall, tet_steps=tet_steps, n_tets=tot_tets, ms_launch=ms_launch,
-                launches=launches, bytes_per_tet_step=btet, conserved=bool(ok),
+                launches=launches, bytes_per_tet_step=btet, conserved=bool(ok), tuned=tuned,
                 max_periods=max(eng.nsteps(r) for r in range(n_real)))
 
 
@@ -222,8 +228,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=338)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--block", type=int, default=256)
-    ap.add_argument("--graph", action="store_true")
+    ap.add_argument("--block", type=int, default=0,
+                    help="threads per block of the period kernel; 0 = chosen by the autotuner")
+    ap.add_argument("--graph", action="store_true", help="with --block: replay launches as a graph")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -273,8 +280,9 @@ def main():
             "n_tets": w.n_tets,
             "period_s": p["dt"],
             "simulated_s": p["periods"] * p["dt"],
-            "block": args.block,
-            "graph": bool(args.graph),
+            "block": p["block"],
+            "graph": bool(p["graph"]),
+            "schedule": "autotuned" if p["tuned"] else "fixed",
         },
         "roofline": {
             "bound": "hbm",
@@ -293,6 +301,7 @@ def main():
         small = W.ab_binding_cube(n=13, side=10e-6, seed=0)
         ps = time_plain(small, 9, 1)
         ens = time_ensemble()
+        sched = ens["tuned"] or {}
         out["batched"] = {
             "workload": "ab_binding_ensemble: 1,024 realizations, Kuhn cubes n=6..20 "
                         "(1,296..48,000 tets), kf/kb log-uniform +-1 decade, 0.1 s each",
@@ -305,6 +314,9 @@ def main():
             "conserved": ens["conserved"],
             "plain_same_model_tet_steps_per_s": ps["tet_steps"] / ps["wall"],
             "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
+            "schedule": {"block": sched.get("block"), "graph": sched.get("graph"),
+                         "autotune_wall_ms_per_launch": sched.get("timings")},
+            "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"])},
         }
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])

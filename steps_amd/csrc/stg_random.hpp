@@ -174,19 +174,23 @@ __device__ __forceinline__ double ipow(double base, int32_t n) {
 // On a wavefront the loop runs to the largest draw among the lanes; below
 // n*q = 30 that costs fewer instructions than the rejection sampler, whose
 // wavefront runs to its slowest lane's trial count and exact test.
+// The search stops at x = min(n, 150); x and n - x are carried as doubles
+// (exact integers), which gives the same operands as converting them each step.
 __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double q) {
     double r = ipow(1.0 - q, n);
     const double s = q / (1.0 - q);
     double u = g.unif();
+    const int32_t lim = n < 150 ? n : 150;
     int32_t x = 0;
-    while (true) {
-        if (u <= r) return x;
-        if (x >= n) return n;
-        if (x >= 150) return x;
-        x += 1;
-        u = (u - r) * (double)x;
-        r = r * ((double)(n - x + 1) * s);
+    double xd = 0.0, nx = (double)n;
+    while (!(u <= r) && x < lim) {
+        ++x;
+        xd = xd + 1.0;
+        u = (u - r) * xd;
+        r = r * (nx * s);
+        nx = nx - 1.0;
     }
+    return x;
 }
 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
