@@ -87,7 +87,7 @@ def load_engine(w, seed, block=256, graph=False):
 def time_plain(w, steps, warmup, seed=1, block=0, graph=False):
     """block = 0: launch shape chosen by the autotuner on this state."""
     eng = load_engine(w, seed, block or 256, graph)
-    tuned = autotune(eng, trial_periods=4) if not block else None
+    tuned = autotune(eng, trial_periods=20) if not block else None
     dt = eng.update_period()
     if warmup:
         eng.run(warmup * dt)
@@ -283,6 +283,7 @@ def main():
             "block": p["block"],
             "graph": bool(p["graph"]),
             "schedule": "autotuned" if p["tuned"] else "fixed",
+            "autotune_wall_ms_per_launch": p["tuned"]["timings"] if p["tuned"] else None,
         },
         "roofline": {
             "bound": "hbm",

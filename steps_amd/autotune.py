@@ -17,7 +17,9 @@ from .engine import Engine
 
 
 def autotune(engine: Engine, trial_periods: int = 6, blocks=(64, 128, 256, 512),
-             graphs=(False, True)) -> dict:
+             graphs=(False, True), repeats: int = 2) -> dict:
+    """Each shape runs trial_periods periods once to warm up and then `repeats`
+    times timed; its figure is the fastest of those."""
     state = engine.checkpoint()
     t0 = engine.time()
     dts = [engine.update_period(r) for r in range(engine.n_real)]
@@ -29,12 +31,17 @@ def autotune(engine: Engine, trial_periods: int = 6, blocks=(64, 128, 256, 512),
             engine.restore(state)
             engine.set_launch(b, g)
             engine.run(t0 + span)            # warm-up (captures the graph)
-            engine.restore(state)
-            tic = time.perf_counter()
-            engine.run(t0 + span)            # returns once the stream has drained
-            wall = time.perf_counter() - tic
-            ms, n = engine.last_run_kernel_ms()
-            timings[(b, g)] = (wall * 1e3 / max(n, 1), ms)
+            best_t = None
+            for _ in range(max(1, repeats)):
+                engine.restore(state)
+                tic = time.perf_counter()
+                engine.run(t0 + span)        # returns once the stream has drained
+                wall = time.perf_counter() - tic
+                ms, n = engine.last_run_kernel_ms()
+                cand = (wall * 1e3 / max(n, 1), ms)
+                if best_t is None or cand[0] < best_t[0]:
+                    best_t = cand
+            timings[(b, g)] = best_t
     best = min(timings, key=lambda k: timings[k][0])
     engine.restore(state)
     engine.set_launch(*best)
