@@ -153,7 +153,7 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
     """EField on the membrane cube's 1,030,301 vertices, coupled to the reaction-diffusion
     step: passive leak 1 S/m^2 at -65 mV, 1e-2 F/m^2, 1 ohm.m, 1 nA on the x = 0 face.
     Mixed-precision solve timed; its dV compared with the FP64 CG solution of the same step."""
-    from steps_amd.efield import EField, FP64, MIXED
+    from steps_amd.efield import EField, FP64, JACOBI, MIXED, MULTIGRID
 
     mesh = w.mesh
     surf = mesh.getSurfTris()
@@ -190,6 +190,16 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
     it_64 = ef.stats()
     ef.set_solver(MIXED)
     err = float(np.abs(dv_mixed - dv_64).max() / np.abs(dv_64).max())
+    info = ef.precond_info()
+    # a few more coupled steps with the Jacobi preconditioner, for comparison
+    ef.set_precond(JACOBI)
+    jsteps = 3
+    t1 = eng.time()
+    tic = time.perf_counter()
+    ef.run(eng, t1 + jsteps * ef_dt, ef_dt)
+    jacobi_s = (time.perf_counter() - tic) / jsteps
+    it_j = ef.stats()
+    ef.set_precond(MULTIGRID)
     nv, nt, nnz = ef.sizes()
     return dict(
         workload="EField on calcium_membrane_cube: 1,030,301 vertices, 120,000 membrane "
@@ -198,7 +208,10 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
         ef_steps=ef_steps, cg_iterations_last=stats["iterations"],
         refinements_last=stats["refinements"], rel_residual_last=stats["rel_residual"],
         mixed_vs_fp64_dv_rel_err=err, mixed_iterations=it_mixed["iterations"],
-        fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3)
+        fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3,
+        multigrid_levels=info["levels"], multigrid_coarsest=info["coarsest"],
+        multigrid_setup_s=info["setup_s"], jacobi_step_ms=jacobi_s * 1e3,
+        jacobi_iterations=it_j["iterations"])
 
 
 def cpu_baseline(w, budget_s=15.0, seed=1):

@@ -319,6 +319,16 @@ int stg_efield_run(stg_efield *ef, stg_sim *sim, double endtime, double efdt);
  * gradients with FP64 dot products inside FP64 iterative refinement, 1 = FP64
  * conjugate gradients; tol = relative FP64 residual ||b - A dV|| / ||b||. */
 int stg_efield_set_solver(stg_efield *ef, int32_t precision, double tol, int32_t max_iter);
+/* Preconditioner (no reference counterpart; the reference's PETSc path uses
+ * point-block Jacobi, efield_operator.cpp:116-120): 1 = one V-cycle of
+ * smoothed-aggregation algebraic multigrid for systems above 16,384 vertices
+ * (default), set up on the host when the step size or a membrane parameter
+ * changes; 0 = Jacobi. Systems of up to 16,384 vertices always use the
+ * one-workgroup Jacobi solver. */
+int stg_efield_set_precond(stg_efield *ef, int32_t kind);
+/* Multigrid levels in use (0 = none), size of the dense coarsest level and
+ * the host time of the last set-up in seconds. */
+int stg_efield_precond_info(const stg_efield *ef, int32_t *levels, int32_t *coarsest, double *setup_s);
 int stg_efield_stats(const stg_efield *ef, int32_t *iters, int32_t *refinements,
                      double *rel_residual);
 int stg_efield_sizes(const stg_efield *ef, int32_t *n_verts, int32_t *n_tris, int32_t *nnz);

@@ -167,6 +167,8 @@ SIGNATURES = {
     "stg_efield_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),
     "stg_efield_set_solver": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, C.c_int32]),
     "stg_efield_stats": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
+    "stg_efield_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
+    "stg_efield_precond_info": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
     "stg_efield_sizes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
 }
 

@@ -9,11 +9,14 @@
 // over the barycentric dual cells (TetCoupler, tetcoupler.cpp:48-290), which is
 // the P1 stiffness -sum_tets V grad(phi_i).grad(phi_j) assembled here directly.
 // The matrix is symmetric positive definite; the reference solves it by banded
-// LU (bdsystem.cpp) or PETSc CG (dVsolver_petsc.cpp:163-268). Here: Jacobi-
-// preconditioned CG with the matrix and vectors in FP32 and every dot product
-// accumulated in FP64, wrapped in FP64 iterative refinement (residual b - A x
-// in FP64) until the FP64 residual meets the tolerance. An all-FP64 CG mode
-// gives the reference solution on the device.
+// LU (bdsystem.cpp) or PETSc CG (dVsolver_petsc.cpp:163-268). Here: CG with the
+// matrix and vectors in FP32 and every dot product accumulated in FP64, wrapped
+// in FP64 iterative refinement (residual b - A x in FP64) until the FP64
+// residual meets the tolerance. An all-FP64 CG mode gives the reference
+// solution on the device. The preconditioner is one V-cycle of smoothed-
+// aggregation algebraic multigrid (set up on the host once per time step size
+// and membrane parameters, applied on the device) for systems above 16,384
+// vertices, and Jacobi below that or when chosen.
 //
 // Reductions are deterministic and atomic-free: each block writes its partial,
 // then one workgroup sums the partials in a fixed order and updates the CG
@@ -23,8 +26,10 @@
 #include <algorithm>
 #include <array>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/stg.h"
@@ -294,6 +299,149 @@ __global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *
     block_partial(rz, part);
 }
 
+// ------------------------------------------------------------- multigrid
+//
+// One V-cycle z = M r of smoothed-aggregation AMG (Vanek, Mandel & Brezina
+// 1996). Level 0 is the ELL matrix above; coarser levels and the transfer
+// operators are CSR. Smoothing is damped Jacobi from a zero guess, one sweep
+// down and one up, which keeps M symmetric for CG.
+
+// (A x)_i of the fine matrix for x given element-wise by xv(j)
+template <typename T, typename F>
+__device__ __forceinline__ double ell_row(const EfDev &m, double sigma, int i, F xv) {
+    if constexpr (sizeof(T) == 4) {
+        float acc = m.d32[i] * (float)xv(i);
+        for (int k = 0; k < m.width; ++k) {
+            const int e = k * m.n + i;
+            acc += m.a32[e] * (float)xv(m.col[e]);
+        }
+        return acc;
+    } else {
+        double acc = m.d64[i] * (double)xv(i);
+        if (!m.clamped[i])
+            for (int k = 0; k < m.width; ++k) {
+                const int e = k * m.n + i;
+                const int j = m.col[e];
+                if (!m.clamped[j]) acc -= sigma * m.w[e] * (double)xv(j);
+            }
+        return acc;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fine_diag(const EfDev &m, int i) {
+    if constexpr (sizeof(T) == 4) return m.d32[i];
+    else return m.d64[i];
+}
+
+// down sweep on level 0: x = om D^-1 b, r = b - A x (x of the neighbours formed on the fly)
+template <typename T>
+__global__ void k_amg0_down(EfDev m, double sigma, double om, const T *b, T *x, T *r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    auto xv = [&](int j) { return (T)(om * (double)b[j] / (double)fine_diag<T>(m, j)); };
+    const T xi = xv(i);
+    x[i] = xi;
+    r[i] = (T)((double)b[i] - ell_row<T>(m, sigma, i, xv));
+}
+
+// up sweep on level 0: out = x + om D^-1 (b - A x)
+template <typename T>
+__global__ void k_amg0_up(EfDev m, double sigma, double om, const T *b, const T *x, T *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.n) return;
+    auto xv = [&](int j) { return x[j]; };
+    const double ax = ell_row<T>(m, sigma, i, xv);
+    out[i] = (T)((double)x[i] + om * ((double)b[i] - ax) / (double)fine_diag<T>(m, i));
+}
+
+// CSR kernels give each row a group of G lanes (G a power of two sized to the
+// operator's mean row length): the group reads the row with stride G, so the
+// loads of a wavefront are coalesced, and sums it with a fixed shuffle tree,
+// so results do not depend on timing.
+template <typename T, int G>
+__device__ __forceinline__ double csr_row_dot(const int *ptr, const int *col, const T *val, const T *x, int row,
+                                              bool valid, int lane) {
+    double acc = 0.0;
+    if (valid)
+        for (int e = ptr[row] + lane; e < ptr[row + 1]; e += G) acc += (double)val[e] * (double)x[col[e]];
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+    return acc;
+}
+
+// x = om D^-1 b on a CSR level (pre-smoothing from a zero guess)
+template <typename T>
+__global__ void k_scale(int n, const T *dinv, double om, const T *b, T *x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = (T)(om * (double)b[i] * (double)dinv[i]);
+}
+
+// r = b - A x on a CSR level (diagonal stored in the rows)
+template <typename T, int G>
+__global__ void k_csr_resid(int n, const int *ptr, const int *col, const T *val, const T *b, const T *x, T *r) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / G, lane = t & (G - 1);
+    const bool valid = row < n;
+    const double ax = csr_row_dot<T, G>(ptr, col, val, x, row, valid, lane);
+    if (valid && lane == 0) r[row] = (T)((double)b[row] - ax);
+}
+
+// out = x + om D^-1 (b - A x) on a CSR level (post-smoothing)
+template <typename T, int G>
+__global__ void k_csr_up(int n, const int *ptr, const int *col, const T *val, const T *dinv, double om,
+                         const T *b, const T *x, T *out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / G, lane = t & (G - 1);
+    const bool valid = row < n;
+    const double ax = csr_row_dot<T, G>(ptr, col, val, x, row, valid, lane);
+    if (valid && lane == 0) out[row] = (T)((double)x[row] + om * ((double)b[row] - ax) * (double)dinv[row]);
+}
+
+// out (+)= M in for a CSR operator (restriction R = P^T, or prolongation P)
+template <typename T, int G>
+__global__ void k_csr_apply(int n, const int *ptr, const int *col, const T *val, const T *in, T *out, int add) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = t / G, lane = t & (G - 1);
+    const bool valid = row < n;
+    const double acc = csr_row_dot<T, G>(ptr, col, val, in, row, valid, lane);
+    if (valid && lane == 0) out[row] = (T)((add ? (double)out[row] : 0.0) + acc);
+}
+
+// coarsest level: x = A^-1 b with the dense inverse
+template <typename T>
+__global__ void k_dense_apply(int n, const T *inv, const T *b, T *x) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc += (double)inv[(size_t)i * n + j] * (double)b[j];
+    x[i] = (T)acc;
+}
+
+// PCG pieces around the multigrid preconditioner
+template <typename T>
+__global__ void k_cg_update(int n, const double *scal, T *x, T *r, const T *pnew, const T *q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = scal[0] / scal[3];
+    x[i] = (T)((double)x[i] + alpha * (double)pnew[i]);
+    r[i] = (T)((double)r[i] - alpha * (double)q[i]);
+}
+
+template <typename T>
+__global__ void k_cg_start(int n, const double *r64, T *x, T *r) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = (T)0;
+    r[i] = (T)r64[i];
+}
+
+template <typename T>
+__global__ void k_dot(int n, const T *a, const T *b, double *part) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double v = i < n ? (double)a[i] * (double)b[i] : 0.0;
+    block_partial(v, part);
+}
+
 // Sum over the block of one double per thread; every thread gets the total.
 __device__ __forceinline__ double block_allsum(double v, double *red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -418,6 +566,55 @@ __global__ void k_sumsq(int n, const double *v, double *part) {
 
 }  // namespace
 
+// Multigrid hierarchy on the device. Level l < nlev-1 holds the operator of
+// that level (level 0: the fine ELL matrix, not stored here), its prolongation
+// P_l to level l+1 and restriction R_l = P_l^T; the last level is dense.
+struct AmgLevel {
+    int n = 0;
+    double om = 0.0;                                    // Jacobi weight 4 / (3 rho(D^-1 A))
+    int ga = 1, gp = 1, gr = 1;                         // lanes per row of A_l, P_l, R_l
+    int *a_ptr = nullptr, *a_col = nullptr;             // A_l (l >= 1), diagonal included
+    float *a32 = nullptr, *dinv32 = nullptr;
+    double *a64 = nullptr, *dinv64 = nullptr;
+    int *p_ptr = nullptr, *p_col = nullptr;             // P_l: n rows
+    float *p32 = nullptr;
+    double *p64 = nullptr;
+    int *r_ptr = nullptr, *r_col = nullptr;             // R_l: n_{l+1} rows
+    float *r32 = nullptr;
+    double *r64 = nullptr;
+    double *b = nullptr, *x = nullptr, *r = nullptr, *out = nullptr;  // work vectors, FP64-sized
+};
+
+struct AmgDev {
+    bool built = false, usable = false;
+    double key_oodt = -1.0, key_sigma = -1.0;
+    long key_ver = -1;
+    std::vector<AmgLevel> lev;   // all but the coarsest
+    int nc = 0;                  // coarsest size
+    float *inv32 = nullptr;
+    double *inv64 = nullptr;
+    double *cb = nullptr, *cx = nullptr;
+    double setup_s = 0.0;
+
+    void release() {
+        for (AmgLevel &L : lev) {
+            void *ptrs[] = {L.a_ptr, L.a_col, L.a32, L.dinv32, L.a64, L.dinv64, L.p_ptr, L.p_col, L.p32, L.p64,
+                            L.r_ptr, L.r_col, L.r32, L.r64, L.b, L.x, L.r, L.out};
+            for (void *q : ptrs)
+                if (q) (void)hipFree(q);
+        }
+        lev.clear();
+        void *ptrs[] = {inv32, inv64, cb, cx};
+        for (void *q : ptrs)
+            if (q) (void)hipFree(q);
+        inv32 = nullptr;
+        inv64 = nullptr;
+        cb = cx = nullptr;
+        built = usable = false;
+        nc = 0;
+    }
+};
+
 struct stg_efield {
     int n = 0, n_tris = 0, nnz = 0, width = 0;
     std::vector<int> tri_v;                 // [n_tris*3]
@@ -435,6 +632,11 @@ struct stg_efield {
     int max_iter = 20000;
     int last_iters = 0, last_refine = 0;
     double last_rel = 0.0;
+    int precond = 1;                        // 1 = multigrid above 16,384 vertices, 0 = Jacobi
+    long params_ver = 0;                    // bumped whenever the matrix parameters change
+    std::vector<int> col_h;                 // host copy of the coupling table ([width][n])
+    std::vector<double> w_h, wsum_h;
+    AmgDev amg;
     OhmicK oc{};
     stg_sim *sim = nullptr;
     int sim_real = 0;
@@ -463,6 +665,7 @@ struct stg_efield {
     }
 
     ~stg_efield() {
+        amg.release();
         void *ptrs[] = {col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
                         tclamp, x64, r64, dv_prev, a32, d32, clamped, xs, rs, zs, ps, qs, part,
                         scal, outs, cnt};
@@ -506,6 +709,7 @@ int ef_guard(stg_efield *e, F &&f) {
 }
 
 void upload_params(stg_efield *e) {
+    ++e->params_ver;
     ef_put(e->cap, e->cap_h);
     ef_put(e->gext, e->gext_h);
     ef_put(e->vclamp, e->vert_clamp_h);
@@ -661,6 +865,9 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     ef_put(e->col, col);
     ef_put(e->w, w);
     ef_put(e->wsum, wsum);
+    e->col_h = std::move(col);
+    e->w_h = std::move(w);
+    e->wsum_h = std::move(wsum);
     ef_put(e->vt_ptr, vt_ptr);
     ef_put(e->vt_tri, vt_tri);
     ef_put(e->tri_vd, e->tri_v);
@@ -682,6 +889,9 @@ double fetch(stg_efield *e, const double *d) {
 constexpr int kSmallSystem = 16384;  // solved by one persistent workgroup
 
 template <typename T>
+int pcg_amg(stg_efield *e, double rel_tol, int max_iter);
+
+template <typename T>
 int pcg(stg_efield *e, double rel_tol, int max_iter) {
     const EfDev m = e->dev();
     const int nb = e->nb, n = e->n;
@@ -694,6 +904,7 @@ int pcg(stg_efield *e, double rel_tol, int max_iter) {
         EF_HIP(hipStreamSynchronize(e->stream));
         return it;
     }
+    if (e->precond == 1 && e->amg.usable) return pcg_amg<T>(e, rel_tol, max_iter);
     hipLaunchKernelGGL(k_cg_init<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->r64, x, r, z, e->part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 0, e->scal, e->outs);
     const double rz0 = fetch(e, e->scal);
@@ -712,6 +923,482 @@ int pcg(stg_efield *e, double rel_tol, int max_iter) {
         }
         const double rz = fetch(e, e->scal);
         if (!(rz > rel_tol * rel_tol * rz0)) break;
+    }
+    EF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_accumulate<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->x64, x);
+    return it;
+}
+
+// ------------------------------------------------------------- multigrid setup (host, FP64)
+
+struct Csr {
+    int n = 0, m = 0;  // rows, columns
+    std::vector<int> ptr, col;
+    std::vector<double> val;
+};
+
+// The fine matrix of a step with 1/dt = oodt (1/ms), exactly as k_assemble and
+// k_offdiag32 form it: diagonal C/dt + G + sigma sum_j w_ij, couplings
+// -sigma w_ij between unclamped vertices, identity rows for clamped ones.
+Csr fine_matrix(const stg_efield *e, double oodt) {
+    const int n = e->n, W = e->width;
+    Csr A;
+    A.n = A.m = n;
+    A.ptr.assign(n + 1, 0);
+    A.col.reserve((size_t)n * (W + 1));
+    A.val.reserve((size_t)n * (W + 1));
+    for (int i = 0; i < n; ++i) {
+        if (e->clamped_h[i]) {
+            A.col.push_back(i);
+            A.val.push_back(1.0);
+        } else {
+            A.col.push_back(i);
+            A.val.push_back(e->cap_h[i] * oodt + e->gext_h[i] + e->sigma * e->wsum_h[i]);
+            for (int k = 0; k < W; ++k) {
+                const int j = e->col_h[(size_t)k * n + i];
+                const double w = e->w_h[(size_t)k * n + i];
+                if (j == i || w == 0.0 || e->clamped_h[j]) continue;
+                A.col.push_back(j);
+                A.val.push_back(-e->sigma * w);
+            }
+        }
+        A.ptr[i + 1] = (int)A.col.size();
+    }
+    return A;
+}
+
+double diag_of(const Csr &A, int i) {
+    for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e)
+        if (A.col[e] == i) return A.val[e];
+    return 0.0;
+}
+
+// Upper estimate of the largest eigenvalue of D^-1 A: the Rayleigh quotient
+// (x, A x) / (x, D x) after 20 power steps, with a 10% margin, capped by the
+// Gershgorin bound max_i sum_j |a_ij| / a_ii.
+double rho_dinv_a(const Csr &A, const std::vector<double> &dinv) {
+    const int n = A.n;
+    std::vector<double> x(n), y(n);
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+    for (int i = 0; i < n; ++i) {
+        h ^= h << 13;
+        h ^= h >> 7;
+        h ^= h << 17;
+        x[i] = 0.5 + (double)(h >> 11) * 0x1p-53;
+    }
+    double lam = 0.0;
+    for (int it = 0; it < 20; ++it) {
+        double num = 0.0, den = 0.0, nrm = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) s += A.val[e] * x[A.col[e]];
+            num += x[i] * s;
+            den += x[i] * x[i] / dinv[i];
+            y[i] = dinv[i] * s;
+            nrm += y[i] * y[i];
+        }
+        lam = num / den;
+        nrm = std::sqrt(nrm);
+        if (!(nrm > 0.0)) break;
+        for (int i = 0; i < n; ++i) x[i] = y[i] / nrm;
+    }
+    double g = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) s += std::fabs(A.val[e]);
+        g = std::max(g, s * dinv[i]);
+    }
+    return std::min(1.1 * lam, g);
+}
+
+// Greedy aggregation on the strength graph (|a_ij| >= theta sqrt(a_ii a_jj)):
+// 1) a vertex whose strong neighbours are all free seeds an aggregate with
+// them; 2) a free vertex joins the first-pass aggregate of its strongest
+// neighbour; 3) what is left seeds aggregates with its free strong
+// neighbours. Vertices without strong neighbours (clamped or membrane rows the
+// capacitance dominates) stay outside every aggregate: -1.
+std::vector<int> aggregate(const Csr &A, const std::vector<double> &diag, double theta, int &nagg) {
+    const int n = A.n;
+    std::vector<int> sp(n + 1, 0), sc;
+    std::vector<double> sv;
+    sc.reserve(A.col.size());
+    sv.reserve(A.col.size());
+    for (int i = 0; i < n; ++i) {
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+            const int j = A.col[e];
+            const double a = A.val[e];
+            if (j == i || a == 0.0) continue;
+            if (a * a >= theta * theta * std::fabs(diag[i] * diag[j])) {
+                sc.push_back(j);
+                sv.push_back(std::fabs(a));
+            }
+        }
+        sp[i + 1] = (int)sc.size();
+    }
+    std::vector<int> agg(n, -1);
+    nagg = 0;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0 || sp[i] == sp[i + 1]) continue;
+        bool free = true;
+        for (int k = sp[i]; k < sp[i + 1] && free; ++k) free = agg[sc[k]] < 0;
+        if (!free) continue;
+        agg[i] = nagg;
+        for (int k = sp[i]; k < sp[i + 1]; ++k) agg[sc[k]] = nagg;
+        ++nagg;
+    }
+    const std::vector<int> agg1 = agg;
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0) continue;
+        int best = -1;
+        double bw = 0.0;
+        for (int k = sp[i]; k < sp[i + 1]; ++k)
+            if (agg1[sc[k]] >= 0 && sv[k] > bw) {
+                bw = sv[k];
+                best = agg1[sc[k]];
+            }
+        if (best >= 0) agg[i] = best;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (agg[i] >= 0 || sp[i] == sp[i + 1]) continue;
+        agg[i] = nagg;
+        for (int k = sp[i]; k < sp[i + 1]; ++k)
+            if (agg[sc[k]] < 0) agg[sc[k]] = nagg;
+        ++nagg;
+    }
+    return agg;
+}
+
+// P = (I - om D^-1 A) P_tent with P_tent the aggregate indicator
+Csr smoothed_prolongator(const Csr &A, const std::vector<double> &dinv, const std::vector<int> &agg, int nagg,
+                         double om) {
+    Csr P;
+    P.n = A.n;
+    P.m = nagg;
+    P.ptr.assign(A.n + 1, 0);
+    std::vector<std::pair<int, double>> row;
+    for (int i = 0; i < A.n; ++i) {
+        row.clear();
+        auto addv = [&row](int c, double v) {
+            for (auto &kv : row)
+                if (kv.first == c) {
+                    kv.second += v;
+                    return;
+                }
+            row.emplace_back(c, v);
+        };
+        if (agg[i] >= 0) addv(agg[i], 1.0);
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) {
+            const int j = A.col[e];
+            if (agg[j] >= 0) addv(agg[j], -om * dinv[i] * A.val[e]);
+        }
+        std::sort(row.begin(), row.end());
+        for (const auto &kv : row)
+            if (kv.second != 0.0) {
+                P.col.push_back(kv.first);
+                P.val.push_back(kv.second);
+            }
+        P.ptr[i + 1] = (int)P.col.size();
+    }
+    return P;
+}
+
+Csr transpose(const Csr &P) {
+    Csr R;
+    R.n = P.m;
+    R.m = P.n;
+    R.ptr.assign(P.m + 1, 0);
+    for (int c : P.col) ++R.ptr[c + 1];
+    for (int c = 0; c < P.m; ++c) R.ptr[c + 1] += R.ptr[c];
+    R.col.resize(P.col.size());
+    R.val.resize(P.val.size());
+    std::vector<int> pos(R.ptr.begin(), R.ptr.end() - 1);
+    for (int i = 0; i < P.n; ++i)
+        for (int e = P.ptr[i]; e < P.ptr[i + 1]; ++e) {
+            const int k = pos[P.col[e]]++;
+            R.col[k] = i;
+            R.val[k] = P.val[e];
+        }
+    return R;
+}
+
+// C = X Y with a dense accumulator per row (columns sorted)
+Csr spgemm(const Csr &X, const Csr &Y) {
+    Csr C;
+    C.n = X.n;
+    C.m = Y.m;
+    C.ptr.assign(X.n + 1, 0);
+    std::vector<double> acc(Y.m, 0.0);
+    std::vector<int> mark(Y.m, -1), touched;
+    for (int i = 0; i < X.n; ++i) {
+        touched.clear();
+        for (int e = X.ptr[i]; e < X.ptr[i + 1]; ++e) {
+            const int j = X.col[e];
+            const double a = X.val[e];
+            for (int f = Y.ptr[j]; f < Y.ptr[j + 1]; ++f) {
+                const int c = Y.col[f];
+                if (mark[c] != i) {
+                    mark[c] = i;
+                    acc[c] = 0.0;
+                    touched.push_back(c);
+                }
+                acc[c] += a * Y.val[f];
+            }
+        }
+        std::sort(touched.begin(), touched.end());
+        for (int c : touched) {
+            C.col.push_back(c);
+            C.val.push_back(acc[c]);
+        }
+        C.ptr[i + 1] = (int)C.col.size();
+    }
+    return C;
+}
+
+// Inverse of a small dense matrix by Gauss-Jordan elimination with partial pivoting
+std::vector<double> dense_inverse(const Csr &A) {
+    const int n = A.n;
+    std::vector<double> M((size_t)n * n, 0.0), I((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) {
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) M[(size_t)i * n + A.col[e]] = A.val[e];
+        I[(size_t)i * n + i] = 1.0;
+    }
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[(size_t)r * n + c]) > std::fabs(M[(size_t)piv * n + c])) piv = r;
+        const double pv = M[(size_t)piv * n + c];
+        if (!(std::fabs(pv) > 0.0)) throw EfError{STG_ERR_PROG, "singular coarsest multigrid level"};
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(M[(size_t)piv * n + k], M[(size_t)c * n + k]);
+                std::swap(I[(size_t)piv * n + k], I[(size_t)c * n + k]);
+            }
+        const double ip = 1.0 / pv;
+        for (int k = 0; k < n; ++k) {
+            M[(size_t)c * n + k] *= ip;
+            I[(size_t)c * n + k] *= ip;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = M[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+                I[(size_t)r * n + k] -= f * I[(size_t)c * n + k];
+            }
+        }
+    }
+    return I;
+}
+
+int *up_int(const std::vector<int> &v) {
+    int *d = ef_alloc<int>(v.size());
+    ef_put(d, v);
+    return d;
+}
+
+double *up_f64(const std::vector<double> &v) {
+    double *d = ef_alloc<double>(v.size());
+    ef_put(d, v);
+    return d;
+}
+
+float *up_f32(const std::vector<double> &v) {
+    std::vector<float> f(v.begin(), v.end());
+    float *d = ef_alloc<float>(f.size());
+    ef_put(d, f);
+    return d;
+}
+
+constexpr int kCoarsest = 400;  // levels at or below this size get a dense inverse
+
+// lanes per row: the power of two nearest the mean row length, 1..64
+int group_size(const Csr &M) {
+    const double mean = M.n > 0 ? (double)M.col.size() / M.n : 1.0;
+    int g = 1;
+    while (g < 64 && g * 1.5 < mean) g *= 2;
+    return g;
+}
+
+// Build the hierarchy for the matrix of a step with 1/dt = oodt. Leaves
+// amg.usable false when coarsening stalls above a size a dense inverse can take.
+void amg_build(stg_efield *e, double oodt) {
+    const auto t0 = std::chrono::steady_clock::now();
+    AmgDev &G = e->amg;
+    G.release();
+    G.built = true;
+    G.key_oodt = oodt;
+    G.key_sigma = e->sigma;
+    G.key_ver = e->params_ver;
+    Csr A = fine_matrix(e, oodt);
+    for (int l = 0; l < 12 && A.n > kCoarsest; ++l) {
+        std::vector<double> diag(A.n), dinv(A.n);
+        for (int i = 0; i < A.n; ++i) {
+            diag[i] = diag_of(A, i);
+            if (!(diag[i] > 0.0)) throw EfError{STG_ERR_PROG, "multigrid level with a non-positive diagonal"};
+            dinv[i] = 1.0 / diag[i];
+        }
+        const double om = 4.0 / (3.0 * rho_dinv_a(A, dinv));
+        int nagg = 0;
+        // every coupling counts as strong: on these P1 stiffness matrices a
+        // threshold leaves small aggregates and dense coarse operators
+        const std::vector<int> agg = aggregate(A, diag, 0.0, nagg);
+        if (nagg == 0 || nagg > 0.85 * A.n) break;  // coarsening stalled
+        Csr P = smoothed_prolongator(A, dinv, agg, nagg, om);
+        Csr R = transpose(P);
+        Csr Ac = spgemm(R, spgemm(A, P));
+        AmgLevel L;
+        L.n = A.n;
+        L.om = om;
+        L.ga = group_size(A);
+        L.gp = group_size(P);
+        L.gr = group_size(R);
+        if (l > 0) {
+            L.a_ptr = up_int(A.ptr);
+            L.a_col = up_int(A.col);
+            L.a32 = up_f32(A.val);
+            L.a64 = up_f64(A.val);
+            L.dinv32 = up_f32(dinv);
+            L.dinv64 = up_f64(dinv);
+            L.b = ef_alloc<double>(A.n);
+            L.out = ef_alloc<double>(A.n);
+        }
+        L.p_ptr = up_int(P.ptr);
+        L.p_col = up_int(P.col);
+        L.p32 = up_f32(P.val);
+        L.p64 = up_f64(P.val);
+        L.r_ptr = up_int(R.ptr);
+        L.r_col = up_int(R.col);
+        L.r32 = up_f32(R.val);
+        L.r64 = up_f64(R.val);
+        L.x = ef_alloc<double>(A.n);
+        L.r = ef_alloc<double>(A.n);
+        G.lev.push_back(L);
+        A = std::move(Ac);
+    }
+    if (G.lev.empty() || A.n > 5 * kCoarsest) {
+        G.release();
+        G.built = true;  // remember the key: Jacobi for this matrix
+        G.key_oodt = oodt;
+        G.key_sigma = e->sigma;
+        G.key_ver = e->params_ver;
+        return;
+    }
+    const std::vector<double> inv = dense_inverse(A);
+    G.nc = A.n;
+    G.inv32 = up_f32(inv);
+    G.inv64 = up_f64(inv);
+    G.cb = ef_alloc<double>(A.n);
+    G.cx = ef_alloc<double>(A.n);
+    G.usable = true;
+    G.setup_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// launch K<T, G> with G lanes per row over n rows
+template <typename T, template <typename, int> class K>
+struct GroupLaunch {
+    template <typename... A>
+    static void run(int g, int n, hipStream_t st, A... args) {
+        const long threads = (long)n * g;
+        const dim3 grid((unsigned)((threads + kB - 1) / kB));
+        switch (g) {
+            case 1: hipLaunchKernelGGL((K<T, 1>::fn), grid, dim3(kB), 0, st, args...); break;
+            case 2: hipLaunchKernelGGL((K<T, 2>::fn), grid, dim3(kB), 0, st, args...); break;
+            case 4: hipLaunchKernelGGL((K<T, 4>::fn), grid, dim3(kB), 0, st, args...); break;
+            case 8: hipLaunchKernelGGL((K<T, 8>::fn), grid, dim3(kB), 0, st, args...); break;
+            case 16: hipLaunchKernelGGL((K<T, 16>::fn), grid, dim3(kB), 0, st, args...); break;
+            case 32: hipLaunchKernelGGL((K<T, 32>::fn), grid, dim3(kB), 0, st, args...); break;
+            default: hipLaunchKernelGGL((K<T, 64>::fn), grid, dim3(kB), 0, st, args...); break;
+        }
+    }
+};
+template <typename T, int G> struct ResidK { static constexpr auto fn = k_csr_resid<T, G>; };
+template <typename T, int G> struct UpK { static constexpr auto fn = k_csr_up<T, G>; };
+template <typename T, int G> struct ApplyK { static constexpr auto fn = k_csr_apply<T, G>; };
+
+template <typename T>
+const T *pick(const float *f, const double *d) {
+    if constexpr (sizeof(T) == 4) return reinterpret_cast<const T *>(f);
+    else return reinterpret_cast<const T *>(d);
+}
+
+// z = M b0: one V-cycle on the device
+template <typename T>
+void vcycle(stg_efield *e, const T *b0, T *z0) {
+    AmgDev &G = e->amg;
+    const int L = (int)G.lev.size();
+    const EfDev m = e->dev();
+    auto blocks = [](int n) { return dim3((n + kB - 1) / kB); };
+    for (int l = 0; l < L; ++l) {
+        AmgLevel &V = G.lev[l];
+        const T *b = l == 0 ? b0 : reinterpret_cast<const T *>(V.b);
+        T *x = reinterpret_cast<T *>(V.x), *r = reinterpret_cast<T *>(V.r);
+        if (l == 0) {
+            hipLaunchKernelGGL(k_amg0_down<T>, blocks(V.n), dim3(kB), 0, e->stream, m, e->sigma, V.om, b, x, r);
+        } else {
+            hipLaunchKernelGGL(k_scale<T>, blocks(V.n), dim3(kB), 0, e->stream, V.n, pick<T>(V.dinv32, V.dinv64),
+                               V.om, b, x);
+            GroupLaunch<T, ResidK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
+                                        pick<T>(V.a32, V.a64), b, (const T *)x, r);
+        }
+        const int nn = l + 1 < L ? G.lev[l + 1].n : G.nc;
+        T *bc = reinterpret_cast<T *>(l + 1 < L ? G.lev[l + 1].b : G.cb);
+        GroupLaunch<T, ApplyK>::run(V.gr, nn, e->stream, nn, (const int *)V.r_ptr, (const int *)V.r_col,
+                                    pick<T>(V.r32, V.r64), (const T *)r, bc, 0);
+    }
+    hipLaunchKernelGGL(k_dense_apply<T>, blocks(G.nc), dim3(kB), 0, e->stream, G.nc, pick<T>(G.inv32, G.inv64),
+                       (const T *)G.cb, reinterpret_cast<T *>(G.cx));
+    for (int l = L - 1; l >= 0; --l) {
+        AmgLevel &V = G.lev[l];
+        const T *xc = reinterpret_cast<const T *>(l + 1 < L ? G.lev[l + 1].out : G.cx);
+        T *x = reinterpret_cast<T *>(V.x);
+        GroupLaunch<T, ApplyK>::run(V.gp, V.n, e->stream, V.n, (const int *)V.p_ptr, (const int *)V.p_col,
+                                    pick<T>(V.p32, V.p64), xc, x, 1);
+        const T *b = l == 0 ? b0 : reinterpret_cast<const T *>(V.b);
+        T *out = l == 0 ? z0 : reinterpret_cast<T *>(V.out);
+        if (l == 0)
+            hipLaunchKernelGGL(k_amg0_up<T>, blocks(V.n), dim3(kB), 0, e->stream, m, e->sigma, V.om, b,
+                               (const T *)x, out);
+        else
+            GroupLaunch<T, UpK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
+                                     pick<T>(V.a32, V.a64), pick<T>(V.dinv32, V.dinv64), V.om, b, (const T *)x,
+                                     out);
+    }
+}
+
+// CG on A d = r (r given in r64) preconditioned by one V-cycle per iteration;
+// result added to x64. Returns iterations.
+template <typename T>
+int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
+    const EfDev m = e->dev();
+    const int nb = e->nb, n = e->n;
+    T *x = (T *)e->xs, *r = (T *)e->rs, *z = (T *)e->zs, *p = (T *)e->ps, *q = (T *)e->qs;
+    hipLaunchKernelGGL(k_cg_start<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->r64, x, r);
+    vcycle<T>(e, r, z);
+    hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kB), 0, e->stream, n, (const T *)r, (const T *)z, e->part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 0, e->scal, e->outs);
+    const double rz0 = fetch(e, e->scal);
+    int it = 0;
+    if (rz0 > 0.0) {
+        const int check = 2;
+        while (it < max_iter) {
+            for (int k = 0; k < check && it < max_iter; ++k, ++it) {
+                T *pold = p + (size_t)(it & 1) * n, *pnew = p + (size_t)((it + 1) & 1) * n;
+                hipLaunchKernelGGL(k_cg_a<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, z, pold, pnew, q,
+                                   e->scal, it == 0 ? 1 : 0, e->part);
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 1, e->scal,
+                                   e->outs);
+                hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->scal, x, r,
+                                   (const T *)pnew, (const T *)q);
+                vcycle<T>(e, r, z);
+                hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kB), 0, e->stream, n, (const T *)r, (const T *)z,
+                                   e->part);
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 2, e->scal,
+                                   e->outs);
+            }
+            const double rz = fetch(e, e->scal);
+            if (!(rz > rel_tol * rel_tol * rz0)) break;
+        }
     }
     EF_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_accumulate<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->x64, x);
@@ -801,6 +1488,16 @@ void advance_impl(stg_efield *e, double dt_s) {
         e->intg_t0 = now;
     }
     const double oodt = 1.0 / (dt_s * 1e3);
+    // The hierarchy only preconditions: CG itself runs on the matrix assembled
+    // for this exact step. It is rebuilt when a membrane or bulk parameter
+    // changed, or when 1/dt moved by more than a quarter (step sizes from
+    // accumulated times differ in their last bits from step to step).
+    if (e->precond == 1 && n > kSmallSystem) {
+        const AmgDev &G = e->amg;
+        if (!G.built || std::fabs(G.key_oodt - oodt) > 0.25 * oodt || G.key_sigma != e->sigma ||
+            G.key_ver != e->params_ver)
+            amg_build(e, oodt);
+    }
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, oodt, e->cap, e->gext,
                        e->vext, e->V, e->I, e->b, e->d64, e->d32);
     if (e->a32_dirty) {
@@ -1062,6 +1759,22 @@ int stg_efield_set_solver(stg_efield *e, int32_t precision, double tol, int32_t 
         e->tol = tol;
         e->max_iter = max_iter;
     });
+}
+
+int stg_efield_set_precond(stg_efield *e, int32_t kind) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (kind != 0 && kind != 1) throw EfError{STG_ERR_ARG, "preconditioner is 0 (Jacobi) or 1 (multigrid)"};
+        e->precond = kind;
+    });
+}
+
+int stg_efield_precond_info(const stg_efield *e, int32_t *levels, int32_t *coarsest, double *setup_s) {
+    if (!e || !levels || !coarsest || !setup_s) return STG_ERR_ARG;
+    *levels = e->amg.usable ? (int32_t)e->amg.lev.size() + 1 : 0;
+    *coarsest = e->amg.usable ? e->amg.nc : 0;
+    *setup_s = e->amg.usable ? e->amg.setup_s : 0.0;
+    return STG_OK;
 }
 
 int stg_efield_stats(const stg_efield *e, int32_t *iters, int32_t *refinements, double *rel_residual) {

@@ -13,6 +13,7 @@ import numpy as np
 from . import _native as N
 
 MIXED, FP64 = 0, 1
+JACOBI, MULTIGRID = 0, 1
 
 
 class EField:
@@ -96,6 +97,16 @@ class EField:
 
     def set_solver(self, precision=MIXED, tol=1e-8, max_iter=20000):
         self._chk(self.lib.stg_efield_set_solver(self.h, int(precision), float(tol), int(max_iter)))
+
+    def set_precond(self, kind=MULTIGRID):
+        """MULTIGRID (default): smoothed-aggregation multigrid V-cycle for
+        systems above 16,384 vertices; JACOBI: diagonal scaling."""
+        self._chk(self.lib.stg_efield_set_precond(self.h, int(kind)))
+
+    def precond_info(self):
+        lv, nc, t = C.c_int32(), C.c_int32(), C.c_double()
+        self._chk(self.lib.stg_efield_precond_info(self.h, C.byref(lv), C.byref(nc), C.byref(t)))
+        return dict(levels=lv.value, coarsest=nc.value, setup_s=t.value)
 
     def stats(self):
         i, r, x = C.c_int32(), C.c_int32(), C.c_double()

@@ -296,3 +296,77 @@ def test_rallpack1_through_the_solver_interface():
     assert rms0 < 1e-3 and rms1 < 1e-3
     assert abs(sim.getTriV(int(memb_tris[0])) - p["E_M"]) < 0.05
     assert sim.getEfieldDT() == p["EF_dt"]
+
+
+def _large_case(n=28, side=14e-6):
+    """29^3 = 24,389 vertices: above the one-workgroup solver, so the
+    multigrid-preconditioned CG runs."""
+    mesh = geom.kuhn_cube(n, side)
+    tris = mesh.tris[mesh.getSurfTris()]
+    return mesh, tris
+
+
+def _drive(mesh, tris, objs, clamp=()):
+    for obj in objs:
+        obj.set_potential(-0.065)
+        obj.set_capac(0.01)
+        obj.set_volres(1.0)
+        obj.set_memb_res(1.0, -0.065)
+    face = np.nonzero(np.all(mesh.verts[tris][:, :, 0] < 1e-12, axis=1))[0]
+    for t in face:
+        for obj in objs:
+            obj.set_tri_iclamp(int(t), 1e-9 / len(face))
+    for v in clamp:
+        objs[0].set_vert_clamped(v)
+        if len(objs) > 1:
+            objs[1].clamped[v] = True
+
+
+@pytest.mark.parametrize("precision,tol", [(MIXED, 1e-6), (FP64, 1e-9)])
+def test_multigrid_steps_match_fp64_oracle(precision, tol):
+    from steps_amd.efield import MULTIGRID
+
+    mesh, tris = _large_case()
+    print(f"case efield multigrid cube n=28 precision={precision}", flush=True)
+    ef = EField(mesh.verts, mesh.tets, tris)
+    ef.set_solver(precision, 1e-11)
+    ef.set_precond(MULTIGRID)
+    ora = EFieldOracle(mesh.verts, mesh.tets, tris)
+    _drive(mesh, tris, [ef, ora], clamp=(7, 4000))
+    v_prev = ora.V.copy() * 1e-3
+    for step in range(3):
+        ef.advance(1e-5)
+        ora.advance(1e-5)
+        gpu = ef.verts_v()
+        ref = ora.V * 1e-3
+        dv_ref = ref - v_prev
+        err = np.abs((gpu - v_prev) - dv_ref).max() / np.abs(dv_ref).max()
+        assert err < tol, (step, err, ef.stats())
+        assert gpu[7] == -0.065 and gpu[4000] == -0.065
+        v_prev = ref
+        ef.set_verts_v(ref)
+    info = ef.precond_info()
+    assert info["levels"] >= 2 and 0 < info["coarsest"] <= 2000, info
+    assert ef.stats()["rel_residual"] <= 1e-9
+
+
+def test_multigrid_cuts_iterations():
+    """The same step with Jacobi and with the multigrid V-cycle: both solve it
+    to the FP64 tolerance; the V-cycle needs far fewer CG iterations."""
+    from steps_amd.efield import JACOBI, MULTIGRID
+
+    mesh, tris = _large_case()
+    print("case efield multigrid vs jacobi iterations", flush=True)
+    out = {}
+    for kind in (JACOBI, MULTIGRID):
+        ef = EField(mesh.verts, mesh.tets, tris)
+        ef.set_precond(kind)
+        _drive(mesh, tris, [ef])
+        ef.advance(1e-5)
+        out[kind] = (ef.verts_v(), ef.stats()["iterations"], ef.precond_info())
+    v_j, it_j, info_j = out[JACOBI]
+    v_m, it_m, info_m = out[MULTIGRID]
+    dv = np.abs(v_j + 0.065).max()
+    assert np.abs(v_m - v_j).max() <= 1e-6 * dv
+    assert info_j["levels"] == 0 and info_m["levels"] >= 2
+    assert it_m * 4 <= it_j, (it_m, it_j)
