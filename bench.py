@@ -268,7 +268,8 @@ def main():
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    w = W.four_species_cube(n=40, side=40e-6, seed=0)
+    # mesh topology built on the device (stg_mesh_build), as a large-mesh user would
+    w = W.four_species_cube(n=40, side=40e-6, seed=0, device=True)
     btet = model_bytes(w)
     p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph)
     value = p["tet_steps"] / p["wall"]
@@ -335,7 +336,7 @@ def main():
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])
     if args.membrane:
-        wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0)
+        wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0, device=True)
         out["membrane"] = time_membrane(wm)
         out["efield"] = time_efield(wm)
     if not args.no_cpu:

@@ -333,6 +333,31 @@ int stg_efield_stats(const stg_efield *ef, int32_t *iters, int32_t *refinements,
                      double *rel_residual);
 int stg_efield_sizes(const stg_efield *ef, int32_t *n_verts, int32_t *n_tris, int32_t *nnz);
 
+/* ------------------------------------------------------------- mesh ingest
+ * Face topology and measures of a tetrahedral mesh built on the device, the
+ * work of the Tetmesh constructor (src/steps/geom/tetmesh.cpp:78-233) that
+ * the mesh loaders feed (pysteps/steps/API_2/geom.py:1466-1592). Triangles are the
+ * distinct faces, numbered by their sorted vertex triple; tri_tets lists the
+ * first tet having the face and the neighbour across it (-1 on the boundary).
+ * Coordinates in m; outputs in SI. Errors: STG_ERR_ARG for an index out of
+ * range, a degenerate tet or a triangle shared by more than two tets. */
+typedef struct stg_meshbuild stg_meshbuild;
+
+int stg_mesh_build(const double *verts, int32_t n_verts, const int32_t *tets, int32_t n_tets,
+                   stg_meshbuild **out);
+int stg_mesh_build_sizes(const stg_meshbuild *mb, int32_t *n_tets, int32_t *n_tris);
+/* Any output pointer may be NULL. Sizes: tet_vol [n_tets], tet_bary
+ * [n_tets*3], tet_tri / tet_nbr / tet_nbr_face / tet_face_area /
+ * tet_face_dist [n_tets*4] (face f is opposite vertex f), tris [n_tris*3],
+ * tri_tets [n_tris*2], tri_area [n_tris], tri_bary [n_tris*3]. */
+int stg_mesh_build_fetch(stg_meshbuild *mb, double *tet_vol, double *tet_bary, int32_t *tet_tri,
+                         int32_t *tet_nbr, int32_t *tet_nbr_face, double *tet_face_area,
+                         double *tet_face_dist, int32_t *tris, int32_t *tri_tets, double *tri_area,
+                         double *tri_bary);
+int stg_mesh_build_destroy(stg_meshbuild *mb);
+/* Message of the last failed stg_mesh_build* call on this thread. */
+const char *stg_mesh_build_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

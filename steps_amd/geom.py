@@ -37,10 +37,56 @@ def _face_keys(faces: np.ndarray, nverts: int) -> np.ndarray:
 ELEM_VERTEX, ELEM_TRI, ELEM_TET, ELEM_UNDEFINED = 0, 1, 2, 99   # src/steps/geom/tetmesh.hpp:46
 
 
-class Tetmesh:
-    """Unstructured tetrahedral mesh with its face topology and measures (SI units)."""
+def _device_topology(verts: np.ndarray, tets: np.ndarray) -> dict:
+    """The face topology and measures of Tetmesh, built by the engine library on
+    the GPU (stg_mesh_build): same arrays, same triangle numbering."""
+    import ctypes as C
 
-    def __init__(self, verts, tets):
+    from . import _native as N
+
+    lib = N.load()
+    v = np.ascontiguousarray(verts, dtype=np.float64)
+    t = np.ascontiguousarray(tets, dtype=np.int32)
+    h = C.c_void_p()
+    rc = lib.stg_mesh_build(N.ptr(v, C.c_double), len(v), N.ptr(t, C.c_int32), len(t), C.byref(h))
+    if rc != N.STG_OK:
+        msg = lib.stg_mesh_build_last_error()
+        if rc == N.STG_ERR_ARG:
+            raise ValueError(msg.decode() if msg else "bad mesh")
+        raise N.StgError(rc, msg.decode() if msg else "mesh build failed")
+    try:
+        nt, ntri = C.c_int32(), C.c_int32()
+        N.check(lib.stg_mesh_build_sizes(h, C.byref(nt), C.byref(ntri)))
+        nt, ntri = nt.value, ntri.value
+        out = dict(tet_vol=np.empty(nt), tet_bary=np.empty((nt, 3)), tet_tri=np.empty((nt, 4), np.int32),
+                   tet_nbr=np.empty((nt, 4), np.int32), tet_nbr_face=np.empty((nt, 4), np.int32),
+                   tet_face_area=np.empty((nt, 4)), tet_face_dist=np.empty((nt, 4)),
+                   tris=np.empty((ntri, 3), np.int32), tri_tets=np.empty((ntri, 2), np.int32),
+                   tri_area=np.empty(ntri), tri_bary=np.empty((ntri, 3)))
+        cd, ci = C.c_double, C.c_int32
+        rc = lib.stg_mesh_build_fetch(
+            h, N.ptr(out["tet_vol"], cd), N.ptr(out["tet_bary"], cd), N.ptr(out["tet_tri"], ci),
+            N.ptr(out["tet_nbr"], ci), N.ptr(out["tet_nbr_face"], ci), N.ptr(out["tet_face_area"], cd),
+            N.ptr(out["tet_face_dist"], cd), N.ptr(out["tris"], ci), N.ptr(out["tri_tets"], ci),
+            N.ptr(out["tri_area"], cd), N.ptr(out["tri_bary"], cd))
+        if rc != N.STG_OK:
+            msg = lib.stg_mesh_build_last_error()
+            raise N.StgError(rc, msg.decode() if msg else "mesh fetch failed")
+    finally:
+        lib.stg_mesh_build_destroy(h)
+    for k in ("tet_tri", "tet_nbr", "tet_nbr_face", "tris", "tri_tets"):
+        out[k] = out[k].astype(np.int64)
+    return out
+
+
+class Tetmesh:
+    """Unstructured tetrahedral mesh with its face topology and measures (SI units).
+
+    device=True builds the topology and measures on the GPU (the engine
+    library's stg_mesh_build) instead of with numpy on the host: the same
+    arrays, for meshes of millions of tets."""
+
+    def __init__(self, verts, tets, device: bool = False):
         self.verts = np.ascontiguousarray(np.asarray(verts, dtype=np.float64))
         self.tets = np.ascontiguousarray(np.asarray(tets, dtype=np.int64))
         if self.verts.ndim != 2 or self.verts.shape[1] != 3:
@@ -53,6 +99,15 @@ class Tetmesh:
             raise ValueError("mesh has no tetrahedra")
         if self.tets.min() < 0 or self.tets.max() >= nv:
             raise ValueError("tet vertex index out of range")
+        self.comps: dict[str, TmComp] = {}
+        self.patches: dict[str, TmPatch] = {}
+        self.membs: dict[str, Memb] = {}
+        self.diffbs: dict[str, DiffBoundary] = {}
+        self.rois: dict[str, tuple[int, np.ndarray]] = {}
+        if device:
+            for k, v in _device_topology(self.verts, self.tets).items():
+                setattr(self, k, v)
+            return
         p = self.verts[self.tets]  # (nt, 4, 3)
         e1, e2, e3 = p[:, 1] - p[:, 0], p[:, 2] - p[:, 0], p[:, 3] - p[:, 0]
         self.tet_vol = np.abs(np.einsum("ij,ij->i", e1, np.cross(e2, e3))) / 6.0
@@ -95,11 +150,6 @@ class Tetmesh:
         ti = np.nonzero(has)
         d[ti] = np.linalg.norm(self.tet_bary[ti[0]] - self.tet_bary[self.tet_nbr[ti]], axis=1)
         self.tet_face_dist = d
-        self.comps: dict[str, TmComp] = {}
-        self.patches: dict[str, TmPatch] = {}
-        self.membs: dict[str, Memb] = {}
-        self.diffbs: dict[str, DiffBoundary] = {}
-        self.rois: dict[str, tuple[int, np.ndarray]] = {}
 
     # --- regions of interest (Tetmesh::addROI, src/steps/geom/tetmesh.cpp) ---
     def addROI(self, id: str, type: int, indices) -> None:
@@ -290,11 +340,12 @@ class Memb:
         return len(np.unique(self.mesh.tets[self.tets]))
 
 
-def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0)) -> Tetmesh:
+def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0), device: bool = False) -> Tetmesh:
     """Cube of edge ``side`` (m) cut into n^3 sub-cubes, each split into the six
     Kuhn (Freudenthal) tetrahedra sharing the main diagonal. The split is
     conforming across sub-cubes, every tet has volume (side/n)^3/6, and the six
-    tets of a sub-cube are consecutive (cube-major, x fastest)."""
+    tets of a sub-cube are consecutive (cube-major, x fastest). device=True
+    builds the topology on the GPU."""
     if n < 1:
         raise ValueError("n must be >= 1")
     h = side / n
@@ -316,7 +367,7 @@ def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0)) -> Tetmesh:
         for s, ax in enumerate(perm):
             cur[ax] = cur[ax] + 1
             tets[:, pi, s + 1] = vid(*cur)
-    return Tetmesh(verts, tets.reshape(-1, 4))
+    return Tetmesh(verts, tets.reshape(-1, 4), device=device)
 
 
 def load_gmsh(path, scale: float = 1.0):

@@ -36,8 +36,8 @@ class Workload:
         return len(self.mt.vol)
 
 
-def _cube(n, side, with_patch):
-    mesh = geom.kuhn_cube(n, side)
+def _cube(n, side, with_patch, device=False):
+    mesh = geom.kuhn_cube(n, side, device=device)
     comp = geom.TmComp("cyto", mesh)
     comp.addVolsys("vsys")
     tris = mesh.getSurfTris() if with_patch else None
@@ -78,14 +78,14 @@ def ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=0, end_time=0.
                     dict(n=n, side=side, n_a=n_a, n_b=n_b, seed=seed, kf=kf, kb=kb), mesh)
 
 
-def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> Workload:
+def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50, device=False) -> Workload:
     """A + B <-> C, C + B <-> D (kf 1e6 /M/s, kb 1 /s), 0 -> A at 10 /s per tet,
     D -> 0 at 0.5 /s; D log-uniform in [1e-13, 1e-11] m^2/s per species;
     initial counts Poisson(mean_count) per (tet, species)."""
     rng = np.random.default_rng(seed)
     dcst = 10.0 ** rng.uniform(-13, -11, size=4)
     dcst[int(np.argmax(dcst))] = 1e-11  # the fastest species sets dt at the stability bound
-    mesh, comp, _ = _cube(n, side, False)
+    mesh, comp, _ = _cube(n, side, False, device)
     vol_t = float(mesh.tet_vol[0])
     mdl = M.Model()
     A, B, Cs, D = (M.Spec(x, mdl) for x in "ABCD")
@@ -107,7 +107,7 @@ def four_species_cube(n=40, side=40e-6, seed=0, end_time=1.0, mean_count=50) -> 
 
 def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0,
                           ca_um=0.1, pump_density=10.0, influx=1e5,
-                          vdep_influx=False, ghk_influx=False, pump_dcst=0.0) -> Workload:
+                          vdep_influx=False, ghk_influx=False, pump_dcst=0.0, device=False) -> Workload:
     """Ca + Buf <-> CaB (kf 1e8 /M/s, kb 10 /s), D_Ca 2e-10 m^2/s (Buf, CaB
     immobile); every boundary triangle is membrane. Pumps P (pump_density per
     um^2) run P + Ca_in -> P + Ca_out with k = kcat/Km = 50 /s / 1 uM; Ca_out
@@ -119,7 +119,7 @@ def calcium_membrane_cube(n=100, side=20e-6, seed=0, end_time=0.05, buf_um=100.0
     2, virtual outer concentration 2 mM, 20 pS slope conductance at -65 mV).
     pump_dcst > 0 lets the pumps diffuse on the membrane (surface diffusion)."""
     rng = np.random.default_rng(seed)
-    mesh, comp, tris = _cube(n, side, True)
+    mesh, comp, tris = _cube(n, side, True, device)
     mdl = M.Model()
     Ca, Buf, CaB = M.Spec("Ca", mdl, valence=2), M.Spec("Buf", mdl), M.Spec("CaB", mdl)
     P, CaOut, Chan = M.Spec("Pump", mdl), M.Spec("CaOut", mdl), M.Spec("Chan", mdl)
