@@ -74,19 +74,21 @@ def measured_traffic():
     return None
 
 
-def load_engine(w, seed, block=256, graph=False):
+def load_engine(w, seed, block=256, graph=False, specialize=False):
     eng = Engine(w.fm, [w.mt], [seed])
     for s in range(len(w.fm.specs)):
         eng.set_tet_counts(s, w.counts[s])
     for s in range(len(w.fm.sspecs)):
         eng.set_tri_counts(s, w.tri_counts[s])
     eng.set_launch(block, graph)
+    if specialize:
+        eng.set_specialize(True)
     return eng
 
 
-def time_plain(w, steps, warmup, seed=1, block=0, graph=False):
-    """block = 0: launch shape chosen by the autotuner on this state."""
-    eng = load_engine(w, seed, block or 256, graph)
+def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False):
+    """block = 0: launch shape and kernel chosen by the autotuner on this state."""
+    eng = load_engine(w, seed, block or 256, graph, specialize and bool(block))
     tuned = autotune(eng, trial_periods=20) if not block else None
     dt = eng.update_period()
     if warmup:
@@ -99,7 +101,8 @@ def time_plain(w, steps, warmup, seed=1, block=0, graph=False):
     ms_launch, launches = eng.last_run_kernel_ms()
     return dict(eng=eng, wall=wall, periods=periods, tet_steps=periods * w.n_tets,
                 ms_launch=ms_launch, launches=launches, dt=dt, tuned=tuned,
-                block=tuned["block"] if tuned else block, graph=tuned["graph"] if tuned else graph)
+                block=tuned["block"] if tuned else block, graph=tuned["graph"] if tuned else graph,
+                specialized=eng.specialized)
 
 
 def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True):
@@ -244,6 +247,8 @@ def main():
     ap.add_argument("--block", type=int, default=0,
                     help="threads per block of the period kernel; 0 = chosen by the autotuner")
     ap.add_argument("--graph", action="store_true", help="with --block: replay launches as a graph")
+    ap.add_argument("--specialize", action="store_true",
+                    help="with --block: use the model-specialised kernel (compiled at run time)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -271,7 +276,7 @@ def main():
     # mesh topology built on the device (stg_mesh_build), as a large-mesh user would
     w = W.four_species_cube(n=40, side=40e-6, seed=0, device=True)
     btet = model_bytes(w)
-    p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph)
+    p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph, specialize=args.specialize)
     value = p["tet_steps"] / p["wall"]
     # roofline of the period kernel: algorithmic bytes per launch / mean launch time
     achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
@@ -296,6 +301,7 @@ def main():
             "simulated_s": p["periods"] * p["dt"],
             "block": p["block"],
             "graph": bool(p["graph"]),
+            "specialized_kernel": bool(p["specialized"]),
             "schedule": "autotuned" if p["tuned"] else "fixed",
             "autotune_wall_ms_per_launch": p["tuned"]["timings"] if p["tuned"] else None,
         },
@@ -330,6 +336,7 @@ def main():
             "plain_same_model_tet_steps_per_s": ps["tet_steps"] / ps["wall"],
             "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
             "schedule": {"block": sched.get("block"), "graph": sched.get("graph"),
+                         "specialized_kernel": sched.get("specialized"),
                          "autotune_wall_ms_per_launch": sched.get("timings")},
             "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"])},
         }

@@ -218,6 +218,15 @@ int stg_set_sdcst(stg_sim *sim, int32_t real, int32_t sspec, double dcst);
  * No reference counterpart (device tuning). */
 int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
 
+/* Model-specialised kernel (no reference counterpart; device tuning): on = 1
+ * compiles, at run time with hipRTC (about a second), a period kernel with
+ * this model's reaction network as compile-time constants and uses it for
+ * launches of up to 256 threads per block; results are identical to the
+ * generic kernel's. Volume-only models of one compartment (STG_ERR_NOTIMPL
+ * otherwise). on = 0 returns to the generic kernel. */
+int stg_set_specialize(stg_sim *sim, int32_t on);
+int stg_get_specialize(const stg_sim *sim, int32_t *on);
+
 /* TetOpSplitP::setDiffApplyThreshold (tetopsplit.hpp:995,
  * pysteps/cysteps_mpi.pyx:1288; DiffusionOperator default 10,
  * diffusion_operator.hpp:99): leaving counts up to the threshold are split

@@ -1,11 +1,12 @@
 """Launch-shape autotuner for the period kernel.
 
 Times a few periods of the simulation's own state under each candidate launch
-shape (threads per block, graph replay of a run's launch sequence) and keeps
-the fastest by wall time, which includes the launch overhead that graph replay
-removes. Every random draw is keyed by (seed, tet, period), so the launch shape
-changes speed only, never results; the state is checkpointed before the trials
-and restored after them.
+shape (threads per block, graph replay of a run's launch sequence, the generic
+or the model-specialised kernel) and keeps the fastest by wall time, which
+includes the launch overhead that graph replay removes. Every random draw is
+keyed by (seed, tet, period) and the specialised kernel performs the generic
+kernel's operations, so the shape changes speed only, never results; the state
+is checkpointed before the trials and restored after them.
 """
 from __future__ import annotations
 
@@ -17,34 +18,51 @@ from .engine import Engine
 
 
 def autotune(engine: Engine, trial_periods: int = 6, blocks=(64, 128, 256, 512),
-             graphs=(False, True), repeats: int = 2) -> dict:
+             graphs=(False, True), repeats: int = 2, specialize=(False, True)) -> dict:
     """Each shape runs trial_periods periods once to warm up and then `repeats`
-    times timed; its figure is the fastest of those."""
+    times timed; its figure is the fastest of those. The specialised kernel is
+    tried for blocks of up to 256 when the model allows it."""
+    from ._native import StgError
+
+    spec_opts = [False]
+    if True in specialize:
+        try:
+            engine.set_specialize(True)
+            spec_opts.append(True)
+        except StgError:
+            pass
+        engine.set_specialize(False)
+    if False not in specialize and True in spec_opts:
+        spec_opts = [True]
     state = engine.checkpoint()
     t0 = engine.time()
     dts = [engine.update_period(r) for r in range(engine.n_real)]
     finite = [d for d in dts if np.isfinite(d)]
     span = trial_periods * (max(finite) if finite else 1e-3)
     timings = {}
-    for b in blocks:
-        for g in graphs:
+    cands = [(b, g, sp) for sp in spec_opts for b in blocks for g in graphs if not (sp and b > 256)]
+    for b, g, sp in cands:
+        if sp != engine.specialized:
+            engine.set_specialize(sp)
+        engine.restore(state)
+        engine.set_launch(b, g)
+        engine.run(t0 + span)            # warm-up (captures the graph)
+        best_t = None
+        for _ in range(max(1, repeats)):
             engine.restore(state)
-            engine.set_launch(b, g)
-            engine.run(t0 + span)            # warm-up (captures the graph)
-            best_t = None
-            for _ in range(max(1, repeats)):
-                engine.restore(state)
-                tic = time.perf_counter()
-                engine.run(t0 + span)        # returns once the stream has drained
-                wall = time.perf_counter() - tic
-                ms, n = engine.last_run_kernel_ms()
-                cand = (wall * 1e3 / max(n, 1), ms)
-                if best_t is None or cand[0] < best_t[0]:
-                    best_t = cand
-            timings[(b, g)] = best_t
+            tic = time.perf_counter()
+            engine.run(t0 + span)        # returns once the stream has drained
+            wall = time.perf_counter() - tic
+            ms, n = engine.last_run_kernel_ms()
+            cand = (wall * 1e3 / max(n, 1), ms)
+            if best_t is None or cand[0] < best_t[0]:
+                best_t = cand
+        timings[(b, g, sp)] = best_t
     best = min(timings, key=lambda k: timings[k][0])
     engine.restore(state)
-    engine.set_launch(*best)
-    return {"block": best[0], "graph": best[1], "wall_ms_per_launch": timings[best][0],
-            "kernel_ms_per_launch": timings[best][1],
-            "timings": {f"{b}/{'graph' if g else 'stream'}": round(v[0], 5) for (b, g), v in timings.items()}}
+    engine.set_specialize(best[2])
+    engine.set_launch(best[0], best[1])
+    return {"block": best[0], "graph": best[1], "specialized": best[2],
+            "wall_ms_per_launch": timings[best][0], "kernel_ms_per_launch": timings[best][1],
+            "timings": {f"{b}/{'graph' if g else 'stream'}{'/spec' if sp else ''}": round(v[0], 5)
+                        for (b, g, sp), v in timings.items()}}

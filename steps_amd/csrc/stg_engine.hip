@@ -2,17 +2,20 @@
 // one simulation (one or many realizations), builds the period schedule and
 // launches the fused period kernel on the handle's HIP stream.
 #include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <sstream>
 #include <string>
 #include <vector>
 
 #include "../../include/stg.h"
 #include "stg_kernels.hpp"
+#include "stg_rtc_sources.inc"
 
 namespace {
 
@@ -152,6 +155,10 @@ struct stg_sim {
     double *d_gather = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // model-specialised volume kernel (blocks of up to 256), compiled at run time
+    hipModule_t rtc_mod = nullptr;
+    hipFunction_t rtc_fn = nullptr;
+    bool use_rtc = false;
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_launches = -1;
@@ -223,6 +230,7 @@ struct stg_sim {
 
     ~stg_sim() {
         free_graph();
+        if (rtc_mod) (void)hipModuleUnload(rtc_mod);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
                         d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
@@ -793,14 +801,148 @@ PeriodKernel period_kernel(const stg_sim *s, int B) {
     return memb ? stg::k_period<1024, true> : stg::k_period<1024, false>;
 }
 
+// ------------------------------------------------------------- specialisation
+//
+// The volume kernel compiled at run time with this model's reaction network
+// as compile-time constants (stg::DynTables describes the fields): the loops
+// over reactions, reactants and diffusing species unroll and the table reads
+// disappear. Same operations in the same order, so results are identical.
+
+std::string rtc_join(const char *const *parts, size_t n) {
+    std::string out;
+    for (size_t i = 0; i < n; ++i) out += parts[i];
+    return out;
+}
+
+// The model's tables as a struct of constants
+std::string spec_tables_source(const stg_sim *s) {
+    const stg::ModelK &k = s->mk;
+    std::ostringstream o;
+    auto arr = [&o](const char *name, const std::vector<int> &v) {
+        o << "    static constexpr int " << name << "[" << std::max<size_t>(v.size(), 1) << "] = {";
+        if (v.empty()) o << "0";
+        for (size_t i = 0; i < v.size(); ++i) o << (i ? ", " : "") << v[i];
+        o << "};\n";
+    };
+    const int R = k.n_reacs;
+    std::vector<int> order, lhs_off, lhs_sp, lhs_mult, upd_off, upd_sp, upd_dv, dspec;
+    for (int q = 0; q < R; ++q) order.push_back(k.order[q]);
+    for (int q = 0; q <= R; ++q) {
+        lhs_off.push_back(k.lhs_off[q] - k.lhs_off[0]);
+        upd_off.push_back(k.upd_off[q] - k.upd_off[0]);
+    }
+    for (int e = k.lhs_off[0]; e < k.lhs_off[R]; ++e) {
+        lhs_sp.push_back(k.ent[e] & 0xffff);
+        lhs_mult.push_back(k.ent[e] >> 16);
+    }
+    for (int e = k.upd_off[0]; e < k.upd_off[R]; ++e) {
+        upd_sp.push_back(k.ent[e] & 0xffff);
+        upd_dv.push_back(k.ent[e] >> 16);
+    }
+    for (int d = 0; d < k.n_diff; ++d) dspec.push_back(k.dspec[d]);
+    o << "struct StgModelTables {\n    static constexpr bool kStatic = true;\n";
+    o << "    static constexpr int S = " << k.n_specs << ", R = " << R << ", n_diff = " << k.n_diff
+      << ", tracked_mask = " << k.tracked_mask << ";\n";
+    arr("order", order);
+    arr("lhs_off", lhs_off);
+    arr("lhs_sp", lhs_sp);
+    arr("lhs_mult", lhs_mult);
+    arr("upd_off", upd_off);
+    arr("upd_sp", upd_sp);
+    arr("upd_dv", upd_dv);
+    arr("dspec", dspec);
+    o << "};\n";
+    return o.str();
+}
+
+void specialize(stg_sim *s) {
+    if (s->needs_memb())
+        throw StgError{STG_ERR_NOTIMPL,
+                       "model-specialised kernels cover volume-only models of one compartment"};
+    if (s->rtc_fn) {
+        if (!s->use_rtc) s->free_graph();
+        s->use_rtc = true;
+        return;
+    }
+    const std::string kern = rtc_join(kRtcKernels, sizeof(kRtcKernels) / sizeof(kRtcKernels[0]));
+    const std::string rnd = rtc_join(kRtcRandom, sizeof(kRtcRandom) / sizeof(kRtcRandom[0]));
+    const std::string lfact = rtc_join(kRtcLfact, sizeof(kRtcLfact) / sizeof(kRtcLfact[0]));
+    const std::string name = "stg::k_period<256, false, StgModelTables>";
+    std::string src =
+        "using __hip_internal::int8_t; using __hip_internal::uint8_t;\n"
+        "using __hip_internal::int32_t; using __hip_internal::uint32_t;\n"
+        "using __hip_internal::int64_t; using __hip_internal::uint64_t;\n"
+        "#include \"stg_kernels.hpp\"\n" +
+        spec_tables_source(s) + "template __global__ void " + name +
+        "(const stg::ModelK, const stg::SysK, int);\n";
+    const char *hdrs[] = {kern.c_str(), rnd.c_str(), lfact.c_str()};
+    const char *names[] = {"stg_kernels.hpp", "stg_random.hpp", "stg_lfact_table.inc"};
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, dev));
+    const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+    const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
+                          "-Xclang", "-packed-fp32-ops"};
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "stg_period_spec.hip", 3, hdrs, names) != HIPRTC_SUCCESS)
+        throw StgError{STG_ERR_DEVICE, "hiprtcCreateProgram failed"};
+    std::vector<char> code;
+    std::string lowered, log;
+    hiprtcAddNameExpression(prog, name.c_str());
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+    size_t ls = 0;
+    if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+        log.resize(ls);
+        hiprtcGetProgramLog(prog, &log[0]);
+    }
+    if (rc == HIPRTC_SUCCESS) {
+        const char *low = nullptr;
+        size_t cs = 0;
+        if (hiprtcGetLoweredName(prog, name.c_str(), &low) == HIPRTC_SUCCESS && low) lowered = low;
+        if (hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0) {
+            code.resize(cs);
+            hiprtcGetCode(prog, code.data());
+        }
+    }
+    hiprtcDestroyProgram(&prog);
+    if (rc != HIPRTC_SUCCESS || code.empty() || lowered.empty())
+        throw StgError{STG_ERR_DEVICE, "run-time compilation of the specialised kernel failed: " +
+                                           std::string(hiprtcGetErrorString(rc)) + "\n" + log.substr(0, 2000)};
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    HIP_OK(hipModuleLoadData(&mod, code.data()));
+    if (hipModuleGetFunction(&fn, mod, lowered.c_str()) != hipSuccess) {
+        (void)hipModuleUnload(mod);
+        throw StgError{STG_ERR_DEVICE, "specialised kernel not found in its module"};
+    }
+    s->rtc_mod = mod;
+    s->rtc_fn = fn;
+    s->use_rtc = true;
+    s->free_graph();
+}
+
+// back to the generic kernel; the compiled module stays loaded for a later switch
+void unspecialize(stg_sim *s) {
+    if (s->use_rtc) s->free_graph();
+    s->use_rtc = false;
+}
+
 void launch_all(stg_sim *s, int L) {
     const int B = s->block;
     const int nb = (int)s->block_real.size();
     const size_t smem = s->smem_bytes(B);
     const PeriodKernel kern = period_kernel(s, B);
+    const bool spec = s->use_rtc && s->rtc_fn != nullptr && B <= 256 && !needs_memb(s);
     for (int l = 0; l <= L; ++l) {
-        const stg::SysK k = make_sys(s, l & 1);
-        hipLaunchKernelGGL(kern, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
+        stg::SysK k = make_sys(s, l & 1);
+        if (spec) {
+            int launch = l;
+            void *args[] = {(void *)&s->mk, (void *)&k, (void *)&launch};
+            HIP_OK(hipModuleLaunchKernel(s->rtc_fn, nb, 1, 1, B, 1, 1, (unsigned)smem, s->stream, args, nullptr));
+        } else {
+            hipLaunchKernelGGL(kern, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
+        }
     }
     HIP_OK(hipGetLastError());
 }
@@ -1254,6 +1396,20 @@ int stg_set_sdcst(stg_sim *s, int32_t real, int32_t sspec, double dcst) {
         upload_real_tables(s);
         HIP_OK(hipStreamSynchronize(s->stream));
     });
+}
+
+int stg_set_specialize(stg_sim *s, int32_t on) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (on) specialize(s);
+        else unspecialize(s);
+    });
+}
+
+int stg_get_specialize(const stg_sim *s, int32_t *on) {
+    if (!s || !on) return STG_ERR_ARG;
+    *on = s->use_rtc && s->rtc_fn != nullptr ? 1 : 0;
+    return STG_OK;
 }
 
 int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {

@@ -19,8 +19,10 @@
 //   diffusion   operator/diffusion_operator.cpp:41-238, kproc/diffusions.hpp:211-224
 #pragma once
 
+#ifndef __HIPCC_RTC__  // a runtime-compiled build gets these from hipRTC
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "stg_random.hpp"
 
@@ -65,6 +67,45 @@ struct ModelK {
     double g_perm[kMaxGhk], g_oconc[kMaxGhk], g_vshift[kMaxGhk];
     double temp;  // K
 };
+
+// Where the volume kernel reads the reaction network from. DynTables: the
+// ModelK tables at run time (the library's own build). A model-specialised
+// build (compiled at run time for one model) supplies a struct with
+// kStatic = true and the same tables as compile-time constants:
+//   S, R, order[R], lhs_off[R+1], lhs_sp[], lhs_mult[], upd_off[R+1],
+//   upd_sp[], upd_dv[], tracked_mask, n_diff, dspec[n_diff]
+// so the loops over reactions, reactants and species unroll and the
+// constants fold; every operation and its order stay the same, so both
+// builds produce identical results.
+struct DynTables {
+    static constexpr bool kStatic = false;
+};
+
+template <class MT>
+__device__ __forceinline__ int tab_S(const ModelK &m) {
+    if constexpr (MT::kStatic) return MT::S;
+    else return m.n_specs;
+}
+template <class MT>
+__device__ __forceinline__ int tab_R(const ModelK &m) {
+    if constexpr (MT::kStatic) return MT::R;
+    else return m.n_reacs;
+}
+template <class MT>
+__device__ __forceinline__ int tab_ndiff(const ModelK &m) {
+    if constexpr (MT::kStatic) return MT::n_diff;
+    else return m.n_diff;
+}
+template <class MT>
+__device__ __forceinline__ int tab_dspec(const ModelK &m, int d) {
+    if constexpr (MT::kStatic) return MT::dspec[d];
+    else return m.dspec[d];
+}
+template <class MT>
+__device__ __forceinline__ int tab_tracked(const ModelK &m) {
+    if constexpr (MT::kStatic) return MT::tracked_mask;
+    else return m.tracked_mask;
+}
 
 // Mesh, state and per-realization tables (device pointers).
 struct SysK {
@@ -223,10 +264,11 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 // the per-period event cap is hit.
 // MEMB = false compiles a variant without triangles (no surface reactions,
 // GHK currents, k(V) tables or channel integrals) for volume-only models.
-template <bool MEMB>
+template <bool MEMB, class MT>
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
                           uint32_t ltet, uint32_t period, double dt) {
-    const int R = m.n_reacs, RS = m.n_sreacs, NG = m.n_ghk;
+    static_assert(!(MT::kStatic && MEMB), "model-specialised tables cover the volume kernel only");
+    const int R = tab_R<MT>(m), RS = m.n_sreacs, NG = m.n_ghk;
     const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
     if (MEMB && sys.tri_ptr != nullptr && RT > 0) {
@@ -242,7 +284,13 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
     // rate constants scaled by volume / area, formed once per period
     {
         int j = 0;
-        for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale, ivscale);
+        if constexpr (MT::kStatic) {
+#pragma unroll
+            for (int q = 0; q < MT::R; ++q) L.c(q, 0) = scaled_const(kr[q], MT::order[q], vscale, ivscale);
+            j = MT::R;
+        } else {
+            for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale, ivscale);
+        }
         for (int i = 0; i < ntri; ++i) {
             const double ascale = sys.tri_area[tp0 + i] * kAvogadro;
             const double iascale = 1.0 / ascale;
@@ -268,22 +316,42 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
     while (true) {
         double a0 = 0.0;
         int j = 0;
-        for (int q = 0; q < R; ++q, ++j) {
-            double h = 1.0;
-            bool zero = false;
-            for (int e = m.lhs_off[q]; e < m.lhs_off[q + 1]; ++e) {
-                const int sp = m.ent[e] & 0xffff;
-                const int mult = m.ent[e] >> 16;
-                const int32_t n = L.c(sp);
-                if (mult > n) {
-                    zero = true;
-                    break;
+        if constexpr (MT::kStatic) {
+            // the same products in the same order; a short reactant zeroes the
+            // propensity whatever the later factors are
+#pragma unroll
+            for (int q = 0; q < MT::R; ++q) {
+                double h = 1.0;
+                bool zero = false;
+#pragma unroll
+                for (int e = MT::lhs_off[q]; e < MT::lhs_off[q + 1]; ++e) {
+                    const int32_t n = L.c(MT::lhs_sp[e]);
+                    if (MT::lhs_mult[e] > n) zero = true;
+                    else h = falling(h, n, MT::lhs_mult[e]);
                 }
-                h = falling(h, n, mult);
+                const double av = zero ? 0.0 : h * L.c(q, 0);
+                L.p(q) = av;
+                a0 = a0 + av;
             }
-            const double av = zero ? 0.0 : h * L.c(j, 0);
-            L.p(j) = av;
-            a0 = a0 + av;
+            j = MT::R;
+        } else {
+            for (int q = 0; q < R; ++q, ++j) {
+                double h = 1.0;
+                bool zero = false;
+                for (int e = m.lhs_off[q]; e < m.lhs_off[q + 1]; ++e) {
+                    const int sp = m.ent[e] & 0xffff;
+                    const int mult = m.ent[e] >> 16;
+                    const int32_t n = L.c(sp);
+                    if (mult > n) {
+                        zero = true;
+                        break;
+                    }
+                    h = falling(h, n, mult);
+                }
+                const double av = zero ? 0.0 : h * L.c(j, 0);
+                L.p(j) = av;
+                a0 = a0 + av;
+            }
         }
         for (int i = 0; i < ntri; ++i) {
             for (int q = 0; q < RS; ++q, ++j) {
@@ -321,7 +389,19 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             if (pq > 0.0) last = q;
         }
         if (sel < 0) sel = last;
-        if (!MEMB || sel < R) {
+        if constexpr (MT::kStatic) {
+            // the selected reaction's entries, in table order
+#pragma unroll
+            for (int q = 0; q < MT::R; ++q) {
+                if (sel != q) continue;
+#pragma unroll
+                for (int e = MT::upd_off[q]; e < MT::upd_off[q + 1]; ++e) {
+                    const int sp = MT::upd_sp[e], dv = MT::upd_dv[e];
+                    L.c(sp) += dv;
+                    if ((MT::tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                }
+            }
+        } else if (!MEMB || sel < R) {
             for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
                 const int sp = m.ent[e] & 0xffff;
                 const int dv = m.ent[e] >> 16;
@@ -526,7 +606,7 @@ __device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int
 #define STG_VOL_WAVES_PER_EU 1
 #endif
 
-template <int MAXB, bool MEMB>
+template <int MAXB, bool MEMB, class MT = DynTables>
 __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL_WAVES_PER_EU)
     k_period(const ModelK m, const SysK sys, int launch) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -549,7 +629,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     const bool do_step = launch < np;
     if (!do_enter && !do_step) return;  // uniform over the block
 
-    const int S = m.n_specs;
+    const int S = tab_S<MT>(m);
     Lds L;
     L.B = B;
     L.tid = tid;
@@ -580,7 +660,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
 #pragma unroll
             for (int f = 0; f < 4; ++f)
                 off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
-            for (int d = 0; d < m.n_diff; ++d) {
+            for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
                 const int32_t *op = sys.out_prev + (size_t)d * NT * 4;
                 int32_t v[4];
 #pragma unroll
@@ -588,7 +668,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 int32_t add = 0;
 #pragma unroll
                 for (int f = 0; f < 4; ++f) add += nbv[f] >= 0 ? v[f] : 0;
-                L.c(m.dspec[d]) += add;
+                L.c(tab_dspec<MT>(m, d)) += add;
             }
         }
         int tp0 = 0, tp1 = 0;
@@ -621,7 +701,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 for (int q = 0; q < m.n_sspecs; ++q)
                     if ((m.tri_tracked_mask >> q) & 1)
                         for (int tri = tp0; tri < tp1; ++tri) sys.tri_corr[(size_t)q * sys.n_tris + tri] = 0.0;
-            nev = ssa_group<MEMB>(m, sys, L, r, t, ltet, period, dt);
+            nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
@@ -635,7 +715,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     }
                 }
             }
-            if (m.n_diff > 0) {
+            if (tab_ndiff<MT>(m) > 0) {
                 const double2 g01 = sys.g[2 * (size_t)t];
                 const double2 g23 = sys.g[2 * (size_t)t + 1];
                 const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
@@ -644,10 +724,10 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 const uint64_t seed = sys.real_seed[r];
                 const uint32_t stream = sys.real_stream[r];
                 int4 *oc = reinterpret_cast<int4 *>(sys.out_cur);
-                for (int d = 0; d < m.n_diff; ++d) {
-                    const int sp = m.dspec[d];
+                for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
+                    const int sp = tab_dspec<MT>(m, d);
                     int4 o;
-                    moved += leave(sys, L, sp, (m.tracked_mask >> sp) & 1, Dr[sp], r4, bdir, S, dt,
+                    moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, Dr[sp], r4, bdir, S, dt,
                                    seed, ltet, period, stream, o);
                     oc[(size_t)d * NT + t] = o;
                 }

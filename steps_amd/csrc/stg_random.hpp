@@ -13,8 +13,10 @@
 // same reason.
 #pragma once
 
+#ifndef __HIPCC_RTC__  // a runtime-compiled build gets these from hipRTC
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 namespace stg {
 

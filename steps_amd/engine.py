@@ -204,6 +204,17 @@ class Engine:
     def set_launch(self, block: int = 256, use_graph: bool = False):
         self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
 
+    def set_specialize(self, on: bool = True):
+        """Use a period kernel compiled at run time for this model's reaction
+        network (identical results; volume-only models, blocks up to 256)."""
+        self._chk(self.lib.stg_set_specialize(self.h, 1 if on else 0))
+
+    @property
+    def specialized(self) -> bool:
+        v = C.c_int32()
+        self._chk(self.lib.stg_get_specialize(self.h, C.byref(v)))
+        return bool(v.value)
+
     def set_tri_v(self, v, real: int = 0):
         a = N.as_f64(np.broadcast_to(np.asarray(v, dtype=np.float64), (self.n_tris[real],)))
         self._chk(self.lib.stg_set_tri_v(self.h, real, N.ptr(a, C.c_double)))

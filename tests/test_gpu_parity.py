@@ -188,7 +188,8 @@ def test_autotune_keeps_results():
     ref.run(w.end_time)
     eng = _engine(w, 2)
     res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True))
-    assert res["block"] in (64, 256) and len(res["timings"]) == 4
+    # generic and model-specialised kernels, both blocks, stream and graph
+    assert res["block"] in (64, 256) and len(res["timings"]) == 8
     assert eng.time() == 0.0
     eng.run(w.end_time)
     assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
@@ -273,3 +274,88 @@ def test_surface_diffusion_bit_exact(mode):
     p = w.fm.sspec_index("Pump")
     assert _gpu_tri_counts(eng, w)[p].sum() == tot0[p]          # pumps are conserved
     assert not np.array_equal(_gpu_tri_counts(eng, w)[p], w.tri_counts[p])   # and moved
+
+
+def _higher_order_workload(seed=3):
+    """Reactions of orders 0-3 with repeated reactants (2A, A + 2B, 3C), a
+    species that does not diffuse (E) and one that is only produced."""
+    from steps_amd import geom, model as M
+
+    mdl = M.Model()
+    A, B, C, D, E = (M.Spec(x, mdl) for x in "ABCDE")
+    vs = M.Volsys("vsys", mdl)
+    M.Reac("dim", vs, lhs=[A, A], rhs=[B], kcst=2e6)
+    M.Reac("undim", vs, lhs=[B], rhs=[A, A], kcst=3.0)
+    M.Reac("tri", vs, lhs=[A, B, B], rhs=[C], kcst=1e12)
+    M.Reac("tri3", vs, lhs=[C, C, C], rhs=[D, E], kcst=1e11)
+    M.Reac("prod", vs, lhs=[], rhs=[A], kcst=1e-7)
+    M.Reac("deg", vs, lhs=[D], rhs=[], kcst=0.7)
+    M.Reac("conv", vs, lhs=[E], rhs=[C], kcst=0.3)
+    for sp, d in zip((A, B, C, D), (1e-12, 4e-13, 2e-13, 1e-12)):
+        M.Diff("d" + sp.id, vs, sp, d)
+    fm = M.flatten(mdl, ["vsys"])
+    mesh = geom.kuhn_cube(5, 5e-6)
+    comp = geom.TmComp("c", mesh)
+    comp.addVolsys("vsys")
+    mt = geom.mesh_tables(mesh, comp)
+    rng = np.random.default_rng(seed)
+    counts = rng.poisson([[60], [40], [30], [5], [2]], size=(5, len(mt.vol))).astype(np.int32)
+    return W.Workload("higher_order_cube", fm, mt, counts, np.zeros((0, 0), np.int32), 0.05, {}, mesh)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("which", ["higher_order", "four_species"])
+def test_specialised_kernel_matches_generic_and_oracle(mode, which):
+    """The kernel compiled at run time with the model's tables as constants
+    reproduces the generic kernel and the oracle bit for bit."""
+    w = (_higher_order_workload() if which == "higher_order"
+         else W.four_species_cube(n=5, side=5e-6, seed=2, end_time=0.05))
+    print(f"case specialised {which} mode={mode}", flush=True)
+    gen = _engine(w, 9, mode)
+    spec = _engine(w, 9, mode)
+    spec.set_specialize(True)
+    assert spec.specialized and not gen.specialized
+    o = _oracle(w, 9)
+    for t in (0.004, 0.02, w.end_time):
+        gen.run(t)
+        spec.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(spec, w), _gpu_counts(gen, w)), f"t={t}"
+        assert np.array_equal(_gpu_counts(spec, w), o.counts), f"t={t}"
+    assert spec.extents() == gen.extents() and spec.nsteps() == gen.nsteps()
+    # switching back and forth keeps the results
+    spec.set_specialize(False)
+    gen.run(w.end_time + 0.01)
+    spec.run(w.end_time + 0.01)
+    assert np.array_equal(_gpu_counts(spec, w), _gpu_counts(gen, w))
+
+
+def test_specialised_kernel_batch_and_graph():
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=6, n_min=3, n_max=7,
+                                                             side=4e-6, n_a=900, n_b=700, seed=4)
+    print("case specialised batch of 6 with graph replay", flush=True)
+    engs = []
+    for spec in (False, True):
+        eng = Engine(fm, meshes, [21] * 6, reac_k=reac_k)
+        for r in range(6):
+            for s in range(3):
+                eng.set_tet_counts(s, counts[r][s], real=r)
+        if spec:
+            eng.set_specialize(True)
+            eng.set_launch(128, True)
+        eng.run(0.03)
+        eng.run(0.08)
+        engs.append(eng)
+    for r in range(6):
+        a = np.stack([engs[0].get_tet_counts(s, r) for s in range(3)])
+        b = np.stack([engs[1].get_tet_counts(s, r) for s in range(3)])
+        assert np.array_equal(a, b), f"realization {r}"
+        assert engs[0].nsteps(r) == engs[1].nsteps(r)
+
+
+def test_specialised_kernel_rejects_membrane_models():
+    w = W.calcium_membrane_cube(n=4, side=2e-6, seed=1)
+    eng = _engine(w, 1)
+    with pytest.raises(N.NotImplErr):
+        eng.set_specialize(True)
+    assert not eng.specialized
