@@ -302,6 +302,7 @@ def main():
             "block": p["block"],
             "graph": bool(p["graph"]),
             "specialized_kernel": bool(p["specialized"]),
+            "spec_min_waves": p["tuned"].get("spec_min_waves") if p["tuned"] else None,
             "schedule": "autotuned" if p["tuned"] else "fixed",
             "autotune_wall_ms_per_launch": p["tuned"]["timings"] if p["tuned"] else None,
         },
@@ -337,6 +338,7 @@ def main():
             "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
             "schedule": {"block": sched.get("block"), "graph": sched.get("graph"),
                          "specialized_kernel": sched.get("specialized"),
+                         "spec_min_waves": sched.get("spec_min_waves"),
                          "autotune_wall_ms_per_launch": sched.get("timings")},
             "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"])},
         }

@@ -222,9 +222,11 @@ int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
  * compiles, at run time with hipRTC (about a second), a period kernel with
  * this model's reaction network as compile-time constants and uses it for
  * launches of up to 256 threads per block; results are identical to the
- * generic kernel's. Volume-only models of one compartment (STG_ERR_NOTIMPL
- * otherwise). on = 0 returns to the generic kernel. */
-int stg_set_specialize(stg_sim *sim, int32_t on);
+ * generic kernel's. min_waves (0..8) asks the compiler to keep at least that
+ * many wavefronts per SIMD resident (0: its own choice); each bound is
+ * compiled once and kept. Volume-only models of one compartment
+ * (STG_ERR_NOTIMPL otherwise). on = 0 returns to the generic kernel. */
+int stg_set_specialize(stg_sim *sim, int32_t on, int32_t min_waves);
 int stg_get_specialize(const stg_sim *sim, int32_t *on);
 
 /* TetOpSplitP::setDiffApplyThreshold (tetopsplit.hpp:995,

@@ -156,9 +156,10 @@ struct stg_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // model-specialised volume kernel (blocks of up to 256), compiled at run time
-    hipModule_t rtc_mod = nullptr;
-    hipFunction_t rtc_fn = nullptr;
-    bool use_rtc = false;
+    // one compiled module per waves-per-SIMD bound (0 = the compiler's choice)
+    hipModule_t rtc_mods[9] = {};
+    hipFunction_t rtc_fns[9] = {};
+    hipFunction_t rtc_fn = nullptr;  // the one in use, or null for the generic kernel
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_launches = -1;
@@ -230,7 +231,8 @@ struct stg_sim {
 
     ~stg_sim() {
         free_graph();
-        if (rtc_mod) (void)hipModuleUnload(rtc_mod);
+        for (hipModule_t m : rtc_mods)
+            if (m) (void)hipModuleUnload(m);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
                         d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
@@ -855,13 +857,14 @@ std::string spec_tables_source(const stg_sim *s) {
     return o.str();
 }
 
-void specialize(stg_sim *s) {
+void specialize(stg_sim *s, int min_waves) {
     if (s->needs_memb())
         throw StgError{STG_ERR_NOTIMPL,
                        "model-specialised kernels cover volume-only models of one compartment"};
-    if (s->rtc_fn) {
-        if (!s->use_rtc) s->free_graph();
-        s->use_rtc = true;
+    if (min_waves < 0 || min_waves > 8) throw StgError{STG_ERR_ARG, "waves per SIMD bound must be in [0, 8]"};
+    if (s->rtc_fns[min_waves]) {
+        if (s->rtc_fn != s->rtc_fns[min_waves]) s->free_graph();
+        s->rtc_fn = s->rtc_fns[min_waves];
         return;
     }
     const std::string kern = rtc_join(kRtcKernels, sizeof(kRtcKernels) / sizeof(kRtcKernels[0]));
@@ -882,15 +885,17 @@ void specialize(stg_sim *s) {
     hipDeviceProp_t prop;
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
+    // a bound of n waves per SIMD caps the kernel at 512 / n registers per lane
+    const std::string waves = "-DSTG_VOL_WAVES_PER_EU=" + std::to_string(min_waves > 0 ? min_waves : 1);
     const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
-                          "-Xclang", "-packed-fp32-ops"};
+                          "-Xclang", "-packed-fp32-ops", waves.c_str()};
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "stg_period_spec.hip", 3, hdrs, names) != HIPRTC_SUCCESS)
         throw StgError{STG_ERR_DEVICE, "hiprtcCreateProgram failed"};
     std::vector<char> code;
     std::string lowered, log;
     hiprtcAddNameExpression(prog, name.c_str());
-    const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 9, opts);
     size_t ls = 0;
     if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
         log.resize(ls);
@@ -916,16 +921,16 @@ void specialize(stg_sim *s) {
         (void)hipModuleUnload(mod);
         throw StgError{STG_ERR_DEVICE, "specialised kernel not found in its module"};
     }
-    s->rtc_mod = mod;
+    s->rtc_mods[min_waves] = mod;
+    s->rtc_fns[min_waves] = fn;
     s->rtc_fn = fn;
-    s->use_rtc = true;
     s->free_graph();
 }
 
-// back to the generic kernel; the compiled module stays loaded for a later switch
+// back to the generic kernel; compiled modules stay loaded for a later switch
 void unspecialize(stg_sim *s) {
-    if (s->use_rtc) s->free_graph();
-    s->use_rtc = false;
+    if (s->rtc_fn) s->free_graph();
+    s->rtc_fn = nullptr;
 }
 
 void launch_all(stg_sim *s, int L) {
@@ -933,7 +938,7 @@ void launch_all(stg_sim *s, int L) {
     const int nb = (int)s->block_real.size();
     const size_t smem = s->smem_bytes(B);
     const PeriodKernel kern = period_kernel(s, B);
-    const bool spec = s->use_rtc && s->rtc_fn != nullptr && B <= 256 && !needs_memb(s);
+    const bool spec = s->rtc_fn != nullptr && B <= 256 && !needs_memb(s);
     for (int l = 0; l <= L; ++l) {
         stg::SysK k = make_sys(s, l & 1);
         if (spec) {
@@ -1398,17 +1403,17 @@ int stg_set_sdcst(stg_sim *s, int32_t real, int32_t sspec, double dcst) {
     });
 }
 
-int stg_set_specialize(stg_sim *s, int32_t on) {
+int stg_set_specialize(stg_sim *s, int32_t on, int32_t min_waves) {
     if (!s) return STG_ERR_ARG;
     return guard(s, [&] {
-        if (on) specialize(s);
+        if (on) specialize(s, min_waves);
         else unspecialize(s);
     });
 }
 
 int stg_get_specialize(const stg_sim *s, int32_t *on) {
     if (!s || !on) return STG_ERR_ARG;
-    *on = s->use_rtc && s->rtc_fn != nullptr ? 1 : 0;
+    *on = s->rtc_fn != nullptr ? 1 : 0;
     return STG_OK;
 }
 

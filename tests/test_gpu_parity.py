@@ -188,8 +188,8 @@ def test_autotune_keeps_results():
     ref.run(w.end_time)
     eng = _engine(w, 2)
     res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True))
-    # generic and model-specialised kernels, both blocks, stream and graph
-    assert res["block"] in (64, 256) and len(res["timings"]) == 8
+    # generic and model-specialised kernels (two register bounds), both blocks, stream and graph
+    assert res["block"] in (64, 256) and len(res["timings"]) == 12
     assert eng.time() == 0.0
     eng.run(w.end_time)
     assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
@@ -313,7 +313,7 @@ def test_specialised_kernel_matches_generic_and_oracle(mode, which):
     print(f"case specialised {which} mode={mode}", flush=True)
     gen = _engine(w, 9, mode)
     spec = _engine(w, 9, mode)
-    spec.set_specialize(True)
+    spec.set_specialize(True, 4 if which == "four_species" else 0)
     assert spec.specialized and not gen.specialized
     o = _oracle(w, 9)
     for t in (0.004, 0.02, w.end_time):
