@@ -224,8 +224,9 @@ int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
  * launches of up to 256 threads per block; results are identical to the
  * generic kernel's. min_waves (0..8) asks the compiler to keep at least that
  * many wavefronts per SIMD resident (0: its own choice); each bound is
- * compiled once and kept. Volume-only models of one compartment
- * (STG_ERR_NOTIMPL otherwise). on = 0 returns to the generic kernel. */
+ * compiled once and kept. The volume reactions become constants; surface
+ * reactions, GHK currents and k(V) tables stay run-time tables. on = 0
+ * returns to the generic kernel. */
 int stg_set_specialize(stg_sim *sim, int32_t on, int32_t min_waves);
 int stg_get_specialize(const stg_sim *sim, int32_t *on);
 

@@ -156,10 +156,12 @@ struct stg_sim {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // model-specialised volume kernel (blocks of up to 256), compiled at run time
-    // one compiled module per waves-per-SIMD bound (0 = the compiler's choice)
-    hipModule_t rtc_mods[9] = {};
-    hipFunction_t rtc_fns[9] = {};
+    // one compiled module per (membrane variant, waves-per-SIMD bound; 0 = the
+    // compiler's choice)
+    hipModule_t rtc_mods[2][9] = {};
+    hipFunction_t rtc_fns[2][9] = {};
     hipFunction_t rtc_fn = nullptr;  // the one in use, or null for the generic kernel
+    bool rtc_memb = false;           // whether rtc_fn is the membrane variant
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_launches = -1;
@@ -231,8 +233,9 @@ struct stg_sim {
 
     ~stg_sim() {
         free_graph();
-        for (hipModule_t m : rtc_mods)
-            if (m) (void)hipModuleUnload(m);
+        for (auto &row : rtc_mods)
+            for (hipModule_t m : row)
+                if (m) (void)hipModuleUnload(m);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
                         d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
@@ -858,19 +861,18 @@ std::string spec_tables_source(const stg_sim *s) {
 }
 
 void specialize(stg_sim *s, int min_waves) {
-    if (s->needs_memb())
-        throw StgError{STG_ERR_NOTIMPL,
-                       "model-specialised kernels cover volume-only models of one compartment"};
     if (min_waves < 0 || min_waves > 8) throw StgError{STG_ERR_ARG, "waves per SIMD bound must be in [0, 8]"};
-    if (s->rtc_fns[min_waves]) {
-        if (s->rtc_fn != s->rtc_fns[min_waves]) s->free_graph();
-        s->rtc_fn = s->rtc_fns[min_waves];
+    const int memb = s->needs_memb() ? 1 : 0;
+    if (s->rtc_fns[memb][min_waves]) {
+        if (s->rtc_fn != s->rtc_fns[memb][min_waves]) s->free_graph();
+        s->rtc_fn = s->rtc_fns[memb][min_waves];
+        s->rtc_memb = memb != 0;
         return;
     }
     const std::string kern = rtc_join(kRtcKernels, sizeof(kRtcKernels) / sizeof(kRtcKernels[0]));
     const std::string rnd = rtc_join(kRtcRandom, sizeof(kRtcRandom) / sizeof(kRtcRandom[0]));
     const std::string lfact = rtc_join(kRtcLfact, sizeof(kRtcLfact) / sizeof(kRtcLfact[0]));
-    const std::string name = "stg::k_period<256, false, StgModelTables>";
+    const std::string name = std::string("stg::k_period<256, ") + (memb ? "true" : "false") + ", StgModelTables>";
     std::string src =
         "using __hip_internal::int8_t; using __hip_internal::uint8_t;\n"
         "using __hip_internal::int32_t; using __hip_internal::uint32_t;\n"
@@ -886,7 +888,8 @@ void specialize(stg_sim *s, int min_waves) {
     HIP_OK(hipGetDeviceProperties(&prop, dev));
     const std::string arch = std::string("--offload-arch=") + prop.gcnArchName;
     // a bound of n waves per SIMD caps the kernel at 512 / n registers per lane
-    const std::string waves = "-DSTG_VOL_WAVES_PER_EU=" + std::to_string(min_waves > 0 ? min_waves : 1);
+    const std::string waves = std::string(memb ? "-DSTG_PERIOD_WAVES_PER_EU=" : "-DSTG_VOL_WAVES_PER_EU=") +
+                              std::to_string(min_waves > 0 ? min_waves : 1);
     const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
                           "-Xclang", "-packed-fp32-ops", waves.c_str()};
     hiprtcProgram prog;
@@ -921,9 +924,10 @@ void specialize(stg_sim *s, int min_waves) {
         (void)hipModuleUnload(mod);
         throw StgError{STG_ERR_DEVICE, "specialised kernel not found in its module"};
     }
-    s->rtc_mods[min_waves] = mod;
-    s->rtc_fns[min_waves] = fn;
+    s->rtc_mods[memb][min_waves] = mod;
+    s->rtc_fns[memb][min_waves] = fn;
     s->rtc_fn = fn;
+    s->rtc_memb = memb != 0;
     s->free_graph();
 }
 
@@ -938,7 +942,9 @@ void launch_all(stg_sim *s, int L) {
     const int nb = (int)s->block_real.size();
     const size_t smem = s->smem_bytes(B);
     const PeriodKernel kern = period_kernel(s, B);
-    const bool spec = s->rtc_fn != nullptr && B <= 256 && !needs_memb(s);
+    // a volume-only module cannot serve a model that has come to need the
+    // membrane variant (e.g. channel integrals requested by an EField)
+    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
     for (int l = 0; l <= L; ++l) {
         stg::SysK k = make_sys(s, l & 1);
         if (spec) {

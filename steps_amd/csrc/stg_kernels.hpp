@@ -68,7 +68,7 @@ struct ModelK {
     double temp;  // K
 };
 
-// Where the volume kernel reads the reaction network from. DynTables: the
+// Where the period kernel reads the volume reaction network from. DynTables: the
 // ModelK tables at run time (the library's own build). A model-specialised
 // build (compiled at run time for one model) supplies a struct with
 // kStatic = true and the same tables as compile-time constants:
@@ -267,7 +267,6 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 template <bool MEMB, class MT>
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
                           uint32_t ltet, uint32_t period, double dt) {
-    static_assert(!(MT::kStatic && MEMB), "model-specialised tables cover the volume kernel only");
     const int R = tab_R<MT>(m), RS = m.n_sreacs, NG = m.n_ghk;
     const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
@@ -389,24 +388,26 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             if (pq > 0.0) last = q;
         }
         if (sel < 0) sel = last;
-        if constexpr (MT::kStatic) {
-            // the selected reaction's entries, in table order
+        if (!MEMB || sel < R) {
+            if constexpr (MT::kStatic) {
+                // the selected reaction's entries, in table order
 #pragma unroll
-            for (int q = 0; q < MT::R; ++q) {
-                if (sel != q) continue;
+                for (int q = 0; q < MT::R; ++q) {
+                    if (sel != q) continue;
 #pragma unroll
-                for (int e = MT::upd_off[q]; e < MT::upd_off[q + 1]; ++e) {
-                    const int sp = MT::upd_sp[e], dv = MT::upd_dv[e];
-                    L.c(sp) += dv;
-                    if ((MT::tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                    for (int e = MT::upd_off[q]; e < MT::upd_off[q + 1]; ++e) {
+                        const int sp = MT::upd_sp[e], dv = MT::upd_dv[e];
+                        L.c(sp) += dv;
+                        if ((MT::tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                    }
                 }
-            }
-        } else if (!MEMB || sel < R) {
-            for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
-                const int sp = m.ent[e] & 0xffff;
-                const int dv = m.ent[e] >> 16;
-                L.c(sp) += dv;
-                if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+            } else {
+                for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
+                    const int sp = m.ent[e] & 0xffff;
+                    const int dv = m.ent[e] >> 16;
+                    L.c(sp) += dv;
+                    if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                }
             }
         } else if ((sel - R) % RT >= RS) {
             // GHK current: one ion crosses in the direction of the flux

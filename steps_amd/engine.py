@@ -205,8 +205,8 @@ class Engine:
         self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
 
     def set_specialize(self, on: bool = True, min_waves: int = 0):
-        """Use a period kernel compiled at run time for this model's reaction
-        network (identical results; volume-only models, blocks up to 256).
+        """Use a period kernel compiled at run time for this model's volume
+        reaction network (identical results; blocks up to 256).
         min_waves: wavefronts per SIMD the compiler must allow (0: its choice)."""
         self._chk(self.lib.stg_set_specialize(self.h, 1 if on else 0, int(min_waves)))
 

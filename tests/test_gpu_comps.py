@@ -25,13 +25,17 @@ def _gpu_counts(eng, S, n_comp):
                            for c in range(n_comp)], axis=1)
 
 
+@pytest.mark.parametrize("spec", [False, True])
 @pytest.mark.parametrize("mode", [0, 1])
-def test_two_compartments_bit_exact(mode):
+def test_two_compartments_bit_exact(mode, spec):
+    """spec: the period kernel compiled at run time for the model."""
     ct, mts, dirs = _setup()
-    print(f"case two compartments mode={mode}", flush=True)
+    print(f"case two compartments mode={mode} specialised={spec}", flush=True)
     S = len(ct.fm.specs)
     eng = Engine(ct.fm, mts, [21, 21], streams=[0, 1], reac_k=ct.comp_k, spec_dcst=ct.comp_dcst,
                  mode=mode, comps=True)
+    if spec:
+        eng.set_specialize(True)
     o = bind.OracleSim.comps(ct.fm, mts, 21, ct.comp_k, ct.comp_dcst, n_bdir=len(dirs))
     a, b, c = (ct.fm.specs.index(x) for x in "ABC")
     # A open both ways at its own constant; C one-way right -> left at 1e-12

@@ -353,9 +353,26 @@ def test_specialised_kernel_batch_and_graph():
         assert engs[0].nsteps(r) == engs[1].nsteps(r)
 
 
-def test_specialised_kernel_rejects_membrane_models():
-    w = W.calcium_membrane_cube(n=4, side=2e-6, seed=1)
-    eng = _engine(w, 1)
-    with pytest.raises(N.NotImplErr):
-        eng.set_specialize(True)
-    assert not eng.specialized
+@pytest.mark.parametrize("kind", ["calcium", "vdep", "ghk", "sdiff"])
+def test_specialised_membrane_kernel_matches_generic(kind):
+    """Membrane models: the volume reactions as constants, surface reactions,
+    GHK currents, k(V) and surface diffusion from run-time tables."""
+    kw = dict(calcium=dict(ca_um=20.0, influx=3e6), vdep=dict(ca_um=20.0, influx=3e6, vdep_influx=True),
+              ghk=dict(ca_um=20.0, ghk_influx=True),
+              sdiff=dict(ca_um=30.0, pump_density=300.0, pump_dcst=1e-12))[kind]
+    w = W.calcium_membrane_cube(n=6, side=2e-6, seed=8, end_time=3e-4, buf_um=100.0, **kw)
+    print(f"case specialised membrane {kind}", flush=True)
+    gen = _engine(w, 5)
+    spec = _engine(w, 5)
+    spec.set_specialize(True)
+    assert spec.specialized
+    if kind in ("vdep", "ghk"):
+        v = np.linspace(-0.09, 0.03, len(w.mt.tri_tet))
+        gen.set_tri_v(v)
+        spec.set_tri_v(v)
+    for t in (1e-4, w.end_time):
+        gen.run(t)
+        spec.run(t)
+        assert np.array_equal(_gpu_counts(spec, w), _gpu_counts(gen, w)), f"t={t}"
+        assert np.array_equal(_gpu_tri_counts(spec, w), _gpu_tri_counts(gen, w)), f"t={t}"
+    assert spec.extents() == gen.extents() and spec.extents()[0] > 0
