@@ -19,6 +19,8 @@ from .model import Model, flatten, flatten_comps
 
 EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC = 0, 1, 2, 3  # src/steps/solver/api.hpp:61-66
 AVOGADRO = 6.02214179e23
+# meshes of at least this many tets run the period kernel compiled for their model
+SPECIALIZE_MIN_TETS = 200_000
 
 
 def _argerr(msg):
@@ -127,6 +129,14 @@ class TetOpSplit:
         else:
             self.eng = Engine(self.fm, self.mts, [self._seed] * nc, streams=np.arange(nc),
                               reac_k=self._ck, spec_dcst=self._cd, mode=self._mode, comps=True)
+        # Large meshes get the period kernel compiled for this model (identical
+        # results, about a second to compile); without run-time compilation
+        # the generic kernel stays.
+        if geom.countTets() >= SPECIALIZE_MIN_TETS:
+            try:
+                self.eng.set_specialize(True, 4)
+            except N.StgError:
+                pass
         # diffusion boundaries: open directions and directional constants
         self._db_active = set()        # (boundary, species, source comp)
         self._db_dcst = {}             # (boundary, species, source comp) -> constant

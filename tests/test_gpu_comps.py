@@ -156,3 +156,25 @@ def test_roi_totals_on_device():
     with pytest.raises(N.ArgErr):
         sim.getROIVol("nope")
     assert abs(sim.getROIArea("faces") - mesh.tri_area[mesh.getSurfTris()[:10]].sum()) < 1e-30
+
+
+def test_solver_specialises_large_meshes(monkeypatch):
+    """The solver compiles the model's kernel for meshes at or above its size
+    threshold; lowered here, the results equal the generic kernel's."""
+    import steps_amd.solver as S
+
+    out = []
+    for threshold in (10**9, 0):
+        monkeypatch.setattr(S, "SPECIALIZE_MIN_TETS", threshold)
+        mdl, mesh, comps, db = W.two_comp_cube(n=4, side=4e-6)
+        r = R.create("r123", 512)
+        r.initialize(23)
+        sim = DistTetOpSplit(mdl, mesh, r)
+        assert sim.eng.specialized == (threshold == 0)
+        sim.setDiffBoundarySpecDiffusionActive("mid", "A", True)
+        sim.setCompSpecCount("left", "A", 3000)
+        sim.setCompSpecCount("right", "C", 2000)
+        sim.run(0.2)
+        out.append([sim.getCompSpecCount(c, s) for c in ("left", "right") for s in ("A", "B", "C")])
+    print("case solver specialisation threshold", out, flush=True)
+    assert out[0] == out[1]
