@@ -74,7 +74,7 @@ def measured_traffic():
     return None
 
 
-def load_engine(w, seed, block=256, graph=False, specialize=False):
+def load_engine(w, seed, block=256, graph=False, specialize=False, spec_waves=0):
     eng = Engine(w.fm, [w.mt], [seed])
     for s in range(len(w.fm.specs)):
         eng.set_tet_counts(s, w.counts[s])
@@ -82,13 +82,13 @@ def load_engine(w, seed, block=256, graph=False, specialize=False):
         eng.set_tri_counts(s, w.tri_counts[s])
     eng.set_launch(block, graph)
     if specialize:
-        eng.set_specialize(True)
+        eng.set_specialize(True, spec_waves)
     return eng
 
 
-def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False):
+def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False, spec_waves=0):
     """block = 0: launch shape and kernel chosen by the autotuner on this state."""
-    eng = load_engine(w, seed, block or 256, graph, specialize and bool(block))
+    eng = load_engine(w, seed, block or 256, graph, specialize and bool(block), spec_waves)
     tuned = autotune(eng, trial_periods=20) if not block else None
     dt = eng.update_period()
     if warmup:
@@ -249,6 +249,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="with --block: replay launches as a graph")
     ap.add_argument("--specialize", action="store_true",
                     help="with --block: use the model-specialised kernel (compiled at run time)")
+    ap.add_argument("--spec-waves", type=int, default=0,
+                    help="with --specialize: wavefronts per SIMD the compiled kernel must allow")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -276,7 +278,8 @@ def main():
     # mesh topology built on the device (stg_mesh_build), as a large-mesh user would
     w = W.four_species_cube(n=40, side=40e-6, seed=0, device=True)
     btet = model_bytes(w)
-    p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph, specialize=args.specialize)
+    p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph, specialize=args.specialize,
+                   spec_waves=args.spec_waves)
     value = p["tet_steps"] / p["wall"]
     # roofline of the period kernel: algorithmic bytes per launch / mean launch time
     achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
@@ -302,7 +305,8 @@ def main():
             "block": p["block"],
             "graph": bool(p["graph"]),
             "specialized_kernel": bool(p["specialized"]),
-            "spec_min_waves": p["tuned"].get("spec_min_waves") if p["tuned"] else None,
+            "spec_min_waves": p["tuned"].get("spec_min_waves") if p["tuned"] else (
+                args.spec_waves if args.specialize else None),
             "schedule": "autotuned" if p["tuned"] else "fixed",
             "autotune_wall_ms_per_launch": p["tuned"]["timings"] if p["tuned"] else None,
         },

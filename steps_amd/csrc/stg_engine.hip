@@ -9,8 +9,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/stg.h"
@@ -813,6 +816,11 @@ PeriodKernel period_kernel(const stg_sim *s, int B) {
 // over reactions, reactants and diffusing species unroll and the table reads
 // disappear. Same operations in the same order, so results are identical.
 
+// Code objects compiled in this process, by program source and options: a
+// second engine with the same model and bound loads the same code.
+std::mutex g_rtc_mutex;
+std::map<std::string, std::pair<std::vector<char>, std::string>> g_rtc_cache;
+
 std::string rtc_join(const char *const *parts, size_t n) {
     std::string out;
     for (size_t i = 0; i < n; ++i) out += parts[i];
@@ -882,6 +890,8 @@ void specialize(stg_sim *s, int min_waves) {
         "(const stg::ModelK, const stg::SysK, int);\n";
     const char *hdrs[] = {kern.c_str(), rnd.c_str(), lfact.c_str()};
     const char *names[] = {"stg_kernels.hpp", "stg_random.hpp", "stg_lfact_table.inc"};
+    std::vector<char> code;
+    std::string lowered, log;
     int dev = 0;
     HIP_OK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
@@ -892,31 +902,42 @@ void specialize(stg_sim *s, int min_waves) {
                               std::to_string(min_waves > 0 ? min_waves : 1);
     const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
                           "-Xclang", "-packed-fp32-ops", waves.c_str()};
-    hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "stg_period_spec.hip", 3, hdrs, names) != HIPRTC_SUCCESS)
-        throw StgError{STG_ERR_DEVICE, "hiprtcCreateProgram failed"};
-    std::vector<char> code;
-    std::string lowered, log;
-    hiprtcAddNameExpression(prog, name.c_str());
-    const hiprtcResult rc = hiprtcCompileProgram(prog, 9, opts);
-    size_t ls = 0;
-    if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
-        log.resize(ls);
-        hiprtcGetProgramLog(prog, &log[0]);
-    }
-    if (rc == HIPRTC_SUCCESS) {
-        const char *low = nullptr;
-        size_t cs = 0;
-        if (hiprtcGetLoweredName(prog, name.c_str(), &low) == HIPRTC_SUCCESS && low) lowered = low;
-        if (hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0) {
-            code.resize(cs);
-            hiprtcGetCode(prog, code.data());
+    const std::string key = arch + "\n" + waves + "\n" + src;
+    {
+        std::lock_guard<std::mutex> lk(g_rtc_mutex);
+        auto it = g_rtc_cache.find(key);
+        if (it != g_rtc_cache.end()) {
+            code = it->second.first;
+            lowered = it->second.second;
         }
     }
-    hiprtcDestroyProgram(&prog);
-    if (rc != HIPRTC_SUCCESS || code.empty() || lowered.empty())
-        throw StgError{STG_ERR_DEVICE, "run-time compilation of the specialised kernel failed: " +
-                                           std::string(hiprtcGetErrorString(rc)) + "\n" + log.substr(0, 2000)};
+    if (code.empty()) {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "stg_period_spec.hip", 3, hdrs, names) != HIPRTC_SUCCESS)
+            throw StgError{STG_ERR_DEVICE, "hiprtcCreateProgram failed"};
+        hiprtcAddNameExpression(prog, name.c_str());
+        const hiprtcResult rc = hiprtcCompileProgram(prog, 9, opts);
+        size_t ls = 0;
+        if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
+            log.resize(ls);
+            hiprtcGetProgramLog(prog, &log[0]);
+        }
+        if (rc == HIPRTC_SUCCESS) {
+            const char *low = nullptr;
+            size_t cs = 0;
+            if (hiprtcGetLoweredName(prog, name.c_str(), &low) == HIPRTC_SUCCESS && low) lowered = low;
+            if (hiprtcGetCodeSize(prog, &cs) == HIPRTC_SUCCESS && cs > 0) {
+                code.resize(cs);
+                hiprtcGetCode(prog, code.data());
+            }
+        }
+        hiprtcDestroyProgram(&prog);
+        if (rc != HIPRTC_SUCCESS || code.empty() || lowered.empty())
+            throw StgError{STG_ERR_DEVICE, "run-time compilation of the specialised kernel failed: " +
+                                               std::string(hiprtcGetErrorString(rc)) + "\n" + log.substr(0, 2000)};
+        std::lock_guard<std::mutex> lk(g_rtc_mutex);
+        g_rtc_cache[key] = {code, lowered};
+    }
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
     HIP_OK(hipModuleLoadData(&mod, code.data()));
