@@ -313,6 +313,12 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
     long nev = 0;
     double tcur = 0.0;
     while (true) {
+        // With compile-time reaction tables the per-tet constants in LDS are
+        // loop-invariant and the compiler would hold all of them in registers
+        // across the event loop (137 VGPRs for four species, spilling under a
+        // 4-wave bound); this compiler-only fence keeps them as LDS reads inside
+        // each event (117 VGPRs, no scratch). No instruction is emitted.
+        if constexpr (MT::kStatic) asm volatile("" ::: "memory");
         double a0 = 0.0;
         int j = 0;
         if constexpr (MT::kStatic) {
