@@ -122,9 +122,17 @@ struct EfDev {
     const double *wsum;    // [n]   row sums of w
     const float *a32;      // [width][n] -c_ij in FP32 (0 for clamped rows/cols)
     const float *d32;      // [n]   diagonal in FP32
+    const float *m32;      // [n]   row sum in FP32: C/dt + G + sigma * (couplings to clamped vertices)
     const double *d64;     // [n]   diagonal in FP64
     const unsigned char *clamped;
 };
+
+// FP32 products use the difference form (A x)_i = m_i x_i + sum_j a_ij (x_j - x_i),
+// with m_i the row sum of the FP64 matrix rounded once. Rounding a_ij and the
+// diagonal separately would leave each Laplacian row (zero sum) off by ~1e-7
+// of its diagonal, a spurious mass term that swamps the membrane capacitance on
+// the smooth modes and costs refinements; the difference form is exact on
+// constants, like the FP64 operator.
 
 // membrane current of one triangle (pA, outward positive) from ohmic channels
 // (Tri::computeI, src/steps/mpi/tetopsplit/tri.cpp:664-693)
@@ -168,25 +176,30 @@ __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, co
 // b = I + G (Vext - V) + sum_j c_ij (V_j - V_i); diagonal C/dt + G + sigma*wsum
 __global__ void k_assemble(EfDev m, double sigma, double oodt, const double *cap, const double *gext,
                            double vext, const double *V, const double *I, double *b, double *d64,
-                           float *d32) {
+                           float *d32, float *m32) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.n) return;
     if (m.clamped[i]) {
         b[i] = 0.0;
         d64[i] = 1.0;
         d32[i] = 1.0f;
+        m32[i] = 1.0f;
         return;
     }
     const double vi = V[i];
     double rhs = I[i] + gext[i] * (vext - vi);
+    double wcl = 0.0;  // couplings to clamped vertices: a diagonal term only
     for (int k = 0; k < m.width; ++k) {
         const int e = k * m.n + i;
-        rhs += sigma * m.w[e] * (V[m.col[e]] - vi);
+        const int j = m.col[e];
+        rhs += sigma * m.w[e] * (V[j] - vi);
+        if (m.clamped[j]) wcl += m.w[e];
     }
     b[i] = rhs;
     const double d = cap[i] * oodt + gext[i] + sigma * m.wsum[i];
     d64[i] = d;
     d32[i] = (float)d;
+    m32[i] = (float)(cap[i] * oodt + gext[i] + sigma * wcl);
 }
 
 __global__ void k_offdiag32(EfDev m, double sigma, float *a32) {
@@ -234,12 +247,12 @@ __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *
         const T pi = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
         T qi;
         if constexpr (sizeof(T) == 4) {
-            float acc = m.d32[i] * (float)pi;
+            float acc = m.m32[i] * (float)pi;
             for (int k = 0; k < m.width; ++k) {
                 const int e = k * m.n + i;
                 const int j = m.col[e];
                 const float pj = first ? (float)z[j] : (float)((double)z[j] + beta * (double)p[j]);
-                acc += m.a32[e] * pj;
+                acc += m.a32[e] * (pj - (float)pi);
             }
             qi = acc;
         } else {
@@ -310,10 +323,11 @@ __global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *
 template <typename T, typename F>
 __device__ __forceinline__ double ell_row(const EfDev &m, double sigma, int i, F xv) {
     if constexpr (sizeof(T) == 4) {
-        float acc = m.d32[i] * (float)xv(i);
+        const float xi = (float)xv(i);
+        float acc = m.m32[i] * xi;
         for (int k = 0; k < m.width; ++k) {
             const int e = k * m.n + i;
-            acc += m.a32[e] * (float)xv(m.col[e]);
+            acc += m.a32[e] * ((float)xv(m.col[e]) - xi);
         }
         return acc;
     } else {
@@ -369,6 +383,26 @@ __device__ __forceinline__ double csr_row_dot(const int *ptr, const int *col, co
     return acc;
 }
 
+// (A x)_row of a level operator. FP32 levels use the difference form with the
+// row sums rs of the FP64 Galerkin operator (the diagonal entry adds nothing);
+// FP64 levels the plain product.
+template <typename T, int G>
+__device__ __forceinline__ double csr_row_apply(const int *ptr, const int *col, const T *val, const float *rs,
+                                                const T *x, int row, bool valid, int lane) {
+    if constexpr (sizeof(T) == 4) {
+        double acc = 0.0;
+        if (valid) {
+            const float xr = (float)x[row];
+            for (int e = ptr[row] + lane; e < ptr[row + 1]; e += G)
+                acc += (double)val[e] * (double)((float)x[col[e]] - xr);
+        }
+        for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
+        return valid ? acc + (double)rs[row] * (double)x[row] : 0.0;
+    } else {
+        return csr_row_dot<T, G>(ptr, col, val, x, row, valid, lane);
+    }
+}
+
 // x = om D^-1 b on a CSR level (pre-smoothing from a zero guess)
 template <typename T>
 __global__ void k_scale(int n, const T *dinv, double om, const T *b, T *x) {
@@ -378,22 +412,23 @@ __global__ void k_scale(int n, const T *dinv, double om, const T *b, T *x) {
 
 // r = b - A x on a CSR level (diagonal stored in the rows)
 template <typename T, int G>
-__global__ void k_csr_resid(int n, const int *ptr, const int *col, const T *val, const T *b, const T *x, T *r) {
+__global__ void k_csr_resid(int n, const int *ptr, const int *col, const T *val, const float *rs, const T *b,
+                            const T *x, T *r) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = t / G, lane = t & (G - 1);
     const bool valid = row < n;
-    const double ax = csr_row_dot<T, G>(ptr, col, val, x, row, valid, lane);
+    const double ax = csr_row_apply<T, G>(ptr, col, val, rs, x, row, valid, lane);
     if (valid && lane == 0) r[row] = (T)((double)b[row] - ax);
 }
 
 // out = x + om D^-1 (b - A x) on a CSR level (post-smoothing)
 template <typename T, int G>
-__global__ void k_csr_up(int n, const int *ptr, const int *col, const T *val, const T *dinv, double om,
-                         const T *b, const T *x, T *out) {
+__global__ void k_csr_up(int n, const int *ptr, const int *col, const T *val, const float *rs, const T *dinv,
+                         double om, const T *b, const T *x, T *out) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = t / G, lane = t & (G - 1);
     const bool valid = row < n;
-    const double ax = csr_row_dot<T, G>(ptr, col, val, x, row, valid, lane);
+    const double ax = csr_row_apply<T, G>(ptr, col, val, rs, x, row, valid, lane);
     if (valid && lane == 0) out[row] = (T)((double)x[row] + om * ((double)b[row] - ax) * (double)dinv[row]);
 }
 
@@ -487,10 +522,11 @@ __global__ void __launch_bounds__(1024) k_cg_small(EfDev m, double sigma, const 
         for (int i = tid; i < n; i += B) {
             double qi;
             if (f32) {
-                float acc = m.d32[i] * (float)p[i];
+                const float pi = (float)p[i];
+                float acc = m.m32[i] * pi;
                 for (int k = 0; k < m.width; ++k) {
                     const int e = k * n + i;
-                    acc += m.a32[e] * (float)p[m.col[e]];
+                    acc += m.a32[e] * ((float)p[m.col[e]] - pi);
                 }
                 qi = acc;
             } else {
@@ -574,7 +610,7 @@ struct AmgLevel {
     double om = 0.0;                                    // Jacobi weight 4 / (3 rho(D^-1 A))
     int ga = 1, gp = 1, gr = 1;                         // lanes per row of A_l, P_l, R_l
     int *a_ptr = nullptr, *a_col = nullptr;             // A_l (l >= 1), diagonal included
-    float *a32 = nullptr, *dinv32 = nullptr;
+    float *a32 = nullptr, *dinv32 = nullptr, *rs32 = nullptr;  // rs32: row sums of A_l
     double *a64 = nullptr, *dinv64 = nullptr;
     int *p_ptr = nullptr, *p_col = nullptr;             // P_l: n rows
     float *p32 = nullptr;
@@ -598,8 +634,8 @@ struct AmgDev {
 
     void release() {
         for (AmgLevel &L : lev) {
-            void *ptrs[] = {L.a_ptr, L.a_col, L.a32, L.dinv32, L.a64, L.dinv64, L.p_ptr, L.p_col, L.p32, L.p64,
-                            L.r_ptr, L.r_col, L.r32, L.r64, L.b, L.x, L.r, L.out};
+            void *ptrs[] = {L.a_ptr, L.a_col, L.a32, L.dinv32, L.rs32, L.a64, L.dinv64, L.p_ptr, L.p_col, L.p32,
+                            L.p64, L.r_ptr, L.r_col, L.r32, L.r64, L.b, L.x, L.r, L.out};
             for (void *q : ptrs)
                 if (q) (void)hipFree(q);
         }
@@ -652,7 +688,7 @@ struct stg_efield {
     double *w = nullptr, *wsum = nullptr, *d64 = nullptr, *V = nullptr, *I = nullptr, *b = nullptr;
     double *cap = nullptr, *gext = nullptr, *vclamp = nullptr, *tclamp = nullptr;
     double *x64 = nullptr, *r64 = nullptr, *dv_prev = nullptr;
-    float *a32 = nullptr, *d32 = nullptr;
+    float *a32 = nullptr, *d32 = nullptr, *m32 = nullptr;
     unsigned char *clamped = nullptr;
     void *xs = nullptr, *rs = nullptr, *zs = nullptr, *ps = nullptr, *qs = nullptr;  // T-typed work
     double *part = nullptr, *scal = nullptr, *outs = nullptr;
@@ -661,13 +697,13 @@ struct stg_efield {
     int nb = 1;
 
     EfDev dev() const {
-        return EfDev{n, width, col, w, wsum, a32, d32, d64, clamped};
+        return EfDev{n, width, col, w, wsum, a32, d32, m32, d64, clamped};
     }
 
     ~stg_efield() {
         amg.release();
         void *ptrs[] = {col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
-                        tclamp, x64, r64, dv_prev, a32, d32, clamped, xs, rs, zs, ps, qs, part,
+                        tclamp, x64, r64, dv_prev, a32, d32, m32, clamped, xs, rs, zs, ps, qs, part,
                         scal, outs, cnt};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
@@ -838,6 +874,7 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     e->wsum = ef_alloc<double>(n);
     e->a32 = ef_alloc<float>(w.size());
     e->d32 = ef_alloc<float>(n);
+    e->m32 = ef_alloc<float>(n);
     e->d64 = ef_alloc<double>(n);
     e->V = ef_alloc<double>(n);
     e->I = ef_alloc<double>(n);
@@ -1260,6 +1297,10 @@ void amg_build(stg_efield *e, double oodt) {
             L.a64 = up_f64(A.val);
             L.dinv32 = up_f32(dinv);
             L.dinv64 = up_f64(dinv);
+            std::vector<double> rsum(A.n, 0.0);
+            for (int i = 0; i < A.n; ++i)
+                for (int e = A.ptr[i]; e < A.ptr[i + 1]; ++e) rsum[i] += A.val[e];
+            L.rs32 = up_f32(rsum);
             L.b = ef_alloc<double>(A.n);
             L.out = ef_alloc<double>(A.n);
         }
@@ -1339,7 +1380,7 @@ void vcycle(stg_efield *e, const T *b0, T *z0) {
             hipLaunchKernelGGL(k_scale<T>, blocks(V.n), dim3(kB), 0, e->stream, V.n, pick<T>(V.dinv32, V.dinv64),
                                V.om, b, x);
             GroupLaunch<T, ResidK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
-                                        pick<T>(V.a32, V.a64), b, (const T *)x, r);
+                                        pick<T>(V.a32, V.a64), (const float *)V.rs32, b, (const T *)x, r);
         }
         const int nn = l + 1 < L ? G.lev[l + 1].n : G.nc;
         T *bc = reinterpret_cast<T *>(l + 1 < L ? G.lev[l + 1].b : G.cb);
@@ -1361,8 +1402,8 @@ void vcycle(stg_efield *e, const T *b0, T *z0) {
                                (const T *)x, out);
         else
             GroupLaunch<T, UpK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
-                                     pick<T>(V.a32, V.a64), pick<T>(V.dinv32, V.dinv64), V.om, b, (const T *)x,
-                                     out);
+                                     pick<T>(V.a32, V.a64), (const float *)V.rs32, pick<T>(V.dinv32, V.dinv64),
+                                     V.om, b, (const T *)x, out);
     }
 }
 
@@ -1499,7 +1540,7 @@ void advance_impl(stg_efield *e, double dt_s) {
             amg_build(e, oodt);
     }
     hipLaunchKernelGGL(k_assemble, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, oodt, e->cap, e->gext,
-                       e->vext, e->V, e->I, e->b, e->d64, e->d32);
+                       e->vext, e->V, e->I, e->b, e->d64, e->d32, e->m32);
     if (e->a32_dirty) {
         hipLaunchKernelGGL(k_offdiag32, dim3(nb), dim3(kB), 0, e->stream, e->dev(), e->sigma, e->a32);
         e->a32_dirty = false;
