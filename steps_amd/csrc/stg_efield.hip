@@ -212,20 +212,71 @@ __global__ void k_offdiag32(EfDev m, double sigma, float *a32) {
     }
 }
 
+// (A x)_i of the fine matrix for x given element-wise by xv(j), in slot order.
+// The slots are taken kEllChunk at a time: their column and coupling loads are
+// issued together, then their gathers, then the products are summed in slot
+// order, so a row costs about one dependent round trip per chunk instead of
+// one per slot, and every sum is the one the plain slot loop forms.
+constexpr int kEllChunk = 8;
+
+template <typename T, typename F>
+__device__ __forceinline__ double ell_row(const EfDev &m, double sigma, int i, F xv) {
+    constexpr int U = kEllChunk;
+    if constexpr (sizeof(T) == 4) {
+        const float xi = (float)xv(i);
+        float acc = m.m32[i] * xi;
+        for (int k0 = 0; k0 < m.width; k0 += U) {
+            int c[U];
+            float a[U], v[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool on = k0 + u < m.width;
+                const int e = (k0 + u) * m.n + i;
+                c[u] = on ? m.col[e] : i;
+                a[u] = on ? m.a32[e] : 0.0f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) v[u] = k0 + u < m.width ? (float)xv(c[u]) : xi;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u < m.width) acc += a[u] * (v[u] - xi);
+        }
+        return acc;
+    } else {
+        double acc = m.d64[i] * (double)xv(i);
+        if (!m.clamped[i])
+            for (int k0 = 0; k0 < m.width; k0 += U) {
+                int c[U];
+                double w[U], v[U];
+                bool use[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool on = k0 + u < m.width;
+                    const int e = (k0 + u) * m.n + i;
+                    c[u] = on ? m.col[e] : i;
+                    w[u] = on ? m.w[e] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool on = k0 + u < m.width;
+                    use[u] = on && !m.clamped[c[u]];
+                    v[u] = on ? (double)xv(c[u]) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (use[u]) acc -= sigma * w[u] * v[u];
+            }
+        return acc;
+    }
+}
+
 // r = b - A x in FP64, and ||r||^2, ||b||^2
 __global__ void k_residual64(EfDev m, double sigma, const double *b, const double *x, double *r,
                              double *part_r) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     double rr = 0.0;
     if (i < m.n) {
-        double ax = m.d64[i] * x[i];
-        if (!m.clamped[i]) {
-            for (int k = 0; k < m.width; ++k) {
-                const int e = k * m.n + i;
-                const int j = m.col[e];
-                if (!m.clamped[j]) ax -= sigma * m.w[e] * x[j];
-            }
-        }
+        const double ax = ell_row<double>(m, sigma, i, [&](int j) { return x[j]; });
         const double ri = b[i] - ax;
         r[i] = ri;
         rr = ri * ri;
@@ -244,30 +295,9 @@ __global__ void k_cg_a(EfDev m, double sigma, const T *z, const T *p, T *pn, T *
     if (i < m.n) {
         // the first iteration never reads p_old (it is uninitialised memory)
         const double beta = first ? 0.0 : scal[1];
-        const T pi = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
-        T qi;
-        if constexpr (sizeof(T) == 4) {
-            float acc = m.m32[i] * (float)pi;
-            for (int k = 0; k < m.width; ++k) {
-                const int e = k * m.n + i;
-                const int j = m.col[e];
-                const float pj = first ? (float)z[j] : (float)((double)z[j] + beta * (double)p[j]);
-                acc += m.a32[e] * (pj - (float)pi);
-            }
-            qi = acc;
-        } else {
-            double acc = m.d64[i] * (double)pi;
-            if (!m.clamped[i]) {
-                for (int k = 0; k < m.width; ++k) {
-                    const int e = k * m.n + i;
-                    const int j = m.col[e];
-                    if (m.clamped[j]) continue;
-                    const double pj = first ? (double)z[j] : (double)z[j] + beta * (double)p[j];
-                    acc -= sigma * m.w[e] * pj;
-                }
-            }
-            qi = acc;
-        }
+        auto pv = [&](int j) { return first ? z[j] : (T)((double)z[j] + beta * (double)p[j]); };
+        const T pi = pv(i);
+        const T qi = (T)ell_row<T>(m, sigma, i, pv);
         q[i] = qi;
         pn[i] = pi;
         pq = (double)pi * (double)qi;
@@ -319,29 +349,6 @@ __global__ void k_cg_init(EfDev m, const double *r64, T *x, T *r, T *z, double *
 // operators are CSR. Smoothing is damped Jacobi from a zero guess, one sweep
 // down and one up, which keeps M symmetric for CG.
 
-// (A x)_i of the fine matrix for x given element-wise by xv(j)
-template <typename T, typename F>
-__device__ __forceinline__ double ell_row(const EfDev &m, double sigma, int i, F xv) {
-    if constexpr (sizeof(T) == 4) {
-        const float xi = (float)xv(i);
-        float acc = m.m32[i] * xi;
-        for (int k = 0; k < m.width; ++k) {
-            const int e = k * m.n + i;
-            acc += m.a32[e] * ((float)xv(m.col[e]) - xi);
-        }
-        return acc;
-    } else {
-        double acc = m.d64[i] * (double)xv(i);
-        if (!m.clamped[i])
-            for (int k = 0; k < m.width; ++k) {
-                const int e = k * m.n + i;
-                const int j = m.col[e];
-                if (!m.clamped[j]) acc -= sigma * m.w[e] * (double)xv(j);
-            }
-        return acc;
-    }
-}
-
 template <typename T>
 __device__ __forceinline__ T fine_diag(const EfDev &m, int i) {
     if constexpr (sizeof(T) == 4) return m.d32[i];
@@ -359,26 +366,62 @@ __global__ void k_amg0_down(EfDev m, double sigma, double om, const T *b, T *x, 
     r[i] = (T)((double)b[i] - ell_row<T>(m, sigma, i, xv));
 }
 
-// up sweep on level 0: out = x + om D^-1 (b - A x)
+// up sweep on level 0: out = x + om D^-1 (b - A x); with part given, also the
+// block partials of (b, out), which CG needs next (the preconditioned rz)
 template <typename T>
-__global__ void k_amg0_up(EfDev m, double sigma, double om, const T *b, const T *x, T *out) {
+__global__ void k_amg0_up(EfDev m, double sigma, double om, const T *b, const T *x, T *out, double *part) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.n) return;
-    auto xv = [&](int j) { return x[j]; };
-    const double ax = ell_row<T>(m, sigma, i, xv);
-    out[i] = (T)((double)x[i] + om * ((double)b[i] - ax) / (double)fine_diag<T>(m, i));
+    double bz = 0.0;
+    if (i < m.n) {
+        auto xv = [&](int j) { return x[j]; };
+        const double ax = ell_row<T>(m, sigma, i, xv);
+        const T zi = (T)((double)x[i] + om * ((double)b[i] - ax) / (double)fine_diag<T>(m, i));
+        out[i] = zi;
+        bz = (double)b[i] * (double)zi;
+    }
+    if (part) block_partial(bz, part);
 }
 
 // CSR kernels give each row a group of G lanes (G a power of two sized to the
 // operator's mean row length): the group reads the row with stride G, so the
 // loads of a wavefront are coalesced, and sums it with a fixed shuffle tree,
-// so results do not depend on timing.
+// so results do not depend on timing. Each lane takes kCsrChunk of its entries
+// at a time (loads first, then gathers, then the sum in entry order), so a
+// lane waits on one dependent round trip per chunk rather than per entry.
+constexpr int kCsrChunk = 4;
+
+// sum over the lane's entries e of f(val[e], x[col[e]]), in entry order
+template <typename T, int G, typename F>
+__device__ __forceinline__ double csr_lane_sum(const int *ptr, const int *col, const T *val, const T *x, int row,
+                                               int lane, F f) {
+    constexpr int U = kCsrChunk;
+    const int end = ptr[row + 1];
+    double acc = 0.0;
+    for (int e0 = ptr[row] + lane; e0 < end; e0 += U * G) {
+        int c[U];
+        T a[U], v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int e = e0 + u * G;
+            c[u] = e < end ? col[e] : row;
+            a[u] = e < end ? val[e] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = e0 + u * G < end ? x[c[u]] : (T)0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (e0 + u * G < end) acc += f(a[u], v[u]);
+    }
+    return acc;
+}
+
 template <typename T, int G>
 __device__ __forceinline__ double csr_row_dot(const int *ptr, const int *col, const T *val, const T *x, int row,
                                               bool valid, int lane) {
     double acc = 0.0;
     if (valid)
-        for (int e = ptr[row] + lane; e < ptr[row + 1]; e += G) acc += (double)val[e] * (double)x[col[e]];
+        acc = csr_lane_sum<T, G>(ptr, col, val, x, row, lane,
+                                 [](T a, T v) { return (double)a * (double)v; });
     for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
     return acc;
 }
@@ -393,8 +436,8 @@ __device__ __forceinline__ double csr_row_apply(const int *ptr, const int *col, 
         double acc = 0.0;
         if (valid) {
             const float xr = (float)x[row];
-            for (int e = ptr[row] + lane; e < ptr[row + 1]; e += G)
-                acc += (double)val[e] * (double)((float)x[col[e]] - xr);
+            acc = csr_lane_sum<T, G>(ptr, col, val, x, row, lane,
+                                     [xr](T a, T v) { return (double)a * (double)((float)v - xr); });
         }
         for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
         return valid ? acc + (double)rs[row] * (double)x[row] : 0.0;
@@ -470,12 +513,6 @@ __global__ void k_cg_start(int n, const double *r64, T *x, T *r) {
     r[i] = (T)r64[i];
 }
 
-template <typename T>
-__global__ void k_dot(int n, const T *a, const T *b, double *part) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const double v = i < n ? (double)a[i] * (double)b[i] : 0.0;
-    block_partial(v, part);
-}
 
 // Sum over the block of one double per thread; every thread gets the total.
 __device__ __forceinline__ double block_allsum(double v, double *red) {
@@ -631,8 +668,14 @@ struct AmgDev {
     double *inv64 = nullptr;
     double *cb = nullptr, *cx = nullptr;
     double setup_s = 0.0;
+    // two CG iterations (V-cycles included) captured once per hierarchy and
+    // precision: the launch sequence holds this hierarchy's pointers and weights
+    hipGraphExec_t pair32 = nullptr, pair64 = nullptr;
 
     void release() {
+        if (pair32) (void)hipGraphExecDestroy(pair32);
+        if (pair64) (void)hipGraphExecDestroy(pair64);
+        pair32 = pair64 = nullptr;
         for (AmgLevel &L : lev) {
             void *ptrs[] = {L.a_ptr, L.a_col, L.a32, L.dinv32, L.rs32, L.a64, L.dinv64, L.p_ptr, L.p_col, L.p32,
                             L.p64, L.r_ptr, L.r_col, L.r32, L.r64, L.b, L.x, L.r, L.out};
@@ -812,8 +855,16 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
                 add(v[a], v[c], -kij);
             }
     }
+    // Couplings that cancel to exactly zero over their tets carry nothing and
+    // are dropped (removing an exact zero term leaves every sum bit-identical).
+    // On Kuhn (Freudenthal) meshes the P1 stiffness is the 7-point stencil:
+    // 8 of an interior vertex's 14 edges cancel, and the table width falls from
+    // 14 to 6, which every fine-level pass reads.
     size_t width = 1, nnz = 0;
     for (int i = 0; i < n; ++i) {
+        auto &r = rows[i];
+        r.erase(std::remove_if(r.begin(), r.end(), [](const std::pair<int, double> &kv) { return kv.second == 0.0; }),
+                r.end());
         width = std::max(width, rows[i].size());
         nnz += rows[i].size();
     }
@@ -1250,10 +1301,14 @@ float *up_f32(const std::vector<double> &v) {
 constexpr int kCoarsest = 400;  // levels at or below this size get a dense inverse
 
 // lanes per row: the power of two nearest the mean row length, 1..64
+// Lanes per row: about one chunk of kCsrChunk entries per lane. A group as
+// wide as the row (one entry per lane) costs more threads and shuffle steps:
+// on the 1 M-vertex cube's hierarchy the V-cycle's CSR kernels took 196 us
+// per CG iteration that way against 165 us with this choice.
 int group_size(const Csr &M) {
     const double mean = M.n > 0 ? (double)M.col.size() / M.n : 1.0;
     int g = 1;
-    while (g < 64 && g * 1.5 < mean) g *= 2;
+    while (g < 64 && g * 1.5 * kCsrChunk < mean) g *= 2;
     return g;
 }
 
@@ -1363,9 +1418,10 @@ const T *pick(const float *f, const double *d) {
     else return reinterpret_cast<const T *>(d);
 }
 
-// z = M b0: one V-cycle on the device
+// z = M b0: one V-cycle on the device; the block partials of (b0, z0) go to
+// part (one per block of the fine level), for k_reduce
 template <typename T>
-void vcycle(stg_efield *e, const T *b0, T *z0) {
+void vcycle(stg_efield *e, const T *b0, T *z0, double *part) {
     AmgDev &G = e->amg;
     const int L = (int)G.lev.size();
     const EfDev m = e->dev();
@@ -1399,7 +1455,7 @@ void vcycle(stg_efield *e, const T *b0, T *z0) {
         T *out = l == 0 ? z0 : reinterpret_cast<T *>(V.out);
         if (l == 0)
             hipLaunchKernelGGL(k_amg0_up<T>, blocks(V.n), dim3(kB), 0, e->stream, m, e->sigma, V.om, b,
-                               (const T *)x, out);
+                               (const T *)x, out, part);
         else
             GroupLaunch<T, UpK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
                                      pick<T>(V.a32, V.a64), (const float *)V.rs32, pick<T>(V.dinv32, V.dinv64),
@@ -1415,27 +1471,44 @@ int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
     const int nb = e->nb, n = e->n;
     T *x = (T *)e->xs, *r = (T *)e->rs, *z = (T *)e->zs, *p = (T *)e->ps, *q = (T *)e->qs;
     hipLaunchKernelGGL(k_cg_start<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->r64, x, r);
-    vcycle<T>(e, r, z);
-    hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kB), 0, e->stream, n, (const T *)r, (const T *)z, e->part);
+    vcycle<T>(e, r, z, e->part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 0, e->scal, e->outs);
     const double rz0 = fetch(e, e->scal);
+    // iteration it: the search directions alternate between two buffers, and
+    // the first iteration starts from p = z
+    auto iteration = [&](int it) {
+        T *pold = p + (size_t)(it & 1) * n, *pnew = p + (size_t)((it + 1) & 1) * n;
+        hipLaunchKernelGGL(k_cg_a<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, z, pold, pnew, q, e->scal,
+                           it == 0 ? 1 : 0, e->part);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 1, e->scal, e->outs);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->scal, x, r, (const T *)pnew,
+                           (const T *)q);
+        vcycle<T>(e, r, z, e->part);
+        hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 2, e->scal, e->outs);
+    };
+    // Iterations run in pairs between convergence checks (each check is a
+    // host round trip). After the first pair, a pair (even it, odd it) is the
+    // same launch sequence every time, so it is captured once as a graph and
+    // replayed: about 20 launches per iteration cost their launch gaps once.
+    hipGraphExec_t &pair = sizeof(T) == 4 ? e->amg.pair32 : e->amg.pair64;
     int it = 0;
     if (rz0 > 0.0) {
-        const int check = 2;
         while (it < max_iter) {
-            for (int k = 0; k < check && it < max_iter; ++k, ++it) {
-                T *pold = p + (size_t)(it & 1) * n, *pnew = p + (size_t)((it + 1) & 1) * n;
-                hipLaunchKernelGGL(k_cg_a<T>, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, z, pold, pnew, q,
-                                   e->scal, it == 0 ? 1 : 0, e->part);
-                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 1, e->scal,
-                                   e->outs);
-                hipLaunchKernelGGL(k_cg_update<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->scal, x, r,
-                                   (const T *)pnew, (const T *)q);
-                vcycle<T>(e, r, z);
-                hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(kB), 0, e->stream, n, (const T *)r, (const T *)z,
-                                   e->part);
-                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 2, e->scal,
-                                   e->outs);
+            if (it > 0 && (it & 1) == 0 && it + 2 <= max_iter) {
+                if (!pair) {
+                    hipGraph_t g = nullptr;
+                    EF_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+                    iteration(it);
+                    iteration(it + 1);
+                    EF_HIP(hipStreamEndCapture(e->stream, &g));
+                    const hipError_t rc = hipGraphInstantiate(&pair, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    EF_HIP(rc);
+                }
+                EF_HIP(hipGraphLaunch(pair, e->stream));
+                it += 2;
+            } else {
+                for (int k = 0; k < 2 && it < max_iter; ++k, ++it) iteration(it);
             }
             const double rz = fetch(e, e->scal);
             if (!(rz > rel_tol * rel_tol * rz0)) break;
