@@ -61,6 +61,7 @@ void ef_put(T *d, const std::vector<T> &h) {
 }
 
 constexpr int kB = 256;
+constexpr int kCoarsest = 400;  // levels at or below this size get a dense inverse
 
 // ------------------------------------------------------------- reductions
 
@@ -446,13 +447,6 @@ __device__ __forceinline__ double csr_row_apply(const int *ptr, const int *col, 
     }
 }
 
-// x = om D^-1 b on a CSR level (pre-smoothing from a zero guess)
-template <typename T>
-__global__ void k_scale(int n, const T *dinv, double om, const T *b, T *x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = (T)(om * (double)b[i] * (double)dinv[i]);
-}
-
 // r = b - A x on a CSR level (diagonal stored in the rows)
 template <typename T, int G>
 __global__ void k_csr_resid(int n, const int *ptr, const int *col, const T *val, const float *rs, const T *b,
@@ -475,24 +469,35 @@ __global__ void k_csr_up(int n, const int *ptr, const int *col, const T *val, co
     if (valid && lane == 0) out[row] = (T)((double)x[row] + om * ((double)b[row] - ax) * (double)dinv[row]);
 }
 
-// out (+)= M in for a CSR operator (restriction R = P^T, or prolongation P)
+// out (+)= M in for a CSR operator (restriction R = P^T, or prolongation P).
+// With xs given, a restriction also starts the coarse level's down sweep:
+// xs = om D^-1 out (damped Jacobi from a zero guess).
 template <typename T, int G>
-__global__ void k_csr_apply(int n, const int *ptr, const int *col, const T *val, const T *in, T *out, int add) {
+__global__ void k_csr_apply(int n, const int *ptr, const int *col, const T *val, const T *in, T *out, int add,
+                            const T *dinv, double om, T *xs) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = t / G, lane = t & (G - 1);
     const bool valid = row < n;
     const double acc = csr_row_dot<T, G>(ptr, col, val, in, row, valid, lane);
-    if (valid && lane == 0) out[row] = (T)((add ? (double)out[row] : 0.0) + acc);
+    if (valid && lane == 0) {
+        const T o = (T)((add ? (double)out[row] : 0.0) + acc);
+        out[row] = o;
+        if (xs) xs[row] = (T)(om * (double)o * (double)dinv[row]);
+    }
 }
 
-// coarsest level: x = A^-1 b with the dense inverse
+// Coarsest level: x_c = A_c^-1 b_c with the dense inverse, 16 lanes per row
+// (coalesced reads of the row, fixed shuffle tree)
 template <typename T>
-__global__ void k_dense_apply(int n, const T *inv, const T *b, T *x) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_dense_apply(int nc, const T *inv, const T *b, T *xc) {
+    constexpr int H = 16;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / H, hl = t & (H - 1);
     double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc += (double)inv[(size_t)i * n + j] * (double)b[j];
-    x[i] = (T)acc;
+    if (i < nc)
+        for (int j = hl; j < nc; j += H) acc += (double)inv[(size_t)i * nc + j] * (double)b[j];
+    for (int off = H / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, H);
+    if (i < nc && hl == 0) xc[i] = (T)acc;
 }
 
 // PCG pieces around the multigrid preconditioner
@@ -1298,7 +1303,6 @@ float *up_f32(const std::vector<double> &v) {
     return d;
 }
 
-constexpr int kCoarsest = 400;  // levels at or below this size get a dense inverse
 
 // lanes per row: the power of two nearest the mean row length, 1..64
 // Lanes per row: about one chunk of kCsrChunk entries per lane. A group as
@@ -1412,6 +1416,7 @@ template <typename T, int G> struct ResidK { static constexpr auto fn = k_csr_re
 template <typename T, int G> struct UpK { static constexpr auto fn = k_csr_up<T, G>; };
 template <typename T, int G> struct ApplyK { static constexpr auto fn = k_csr_apply<T, G>; };
 
+
 template <typename T>
 const T *pick(const float *f, const double *d) {
     if constexpr (sizeof(T) == 4) return reinterpret_cast<const T *>(f);
@@ -1432,25 +1437,29 @@ void vcycle(stg_efield *e, const T *b0, T *z0, double *part) {
         T *x = reinterpret_cast<T *>(V.x), *r = reinterpret_cast<T *>(V.r);
         if (l == 0) {
             hipLaunchKernelGGL(k_amg0_down<T>, blocks(V.n), dim3(kB), 0, e->stream, m, e->sigma, V.om, b, x, r);
-        } else {
-            hipLaunchKernelGGL(k_scale<T>, blocks(V.n), dim3(kB), 0, e->stream, V.n, pick<T>(V.dinv32, V.dinv64),
-                               V.om, b, x);
+        } else {  // x = om D^-1 b was written with b by the restriction
             GroupLaunch<T, ResidK>::run(V.ga, V.n, e->stream, V.n, (const int *)V.a_ptr, (const int *)V.a_col,
                                         pick<T>(V.a32, V.a64), (const float *)V.rs32, b, (const T *)x, r);
         }
-        const int nn = l + 1 < L ? G.lev[l + 1].n : G.nc;
-        T *bc = reinterpret_cast<T *>(l + 1 < L ? G.lev[l + 1].b : G.cb);
-        GroupLaunch<T, ApplyK>::run(V.gr, nn, e->stream, nn, (const int *)V.r_ptr, (const int *)V.r_col,
-                                    pick<T>(V.r32, V.r64), (const T *)r, bc, 0);
+        if (l + 1 < L) {
+            AmgLevel &C = G.lev[l + 1];
+            GroupLaunch<T, ApplyK>::run(V.gr, C.n, e->stream, C.n, (const int *)V.r_ptr, (const int *)V.r_col,
+                                        pick<T>(V.r32, V.r64), (const T *)r, reinterpret_cast<T *>(C.b), 0,
+                                        pick<T>(C.dinv32, C.dinv64), C.om, reinterpret_cast<T *>(C.x));
+        } else {
+            GroupLaunch<T, ApplyK>::run(V.gr, G.nc, e->stream, G.nc, (const int *)V.r_ptr, (const int *)V.r_col,
+                                        pick<T>(V.r32, V.r64), (const T *)r, reinterpret_cast<T *>(G.cb), 0,
+                                        (const T *)nullptr, 0.0, (T *)nullptr);
+        }
     }
-    hipLaunchKernelGGL(k_dense_apply<T>, blocks(G.nc), dim3(kB), 0, e->stream, G.nc, pick<T>(G.inv32, G.inv64),
-                       (const T *)G.cb, reinterpret_cast<T *>(G.cx));
+    hipLaunchKernelGGL(k_dense_apply<T>, blocks(G.nc * 16), dim3(kB), 0, e->stream, G.nc,
+                       pick<T>(G.inv32, G.inv64), (const T *)G.cb, reinterpret_cast<T *>(G.cx));
     for (int l = L - 1; l >= 0; --l) {
         AmgLevel &V = G.lev[l];
         const T *xc = reinterpret_cast<const T *>(l + 1 < L ? G.lev[l + 1].out : G.cx);
         T *x = reinterpret_cast<T *>(V.x);
         GroupLaunch<T, ApplyK>::run(V.gp, V.n, e->stream, V.n, (const int *)V.p_ptr, (const int *)V.p_col,
-                                    pick<T>(V.p32, V.p64), xc, x, 1);
+                                    pick<T>(V.p32, V.p64), xc, x, 1, (const T *)nullptr, 0.0, (T *)nullptr);
         const T *b = l == 0 ? b0 : reinterpret_cast<const T *>(V.b);
         T *out = l == 0 ? z0 : reinterpret_cast<T *>(V.out);
         if (l == 0)
