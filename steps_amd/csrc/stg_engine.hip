@@ -900,8 +900,10 @@ void specialize(stg_sim *s, int min_waves) {
     // a bound of n waves per SIMD caps the kernel at 512 / n registers per lane
     const std::string waves = std::string(memb ? "-DSTG_PERIOD_WAVES_PER_EU=" : "-DSTG_VOL_WAVES_PER_EU=") +
                               std::to_string(min_waves > 0 ? min_waves : 1);
+    // no machine-level loop-invariant motion: it holds constants and addresses
+    // in vector registers across the event loop and costs occupancy
     const char *opts[] = {arch.c_str(), "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang", "-target-feature",
-                          "-Xclang", "-packed-fp32-ops", waves.c_str()};
+                          "-Xclang", "-packed-fp32-ops", "-mllvm", "-disable-machine-licm", waves.c_str()};
     const std::string key = arch + "\n" + waves + "\n" + src;
     {
         std::lock_guard<std::mutex> lk(g_rtc_mutex);
@@ -916,7 +918,7 @@ void specialize(stg_sim *s, int min_waves) {
         if (hiprtcCreateProgram(&prog, src.c_str(), "stg_period_spec.hip", 3, hdrs, names) != HIPRTC_SUCCESS)
             throw StgError{STG_ERR_DEVICE, "hiprtcCreateProgram failed"};
         hiprtcAddNameExpression(prog, name.c_str());
-        const hiprtcResult rc = hiprtcCompileProgram(prog, 9, opts);
+        const hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
         size_t ls = 0;
         if (hiprtcGetProgramLogSize(prog, &ls) == HIPRTC_SUCCESS && ls > 1) {
             log.resize(ls);

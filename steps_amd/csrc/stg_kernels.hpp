@@ -723,8 +723,11 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 }
             }
             if (tab_ndiff<MT>(m) > 0) {
-                const double2 g01 = sys.g[2 * (size_t)t];
-                const double2 g23 = sys.g[2 * (size_t)t + 1];
+                // addresses formed here from a fenced index, so the loads stay
+                // after the SSA and the address is not held through it
+                const int tg = pin(t);
+                const double2 g01 = sys.g[2 * (size_t)tg];
+                const double2 g23 = sys.g[2 * (size_t)tg + 1];
                 const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
                 const uint32_t bdir = (MEMB && sys.tet_bdir != nullptr) ? sys.tet_bdir[t] : 0xffffffffu;
                 const double *Dr = sys.real_D + (size_t)r * S;
@@ -736,7 +739,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     int4 o;
                     moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, Dr[sp], r4, bdir, S, dt,
                                    seed, ltet, period, stream, o);
-                    oc[(size_t)d * NT + t] = o;
+                    oc[(size_t)d * NT + pin(t)] = o;
                 }
             }
             if (MEMB && m.n_sdiff > 0) {
@@ -755,7 +758,9 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 }
             }
         }
-        for (int s = 0; s < S; ++s) sys.counts[(size_t)s * NT + t] = L.c(s);
+        // store addresses formed again rather than held from the loads
+        const int tw = pin(t);
+        for (int s = 0; s < S; ++s) sys.counts[(size_t)s * NT + tw] = L.c(s);
     }
     // per-block extents: wavefront reduction, then one slot per block
     unsigned long long ev = (unsigned long long)nev, mv = (unsigned long long)moved;

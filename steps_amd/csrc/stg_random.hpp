@@ -20,6 +20,23 @@
 
 namespace stg {
 
+// Register-pressure fences. The empty asm takes its operands in vector
+// registers and hands them back as if changed: no instruction is emitted, and
+// the arithmetic is untouched, but nothing that uses the results can be
+// scheduled above this point. Used to run a chain of independent pieces one
+// after another (instead of all at once, each holding its temporaries) and to
+// keep the compiler from holding a value live across a long loop when it is
+// cheap to form again.
+template <class T>
+__device__ __forceinline__ T pin(T x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+template <class T, class U>
+__device__ __forceinline__ void pin2(T &x, U &y) {
+    asm volatile("" : "+v"(x), "+v"(y));
+}
+
 struct Philox {
     uint32_t k0, k1;
     uint32_t c0, c1, c2, c3;
@@ -198,7 +215,11 @@ __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
 // q <= 1/2 and n*q >= 30. The log-factorial bound is only formed when the
 // squeeze fails (about one trial in seven). Bounded at 1000 trials
-// (acceptance is > 0.8 per trial).
+// (acceptance is > 0.8 per trial). The logarithms of the exact test (up to
+// five, each a Stirling series above 1023) are fenced to run one at a time:
+// evaluated side by side they set the register budget of the whole period
+// kernel although this branch rarely runs. The operations and their order are
+// those of h - lfact(k) - lfact(n - k) + (k - m) lpq.
 __device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
     const double nd = (double)n;
     const double spq = dsqrt(nd * q * (1.0 - q));
@@ -221,11 +242,24 @@ __device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
         if (us >= 0.07 && v <= vr) return k;
         if (!tail) {
             lpq = dlog(q / (1.0 - q));
-            h = lfact(m) + lfact(n - m);
+            int32_t a1 = m;
+            pin2(lpq, a1);
+            h = lfact(a1);
+            int32_t a2 = n - m;
+            pin2(h, a2);
+            h = h + lfact(a2);
             tail = true;
         }
-        v = dlog(v * alpha / (a / (us * us) + b));
-        if (v <= h - lfact(k) - lfact(n - k) + (double)(k - m) * lpq) return k;
+        double vv = v * alpha / (a / (us * us) + b);
+        pin2(vv, h);
+        v = dlog(vv);
+        int32_t k1 = k;
+        pin2(v, k1);
+        double acc = h - lfact(k1);
+        int32_t k2 = n - k;
+        pin2(acc, k2);
+        acc = acc - lfact(k2);
+        if (v <= acc + (double)(k - m) * lpq) return k;
     }
     return m;
 }
