@@ -19,7 +19,7 @@ from .engine import Engine
 
 def autotune(engine: Engine, trial_periods: int = 6, blocks=(64, 128, 256, 512),
              graphs=(False, True), repeats: int = 2, specialize=(False, True),
-             spec_waves=(0, 6)) -> dict:
+             spec_waves=(0,)) -> dict:
     """Each shape runs trial_periods periods once to warm up and then `repeats`
     times timed; its figure is the fastest of those. The specialised kernel is
     tried for blocks of up to 256 when the model allows it, compiled with each

@@ -213,26 +213,29 @@ __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double 
 }
 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
-// q <= 1/2 and n*q >= 30. The log-factorial bound is only formed when the
-// squeeze fails (about one trial in seven). Bounded at 1000 trials
-// (acceptance is > 0.8 per trial). The logarithms of the exact test (up to
-// five, each a Stirling series above 1023) are fenced to run one at a time:
-// evaluated side by side they set the register budget of the whole period
-// kernel although this branch rarely runs. The operations and their order are
-// those of h - lfact(k) - lfact(n - k) + (k - m) lpq.
+// q <= 1/2 and n*q >= 30. Bounded at 1000 trials (acceptance is > 0.8 per
+// trial). The branch rarely runs, but its live values set the register budget
+// (and so the occupancy) of the whole period kernel, so it is written to hold
+// as little as possible across trials:
+//   - the setup constants (spq, b, a, c, vr, alpha, m) are formed again in
+//     every trial from (n, q), fenced so that they are not hoisted out of the
+//     loop;
+//   - the log-factorial bound of the exact test (about one trial in seven) is
+//     formed when the squeeze fails, its logarithms (up to five, each a
+//     Stirling series above 1023) fenced to run one at a time.
+// Every value is the one the plain formulas give, in the same order.
 __device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
-    const double nd = (double)n;
-    const double spq = dsqrt(nd * q * (1.0 - q));
-    const double b = 1.15 + 2.53 * spq;
-    const double ib = 1.0 / b;
-    const double a = -0.0873 + 0.0248 * b + 0.01 * q;
-    const double c = nd * q + 0.5;
-    const double alpha = (2.83 + 5.1 * ib) * spq;
-    const double vr = 0.92 - 4.2 * ib;
-    const int32_t m = (int32_t)floor((nd + 1.0) * q);
-    bool tail = false;
-    double lpq = 0.0, h = 0.0;
     for (int it = 0; it < 1000; ++it) {
+        int32_t nn = n;
+        double qq = q;
+        pin2(nn, qq);
+        const double nd = (double)nn;
+        const double spq = dsqrt(nd * qq * (1.0 - qq));
+        const double b = 1.15 + 2.53 * spq;
+        const double ib = 1.0 / b;
+        const double a = -0.0873 + 0.0248 * b + 0.01 * qq;
+        const double c = nd * qq + 0.5;
+        const double vr = 0.92 - 4.2 * ib;
         const double u = g.unif() - 0.5;
         double v = g.unif();
         const double us = 0.5 - fabs(u);
@@ -240,28 +243,29 @@ __device__ __forceinline__ int32_t binom_btrs(Philox& g, int32_t n, double q) {
         if (kd < 0.0 || kd > nd) continue;
         const int32_t k = (int32_t)kd;
         if (us >= 0.07 && v <= vr) return k;
-        if (!tail) {
-            lpq = dlog(q / (1.0 - q));
-            int32_t a1 = m;
-            pin2(lpq, a1);
-            h = lfact(a1);
-            int32_t a2 = n - m;
-            pin2(h, a2);
-            h = h + lfact(a2);
-            tail = true;
-        }
+        // exact test: log v' <= h - log k! - log (n-k)! + (k - m) log(q/(1-q))
+        const double alpha = (2.83 + 5.1 * ib) * spq;
         double vv = v * alpha / (a / (us * us) + b);
+        const int32_t m = (int32_t)floor((nd + 1.0) * qq);
+        pin2(vv, qq);
+        double lpq = dlog(qq / (1.0 - qq));
+        int32_t a1 = m;
+        pin2(lpq, a1);
+        double h = lfact(a1);
+        int32_t a2 = nn - m;
+        pin2(h, a2);
+        h = h + lfact(a2);
         pin2(vv, h);
         v = dlog(vv);
         int32_t k1 = k;
         pin2(v, k1);
         double acc = h - lfact(k1);
-        int32_t k2 = n - k;
+        int32_t k2 = nn - k;
         pin2(acc, k2);
         acc = acc - lfact(k2);
         if (v <= acc + (double)(k - m) * lpq) return k;
     }
-    return m;
+    return (int32_t)floor(((double)n + 1.0) * q);
 }
 
 __device__ __forceinline__ int32_t binomial(Philox& g, int32_t n, double p) {

@@ -187,7 +187,7 @@ def test_autotune_keeps_results():
     ref = _engine(w, 2)
     ref.run(w.end_time)
     eng = _engine(w, 2)
-    res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True))
+    res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True), spec_waves=(0, 8))
     # generic and model-specialised kernels (two register bounds), both blocks, stream and graph
     assert res["block"] in (64, 256) and len(res["timings"]) == 12
     assert eng.time() == 0.0
