@@ -266,7 +266,7 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 // GHK currents, k(V) tables or channel integrals) for volume-only models.
 template <bool MEMB, class MT>
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
-                          uint32_t ltet, uint32_t period, double dt) {
+                          uint32_t ltet, uint32_t period, double dt, uint64_t seed, uint32_t stream) {
     const int R = tab_R<MT>(m), RS = m.n_sreacs, NG = m.n_ghk;
     const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
@@ -309,7 +309,7 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
         }
     }
     Philox g;
-    g.init(sys.real_seed[r], ltet, period, sys.real_stream[r], 0u);
+    g.init(seed, ltet, period, stream, 0u);
     long nev = 0;
     double tcur = 0.0;
     while (true) {
@@ -702,13 +702,16 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
         if (do_step) {
             const int tet0 = sys.real_tet0[r];
             const uint32_t ltet = (uint32_t)(t - tet0);
-            const uint32_t period = sys.real_step0[r] + (uint32_t)launch;
-            const double dt = sys.dt_tab[(size_t)launch * sys.n_real + r];
+            // per-realization constants, in scalar registers
+            const uint32_t period = uni(sys.real_step0[r]) + (uint32_t)launch;
+            const double dt = uni(sys.dt_tab[(size_t)launch * sys.n_real + r]);
+            const uint64_t seed = uni(sys.real_seed[r]);
+            const uint32_t stream = uni(sys.real_stream[r]);
             if (MEMB && m.tri_tracked_mask != 0)
                 for (int q = 0; q < m.n_sspecs; ++q)
                     if ((m.tri_tracked_mask >> q) & 1)
                         for (int tri = tp0; tri < tp1; ++tri) sys.tri_corr[(size_t)q * sys.n_tris + tri] = 0.0;
-            nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt);
+            nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
@@ -723,35 +726,32 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 }
             }
             if (tab_ndiff<MT>(m) > 0) {
-                // addresses formed here from a fenced index, so the loads stay
-                // after the SSA and the address is not held through it
-                const int tg = pin(t);
-                const double2 g01 = sys.g[2 * (size_t)tg];
-                const double2 g23 = sys.g[2 * (size_t)tg + 1];
-                const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
                 const uint32_t bdir = (MEMB && sys.tet_bdir != nullptr) ? sys.tet_bdir[t] : 0xffffffffu;
                 const double *Dr = sys.real_D + (size_t)r * S;
-                const uint64_t seed = sys.real_seed[r];
-                const uint32_t stream = sys.real_stream[r];
                 int4 *oc = reinterpret_cast<int4 *>(sys.out_cur);
                 for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
                     const int sp = tab_dspec<MT>(m, d);
+                    // the face factors are read again for every species (cache
+                    // hits) from a fenced index: held across the species loop
+                    // they would cost 8 vector registers at its peak
+                    const int tg = pin(t);
+                    const double2 g01 = sys.g[2 * (size_t)tg];
+                    const double2 g23 = sys.g[2 * (size_t)tg + 1];
+                    const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
                     int4 o;
-                    moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, Dr[sp], r4, bdir, S, dt,
+                    moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, uni(Dr[sp]), r4, bdir, S, dt,
                                    seed, ltet, period, stream, o);
                     oc[(size_t)d * NT + pin(t)] = o;
                 }
             }
             if (MEMB && m.n_sdiff > 0) {
                 const double *sDr = sys.real_sD + (size_t)r * m.n_sspecs;
-                const uint64_t seed = sys.real_seed[r];
-                const uint32_t stream = sys.real_stream[r];
                 int4 *toc = reinterpret_cast<int4 *>(sys.tout_cur);
                 for (int tri = tp0; tri < tp1; ++tri) {
                     for (int d = 0; d < m.n_sdiff; ++d) {
                         const int q = m.sdspec[d];
                         int4 o;
-                        moved += tri_leave(sys, q, (m.tri_tracked_mask >> q) & 1, sDr[q], tri, dt, seed,
+                        moved += tri_leave(sys, q, (m.tri_tracked_mask >> q) & 1, uni(sDr[q]), tri, dt, seed,
                                            period, stream, o);
                         toc[(size_t)d * sys.n_tris + tri] = o;
                     }

@@ -37,6 +37,19 @@ __device__ __forceinline__ void pin2(T &x, U &y) {
     asm volatile("" : "+v"(x), "+v"(y));
 }
 
+// Values that are the same in every lane (per-realization constants: the
+// blocks of a launch never straddle realizations) taken into scalar registers.
+// A load issued after the kernel's first global store cannot be proven
+// unclobbered, so the compiler loads it per lane and keeps it, and everything
+// formed from it (the Philox round keys of a seed, say), in vector registers.
+__device__ __forceinline__ uint32_t uni(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
+__device__ __forceinline__ uint64_t uni(uint64_t x) {
+    return (uint64_t)uni((uint32_t)x) | ((uint64_t)uni((uint32_t)(x >> 32)) << 32);
+}
+__device__ __forceinline__ double uni(double x) {
+    return __longlong_as_double((long long)uni((uint64_t)__double_as_longlong(x)));
+}
+
 struct Philox {
     uint32_t k0, k1;
     uint32_t c0, c1, c2, c3;
