@@ -129,7 +129,8 @@ struct stg_sim {
     int32_t *d_tri_ptr = nullptr;
     int32_t *d_tri_counts = nullptr;
     double *d_tri_area = nullptr;
-    int32_t *d_block_real = nullptr, *d_block_tet0 = nullptr, *d_real_tet0 = nullptr;
+    int4 *d_block_desc = nullptr;  // [blocks] {block, realization, first tet, end tet}, launch order
+    int32_t *d_real_tet0 = nullptr;
     double *d_real_k = nullptr, *d_real_sk = nullptr, *d_real_D = nullptr;
     uint64_t *d_real_seed = nullptr;
     uint32_t *d_real_stream = nullptr, *d_real_step0 = nullptr;
@@ -169,6 +170,11 @@ struct stg_sim {
     hipGraph_t graph = nullptr;
     int graph_launches = -1;
     int graph_block = -1;
+    std::vector<int32_t> graph_counts;  // blocks per launch the graph was captured with
+    // Launch order of the blocks: by periods in the current run, most first, so
+    // the blocks still active at launch l are a prefix of launch_blocks[l].
+    std::vector<int32_t> desc_np;        // periods per block the uploaded order was made for
+    std::vector<int32_t> launch_blocks;  // [launches] blocks dispatched
     double last_ms = 0.0;
     int32_t last_launches = 0;
     std::string err;
@@ -241,7 +247,7 @@ struct stg_sim {
                 if (m) (void)hipModuleUnload(m);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
                         d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
-                        d_block_real, d_block_tet0, d_real_tet0,   d_real_k,      d_real_sk,
+                        d_block_desc, d_real_tet0,  d_real_k,      d_real_sk,
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
@@ -733,16 +739,41 @@ void build_blocks(stg_sim *s, int B) {
         }
     }
     const size_t nb = s->block_real.size();
-    if (s->d_block_real) (void)hipFree(s->d_block_real);
-    if (s->d_block_tet0) (void)hipFree(s->d_block_tet0);
+    if (s->d_block_desc) (void)hipFree(s->d_block_desc);
     if (s->d_ext) (void)hipFree(s->d_ext);
-    s->d_block_real = dalloc<int32_t>(nb);
-    s->d_block_tet0 = dalloc<int32_t>(nb);
+    s->d_block_desc = dalloc<int4>(nb);
     s->d_ext = dalloc<unsigned long long>(2 * nb);
-    HIP_OK(hipMemcpy(s->d_block_real, s->block_real.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(s->d_block_tet0, s->block_tet0.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice));
     HIP_OK(hipMemset(s->d_ext, 0, sizeof(unsigned long long) * 2 * nb));
+    s->desc_np.clear();  // the launch order is uploaded at the next run
     s->free_graph();
+}
+
+// Orders the blocks for a run in which realization r takes nper[r] periods:
+// a block takes part in launch l when its realization has a period to enter
+// (1 <= l <= np) or to step (l < np), i.e. when np >= max(l, 1). Blocks sorted
+// by np, most first (stable, so realizations keep their tet order), make the
+// participants of every launch a prefix: launch l dispatches only those,
+// rather than every block of the ensemble with most returning at once.
+void order_blocks(stg_sim *s, const std::vector<int32_t> &nper, int L) {
+    const size_t nb = s->block_real.size();
+    std::vector<int32_t> np(nb);
+    for (size_t b = 0; b < nb; ++b) np[b] = nper[s->block_real[b]];
+    std::vector<int32_t> at_least((size_t)L + 2, 0);  // blocks with np == v, then >= v
+    for (size_t b = 0; b < nb; ++b) at_least[np[b]] += 1;
+    for (int v = L - 1; v >= 0; --v) at_least[v] += at_least[v + 1];
+    s->launch_blocks.assign((size_t)L + 1, 0);
+    for (int l = 0; l <= L; ++l) s->launch_blocks[l] = at_least[std::max(l, 1)];
+    if (np == s->desc_np) return;
+    std::vector<int32_t> pos((size_t)L + 2, 0);  // first slot of each np value
+    for (int v = L - 1; v >= 0; --v) pos[v] = at_least[v + 1];
+    std::vector<int4> desc(nb);
+    for (size_t b = 0; b < nb; ++b) {
+        const int r = s->block_real[b];
+        desc[pos[np[b]]++] = make_int4((int)b, r, s->block_tet0[b], s->tet_off[r + 1]);
+    }
+    HIP_OK(hipMemcpyAsync(s->d_block_desc, desc.data(), sizeof(int4) * nb, hipMemcpyHostToDevice, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));  // the host vector goes out of scope
+    s->desc_np.swap(np);
 }
 
 stg::SysK make_sys(const stg_sim *s, int launch_parity) {
@@ -751,7 +782,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.n_tris = s->NTRI;
     k.n_real = s->n_real;
     k.xcd_remap = s->xcd_remap ? 1 : 0;
-    k.n_blocks = (int32_t)s->block_real.size();
+    k.n_blocks = (int32_t)s->block_real.size();  // set per launch by launch_all
     k.diff_threshold = s->diff_threshold;
     k.k_max = std::max(1, s->R + s->max_tris * (s->RS + s->NG));
     k.vol = s->d_vol;
@@ -764,8 +795,7 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.tri_ptr = s->NTRI > 0 ? s->d_tri_ptr : nullptr;
     k.tri_counts = s->d_tri_counts;
     k.tri_area = s->d_tri_area;
-    k.block_real = s->d_block_real;
-    k.block_tet0 = s->d_block_tet0;
+    k.block_desc = s->d_block_desc;
     k.real_tet0 = s->d_real_tet0;
     k.real_k = s->d_real_k;
     k.real_sk = s->d_real_sk;
@@ -962,14 +992,16 @@ void unspecialize(stg_sim *s) {
 
 void launch_all(stg_sim *s, int L) {
     const int B = s->block;
-    const int nb = (int)s->block_real.size();
     const size_t smem = s->smem_bytes(B);
     const PeriodKernel kern = period_kernel(s, B);
     // a volume-only module cannot serve a model that has come to need the
     // membrane variant (e.g. channel integrals requested by an EField)
     const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
     for (int l = 0; l <= L; ++l) {
+        const int nb = s->launch_blocks[l];
+        if (nb == 0) continue;
         stg::SysK k = make_sys(s, l & 1);
+        k.n_blocks = nb;
         if (spec) {
             int launch = l;
             void *args[] = {(void *)&s->mk, (void *)&k, (void *)&launch};
@@ -1017,13 +1049,14 @@ void run_impl(stg_sim *s, double endtime) {
         HIP_OK(hipMemcpyAsync(s->d_dt_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s->stream));
         HIP_OK(hipMemcpyAsync(s->d_real_nper, nper.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
         HIP_OK(hipMemcpyAsync(s->d_real_step0, step0.data(), sizeof(uint32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
+        order_blocks(s, nper, L);
         const size_t smem = s->smem_bytes(s->block);
         if (smem > 64 * 1024)
             HIP_OK(hipFuncSetAttribute((const void *)period_kernel(s, s->block),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
         HIP_OK(hipEventRecord(s->ev0, s->stream));
         if (s->use_graph) {
-            if (s->graph_launches != L || s->graph_block != s->block) {
+            if (s->graph_launches != L || s->graph_block != s->block || s->graph_counts != s->launch_blocks) {
                 s->free_graph();
                 HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
                 launch_all(s, L);
@@ -1031,6 +1064,7 @@ void run_impl(stg_sim *s, double endtime) {
                 HIP_OK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
                 s->graph_launches = L;
                 s->graph_block = s->block;
+                s->graph_counts = s->launch_blocks;
             }
             HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
         } else {

@@ -113,7 +113,7 @@ struct SysK {
     int32_t n_tris;
     int32_t n_real;
     int32_t xcd_remap;     // nonzero: contiguous block ranges per XCD
-    int32_t n_blocks;
+    int32_t n_blocks;          // blocks in this launch
     int32_t diff_threshold;      // leaving counts up to this are split per molecule
     int32_t k_max;               // most reactions of one tet group (LDS rows per thread)
     const double *vol;           // [NT]
@@ -126,8 +126,8 @@ struct SysK {
     const int32_t *tri_ptr;      // [NT+1] or null when there are no triangles
     int32_t *tri_counts;         // [SS][NTRI]
     const double *tri_area;      // [NTRI]
-    const int32_t *block_real;   // [n_blocks]
-    const int32_t *block_tet0;   // [n_blocks]
+    const int4 *block_desc;      // [launched blocks] {block id (extents slot), realization,
+                                 // first tet, end tet of the realization}
     const int32_t *real_tet0;    // [R+1]
     const double *real_k;        // [R][n_reacs]
     const double *real_sk;       // [R][n_sreacs]
@@ -137,7 +137,7 @@ struct SysK {
     const uint32_t *real_step0;  // [R] period counter at launch 0
     const int32_t *real_nper;    // [R] periods in this run
     const double *dt_tab;        // [L][R]
-    unsigned long long *ext;     // [n_blocks][2] events, molecules moved
+    unsigned long long *ext;     // [blocks][2] events, molecules moved, by block id
     int32_t *err;                // error flags: 1 event cap, 2 potential outside a k(V) table,
                                  // 4 GHK exponent out of range
     const double *tri_v;         // [NTRI] membrane potential of each triangle (V)
@@ -630,7 +630,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
         const int lb = (x < full) ? x * per + i : full * per + (x - full) * (per - 1) + i;
         b = lb;
     }
-    const int r = sys.block_real[b];
+    const int4 bd = sys.block_desc[b];
+    const int r = bd.y;
     const int np = sys.real_nper[r];
     const bool do_enter = launch >= 1 && launch <= np;
     const bool do_step = launch < np;
@@ -646,8 +647,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     L.a = L.kc + (size_t)sys.k_max * B;
     L.gx = MEMB ? L.a + (size_t)sys.k_max * B : nullptr;
 
-    const int t = sys.block_tet0[b] + tid;
-    const int tend = sys.real_tet0[r + 1];
+    const int t = bd.z + tid;
+    const int tend = bd.w;
     const bool valid = t < tend;
     long nev = 0;
     int32_t moved = 0;
@@ -781,8 +782,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             se += red[0][w];
             sm += red[1][w];
         }
-        sys.ext[2 * (size_t)b] += se;
-        sys.ext[2 * (size_t)b + 1] += sm;
+        sys.ext[2 * (size_t)bd.x] += se;
+        sys.ext[2 * (size_t)bd.x + 1] += sm;
     }
 }
 
