@@ -194,6 +194,12 @@ struct stg_sim {
         const size_t rows = 2 * (size_t)K + (needs_memb() ? 2 * (size_t)max_tris * NG : 0);
         return off + (size_t)S * B * 8 + rows * B * 8;
     }
+    // the model-specialised volume kernel keeps only counts and occupancy
+    // corrections in LDS (rate constants and propensities are formed in registers)
+    size_t smem_bytes_lean(int B) const {
+        const size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
+        return off + (size_t)S * B * 8;
+    }
 
     double compute_dt_std(int r) const {
         double mx = 0.0;
@@ -992,11 +998,11 @@ void unspecialize(stg_sim *s) {
 
 void launch_all(stg_sim *s, int L) {
     const int B = s->block;
-    const size_t smem = s->smem_bytes(B);
     const PeriodKernel kern = period_kernel(s, B);
     // a volume-only module cannot serve a model that has come to need the
     // membrane variant (e.g. channel integrals requested by an EField)
     const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
+    const size_t smem = (spec && !s->rtc_memb) ? s->smem_bytes_lean(B) : s->smem_bytes(B);
     for (int l = 0; l <= L; ++l) {
         const int nb = s->launch_blocks[l];
         if (nb == 0) continue;

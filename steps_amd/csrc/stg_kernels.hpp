@@ -454,6 +454,90 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
     return nev;
 }
 
+// Direct-method SSA of one tet for the model-specialised volume kernel. The
+// draws and the floating-point operations are those of ssa_group, in the same
+// order; what differs is where the intermediate values live:
+//   - a reaction's rate constant c = k scale^-(order-1) is formed when it is
+//     used, from k in scalar registers and the tet's volume scale, instead of
+//     being stored per tet;
+//   - the selection forms the propensities again (the counts have not moved
+//     since the sum was taken) instead of reading them back.
+// Its only per-thread LDS is then the counts and occupancy corrections (48 B
+// for four species, against 144 B with the two K-row tables), so LDS no longer
+// caps the kernel at four wavefronts per SIMD.
+template <class MT>
+__device__ long ssa_lean(const SysK &sys, const Lds &L, int r, int t, uint32_t ltet, uint32_t period,
+                         double dt, uint64_t seed, uint32_t stream) {
+    constexpr int R = MT::R;
+    if constexpr (R == 0) {
+        return 0;
+    } else {
+        const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
+        const double ivscale = 1.0 / vscale;
+        double kq[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) kq[q] = uni(sys.real_k[(size_t)r * R + q]);
+        Philox g;
+        g.init(seed, ltet, period, stream, 0u);
+        long nev = 0;
+        double tcur = 0.0;
+        while (true) {
+            asm volatile("" ::: "memory");  // counts are read from LDS in every event
+            double a0 = 0.0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double h = 1.0;
+                bool zero = false;
+#pragma unroll
+                for (int e = MT::lhs_off[q]; e < MT::lhs_off[q + 1]; ++e) {
+                    const int32_t n = L.c(MT::lhs_sp[e]);
+                    if (MT::lhs_mult[e] > n) zero = true;
+                    else h = falling(h, n, MT::lhs_mult[e]);
+                }
+                a0 = a0 + (zero ? 0.0 : h * scaled_const(kq[q], MT::order[q], vscale, ivscale));
+            }
+            if (a0 < kDblEpsilon) break;
+            const double u1 = g.unif();
+            if (1.0 - u1 > a0 * (dt - tcur) * 1.0000001) break;
+            const double u2 = g.unif();
+            const double tev = tcur + (-dlog(u1) / a0);
+            if (tev > dt) break;
+            tcur = tev;
+            const double x = u2 * a0;
+            int sel = -1, last = -1;
+            double acc = 0.0;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                double h = 1.0;
+                bool zero = false;
+#pragma unroll
+                for (int e = MT::lhs_off[q]; e < MT::lhs_off[q + 1]; ++e) {
+                    const int32_t n = L.c(MT::lhs_sp[e]);
+                    if (MT::lhs_mult[e] > n) zero = true;
+                    else h = falling(h, n, MT::lhs_mult[e]);
+                }
+                const double pq = zero ? 0.0 : h * scaled_const(kq[q], MT::order[q], vscale, ivscale);
+                acc = acc + pq;
+                if (sel < 0 && acc > x) sel = q;
+                if (pq > 0.0) last = q;
+            }
+            if (sel < 0) sel = last;
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                if (sel != q) continue;
+#pragma unroll
+                for (int e = MT::upd_off[q]; e < MT::upd_off[q + 1]; ++e) {
+                    const int sp = MT::upd_sp[e], dv = MT::upd_dv[e];
+                    L.c(sp) += dv;
+                    if ((MT::tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
+                }
+            }
+            if (++nev >= kMaxEventsPerPeriod) return -1;
+        }
+        return nev;
+    }
+}
+
 __device__ __forceinline__ bool almost_zero_dt(double dt) {
     // util::almost_equal(dt, 0.0) of src/steps/util/collections.hpp:143-149
     const double ad = fabs(dt);
@@ -712,7 +796,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 for (int q = 0; q < m.n_sspecs; ++q)
                     if ((m.tri_tracked_mask >> q) & 1)
                         for (int tri = tp0; tri < tp1; ++tri) sys.tri_corr[(size_t)q * sys.n_tris + tri] = 0.0;
-            nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
+            if constexpr (MT::kStatic && !MEMB) nev = ssa_lean<MT>(sys, L, r, t, ltet, period, dt, seed, stream);
+            else nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
