@@ -134,7 +134,7 @@ class TetOpSplit:
         # the generic kernel stays.
         if geom.countTets() >= SPECIALIZE_MIN_TETS:
             try:
-                self.eng.set_specialize(True, 4)
+                self.eng.set_specialize(True)
             except N.StgError:
                 pass
         # diffusion boundaries: open directions and directional constants
