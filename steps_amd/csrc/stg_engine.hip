@@ -125,7 +125,8 @@ struct stg_sim {
     uint32_t *d_nface = nullptr;
     double2 *d_g = nullptr;
     int32_t *d_counts = nullptr;
-    int32_t *d_out[2] = {nullptr, nullptr};
+    int16_t *d_out[2] = {nullptr, nullptr};   // 16-bit face slots (ping-pong)
+    int32_t *d_wide[2] = {nullptr, nullptr};  // escaped slot counts
     int32_t *d_tri_ptr = nullptr;
     int32_t *d_tri_counts = nullptr;
     double *d_tri_area = nullptr;
@@ -252,7 +253,7 @@ struct stg_sim {
             for (hipModule_t m : row)
                 if (m) (void)hipModuleUnload(m);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
-                        d_out[0],    d_out[1],      d_tri_ptr,     d_tri_counts,  d_tri_area,
+                        d_out[0],    d_out[1],      d_wide[0],     d_wide[1],     d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_desc, d_real_tet0,  d_real_k,      d_real_sk,
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
                         d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
@@ -662,8 +663,10 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     s->d_nface = dalloc<uint32_t>(NT);
     s->d_g = dalloc<double2>((size_t)NT * 2);
     s->d_counts = dalloc<int32_t>((size_t)std::max(S, 1) * NT);
-    s->d_out[0] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
-    s->d_out[1] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_out[0] = dalloc<int16_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_out[1] = dalloc<int16_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_wide[0] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
+    s->d_wide[1] = dalloc<int32_t>((size_t)std::max(s->ND, 1) * NT * 4);
     s->d_tri_ptr = dalloc<int32_t>(NT + 1);
     s->d_tri_counts = dalloc<int32_t>((size_t)std::max(SS, 1) * std::max(NTRI, 1));
     s->d_tri_area = dalloc<double>(std::max(NTRI, 1));
@@ -798,6 +801,8 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.counts = s->d_counts;
     k.out_prev = s->d_out[launch_parity ^ 1];
     k.out_cur = s->d_out[launch_parity];
+    k.wide_prev = s->d_wide[launch_parity ^ 1];
+    k.wide_cur = s->d_wide[launch_parity];
     k.tri_ptr = s->NTRI > 0 ? s->d_tri_ptr : nullptr;
     k.tri_counts = s->d_tri_counts;
     k.tri_area = s->d_tri_area;
@@ -919,6 +924,7 @@ void specialize(stg_sim *s, int min_waves) {
     const std::string name = std::string("stg::k_period<256, ") + (memb ? "true" : "false") + ", StgModelTables>";
     std::string src =
         "using __hip_internal::int8_t; using __hip_internal::uint8_t;\n"
+        "using __hip_internal::int16_t; using __hip_internal::uint16_t;\n"
         "using __hip_internal::int32_t; using __hip_internal::uint32_t;\n"
         "using __hip_internal::int64_t; using __hip_internal::uint64_t;\n"
         "#include \"stg_kernels.hpp\"\n" +

@@ -121,8 +121,10 @@ struct SysK {
     const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
     const double2 *g;            // [NT*2] face factors (f0,f1),(f2,f3)
     int32_t *counts;             // [S][NT]
-    const int32_t *out_prev;     // [ND][NT][4]
-    int32_t *out_cur;            // [ND][NT][4]
+    const int16_t *out_prev;     // [ND][NT][4] molecules leaving through each face; a count
+    int16_t *out_cur;            // above kSlotMax is kSlotEscape here and lives in wide_*
+    const int32_t *wide_prev;    // [ND][NT][4] escaped face counts (touched only on escape)
+    int32_t *wide_cur;
     const int32_t *tri_ptr;      // [NT+1] or null when there are no triangles
     int32_t *tri_counts;         // [SS][NTRI]
     const double *tri_area;      // [NTRI]
@@ -163,6 +165,29 @@ struct SysK {
 
 // both structures travel in the 4 KiB kernel-argument segment
 static_assert(sizeof(ModelK) + sizeof(SysK) + 16 <= 4096, "kernel arguments exceed 4 KiB");
+
+// Face slots are 16-bit: a period's molecules through one face rarely exceed a
+// few hundred, and halving the slots halves the bytes of the only per-face
+// traffic (written by the owner, gathered by the neighbour). A count above
+// kSlotMax is stored as kSlotEscape with the count itself in a 32-bit side
+// table at the same index, which is read only when the escape is seen.
+constexpr int32_t kSlotMax = 32767;
+constexpr int32_t kSlotEscape = -32768;
+
+__device__ __forceinline__ void put_slots(int16_t *slots, int32_t *wide, size_t base, int4 o) {
+    const int32_t v[4] = {o.x, o.y, o.z, o.w};
+    uint32_t h[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+        if (v[f] > kSlotMax) {
+            wide[base + f] = v[f];
+            h[f] = (uint32_t)(uint16_t)(int16_t)kSlotEscape;
+        } else {
+            h[f] = (uint32_t)(uint16_t)(int16_t)v[f];
+        }
+    }
+    *reinterpret_cast<uint2 *>(slots + base) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+}
 
 // c = k scale^-(order-1) (comp_ccst, src/steps/mpi/tetopsplit/reac.cpp:44-55),
 // as a product with powers of scale or of inv = 1 / scale: no division per call
@@ -753,13 +778,16 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             for (int f = 0; f < 4; ++f)
                 off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
             for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
-                const int32_t *op = sys.out_prev + (size_t)d * NT * 4;
+                const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
                 int32_t v[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) v[f] = op[off[f]];
                 int32_t add = 0;
 #pragma unroll
-                for (int f = 0; f < 4; ++f) add += nbv[f] >= 0 ? v[f] : 0;
+                for (int f = 0; f < 4; ++f) {
+                    if (v[f] == kSlotEscape) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
+                    add += nbv[f] >= 0 ? v[f] : 0;
+                }
                 L.c(tab_dspec<MT>(m, d)) += add;
             }
         }
@@ -814,7 +842,6 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             if (tab_ndiff<MT>(m) > 0) {
                 const uint32_t bdir = (MEMB && sys.tet_bdir != nullptr) ? sys.tet_bdir[t] : 0xffffffffu;
                 const double *Dr = sys.real_D + (size_t)r * S;
-                int4 *oc = reinterpret_cast<int4 *>(sys.out_cur);
                 for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
                     const int sp = tab_dspec<MT>(m, d);
                     // the face factors are read again for every species (cache
@@ -827,7 +854,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     int4 o;
                     moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, uni(Dr[sp]), r4, bdir, S, dt,
                                    seed, ltet, period, stream, o);
-                    oc[(size_t)d * NT + pin(t)] = o;
+                    put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
                 }
             }
             if (MEMB && m.n_sdiff > 0) {

@@ -59,6 +59,27 @@ def test_ab_binding_bit_exact(seed, mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+def test_escaped_face_slots_bit_exact(mode):
+    """Face slots are 16-bit, with counts above 32,767 escaped to a 32-bit side
+    table. About 400,000 molecules per tet (no reactions) send more than 32,767
+    through most faces in every period, so most gathers take the escape; a few
+    faces stay below it. Counts must equal the oracle's, and the total is kept."""
+    w = W.ab_binding_cube(n=2, side=2e-6, n_a=10_000_000, n_b=10_000_000, seed=6,
+                          end_time=0.15, kf=0.0, kb=0.0)
+    print(f"case escaped face slots mode={mode}", flush=True)
+    eng = _engine(w, 31, mode)
+    o = _oracle(w, 31)
+    for t in (0.05, 0.15):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+    assert eng.nsteps() == o.nsteps and eng.nsteps() >= 4
+    tot = o.counts.sum(axis=1)
+    assert tot[0] == 10_000_000 and tot[1] == 10_000_000 and tot[2] == 0
+    assert eng.extents()[1] == o.moved.value and o.moved.value > 100 * 32767
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_four_species_bit_exact(mode):
     w = W.four_species_cube(n=6, side=6e-6, seed=5, end_time=0.05)
     print(f"case four_species n=6 mode={mode}", flush=True)
