@@ -80,6 +80,40 @@ def test_escaped_face_slots_bit_exact(mode):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
+def test_irregular_mesh_bit_exact(mode):
+    """A Kuhn cube with jittered interior vertices: irregular geometry, every tet
+    with its own volume and face factors (the other cases use regular cubes,
+    whose tets fall into a few shape classes)."""
+    from steps_amd import geom, model as M
+
+    n, side = 5, 5e-6
+    mesh = geom.kuhn_cube(n, side)
+    v = mesh.verts.copy()
+    h = side / n
+    inner = np.all((v > 0.5 * h) & (v < side - 0.5 * h), axis=1)
+    v[inner] += np.random.default_rng(8).uniform(-0.2 * h, 0.2 * h, size=(inner.sum(), 3))
+    mesh = geom.Tetmesh(v, mesh.tets)
+    comp = geom.TmComp("cyto", mesh)
+    comp.addVolsys("vsys")
+    fm = M.flatten(W.ab_binding_model(), ["vsys"])
+    mt = geom.mesh_tables(mesh, comp)
+    assert len(np.unique(mt.vol)) > len(mt.vol) // 2
+    rng = np.random.default_rng(3)
+    counts = np.zeros((3, len(mt.vol)), dtype=np.int32)
+    counts[0] = W.place_uniform(4000, mt.vol, rng)
+    counts[1] = W.place_uniform(3000, mt.vol, rng)
+    w = W.Workload("jittered", fm, mt, counts, np.zeros((0, 0), np.int32), 0.2)
+    print(f"case irregular mesh mode={mode}", flush=True)
+    eng = _engine(w, 17, mode)
+    o = _oracle(w, 17)
+    for t in (0.05, 0.2):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, w), o.counts), f"t={t}"
+    assert eng.extents() == (o.events.value, o.moved.value)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
 def test_four_species_bit_exact(mode):
     w = W.four_species_cube(n=6, side=6e-6, seed=5, end_time=0.05)
     print(f"case four_species n=6 mode={mode}", flush=True)
