@@ -196,10 +196,13 @@ struct stg_sim {
         return off + (size_t)S * B * 8 + rows * B * 8;
     }
     // the model-specialised volume kernel keeps only counts and occupancy
-    // corrections in LDS (rate constants and propensities are formed in registers)
-    size_t smem_bytes_lean(int B) const {
+    // corrections in LDS (rate constants and propensities are formed in
+    // registers); the membrane one adds the triangles' constant rows and the
+    // GHK rows
+    size_t smem_bytes_lean(int B, bool memb) const {
         const size_t off = ((size_t)S * B * 4 + 15) & ~(size_t)15;
-        return off + (size_t)S * B * 8;
+        const size_t rows = memb ? (size_t)max_tris * (RS + NG) + 2 * (size_t)max_tris * NG : 0;
+        return off + (size_t)S * B * 8 + rows * B * 8;
     }
 
     double compute_dt_std(int r) const {
@@ -1008,7 +1011,7 @@ void launch_all(stg_sim *s, int L) {
     // a volume-only module cannot serve a model that has come to need the
     // membrane variant (e.g. channel integrals requested by an EField)
     const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
-    const size_t smem = (spec && !s->rtc_memb) ? s->smem_bytes_lean(B) : s->smem_bytes(B);
+    const size_t smem = spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
     for (int l = 0; l <= L; ++l) {
         const int nb = s->launch_blocks[l];
         if (nb == 0) continue;
