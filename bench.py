@@ -4,12 +4,18 @@ A step is one operator-split period (SSA + diffusion leaving + entering) over
 every tet of the workload, with inputs already resident in HBM. The default
 workload is the 4-species / 6-reaction model on a 40^3 x 6 Kuhn cube of side
 40 um (384,000 tets) at the diffusion stability bound (2.96 ms per period);
-338 steps are 1 s of simulated time. The JSON line also carries the batched
-ensemble (1,024 realizations of A + B <-> C on cubes of 1,296..48,000 tets,
-0.1 s each) and the same A + B <-> C model run as a single realization, so
-batched and plain throughput are measured on the same GPU in the same run.
+338 steps are 1 s of simulated time. The JSON line also carries:
+  - the batched ensemble (1,024 realizations of A + B <-> C on cubes of
+    1,296..48,000 tets, 0.1 s each) and the same model run as one realization,
+    so batched and plain throughput are measured on the same GPU in one run;
+  - the 6,000,000-tet calcium membrane cube (pumps, influx on one face) and its
+    EField on 1,030,301 vertices (mixed precision vs FP64 CG);
+  - the 3,072-tet A + B <-> C protocol (run() every 1 ms to 0.1 s, totals
+    read at every save) over 100 seeds;
+  - the CPU oracle on bounded samples of these workloads on the host cores.
 
 Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-batched]
+                       [--no-membrane]
 """
 from __future__ import annotations
 
@@ -60,14 +66,17 @@ def model_bytes(w) -> float:
 
 
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "k_period_four_species_cube_384k_hbm_traffic.md")
+ENSEMBLE_TRAFFIC = os.path.join(ROOT, "profiles", "k_period_ab_binding_ensemble_1024_hbm_traffic.md")
+MEMBRANE_TRAFFIC = os.path.join(ROOT, "profiles", "k_period_calcium_membrane_6M_hbm_traffic.md")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the period kernel on the default workload, from the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the copy calibration)."""
+def measured_traffic(path=TRAFFIC_SUMMARY, key="traffic_bytes_per_launch"):
+    """HBM bytes of the period kernel from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy calibration): per launch of the
+    plain workloads, per tet-step of the ensemble."""
     try:
-        for line in open(TRAFFIC_SUMMARY):
-            if line.startswith("traffic_bytes_per_launch:"):
+        for line in open(path):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass
@@ -143,13 +152,18 @@ def time_membrane(w, steps=300, warmup=5):
     """Calcium buffering with membrane pumps and influx on the 100^3 x 6 cube."""
     p = time_plain(w, steps, warmup)
     btet = model_bytes(w)
+    achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
+    traffic = measured_traffic(MEMBRANE_TRAFFIC)
     return dict(
         workload="calcium_membrane_cube: 100^3 x 6 Kuhn cube, side 20 um, 6,000,000 tets, "
                  "120,000 membrane triangles, Ca + Buf <-> CaB, pumps, influx on one face",
         tet_steps_per_s=p["tet_steps"] / p["wall"], periods=p["periods"], period_s=p["dt"],
+        ms_per_step=p["wall"] * 1e3 / max(p["periods"], 1),
         kernel_ms_per_launch=p["ms_launch"], bytes_per_tet_step=btet,
-        achieved_gbs=btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9,
-        frac=btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
+        traffic=traffic,
+        traffic_frac=(traffic / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
+        schedule={"block": p["block"], "graph": bool(p["graph"]), "specialized_kernel": bool(p["specialized"])})
 
 
 def time_efield(w, ef_steps=10, ef_dt=1e-5):
@@ -221,8 +235,18 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
         jacobi_iterations=it_j["iterations"])
 
 
-def cpu_baseline(w, budget_s=15.0, seed=1):
-    """The CPU oracle on this host, a bounded number of periods of the same workload."""
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_periods(w, budget_s, seed=1):
+    """Periods of w on the CPU oracle within about budget_s: (tet-steps/s, periods, wall)."""
     from oracle import bind
 
     o = bind.OracleSim(w.fm, w.mt, seed=seed)
@@ -237,10 +261,87 @@ def cpu_baseline(w, budget_s=15.0, seed=1):
     tic = time.perf_counter()
     o.run_periods([dt] * n)
     wall = time.perf_counter() - tic
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=n * w.n_tets / wall, unit="tet-steps/s", cores=threads, kind="port",
-                sample=f"{n} periods x {w.n_tets} tets of the same workload on the CPU oracle "
-                       f"(OpenMP, {threads} threads), {wall:.1f} s")
+    return n * w.n_tets / wall, n, wall
+
+
+def config0_protocol_oracle(n_seeds, budget_s):
+    """3,072-tet A + B <-> C (10,000 A and 10,000 B placed by seed), run() every
+    1 ms to 0.1 s with the species totals read at every save, on the CPU
+    oracle; seeds 0, 1, ... until n_seeds or the budget."""
+    from oracle import bind
+
+    done, steps, tic = 0, 0, time.perf_counter()
+    for seed in range(n_seeds):
+        w = W.ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=seed)
+        o = bind.OracleSim(w.fm, w.mt, seed=seed)
+        o.counts[:] = w.counts
+        for k in range(1, 101):
+            o.run(k * 1e-3, 0)
+            o.counts.sum(axis=1)
+        done += 1
+        steps += o.nsteps * w.n_tets
+        if time.perf_counter() - tic > budget_s:
+            break
+    wall = time.perf_counter() - tic
+    return dict(tet_steps_per_s=steps / wall, seeds=done, wall_s=wall, periods_per_seed=100)
+
+
+def config0_protocol_gpu(n_seeds=100):
+    """The same protocol on the GPU: one engine per seed run one after another
+    (plain), and all seeds as one batched engine (each seed a realization with its
+    own Philox key); totals reduced on the device at every 1 ms save."""
+    ws = [W.ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=sd) for sd in range(n_seeds)]
+    eng = Engine(ws[0].fm, [ws[0].mt], [0])
+    tic = time.perf_counter()
+    steps = 0
+    for sd, w in enumerate(ws[:10]):
+        eng.reset()
+        eng.set_seed(sd)
+        for s_ in range(3):
+            eng.set_tet_counts(s_, w.counts[s_])
+        for k in range(1, 101):
+            eng.run(k * 1e-3)
+            [eng.sum_tet_counts(s_) for s_ in range(3)]
+        steps += eng.nsteps() * w.n_tets
+    plain = steps / (time.perf_counter() - tic)
+    beng = Engine(ws[0].fm, [w.mt for w in ws], list(range(n_seeds)), streams=np.zeros(n_seeds))
+    for r, w in enumerate(ws):
+        for s_ in range(3):
+            beng.set_tet_counts(s_, w.counts[s_], real=r)
+    tic = time.perf_counter()
+    for k in range(1, 101):
+        beng.run(k * 1e-3)
+        [beng.sum_tet_counts(s_, r) for r in range(n_seeds) for s_ in range(3)]
+    wall = time.perf_counter() - tic
+    steps = sum(beng.nsteps(r) * w.n_tets for r, w in enumerate(ws))
+    return dict(plain_one_seed_at_a_time_tet_steps_per_s=plain, plain_seeds=10,
+                batched_tet_steps_per_s=steps / wall, batched_seeds=n_seeds, batched_wall_s=wall)
+
+
+def cpu_baseline(w, budget_s=15.0, seed=1, membrane=None, n_seeds0=100):
+    """The CPU oracle on this host: a bounded number of periods of the headline
+    workload (the reported value), and bounded samples of the 3,072-tet
+    protocol, the membrane cube and a loop over ensemble realizations."""
+    from oracle import bind
+
+    threads = bind.threads()
+    value, n, wall = oracle_periods(w, budget_s, seed)
+    out = dict(value=value, unit="tet-steps/s", cores=threads, kind="port", cpu_model=cpu_model(),
+               sample=f"{n} periods x {w.n_tets} tets of the same workload on the CPU oracle "
+                      f"(OpenMP, {threads} threads), {wall:.1f} s")
+    out["config0_protocol"] = config0_protocol_oracle(n_seeds0, budget_s / 2)
+    if membrane is not None:
+        v, n, wall = oracle_periods(membrane, budget_s / 3)
+        out["membrane_cube"] = dict(tet_steps_per_s=v, periods=n, wall_s=wall)
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=16, seed=0)
+    tic, steps = time.perf_counter(), 0
+    for r in range(16):
+        o = bind.OracleSim(fm, meshes[r], seed=1, stream=r, reac_k=reac_k[r])
+        o.counts[:] = counts[r]
+        o.run(0.1, 0)
+        steps += o.nsteps * len(meshes[r].vol)
+    out["ensemble_loop"] = dict(tet_steps_per_s=steps / (time.perf_counter() - tic), realizations=16)
+    return out
 
 
 def main():
@@ -258,8 +359,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--membrane", action="store_true",
-                    help="also time the 6,000,000-tet membrane cube and its EField")
+    ap.add_argument("--no-membrane", action="store_true",
+                    help="skip the 6,000,000-tet membrane cube and its EField")
+    ap.add_argument("--no-config0", action="store_true",
+                    help="skip the 3,072-tet run-every-1-ms protocol")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -287,6 +390,7 @@ def main():
     value = p["tet_steps"] / p["wall"]
     # roofline of the period kernel: algorithmic bytes per launch / mean launch time
     achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
+    traffic = measured_traffic()
     out = {
         "metric": METRIC,
         "value": value,
@@ -320,11 +424,14 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(),
+            "traffic": traffic,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
+            "traffic_frac": (traffic / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
             "algorithmic_bytes_per_launch": btet * w.n_tets,
             "bytes_per_tet_step": btet,
             "kernel_ms_per_launch": p["ms_launch"],
+            "kernel_timing": "HIP events around the full-period launches of the timed run (the "
+                             "trailing entering-only pass is a separate kernel, not included)",
         },
     }
     if not args.no_batched:
@@ -332,6 +439,8 @@ def main():
         ps = time_plain(small, 9, 1)
         ens = time_ensemble()
         sched = ens["tuned"] or {}
+        ens_traffic = measured_traffic(ENSEMBLE_TRAFFIC, "traffic_bytes_per_tet_step")
+        kern_s = ens["ms_launch"] * ens["launches"] * 1e-3
         out["batched"] = {
             "workload": "ab_binding_ensemble: 1,024 realizations, Kuhn cubes n=6..20 "
                         "(1,296..48,000 tets), kf/kb log-uniform +-1 decade, 0.1 s each",
@@ -339,8 +448,10 @@ def main():
             "n_tets": ens["n_tets"],
             "kernel_ms_per_launch": ens["ms_launch"],
             "launches": ens["launches"],
-            "achieved_gbs": ens["bytes_per_tet_step"] * ens["tet_steps"] /
-                            (ens["ms_launch"] * ens["launches"] * 1e-3) / 1e9,
+            "achieved_gbs": ens["bytes_per_tet_step"] * ens["tet_steps"] / kern_s / 1e9,
+            "frac": ens["bytes_per_tet_step"] * ens["tet_steps"] / kern_s / 1e9 / HBM_PEAK_GBS,
+            "traffic_bytes_per_tet_step": ens_traffic,
+            "traffic_frac": (ens_traffic * ens["tet_steps"] / kern_s / 1e9 / HBM_PEAK_GBS) if ens_traffic else None,
             "conserved": ens["conserved"],
             "plain_same_model_tet_steps_per_s": ps["tet_steps"] / ps["wall"],
             "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
@@ -352,12 +463,22 @@ def main():
         }
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])
-    if args.membrane:
+        out["batched"]["batched_over_plain_384k_cube"] = out["batched"]["tet_steps_per_s"] / value
+        print(f"bench: batched {out['batched']['tet_steps_per_s']:.3e} tet-steps/s", file=sys.stderr, flush=True)
+    wm = None
+    if not args.no_membrane:
         wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0, device=True)
         out["membrane"] = time_membrane(wm)
+        print(f"bench: membrane {out['membrane']['tet_steps_per_s']:.3e} tet-steps/s", file=sys.stderr, flush=True)
         out["efield"] = time_efield(wm)
+        print(f"bench: efield {out['efield']['ms_per_ef_step']:.2f} ms per step", file=sys.stderr, flush=True)
+    if not args.no_config0:
+        out["config0_protocol"] = config0_protocol_gpu()
+        out["config0_protocol"]["workload"] = ("ab_binding_cube: 8^3 x 6 Kuhn cube, side 10 um, 3,072 tets, "
+                                               "10,000 A + 10,000 B by seed, run() every 1 ms to 0.1 s, "
+                                               "totals read at every save")
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(w, args.cpu_budget, membrane=wm)
     print(json.dumps(out))
 
 

@@ -151,18 +151,26 @@ int stg_destroy(stg_sim *sim);
  * sim is NULL). Never NULL. */
 const char *stg_last_error(const stg_sim *sim);
 
-/* TetOpSplitP::reset (tetopsplit.cpp:1436-1503): counts 0, time 0, steps 0. */
+/* TetOpSplitP::reset (tetopsplit.cpp:1436-1503): counts 0, time 0, steps 0.
+ * The random streams are not rewound: as the reference's RNG keeps running
+ * through a reset, each realization's Philox period counter carries on, so
+ * runs after a reset draw fresh numbers. stg_set_seed restarts it. */
 int stg_reset(stg_sim *sim);
 
 /* TetOpSplitP::run (tetopsplit.cpp:1507-1526) / OmegaHSimulation::run
- * (simulation.cpp:1418-1453): advance every realization to endtime. */
+ * (simulation.cpp:1418-1453): advance every realization to endtime. When the
+ * device reports an error (STG_ERR_PROG: event cap, a potential outside a
+ * k(V) table, GHK overflow) the periods have run: time, step counts and the
+ * Philox counters stand at endtime, and the counts are those the kernels left;
+ * restore a checkpoint or reset to continue from a known state. */
 int stg_run(stg_sim *sim, double endtime);
 
 /* TetOpSplitP::advance (tetopsplit.cpp:1956-1966). */
 int stg_advance(stg_sim *sim, double adv);
 
-/* Re-key one realization's Philox streams, as re-seeding the solver's RNG
- * (RNG::initialize, src/steps/rng/rng.hpp) does for the reference. */
+/* Re-key one realization's Philox streams and restart them (period counter
+ * 0), as re-seeding the solver's RNG (RNG::initialize, src/steps/rng/rng.hpp)
+ * restarts the reference's sequence. */
 int stg_set_seed(stg_sim *sim, int32_t real, uint64_t seed);
 
 /* API::getTime / setTime. */
@@ -238,8 +246,9 @@ int stg_get_specialize(const stg_sim *sim, int32_t *on);
 int stg_set_diff_apply_threshold(stg_sim *sim, int32_t threshold);
 
 /* TetOpSplitP::checkpoint / restore (tetopsplit.cpp:217-453), own format:
- * time, per-realization period counters, seeds, extents and constants, and all
- * molecule counts. Because every random draw is keyed by (seed, tet, period),
+ * time, per-realization period and Philox counters, seeds, extents and
+ * constants, all molecule counts, triangle potentials, channel-count
+ * integrals and GHK charges. Because every random draw is keyed by (seed, tet, period),
  * a restored simulation continues bit-for-bit as the original would have.
  * stg_restore requires a simulation of the same shape. */
 int stg_state_size(const stg_sim *sim, size_t *bytes);
@@ -262,7 +271,10 @@ int stg_get_sizes(const stg_sim *sim, int32_t real, int32_t *n_real, int32_t *n_
                   int32_t *n_tris);
 
 /* Average duration (ms) of the period kernel over the launches of the last
- * stg_run, timed with HIP events on the handle's stream; *n = launches. */
+ * stg_run, timed with HIP events on the handle's stream; *n = launches. A run
+ * of L periods makes L period-kernel launches (each realization steps in the
+ * first launches it has periods for) and then one entering-only launch of a
+ * separate kernel, which is not included. */
 int stg_last_run_kernel_ms(const stg_sim *sim, double *ms_per_launch, int32_t *n);
 
 /* ------------------------------------------------------------------ EField
@@ -326,6 +338,16 @@ int stg_efield_advance(stg_efield *ef, double dt);
 /* TetOpSplitP::_runWithEField (tetopsplit.cpp:1851-1954): alternate
  * reaction-diffusion to t+efdt (stg_run) and one EField step, to endtime. */
 int stg_efield_run(stg_efield *ef, stg_sim *sim, double endtime, double efdt);
+
+/* EField::checkpoint / restore as TetOpSplitP::checkpoint / restore call
+ * them (pEField->checkpoint, tetopsplit.cpp:309; pEField->restore, :446),
+ * own format: vertex potentials, the previous step's increment, membrane and
+ * clamp parameters, and the start time of the running channel-count
+ * integrals (those integrals and the GHK charges are in the attached
+ * simulation's checkpoint). stg_efield_restore requires the same mesh. */
+int stg_efield_state_size(const stg_efield *ef, size_t *bytes);
+int stg_efield_checkpoint(stg_efield *ef, void *buf, size_t bytes);
+int stg_efield_restore(stg_efield *ef, const void *buf, size_t bytes);
 
 /* Solver choice (no reference counterpart): precision 0 = FP32 conjugate
  * gradients with FP64 dot products inside FP64 iterative refinement, 1 = FP64

@@ -79,8 +79,15 @@ def load() -> C.CDLL:
     lib.stgo_run.argtypes = [C.POINTER(StgoSystem), _dp, C.c_int32, u32, _ip,
                              C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     lib.stgo_run.restype = C.c_int32
+    lib.stgo_threads.argtypes = []
+    lib.stgo_threads.restype = C.c_int32
     _lib = lib
     return lib
+
+
+def threads() -> int:
+    """OpenMP threads the oracle's period loops run on."""
+    return int(load().stgo_threads())
 
 
 def philox(ctr, key):
@@ -185,6 +192,8 @@ class OracleSim:
         self.sys.sspec_dcst = P(a["sdc"], C.c_double)
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
+        # Philox period counter: advances with nsteps, survives reset()
+        self.rng_steps = 0
         self.time = 0.0
         self.events = C.c_uint64(0)
         self.moved = C.c_uint64(0)
@@ -287,12 +296,26 @@ class OracleSim:
         d = np.ascontiguousarray(np.asarray(dts, dtype=np.float64))
         if len(d) == 0:
             return
-        rc = self.lib.stgo_run(C.byref(self.sys), d.ctypes.data_as(_dp), len(d), self.nsteps,
+        rc = self.lib.stgo_run(C.byref(self.sys), d.ctypes.data_as(_dp), len(d), self.rng_steps,
                                self.scratch.ctypes.data_as(_ip), C.byref(self.events),
                                C.byref(self.moved))
         if rc != 0:
             raise RuntimeError(f"oracle run failed ({rc})")
         self.nsteps += len(d)
+        self.rng_steps += len(d)
+
+    def reset(self) -> None:
+        """Counts, time, step count, extents, channel integrals and GHK charges
+        to zero (TetOpSplitP::reset, tetopsplit.cpp:1436-1503); the Philox
+        period counter carries on, as the reference's RNG does."""
+        self.a["counts"][:] = 0
+        self.a["tcounts"][:] = 0
+        self.a["intg"][:] = 0.0
+        self.a["charge"][:] = 0
+        self.nsteps = 0
+        self.time = 0.0
+        self.events.value = 0
+        self.moved.value = 0
 
     def run(self, endtime: float, mode: int = 0) -> None:
         dts = schedule(self.time, endtime, self.update_period(), mode)

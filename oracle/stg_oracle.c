@@ -43,6 +43,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -51,6 +52,9 @@
 #define GAS_CONSTANT 8.314472  /* src/steps/math/constants.hpp:53 */
 #define E_CHARGE 1.602176487e-19 /* src/steps/math/constants.hpp:57 */
 #define MAX_EVENTS_PER_PERIOD (1 << 22)
+
+/* Threads the OpenMP loops of stgo_run use (for reporting the CPU baseline). */
+int32_t stgo_threads(void) { return (int32_t)omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------ Philox */
 

@@ -100,6 +100,9 @@ typedef struct stgo_system {
     const int32_t *tri_ltri;
 } stgo_system;
 
+/* Threads the OpenMP loops of stgo_run use. */
+int32_t stgo_threads(void);
+
 /* Philox4x32 with 8 rounds (Random123 philox4x32_R(8, ctr, key)). */
 void stgo_philox4x32_8(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -172,6 +172,9 @@ SIGNATURES = {
     "stg_efield_set_precond": (C.c_int, [C.c_void_p, C.c_int32]),
     "stg_efield_precond_info": (C.c_int, [C.c_void_p, _ip, _ip, _dp]),
     "stg_efield_sizes": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "stg_efield_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "stg_efield_checkpoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "stg_efield_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "stg_mesh_build": (C.c_int, [_dp, C.c_int32, _ip, C.c_int32, C.POINTER(C.c_void_p)]),
     "stg_mesh_build_sizes": (C.c_int, [C.c_void_p, _ip, _ip]),
     "stg_mesh_build_fetch": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _dp]),

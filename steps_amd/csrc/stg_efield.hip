@@ -28,6 +28,7 @@
 #include <cfloat>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <string>
 #include <utility>
 #include <vector>
@@ -1870,6 +1871,81 @@ int stg_efield_run(stg_efield *e, stg_sim *sim, double endtime, double efdt) {
             advance_impl(e, t1 - t0);
             t = t1;
         }
+    });
+}
+
+// ------------------------------------------------------------ checkpoint
+// Own format: header, vertex potentials and the previous step's increment
+// (the CG starting guess), the membrane and clamp parameters, and the time
+// the present channel-count integrals started from.
+namespace {
+constexpr uint64_t kEfStateMagic = 0x3153464547544ull;  // "TGEFS1"
+struct EfStateHeader {
+    uint64_t magic;
+    int32_t n, n_tris;
+    double intg_t0, sigma, vext, cm;
+};
+size_t ef_state_bytes(const stg_efield *e) {
+    return sizeof(EfStateHeader) + sizeof(double) * (5 * (size_t)e->n + (size_t)e->n_tris) + (size_t)e->n;
+}
+}  // namespace
+
+int stg_efield_state_size(const stg_efield *e, size_t *bytes) {
+    if (!e || !bytes) return STG_ERR_ARG;
+    *bytes = ef_state_bytes(e);
+    return STG_OK;
+}
+
+int stg_efield_checkpoint(stg_efield *e, void *buf, size_t bytes) {
+    if (!e || !buf) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (bytes < ef_state_bytes(e)) throw EfError{STG_ERR_ARG, "checkpoint buffer too small"};
+        char *p = static_cast<char *>(buf);
+        const EfStateHeader h{kEfStateMagic, e->n, e->n_tris, e->intg_t0, e->sigma, e->vext, e->cm};
+        std::memcpy(p, &h, sizeof(h));
+        p += sizeof(h);
+        const size_t nb = (size_t)e->n * sizeof(double);
+        EF_HIP(hipMemcpy(p, e->V, nb, hipMemcpyDeviceToHost));
+        p += nb;
+        EF_HIP(hipMemcpy(p, e->dv_prev, nb, hipMemcpyDeviceToHost));
+        p += nb;
+        for (const auto *v : {&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h}) {
+            std::memcpy(p, v->data(), v->size() * sizeof(double));
+            p += v->size() * sizeof(double);
+        }
+        std::memcpy(p, e->clamped_h.data(), e->clamped_h.size());
+    });
+}
+
+int stg_efield_restore(stg_efield *e, const void *buf, size_t bytes) {
+    if (!e || !buf) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (bytes < sizeof(EfStateHeader)) throw EfError{STG_ERR_ARG, "checkpoint too short"};
+        const char *p = static_cast<const char *>(buf);
+        EfStateHeader h;
+        std::memcpy(&h, p, sizeof(h));
+        if (h.magic != kEfStateMagic) throw EfError{STG_ERR_ARG, "not an EField checkpoint"};
+        if (h.n != e->n || h.n_tris != e->n_tris)
+            throw EfError{STG_ERR_ARG, "checkpoint was written by an EField of another mesh"};
+        if (bytes < ef_state_bytes(e)) throw EfError{STG_ERR_ARG, "checkpoint too short"};
+        p += sizeof(h);
+        const size_t nb = (size_t)e->n * sizeof(double);
+        EF_HIP(hipMemcpy(e->V, p, nb, hipMemcpyHostToDevice));
+        p += nb;
+        EF_HIP(hipMemcpy(e->dv_prev, p, nb, hipMemcpyHostToDevice));
+        p += nb;
+        for (auto *v : {&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h}) {
+            std::memcpy(v->data(), p, v->size() * sizeof(double));
+            p += v->size() * sizeof(double);
+        }
+        std::memcpy(e->clamped_h.data(), p, e->clamped_h.size());
+        e->intg_t0 = h.intg_t0;
+        e->sigma = h.sigma;
+        e->vext = h.vext;
+        e->cm = h.cm;
+        upload_params(e);
+        publish_tri_v(e);
+        EF_HIP(hipStreamSynchronize(e->stream));
     });
 }
 

@@ -115,7 +115,12 @@ struct stg_sim {
     std::vector<double> vdep_vmin, vdep_dv, vdep_k;
     std::vector<int32_t> vdep_off;
     std::vector<double> dt_std;           // [n_real]
-    std::vector<uint64_t> nsteps;         // [n_real]
+    std::vector<uint64_t> nsteps;         // [n_real] periods since reset (API::getNSteps)
+    // [n_real] Philox period counter: advances with nsteps but survives reset,
+    // so a reset simulation continues its random streams instead of replaying
+    // them (the reference's RNG keeps running through TetOpSplitP::reset,
+    // tetopsplit.cpp:1436-1503); re-keying (stg_set_seed) restarts it
+    std::vector<uint64_t> rng_steps;
     std::vector<uint64_t> seeds;
     std::vector<uint32_t> streams;
     double time = 0.0;
@@ -657,6 +662,7 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     (void)SS;
     s->refresh_dt();
     s->nsteps.assign(n_real, 0);
+    s->rng_steps.assign(n_real, 0);
 
     HIP_OK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     HIP_OK(hipEventCreate(&s->ev0));
@@ -1005,14 +1011,26 @@ void unspecialize(stg_sim *s) {
     s->rtc_fn = nullptr;
 }
 
-void launch_all(stg_sim *s, int L) {
+// LDS bytes of the period kernel that launch_all would launch: dynamic rows
+// plus the block-reduction array (static)
+size_t launch_smem(const stg_sim *s, bool *spec_out) {
+    const int B = s->block;
+    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
+    if (spec_out) *spec_out = spec;
+    return spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
+}
+
+// The periods of a run: launches 0 .. L-1 of the period kernel (launch l
+// steps every realization with more than l periods and enters the molecules
+// of the previous launch).
+void launch_periods(stg_sim *s, int L) {
     const int B = s->block;
     const PeriodKernel kern = period_kernel(s, B);
     // a volume-only module cannot serve a model that has come to need the
     // membrane variant (e.g. channel integrals requested by an EField)
-    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
-    const size_t smem = spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
-    for (int l = 0; l <= L; ++l) {
+    bool spec = false;
+    const size_t smem = launch_smem(s, &spec);
+    for (int l = 0; l < L; ++l) {
         const int nb = s->launch_blocks[l];
         if (nb == 0) continue;
         stg::SysK k = make_sys(s, l & 1);
@@ -1025,6 +1043,18 @@ void launch_all(stg_sim *s, int L) {
             hipLaunchKernelGGL(kern, dim3(nb), dim3(B), smem, s->stream, s->mk, k, l);
         }
     }
+    HIP_OK(hipGetLastError());
+}
+
+// Launch L: the molecules that left in the run's last period enter their
+// destinations (the realizations whose last period was launch L - 1).
+void launch_final_enter(stg_sim *s, int L) {
+    const int nb = s->launch_blocks[L];
+    if (nb == 0) return;
+    stg::SysK k = make_sys(s, L & 1);
+    k.n_blocks = nb;
+    if (needs_memb(s)) hipLaunchKernelGGL(stg::k_enter<true>, dim3(nb), dim3(s->block), 0, s->stream, s->mk, k);
+    else hipLaunchKernelGGL(stg::k_enter<false>, dim3(nb), dim3(s->block), 0, s->stream, s->mk, k);
     HIP_OK(hipGetLastError());
 }
 
@@ -1051,7 +1081,7 @@ void run_impl(stg_sim *s, double endtime) {
         std::vector<uint32_t> step0(s->n_real);
         for (int r = 0; r < s->n_real; ++r) {
             nper[r] = (int32_t)dts[r].size();
-            step0[r] = (uint32_t)s->nsteps[r];
+            step0[r] = (uint32_t)s->rng_steps[r];
             for (size_t i = 0; i < dts[r].size(); ++i) tab[i * s->n_real + r] = dts[r][i];
         }
         if (tab.size() > s->dt_tab_cap) {
@@ -1065,7 +1095,12 @@ void run_impl(stg_sim *s, double endtime) {
         HIP_OK(hipMemcpyAsync(s->d_real_nper, nper.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
         HIP_OK(hipMemcpyAsync(s->d_real_step0, step0.data(), sizeof(uint32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
         order_blocks(s, nper, L);
-        const size_t smem = s->smem_bytes(s->block);
+        // the LDS footprint can grow after stg_set_launch checked it (an
+        // EField asking for channel integrals switches to the membrane variant)
+        const size_t smem = launch_smem(s, nullptr);
+        if (smem + sizeof(unsigned long long) * 32 > 160 * 1024)
+            throw StgError{STG_ERR_ARG, "the model's LDS footprint exceeds 160 KiB at " + std::to_string(s->block) +
+                                            " threads per block; choose a smaller block with stg_set_launch"};
         if (smem > 64 * 1024)
             HIP_OK(hipFuncSetAttribute((const void *)period_kernel(s, s->block),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
@@ -1074,7 +1109,7 @@ void run_impl(stg_sim *s, double endtime) {
             if (s->graph_launches != L || s->graph_block != s->block || s->graph_counts != s->launch_blocks) {
                 s->free_graph();
                 HIP_OK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-                launch_all(s, L);
+                launch_periods(s, L);
                 HIP_OK(hipStreamEndCapture(s->stream, &s->graph));
                 HIP_OK(hipGraphInstantiate(&s->graph_exec, s->graph, nullptr, nullptr, 0));
                 s->graph_launches = L;
@@ -1083,14 +1118,23 @@ void run_impl(stg_sim *s, double endtime) {
             }
             HIP_OK(hipGraphLaunch(s->graph_exec, s->stream));
         } else {
-            launch_all(s, L);
+            launch_periods(s, L);
         }
         HIP_OK(hipEventRecord(s->ev1, s->stream));
+        launch_final_enter(s, L);
         HIP_OK(hipStreamSynchronize(s->stream));
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         s->last_ms = ms;
-        s->last_launches = L + 1;
+        s->last_launches = L;
+        // the launches have run: the clock, the period counts and the Philox
+        // counters move on even when a device error is reported below, so a
+        // later run never repeats this run's draws
+        for (int r = 0; r < s->n_real; ++r) {
+            s->nsteps[r] += dts[r].size();
+            s->rng_steps[r] += dts[r].size();
+        }
+        s->time = endtime;
         int32_t err = 0;
         HIP_OK(hipMemcpy(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
         if (err) {
@@ -1099,7 +1143,6 @@ void run_impl(stg_sim *s, double endtime) {
             if (err & 2) throw StgError{STG_ERR_PROG, "triangle potential outside a voltage-dependent rate table"};
             throw StgError{STG_ERR_PROG, "GHK flux overflow: |z V F / (R T)| too large"};
         }
-        for (int r = 0; r < s->n_real; ++r) s->nsteps[r] += dts[r].size();
     }
     s->time = endtime;
 }
@@ -1204,6 +1247,7 @@ int stg_reset(stg_sim *s) {
             HIP_OK(hipMemsetAsync(s->d_tri_charge, 0, sizeof(int32_t) * (size_t)s->NTRI, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         s->time = 0.0;
+        // the Philox period counters (rng_steps) keep running
         std::fill(s->nsteps.begin(), s->nsteps.end(), 0);
     });
 }
@@ -1226,6 +1270,7 @@ int stg_set_seed(stg_sim *s, int32_t real, uint64_t seed) {
     return guard(s, [&] {
         check_real(s, real);
         s->seeds[real] = seed;
+        s->rng_steps[real] = 0;  // a re-keyed generator starts its streams afresh
         HIP_OK(hipMemcpy(s->d_real_seed + real, &seed, sizeof(uint64_t), hipMemcpyHostToDevice));
     });
 }
@@ -1540,19 +1585,25 @@ int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
 // Layout (all little-endian, host byte order): header, per-realization time
 // counters and constants, then the count arrays as stored on the device.
 namespace {
-constexpr uint64_t kStateMagic = 0x32534154534754ull;  // "TGSTATS2": counts, constants, triangle V
+// "TGSTATS3": counts, constants, triangle V, Philox period counters, channel
+// integrals and GHK charges
+constexpr uint64_t kStateMagic = 0x33534154534754ull;
 struct StateHeader {
     uint64_t magic;
     int32_t n_real, n_tets, n_tris, n_specs, n_sspecs, n_reacs, n_sreacs, diff_threshold;
     double time;
+    int32_t intg_mask, has_charge;
 };
+constexpr size_t kWordsPerReal = 5;  // nsteps, rng_steps, seed, reaction extent, diffusion extent
 
 size_t state_bytes(const stg_sim *s) {
-    return sizeof(StateHeader) + s->n_real * (sizeof(uint64_t) * 4) +
+    return sizeof(StateHeader) + s->n_real * (sizeof(uint64_t) * kWordsPerReal) +
            sizeof(double) * (s->h_k.size() + s->h_sk.size() + s->h_D.size() + s->h_bnd_D.size() +
                              s->h_sD.size()) +
            sizeof(int32_t) * ((size_t)s->S * s->NT + (size_t)s->SS * s->NTRI) +
-           sizeof(double) * (size_t)s->NTRI;
+           sizeof(double) * (size_t)s->NTRI +
+           (s->d_tri_intg ? sizeof(double) * (size_t)s->SS * s->NTRI : 0) +
+           (s->d_tri_charge ? sizeof(int32_t) * (size_t)s->NTRI : 0);
 }
 }  // namespace
 
@@ -1568,7 +1619,7 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
         if (bytes < state_bytes(s)) throw StgError{STG_ERR_ARG, "checkpoint buffer too small"};
         char *p = static_cast<char *>(buf);
         StateHeader h{kStateMagic, s->n_real, s->NT, s->NTRI, s->S, s->SS, s->R, s->RS, s->diff_threshold,
-                      s->time};
+                      s->time, s->d_tri_intg ? s->mk.tri_intg_mask : 0, s->d_tri_charge ? 1 : 0};
         std::memcpy(p, &h, sizeof(h));
         p += sizeof(h);
         std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
@@ -1581,7 +1632,7 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
                     ev += ext[2 * b];
                     mv += ext[2 * b + 1];
                 }
-            const uint64_t v[4] = {s->nsteps[r], s->seeds[r], ev, mv};
+            const uint64_t v[kWordsPerReal] = {s->nsteps[r], s->rng_steps[r], s->seeds[r], ev, mv};
             std::memcpy(p, v, sizeof(v));
             p += sizeof(v);
         }
@@ -1595,6 +1646,12 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
         if (nt) HIP_OK(hipMemcpy(p, s->d_tri_counts, nt * sizeof(int32_t), hipMemcpyDeviceToHost));
         p += nt * sizeof(int32_t);
         if (s->NTRI) HIP_OK(hipMemcpy(p, s->d_tri_v, (size_t)s->NTRI * sizeof(double), hipMemcpyDeviceToHost));
+        p += (size_t)s->NTRI * sizeof(double);
+        if (s->d_tri_intg) {
+            HIP_OK(hipMemcpy(p, s->d_tri_intg, (size_t)s->SS * s->NTRI * sizeof(double), hipMemcpyDeviceToHost));
+            p += (size_t)s->SS * s->NTRI * sizeof(double);
+        }
+        if (s->d_tri_charge) HIP_OK(hipMemcpy(p, s->d_tri_charge, (size_t)s->NTRI * sizeof(int32_t), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1609,20 +1666,27 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
         if (h.n_real != s->n_real || h.n_tets != s->NT || h.n_tris != s->NTRI || h.n_specs != s->S ||
             h.n_sspecs != s->SS || h.n_reacs != s->R || h.n_sreacs != s->RS)
             throw StgError{STG_ERR_ARG, "checkpoint was written by a simulation of another shape"};
-        if (bytes < state_bytes(s)) throw StgError{STG_ERR_ARG, "checkpoint too short"};
         p += sizeof(h);
         std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
         std::vector<bool> placed(s->n_real, false);
+        const bool want_intg = h.intg_mask != 0, want_charge = h.has_charge != 0;
+        if (want_charge && !s->d_tri_charge) throw StgError{STG_ERR_ARG, "checkpoint holds GHK charges this model lacks"};
+        if (want_intg && (s->SS == 0 || s->NTRI == 0))
+            throw StgError{STG_ERR_ARG, "checkpoint holds channel integrals this model lacks"};
+        const size_t need = state_bytes(s) + (want_intg && !s->d_tri_intg ? sizeof(double) * (size_t)s->SS * s->NTRI : 0) -
+                            (!want_intg && s->d_tri_intg ? sizeof(double) * (size_t)s->SS * s->NTRI : 0);
+        if (bytes < need) throw StgError{STG_ERR_ARG, "checkpoint too short"};
         for (int r = 0; r < s->n_real; ++r) {
-            uint64_t v[4];
+            uint64_t v[kWordsPerReal];
             std::memcpy(v, p, sizeof(v));
             p += sizeof(v);
             s->nsteps[r] = v[0];
-            s->seeds[r] = v[1];
+            s->rng_steps[r] = v[1];
+            s->seeds[r] = v[2];
             for (size_t b = 0; b < s->block_real.size(); ++b)
                 if (s->block_real[b] == r && !placed[r]) {
-                    ext[2 * b] = v[2];
-                    ext[2 * b + 1] = v[3];
+                    ext[2 * b] = v[3];
+                    ext[2 * b + 1] = v[4];
                     placed[r] = true;
                 }
         }
@@ -1636,6 +1700,17 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
         if (nt) HIP_OK(hipMemcpy(s->d_tri_counts, p, nt * sizeof(int32_t), hipMemcpyHostToDevice));
         p += nt * sizeof(int32_t);
         if (s->NTRI) HIP_OK(hipMemcpy(s->d_tri_v, p, (size_t)s->NTRI * sizeof(double), hipMemcpyHostToDevice));
+        p += (size_t)s->NTRI * sizeof(double);
+        if (want_intg) {
+            const size_t n = (size_t)s->SS * s->NTRI;
+            if (!s->d_tri_intg) s->d_tri_intg = dalloc<double>(n);
+            HIP_OK(hipMemcpy(s->d_tri_intg, p, n * sizeof(double), hipMemcpyHostToDevice));
+            p += n * sizeof(double);
+            if ((s->mk.tri_intg_mask | h.intg_mask) != s->mk.tri_intg_mask) s->mk.tri_intg_mask |= h.intg_mask;
+        } else if (s->d_tri_intg) {
+            HIP_OK(hipMemset(s->d_tri_intg, 0, (size_t)s->SS * s->NTRI * sizeof(double)));
+        }
+        if (want_charge) HIP_OK(hipMemcpy(s->d_tri_charge, p, (size_t)s->NTRI * sizeof(int32_t), hipMemcpyHostToDevice));
         if (!ext.empty())
             HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->d_real_seed, s->seeds.data(), sizeof(uint64_t) * s->n_real, hipMemcpyHostToDevice));
@@ -1777,6 +1852,8 @@ int stg_get_sizes(const stg_sim *s, int32_t real, int32_t *n_real, int32_t *n_te
     });
 }
 
+// mean over the full-period launches of the last run (the trailing
+// entering-only k_enter is timed outside them)
 int stg_last_run_kernel_ms(const stg_sim *s, double *ms_per_launch, int32_t *n) {
     if (!s || !ms_per_launch || !n) return STG_ERR_ARG;
     *n = s->last_launches;

@@ -1067,6 +1067,51 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     }
 }
 
+// The entering pass alone, for the launch after the last period of a run: the
+// same gather as the head of k_period, for the blocks whose realization's
+// last period was the previous launch (the first launch_blocks[L] block
+// descriptors). A kernel of its own so that every k_period dispatch of a
+// single-realization run is a full period.
+template <bool MEMB>
+__global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) {
+    const int4 bd = sys.block_desc[blockIdx.x];
+    const int t = bd.z + (int)threadIdx.x;
+    if (t >= bd.w) return;
+    const int NT = sys.n_tets;
+    const int4 nb4 = sys.nbr[t];
+    const uint32_t nf = sys.nface[t];
+    const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    size_t off[4];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
+    for (int d = 0; d < m.n_diff; ++d) {
+        const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
+        int32_t add = 0;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+            int32_t v = op[off[f]];
+            if (v == kSlotEscape) v = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
+            add += nbv[f] >= 0 ? v : 0;
+        }
+        sys.counts[(size_t)m.dspec[d] * NT + t] += add;
+    }
+    if (MEMB && m.n_sdiff > 0 && sys.tri_ptr != nullptr) {
+        for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri) {
+            const int4 nb = sys.tri_nbr[tri];
+            const uint32_t ne = sys.tri_nface[tri];
+            const int tnb[3] = {nb.x, nb.y, nb.z};
+            for (int d = 0; d < m.n_sdiff; ++d) {
+                const int32_t *op = sys.tout_prev + (size_t)d * sys.n_tris * 4;
+                int32_t add = 0;
+#pragma unroll
+                for (int e = 0; e < 3; ++e)
+                    if (tnb[e] >= 0) add += op[(size_t)tnb[e] * 4 + ((ne >> (8 * e)) & 0xff)];
+                sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
+            }
+        }
+    }
+}
+
 // getBatchTetSpecCountsNP gather
 __global__ void k_gather_counts(const int32_t *__restrict__ counts, const int64_t *__restrict__ idx,
                                 size_t n, int32_t base, double *__restrict__ out) {

@@ -113,6 +113,17 @@ class EField:
         self._chk(self.lib.stg_efield_stats(self.h, C.byref(i), C.byref(r), C.byref(x)))
         return dict(iterations=i.value, refinements=r.value, rel_residual=x.value)
 
+    def checkpoint(self) -> bytes:
+        n = C.c_size_t()
+        self._chk(self.lib.stg_efield_state_size(self.h, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        self._chk(self.lib.stg_efield_checkpoint(self.h, buf, n.value))
+        return buf.raw
+
+    def restore(self, state: bytes):
+        buf = C.create_string_buffer(state, len(state))
+        self._chk(self.lib.stg_efield_restore(self.h, buf, len(state)))
+
     def sizes(self):
         a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
         self._chk(self.lib.stg_efield_sizes(self.h, C.byref(a), C.byref(b), C.byref(c)))

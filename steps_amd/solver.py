@@ -18,6 +18,7 @@ from .geom import Tetmesh, comps_mesh_tables, mesh_tables
 from .model import Model, flatten, flatten_comps
 
 EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC = 0, 1, 2, 3  # src/steps/solver/api.hpp:61-66
+DEFAULT_MEMB_POT = -65.0e-3                                  # src/steps/util/common.hpp:101
 AVOGADRO = 6.02214179e23
 # meshes of at least this many tets run the period kernel compiled for their model
 SPECIALIZE_MIN_TETS = 200_000
@@ -195,12 +196,23 @@ class TetOpSplit:
 
     # --- time stepping ---
     def reset(self):
+        """TetOpSplitP::reset (tetopsplit.cpp:1436-1503): counts, time, step
+        count and extents to zero and, with an EField, the membrane potential
+        back to DEFAULT_MEMB_POT. The random streams keep running, as the
+        reference's RNG does, so reset-and-run loops give independent runs;
+        re-initialising the RNG with a new seed before reset() re-keys every
+        compartment and restarts the streams from that seed."""
         seed = int(getattr(self.rng, "seed", self._seed))
         if seed != self._seed:
             self._seed = seed
-            self.eng.set_seed(seed)
-        self._host_rng = np.random.default_rng([self._seed, 0x5EED])
+            for r in range(self.eng.n_real):
+                self.eng.set_seed(seed, r)
+            self._host_rng = np.random.default_rng([self._seed, 0x5EED])
         self.eng.reset()
+        if self.ef is not None:
+            self.ef.set_potential(DEFAULT_MEMB_POT)
+            if self.patch is not None:
+                self.ef.attach(self.eng, self._preal)   # channel integrals restart at t = 0
 
     def run(self, endtime: float):
         if self.ef is not None:
@@ -801,14 +813,34 @@ class TetOpSplit:
         for t, v in zip(index_array, vs):
             self.setTriV(int(t), float(v))
 
-    # --- checkpointing (own binary format, stg_checkpoint in include/stg.h) ---
+    # --- checkpointing (own binary format: stg_checkpoint and stg_efield_checkpoint in include/stg.h) ---
+    _CP_MAGIC = b"STGCKPT1"
+
     def checkpoint(self, file_name: str):
+        """TetOpSplitP::checkpoint (tetopsplit.cpp:217-320): the engine state and,
+        with an EField, the EField state (pEField->checkpoint, tetopsplit.cpp:309)."""
+        eng = self.eng.checkpoint()
+        ef = self.ef.checkpoint() if self.ef is not None else b""
         with open(file_name, "wb") as f:
-            f.write(self.eng.checkpoint())
+            f.write(self._CP_MAGIC)
+            f.write(np.array([len(eng), len(ef)], dtype=np.uint64).tobytes())
+            f.write(eng)
+            f.write(ef)
 
     def restore(self, file_name: str):
+        """TetOpSplitP::restore (tetopsplit.cpp:353-453), EField included (:446)."""
         with open(file_name, "rb") as f:
-            self.eng.restore(f.read())
+            data = f.read()
+        if data[:8] != self._CP_MAGIC or len(data) < 24:
+            raise _argerr(f"{file_name!r} is not a checkpoint of this solver")
+        n_eng, n_ef = (int(x) for x in np.frombuffer(data[8:24], dtype=np.uint64))
+        if len(data) < 24 + n_eng + n_ef:
+            raise _argerr("checkpoint file is truncated")
+        if (n_ef > 0) != (self.ef is not None):
+            raise _argerr("checkpoint and solver differ in whether they hold an EField")
+        self.eng.restore(data[24:24 + n_eng])
+        if self.ef is not None:
+            self.ef.restore(data[24 + n_eng:24 + n_eng + n_ef])
 
 
 class DistTetOpSplit(TetOpSplit):

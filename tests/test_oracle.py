@@ -783,3 +783,26 @@ def test_reaction_diffusion_conserves_exactly():
         tot = o.counts.sum(axis=1)
         assert tot[0] + tot[2] == 4000 and tot[1] + tot[2] == 3000
         assert o.counts.min() >= 0
+
+
+def test_oracle_reset_keeps_the_philox_counter():
+    """OracleSim.reset() clears state and step count but not the Philox period
+    counter, as the engine's stg_reset does (the reference's RNG runs on
+    through TetOpSplitP::reset, tetopsplit.cpp:1436-1503)."""
+    w = W.ab_binding_cube(n=3, side=3e-6, n_a=500, n_b=500, seed=1, end_time=0.05)
+    o = bind.OracleSim(w.fm, w.mt, seed=3)
+    o.counts[:] = w.counts
+    o.run(0.05, 0)
+    first = o.counts.copy()
+    n = o.nsteps
+    o.reset()
+    assert o.nsteps == 0 and o.time == 0.0 and o.rng_steps == n and o.counts.sum() == 0
+    o.counts[:] = w.counts
+    o.run(0.05, 0)
+    assert not np.array_equal(o.counts, first)
+    # a fresh oracle started at the same counter reproduces the second run
+    o2 = bind.OracleSim(w.fm, w.mt, seed=3)
+    o2.rng_steps = n
+    o2.counts[:] = w.counts
+    o2.run(0.05, 0)
+    assert np.array_equal(o2.counts, o.counts)
