@@ -221,6 +221,45 @@ int stg_set_dcst(stg_sim *sim, int32_t real, int32_t spec, double dcst);
 /* Surface diffusion constant of surface species sspec (setPatchSDiffD). */
 int stg_set_sdcst(stg_sim *sim, int32_t real, int32_t sspec, double dcst);
 
+/* Per-element tables of one realization (compartment), the device form of
+ * the reference's per-tet and per-triangle setters. Each table replaces the
+ * realization's constants for its elements; the host composes it from the
+ * compartment-level value, per-element overrides and activation flags, as
+ * TetOpSplitP keeps one constant and one active flag per kproc. NULL removes
+ * the table. A realization with a table runs the general (membrane) variant
+ * of the generic period kernel. Triangles are in the realization's input
+ * order.
+ *
+ * Rate constant of every (reaction, tet), [n_reacs][n_tets], SI, 0 for an
+ * inactive reaction: _setTetReacK (tetopsplit.cpp:3996), _setTetReacActive
+ * (:4045), _setCompReacActive (:2309), Reac::rate (reac.cpp:184-232). */
+int stg_set_tet_reac_table(stg_sim *sim, int32_t real, const double *k);
+/* Diffusion constant of every (species, tet, face), [n_specs][n_tets][4]
+ * m^2/s, face f opposite vertex f, 0 where nothing leaves through the face:
+ * _setTetDiffD with and without a direction (tetopsplit.cpp:4103),
+ * _setTetDiffActive (:4166), _setCompDiffActive (:2437), and the
+ * diffusion-boundary constants of those faces. Recomputes the update period
+ * (_computeUpdPeriod, :5373-5411). */
+int stg_set_tet_diff_table(stg_sim *sim, int32_t real, const double *d);
+/* Clamped species per tet, bit s of mask[t] = species s: reactions and
+ * diffusion leave its count unchanged (Reac::apply, reac.cpp:240-243;
+ * Diff::apply, diff.cpp:476-577): _setTetSpecClamped (tetopsplit.cpp:3951),
+ * _setCompSpecClamped (:2189). */
+int stg_set_tet_clamp(stg_sim *sim, int32_t real, const uint32_t *mask);
+/* [n_sreacs + n_ghk][n_tris]: each surface reaction's constant on each
+ * triangle (SI; for a voltage-dependent reaction a factor 1 or 0 on k(V)),
+ * then a factor 1 or 0 on each GHK current: _setTriSReacK (:4464),
+ * _setTriSReacActive, _setTriVDepSReacActive, and patches that hold only
+ * some of the surface systems. */
+int stg_set_tri_sreac_table(stg_sim *sim, int32_t real, const double *k);
+/* Surface diffusion constant of every (surface species, triangle, edge),
+ * [n_sspecs][n_tris][3]: _setTriSDiffD (tetopsplit.cpp, SDiff::setDirectionDcst,
+ * sdiff.cpp) and surface diffusion boundaries. Recomputes the update period. */
+int stg_set_tri_sdiff_table(stg_sim *sim, int32_t real, const double *d);
+/* Clamped surface species per triangle, bit q = surface species q
+ * (_setTriSpecClamped, _setPatchSpecClamped). */
+int stg_set_tri_clamp(stg_sim *sim, int32_t real, const uint32_t *mask);
+
 /* Launch shape used by stg_run: threads per block (64..1024, multiple of 64)
  * and whether the launches of one run are replayed from a captured HIP graph.
  * No reference counterpart (device tuning). */

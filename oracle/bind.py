@@ -39,6 +39,8 @@ class StgoSystem(C.Structure):
         ("tet_bdir", _ip), ("bnd_dcst", _dp),
         ("tri_nbr", _ip), ("tri_nbr_edge", _ip), ("tri_edge_g", _dp), ("sspec_dcst", _dp),
         ("tri_ltri", _ip),
+        ("tet_k", _dp), ("tet_fD", _dp), ("tet_clamp", C.POINTER(C.c_uint32)),
+        ("tri_sk", _dp), ("tri_eD", _dp), ("tri_clamp", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -262,6 +264,26 @@ class OracleSim:
         o.sys.tri_ltri = P(a["trl"], C.c_int32)
         o.comp_off = off
         return o
+
+    def set_tables(self, tet_k=None, tet_fD=None, tet_clamp=None, tri_sk=None, tri_eD=None,
+                   tri_clamp=None) -> None:
+        """Per-element tables in the oracle's element order (None: unused):
+        tet_k [R][N], tet_fD [S][N][4], tet_clamp [N] (bit s), tri_sk
+        [RS + NG][NTRI], tri_eD [SS][NTRI][3], tri_clamp [NTRI] (bit q)."""
+        P = lambda x, t: x.ctypes.data_as(C.POINTER(t))  # noqa: E731
+        for name, arr, ct, dt in (("tet_k", tet_k, C.c_double, np.float64),
+                                  ("tet_fD", tet_fD, C.c_double, np.float64),
+                                  ("tet_clamp", tet_clamp, C.c_uint32, np.uint32),
+                                  ("tri_sk", tri_sk, C.c_double, np.float64),
+                                  ("tri_eD", tri_eD, C.c_double, np.float64),
+                                  ("tri_clamp", tri_clamp, C.c_uint32, np.uint32)):
+            if arr is None:
+                self.a.pop("tab_" + name, None)
+                setattr(self.sys, name, None)
+            else:
+                a = np.ascontiguousarray(np.asarray(arr, dtype=dt))
+                self.a["tab_" + name] = a
+                setattr(self.sys, name, P(a, ct))
 
     @property
     def bnd_dcst(self) -> np.ndarray:

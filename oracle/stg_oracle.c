@@ -354,14 +354,31 @@ static double tet_dcst(const stgo_system *s, int32_t t, int32_t sp) {
 }
 
 /* D_f g_f per face: faces on a diffusion boundary use the boundary's
- * constant (Diff::setDcst / setDirectionDcst, diff.cpp:340-460) */
+ * constant (Diff::setDcst / setDirectionDcst, diff.cpp:340-460); a per-face
+ * table (Diff::setDirectionDcst per tet, tetopsplit.cpp:4103-4164; inactive
+ * rules 0, :4166-4188) replaces both */
 static void face_rates(const stgo_system *s, int32_t t, int32_t sp, double r4[4]) {
     const double D = tet_dcst(s, t, sp);
     for (int f = 0; f < 4; ++f) {
         const int32_t d = s->tet_bdir ? s->tet_bdir[4 * t + f] : -1;
-        const double Df = d < 0 ? D : s->bnd_dcst[(size_t)d * s->n_specs + sp];
+        double Df = d < 0 ? D : s->bnd_dcst[(size_t)d * s->n_specs + sp];
+        if (s->tet_fD) Df = s->tet_fD[((size_t)sp * s->n_tets + t) * 4 + f];
         r4[f] = Df * s->tet_face_g[4 * t + f];
     }
+}
+
+static int tet_clamped(const stgo_system *s, int32_t t, int32_t sp) {
+    return s->tet_clamp && ((s->tet_clamp[t] >> sp) & 1u);
+}
+
+static int tri_clamped(const stgo_system *s, int32_t tr, int32_t q) {
+    return s->tri_clamp && ((s->tri_clamp[tr] >> q) & 1u);
+}
+
+/* surface diffusion constant of species q through edge e of triangle tr */
+static double edge_dcst(const stgo_system *s, int32_t tr, int32_t q, int e) {
+    if (s->tri_eD) return s->tri_eD[((size_t)q * s->n_tris + tr) * 3 + e];
+    return s->sspec_dcst[q];
 }
 
 /* the species diffuses in some compartment */
@@ -384,9 +401,9 @@ double stgo_update_period(const stgo_system *s) {
     }
     for (int32_t tr = 0; tr < s->n_tris && s->tri_nbr && s->sspec_dcst; ++tr) {
         for (int32_t q = 0; q < s->n_sspecs; ++q) {
-            const double D = s->sspec_dcst[q];
             const double *g = s->tri_edge_g + 3 * tr;
-            const double sum = ((D * g[0]) + (D * g[1])) + (D * g[2]);
+            const double sum = ((edge_dcst(s, tr, q, 0) * g[0]) + (edge_dcst(s, tr, q, 1) * g[1])) +
+                               (edge_dcst(s, tr, q, 2) * g[2]);
             if (sum > mx) mx = sum;
         }
     }
@@ -546,7 +563,8 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 if (lhs[sp] > cnt[sp]) zero = 1;
                 else h = falling(h, cnt[sp], lhs[sp]);
             }
-            a[j] = zero ? 0.0 : h * scaled_const(reac_k[r], order, vscale);
+            const double k = s->tet_k ? s->tet_k[(size_t)r * s->n_tets + t] : reac_k[r];
+            a[j] = zero ? 0.0 : h * scaled_const(k, order, vscale);
         }
         for (int32_t i = 0; i < nt; ++i) {
             const int32_t tr = tp->tri_idx[t0 + i];
@@ -576,10 +594,18 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                     a[j] = 0.0;
                 } else {
                     double scale = nvol == 0 ? s->tri_area[tr] * AVOGADRO : vscale;
-                    double k = s->sreac_k[r];
+                    /* the triangle's own constant, or a factor 1 / 0 on k(V); an
+                     * inactive k(V) reaction reads no table (vdepsreac.cpp:217-230) */
+                    const double f = s->tri_sk ? s->tri_sk[(size_t)r * s->n_tris + tr] : 1.0;
+                    double k = s->tri_sk ? f : s->sreac_k[r];
                     if (s->sreac_vdep && s->sreac_vdep[r] >= 0) {
-                        k = vdep_const(s, s->sreac_vdep[r], s->tri_v[tr]);
-                        if (k < 0.0) return -2;
+                        if (s->tri_sk && f == 0.0) {
+                            k = 0.0;
+                        } else {
+                            k = vdep_const(s, s->sreac_vdep[r], s->tri_v[tr]);
+                            if (k < 0.0) return -2;
+                            if (s->tri_sk) k = k * f;
+                        }
                     }
                     a[j] = h * scaled_const(k, order, scale);
                 }
@@ -588,6 +614,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
             for (int32_t g = 0; g < NG; ++g, ++j) {
                 double A, Bo, Dn;
                 if (!ghk_consts(s, g, s->tri_v[tr], &A, &Bo, &Dn)) return -3;
+                if (s->tri_sk) A = A * s->tri_sk[(size_t)(RS + g) * s->n_tris + tr];
                 const int32_t nch = s->tri_counts[(size_t)s->ghk_chan[g] * s->n_tris + tr];
                 const double rt = ghk_rt(s, g, A, Bo, Dn, cnt[s->ghk_ion[g]], vscale);
                 a[j] = fabs(rt) * (double)nch;
@@ -625,9 +652,10 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
         }
         /* apply: reactions.cpp:124-141 / surface_reactions.cpp:158-172 */
         if (sel < R) {
+            /* clamped species keep their counts: Reac::apply, reac.cpp:240-243 */
             const int32_t *upd = s->reac_upd + sel * S;
             for (int32_t sp = 0; sp < S; ++sp) {
-                if (upd[sp] == 0) continue;
+                if (upd[sp] == 0 || tet_clamped(s, t, sp)) continue;
                 cnt[sp] += upd[sp];
                 if (tp->tracked[sp]) corr[sp] -= (double)upd[sp] * tcur;
             }
@@ -639,9 +667,10 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
             const int32_t ion = s->ghk_ion[g];
             double A, Bo, Dn;
             ghk_consts(s, g, s->tri_v[tr], &A, &Bo, &Dn);
+            if (s->tri_sk) A = A * s->tri_sk[(size_t)(RS + g) * s->n_tris + tr];
             const double rt = ghk_rt(s, g, A, Bo, Dn, cnt[ion], vscale);
             const int efflux = rt >= 0.0;
-            if (s->ghk_realflux[g]) {
+            if (s->ghk_realflux[g] && !tet_clamped(s, t, ion)) {
                 const int32_t dv = efflux ? -1 : 1;
                 cnt[ion] += dv;
                 if (tp->tracked[ion]) corr[ion] -= (double)dv * tcur;
@@ -653,7 +682,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
             const int32_t *us_ = s->sreac_upd_s + r * SS;
             const int32_t *ui = s->sreac_upd_i + r * S;
             for (int32_t q = 0; q < SS; ++q) {
-                if (us_[q] == 0) continue;
+                if (us_[q] == 0 || tri_clamped(s, tr, q)) continue;
                 s->tri_counts[(size_t)q * s->n_tris + tr] += us_[q];
                 /* channel-count integral: n_end*dt - sum dn*t_event, the same
                  * value as the reference's piecewise sum (tri.cpp:544-561) */
@@ -662,7 +691,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 if (tp->tri_tracked[q]) tp->tri_corr[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
             }
             for (int32_t sp = 0; sp < S; ++sp) {
-                if (ui[sp] == 0) continue;
+                if (ui[sp] == 0 || tet_clamped(s, t, sp)) continue;
                 cnt[sp] += ui[sp];
                 if (tp->tracked[sp]) corr[sp] -= (double)ui[sp] * tcur;
             }
@@ -698,7 +727,8 @@ static int32_t leave_tet_species(const stgo_system *s, int32_t t, int32_t sp, do
     if (p < 0.0) p = 0.0;
     const int32_t nl = binom(&st, mp, p);
     if (nl <= 0) return 0;
-    cnt[sp] = n - nl;
+    /* a clamped source keeps its count (Diff::apply, diff.cpp:476-520, 577) */
+    if (!tet_clamped(s, t, sp)) cnt[sp] = n - nl;
     int lastf = -1;
     for (int f = 0; f < 4; ++f)
         if (r4[f] > 0.0) lastf = f;
@@ -746,9 +776,8 @@ static int32_t leave_tri_species(const stgo_system *s, const topo *tp, int32_t t
     out3[0] = out3[1] = out3[2] = 0;
     const size_t ix = (size_t)q * s->n_tris + tr;
     const int32_t n = s->tri_counts[ix];
-    const double D = s->sspec_dcst[q];
     double r3[3];
-    for (int e = 0; e < 3; ++e) r3[e] = D * s->tri_edge_g[3 * tr + e];
+    for (int e = 0; e < 3; ++e) r3[e] = edge_dcst(s, tr, q, e) * s->tri_edge_g[3 * tr + e];
     const double sum = (r3[0] + r3[1]) + r3[2];
     if (n <= 0 || !(sum > 0.0)) return 0;
     const int32_t t = s->tri_tet[tr];
@@ -768,7 +797,7 @@ static int32_t leave_tri_species(const stgo_system *s, const topo *tp, int32_t t
     if (p < 0.0) p = 0.0;
     const int32_t nl = binom(&st, mp, p);
     if (nl <= 0) return 0;
-    s->tri_counts[ix] = n - nl;
+    if (!tri_clamped(s, tr, q)) s->tri_counts[ix] = n - nl;
     int lastf = -1;
     for (int e = 0; e < 3; ++e)
         if (r3[e] > 0.0) lastf = e;
@@ -896,7 +925,8 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                     if (nb < 0) continue;
                     add += scratch[((size_t)sp * N + nb) * 4 + s->tet_nbr_face[4 * t + f]];
                 }
-                s->counts[(size_t)sp * N + t] += add;
+                /* nothing enters a clamped destination (diff.cpp:500-503, 548-551, 570-573) */
+                if (!tet_clamped(s, t, sp)) s->counts[(size_t)sp * N + t] += add;
             }
         }
         /* surface molecules entering across triangle edges */
@@ -910,7 +940,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                     if (nb < 0) continue;
                     add += tout[((size_t)q * NTR + nb) * 3 + s->tri_nbr_edge[3 * tr + e]];
                 }
-                s->tri_counts[(size_t)q * NTR + tr] += add;
+                if (!tri_clamped(s, tr, q)) s->tri_counts[(size_t)q * NTR + tr] += add;
             }
         }
     }

@@ -98,6 +98,22 @@ typedef struct stgo_system {
     const double  *tri_edge_g;
     const double  *sspec_dcst;   /* [n_sspecs] */
     const int32_t *tri_ltri;
+    /* per-element tables, each null when unused (the per-tet and per-triangle
+     * setters of TetOpSplitP, tetopsplit.cpp:2189-2437, 3951-4166, 4464):
+     * tet_k [n_reacs][n_tets] rate constant of each reaction in each tet (0 =
+     * inactive); tet_fD [n_specs][n_tets][4] diffusion constant through each
+     * face (replaces D and boundary constants); tet_clamp [n_tets] bit s =
+     * species s clamped; tri_sk [n_sreacs + n_ghk][n_tris] constant of each
+     * surface reaction on each triangle (a factor on k(V) for a
+     * voltage-dependent one) then a factor on each GHK current; tri_eD
+     * [n_sspecs][n_tris][3] surface diffusion constant through each edge;
+     * tri_clamp [n_tris] bit q = surface species q clamped */
+    const double  *tet_k;
+    const double  *tet_fD;
+    const uint32_t *tet_clamp;
+    const double  *tri_sk;
+    const double  *tri_eD;
+    const uint32_t *tri_clamp;
 } stgo_system;
 
 /* Threads the OpenMP loops of stgo_run use. */

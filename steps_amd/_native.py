@@ -150,6 +150,12 @@ SIGNATURES = {
     "stg_get_temp": (C.c_int, [C.c_void_p, _dp]),
     "stg_get_sizes": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _ip]),
     "stg_last_run_kernel_ms": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "stg_set_tet_reac_table": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_set_tet_diff_table": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_set_tet_clamp": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32)]),
+    "stg_set_tri_sreac_table": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_set_tri_sdiff_table": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_set_tri_clamp": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint32)]),
     "stg_efield_create": (C.c_int, [C.POINTER(StgEfieldMesh), C.POINTER(C.c_void_p)]),
     "stg_efield_destroy": (C.c_int, [C.c_void_p]),
     "stg_efield_last_error": (C.c_char_p, [C.c_void_p]),

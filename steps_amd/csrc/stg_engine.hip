@@ -156,6 +156,14 @@ struct stg_sim {
     double *d_real_sD = nullptr;
     int32_t *d_tout[2] = {nullptr, nullptr};
     double *d_tri_corr = nullptr;
+    // per-element tables (stg_set_tet_reac_table and the like): which tables
+    // each realization uses, host copies in device layout, device arrays
+    std::vector<int32_t> real_tabs;                    // [n_real] stg::kTab* bits
+    std::vector<double> h_tet_k, h_tet_fD, h_tri_sk, h_tri_eD;
+    std::vector<uint32_t> h_tet_clamp, h_tri_clamp;
+    int32_t *d_real_tabs = nullptr;
+    double *d_tet_k = nullptr, *d_tet_fD = nullptr, *d_tri_sk = nullptr, *d_tri_eD = nullptr;
+    uint32_t *d_tet_clamp = nullptr, *d_tri_clamp = nullptr;
     double *d_vdep_vmin = nullptr, *d_vdep_dv = nullptr, *d_vdep_k = nullptr;
     int32_t *d_vdep_off = nullptr;
     int32_t *d_err = nullptr;
@@ -189,7 +197,14 @@ struct stg_sim {
     // surface reaction, a GHK current or an integrated channel count, or
     // faces on a diffusion boundary
     bool needs_memb() const {
-        return (NTRI > 0 && (RS > 0 || NG > 0 || NSD > 0 || mk.tri_intg_mask != 0)) || NBD > 0;
+        return (NTRI > 0 && (RS > 0 || NG > 0 || NSD > 0 || mk.tri_intg_mask != 0)) || NBD > 0 || has_tabs();
+    }
+    // some realization uses a per-element table (read by the membrane
+    // variant of the generic kernel only)
+    bool has_tabs() const {
+        for (int32_t v : real_tabs)
+            if (v) return true;
+        return false;
     }
 
     // per-thread LDS rows of the period kernel: counts, occupancy corrections,
@@ -212,6 +227,7 @@ struct stg_sim {
 
     double compute_dt_std(int r) const {
         double mx = 0.0;
+        const bool ftab = real_tabs[r] & stg::kTabDiff, etab = real_tabs[r] & stg::kTabTriDiff;
         for (int t = tet_off[r]; t < tet_off[r + 1]; ++t) {
             for (int s = 0; s < S; ++s) {
                 const double D = h_D[(size_t)r * S + s];
@@ -219,7 +235,7 @@ struct stg_sim {
                 double Df[4];
                 for (int f = 0; f < 4; ++f) {
                     const uint32_t k = h_tet_bdir.empty() ? 0xffu : (h_tet_bdir[t] >> (8 * f)) & 0xffu;
-                    Df[f] = k == 0xffu ? D : h_bnd_D[(size_t)k * S + s];
+                    Df[f] = ftab ? h_tet_fD[((size_t)s * NT + t) * 4 + f] : (k == 0xffu ? D : h_bnd_D[(size_t)k * S + s]);
                 }
                 const double sum = (((Df[0] * g[0]) + (Df[1] * g[1])) + (Df[2] * g[2])) + (Df[3] * g[3]);
                 if (sum > mx) mx = sum;
@@ -230,7 +246,9 @@ struct stg_sim {
             for (int q = 0; q < SS; ++q) {
                 const double D = h_sD[(size_t)r * SS + q];
                 const double *g = &h_tri_g[3 * (size_t)i];
-                const double sum = ((D * g[0]) + (D * g[1])) + (D * g[2]);
+                const double *e = etab ? &h_tri_eD[((size_t)q * NTRI + i) * 3] : nullptr;
+                const double sum = e ? ((e[0] * g[0]) + (e[1] * g[1])) + (e[2] * g[2])
+                                     : ((D * g[0]) + (D * g[1])) + (D * g[2]);
                 if (sum > mx) mx = sum;
             }
         }
@@ -268,7 +286,8 @@ struct stg_sim {
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
                         d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D,
                         d_tri_nbr,   d_tri_nface,   d_tri_g,       d_tri_user,    d_real_sD,
-                        d_tout[0],   d_tout[1],     d_tri_corr};
+                        d_tout[0],   d_tout[1],     d_tri_corr,    d_real_tabs,   d_tet_k,
+                        d_tet_fD,    d_tri_sk,      d_tri_eD,      d_tet_clamp,   d_tri_clamp};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -660,6 +679,7 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     }
     build_diffusion_sets(s, model);
     (void)SS;
+    s->real_tabs.assign(n_real, 0);
     s->refresh_dt();
     s->nsteps.assign(n_real, 0);
     s->rng_steps.assign(n_real, 0);
@@ -844,6 +864,15 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.tout_prev = s->d_tout[launch_parity ^ 1];
     k.tout_cur = s->d_tout[launch_parity];
     k.tri_corr = s->d_tri_corr;
+    if (s->has_tabs()) {
+        k.real_tabs = s->d_real_tabs;
+        k.tet_k = s->d_tet_k;
+        k.tet_fD = s->d_tet_fD;
+        k.tet_clamp = s->d_tet_clamp;
+        k.tri_sk = s->d_tri_sk;
+        k.tri_eD = s->d_tri_eD;
+        k.tri_clamp = s->d_tri_clamp;
+    }
     return k;
 }
 
@@ -1015,7 +1044,8 @@ void unspecialize(stg_sim *s) {
 // plus the block-reduction array (static)
 size_t launch_smem(const stg_sim *s, bool *spec_out) {
     const int B = s->block;
-    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s));
+    // per-element tables are read by the generic membrane kernel only
+    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s)) && !s->has_tabs();
     if (spec_out) *spec_out = spec;
     return spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
 }
@@ -1523,6 +1553,138 @@ int stg_set_sdcst(stg_sim *s, int32_t real, int32_t sspec, double dcst) {
         s->refresh_dt();
         upload_real_tables(s);
         HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
+// ------------------------------------------------------ per-element tables
+}  // extern "C"
+
+namespace {
+
+// Copies realization real's rows of a table into its host copy (layout
+// [rows][n_total][width], the realization's elements at offset off..off+n)
+// and uploads the whole table; src == nullptr drops the realization's use.
+template <typename T>
+void put_table(stg_sim *s, int32_t real, int32_t bit, const T *src, int rows, int32_t n_total, int32_t off,
+               int32_t n, int width, std::vector<T> &host, T *&dev, const std::vector<int32_t> *perm = nullptr) {
+    if (src) {
+        const size_t sz = (size_t)std::max(rows, 1) * std::max(n_total, 1) * width;
+        if (host.size() != sz) host.assign(sz, T{});
+        for (int q = 0; q < rows; ++q)
+            for (int32_t i = 0; i < n; ++i) {
+                const int32_t j = perm ? (*perm)[i] : i;  // user order -> internal order
+                for (int w = 0; w < width; ++w)
+                    host[((size_t)q * n_total + off + j) * width + w] = src[((size_t)q * n + i) * width + w];
+            }
+        if (!dev) dev = dalloc<T>(sz);
+        HIP_OK(hipMemcpy(dev, host.data(), sz * sizeof(T), hipMemcpyHostToDevice));
+        s->real_tabs[real] |= bit;
+    } else {
+        s->real_tabs[real] &= ~bit;
+    }
+    if (!s->d_real_tabs) s->d_real_tabs = dalloc<int32_t>(s->n_real);
+    HIP_OK(hipMemcpy(s->d_real_tabs, s->real_tabs.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice));
+    s->free_graph();  // the kernel variant and its arguments change
+}
+
+}  // namespace
+
+extern "C" {
+
+int stg_set_tet_reac_table(stg_sim *s, int32_t real, const double *k) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tet_off[real + 1] - s->tet_off[real];
+        if (k)
+            for (size_t i = 0; i < (size_t)s->R * n; ++i)
+                if (!(k[i] >= 0.0) || !std::isfinite(k[i]))
+                    throw StgError{STG_ERR_ARG, "rate constants must be finite and >= 0"};
+        if (k && s->R == 0) return;
+        put_table(s, real, stg::kTabReac, k, s->R, s->NT, s->tet_off[real], n, 1, s->h_tet_k, s->d_tet_k);
+    });
+}
+
+int stg_set_tet_diff_table(stg_sim *s, int32_t real, const double *d) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tet_off[real + 1] - s->tet_off[real];
+        if (d) {
+            for (int sp = 0; sp < s->S; ++sp) {
+                const bool diffuses = std::find(s->mk.dspec, s->mk.dspec + s->mk.n_diff, sp) != s->mk.dspec + s->mk.n_diff;
+                for (size_t i = 0; i < (size_t)n * 4; ++i) {
+                    const double v = d[(size_t)sp * n * 4 + i];
+                    if (!(v >= 0.0) || !std::isfinite(v))
+                        throw StgError{STG_ERR_ARG, "diffusion constants must be finite and >= 0"};
+                    if (v > 0.0 && !diffuses)
+                        throw StgError{STG_ERR_NOTIMPL, "enabling diffusion of a species that was immobile at creation"};
+                }
+            }
+        }
+        put_table(s, real, stg::kTabDiff, d, s->S, s->NT, s->tet_off[real], n, 4, s->h_tet_fD, s->d_tet_fD);
+        s->refresh_dt();
+    });
+}
+
+int stg_set_tet_clamp(stg_sim *s, int32_t real, const uint32_t *mask) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tet_off[real + 1] - s->tet_off[real];
+        put_table(s, real, stg::kTabClamp, mask, 1, s->NT, s->tet_off[real], n, 1, s->h_tet_clamp, s->d_tet_clamp);
+    });
+}
+
+int stg_set_tri_sreac_table(stg_sim *s, int32_t real, const double *k) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tri_off[real + 1] - s->tri_off[real];
+        const int rows = s->RS + s->NG;
+        if (k)
+            for (size_t i = 0; i < (size_t)rows * n; ++i)
+                if (!(k[i] >= 0.0) || !std::isfinite(k[i]))
+                    throw StgError{STG_ERR_ARG, "surface constants and factors must be finite and >= 0"};
+        if (k && (rows == 0 || n == 0)) return;
+        put_table(s, real, stg::kTabTriReac, k, rows, s->NTRI, s->tri_off[real], n, 1, s->h_tri_sk, s->d_tri_sk,
+                  &s->tri_perm[real]);
+    });
+}
+
+int stg_set_tri_sdiff_table(stg_sim *s, int32_t real, const double *d) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tri_off[real + 1] - s->tri_off[real];
+        if (d) {
+            if (s->NSD == 0) throw StgError{STG_ERR_NOTIMPL, "no surface species diffuses in this model"};
+            for (int q = 0; q < s->SS; ++q) {
+                const bool diffuses = std::find(s->mk.sdspec, s->mk.sdspec + s->mk.n_sdiff, q) != s->mk.sdspec + s->mk.n_sdiff;
+                for (size_t i = 0; i < (size_t)n * 3; ++i) {
+                    const double v = d[(size_t)q * n * 3 + i];
+                    if (!(v >= 0.0) || !std::isfinite(v))
+                        throw StgError{STG_ERR_ARG, "diffusion constants must be finite and >= 0"};
+                    if (v > 0.0 && !diffuses)
+                        throw StgError{STG_ERR_NOTIMPL, "enabling surface diffusion of a species that was immobile at creation"};
+                }
+            }
+        }
+        if (d && n == 0) return;
+        put_table(s, real, stg::kTabTriDiff, d, s->SS, s->NTRI, s->tri_off[real], n, 3, s->h_tri_eD, s->d_tri_eD,
+                  &s->tri_perm[real]);
+        s->refresh_dt();
+    });
+}
+
+int stg_set_tri_clamp(stg_sim *s, int32_t real, const uint32_t *mask) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        const int32_t n = s->tri_off[real + 1] - s->tri_off[real];
+        if (mask && n == 0) return;
+        put_table(s, real, stg::kTabTriClamp, mask, 1, s->NTRI, s->tri_off[real], n, 1, s->h_tri_clamp,
+                  s->d_tri_clamp, &s->tri_perm[real]);
     });
 }
 

@@ -161,7 +161,20 @@ struct SysK {
     const int32_t *tout_prev;    // [n_sdiff][NTRI][4]
     int32_t *tout_cur;           // [n_sdiff][NTRI][4]
     double *tri_corr;            // [n_sspecs][NTRI] occupancy correction within the period
+    // Per-element tables, null when no realization has one; bit kTab* of
+    // real_tabs[r] says which ones realization r uses (membrane variant only)
+    const int32_t *real_tabs;    // [R]
+    const double *tet_k;         // [n_reacs][NT] rate constant of each reaction in each tet
+    const double *tet_fD;        // [n_specs][NT][4] diffusion constant through each face
+    const uint32_t *tet_clamp;   // [NT] bit s: species s clamped in the tet
+    const double *tri_sk;        // [n_sreacs + n_ghk][NTRI] surface constant / k(V) and GHK factor
+    const double *tri_eD;        // [n_sspecs][NTRI][3] surface diffusion constant through each edge
+    const uint32_t *tri_clamp;   // [NTRI] bit q: surface species q clamped on the triangle
 };
+
+// bits of SysK::real_tabs
+constexpr int32_t kTabReac = 1, kTabDiff = 2, kTabClamp = 4, kTabTriReac = 8, kTabTriDiff = 16,
+                  kTabTriClamp = 32;
 
 // both structures travel in the 4 KiB kernel-argument segment
 static_assert(sizeof(ModelK) + sizeof(SysK) + 16 <= 4096, "kernel arguments exceed 4 KiB");
@@ -291,7 +304,8 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 // GHK currents, k(V) tables or channel integrals) for volume-only models.
 template <bool MEMB, class MT>
 __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
-                          uint32_t ltet, uint32_t period, double dt, uint64_t seed, uint32_t stream) {
+                          uint32_t ltet, uint32_t period, double dt, uint64_t seed, uint32_t stream,
+                          int rtab = 0, uint32_t clampm = 0u) {
     const int R = tab_R<MT>(m), RS = m.n_sreacs, NG = m.n_ghk;
     const int RT = RS + NG;  // entries per triangle: surface reactions, then GHK currents
     int tp0 = 0, ntri = 0;
@@ -313,22 +327,39 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             for (int q = 0; q < MT::R; ++q) L.c(q, 0) = scaled_const(kr[q], MT::order[q], vscale, ivscale);
             j = MT::R;
         } else {
-            for (int q = 0; q < R; ++q, ++j) L.c(j, 0) = scaled_const(kr[q], m.order[q], vscale, ivscale);
+            for (int q = 0; q < R; ++q, ++j) {
+                // a per-tet table replaces the realization's constant
+                const double kq = (MEMB && (rtab & kTabReac)) ? sys.tet_k[(size_t)q * sys.n_tets + t] : kr[q];
+                L.c(j, 0) = scaled_const(kq, m.order[q], vscale, ivscale);
+            }
         }
         for (int i = 0; i < ntri; ++i) {
-            const double ascale = sys.tri_area[tp0 + i] * kAvogadro;
+            const int tri = tp0 + i;
+            const double ascale = sys.tri_area[tri] * kAvogadro;
             const double iascale = 1.0 / ascale;
+            const bool ttab = MEMB && (rtab & kTabTriReac);
             for (int q = 0; q < RS; ++q, ++j) {
-                const double k = m.s_vdep[q] < 0 ? ksr[q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tp0 + i]);
+                double k;
+                if (ttab) {
+                    // the triangle's constant, or its factor (1 active, 0 not) on k(V);
+                    // an inactive k(V) reaction reads no table
+                    const double f = sys.tri_sk[(size_t)q * sys.n_tris + tri];
+                    k = m.s_vdep[q] < 0 ? f : (f == 0.0 ? 0.0 : vdep_k(sys, m.s_vdep[q], sys.tri_v[tri]) * f);
+                } else {
+                    k = m.s_vdep[q] < 0 ? ksr[q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tri]);
+                }
                 L.c(j, 0) = m.s_nvol[q] == 0 ? scaled_const(k, m.sorder[q], ascale, iascale)
                                              : scaled_const(k, m.sorder[q], vscale, ivscale);
             }
             for (int q = 0; q < NG; ++q, ++j) {
-                if (!ghk_consts(m, q, sys.tri_v[tp0 + i], L.c(j, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1))) {
+                if (!ghk_consts(m, q, sys.tri_v[tri], L.c(j, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1))) {
                     atomicOr(sys.err, 4);
                     L.c(j, 0) = 0.0;
                     L.g(i * NG + q, 0) = 0.0;
                     L.g(i * NG + q, 1) = 1.0;
+                } else if (ttab) {
+                    // factor 1 or 0 on the current's P z F nuFoRT term
+                    L.c(j, 0) = L.c(j, 0) * sys.tri_sk[(size_t)(RS + q) * sys.n_tris + tri];
                 }
             }
         }
@@ -433,9 +464,11 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                     }
                 }
             } else {
+                // clamped species keep their counts (Reac::apply, reac.cpp:240-243)
                 for (int e = m.upd_off[sel]; e < m.upd_off[sel + 1]; ++e) {
                     const int sp = m.ent[e] & 0xffff;
                     const int dv = m.ent[e] >> 16;
+                    if (MEMB && ((clampm >> sp) & 1u)) continue;
                     L.c(sp) += dv;
                     if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
                 }
@@ -449,7 +482,7 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             const double rt = ghk_rt(m, q, L.c(sel, 0), L.g(i * NG + q, 0), L.g(i * NG + q, 1), L.c(ion),
                                      vscale);
             const bool efflux = rt >= 0.0;
-            if (m.g_real[q]) {
+            if (m.g_real[q] && !((clampm >> ion) & 1u)) {
                 const int dv = efflux ? -1 : 1;
                 L.c(ion) += dv;
                 if ((m.tracked_mask >> ion) & 1) L.o(ion) -= (double)dv * tcur;
@@ -459,16 +492,19 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             const int i = (sel - R) / RT;
             const int qs = (sel - R) - i * RT;
             const int tri = tp0 + i;
+            const uint32_t tclamp = (MEMB && (rtab & kTabTriClamp)) ? sys.tri_clamp[tri] : 0u;
             for (int e = m.supd_off[qs]; e < m.supd_off[qs + 1]; ++e) {
                 const int sp = m.ent[e] & 0xffff;
                 const int dv = m.ent[e] >> 16;
                 if (sp & kSurfFlag) {
+                    if ((tclamp >> (sp & 0x7f)) & 1u) continue;
                     const size_t ix = (size_t)(sp & 0x7f) * sys.n_tris + tri;
                     sys.tri_counts[ix] += dv;
                     // integral of n over the period = n_end * dt - sum dn * t_event
                     if ((m.tri_intg_mask >> (sp & 0x7f)) & 1) sys.tri_intg[ix] -= (double)dv * tcur;
                     if ((m.tri_tracked_mask >> (sp & 0x7f)) & 1) sys.tri_corr[ix] -= (double)dv * tcur;
                 } else {
+                    if ((clampm >> sp) & 1u) continue;
                     L.c(sp) += dv;
                     if ((m.tracked_mask >> sp) & 1) L.o(sp) -= (double)dv * tcur;
                 }
@@ -731,17 +767,20 @@ __device__ __forceinline__ bool almost_zero_dt(double dt) {
 
 // Leaving pass of one (tet, species): returns molecules moved, writes 4 slots.
 // Faces on a diffusion boundary (bdir byte != 0xff) take the boundary's
-// constant for this species instead of D (0 when it is closed).
+// constant for this species instead of D (0 when it is closed); a per-face
+// table fD (non-null) replaces both. A clamped source keeps its count: the
+// molecules that leave are created (Diff::apply, diff.cpp:476-520, 577).
 __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, double D,
                          const double r4in[4], uint32_t bdir, int S, double dt, uint64_t seed,
-                         uint32_t ltet, uint32_t period, uint32_t stream, int4 &o) {
+                         uint32_t ltet, uint32_t period, uint32_t stream, int4 &o,
+                         const double *fD = nullptr, bool clamped = false) {
     o = make_int4(0, 0, 0, 0);
     const int32_t n = L.c(sp);
     double r4[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
         const uint32_t k = (bdir >> (8 * f)) & 0xffu;
-        const double Df = k == 0xffu ? D : sys.bnd_D[(size_t)k * S + sp];
+        const double Df = fD != nullptr ? fD[f] : (k == 0xffu ? D : sys.bnd_D[(size_t)k * S + sp]);
         r4[f] = Df * r4in[f];
     }
     const double sum = ((r4[0] + r4[1]) + r4[2]) + r4[3];
@@ -760,7 +799,7 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     p = p < 0.0 ? 0.0 : p;
     const int32_t nl = binomial(g, mp, p);
     if (nl <= 0) return 0;
-    L.c(sp) = n - nl;
+    if (!clamped) L.c(sp) = n - nl;
     int lastf = -1;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
@@ -807,14 +846,16 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
 // Surface diffusion out of one (triangle, surface species): the leaving pass
 // of the volume with three edge directions (SDiff, sdiff.cpp:42-90, and the
 // SDiff block of tetopsplit.cpp:1716-1788). Writes three edge slots.
+// A per-edge table eD (non-null) replaces D; a clamped source keeps its count.
 __device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int tri, double dt,
-                             uint64_t seed, uint32_t period, uint32_t stream, int4 &o) {
+                             uint64_t seed, uint32_t period, uint32_t stream, int4 &o,
+                             const double *eD = nullptr, bool clamped = false) {
     o = make_int4(0, 0, 0, 0);
     const size_t ix = (size_t)q * sys.n_tris + tri;
     const int32_t n = sys.tri_counts[ix];
     double r3[3];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) r3[e] = D * sys.tri_g[3 * (size_t)tri + e];
+    for (int e = 0; e < 3; ++e) r3[e] = (eD != nullptr ? eD[e] : D) * sys.tri_g[3 * (size_t)tri + e];
     const double sum = (r3[0] + r3[1]) + r3[2];
     if (n <= 0 || !(sum > 0.0)) return 0;
     Philox g;
@@ -831,7 +872,7 @@ __device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int
     p = p < 0.0 ? 0.0 : p;
     const int32_t nl = binomial(g, mp, p);
     if (nl <= 0) return 0;
-    sys.tri_counts[ix] = n - nl;
+    if (!clamped) sys.tri_counts[ix] = n - nl;
     int lastf = -1;
 #pragma unroll
     for (int e = 0; e < 3; ++e)
@@ -927,8 +968,13 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     const bool valid = t < tend;
     long nev = 0;
     int32_t moved = 0;
+    // per-element tables of this realization (uniform over the block)
+    const int rtab = (MEMB && sys.real_tabs != nullptr) ? sys.real_tabs[r] : 0;
     if (valid) {
         const int NT = sys.n_tets;
+        // clamped species: nothing enters, leaves or reacts away (Diff::apply,
+        // diff.cpp:476-577; Reac::apply, reac.cpp:240-243)
+        const uint32_t clampm = (MEMB && (rtab & kTabClamp)) ? sys.tet_clamp[t] : 0u;
         for (int s = 0; s < S; ++s) {
             L.c(s) = sys.counts[(size_t)s * NT + t];
             L.o(s) = 0.0;
@@ -954,7 +1000,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     if (v[f] == kSlotEscape) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
                     add += nbv[f] >= 0 ? v[f] : 0;
                 }
-                L.c(tab_dspec<MT>(m, d)) += add;
+                const int dsp = tab_dspec<MT>(m, d);
+                if (!MEMB || !((clampm >> dsp) & 1u)) L.c(dsp) += add;
             }
         }
         int tp0 = 0, tp1 = 0;
@@ -968,13 +1015,14 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 const int4 nb = sys.tri_nbr[tri];
                 const uint32_t ne = sys.tri_nface[tri];
                 const int nbv[3] = {nb.x, nb.y, nb.z};
+                const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[tri] : 0u;
                 for (int d = 0; d < m.n_sdiff; ++d) {
                     const int32_t *op = sys.tout_prev + (size_t)d * sys.n_tris * 4;
                     int32_t add = 0;
 #pragma unroll
                     for (int e = 0; e < 3; ++e)
                         if (nbv[e] >= 0) add += op[(size_t)nbv[e] * 4 + ((ne >> (8 * e)) & 0xff)];
-                    sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
+                    if (!((tclamp >> m.sdspec[d]) & 1u)) sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
                 }
             }
         }
@@ -993,7 +1041,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             if constexpr (MT::kStatic && !MEMB) nev = ssa_lean<MT>(sys, L, r, t, ltet, period, dt, seed, stream);
             else if constexpr (MT::kStatic && MEMB)
                 nev = ssa_memb_lean<MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
-            else nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
+            else nev = ssa_group<MEMB, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream, rtab, clampm);
             if (nev < 0) {
                 atomicOr(sys.err, 1);
                 nev = 0;
@@ -1020,8 +1068,9 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     const double2 g23 = sys.g[2 * (size_t)tg + 1];
                     const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
                     int4 o;
+                    const double *fD = (MEMB && (rtab & kTabDiff)) ? sys.tet_fD + ((size_t)sp * NT + t) * 4 : nullptr;
                     moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, uni(Dr[sp]), r4, bdir, S, dt,
-                                   seed, ltet, period, stream, o);
+                                   seed, ltet, period, stream, o, fD, MEMB && ((clampm >> sp) & 1u));
                     put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
                 }
             }
@@ -1029,11 +1078,13 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 const double *sDr = sys.real_sD + (size_t)r * m.n_sspecs;
                 int4 *toc = reinterpret_cast<int4 *>(sys.tout_cur);
                 for (int tri = tp0; tri < tp1; ++tri) {
+                    const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[tri] : 0u;
                     for (int d = 0; d < m.n_sdiff; ++d) {
                         const int q = m.sdspec[d];
                         int4 o;
+                        const double *eD = (rtab & kTabTriDiff) ? sys.tri_eD + ((size_t)q * sys.n_tris + tri) * 3 : nullptr;
                         moved += tri_leave(sys, q, (m.tri_tracked_mask >> q) & 1, uni(sDr[q]), tri, dt, seed,
-                                           period, stream, o);
+                                           period, stream, o, eD, (tclamp >> q) & 1u);
                         toc[(size_t)d * sys.n_tris + tri] = o;
                     }
                 }
@@ -1078,6 +1129,8 @@ __global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) 
     const int t = bd.z + (int)threadIdx.x;
     if (t >= bd.w) return;
     const int NT = sys.n_tets;
+    const int rtab = (MEMB && sys.real_tabs != nullptr) ? sys.real_tabs[bd.y] : 0;
+    const uint32_t clampm = (rtab & kTabClamp) ? sys.tet_clamp[t] : 0u;
     const int4 nb4 = sys.nbr[t];
     const uint32_t nf = sys.nface[t];
     const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
@@ -1093,20 +1146,21 @@ __global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) 
             if (v == kSlotEscape) v = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
             add += nbv[f] >= 0 ? v : 0;
         }
-        sys.counts[(size_t)m.dspec[d] * NT + t] += add;
+        if (!((clampm >> m.dspec[d]) & 1u)) sys.counts[(size_t)m.dspec[d] * NT + t] += add;
     }
     if (MEMB && m.n_sdiff > 0 && sys.tri_ptr != nullptr) {
         for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri) {
             const int4 nb = sys.tri_nbr[tri];
             const uint32_t ne = sys.tri_nface[tri];
             const int tnb[3] = {nb.x, nb.y, nb.z};
+            const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[tri] : 0u;
             for (int d = 0; d < m.n_sdiff; ++d) {
                 const int32_t *op = sys.tout_prev + (size_t)d * sys.n_tris * 4;
                 int32_t add = 0;
 #pragma unroll
                 for (int e = 0; e < 3; ++e)
                     if (tnb[e] >= 0) add += op[(size_t)tnb[e] * 4 + ((ne >> (8 * e)) & 0xff)];
-                sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
+                if (!((tclamp >> m.sdspec[d]) & 1u)) sys.tri_counts[(size_t)m.sdspec[d] * sys.n_tris + tri] += add;
             }
         }
     }

@@ -244,6 +244,44 @@ class Engine:
         self._chk(self.lib.stg_get_temp(self.h, C.byref(x)))
         return x.value
 
+    # --- per-element tables (stg_set_tet_reac_table and the like); None removes ---
+    def _table(self, fn, real, arr, shape, dtype, ctype):
+        if arr is None:
+            self._chk(fn(self.h, int(real), None))
+            return
+        a = np.ascontiguousarray(np.asarray(arr, dtype=dtype))
+        if a.shape != shape:
+            raise ValueError(f"expected a table of shape {shape}, got {a.shape}")
+        self._chk(fn(self.h, int(real), a.ctypes.data_as(C.POINTER(ctype))))
+
+    def set_tet_reac_table(self, k, real: int = 0):
+        """[n_reacs][n_tets] rate constant of each reaction in each tet (0: inactive)."""
+        self._table(self.lib.stg_set_tet_reac_table, real, k, (len(self.fm.reacs), self.n_tets[real]),
+                    np.float64, C.c_double)
+
+    def set_tet_diff_table(self, d, real: int = 0):
+        """[n_specs][n_tets][4] diffusion constant through each face."""
+        self._table(self.lib.stg_set_tet_diff_table, real, d, (len(self.fm.specs), self.n_tets[real], 4),
+                    np.float64, C.c_double)
+
+    def set_tet_clamp(self, mask, real: int = 0):
+        """[n_tets] bit s set: species s clamped in that tet."""
+        self._table(self.lib.stg_set_tet_clamp, real, mask, (self.n_tets[real],), np.uint32, C.c_uint32)
+
+    def set_tri_sreac_table(self, k, real: int = 0):
+        """[n_sreacs + n_ghk][n_tris] surface constants (factors on k(V) and GHK)."""
+        self._table(self.lib.stg_set_tri_sreac_table, real, k,
+                    (len(self.fm.sreacs) + len(self.fm.ghk_chan), self.n_tris[real]), np.float64, C.c_double)
+
+    def set_tri_sdiff_table(self, d, real: int = 0):
+        """[n_sspecs][n_tris][3] surface diffusion constant through each edge."""
+        self._table(self.lib.stg_set_tri_sdiff_table, real, d, (len(self.fm.sspecs), self.n_tris[real], 3),
+                    np.float64, C.c_double)
+
+    def set_tri_clamp(self, mask, real: int = 0):
+        """[n_tris] bit q set: surface species q clamped on that triangle."""
+        self._table(self.lib.stg_set_tri_clamp, real, mask, (self.n_tris[real],), np.uint32, C.c_uint32)
+
     def set_diff_apply_threshold(self, threshold: int):
         self._chk(self.lib.stg_set_diff_apply_threshold(self.h, int(threshold)))
 

@@ -91,9 +91,10 @@ def test_reset_loop_gives_the_binomial_decay_ensemble():
 
 
 def _channel_membrane_sim(seed):
-    """A cube whose whole surface is membrane: channels open with a
-    voltage-dependent rate and pass an ohmic current and a GHK calcium current;
-    calcium diffuses inside."""
+    """A 2 um cube whose whole surface (24 um^2, 0.24 pF) is membrane: 40
+    channels open with a voltage-dependent rate and pass an ohmic current and a
+    GHK calcium current, 6 vertices take 0.1 pA each; calcium diffuses inside.
+    The potential moves by a few mV per ms, inside the k(V) table."""
     from steps_amd.solver import EF_DEFAULT, TetOpSplit
 
     mesh = geom.kuhn_cube(4, 2e-6)
@@ -111,18 +112,18 @@ def _channel_membrane_sim(seed):
     c, o = M.ChanState("C", mdl, ch), M.ChanState("O", mdl, ch)
     M.VDepSReac("open", ss, slhs=[c], srhs=[o], k=lambda v: 400.0 / (1.0 + np.exp(-(v + 0.03) / 0.01)))
     M.SReac("close", ss, slhs=[o], srhs=[c], kcst=200.0)
-    M.OhmicCurr("oc", ss, o, 0.0, 20e-12)
+    M.OhmicCurr("oc", ss, o, 0.0, 2e-12)
     g = M.GHKcurr("ghk", ss, o, ca, computeflux=True, virtual_oconc=2e-3)
-    g.setPInfo(5e-12, -65e-3, 293.15, 2e-3, 1e-7)
+    g.setPInfo(1e-12, -65e-3, 293.15, 2e-3, 1e-7)
     r = R.create("r123", 1024)
     r.initialize(seed)
     sim = TetOpSplit(mdl, mesh, r, EF_DEFAULT)
     sim.setEfieldDT(5e-5)
-    sim.setPatchSpecCount("memb", "C", 400)
+    sim.setPatchSpecCount("memb", "C", 40)
     sim.setCompSpecConc("cyto", "Ca", 5e-6)
     v = np.unique(mesh.tris[mesh.getSurfTris()])[:6]
     for x in v:
-        sim.setVertIClamp(int(x), 2e-11)
+        sim.setVertIClamp(int(x), 1e-13)
     return sim, mesh
 
 
@@ -169,6 +170,6 @@ def test_reset_puts_the_membrane_back_to_its_default_potential():
     sim.reset()
     assert sim.getTime() == 0.0 and sim.getNSteps() == 0
     assert np.all(_state(sim, mesh)[2] == -0.065)
-    sim.setPatchSpecCount("memb", "C", 400)
+    sim.setPatchSpecCount("memb", "C", 40)
     sim.run(1e-3)
     assert sim.getTime() == 1e-3

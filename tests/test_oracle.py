@@ -806,3 +806,77 @@ def test_oracle_reset_keeps_the_philox_counter():
     o2.counts[:] = w.counts
     o2.run(0.05, 0)
     assert np.array_equal(o2.counts, o.counts)
+
+
+def _clamp_decay_case(seed, t_end=3.0):
+    """A (clamped at 100 per tet) -> A + B at k = 0.5 /s, B -> 0 at k2 = 2 /s, A
+    and B diffusing, on a 2^3 x 6 cube (48 tets)."""
+    mdl = M.Model()
+    A, B = M.Spec("A", mdl), M.Spec("B", mdl)
+    vs = M.Volsys("vsys", mdl)
+    M.Reac("make", vs, lhs=[A], rhs=[B], kcst=0.5)
+    M.Reac("decay", vs, lhs=[B], rhs=[], kcst=2.0)
+    M.Diff("dA", vs, A, 1e-12)
+    M.Diff("dB", vs, B, 1e-12)
+    mesh = geom.kuhn_cube(2, 2e-6)
+    comp = geom.TmComp("c", mesh)
+    comp.addVolsys("vsys")
+    fm = M.flatten(mdl, ["vsys"])
+    mt = geom.mesh_tables(mesh, comp)
+    o = bind.OracleSim(fm, mt, seed=seed)
+    o.counts[0] = 100
+    o.set_tables(tet_clamp=np.ones(len(mt.vol), dtype=np.uint32))   # species 0 = A
+    return o, fm, mt
+
+
+def test_oracle_clamped_source_feeds_decay_to_its_steady_state():
+    """A clamped species never changes (reactions consume it and diffusion moves
+    it out, neither reduces it: reac.cpp:240-243, diff.cpp:476-577), so B is an
+    immigration-death process with E[B(t)] = k N_A / k2 (1 - exp(-k2 t)) and
+    Poisson fluctuations. Dropping the clamp (a plausible mistake) drains A
+    and B falls far below."""
+    t, niter = 3.0, 16
+    mean = 0.5 * 4800 / 2.0 * (1 - np.exp(-2.0 * t))
+    out = []
+    for seed in range(niter):
+        o, fm, mt = _clamp_decay_case(seed)
+        o.run(t, 0)
+        assert np.all(o.counts[0] == 100)
+        out.append(o.counts[1].sum())
+    z = (np.mean(out) - mean) / np.sqrt(mean / niter)
+    assert abs(z) < 3.0, (np.mean(out), mean)
+    o, fm, mt = _clamp_decay_case(99)
+    o.set_tables()
+    o.run(t, 0)
+    assert o.counts[0].sum() < 4800 and o.counts[1].sum() < 0.5 * mean
+
+
+def test_oracle_face_table_closes_and_opens_faces():
+    """A per-face diffusion table replaces the constants: zero on every face of
+    the x < side/2 tets and on their neighbours' faces toward them freezes that
+    half (no reactions), while the other half keeps diffusing and its total is
+    conserved; a table equal to the constants reproduces the plain run."""
+    w = W.ab_binding_cube(n=4, side=4e-6, n_a=4000, n_b=0, seed=2, kf=0.0, kb=0.0)
+    left = w.mesh.tet_bary[:, 0] < 2e-6
+    S, N = 3, w.n_tets
+    D = np.broadcast_to(w.fm.spec_dcst[:, None, None], (S, N, 4)).copy()
+    plain = bind.OracleSim(w.fm, w.mt, seed=4)
+    plain.counts[:] = w.counts
+    same = bind.OracleSim(w.fm, w.mt, seed=4)
+    same.counts[:] = w.counts
+    same.set_tables(tet_fD=D)
+    plain.run(0.2, 0)
+    same.run(0.2, 0)
+    assert np.array_equal(plain.counts, same.counts)
+    D[:, left, :] = 0.0
+    nb = w.mt.nbr.reshape(N, 4)
+    for f in range(4):
+        to_left = (nb[:, f] >= 0) & left[np.maximum(nb[:, f], 0)]
+        D[:, to_left, f] = 0.0
+    o = bind.OracleSim(w.fm, w.mt, seed=4)
+    o.counts[:] = w.counts
+    o.set_tables(tet_fD=D)
+    o.run(0.2, 0)
+    assert np.array_equal(o.counts[:, left], w.counts[:, left])
+    assert o.counts[0, ~left].sum() == w.counts[0, ~left].sum()
+    assert not np.array_equal(o.counts[:, ~left], w.counts[:, ~left])
