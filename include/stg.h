@@ -201,6 +201,18 @@ int stg_get_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int32_t *count
 int stg_get_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets,
                              size_t n, double *counts);
 
+/* No reference counterpart. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the result of stg_get_batch_tet_counts for at most
+ * STG_NOALLOC_MAX_TETS tets (STG_ERR_ARG beyond), with no allocation. The
+ * indices and gathered counts are staged in a fixed 1 MiB device array
+ * (16 bytes per tet) that the library holds on each GPU it runs on until the
+ * process ends. All calls on one GPU share that array: calls that overlap in
+ * time on one GPU (other threads or streams) race. */
+#define STG_NOALLOC_MAX_TETS 65536
+int stg_get_batch_tet_counts_noalloc(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets,
+                                     size_t n, double *counts);
+
 /* Total count of species spec over the listed tets of a realization, reduced
  * on the device (sumBatchTetCountsNP / getROISpecCount, tetopsplit.cpp:5715-5778). */
 int stg_sum_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int64_t *tets, size_t n,

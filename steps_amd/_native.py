@@ -144,6 +144,8 @@ SIGNATURES = {
                                    C.POINTER(StgRealization), C.c_int32, C.POINTER(C.c_void_p)]),
     "stg_sum_batch_tet_counts": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64),
                                            C.c_size_t, C.POINTER(C.c_int64)]),
+    "stg_get_batch_tet_counts_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32,
+                                                   C.POINTER(C.c_int64), C.c_size_t, _dp]),
     "stg_set_sdcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_get_boundary_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),

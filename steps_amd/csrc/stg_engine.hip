@@ -1408,11 +1408,15 @@ int stg_get_tri_counts(stg_sim *s, int32_t real, int32_t sspec, int32_t *counts)
     });
 }
 
-// device copy of a tet index list, checked against the realization's size
-static void stage_indices(stg_sim *s, int32_t real, const int64_t *tets, size_t n) {
+static void check_indices(const stg_sim *s, int32_t real, const int64_t *tets, size_t n) {
     const int64_t nt = s->tet_off[real + 1] - s->tet_off[real];
     for (size_t i = 0; i < n; ++i)
         if (tets[i] < 0 || tets[i] >= nt) throw StgError{STG_ERR_ARG, "tetrahedron index out of range"};
+}
+
+// device copy of a tet index list, checked against the realization's size
+static void stage_indices(stg_sim *s, int32_t real, const int64_t *tets, size_t n) {
+    check_indices(s, real, tets, n);
     if (n > s->idx_cap) {
         if (s->d_idx) (void)hipFree(s->d_idx);
         if (s->d_gather) (void)hipFree(s->d_gather);
@@ -1439,6 +1443,61 @@ int stg_get_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64
                            s->d_idx, n, s->tet_off[real], s->d_gather);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(counts, s->d_gather, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipStreamSynchronize(s->stream));
+    });
+}
+
+}  // extern "C"
+
+// Staging of the allocation-free batch getter: up to STG_NOALLOC_MAX_TETS tet
+// indices (int64) followed by as many gathered counts (double), 1 MiB. Each
+// GPU the library runs on holds its own copy until the process ends. Host
+// code reaches it only through the address hipGetSymbolAddress returns for
+// the device (scratch_address), never through the array's name.
+static __device__ __align__(256) unsigned char scratch[STG_NOALLOC_MAX_TETS * (sizeof(int64_t) + sizeof(double))];
+
+namespace {
+std::mutex g_scratch_mutex;
+std::map<int, unsigned char *> g_scratch_addr;  // device -> its copy of scratch
+
+// the current device's copy, looked up once per device with that device current
+unsigned char *scratch_address() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    auto it = g_scratch_addr.find(dev);
+    if (it != g_scratch_addr.end()) return it->second;
+    unsigned char *p = nullptr;
+    HIP_OK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    if (!p) throw StgError{STG_ERR_DEVICE, "no device address for the staging array"};
+    g_scratch_addr[dev] = p;
+    return p;
+}
+}  // namespace
+
+extern "C" {
+
+int stg_get_batch_tet_counts_noalloc(stg_sim *s, int32_t real, int32_t spec, const int64_t *tets, size_t n,
+                                     double *counts) {
+    if (!s || (n > 0 && (!tets || !counts))) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
+        if (n > STG_NOALLOC_MAX_TETS)
+            throw StgError{STG_ERR_ARG, "at most " + std::to_string(STG_NOALLOC_MAX_TETS) +
+                                            " tets per call of the allocation-free batch getter"};
+        check_indices(s, real, tets, n);
+        if (n == 0) return;
+        unsigned char *base = scratch_address();
+        int64_t *d_idx = reinterpret_cast<int64_t *>(base);
+        double *d_out = reinterpret_cast<double *>(base + STG_NOALLOC_MAX_TETS * sizeof(int64_t));
+        HIP_OK(hipMemcpyAsync(d_idx, tets, sizeof(int64_t) * n, hipMemcpyHostToDevice, s->stream));
+        const int B = 256;
+        const int nb = (int)((n + B - 1) / B);
+        hipLaunchKernelGGL(stg::k_gather_counts, dim3(nb), dim3(B), 0, s->stream, s->d_counts + (size_t)spec * s->NT,
+                           d_idx, n, s->tet_off[real], d_out);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(counts, d_out, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
     });
 }
