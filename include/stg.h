@@ -63,6 +63,16 @@ typedef struct stg_mesh {
     const int32_t *tri_nbr;      /* [n_tris*3] */
     const int32_t *tri_nbr_edge; /* [n_tris*3] */
     const double *tri_edge_g;    /* [n_tris*3] */
+    /* Outer side of each triangle (Tri::oTet, src/steps/mpi/tetopsplit/tri.hpp),
+     * NULL when no triangle has one: tri_otet[i] is the tet across the
+     * membrane (-1 none) in compartment tri_ocomp[i] (NULL: this one). A
+     * triangle whose surface reactions or GHK currents reach the outer
+     * compartment joins its inner and outer tets into one SSA group. */
+    const int32_t *tri_otet;     /* [n_tris] */
+    const int32_t *tri_ocomp;    /* [n_tris] */
+    /* Compartment of each edge neighbour (stg_create_comps only; NULL: this
+     * one), for surface diffusion across patches of different compartments. */
+    const int32_t *tri_nbr_comp; /* [n_tris*3] */
 } stg_mesh;
 
 /* Flattened model of one compartment (+ one patch), as the reference's
@@ -96,17 +106,26 @@ typedef struct stg_model {
      * ghkcurr.cpp:168-272; math::GHKcurrent, src/steps/math/ghk.cpp:62-81):
      * on every membrane triangle, ions of volume species ghk_ion cross at the
      * GHK flux of each channel in surface species ghk_chan. The outer side is
-     * a virtual concentration ghk_voconc (M); ghk_realflux = 0 moves charge
-     * but no ions. Up to 4 currents. */
+     * a virtual concentration ghk_voconc (M) when >= 0, else the triangle's
+     * outer tet; ghk_realflux = 0 moves charge but no ions. Up to 4 currents. */
     int32_t n_ghk;
     const int32_t *ghk_chan;     /* [n_ghk] surface species of the open channel */
     const int32_t *ghk_ion;      /* [n_ghk] volume species of the ion */
     const int32_t *ghk_valence;  /* [n_ghk] nonzero */
     const double *ghk_perm;      /* [n_ghk] single-channel permeability, m^3/s */
-    const double *ghk_voconc;    /* [n_ghk] virtual outer concentration, M, >= 0 */
+    const double *ghk_voconc;    /* [n_ghk] virtual outer concentration, M, or < 0: the outer tet */
     const double *ghk_vshift;    /* [n_ghk] V added to the membrane potential */
     const int32_t *ghk_realflux; /* [n_ghk] 1: the ion count changes per event */
     const double *sspec_dcst;    /* [n_sspecs] surface diffusion constant m^2/s, or NULL */
+    /* Outer-compartment reactants and net changes of each surface reaction
+     * (SReac olhs / orhs; SReac::rate / apply, sreac.cpp:305-505), NULL when
+     * none: [n_sreacs*n_specs]. A reaction has inner or outer volume
+     * reactants, not both (model/sreac.cpp:55); with outer ones its constant
+     * scales with the outer tet's volume. A GHK current with ghk_voconc < 0
+     * reads the outer tet's concentration and moves its ions to and from it
+     * (ghkcurr.cpp:176-182, 228-272). */
+    const int32_t *sreac_lhs_o;
+    const int32_t *sreac_upd_o;
 } stg_model;
 
 /* Per-realization parameters of a batch; NULL arrays take the model's values. */

@@ -41,6 +41,7 @@ class StgoSystem(C.Structure):
         ("tri_ltri", _ip),
         ("tet_k", _dp), ("tet_fD", _dp), ("tet_clamp", C.POINTER(C.c_uint32)),
         ("tri_sk", _dp), ("tri_eD", _dp), ("tri_clamp", C.POINTER(C.c_uint32)),
+        ("sreac_lhs_o", _ip), ("sreac_upd_o", _ip), ("tri_otet", _ip),
     ]
 
 
@@ -192,6 +193,13 @@ class OracleSim:
             self.sys.tri_nbr_edge = P(a["te"], C.c_int32)
             self.sys.tri_edge_g = P(a["tg"], C.c_double)
         self.sys.sspec_dcst = P(a["sdc"], C.c_double)
+        if getattr(fm, "sreac_lhs_o", None) is not None and len(fm.sreacs):
+            a["slo"], a["suo"] = i32(fm.sreac_lhs_o), i32(fm.sreac_upd_o)
+            self.sys.sreac_lhs_o = P(a["slo"], C.c_int32)
+            self.sys.sreac_upd_o = P(a["suo"], C.c_int32)
+        if getattr(mt, "tri_otet", None) is not None and self.n_tris:
+            a["tot"] = i32(mt.tri_otet)
+            self.sys.tri_otet = P(a["tot"], C.c_int32)
         self.scratch = np.zeros(S * self.n_tets * 4, dtype=np.int32)
         self.nsteps = 0
         # Philox period counter: advances with nsteps, survives reset()
@@ -236,6 +244,12 @@ class OracleSim:
                                         for c, m in enumerate(comp_meshes) if len(m.tri_tet)]),
                 tri_nbr_edge=np.concatenate([m.tri_nbr_edge for m in comp_meshes if len(m.tri_tet)]),
                 tri_edge_g=np.concatenate([m.tri_edge_g for m in comp_meshes if len(m.tri_tet)]))
+        if any(m.tri_otet is not None for m in comp_meshes):
+            # outer tets in the concatenated (global) tet numbering
+            extra["tri_otet"] = np.concatenate([
+                np.where(m.tri_otet >= 0, m.tri_otet + off[m.tri_ocomp if m.tri_ocomp is not None else c], -1)
+                if m.tri_otet is not None else -np.ones(len(m.tri_tet), dtype=np.int64)
+                for c, m in enumerate(comp_meshes)]).astype(np.int32)
         mt = MeshTables(vol=np.concatenate([m.vol for m in comp_meshes]), nbr=np.concatenate(nbr),
                         nbr_face=np.concatenate(nf), face_g=np.concatenate([m.face_g for m in comp_meshes]),
                         tri_tet=tri_tet.astype(np.int32),

@@ -444,7 +444,8 @@ static int ghk_consts(const stgo_system *s, int32_t g, double v, double *A, doub
     const double nuFoRT = z * (v + s->ghk_vshift[g]) * FARADAY / (GAS_CONSTANT * s->temp);
     if (nuFoRT >= 308 * 2.30 || nuFoRT <= -307 * 2.30) return 0;
     const double e = stgo_exp(-nuFoRT);
-    const double oconc = s->ghk_voconc[g] * 1.0e3;
+    /* a current reading the outer tet, on a triangle without one: 0 */
+    const double oconc = s->ghk_voconc[g] < 0.0 ? 0.0 : s->ghk_voconc[g] * 1.0e3;
     if (fabs(nuFoRT) > DBL_EPSILON) {
         *A = s->ghk_perm[g] * z * FARADAY * nuFoRT;
         *Bo = oconc * e;
@@ -473,6 +474,9 @@ static double falling(double h, int32_t n, int32_t lhs) {
 }
 
 typedef struct {
+    int32_t *tet_grp; /* [n_tets] SSA group of a tet, -1 = its own; null when none */
+    int32_t *grp_ptr; /* [groups+1] */
+    int32_t *grp_mem; /* members, tet order (the first owns the group) */
     int32_t *tri_ptr; /* [n_tets+1] */
     int32_t *tri_idx; /* [n_tris]   */
     int32_t max_tris;
@@ -484,6 +488,8 @@ typedef struct {
 static int sspec_diffuses(const stgo_system *s, int32_t q) {
     return s->sspec_dcst && s->tri_nbr && s->sspec_dcst[q] > 0.0;
 }
+
+static void groups_build(const stgo_system *s, topo *tp);
 
 static void topo_build(const stgo_system *s, topo *tp) {
     tp->tri_ptr = (int32_t *)calloc((size_t)s->n_tets + 1, sizeof(int32_t));
@@ -508,7 +514,9 @@ static void topo_build(const stgo_system *s, topo *tp) {
         for (int32_t r = 0; r < s->n_reacs; ++r)
             if (s->reac_upd[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
         for (int32_t r = 0; r < s->n_sreacs; ++r)
-            if (s->sreac_upd_i[r * s->n_specs + sp] != 0) tp->tracked[sp] = 1;
+            if (s->sreac_upd_i[r * s->n_specs + sp] != 0 ||
+                (s->sreac_upd_o && s->sreac_upd_o[r * s->n_specs + sp] != 0))
+                tp->tracked[sp] = 1;
         for (int32_t g = 0; g < s->n_ghk; ++g)
             if (s->ghk_ion[g] == sp && s->ghk_realflux[g]) tp->tracked[sp] = 1;
     }
@@ -520,9 +528,83 @@ static void topo_build(const stgo_system *s, topo *tp) {
         for (int32_t r = 0; r < s->n_sreacs; ++r)
             if (s->sreac_upd_s[r * s->n_sspecs + q] != 0) tp->tri_tracked[q] = 1;
     }
+    groups_build(s, tp);
+}
+
+/* does the model reach the outer side of the membrane? */
+static int outer_model(const stgo_system *s) {
+    for (int32_t i = 0; i < s->n_sreacs * s->n_specs; ++i)
+        if ((s->sreac_lhs_o && s->sreac_lhs_o[i]) || (s->sreac_upd_o && s->sreac_upd_o[i])) return 1;
+    for (int32_t g = 0; g < s->n_ghk; ++g)
+        if (s->ghk_voconc[g] < 0.0) return 1;
+    return 0;
+}
+
+static int32_t uf_find(int32_t *parent, int32_t x) {
+    while (parent[x] != x) x = parent[x] = parent[parent[x]];
+    return x;
+}
+
+/* SSA groups: a triangle with an outer tet joins it to the inner tet when
+ * the model's surface reactions or GHK currents reach the outer side; the
+ * reaction system of joined tets is one SSA (the reference's TetOpSplitP runs
+ * one SSA over every kproc, tetopsplit.cpp:3217-3354; with diffusion frozen
+ * in the period, groups that share no species evolve independently) */
+static void groups_build(const stgo_system *s, topo *tp) {
+    tp->tet_grp = NULL;
+    tp->grp_ptr = NULL;
+    tp->grp_mem = NULL;
+    if (!s->tri_otet || !outer_model(s)) return;
+    const int32_t N = s->n_tets;
+    int32_t *parent = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    for (int32_t t = 0; t < N; ++t) parent[t] = t;
+    int any = 0;
+    for (int32_t tr = 0; tr < s->n_tris; ++tr) {
+        const int32_t o = s->tri_otet[tr];
+        if (o < 0) continue;
+        int32_t a = uf_find(parent, s->tri_tet[tr]), b = uf_find(parent, o);
+        if (a != b) parent[a > b ? a : b] = a < b ? a : b;
+        any = 1;
+    }
+    if (!any) {
+        free(parent);
+        return;
+    }
+    int32_t *size = (int32_t *)calloc((size_t)N, sizeof(int32_t));
+    for (int32_t t = 0; t < N; ++t) size[uf_find(parent, t)]++;
+    int32_t *gid = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    int32_t ng = 0, nm = 0;
+    for (int32_t t = 0; t < N; ++t) {
+        gid[t] = -1;
+        const int32_t root = uf_find(parent, t);
+        if (size[root] < 2) continue;
+        if (root == t) gid[t] = ng++;
+        nm++;
+    }
+    tp->tet_grp = (int32_t *)malloc(sizeof(int32_t) * (size_t)N);
+    tp->grp_ptr = (int32_t *)calloc((size_t)ng + 1, sizeof(int32_t));
+    tp->grp_mem = (int32_t *)malloc(sizeof(int32_t) * (size_t)(nm > 0 ? nm : 1));
+    for (int32_t t = 0; t < N; ++t) {
+        const int32_t root = uf_find(parent, t);
+        tp->tet_grp[t] = size[root] < 2 ? -1 : gid[root];
+        if (tp->tet_grp[t] >= 0) tp->grp_ptr[tp->tet_grp[t] + 1]++;
+    }
+    for (int32_t g = 0; g < ng; ++g) tp->grp_ptr[g + 1] += tp->grp_ptr[g];
+    int32_t *fill = (int32_t *)calloc((size_t)ng + 1, sizeof(int32_t));
+    for (int32_t t = 0; t < N; ++t) {
+        const int32_t g = tp->tet_grp[t];
+        if (g >= 0) tp->grp_mem[tp->grp_ptr[g] + fill[g]++] = t;
+    }
+    free(fill);
+    free(gid);
+    free(size);
+    free(parent);
 }
 
 static void topo_free(topo *tp) {
+    free(tp->tet_grp);
+    free(tp->grp_ptr);
+    free(tp->grp_mem);
     free(tp->tri_ptr);
     free(tp->tri_idx);
     free(tp->tracked);
@@ -590,6 +672,9 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                     if (li[q] > cnt[q]) zero = 1;
                     else h = falling(h, cnt[q], li[q]);
                 }
+                /* outer reactants, on a triangle outside every SSA group: none */
+                for (int32_t q = 0; q < S && !zero && s->sreac_lhs_o; ++q)
+                    if (s->sreac_lhs_o[r * S + q] > 0) zero = 1;
                 if (zero) {
                     a[j] = 0.0;
                 } else {
@@ -694,6 +779,250 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 if (ui[sp] == 0 || tet_clamped(s, t, sp)) continue;
                 cnt[sp] += ui[sp];
                 if (tp->tracked[sp]) corr[sp] -= (double)ui[sp] * tcur;
+            }
+        }
+        if (++nev >= MAX_EVENTS_PER_PERIOD) return -1;
+    }
+    return nev;
+}
+
+/* ------------------------------------------------------------ SSA groups */
+
+/* the realization constants of tet t: local index, stream, rate constants */
+static uint32_t tet_ltet(const stgo_system *s, int32_t t) {
+    return s->tet_comp ? (uint32_t)s->tet_ltet[t] : (uint32_t)t;
+}
+
+static uint32_t tet_stream(const stgo_system *s, int32_t t) {
+    return s->tet_comp ? s->comp_stream[s->tet_comp[t]] : s->stream;
+}
+
+static double tet_reac_k(const stgo_system *s, int32_t t, int32_t r) {
+    if (s->tet_k) return s->tet_k[(size_t)r * s->n_tets + t];
+    return s->tet_comp ? s->comp_reac_k[(size_t)s->tet_comp[t] * s->n_reacs + r] : s->reac_k[r];
+}
+
+/* propensity of volume reaction r in tet t, counts read from the state
+ * (reactions.cpp:66-119) */
+static double grp_vol_prop(const stgo_system *s, int32_t t, int32_t r) {
+    const int32_t S = s->n_specs, N = s->n_tets;
+    const int32_t *lhs = s->reac_lhs + r * S;
+    int32_t order = 0;
+    for (int32_t sp = 0; sp < S; ++sp) order += lhs[sp];
+    double h = 1.0;
+    for (int32_t sp = 0; sp < S; ++sp) {
+        if (lhs[sp] == 0) continue;
+        const int32_t n = s->counts[(size_t)sp * N + t];
+        if (lhs[sp] > n) return 0.0;
+        h = falling(h, n, lhs[sp]);
+    }
+    return h * scaled_const(tet_reac_k(s, t, r), order, 1.0e3 * s->tet_vol[t] * AVOGADRO);
+}
+
+/* GHK constants with E = e^-nuFoRT (1 in the limit) in place of c_o e^-nuFoRT
+ * (math/ghk.cpp:62-81), so that c_o can be read from the outer tet */
+static int ghk_consts_e(const stgo_system *s, int32_t g, double v, double *A, double *E, double *Dn) {
+    const double z = (double)s->ghk_valence[g];
+    const double nuFoRT = z * (v + s->ghk_vshift[g]) * FARADAY / (GAS_CONSTANT * s->temp);
+    if (nuFoRT >= 308 * 2.30 || nuFoRT <= -307 * 2.30) return 0;
+    const double e = stgo_exp(-nuFoRT);
+    if (fabs(nuFoRT) > DBL_EPSILON) {
+        *A = s->ghk_perm[g] * z * FARADAY * nuFoRT;
+        *E = e;
+        *Dn = 1.0 - e;
+    } else {
+        *A = s->ghk_perm[g] * z * FARADAY;
+        *E = 1.0;
+        *Dn = 1.0;
+    }
+    return 1;
+}
+
+/* signed ions per second through one open channel of GHK current g on
+ * triangle tr (inner tet t), the outer concentration from the outer tet when
+ * ghk_voconc < 0 (ghkcurr.cpp:168-208); *bad on an out-of-range exponent */
+static double grp_ghk_rt(const stgo_system *s, int32_t tr, int32_t t, int32_t g, int *bad) {
+    const int32_t N = s->n_tets;
+    double A, E, Dn;
+    if (!ghk_consts_e(s, g, s->tri_v[tr], &A, &E, &Dn)) {
+        *bad = 1;
+        return 0.0;
+    }
+    if (s->tri_sk) A = A * s->tri_sk[(size_t)(s->n_sreacs + g) * s->n_tris + tr];
+    const int32_t ion = s->ghk_ion[g];
+    double oc = s->ghk_voconc[g] * 1.0e3;
+    if (s->ghk_voconc[g] < 0.0) {
+        const int32_t o = s->tri_otet ? s->tri_otet[tr] : -1;
+        oc = o >= 0 ? (double)s->counts[(size_t)ion * N + o] / (1.0e3 * s->tet_vol[o] * AVOGADRO) * 1.0e3 : 0.0;
+    }
+    return ghk_rt(s, g, A, oc * E, Dn, s->counts[(size_t)ion * N + t], 1.0e3 * s->tet_vol[t] * AVOGADRO);
+}
+
+/* propensity of entry q of triangle tr (surface reaction q < n_sreacs, else
+ * GHK current q - n_sreacs) with inner tet t: surface, inner and outer
+ * reactants (SReac::rate, sreac.cpp:305-420); constant scaled by the area,
+ * the inner or the outer tet's volume (sreac.cpp:150-180) */
+static double grp_tri_prop(const stgo_system *s, int32_t tr, int32_t t, int32_t q, int *bad) {
+    const int32_t S = s->n_specs, SS = s->n_sspecs, N = s->n_tets, RS = s->n_sreacs;
+    if (q >= RS) {
+        const int32_t g = q - RS;
+        const int32_t nch = s->tri_counts[(size_t)s->ghk_chan[g] * s->n_tris + tr];
+        return fabs(grp_ghk_rt(s, tr, t, g, bad)) * (double)nch;
+    }
+    const int32_t o = s->tri_otet ? s->tri_otet[tr] : -1;
+    const int32_t *ls = s->sreac_lhs_s + q * SS;
+    const int32_t *li = s->sreac_lhs_i + q * S;
+    const int32_t *lo = s->sreac_lhs_o ? s->sreac_lhs_o + q * S : NULL;
+    int32_t order = 0, nin = 0, nout = 0;
+    double h = 1.0;
+    for (int32_t x = 0; x < SS; ++x) {
+        order += ls[x];
+        if (ls[x] == 0) continue;
+        const int32_t n = s->tri_counts[(size_t)x * s->n_tris + tr];
+        if (ls[x] > n) return 0.0;
+        h = falling(h, n, ls[x]);
+    }
+    for (int32_t x = 0; x < S; ++x) {
+        order += li[x];
+        nin += li[x];
+        if (li[x] == 0) continue;
+        const int32_t n = s->counts[(size_t)x * N + t];
+        if (li[x] > n) return 0.0;
+        h = falling(h, n, li[x]);
+    }
+    for (int32_t x = 0; x < S && lo; ++x) {
+        order += lo[x];
+        nout += lo[x];
+        if (lo[x] == 0) continue;
+        const int32_t n = o >= 0 ? s->counts[(size_t)x * N + o] : 0;
+        if (lo[x] > n) return 0.0;
+        h = falling(h, n, lo[x]);
+    }
+    const double f = s->tri_sk ? s->tri_sk[(size_t)q * s->n_tris + tr] : 1.0;
+    double k = s->tri_sk ? f : s->sreac_k[q];
+    if (s->sreac_vdep && s->sreac_vdep[q] >= 0) {
+        if (s->tri_sk && f == 0.0) {
+            k = 0.0;
+        } else {
+            k = vdep_const(s, s->sreac_vdep[q], s->tri_v[tr]);
+            if (k < 0.0) {
+                *bad = 2;
+                return 0.0;
+            }
+            if (s->tri_sk) k = k * f;
+        }
+    }
+    double scale;
+    if (nin + nout == 0) scale = s->tri_area[tr] * AVOGADRO;
+    else if (nout > 0) scale = 1.0e3 * s->tet_vol[o] * AVOGADRO;
+    else scale = 1.0e3 * s->tet_vol[t] * AVOGADRO;
+    return h * scaled_const(k, order, scale);
+}
+
+/* change the count of species sp in tet t by dv unless clamped there, with
+ * the occupancy correction of tracked species (mol_state.hpp:87-200) */
+static void grp_add(const stgo_system *s, const topo *tp, int32_t t, int32_t sp, int32_t dv, double tcur,
+                    double *gcorr) {
+    if (tet_clamped(s, t, sp)) return;
+    s->counts[(size_t)sp * s->n_tets + t] += dv;
+    if (tp->tracked[sp]) gcorr[(size_t)sp * s->n_tets + t] -= (double)dv * tcur;
+}
+
+/* Direct SSA of group g over [0, dt): the members' volume reactions in member
+ * order, then each member's triangles' entries; counts in the state and
+ * occupancy corrections in gcorr [n_specs][n_tets]. Returns events, or a
+ * negative code as ssa_tet. */
+static long ssa_grp(stgo_system *s, const topo *tp, int32_t g, double dt, uint32_t step, double *gcorr) {
+    const int32_t N = s->n_tets, S = s->n_specs, SS = s->n_sspecs, R = s->n_reacs, RS = s->n_sreacs;
+    const int32_t RT = RS + s->n_ghk;
+    const int32_t m0 = tp->grp_ptr[g], m1 = tp->grp_ptr[g + 1];
+    const int32_t owner = tp->grp_mem[m0];
+    for (int32_t k = m0; k < m1; ++k) {
+        const int32_t t = tp->grp_mem[k];
+        for (int32_t sp = 0; sp < S; ++sp) gcorr[(size_t)sp * N + t] = 0.0;
+        for (int32_t q = 0; q < SS; ++q)
+            if (tp->tri_tracked[q])
+                for (int32_t x = tp->tri_ptr[t]; x < tp->tri_ptr[t + 1]; ++x)
+                    tp->tri_corr[(size_t)q * s->n_tris + tp->tri_idx[x]] = 0.0;
+    }
+    ostream st;
+    os_init(&st, s->seed, tet_ltet(s, owner), step, tet_stream(s, owner), 0u);
+    long nev = 0;
+    double tcur = 0.0;
+    int bad = 0;
+    for (;;) {
+        double a0 = 0.0;
+        for (int32_t k = m0; k < m1; ++k)
+            for (int32_t r = 0; r < R; ++r) a0 = a0 + grp_vol_prop(s, tp->grp_mem[k], r);
+        for (int32_t k = m0; k < m1; ++k) {
+            const int32_t t = tp->grp_mem[k];
+            for (int32_t x = tp->tri_ptr[t]; x < tp->tri_ptr[t + 1]; ++x)
+                for (int32_t q = 0; q < RT; ++q) a0 = a0 + grp_tri_prop(s, tp->tri_idx[x], t, q, &bad);
+        }
+        if (bad) return bad == 2 ? -2 : -3;
+        if (a0 < DBL_EPSILON) break;
+        const double u1 = os_unif(&st);
+        if (1.0 - u1 > a0 * (dt - tcur) * 1.0000001) break;
+        const double u2 = os_unif(&st);
+        const double tev = tcur + (-stgo_log(u1) / a0);
+        if (tev > dt) break;
+        tcur = tev;
+        const double x = u2 * a0;
+        double acc = 0.0;
+        int32_t st_t = -1, st_tr = -1, st_q = -1, la_t = -1, la_tr = -1, la_q = -1;
+        for (int32_t k = m0; k < m1; ++k) {
+            const int32_t t = tp->grp_mem[k];
+            for (int32_t r = 0; r < R; ++r) {
+                const double p = grp_vol_prop(s, t, r);
+                acc = acc + p;
+                if (st_t < 0 && acc > x) { st_t = t; st_tr = -1; st_q = r; }
+                if (p > 0.0) { la_t = t; la_tr = -1; la_q = r; }
+            }
+        }
+        for (int32_t k = m0; k < m1; ++k) {
+            const int32_t t = tp->grp_mem[k];
+            for (int32_t y = tp->tri_ptr[t]; y < tp->tri_ptr[t + 1]; ++y)
+                for (int32_t q = 0; q < RT; ++q) {
+                    const double p = grp_tri_prop(s, tp->tri_idx[y], t, q, &bad);
+                    acc = acc + p;
+                    if (st_t < 0 && acc > x) { st_t = t; st_tr = tp->tri_idx[y]; st_q = q; }
+                    if (p > 0.0) { la_t = t; la_tr = tp->tri_idx[y]; la_q = q; }
+                }
+        }
+        if (st_t < 0) { st_t = la_t; st_tr = la_tr; st_q = la_q; }
+        if (st_t < 0) break;
+        const int32_t t = st_t;
+        if (st_tr < 0) {
+            const int32_t *upd = s->reac_upd + st_q * S;
+            for (int32_t sp = 0; sp < S; ++sp)
+                if (upd[sp] != 0) grp_add(s, tp, t, sp, upd[sp], tcur, gcorr);
+        } else {
+            const int32_t tr = st_tr, o = s->tri_otet[tr];
+            if (st_q >= RS) {
+                /* GHK: one ion between the inner and the outer tet
+                 * (ghkcurr.cpp:228-272), charge counted outward positive */
+                const int32_t gq = st_q - RS, ion = s->ghk_ion[gq];
+                const int efflux = grp_ghk_rt(s, tr, t, gq, &bad) >= 0.0;
+                if (s->ghk_realflux[gq]) {
+                    grp_add(s, tp, t, ion, efflux ? -1 : 1, tcur, gcorr);
+                    if (s->ghk_voconc[gq] < 0.0 && o >= 0) grp_add(s, tp, o, ion, efflux ? 1 : -1, tcur, gcorr);
+                }
+                s->tri_charge[tr] += efflux ? s->ghk_valence[gq] : -s->ghk_valence[gq];
+            } else {
+                /* SReac::apply, sreac.cpp:440-505 */
+                const int32_t *us_ = s->sreac_upd_s + st_q * SS;
+                const int32_t *ui = s->sreac_upd_i + st_q * S;
+                const int32_t *uo = s->sreac_upd_o ? s->sreac_upd_o + st_q * S : NULL;
+                for (int32_t q = 0; q < SS; ++q) {
+                    if (us_[q] == 0 || tri_clamped(s, tr, q)) continue;
+                    s->tri_counts[(size_t)q * s->n_tris + tr] += us_[q];
+                    if ((s->tri_intg_mask >> q) & 1u) s->tri_intg[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
+                    if (tp->tri_tracked[q]) tp->tri_corr[(size_t)q * s->n_tris + tr] -= (double)us_[q] * tcur;
+                }
+                for (int32_t sp = 0; sp < S; ++sp)
+                    if (ui[sp] != 0) grp_add(s, tp, t, sp, ui[sp], tcur, gcorr);
+                for (int32_t sp = 0; sp < S && uo && o >= 0; ++sp)
+                    if (uo[sp] != 0) grp_add(s, tp, o, sp, uo[sp], tcur, gcorr);
             }
         }
         if (++nev >= MAX_EVENTS_PER_PERIOD) return -1;
@@ -838,6 +1167,44 @@ static int32_t leave_tri_species(const stgo_system *s, const topo *tp, int32_t t
     return nl;
 }
 
+/* after the SSA of tet t (counts cnt, occupancy corrections corr): the
+ * channel-count integrals of its triangles, then the leaving pass of its
+ * species and of its triangles' surface species; writes the counts back and
+ * returns the molecules moved */
+static uint64_t finish_tet(stgo_system *s, topo *tp, int32_t t, double dt, uint32_t step, int32_t *cnt,
+                           const double *corr, int32_t *scratch, int32_t *tout) {
+    const int32_t N = s->n_tets, S = s->n_specs, SS = s->n_sspecs, NTR = s->n_tris;
+    uint64_t mv = 0;
+    if (s->tri_intg_mask != 0) {
+        for (int32_t q = 0; q < SS; ++q) {
+            if (!((s->tri_intg_mask >> q) & 1u)) continue;
+            for (int32_t k = tp->tri_ptr[t]; k < tp->tri_ptr[t + 1]; ++k) {
+                const size_t ix = (size_t)q * NTR + tp->tri_idx[k];
+                s->tri_intg[ix] += (double)s->tri_counts[ix] * dt;
+            }
+        }
+    }
+    for (int32_t sp = 0; sp < S; ++sp) {
+        int32_t *o4 = scratch + ((size_t)sp * N + t) * 4;
+        if (spec_diffuses(s, sp))
+            mv += (uint64_t)leave_tet_species(s, t, sp, dt, step, cnt, corr[sp], tp->tracked[sp], o4);
+        else
+            o4[0] = o4[1] = o4[2] = o4[3] = 0;
+        s->counts[(size_t)sp * N + t] = cnt[sp];
+    }
+    for (int32_t k = tp->tri_ptr[t]; k < tp->tri_ptr[t + 1]; ++k) {
+        const int32_t tr = tp->tri_idx[k];
+        for (int32_t q = 0; q < SS; ++q) {
+            int32_t *o3 = tout + ((size_t)q * NTR + tr) * 3;
+            if (sspec_diffuses(s, q))
+                mv += (uint64_t)leave_tri_species(s, tp, tr, q, dt, step, o3);
+            else
+                o3[0] = o3[1] = o3[2] = 0;
+        }
+    }
+    return mv;
+}
+
 int32_t stgo_period(stgo_system *s, double dt, uint32_t step, int32_t *scratch,
                     uint64_t *n_events, uint64_t *n_moved) {
     return stgo_run(s, &dt, 1, step, scratch, n_events, n_moved);
@@ -854,6 +1221,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
     const int32_t NTR = s->n_tris, SS = s->n_sspecs;
     int32_t *tout = (int32_t *)calloc((size_t)(SS > 0 ? SS : 1) * (size_t)(NTR > 0 ? NTR : 1) * 3,
                                       sizeof(int32_t));
+    double *gcorr = tp.tet_grp ? (double *)calloc((size_t)(S > 0 ? S : 1) * (size_t)N, sizeof(double)) : NULL;
     for (int32_t p = 0; p < n_periods && !err; ++p) {
         const double dt = dts[p];
         const uint32_t step = step0 + (uint32_t)p;
@@ -866,6 +1234,27 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
             double *ps = (double *)malloc(sizeof(double) * (size_t)nk_max);
 #pragma omp for schedule(dynamic, 256)
             for (int32_t t = 0; t < N; ++t) {
+                const int32_t grp = tp.tet_grp ? tp.tet_grp[t] : -1;
+                if (grp >= 0) {
+                    /* an SSA group: its first tet runs the SSA and every
+                     * member's leaving pass; the other members do nothing */
+                    if (tp.grp_mem[tp.grp_ptr[grp]] != t) continue;
+                    long nev = ssa_grp(s, &tp, grp, dt, step, gcorr);
+                    if (nev < 0) {
+                        err |= nev == -2 ? 2 : nev == -3 ? 4 : 1;
+                        nev = 0;
+                    }
+                    ev_total += (uint64_t)nev;
+                    for (int32_t k = tp.grp_ptr[grp]; k < tp.grp_ptr[grp + 1]; ++k) {
+                        const int32_t mt = tp.grp_mem[k];
+                        for (int32_t sp = 0; sp < S; ++sp) {
+                            cnt[sp] = s->counts[(size_t)sp * N + mt];
+                            corr[sp] = gcorr[(size_t)sp * N + mt];
+                        }
+                        mv_total += finish_tet(s, &tp, mt, dt, step, cnt, corr, scratch, tout);
+                    }
+                    continue;
+                }
                 for (int32_t sp = 0; sp < S; ++sp) {
                     cnt[sp] = s->counts[(size_t)sp * N + t];
                     corr[sp] = 0.0;
@@ -879,35 +1268,8 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
                     err |= nev == -2 ? 2 : nev == -3 ? 4 : 1;
                     nev = 0;
                 }
-                if (s->tri_intg_mask != 0) {
-                    for (int32_t q = 0; q < s->n_sspecs; ++q) {
-                        if (!((s->tri_intg_mask >> q) & 1u)) continue;
-                        for (int32_t k = tp.tri_ptr[t]; k < tp.tri_ptr[t + 1]; ++k) {
-                            const size_t ix = (size_t)q * s->n_tris + tp.tri_idx[k];
-                            s->tri_intg[ix] += (double)s->tri_counts[ix] * dt;
-                        }
-                    }
-                }
                 ev_total += (uint64_t)nev;
-                for (int32_t sp = 0; sp < S; ++sp) {
-                    int32_t *o4 = scratch + ((size_t)sp * N + t) * 4;
-                    if (spec_diffuses(s, sp))
-                        mv_total += (uint64_t)leave_tet_species(s, t, sp, dt, step, cnt, corr[sp],
-                                                                tp.tracked[sp], o4);
-                    else
-                        o4[0] = o4[1] = o4[2] = o4[3] = 0;
-                    s->counts[(size_t)sp * N + t] = cnt[sp];
-                }
-                for (int32_t k = tp.tri_ptr[t]; k < tp.tri_ptr[t + 1]; ++k) {
-                    const int32_t tr = tp.tri_idx[k];
-                    for (int32_t q = 0; q < SS; ++q) {
-                        int32_t *o3 = tout + ((size_t)q * NTR + tr) * 3;
-                        if (sspec_diffuses(s, q))
-                            mv_total += (uint64_t)leave_tri_species(s, &tp, tr, q, dt, step, o3);
-                        else
-                            o3[0] = o3[1] = o3[2] = 0;
-                    }
-                }
+                mv_total += finish_tet(s, &tp, t, dt, step, cnt, corr, scratch, tout);
             }
             free(cnt);
             free(corr);
@@ -945,6 +1307,7 @@ int32_t stgo_run(stgo_system *s, const double *dts, int32_t n_periods, uint32_t 
         }
     }
     free(tout);
+    free(gcorr);
     topo_free(&tp);
     if (n_events) *n_events += ev_total;
     if (n_moved) *n_moved += mv_total;

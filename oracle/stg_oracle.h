@@ -114,6 +114,15 @@ typedef struct stgo_system {
     const double  *tri_sk;
     const double  *tri_eD;
     const uint32_t *tri_clamp;
+    /* outer side of the membrane (null when unused): outer-compartment
+     * reactants and net changes of each surface reaction [n_sreacs*n_specs]
+     * (sreac.cpp:305-505), and the outer tet of each triangle [n_tris] (-1
+     * none). A GHK current with ghk_voconc < 0 reads the outer tet's
+     * concentration (ghkcurr.cpp:176-182). Tets joined by a triangle whose
+     * reactions reach both sides form one SSA group (see stgo_run). */
+    const int32_t *sreac_lhs_o;
+    const int32_t *sreac_upd_o;
+    const int32_t *tri_otet;
 } stgo_system;
 
 /* Threads the OpenMP loops of stgo_run use. */

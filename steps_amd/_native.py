@@ -44,6 +44,9 @@ class StgMesh(C.Structure):
         ("tri_nbr", _ip),
         ("tri_nbr_edge", _ip),
         ("tri_edge_g", _dp),
+        ("tri_otet", _ip),
+        ("tri_ocomp", _ip),
+        ("tri_nbr_comp", _ip),
     ]
 
 
@@ -77,6 +80,8 @@ class StgModel(C.Structure):
         ("ghk_vshift", _dp),
         ("ghk_realflux", _ip),
         ("sspec_dcst", _dp),
+        ("sreac_lhs_o", _ip),
+        ("sreac_upd_o", _ip),
     ]
 
 

@@ -103,6 +103,13 @@ struct stg_sim {
     std::vector<std::vector<int32_t>> tri_perm;  // per realization: user tri -> internal tri
     int32_t NT = 0, NTRI = 0, max_tris = 0;
     int32_t NG = 0;  // GHK currents
+    bool outer_model = false;  // surface reactions or GHK currents reach the outer side
+    // SSA groups (tets joined across the membrane): host tables, device arrays
+    std::vector<int32_t> h_tet_grp, h_grp_ptr, h_grp_mem, h_grp_real;
+    int32_t *d_tet_grp = nullptr, *d_grp_ptr = nullptr, *d_grp_mem = nullptr, *d_grp_real = nullptr;
+    int32_t *d_tri_otet = nullptr, *d_tri_oreal = nullptr;
+    double *d_gcorr = nullptr;
+    bool has_groups() const { return d_tet_grp != nullptr; }
     bool linked = false;                   // compartments: one period schedule for all
     int32_t NBD = 0;                       // diffusion-boundary directions
     std::vector<double> h_bnd_D;           // [NBD][S]
@@ -197,7 +204,8 @@ struct stg_sim {
     // surface reaction, a GHK current or an integrated channel count, or
     // faces on a diffusion boundary
     bool needs_memb() const {
-        return (NTRI > 0 && (RS > 0 || NG > 0 || NSD > 0 || mk.tri_intg_mask != 0)) || NBD > 0 || has_tabs();
+        return (NTRI > 0 && (RS > 0 || NG > 0 || NSD > 0 || mk.tri_intg_mask != 0)) || NBD > 0 || has_tabs() ||
+               has_groups();
     }
     // some realization uses a per-element table (read by the membrane
     // variant of the generic kernel only)
@@ -287,7 +295,9 @@ struct stg_sim {
                         d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D,
                         d_tri_nbr,   d_tri_nface,   d_tri_g,       d_tri_user,    d_real_sD,
                         d_tout[0],   d_tout[1],     d_tri_corr,    d_real_tabs,   d_tet_k,
-                        d_tet_fD,    d_tri_sk,      d_tri_eD,      d_tet_clamp,   d_tri_clamp};
+                        d_tet_fD,    d_tri_sk,      d_tri_eD,      d_tet_clamp,   d_tri_clamp,
+                        d_tet_grp,   d_grp_ptr,     d_grp_mem,     d_grp_real,    d_tri_otet,
+                        d_tri_oreal, d_gcorr};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -317,6 +327,7 @@ void build_model(stg_sim *s, const stg_model *m) {
     s->R = m->n_reacs;
     s->SS = m->n_sspecs;
     s->RS = m->n_sreacs;
+    s->outer_model = false;
     stg::ModelK &k = s->mk;
     std::memset(&k, 0, sizeof(k));
     k.n_specs = s->S;
@@ -355,7 +366,7 @@ void build_model(stg_sim *s, const stg_model *m) {
     k.upd_off[s->R] = ne;
     for (int r = 0; r < s->RS; ++r) {
         k.slhs_off[r] = ne;
-        int order = 0, nvol = 0;
+        int order = 0, nvol = 0, nin = 0, nout = 0;
         for (int q = 0; q < s->SS; ++q) {
             const int v = m->sreac_lhs_s[r * s->SS + q];
             if (v < 0 || v > 4) throw StgError{STG_ERR_ARG, "reactant multiplicity must be in [0, 4]"};
@@ -368,9 +379,22 @@ void build_model(stg_sim *s, const stg_model *m) {
             if (v) push(q, v);
             order += v;
             nvol += v;
+            nin += v;
         }
+        for (int q = 0; q < s->S && m->sreac_lhs_o; ++q) {
+            const int v = m->sreac_lhs_o[r * s->S + q];
+            if (v < 0 || v > 4) throw StgError{STG_ERR_ARG, "reactant multiplicity must be in [0, 4]"};
+            if (v) push(stg::kOuterFlag | q, v);
+            order += v;
+            nvol += v;
+            nout += v;
+        }
+        // model/sreac.cpp:55: volume reactants on one side only
+        if (nin > 0 && nout > 0)
+            throw StgError{STG_ERR_ARG, "a surface reaction can have inner or outer volume reactants, not both"};
         k.sorder[r] = order;
         k.s_nvol[r] = nvol;
+        k.s_out[r] = nout > 0 ? 1 : 0;
     }
     k.slhs_off[s->RS] = ne;
     for (int r = 0; r < s->RS; ++r) {
@@ -387,6 +411,14 @@ void build_model(stg_sim *s, const stg_model *m) {
             if (v < 0 && -v > m->sreac_lhs_i[r * s->S + q])
                 throw StgError{STG_ERR_ARG, "a surface reaction consumes a species it does not react with"};
         }
+        for (int q = 0; q < s->S && m->sreac_upd_o; ++q) {
+            const int v = m->sreac_upd_o[r * s->S + q];
+            if (v) push(stg::kOuterFlag | q, v);
+            if (v < 0 && -v > (m->sreac_lhs_o ? m->sreac_lhs_o[r * s->S + q] : 0))
+                throw StgError{STG_ERR_ARG, "a surface reaction consumes a species it does not react with"};
+            if (v) s->outer_model = true;
+        }
+        if (k.s_out[r]) s->outer_model = true;
     }
     k.supd_off[s->RS] = ne;
     // voltage-dependent rate tables
@@ -424,14 +456,16 @@ void build_model(stg_sim *s, const stg_model *m) {
         if (m->ghk_valence[g] == 0 || m->ghk_valence[g] < -127 || m->ghk_valence[g] > 127)
             throw StgError{STG_ERR_ARG, "GHK ion valence must be nonzero"};
         if (!(m->ghk_perm[g] > 0.0)) throw StgError{STG_ERR_ARG, "GHK permeability must be > 0"};
-        if (!(m->ghk_voconc[g] >= 0.0))
-            throw StgError{STG_ERR_NOTIMPL, "GHK currents need a virtual outer concentration >= 0"};
+        if (std::isnan(m->ghk_voconc[g])) throw StgError{STG_ERR_ARG, "GHK outer concentration is NaN"};
+        k.g_outer[g] = m->ghk_voconc[g] < 0.0 ? 1 : 0;  // the outer tet's concentration
+        if (k.g_outer[g]) s->outer_model = true;
         k.g_chan[g] = m->ghk_chan[g];
         k.g_ion[g] = m->ghk_ion[g];
         k.g_z[g] = m->ghk_valence[g];
         k.g_real[g] = m->ghk_realflux[g] ? 1 : 0;
         k.g_perm[g] = m->ghk_perm[g];
-        k.g_oconc[g] = m->ghk_voconc[g];
+        // no outer tet (a triangle outside every SSA group): concentration 0
+        k.g_oconc[g] = k.g_outer[g] ? 0.0 : m->ghk_voconc[g];
         k.g_vshift[g] = m->ghk_vshift[g];
     }
     k.temp = 293.15;
@@ -451,6 +485,7 @@ void build_diffusion_sets(stg_sim *s, const stg_model *m) {
         bool reacts = false;
         for (int r = 0; r < s->R; ++r) reacts = reacts || m->reac_upd[r * s->S + sp] != 0;
         for (int r = 0; r < s->RS; ++r) reacts = reacts || m->sreac_upd_i[r * s->S + sp] != 0;
+        for (int r = 0; r < s->RS && m->sreac_upd_o; ++r) reacts = reacts || m->sreac_upd_o[r * s->S + sp] != 0;
         for (int g = 0; g < m->n_ghk; ++g) reacts = reacts || (m->ghk_ion[g] == sp && m->ghk_realflux[g]);
         if (reacts) k.tracked_mask |= 1 << sp;
     }
@@ -605,7 +640,8 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     }
     if (s->max_tris > 4) throw StgError{STG_ERR_ARG, "a tet has more than four triangles"};
 
-    // surface diffusion: edge neighbours in internal triangle order
+    // surface diffusion: edge neighbours in internal triangle order (in
+    // another compartment's patch when tri_nbr_comp says so)
     std::vector<int4> tri_nbr(std::max(NTRI, 1), make_int4(-1, -1, -1, -1));
     std::vector<uint32_t> tri_nface(std::max(NTRI, 1), 0);
     std::vector<int32_t> tri_user(std::max(NTRI, 1), 0);
@@ -613,6 +649,7 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     for (int r = 0; r < n_real; ++r) {
         const stg_mesh &m = meshes[r];
         const int32_t i0 = s->tri_off[r];
+        if (!linked && m.tri_nbr_comp) throw StgError{STG_ERR_ARG, "compartment links need stg_create_comps"};
         for (int u = 0; u < m.n_tris; ++u) {
             const int32_t i = i0 + s->tri_perm[r][u];
             tri_user[i] = u;
@@ -622,16 +659,87 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
                 const int32_t j = m.tri_nbr[3 * u + e];
                 const int32_t je = m.tri_nbr_edge ? m.tri_nbr_edge[3 * u + e] : -1;
                 const double g = m.tri_edge_g ? m.tri_edge_g[3 * u + e] : 0.0;
-                if (j < -1 || j >= m.n_tris) throw StgError{STG_ERR_ARG, "triangle neighbour out of range"};
-                if (j >= 0 && (je < 0 || je > 2 || m.tri_nbr[3 * j + je] != u))
+                const int32_t jr = m.tri_nbr_comp ? m.tri_nbr_comp[3 * u + e] : r;
+                if (jr < 0 || jr >= n_real) throw StgError{STG_ERR_ARG, "triangle neighbour compartment out of range"};
+                const stg_mesh &mj = meshes[jr];
+                if (j < -1 || j >= mj.n_tris) throw StgError{STG_ERR_ARG, "triangle neighbour out of range"};
+                if (j >= 0 && (je < 0 || je > 2 || !mj.tri_nbr || mj.tri_nbr[3 * j + je] != u ||
+                               (mj.tri_nbr_comp ? mj.tri_nbr_comp[3 * j + je] : jr) != r))
                     throw StgError{STG_ERR_ARG, "triangle neighbour table is not symmetric"};
                 if (!(g >= 0.0) || !std::isfinite(g) || (j < 0 && g != 0.0))
                     throw StgError{STG_ERR_ARG, "edge factor must be finite, >= 0 and 0 on open edges"};
-                nb[e] = j >= 0 ? i0 + s->tri_perm[r][j] : -1;
+                nb[e] = j >= 0 ? s->tri_off[jr] + s->tri_perm[jr][j] : -1;
                 tri_nface[i] |= (uint32_t)(j >= 0 ? je : 0) << (8 * e);
                 s->h_tri_g[3 * (size_t)i + e] = g;
             }
             tri_nbr[i] = make_int4(nb[0], nb[1], nb[2], -1);
+        }
+    }
+
+    // outer tets of the triangles (internal order, global tets) and the SSA
+    // groups they make: union of inner and outer tet over every triangle
+    // with an outer tet, when the model reaches the outer side
+    std::vector<int32_t> tri_otet(std::max(NTRI, 1), -1), tri_oreal(std::max(NTRI, 1), -1);
+    bool any_outer = false;
+    for (int r = 0; r < n_real; ++r) {
+        const stg_mesh &m = meshes[r];
+        if (!m.tri_otet) continue;
+        if (!linked && m.tri_ocomp) throw StgError{STG_ERR_ARG, "compartment links need stg_create_comps"};
+        for (int u = 0; u < m.n_tris; ++u) {
+            const int32_t o = m.tri_otet[u];
+            if (o < 0) continue;
+            const int32_t orl = m.tri_ocomp ? m.tri_ocomp[u] : r;
+            if (orl < 0 || orl >= n_real || o >= meshes[orl].n_tets)
+                throw StgError{STG_ERR_ARG, "triangle outer tet out of range"};
+            if (orl == r && o == m.tri_tet[u]) throw StgError{STG_ERR_ARG, "a triangle's outer tet is its inner tet"};
+            const int32_t i = s->tri_off[r] + s->tri_perm[r][u];
+            tri_otet[i] = s->tet_off[orl] + o;
+            tri_oreal[i] = orl;
+            any_outer = true;
+        }
+    }
+    if (s->outer_model && any_outer) {
+        std::vector<int32_t> parent(NT);
+        for (int t = 0; t < NT; ++t) parent[t] = t;
+        auto find = [&parent](int32_t x) {
+            while (parent[x] != x) x = parent[x] = parent[parent[x]];
+            return x;
+        };
+        for (int t = 0; t < NT; ++t)
+            for (int i = tri_ptr[t]; i < tri_ptr[t + 1]; ++i)
+                if (tri_otet[i] >= 0) {
+                    const int32_t a = find(t), b = find(tri_otet[i]);
+                    if (a != b) parent[std::max(a, b)] = std::min(a, b);  // the root is the smallest tet
+                }
+        std::vector<int32_t> size(NT, 0);
+        for (int t = 0; t < NT; ++t) size[find(t)] += 1;
+        s->h_tet_grp.assign(NT, -1);
+        s->h_grp_ptr.assign(1, 0);
+        s->h_grp_mem.clear();
+        s->h_grp_real.clear();
+        std::vector<int32_t> gid(NT, -1);
+        for (int t = 0; t < NT; ++t) {  // members in tet order, so the root (smallest) comes first
+            const int32_t root = find(t);
+            if (size[root] < 2) continue;
+            if (gid[root] < 0) {
+                gid[root] = (int32_t)s->h_grp_ptr.size() - 1;
+                s->h_grp_ptr.push_back(0);
+            }
+        }
+        std::vector<std::vector<int32_t>> mem(s->h_grp_ptr.size() - 1);
+        for (int t = 0; t < NT; ++t) {
+            const int32_t root = find(t);
+            if (size[root] < 2) continue;
+            mem[gid[root]].push_back(t);
+            s->h_tet_grp[t] = gid[root];
+        }
+        for (size_t g = 0; g < mem.size(); ++g) {
+            for (int32_t t : mem[g]) {
+                s->h_grp_mem.push_back(t);
+                const int32_t r = (int32_t)(std::upper_bound(s->tet_off.begin(), s->tet_off.end(), t) - s->tet_off.begin()) - 1;
+                s->h_grp_real.push_back(r);
+            }
+            s->h_grp_ptr[g + 1] = (int32_t)s->h_grp_mem.size();
         }
     }
 
@@ -762,6 +870,23 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
         HIP_OK(hipMemcpy(s->d_vdep_off, s->vdep_off.data(), sizeof(int32_t) * s->vdep_off.size(), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->d_vdep_k, s->vdep_k.data(), sizeof(double) * s->vdep_k.size(), hipMemcpyHostToDevice));
     }
+    if (!s->h_grp_mem.empty()) {
+        s->d_tet_grp = dalloc<int32_t>(NT);
+        s->d_grp_ptr = dalloc<int32_t>(s->h_grp_ptr.size());
+        s->d_grp_mem = dalloc<int32_t>(s->h_grp_mem.size());
+        s->d_grp_real = dalloc<int32_t>(s->h_grp_real.size());
+        s->d_gcorr = dalloc<double>((size_t)std::max(S, 1) * NT);
+        s->d_tri_otet = dalloc<int32_t>(std::max(NTRI, 1));
+        s->d_tri_oreal = dalloc<int32_t>(std::max(NTRI, 1));
+        HIP_OK(hipMemcpy(s->d_tet_grp, s->h_tet_grp.data(), sizeof(int32_t) * NT, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_grp_ptr, s->h_grp_ptr.data(), sizeof(int32_t) * s->h_grp_ptr.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_grp_mem, s->h_grp_mem.data(), sizeof(int32_t) * s->h_grp_mem.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_grp_real, s->h_grp_real.data(), sizeof(int32_t) * s->h_grp_real.size(),
+                         hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_tri_otet, tri_otet.data(), sizeof(int32_t) * tri_otet.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(s->d_tri_oreal, tri_oreal.data(), sizeof(int32_t) * tri_oreal.size(), hipMemcpyHostToDevice));
+        HIP_OK(hipMemset(s->d_gcorr, 0, sizeof(double) * (size_t)std::max(S, 1) * NT));
+    }
     upload_real_tables(s);
 }
 
@@ -864,6 +989,15 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.tout_prev = s->d_tout[launch_parity ^ 1];
     k.tout_cur = s->d_tout[launch_parity];
     k.tri_corr = s->d_tri_corr;
+    if (s->has_groups()) {
+        k.tet_grp = s->d_tet_grp;
+        k.grp_ptr = s->d_grp_ptr;
+        k.grp_mem = s->d_grp_mem;
+        k.grp_real = s->d_grp_real;
+        k.gcorr = s->d_gcorr;
+        k.tri_otet = s->d_tri_otet;
+        k.tri_oreal = s->d_tri_oreal;
+    }
     if (s->has_tabs()) {
         k.real_tabs = s->d_real_tabs;
         k.tet_k = s->d_tet_k;
@@ -1044,8 +1178,9 @@ void unspecialize(stg_sim *s) {
 // plus the block-reduction array (static)
 size_t launch_smem(const stg_sim *s, bool *spec_out) {
     const int B = s->block;
-    // per-element tables are read by the generic membrane kernel only
-    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s)) && !s->has_tabs();
+    // per-element tables and SSA groups are read by the generic membrane kernel only
+    const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s)) && !s->has_tabs() &&
+                      !s->has_groups();
     if (spec_out) *spec_out = spec;
     return spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
 }

@@ -34,7 +34,8 @@ constexpr int kMaxSSpecs = 8;
 constexpr int kMaxSReacs = 16;
 constexpr int kMaxEntries = 384;
 constexpr int kMaxGhk = 4;
-constexpr int kSurfFlag = 0x80;  // entry species >= 0x80: surface species (idx & 0x7f)
+constexpr int kSurfFlag = 0x80;   // entry species with this bit: surface species (idx & 0x7f)
+constexpr int kOuterFlag = 0x100; // entry species with this bit: outer-compartment species (idx & 0xff)
 constexpr long kMaxEventsPerPeriod = 1L << 22;
 constexpr double kAvogadro = 6.02214179e23;  // src/steps/math/constants.hpp:45
 constexpr double kDblEpsilon = 2.220446049250313080847e-16;
@@ -66,6 +67,12 @@ struct ModelK {
     int32_t g_chan[kMaxGhk], g_ion[kMaxGhk], g_z[kMaxGhk], g_real[kMaxGhk];
     double g_perm[kMaxGhk], g_oconc[kMaxGhk], g_vshift[kMaxGhk];
     double temp;  // K
+    // outer side of the membrane: s_out[q] = 1 when surface reaction q's
+    // volume reactants are outer species (its constant scales with the outer
+    // tet's volume, sreac.cpp:150-180); g_outer[g] = 1 when GHK current g
+    // reads the outer tet's concentration (ghkcurr.cpp:176-182)
+    int32_t s_out[kMaxSReacs];
+    int32_t g_outer[kMaxGhk];
 };
 
 // Where the period kernel reads the volume reaction network from. DynTables: the
@@ -170,6 +177,18 @@ struct SysK {
     const double *tri_sk;        // [n_sreacs + n_ghk][NTRI] surface constant / k(V) and GHK factor
     const double *tri_eD;        // [n_sspecs][NTRI][3] surface diffusion constant through each edge
     const uint32_t *tri_clamp;   // [NTRI] bit q: surface species q clamped on the triangle
+    // SSA groups (membrane models coupling both sides): tets joined by
+    // triangles whose reactions or GHK currents reach both sides; null when
+    // there are none. tet_grp[t] = group of t or -1; members of group g are
+    // grp_mem[grp_ptr[g] .. grp_ptr[g+1]) (global tets, first = owner) with
+    // realizations grp_real[]; gcorr holds their occupancy corrections
+    const int32_t *tet_grp;      // [NT]
+    const int32_t *grp_ptr;      // [groups + 1]
+    const int32_t *grp_mem;      // [members]
+    const int32_t *grp_real;     // [members]
+    double *gcorr;               // [n_specs][NT]
+    const int32_t *tri_otet;     // [NTRI] outer tet (global) of each triangle, -1 none
+    const int32_t *tri_oreal;    // [NTRI] its realization
 };
 
 // bits of SysK::real_tabs
@@ -290,6 +309,7 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
         const int sp = m.ent[e] & 0xffff;
         const int mult = m.ent[e] >> 16;
         int32_t n;
+        if (sp & kOuterFlag) return 0.0;  // a triangle outside every SSA group has no outer tet
         if (sp & kSurfFlag) n = sys.tri_counts[(size_t)(sp & 0x7f) * sys.n_tris + tri];
         else n = L.c(sp);
         if (mult > n) return 0.0;
@@ -496,6 +516,7 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
             for (int e = m.supd_off[qs]; e < m.supd_off[qs + 1]; ++e) {
                 const int sp = m.ent[e] & 0xffff;
                 const int dv = m.ent[e] >> 16;
+                if (sp & kOuterFlag) continue;  // no outer tet outside an SSA group
                 if (sp & kSurfFlag) {
                     if ((tclamp >> (sp & 0x7f)) & 1u) continue;
                     const size_t ix = (size_t)(sp & 0x7f) * sys.n_tris + tri;
@@ -743,6 +764,7 @@ __device__ long ssa_memb_lean(const ModelK &m, const SysK &sys, const Lds &L, in
             for (int e = m.supd_off[qs]; e < m.supd_off[qs + 1]; ++e) {
                 const int sp = m.ent[e] & 0xffff;
                 const int dv = m.ent[e] >> 16;
+                if (sp & kOuterFlag) continue;  // no outer tet outside an SSA group
                 if (sp & kSurfFlag) {
                     const size_t ix = (size_t)(sp & 0x7f) * sys.n_tris + tri;
                     sys.tri_counts[ix] += dv;
@@ -765,17 +787,18 @@ __device__ __forceinline__ bool almost_zero_dt(double dt) {
     return ad <= kDblEpsilon * ad * 4 || ad < 2.2250738585072014e-308;
 }
 
-// Leaving pass of one (tet, species): returns molecules moved, writes 4 slots.
-// Faces on a diffusion boundary (bdir byte != 0xff) take the boundary's
-// constant for this species instead of D (0 when it is closed); a per-face
-// table fD (non-null) replaces both. A clamped source keeps its count: the
-// molecules that leave are created (Diff::apply, diff.cpp:476-520, 577).
-__device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, double D,
+// Leaving pass of one (tet, species) holding n molecules with occupancy
+// correction corr: returns the molecules that leave (the caller takes them
+// off the count unless the species is clamped there: a clamped source keeps
+// its count, Diff::apply, diff.cpp:476-520, 577) and writes their split over
+// the 4 faces. Faces on a diffusion boundary (bdir byte != 0xff) take the
+// boundary's constant for this species instead of D (0 when it is closed); a
+// per-face table fD (non-null) replaces both.
+__device__ int32_t leave(const SysK &sys, int32_t n, double corr, int sp, bool tracked, double D,
                          const double r4in[4], uint32_t bdir, int S, double dt, uint64_t seed,
                          uint32_t ltet, uint32_t period, uint32_t stream, int4 &o,
-                         const double *fD = nullptr, bool clamped = false) {
+                         const double *fD = nullptr) {
     o = make_int4(0, 0, 0, 0);
-    const int32_t n = L.c(sp);
     double r4[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -788,7 +811,7 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     Philox g;
     g.init(seed, ltet, period, stream, 1u + (uint32_t)sp);
     double occ = (double)n;
-    if (tracked && !almost_zero_dt(dt)) occ = (double)n + L.o(sp) / dt;
+    if (tracked && !almost_zero_dt(dt)) occ = (double)n + corr / dt;
     const double u = g.unif();
     const double fl = floor(occ);
     int32_t mp = (int32_t)fl + ((occ - fl) > u ? 1 : 0);
@@ -799,7 +822,6 @@ __device__ int32_t leave(const SysK &sys, const Lds &L, int sp, bool tracked, do
     p = p < 0.0 ? 0.0 : p;
     const int32_t nl = binomial(g, mp, p);
     if (nl <= 0) return 0;
-    if (!clamped) L.c(sp) = n - nl;
     int lastf = -1;
 #pragma unroll
     for (int f = 0; f < 4; ++f)
@@ -916,6 +938,340 @@ __device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int
     return nl;
 }
 
+// ---------------------------------------------------------------- SSA groups
+//
+// Membrane triangles whose surface reactions or GHK currents reach the outer
+// compartment couple their inner and outer tets into one SSA group: the
+// reaction system of the group is one Direct SSA over the period (diffusion
+// is frozen within it, as in the reference's operator splitting), run by the
+// thread of the group's first tet. Members' counts stay in HBM and their
+// occupancy corrections in gcorr; the other members' threads do nothing.
+
+// GHK constants as ghk_consts, with the factor E = e^-nuFoRT (1 in the
+// nuFoRT -> 0 limit) in place of c_o e^-nuFoRT, so that the outer
+// concentration c_o can be the outer tet's at each evaluation: Bo = c_o E.
+__device__ __forceinline__ bool ghk_consts_e(const ModelK &m, int g, double v, double &A, double &E, double &Dn) {
+    const double z = (double)m.g_z[g];
+    const double nuFoRT = z * (v + m.g_vshift[g]) * kFaraday / (kGasConstant * m.temp);
+    if (nuFoRT >= 308 * 2.30 || nuFoRT <= -307 * 2.30) return false;
+    const double e = dexp(-nuFoRT);
+    if (fabs(nuFoRT) > kDblEpsilon) {
+        A = m.g_perm[g] * z * kFaraday * nuFoRT;
+        E = e;
+        Dn = 1.0 - e;
+    } else {
+        A = m.g_perm[g] * z * kFaraday;
+        E = 1.0;
+        Dn = 1.0;
+    }
+    return true;
+}
+
+__device__ __forceinline__ int real_tab(const SysK &sys, int r) {
+    return sys.real_tabs != nullptr ? sys.real_tabs[r] : 0;
+}
+
+__device__ __forceinline__ uint32_t tet_clamp_of(const SysK &sys, int t, int rtab) {
+    return (rtab & kTabClamp) ? sys.tet_clamp[t] : 0u;
+}
+
+// propensity of volume reaction q in group member t (realization r)
+__device__ double grp_vol_prop(const ModelK &m, const SysK &sys, int t, int r, int rtab, int q) {
+    const int NT = sys.n_tets;
+    double h = 1.0;
+    for (int e = m.lhs_off[q]; e < m.lhs_off[q + 1]; ++e) {
+        const int sp = m.ent[e] & 0xffff;
+        const int mult = m.ent[e] >> 16;
+        const int32_t n = sys.counts[(size_t)sp * NT + t];
+        if (mult > n) return 0.0;
+        h = falling(h, n, mult);
+    }
+    const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
+    const double k = (rtab & kTabReac) ? sys.tet_k[(size_t)q * NT + t] : sys.real_k[(size_t)r * m.n_reacs + q];
+    return h * scaled_const(k, m.order[q], vscale, 1.0 / vscale);
+}
+
+// GHK current g on triangle tri (inner tet t): signed ions per second
+// through one open channel (0 and error bit 4 when nuFoRT is out of range)
+__device__ double grp_ghk_rt(const ModelK &m, const SysK &sys, int tri, int t, int rtab, int g) {
+    const int NT = sys.n_tets;
+    double A, E, Dn;
+    if (!ghk_consts_e(m, g, sys.tri_v[tri], A, E, Dn)) {
+        atomicOr(sys.err, 4);
+        return 0.0;
+    }
+    if (rtab & kTabTriReac) A = A * sys.tri_sk[(size_t)(m.n_sreacs + g) * sys.n_tris + tri];
+    const int ion = m.g_ion[g];
+    double oc = m.g_oconc[g] * 1.0e3;
+    if (m.g_outer[g]) {
+        const int o = sys.tri_otet[tri];
+        oc = o >= 0 ? (double)sys.counts[(size_t)ion * NT + o] / (1.0e3 * sys.vol[o] * kAvogadro) * 1.0e3 : 0.0;
+    }
+    const double vscale = 1.0e3 * sys.vol[t] * kAvogadro;
+    return ghk_rt(m, g, A, oc * E, Dn, sys.counts[(size_t)ion * NT + t], vscale);
+}
+
+// propensity of triangle entry q (surface reaction q < RS, else GHK current
+// q - RS) on triangle tri of group member t
+__device__ double grp_tri_prop(const ModelK &m, const SysK &sys, int tri, int t, int r, int rtab, int q) {
+    const int NT = sys.n_tets, NTR = sys.n_tris, RS = m.n_sreacs;
+    if (q >= RS) {
+        const int g = q - RS;
+        const int32_t nch = sys.tri_counts[(size_t)m.g_chan[g] * NTR + tri];
+        return fabs(grp_ghk_rt(m, sys, tri, t, rtab, g)) * (double)nch;
+    }
+    const int o = sys.tri_otet[tri];
+    double h = 1.0;
+    for (int e = m.slhs_off[q]; e < m.slhs_off[q + 1]; ++e) {
+        const int sp = m.ent[e] & 0xffff;
+        const int mult = m.ent[e] >> 16;
+        int32_t n;
+        if (sp & kSurfFlag) n = sys.tri_counts[(size_t)(sp & 0x7f) * NTR + tri];
+        else if (sp & kOuterFlag) n = o >= 0 ? sys.counts[(size_t)(sp & 0xff) * NT + o] : 0;
+        else n = sys.counts[(size_t)sp * NT + t];
+        if (mult > n) return 0.0;
+        h = falling(h, n, mult);
+    }
+    double k;
+    if (rtab & kTabTriReac) {
+        const double f = sys.tri_sk[(size_t)q * NTR + tri];
+        k = m.s_vdep[q] < 0 ? f : (f == 0.0 ? 0.0 : vdep_k(sys, m.s_vdep[q], sys.tri_v[tri]) * f);
+    } else {
+        k = m.s_vdep[q] < 0 ? sys.real_sk[(size_t)r * RS + q] : vdep_k(sys, m.s_vdep[q], sys.tri_v[tri]);
+    }
+    double scale;
+    if (m.s_nvol[q] == 0) scale = sys.tri_area[tri] * kAvogadro;
+    else if (m.s_out[q]) scale = 1.0e3 * sys.vol[o] * kAvogadro;   // o >= 0: its reactants are present
+    else scale = 1.0e3 * sys.vol[t] * kAvogadro;
+    return h * scaled_const(k, m.sorder[q], scale, 1.0 / scale);
+}
+
+// One period of group g (see above); returns through nev / moved.
+__device__ void group_period(const ModelK &m, const SysK &sys, int g, int launch, bool do_enter, bool do_step,
+                             long &nev_out, int32_t &moved_out) {
+    const int NT = sys.n_tets, NTR = sys.n_tris, S = m.n_specs, R = m.n_reacs, RS = m.n_sreacs;
+    const int RT = RS + m.n_ghk;
+    const int k0 = sys.grp_ptr[g], k1 = sys.grp_ptr[g + 1];
+    if (do_enter) {
+        for (int k = k0; k < k1; ++k) {
+            const int t = sys.grp_mem[k], r = sys.grp_real[k];
+            const int rtab = real_tab(sys, r);
+            const uint32_t clampm = tet_clamp_of(sys, t, rtab);
+            const int4 nb4 = sys.nbr[t];
+            const uint32_t nf = sys.nface[t];
+            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            for (int d = 0; d < m.n_diff; ++d) {
+                const int sp = m.dspec[d];
+                int32_t add = 0;
+                for (int f = 0; f < 4; ++f) {
+                    if (nbv[f] < 0) continue;
+                    const size_t ix = (size_t)d * NT * 4 + (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff);
+                    int32_t v = sys.out_prev[ix];
+                    if (v == kSlotEscape) v = sys.wide_prev[ix];
+                    add += v;
+                }
+                if (!((clampm >> sp) & 1u)) sys.counts[(size_t)sp * NT + t] += add;
+            }
+            if (m.n_sdiff > 0 && sys.tri_ptr != nullptr) {
+                for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri) {
+                    const int4 nb = sys.tri_nbr[tri];
+                    const uint32_t ne = sys.tri_nface[tri];
+                    const int tnb[3] = {nb.x, nb.y, nb.z};
+                    const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[tri] : 0u;
+                    for (int d = 0; d < m.n_sdiff; ++d) {
+                        const int32_t *op = sys.tout_prev + (size_t)d * NTR * 4;
+                        int32_t add = 0;
+                        for (int e = 0; e < 3; ++e)
+                            if (tnb[e] >= 0) add += op[(size_t)tnb[e] * 4 + ((ne >> (8 * e)) & 0xff)];
+                        if (!((tclamp >> m.sdspec[d]) & 1u)) sys.tri_counts[(size_t)m.sdspec[d] * NTR + tri] += add;
+                    }
+                }
+            }
+        }
+    }
+    if (!do_step) return;
+    const int ro = sys.grp_real[k0], to = sys.grp_mem[k0];
+    const uint32_t period = sys.real_step0[ro] + (uint32_t)launch;
+    const double dt = sys.dt_tab[(size_t)launch * sys.n_real + ro];
+    // occupancy corrections of the period start at zero
+    for (int k = k0; k < k1; ++k) {
+        const int t = sys.grp_mem[k];
+        for (int sp = 0; sp < S; ++sp)
+            if ((m.tracked_mask >> sp) & 1) sys.gcorr[(size_t)sp * NT + t] = 0.0;
+        if (m.tri_tracked_mask != 0 && sys.tri_ptr != nullptr)
+            for (int q = 0; q < m.n_sspecs; ++q)
+                if ((m.tri_tracked_mask >> q) & 1)
+                    for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri) sys.tri_corr[(size_t)q * NTR + tri] = 0.0;
+    }
+    // Direct SSA over the group: the members' volume reactions in member
+    // order, then each member's triangles' entries
+    Philox gph;
+    gph.init(sys.real_seed[ro], (uint32_t)(to - sys.real_tet0[ro]), period, sys.real_stream[ro], 0u);
+    long nev = 0;
+    double tcur = 0.0;
+    while (true) {
+        double a0 = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const int t = sys.grp_mem[k], r = sys.grp_real[k], rtab = real_tab(sys, r);
+            for (int q = 0; q < R; ++q) a0 = a0 + grp_vol_prop(m, sys, t, r, rtab, q);
+        }
+        for (int k = k0; k < k1 && sys.tri_ptr != nullptr; ++k) {
+            const int t = sys.grp_mem[k], r = sys.grp_real[k], rtab = real_tab(sys, r);
+            for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri)
+                for (int q = 0; q < RT; ++q) a0 = a0 + grp_tri_prop(m, sys, tri, t, r, rtab, q);
+        }
+        if (a0 < kDblEpsilon) break;
+        const double u1 = gph.unif();
+        if (1.0 - u1 > a0 * (dt - tcur) * 1.0000001) break;
+        const double u2 = gph.unif();
+        const double tev = tcur + (-dlog(u1) / a0);
+        if (tev > dt) break;
+        tcur = tev;
+        // selection: the first entry whose running sum exceeds x, else the
+        // last nonzero one; an entry is (member k, triangle or -1, index q)
+        const double x = u2 * a0;
+        int sk = -1, stri = -1, sq = -1, lk = -1, ltri = -1, lq = -1;
+        double acc = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const int t = sys.grp_mem[k], r = sys.grp_real[k], rtab = real_tab(sys, r);
+            for (int q = 0; q < R; ++q) {
+                const double pq = grp_vol_prop(m, sys, t, r, rtab, q);
+                acc = acc + pq;
+                if (sk < 0 && acc > x) { sk = k; stri = -1; sq = q; }
+                if (pq > 0.0) { lk = k; ltri = -1; lq = q; }
+            }
+        }
+        for (int k = k0; k < k1 && sys.tri_ptr != nullptr; ++k) {
+            const int t = sys.grp_mem[k], r = sys.grp_real[k], rtab = real_tab(sys, r);
+            for (int tri = sys.tri_ptr[t]; tri < sys.tri_ptr[t + 1]; ++tri)
+                for (int q = 0; q < RT; ++q) {
+                    const double pq = grp_tri_prop(m, sys, tri, t, r, rtab, q);
+                    acc = acc + pq;
+                    if (sk < 0 && acc > x) { sk = k; stri = tri; sq = q; }
+                    if (pq > 0.0) { lk = k; ltri = tri; lq = q; }
+                }
+        }
+        if (sk < 0) { sk = lk; stri = ltri; sq = lq; }
+        if (sk < 0) break;  // every propensity is zero (a0 came from rounding)
+        const int t = sys.grp_mem[sk], r = sys.grp_real[sk], rtab = real_tab(sys, r);
+        const uint32_t clampm = tet_clamp_of(sys, t, rtab);
+        if (stri < 0) {
+            for (int e = m.upd_off[sq]; e < m.upd_off[sq + 1]; ++e) {
+                const int sp = m.ent[e] & 0xffff;
+                const int dv = m.ent[e] >> 16;
+                if ((clampm >> sp) & 1u) continue;
+                sys.counts[(size_t)sp * NT + t] += dv;
+                if ((m.tracked_mask >> sp) & 1) sys.gcorr[(size_t)sp * NT + t] -= (double)dv * tcur;
+            }
+        } else {
+            const int o = sys.tri_otet[stri];
+            const uint32_t oclamp = o >= 0 ? tet_clamp_of(sys, o, real_tab(sys, sys.tri_oreal[stri])) : 0u;
+            if (sq >= RS) {
+                // GHK current: one ion crosses in the direction of the flux,
+                // from the inner tet to the outer one or back
+                // (ghkcurr.cpp:228-272); charge counted outward positive
+                const int gq = sq - RS;
+                const int ion = m.g_ion[gq];
+                const bool efflux = grp_ghk_rt(m, sys, stri, t, rtab, gq) >= 0.0;
+                if (m.g_real[gq]) {
+                    const int dv = efflux ? -1 : 1;
+                    if (!((clampm >> ion) & 1u)) {
+                        sys.counts[(size_t)ion * NT + t] += dv;
+                        if ((m.tracked_mask >> ion) & 1) sys.gcorr[(size_t)ion * NT + t] -= (double)dv * tcur;
+                    }
+                    if (m.g_outer[gq] && o >= 0 && !((oclamp >> ion) & 1u)) {
+                        sys.counts[(size_t)ion * NT + o] -= dv;
+                        if ((m.tracked_mask >> ion) & 1) sys.gcorr[(size_t)ion * NT + o] += (double)dv * tcur;
+                    }
+                }
+                sys.tri_charge[stri] += efflux ? m.g_z[gq] : -m.g_z[gq];
+            } else {
+                // surface reaction: triangle, inner and outer tet counts
+                // (SReac::apply, sreac.cpp:440-505)
+                const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[stri] : 0u;
+                for (int e = m.supd_off[sq]; e < m.supd_off[sq + 1]; ++e) {
+                    const int sp = m.ent[e] & 0xffff;
+                    const int dv = m.ent[e] >> 16;
+                    if (sp & kSurfFlag) {
+                        const int qs = sp & 0x7f;
+                        if ((tclamp >> qs) & 1u) continue;
+                        const size_t ix = (size_t)qs * NTR + stri;
+                        sys.tri_counts[ix] += dv;
+                        if ((m.tri_intg_mask >> qs) & 1) sys.tri_intg[ix] -= (double)dv * tcur;
+                        if ((m.tri_tracked_mask >> qs) & 1) sys.tri_corr[ix] -= (double)dv * tcur;
+                    } else if (sp & kOuterFlag) {
+                        const int so = sp & 0xff;
+                        if (o < 0 || ((oclamp >> so) & 1u)) continue;
+                        sys.counts[(size_t)so * NT + o] += dv;
+                        if ((m.tracked_mask >> so) & 1) sys.gcorr[(size_t)so * NT + o] -= (double)dv * tcur;
+                    } else {
+                        if ((clampm >> sp) & 1u) continue;
+                        sys.counts[(size_t)sp * NT + t] += dv;
+                        if ((m.tracked_mask >> sp) & 1) sys.gcorr[(size_t)sp * NT + t] -= (double)dv * tcur;
+                    }
+                }
+            }
+        }
+        if (++nev >= kMaxEventsPerPeriod) {
+            atomicOr(sys.err, 1);
+            nev = 0;
+            break;
+        }
+    }
+    nev_out += nev;
+    // channel-count integrals, then every member's leaving pass
+    int32_t moved = 0;
+    for (int k = k0; k < k1; ++k) {
+        const int t = sys.grp_mem[k], r = sys.grp_real[k], rtab = real_tab(sys, r);
+        const uint32_t clampm = tet_clamp_of(sys, t, rtab);
+        const int tp0 = sys.tri_ptr != nullptr ? sys.tri_ptr[t] : 0;
+        const int tp1 = sys.tri_ptr != nullptr ? sys.tri_ptr[t + 1] : 0;
+        if (m.tri_intg_mask != 0)
+            for (int q = 0; q < m.n_sspecs; ++q)
+                if ((m.tri_intg_mask >> q) & 1)
+                    for (int tri = tp0; tri < tp1; ++tri) {
+                        const size_t ix = (size_t)q * NTR + tri;
+                        sys.tri_intg[ix] += (double)sys.tri_counts[ix] * dt;
+                    }
+        const uint32_t ltet = (uint32_t)(t - sys.real_tet0[r]);
+        const uint32_t per = sys.real_step0[r] + (uint32_t)launch;
+        const double dtr = sys.dt_tab[(size_t)launch * sys.n_real + r];
+        const uint64_t seed = sys.real_seed[r];
+        const uint32_t stream = sys.real_stream[r];
+        const uint32_t bdir = sys.tet_bdir != nullptr ? sys.tet_bdir[t] : 0xffffffffu;
+        for (int d = 0; d < m.n_diff; ++d) {
+            const int sp = m.dspec[d];
+            const double2 g01 = sys.g[2 * (size_t)t];
+            const double2 g23 = sys.g[2 * (size_t)t + 1];
+            const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
+            const double *fD = (rtab & kTabDiff) ? sys.tet_fD + ((size_t)sp * NT + t) * 4 : nullptr;
+            const size_t ci = (size_t)sp * NT + t;
+            const int32_t n = sys.counts[ci];
+            int4 o;
+            const int32_t nl = leave(sys, n, sys.gcorr[ci], sp, (m.tracked_mask >> sp) & 1,
+                                     sys.real_D[(size_t)r * S + sp], r4, bdir, S, dtr, seed, ltet, per, stream, o, fD);
+            if (nl > 0 && !((clampm >> sp) & 1u)) sys.counts[ci] = n - nl;
+            moved += nl;
+            put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + t) * 4, o);
+        }
+        if (m.n_sdiff > 0) {
+            int4 *toc = reinterpret_cast<int4 *>(sys.tout_cur);
+            for (int tri = tp0; tri < tp1; ++tri) {
+                const uint32_t tclamp = (rtab & kTabTriClamp) ? sys.tri_clamp[tri] : 0u;
+                for (int d = 0; d < m.n_sdiff; ++d) {
+                    const int q = m.sdspec[d];
+                    int4 o;
+                    const double *eD = (rtab & kTabTriDiff) ? sys.tri_eD + ((size_t)q * NTR + tri) * 3 : nullptr;
+                    moved += tri_leave(sys, q, (m.tri_tracked_mask >> q) & 1, sys.real_sD[(size_t)r * m.n_sspecs + q],
+                                       tri, dtr, seed, per, stream, o, eD, (tclamp >> q) & 1u);
+                    toc[(size_t)d * NTR + tri] = o;
+                }
+            }
+        }
+    }
+    moved_out += moved;
+}
+
 #ifndef STG_PERIOD_WAVES_PER_EU
 #define STG_PERIOD_WAVES_PER_EU 1
 #endif
@@ -968,13 +1324,25 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     const bool valid = t < tend;
     long nev = 0;
     int32_t moved = 0;
-    // per-element tables of this realization (uniform over the block)
-    const int rtab = (MEMB && sys.real_tabs != nullptr) ? sys.real_tabs[r] : 0;
-    if (valid) {
+    // per-element tables of this realization (uniform over the block); the
+    // engine runs the generic kernel whenever a realization has one
+    const int rtab = (MEMB && !MT::kStatic && sys.real_tabs != nullptr) ? sys.real_tabs[r] : 0;
+    // a tet of a multi-tet SSA group: the group's first tet runs it all
+    bool in_group = false;
+    if constexpr (MEMB && !MT::kStatic) {
+        if (valid && sys.tet_grp != nullptr) {
+            const int grp = sys.tet_grp[t];
+            if (grp >= 0) {
+                in_group = true;
+                if (sys.grp_mem[sys.grp_ptr[grp]] == t) group_period(m, sys, grp, launch, do_enter, do_step, nev, moved);
+            }
+        }
+    }
+    if (valid && !in_group) {
         const int NT = sys.n_tets;
         // clamped species: nothing enters, leaves or reacts away (Diff::apply,
         // diff.cpp:476-577; Reac::apply, reac.cpp:240-243)
-        const uint32_t clampm = (MEMB && (rtab & kTabClamp)) ? sys.tet_clamp[t] : 0u;
+        const uint32_t clampm = (MEMB && !MT::kStatic && (rtab & kTabClamp)) ? sys.tet_clamp[t] : 0u;
         for (int s = 0; s < S; ++s) {
             L.c(s) = sys.counts[(size_t)s * NT + t];
             L.o(s) = 0.0;
@@ -1069,8 +1437,11 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                     const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
                     int4 o;
                     const double *fD = (MEMB && (rtab & kTabDiff)) ? sys.tet_fD + ((size_t)sp * NT + t) * 4 : nullptr;
-                    moved += leave(sys, L, sp, (tab_tracked<MT>(m) >> sp) & 1, uni(Dr[sp]), r4, bdir, S, dt,
-                                   seed, ltet, period, stream, o, fD, MEMB && ((clampm >> sp) & 1u));
+                    const int32_t n = L.c(sp);
+                    const int32_t nl = leave(sys, n, L.o(sp), sp, (tab_tracked<MT>(m) >> sp) & 1, uni(Dr[sp]), r4,
+                                             bdir, S, dt, seed, ltet, period, stream, o, fD);
+                    if (nl > 0 && !(MEMB && ((clampm >> sp) & 1u))) L.c(sp) = n - nl;
+                    moved += nl;
                     put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
                 }
             }

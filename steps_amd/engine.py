@@ -33,6 +33,18 @@ def _mesh_struct(mt: MeshTables, keep: list) -> N.StgMesh:
         m.tri_nbr = N.ptr(tn, C.c_int32)
         m.tri_nbr_edge = N.ptr(te, C.c_int32)
         m.tri_edge_g = N.ptr(tg, C.c_double)
+        if getattr(mt, "tri_nbr_comp", None) is not None:
+            tc = N.as_i32(mt.tri_nbr_comp)
+            keep.append(tc)
+            m.tri_nbr_comp = N.ptr(tc, C.c_int32)
+    if getattr(mt, "tri_otet", None) is not None and len(mt.tri_tet):
+        ot = N.as_i32(mt.tri_otet)
+        keep.append(ot)
+        m.tri_otet = N.ptr(ot, C.c_int32)
+        if mt.tri_ocomp is not None:
+            oc = N.as_i32(mt.tri_ocomp)
+            keep.append(oc)
+            m.tri_ocomp = N.ptr(oc, C.c_int32)
     return m
 
 
@@ -48,7 +60,7 @@ def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
     keep.extend(arrs)
     (rl, ru, rk, dc, sls, sus, sli, sui, sk, svd, vmin, vdv, voff, vk,
      gc, gi, gz, gp, go, gv, gr, sdc) = arrs
-    return N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
+    mdl = N.StgModel(len(fm.specs), len(fm.reacs), N.ptr(rl, C.c_int32), N.ptr(ru, C.c_int32),
                       N.ptr(rk, C.c_double), N.ptr(dc, C.c_double), len(fm.sspecs), len(fm.sreacs),
                       N.ptr(sls, C.c_int32), N.ptr(sus, C.c_int32), N.ptr(sli, C.c_int32),
                       N.ptr(sui, C.c_int32), N.ptr(sk, C.c_double), N.ptr(svd, C.c_int32),
@@ -57,6 +69,12 @@ def _model_struct(fm: FlatModel, keep: list) -> N.StgModel:
                       N.ptr(gc, C.c_int32), N.ptr(gi, C.c_int32), N.ptr(gz, C.c_int32),
                       N.ptr(gp, C.c_double), N.ptr(go, C.c_double), N.ptr(gv, C.c_double),
                       N.ptr(gr, C.c_int32), N.ptr(sdc, C.c_double))
+    if getattr(fm, "sreac_lhs_o", None) is not None and len(fm.sreacs):
+        lo, uo = N.as_i32(fm.sreac_lhs_o), N.as_i32(fm.sreac_upd_o)
+        keep.extend([lo, uo])
+        mdl.sreac_lhs_o = N.ptr(lo, C.c_int32)
+        mdl.sreac_upd_o = N.ptr(uo, C.c_int32)
+    return mdl
 
 
 class Engine:

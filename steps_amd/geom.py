@@ -435,6 +435,9 @@ class MeshTables:
     tri_nbr: np.ndarray | None = None    # (m,3) int32 neighbour triangle across each edge
     tri_nbr_edge: np.ndarray | None = None   # (m,3) int32 edge slot in the neighbour
     tri_edge_g: np.ndarray | None = None     # (m,3) float64 l_e / (A d_e), 0 = closed
+    tri_otet: np.ndarray | None = None   # (m,) int32 outer tet (local in its compartment), -1 none
+    tri_ocomp: np.ndarray | None = None  # (m,) int32 compartment of the outer tet
+    tri_nbr_comp: np.ndarray | None = None   # (m,3) int32 compartment of each edge neighbour
 
 
 def patch_tri_tables(mesh: Tetmesh, tris):
@@ -469,13 +472,21 @@ def patch_tri_tables(mesh: Tetmesh, tris):
     return nbr, nbe, g
 
 
-def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
+def comps_mesh_tables(mesh: Tetmesh, comps, patches, diffbs):
     """Tables of several compartments of one mesh (one MeshTables per
     compartment, local tet numbering in mesh order). A face between two
     compartments stays open only on a diffusion boundary: it then keeps its
     neighbour (nbr_comp, nbr) and carries direction 2 b + side, side 0 when
     the tet is in boundary b's compA (tetopsplit.cpp:2745-2931). Returns the
-    tables and, per direction, (boundary index, source comp, destination comp)."""
+    tables and, per direction, (boundary index, source comp, destination comp).
+
+    patches: a list of (triangles, inner compartment index, outer compartment
+    index or None), or {inner compartment index: triangles}. A compartment's
+    triangles are those of its patches in patch order; each triangle gets its
+    outer tet when the patch has an outer compartment; surface diffusion joins
+    triangles of the same patch only."""
+    if isinstance(patches, dict):
+        patches = [(t, ci, None) for ci, t in patches.items()]
     ncomp = len(comps)
     comp_of = -np.ones(mesh.countTets(), dtype=np.int64)
     local = -np.ones(mesh.countTets(), dtype=np.int64)
@@ -514,17 +525,32 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
         with np.errstate(divide="ignore", invalid="ignore"):
             g = np.where(keep, area / (vol[:, None] * dist), 0.0)
         g = np.where(np.isfinite(g), g, 0.0)
-        ptris = patch_tris_by_comp.get(ci)
-        if ptris is None:
+        mine = [(np.asarray(t, dtype=np.int64), oc) for t, ic, oc in patches if ic == ci]
+        tri_otet = tri_ocomp = None
+        if not mine:
             tri_tet, tri_area = np.zeros(0, dtype=np.int32), np.zeros(0)
+            tn = (None, None, None)
         else:
-            tris = np.asarray(ptris, dtype=np.int64)
+            tris = np.concatenate([t for t, _ in mine])
             tt = mesh.tri_tets[tris]
             inner = np.where(comp_of[np.maximum(tt[:, 0], 0)] == ci, tt[:, 0], tt[:, 1])
             if np.any(inner < 0) or np.any(comp_of[inner] != ci):
                 raise ValueError("patch triangle without an inner tet in its compartment")
             tri_tet, tri_area = local[inner].astype(np.int32), mesh.tri_area[tris]
-        tn = patch_tri_tables(mesh, ptris) if ptris is not None else (None, None, None)
+            outer = np.where(tt[:, 0] == inner, tt[:, 1], tt[:, 0])
+            if any(oc is not None for _, oc in mine):
+                ocs = np.concatenate([np.full(len(t), -1 if oc is None else oc) for t, oc in mine])
+                ok = (ocs >= 0) & (outer >= 0)
+                if np.any(ok & (comp_of[np.maximum(outer, 0)] != ocs)):
+                    raise ValueError("patch triangle whose outer tet is not in the outer compartment")
+                tri_otet = np.where(ok, local[np.maximum(outer, 0)], -1).astype(np.int32)
+                tri_ocomp = np.where(ok, ocs, ci).astype(np.int32)
+            # surface diffusion within each patch; edges between patches closed
+            parts = [patch_tri_tables(mesh, t) for t, _ in mine]
+            off = np.concatenate([[0], np.cumsum([len(t) for t, _ in mine])])
+            tn = (np.concatenate([np.where(p[0] >= 0, p[0] + off[i], -1) for i, p in enumerate(parts)]).astype(np.int32),
+                  np.concatenate([p[1] for p in parts]).astype(np.int32),
+                  np.concatenate([p[2] for p in parts]))
         out.append(MeshTables(
             vol=np.ascontiguousarray(vol, dtype=np.float64),
             nbr=np.ascontiguousarray(lnbr, dtype=np.int32),
@@ -534,7 +560,8 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patch_tris_by_comp, diffbs):
             tri_area=np.ascontiguousarray(tri_area, dtype=np.float64),
             nbr_comp=np.ascontiguousarray(np.where(keep, ncomp_arr, ci), dtype=np.int32),
             bdir=np.ascontiguousarray(bdir, dtype=np.int32),
-            tri_nbr=tn[0], tri_nbr_edge=tn[1], tri_edge_g=tn[2]))
+            tri_nbr=tn[0], tri_nbr_edge=tn[1], tri_edge_g=tn[2],
+            tri_otet=tri_otet, tri_ocomp=tri_ocomp))
     return out, dirs
 
 

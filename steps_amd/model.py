@@ -19,6 +19,16 @@ FARADAY = 96485.3399        # src/steps/math/constants.hpp:49
 GAS_CONSTANT = 8.314472     # src/steps/math/constants.hpp:53
 
 
+def _argerr(msg):
+    from ._native import STG_ERR_ARG, ArgErr
+    return ArgErr(STG_ERR_ARG, msg)
+
+
+def _notimpl(msg):
+    from ._native import STG_ERR_NOTIMPL, NotImplErr
+    return NotImplErr(STG_ERR_NOTIMPL, msg)
+
+
 class Model:
     def __init__(self):
         self.specs: dict[str, Spec] = {}
@@ -110,9 +120,11 @@ class Reac:
 
 
 class SReac:
-    """Surface reaction over patch species (slhs/srhs) and inner-compartment
-    species (ilhs/irhs). Outer-compartment species (olhs/orhs) need a tet on
-    both sides of the membrane and are rejected by the solver."""
+    """Surface reaction over patch species (slhs/srhs), inner-compartment
+    species (ilhs/irhs) and outer-compartment species (olhs/orhs); volume
+    reactants on one side only (src/steps/model/sreac.cpp:55). On a patch
+    with an outer compartment the triangle's inner and outer tets react in one
+    SSA group."""
 
     def __init__(self, id: str, surfsys: Surfsys, ilhs=(), slhs=(), olhs=(), irhs=(), srhs=(),
                  orhs=(), kcst: float = 0.0):
@@ -121,9 +133,16 @@ class SReac:
         self.ilhs, self.slhs, self.olhs = list(ilhs), list(slhs), list(olhs)
         self.irhs, self.srhs, self.orhs = list(irhs), list(srhs), list(orhs)
         self.kcst = float(kcst)
+        if self.ilhs and self.olhs:
+            raise ValueError("Volume lhs species must belong to either inner or outer compartment, "
+                             "not both.")
         if len(self.ilhs) + len(self.slhs) + len(self.olhs) > 4:
             raise ValueError("reaction order above 4 is not supported")
         surfsys.sreacs[id] = self
+
+    def reqOutside(self) -> bool:
+        """True when the reaction reads or changes outer-compartment species."""
+        return bool(self.olhs or self.orhs)
 
     def getOrder(self) -> int:
         return len(self.ilhs) + len(self.slhs) + len(self.olhs)
@@ -206,7 +225,9 @@ class GHKcurr:
     """GHK current of ``ion`` through channels in state chanstate
     (src/steps/model/ghkcurr.cpp:41-170; cysteps_model.pyx:3336-3562). The
     permeability comes from setP or setPInfo. computeflux=False moves charge
-    but no ions. virtual_oconc (M) stands in for the outer compartment."""
+    but no ions. virtual_oconc (M) stands in for the outer compartment; when
+    it is negative (the default) the patch's outer compartment is read and
+    exchanges the ions."""
 
     def __init__(self, id: str, surfsys: Surfsys, chanstate: Spec, ion: Spec,
                  computeflux: bool = True, virtual_oconc: float = -1.0, vshift: float = 0.0):
@@ -290,6 +311,19 @@ class FlatModel:
     ohmic_erev: np.ndarray = None                    # V
     sspec_dcst: np.ndarray = None                    # [n_sspecs] surface diffusion m^2/s
     sdiffs: dict = field(default_factory=dict)       # surface diff id -> surface species index
+    sreac_lhs_o: np.ndarray = None                   # [n_sreacs][S] outer-compartment reactants
+    sreac_upd_o: np.ndarray = None                   # [n_sreacs][S] outer-compartment changes
+    # several patches (flatten_comps): per patch the surface reactions, GHK
+    # currents and surface species it holds (index sets), its compartments
+    patch_sreacs: list = field(default_factory=list)
+    patch_ghk: list = field(default_factory=list)
+    patch_sspecs: list = field(default_factory=list)
+
+    def outer(self) -> bool:
+        """The model reaches the outer side of a membrane."""
+        return bool((self.sreac_lhs_o is not None and np.any(self.sreac_lhs_o))
+                    or (self.sreac_upd_o is not None and np.any(self.sreac_upd_o))
+                    or (self.ghk_voconc is not None and np.any(self.ghk_voconc < 0)))
 
     def spec_index(self, name: str) -> int:
         try:
@@ -326,18 +360,16 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     for ss in ssys:
         for r in ss.sreacs.values():
             if r.olhs or r.orhs:
-                raise NotImplementedError(
-                    f"surface reaction {r.id!r} involves the outer compartment, which needs a "
-                    "tet on both sides of the membrane")
+                raise _argerr(f"surface reaction {r.id!r} involves the outer compartment, but the "
+                              "patch has none")
             used_v.update(s.id for s in r.ilhs + r.irhs)
             used_s.update(s.id for s in r.slhs + r.srhs)
         for c in ss.ohmiccurrs.values():
             used_s.add(c.chanstate.id)
         for c in ss.ghkcurrs.values():
             if c.virtual_oconc < 0:
-                raise NotImplementedError(
-                    f"GHK current {c.id!r} needs an outer compartment; give it a "
-                    "virtual_oconc >= 0")
+                raise _argerr(f"GHK current {c.id!r} reads the outer compartment, but the patch has "
+                              "none; give it a virtual_oconc >= 0")
             if not c.P > 0.0:
                 raise ValueError(f"Permeability not defined for GHK current object {c.id!r}")
             used_s.add(c.chanstate.id)
@@ -386,7 +418,7 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.vdep_k = (np.concatenate([r.ktab for r in vd]) if vd else np.zeros(0)).astype(np.float64)
     ghk = [c for ss in ssys for c in ss.ghkcurrs.values()]
     if len(ghk) > 4:
-        raise NotImplementedError("at most 4 GHK currents are supported")
+        raise _notimpl("at most 4 GHK currents are supported")
     fm.ghk = [c.id for c in ghk]
     fm.ghk_chan = np.array([sspecs.index(c.chanstate.id) for c in ghk], dtype=np.int32)
     fm.ghk_ion = np.array([specs.index(c.ion.id) for c in ghk], dtype=np.int32)
@@ -405,16 +437,25 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
     fm.ohmic_g = np.array([c.g for c in ohm], dtype=np.float64)
     fm.ohmic_erev = np.array([c.erev for c in ohm], dtype=np.float64)
+    fm.sreac_lhs_o = np.zeros((n, S), dtype=np.int32)
+    fm.sreac_upd_o = np.zeros((n, S), dtype=np.int32)
+    if ssys:
+        fm.patch_sreacs = [set(range(n))]
+        fm.patch_ghk = [set(range(len(ghk)))]
+        fm.patch_sspecs = [set(sspecs)]
     return fm
 
 
 @dataclass
 class CompTables:
-    """Several compartments of one simulation: one flat model over the union of
-    their species and reactions (reaction i belongs to compartment
-    reac_comp[i]), with per-compartment constants (zero where a compartment
-    lacks a reaction or a diffusion rule), as Statedef holds one Compdef per
-    compartment (src/steps/solver/statedef.cpp, compdef.cpp)."""
+    """Several compartments and patches of one simulation: one flat model over
+    the union of their species and reactions (reaction i belongs to
+    compartment reac_comp[i]), with per-compartment constants (zero where a
+    compartment lacks a reaction or a diffusion rule), as Statedef holds one
+    Compdef per compartment and one Patchdef per patch (src/steps/solver/
+    statedef.cpp, compdef.cpp, patchdef.cpp). The surface reactions, GHK
+    currents and surface species are the union over the patches; fm.patch_*
+    says which of them each patch holds."""
 
     fm: FlatModel
     comp_reacs: list          # [comp] {reaction id: global index}
@@ -424,25 +465,57 @@ class CompTables:
     comp_dcst: np.ndarray     # [comp][species]
 
 
-def flatten_comps(model: Model, comps_volsys, surfsys_ids=()) -> CompTables:
+def flatten_comps(model: Model, comps_volsys, patches=()) -> CompTables:
     """Compartments with the given volume systems (one list per compartment)
-    and at most one patch (surface systems) on the inner side of one of them."""
+    and patches given as (surface system ids, inner compartment index, outer
+    compartment index or None). Surface reactions with outer-compartment
+    species and GHK currents reading the outer compartment need a patch with an
+    outer compartment."""
     per = [flatten(model, vs, ()) for vs in comps_volsys]
-    patch = flatten(model, [], surfsys_ids) if surfsys_ids else None
+    ssys_ids = []
+    for sids, _, _ in patches:
+        for sid in sids:
+            if sid not in ssys_ids:
+                ssys_ids.append(sid)
+    ssys = [model.surfsys[x] for x in ssys_ids]
+    for sids, _, oc in patches:
+        for sid in sids:
+            ss = model.surfsys[sid]
+            for r in ss.sreacs.values():
+                if r.reqOutside() and oc is None:
+                    raise _argerr(f"surface reaction {r.id!r} involves the outer compartment, but "
+                                  "the patch has none")
+            for c in ss.ghkcurrs.values():
+                if c.virtual_oconc < 0 and oc is None:
+                    raise _argerr(f"GHK current {c.id!r} reads the outer compartment, but the patch "
+                                  "has none; give it a virtual_oconc >= 0")
+                if not c.P > 0.0:
+                    raise ValueError(f"Permeability not defined for GHK current object {c.id!r}")
     used = set()
     for f in per:
         used.update(f.specs)
-    if patch is not None:
-        used.update(patch.specs)
+    used_s = set()
+    for ss in ssys:
+        for r in ss.sreacs.values():
+            used.update(x.id for x in r.ilhs + r.irhs + r.olhs + r.orhs)
+            used_s.update(x.id for x in r.slhs + r.srhs)
+        for c in ss.ohmiccurrs.values():
+            used_s.add(c.chanstate.id)
+        for c in ss.ghkcurrs.values():
+            used_s.add(c.chanstate.id)
+            used.add(c.ion.id)
+        for d in ss.diffs.values():
+            used_s.add(d.lig.id)
     order = list(model.specs)
-    specs = [s for s in order if s in used]
-    S = len(specs)
+    specs = [x for x in order if x in used]
+    sspecs = [x for x in order if x in used_s]
+    S, SS = len(specs), len(sspecs)
     reacs = [(ci, model.volsys[v].reacs[r]) for ci, vs in enumerate(comps_volsys)
              for v in vs for r in model.volsys[v].reacs]
     R = len(reacs)
-    fm = FlatModel(specs=specs, sspecs=patch.sspecs if patch else [],
-                   reacs=[f"{ci}:{r.id}" for ci, r in reacs],
-                   sreacs=patch.sreacs if patch else [])
+    sreacs = [r for ss in ssys for r in ss.sreacs.values()]
+    fm = FlatModel(specs=specs, sspecs=sspecs, reacs=[f"{ci}:{r.id}" for ci, r in reacs],
+                   sreacs=[r.id for r in sreacs])
     fm.reac_lhs = np.zeros((R, S), dtype=np.int32)
     fm.reac_upd = np.zeros((R, S), dtype=np.int32)
     for i, (_, r) in enumerate(reacs):
@@ -464,34 +537,62 @@ def flatten_comps(model: Model, comps_volsys, surfsys_ids=()) -> CompTables:
             comp_dcst[ci, sg] = f.spec_dcst[sj]
             d[did] = sg
         comp_diffs.append(d)
-    fm.spec_dcst = comp_dcst.max(axis=0)
+    fm.spec_dcst = comp_dcst.max(axis=0) if len(per) else np.zeros(S)
     for did in set().union(*[d.keys() for d in comp_diffs]) if comp_diffs else ():
         fm.diffs[did] = next(d[did] for d in comp_diffs if did in d)
-    if patch is not None:
-        SS = len(patch.sspecs)
-        n = len(patch.sreacs)
-        fm.sreac_lhs_s, fm.sreac_upd_s = patch.sreac_lhs_s, patch.sreac_upd_s
-        fm.sreac_lhs_i = np.zeros((n, S), dtype=np.int32)
-        fm.sreac_upd_i = np.zeros((n, S), dtype=np.int32)
-        for j, sp in enumerate(patch.specs):
-            g = specs.index(sp)
-            fm.sreac_lhs_i[:, g] = patch.sreac_lhs_i[:, j]
-            fm.sreac_upd_i[:, g] = patch.sreac_upd_i[:, j]
-        for name in ("sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv", "vdep_off", "vdep_k",
-                     "ghk", "ghk_chan", "ghk_valence", "ghk_perm", "ghk_voconc", "ghk_vshift",
-                     "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g", "ohmic_erev", "sspec_dcst",
-                     "sdiffs"):
-            setattr(fm, name, getattr(patch, name))
-        fm.ghk_ion = np.array([specs.index(patch.specs[i]) for i in patch.ghk_ion], dtype=np.int32)
-        del SS
-    else:
-        empty = flatten(Model(), [], ())
-        for name in ("sreac_lhs_s", "sreac_upd_s", "sreac_k", "sreac_vdep", "vdep_vmin", "vdep_dv",
-                     "vdep_off", "vdep_k", "ghk", "ghk_chan", "ghk_ion", "ghk_valence", "ghk_perm",
-                     "ghk_voconc", "ghk_vshift", "ghk_realflux", "ohmic", "ohmic_chan", "ohmic_g",
-                     "ohmic_erev", "sspec_dcst", "sdiffs"):
-            setattr(fm, name, getattr(empty, name))
-        fm.sreac_lhs_i = np.zeros((0, S), dtype=np.int32)
-        fm.sreac_upd_i = np.zeros((0, S), dtype=np.int32)
+    n = len(sreacs)
+    fm.sreac_lhs_s = np.zeros((n, SS), dtype=np.int32)
+    fm.sreac_upd_s = np.zeros((n, SS), dtype=np.int32)
+    fm.sreac_lhs_i = np.zeros((n, S), dtype=np.int32)
+    fm.sreac_upd_i = np.zeros((n, S), dtype=np.int32)
+    fm.sreac_lhs_o = np.zeros((n, S), dtype=np.int32)
+    fm.sreac_upd_o = np.zeros((n, S), dtype=np.int32)
+    for i, r in enumerate(sreacs):
+        ls, li, lo = _count(r.slhs, sspecs), _count(r.ilhs, specs), _count(r.olhs, specs)
+        fm.sreac_lhs_s[i], fm.sreac_lhs_i[i], fm.sreac_lhs_o[i] = ls, li, lo
+        fm.sreac_upd_s[i] = _count(r.srhs, sspecs) - ls
+        fm.sreac_upd_i[i] = _count(r.irhs, specs) - li
+        fm.sreac_upd_o[i] = _count(r.orhs, specs) - lo
+    fm.sreac_k = np.array([r.kcst for r in sreacs], dtype=np.float64)
+    vd = [r for r in sreacs if isinstance(r, VDepSReac)]
+    fm.sreac_vdep = np.array([vd.index(r) if isinstance(r, VDepSReac) else -1 for r in sreacs],
+                             dtype=np.int32)
+    fm.vdep_vmin = np.array([r.vmin for r in vd], dtype=np.float64)
+    fm.vdep_dv = np.array([r.dv for r in vd], dtype=np.float64)
+    fm.vdep_off = np.concatenate([[0], np.cumsum([len(r.ktab) for r in vd])]).astype(np.int32)
+    fm.vdep_k = (np.concatenate([r.ktab for r in vd]) if vd else np.zeros(0)).astype(np.float64)
+    ghk = [c for ss in ssys for c in ss.ghkcurrs.values()]
+    if len(ghk) > 4:
+        raise _notimpl("at most 4 GHK currents are supported")
+    fm.ghk = [c.id for c in ghk]
+    fm.ghk_chan = np.array([sspecs.index(c.chanstate.id) for c in ghk], dtype=np.int32)
+    fm.ghk_ion = np.array([specs.index(c.ion.id) for c in ghk], dtype=np.int32)
+    fm.ghk_valence = np.array([c.ion.valence for c in ghk], dtype=np.int32)
+    fm.ghk_perm = np.array([c.P for c in ghk], dtype=np.float64)
+    fm.ghk_voconc = np.array([c.virtual_oconc for c in ghk], dtype=np.float64)
+    fm.ghk_vshift = np.array([c.vshift for c in ghk], dtype=np.float64)
+    fm.ghk_realflux = np.array([int(c.computeflux) for c in ghk], dtype=np.int32)
+    fm.sspec_dcst = np.zeros(SS)
+    for ss in ssys:
+        for d in ss.diffs.values():
+            fm.sspec_dcst[sspecs.index(d.lig.id)] = d.dcst
+            fm.sdiffs[d.id] = sspecs.index(d.lig.id)
+    ohm = [c for ss in ssys for c in ss.ohmiccurrs.values()]
+    fm.ohmic = [c.id for c in ohm]
+    fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
+    fm.ohmic_g = np.array([c.g for c in ohm], dtype=np.float64)
+    fm.ohmic_erev = np.array([c.erev for c in ohm], dtype=np.float64)
+    for sids, _, _ in patches:
+        held = [model.surfsys[x] for x in sids]
+        fm.patch_sreacs.append({sreacs.index(r) for ss in held for r in ss.sreacs.values()})
+        fm.patch_ghk.append({ghk.index(c) for ss in held for c in ss.ghkcurrs.values()})
+        ps = set()
+        for ss in held:
+            for r in ss.sreacs.values():
+                ps.update(x.id for x in r.slhs + r.srhs)
+            ps.update(c.chanstate.id for c in ss.ohmiccurrs.values())
+            ps.update(c.chanstate.id for c in ss.ghkcurrs.values())
+            ps.update(d.lig.id for d in ss.diffs.values())
+        fm.patch_sspecs.append(ps)
     comp_specs = [set(f.specs) for f in per]
     return CompTables(fm, comp_reacs, comp_diffs, comp_specs, comp_k, comp_dcst)

@@ -8,6 +8,8 @@ engine, the CPU oracle, the tests and bench.py.
   four_species_cube        4 species, 6 reactions on a 40^3 x 6 cube of side 40 um
   calcium_membrane_cube    Ca buffering with membrane pumps and influx, 100^3 x 6 cube, 20 um
   ab_binding_ensemble      1024 realizations of A + B <-> C on cubes n = 6..20
+  two_comp_membrane        two compartments joined by a membrane with transport
+                           reactions and a GHK current between them
 """
 from __future__ import annotations
 
@@ -218,3 +220,48 @@ def two_comp_cube(n=6, side=6e-6, dA=1e-12, dC_left=5e-13, dC_right=2e-12):
     M.Diff("dC", vr, C, dC_right)
     M.Diff("dR", vr, M.Spec("R", mdl), dA)
     return mdl, mesh, {"left": cl, "right": cr}, db
+
+
+def two_comp_membrane(n=4, side=4e-6, k_in=2e9, k_out=1e9, n_a=4000, n_t=1, binding=True,
+                      ghk=True, dA=1e-12, seed=0):
+    """A Kuhn cube cut at x = side/2: 'inner' (x < side/2) and 'outer'
+    compartments, both with A (and Ca) diffusing at dA; the triangles of the
+    cut form patch 'memb' (inner 'inner', outer 'outer') with n_t transporters
+    T per triangle: T + A(outer) -> T + A(inner) at k_in and T + A(inner) ->
+    T + A(outer) at k_out (/M/s). With ``binding`` the inner compartment also
+    holds A + B <-> C (1e6 /M/s, 1 /s; B and C immobile); with ``ghk`` every
+    triangle has one open channel Ch passing a GHK calcium current (20 pS at
+    -65 mV) between the two compartments. A and Ca start in the outer
+    compartment. Returns (model, mesh, comps {'inner', 'outer'}, patch)."""
+    mesh = geom.kuhn_cube(n, side)
+    left = np.nonzero(mesh.tet_bary[:, 0] < side / 2)[0]
+    right = np.nonzero(mesh.tet_bary[:, 0] >= side / 2)[0]
+    ci = geom.TmComp("inner", mesh, left)
+    co = geom.TmComp("outer", mesh, right)
+    ci.addVolsys("vin")
+    co.addVolsys("vout")
+    tt = mesh.tri_tets
+    side_of = np.zeros(mesh.countTets(), dtype=np.int64)
+    side_of[right] = 1
+    both = tt[:, 1] >= 0
+    cut = np.nonzero(both & (side_of[tt[:, 0]] != side_of[np.maximum(tt[:, 1], 0)]))[0]
+    patch = geom.TmPatch("memb", mesh, cut, ci, co)
+    patch.addSurfsys("ssys")
+    mdl = M.Model()
+    A, B, C = M.Spec("A", mdl), M.Spec("B", mdl), M.Spec("C", mdl)
+    Ca = M.Spec("Ca", mdl, valence=2)
+    T, Ch = M.Spec("T", mdl), M.Spec("Ch", mdl)
+    vin, vout = M.Volsys("vin", mdl), M.Volsys("vout", mdl)
+    for vs in (vin, vout):
+        M.Diff("dA", vs, A, dA)
+        M.Diff("dCa", vs, Ca, dA)
+    if binding:
+        M.Reac("bind", vin, lhs=[A, B], rhs=[C], kcst=1e6)
+        M.Reac("unbind", vin, lhs=[C], rhs=[A, B], kcst=1.0)
+    ss = M.Surfsys("ssys", mdl)
+    M.SReac("in", ss, slhs=[T], olhs=[A], srhs=[T], irhs=[A], kcst=k_in)
+    M.SReac("out", ss, slhs=[T], ilhs=[A], srhs=[T], orhs=[A], kcst=k_out)
+    if ghk:
+        g = M.GHKcurr("gca", ss, Ch, Ca, computeflux=True)
+        g.setPInfo(20e-12, -65e-3, 293.15, 2e-3, 1e-7)
+    return mdl, mesh, {"inner": ci, "outer": co}, patch

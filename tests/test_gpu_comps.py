@@ -178,3 +178,77 @@ def test_solver_specialises_large_meshes(monkeypatch):
         out.append([sim.getCompSpecCount(c, s) for c in ("left", "right") for s in ("A", "B", "C")])
     print("case solver specialisation threshold", out, flush=True)
     assert out[0] == out[1]
+
+
+def _membrane_engine_and_oracle(seed, mode, binding=True, ghk=True):
+    mdl, mesh, comps, patch = W.two_comp_membrane(n=4, side=4e-6, binding=binding, ghk=ghk)
+    cl = [comps["inner"], comps["outer"]]
+    ct = M.flatten_comps(mdl, [c.volsys for c in cl], [(patch.surfsys, 0, 1)])
+    mts, _ = geom.comps_mesh_tables(mesh, cl, [(patch.tris, 0, 1)], [])
+    S, SS = len(ct.fm.specs), len(ct.fm.sspecs)
+    eng = Engine(ct.fm, mts, [seed, seed], streams=[0, 1], reac_k=ct.comp_k, spec_dcst=ct.comp_dcst,
+                 mode=mode, comps=True)
+    o = bind.OracleSim.comps(ct.fm, mts, seed, ct.comp_k, ct.comp_dcst)
+    rng = np.random.default_rng(seed)
+    init = np.zeros((S, o.comp_off[2]), dtype=np.int32)
+    a, b, ca = (ct.fm.specs.index(x) for x in ("A", "B", "Ca"))
+    init[a, o.comp_off[1]:] = rng.poisson(20, o.comp_off[2] - o.comp_off[1])
+    init[b, :o.comp_off[1]] = rng.poisson(10, o.comp_off[1])
+    init[ca, :] = rng.poisson(15, o.comp_off[2])
+    tri = np.zeros((SS, len(patch.tris)), dtype=np.int32)
+    tri[ct.fm.sspecs.index("T")] = rng.integers(1, 4, len(patch.tris))
+    tri[ct.fm.sspecs.index("Ch")] = rng.integers(0, 3, len(patch.tris))
+    o.counts[:] = init
+    o.tri_counts[:SS, :len(patch.tris)] = tri
+    for ci in range(2):
+        for s in range(S):
+            eng.set_tet_counts(s, init[s, o.comp_off[ci]:o.comp_off[ci + 1]], real=ci)
+    for q in range(SS):
+        eng.set_tri_counts(q, tri[q], real=0)
+    v = np.linspace(-0.08, 0.02, len(patch.tris))
+    eng.set_tri_v(v, real=0)
+    o.tri_v[:len(patch.tris)] = v
+    return eng, o, ct, S, SS, len(patch.tris)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_membrane_between_compartments_bit_exact(mode):
+    """Transport reactions with outer-compartment reactants and products and a
+    GHK current reading the outer tet, on a membrane between two compartments
+    (every cut triangle joins an inner and an outer tet into one SSA group),
+    with a binding reaction on the inner side: GPU against the oracle."""
+    eng, o, ct, S, SS, T = _membrane_engine_and_oracle(7, mode)
+    print(f"case membrane between compartments mode={mode}", flush=True)
+    for t in (0.01, 0.04, 0.1):
+        eng.run(t)
+        o.run(t, mode)
+        assert np.array_equal(_gpu_counts(eng, S, 2), o.counts), t
+        assert np.array_equal(np.stack([eng.get_tri_counts(q, 0) for q in range(SS)]),
+                              o.tri_counts[:SS, :T]), t
+    ev = sum(eng.extents(r)[0] for r in range(2))
+    mv = sum(eng.extents(r)[1] for r in range(2))
+    assert (ev, mv) == (o.events.value, o.moved.value) and ev > 0
+    ca = ct.fm.specs.index("Ca")
+    assert o.counts[ca].sum() == _gpu_counts(eng, S, 2)[ca].sum()
+
+
+def test_membrane_groups_with_tables_and_graph():
+    """The group path with per-element tables (clamped calcium in a few tets,
+    per-tet binding constants) and graph replay at block 128."""
+    eng, o, ct, S, SS, T = _membrane_engine_and_oracle(11, 0)
+    print("case membrane groups with tables and graph", flush=True)
+    n0 = o.comp_off[1]
+    rng = np.random.default_rng(1)
+    clamp = np.where(rng.random(n0) < 0.2, 1 << ct.fm.specs.index("Ca"), 0).astype(np.uint32)
+    k = ct.comp_k[0][:, None] * rng.uniform(0.5, 2.0, size=(len(ct.fm.reacs), n0))
+    eng.set_tet_clamp(clamp, real=0)
+    eng.set_tet_reac_table(k, real=0)
+    eng.set_launch(128, True)
+    full_clamp = np.zeros(o.comp_off[2], dtype=np.uint32)
+    full_clamp[:n0] = clamp
+    full_k = np.concatenate([k, np.repeat(ct.comp_k[1][:, None], o.comp_off[2] - n0, axis=1)], axis=1)
+    o.set_tables(tet_clamp=full_clamp, tet_k=full_k)
+    for t in (0.02, 0.05):
+        eng.run(t)
+        o.run(t, 0)
+        assert np.array_equal(_gpu_counts(eng, S, 2), o.counts), t

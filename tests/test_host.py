@@ -25,13 +25,22 @@ def test_flatten_stoichiometry_and_order():
     assert fm.sreac_lhs_s.tolist() == [[1]] and fm.sreac_upd_s.tolist() == [[0]]
 
 
-def test_flatten_rejects_outer_compartment_and_high_order():
+def test_flatten_needs_an_outer_compartment_and_rejects_high_order():
+    from steps_amd import _native as N
+
     mdl = M.Model()
     A = M.Spec("A", mdl)
     ss = M.Surfsys("s", mdl)
     M.SReac("sr", ss, olhs=[A], kcst=1.0)
-    with pytest.raises(NotImplementedError):
-        M.flatten(mdl, [], ["s"])
+    with pytest.raises(N.ArgErr, match="outer compartment"):
+        M.flatten(mdl, [], ["s"])                      # a patch without an outer compartment
+    with pytest.raises(N.ArgErr, match="outer compartment"):
+        M.flatten_comps(mdl, [[], []], [(["s"], 0, None)])
+    ct = M.flatten_comps(mdl, [[], []], [(["s"], 0, 1)])
+    assert ct.fm.sreac_lhs_o.tolist() == [[1]] and ct.fm.sreac_upd_o.tolist() == [[-1]]
+    assert ct.fm.outer() and ct.fm.patch_sreacs == [{0}]
+    with pytest.raises(ValueError):
+        M.SReac("both", ss, ilhs=[A], olhs=[A], kcst=1.0)
     vs = M.Volsys("v", mdl)
     with pytest.raises(ValueError):
         M.Reac("r", vs, lhs=[A] * 5, rhs=[], kcst=1.0)
@@ -147,5 +156,6 @@ def test_channel_and_current_tables():
         M.GHKcurr("bad", ss, n1, M.Spec("neutral", mdl))   # valence zero
     g2 = M.GHKcurr("nooc", ss, n1, k)
     g2.setP(1e-19)
-    with pytest.raises(NotImplementedError):
-        M.flatten(mdl, ["vsys"], ["ssys"])          # needs a virtual outer concentration
+    from steps_amd import _native as N
+    with pytest.raises(N.ArgErr, match="outer compartment"):
+        M.flatten(mdl, ["vsys"], ["ssys"])          # reads the outer compartment: the patch has none

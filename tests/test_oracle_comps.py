@@ -212,3 +212,111 @@ def test_surface_point_source_spreads_as_2d_diffusion():
     assert c.sum() == 20000
     r2 = (np.linalg.norm(bary - bary[src], axis=1) ** 2 * c).sum() / c.sum()
     assert abs(r2 / (4 * D * t) - 1) < 0.1, r2 / (4 * D * t)
+
+
+def _transport(k_in, k_out, seed, binding=False, ghk=False, n_a=4000, dA=1e-11):
+    mdl, mesh, comps, patch = W.two_comp_membrane(n=4, side=4e-6, k_in=k_in, k_out=k_out,
+                                                  binding=binding, ghk=ghk, dA=dA)
+    cl = [comps["inner"], comps["outer"]]
+    ct = M.flatten_comps(mdl, [c.volsys for c in cl], [(patch.surfsys, 0, 1)])
+    mts, dirs = geom.comps_mesh_tables(mesh, cl, [(patch.tris, 0, 1)], [])
+    o = bind.OracleSim.comps(ct.fm, mts, seed, ct.comp_k, ct.comp_dcst)
+    a = ct.fm.specs.index("A")
+    o.counts[a, o.comp_off[1]:] = W.place_uniform(n_a, mts[1].vol, np.random.default_rng(seed))
+    o.tri_counts[ct.fm.sspecs.index("T"), :len(patch.tris)] = 1
+    return o, ct, mts
+
+
+def test_transport_across_a_membrane_reaches_its_equilibrium():
+    """T + A(outer) -> T + A(inner) at k_in and the reverse at k_out, with the
+    outer species read from the triangle's outer tet (SReac::rate/apply with
+    olhs/orhs, sreac.cpp:305-505). Each molecule's stationary distribution
+    satisfies detailed balance with weight V_c per compartment scaled by k_in
+    (inner) and k_out (outer), so the inner share is
+    k_in V_in / (k_in V_in + k_out V_out) = 2/3 for k_in = 2 k_out and equal
+    halves. With D = 1e-11 m^2/s each half mixes in L^2/D = 0.4 s; the time
+    average over 2..8 s must match within 2% (reading the inner tet's count
+    for the outer reactant, a plausible mistake, moves it far away; the
+    scaling volume is pinned by the two-tet test below)."""
+    o, ct, mts = _transport(2e9, 1e9, 3)
+    a = ct.fm.specs.index("A")
+    dt = o.update_period()
+    samples = []
+    for k in range(1, 81):
+        o.run(k * 0.1, 0)
+        if k > 20:
+            samples.append(o.counts[a, :o.comp_off[1]].sum())
+        assert o.counts[a].sum() == 4000
+    share = np.mean(samples) / 4000
+    assert abs(share - 2 / 3) < 0.02 * (2 / 3), share
+    assert o.events.value > 1000
+    assert mts[0].vol.sum() == pytest.approx(mts[1].vol.sum())
+
+
+def test_outer_ghk_current_exchanges_ions_between_compartments():
+    """A GHK calcium current with no virtual concentration reads the outer
+    tet's calcium and moves ions between the two tets (ghkcurr.cpp:176-182,
+    228-272): calcium is conserved over the two compartments and the net
+    charge moved equals twice the ions that left the inner compartment."""
+    o, ct, mts = _transport(0.0, 0.0, 5, ghk=True, n_a=0)
+    ca = ct.fm.specs.index("Ca")
+    o.counts[ca, o.comp_off[1]:] = 300
+    o.tri_counts[ct.fm.sspecs.index("Ch"), :] = 1
+    o.tri_v[:] = -0.065
+    total = int(o.counts[ca].sum())
+    o.run(0.05, 0)
+    inner = int(o.counts[ca, :o.comp_off[1]].sum())
+    assert int(o.counts[ca].sum()) == total and inner > 0
+    assert int(o.tri_charge.sum()) == -2 * inner
+
+
+def _two_tet_membrane(k, outer_reactant):
+    """Two tets sharing one triangle, the outer one 8 times the inner's volume;
+    the triangle is a patch with compartments 'in' and 'out' and one
+    transporter T moving A irreversibly across it at k (/M/s)."""
+    verts = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 0, -8]], dtype=float) * 1e-6
+    mesh = geom.Tetmesh(verts, np.array([[0, 1, 2, 3], [0, 1, 2, 4]]))
+    ci, co = geom.TmComp("in", mesh, [0]), geom.TmComp("out", mesh, [1])
+    ci.addVolsys("v")
+    co.addVolsys("v")
+    tri = int(np.intersect1d(mesh.tet_tri[0], mesh.tet_tri[1])[0])
+    patch = geom.TmPatch("p", mesh, [tri], ci, co)
+    patch.addSurfsys("s")
+    mdl = M.Model()
+    A, T = M.Spec("A", mdl), M.Spec("T", mdl)
+    M.Volsys("v", mdl)
+    ss = M.Surfsys("s", mdl)
+    if outer_reactant:
+        M.SReac("in", ss, slhs=[T], olhs=[A], srhs=[T], irhs=[A], kcst=k)
+    else:
+        M.SReac("out", ss, slhs=[T], ilhs=[A], srhs=[T], orhs=[A], kcst=k)
+    ct = M.flatten_comps(mdl, [["v"], ["v"]], [(["s"], 0, 1)])
+    mts, _ = geom.comps_mesh_tables(mesh, [ci, co], [(patch.tris, 0, 1)], [])
+    return ct, mts, mesh
+
+
+@pytest.mark.parametrize("outer_reactant", [True, False])
+def test_surface_reaction_scales_with_the_reacting_sides_volume(outer_reactant):
+    """An irreversible T + A -> T + A' across one triangle: each A leaves
+    independently at k / (1e3 V N_A) with V the volume of the tet it reacts
+    from, the outer tet for outer reactants and the inner tet for inner ones
+    (SReac::setKcst, sreac.cpp:160-180). The survivors of 200 molecules after
+    t are Binomial(200, exp(-k t / (1e3 V N_A))) over 60 seeds; the two tets'
+    volumes differ 8-fold, so scaling with the wrong one fails by far."""
+    AV = 6.02214179e23
+    k, t, n0 = 5e9, 0.02, 200
+    ct, mts, mesh = _two_tet_membrane(k, outer_reactant)
+    src = 1 if outer_reactant else 0
+    v = mesh.tet_vol[src]
+    assert mesh.tet_vol[1] == pytest.approx(8 * mesh.tet_vol[0])
+    p = np.exp(-k * t / (1e3 * v * AV))
+    out = []
+    for seed in range(60):
+        o = bind.OracleSim.comps(ct.fm, [mts[0], mts[1]], seed, ct.comp_k, ct.comp_dcst)
+        o.counts[0, src] = n0
+        o.tri_counts[ct.fm.sspecs.index("T"), 0] = 1
+        o.run_periods([t])
+        assert o.counts[0].sum() == n0
+        out.append(o.counts[0, src])
+    z = (np.mean(out) - n0 * p) / np.sqrt(n0 * p * (1 - p) / len(out))
+    assert abs(z) < 3.0, (np.mean(out), n0 * p)
