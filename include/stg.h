@@ -199,6 +199,11 @@ int stg_set_time(stg_sim *sim, double t);
 /* API::getNSteps: periods executed since reset, per realization. */
 int stg_get_nsteps(const stg_sim *sim, int32_t real, uint64_t *n);
 
+/* TetOpSplitP::setNSteps (tetopsplit.cpp:1995-1997): overwrite the period
+ * count of one realization. The random streams are keyed by their own counter,
+ * so this changes no later result. */
+int stg_set_nsteps(stg_sim *sim, int32_t real, uint64_t n);
+
 /* TetOpSplitP::getUpdPeriod (tetopsplit.cpp:5373-5411, getter at :8302-8328). */
 int stg_get_update_period(const stg_sim *sim, int32_t real, double *dt);
 
@@ -397,6 +402,25 @@ int stg_efield_get_tris_v(stg_efield *ef, double *v);
 int stg_efield_set_ohmic(stg_efield *ef, int32_t n, const int32_t *sspec, const double *g,
                          const double *erev);
 
+/* TetOpSplitP::_setTriOhmicErev / _getTriOhmicErev (tetopsplit.cpp:4933-4990,
+ * Tri::setOCerev): reversal potential (V) of ohmic current k on one membrane
+ * triangle. stg_efield_set_ohmic puts every triangle back to erev[k]. */
+int stg_efield_set_tri_ohmic_erev(stg_efield *ef, int32_t k, int32_t tri, double erev);
+int stg_efield_get_tri_ohmic_erev(stg_efield *ef, int32_t k, int32_t tri, double *erev);
+
+/* TetOpSplitP::_setTriCapac (tetopsplit.cpp:5187-5203, TetMesh::
+ * applyTriCapacitance): specific capacitance (F/m^2) of one membrane triangle;
+ * each of its vertices gains (cm - previous) * area / 3. stg_efield_set_capac
+ * sets every triangle. */
+int stg_efield_set_tri_capac(stg_efield *ef, int32_t tri, double cm);
+int stg_efield_get_tri_capac(stg_efield *ef, int32_t tri, double *cm);
+
+/* Tri::getGHKI (tri.cpp:634-660) for every membrane triangle: the charge GHK
+ * current g (or every current, g = -1) moved outward over the last EField
+ * step, times the elementary charge, over the step (A, outward positive);
+ * 0 before the first step after attaching. */
+int stg_efield_get_tri_ghk_currents(stg_efield *ef, int32_t g, double *amps);
+
 /* Couple to realization real of a simulation whose patch triangles are the
  * EField's membrane triangles (NULL detaches). */
 int stg_efield_attach(stg_efield *ef, stg_sim *sim, int32_t real);
@@ -412,9 +436,10 @@ int stg_efield_run(stg_efield *ef, stg_sim *sim, double endtime, double efdt);
 /* EField::checkpoint / restore as TetOpSplitP::checkpoint / restore call
  * them (pEField->checkpoint, tetopsplit.cpp:309; pEField->restore, :446),
  * own format: vertex potentials, the previous step's increment, membrane and
- * clamp parameters, and the start time of the running channel-count
- * integrals (those integrals and the GHK charges are in the attached
- * simulation's checkpoint). stg_efield_restore requires the same mesh. */
+ * clamp parameters (per-triangle capacitance and ohmic reversal potentials
+ * included), the start time of the running channel-count integrals and the
+ * GHK charges of the last step (the running integrals and charges are in the
+ * attached simulation's checkpoint). stg_efield_restore requires the same mesh. */
 int stg_efield_state_size(const stg_efield *ef, size_t *bytes);
 int stg_efield_checkpoint(stg_efield *ef, void *buf, size_t bytes);
 int stg_efield_restore(stg_efield *ef, const void *buf, size_t bytes);

@@ -168,7 +168,7 @@ class OracleSim:
             gz=i32(_or(fm, "ghk_valence", [])), gp=f64(_or(fm, "ghk_perm", [])),
             go=f64(_or(fm, "ghk_voconc", [])), gv=f64(_or(fm, "ghk_vshift", [])),
             gr=i32(_or(fm, "ghk_realflux", [])),
-            charge=np.zeros(max(self.n_tris, 1), dtype=np.int32),
+            charge=np.zeros((max(len(_or(fm, "ghk_chan", [])), 1), max(self.n_tris, 1)), dtype=np.int32),
             sdc=f64(_or(fm, "sspec_dcst", np.zeros(SS))),
         )
         if getattr(mt, "tri_nbr", None) is not None and self.n_tris:
@@ -306,8 +306,17 @@ class OracleSim:
 
     @property
     def tri_charge(self) -> np.ndarray:
-        """GHK charge moved out through each triangle (elementary charges)."""
+        """GHK charge moved out through each triangle by all currents
+        (elementary charges; a copy)."""
+        return self.a["charge"].sum(axis=0, dtype=np.int64).astype(np.int32)
+
+    @property
+    def tri_charge_per_current(self) -> np.ndarray:
+        """[current][triangle] GHK charge moved out (elementary charges)."""
         return self.a["charge"]
+
+    def clear_charge(self) -> None:
+        self.a["charge"][:] = 0
 
     def set_temp(self, kelvin: float) -> None:
         self.sys.temp = kelvin

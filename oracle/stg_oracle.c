@@ -760,7 +760,7 @@ static long ssa_tet(const stgo_system *s, const topo *tp, int32_t t, double dt, 
                 cnt[ion] += dv;
                 if (tp->tracked[ion]) corr[ion] -= (double)dv * tcur;
             }
-            s->tri_charge[tr] += efflux ? s->ghk_valence[g] : -s->ghk_valence[g];
+            s->tri_charge[(size_t)g * s->n_tris + tr] += efflux ? s->ghk_valence[g] : -s->ghk_valence[g];
         } else {
             const int32_t i = (sel - R) / RT, r = (sel - R) % RT;
             const int32_t tr = tp->tri_idx[t0 + i];
@@ -1007,7 +1007,7 @@ static long ssa_grp(stgo_system *s, const topo *tp, int32_t g, double dt, uint32
                     grp_add(s, tp, t, ion, efflux ? -1 : 1, tcur, gcorr);
                     if (s->ghk_voconc[gq] < 0.0 && o >= 0) grp_add(s, tp, o, ion, efflux ? 1 : -1, tcur, gcorr);
                 }
-                s->tri_charge[tr] += efflux ? s->ghk_valence[gq] : -s->ghk_valence[gq];
+                s->tri_charge[(size_t)gq * s->n_tris + tr] += efflux ? s->ghk_valence[gq] : -s->ghk_valence[gq];
             } else {
                 /* SReac::apply, sreac.cpp:440-505 */
                 const int32_t *us_ = s->sreac_upd_s + st_q * SS;

@@ -63,8 +63,8 @@ typedef struct stgo_system {
     uint32_t tri_intg_mask;
     double  *tri_intg;
     /* GHK currents (ghkcurr.cpp:168-272) with a virtual outer concentration;
-     * tri_charge [n_tris] accumulates the charge moved out (elementary
-     * charges); may be null when n_ghk is 0 */
+     * tri_charge [n_ghk][n_tris] accumulates the charge each current moved
+     * out (elementary charges); may be null when n_ghk is 0 */
     int32_t n_ghk;
     const int32_t *ghk_chan;     /* [n_ghk] surface species */
     const int32_t *ghk_ion;      /* [n_ghk] volume species */

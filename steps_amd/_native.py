@@ -121,6 +121,7 @@ SIGNATURES = {
     "stg_get_time": (C.c_int, [C.c_void_p, _dp]),
     "stg_set_time": (C.c_int, [C.c_void_p, C.c_double]),
     "stg_get_nsteps": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64)]),
+    "stg_set_nsteps": (C.c_int, [C.c_void_p, C.c_int32, C.c_uint64]),
     "stg_get_update_period": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_get_extents": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_uint64),
                                   C.POINTER(C.c_uint64)]),
@@ -177,6 +178,11 @@ SIGNATURES = {
     "stg_efield_get_verts_v": (C.c_int, [C.c_void_p, _dp]),
     "stg_efield_get_tris_v": (C.c_int, [C.c_void_p, _dp]),
     "stg_efield_set_ohmic": (C.c_int, [C.c_void_p, C.c_int32, _ip, _dp, _dp]),
+    "stg_efield_set_tri_ohmic_erev": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
+    "stg_efield_get_tri_ohmic_erev": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, _dp]),
+    "stg_efield_set_tri_capac": (C.c_int, [C.c_void_p, C.c_int32, C.c_double]),
+    "stg_efield_get_tri_capac": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
+    "stg_efield_get_tri_ghk_currents": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "stg_efield_attach": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32]),
     "stg_efield_advance": (C.c_int, [C.c_void_p, C.c_double]),
     "stg_efield_run": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_double]),

@@ -147,11 +147,13 @@ struct OhmicK {
 
 // The open-channel count is the time average over the last span seconds
 // (integral / span, Tri::computeI) when intg is given, else the present count.
+// GHK charges: current g of EField triangle t at charge[g * charge_stride + tri_map[t]].
+// oc_erev: [oc.n][n_tris] reversal potential per triangle, or null for oc.erev.
 __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, const int *tri_v,
                                  const double *V, const double *tri_clamp, const double *vert_clamp,
                                  const int *tri_counts, const double *intg, double span,
-                                 const int *charge, const int *tri_map, int n_tris_sim, OhmicK oc,
-                                 double *I) {
+                                 const int *charge, int n_ghk, int charge_stride, const int *tri_map,
+                                 int n_tris_sim, OhmicK oc, const double *oc_erev, int n_tris, double *I) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double cur = vert_clamp[i];
@@ -159,14 +161,19 @@ __global__ void k_vertex_current(int n, const int *vt_ptr, const int *vt_tri, co
         const int t = vt_tri[e];
         double c = tri_clamp[t];
         // GHK charge moved out over the span, as a mean current (Tri::computeI)
-        if (charge != nullptr) c -= (double)charge[tri_map[t]] * 1.602176487e-19 / span * 1e12;
+        if (charge != nullptr) {
+            int q = 0;
+            for (int g = 0; g < n_ghk; ++g) q += charge[(size_t)g * charge_stride + tri_map[t]];
+            c -= (double)q * 1.602176487e-19 / span * 1e12;
+        }
         if (oc.n > 0 && tri_counts != nullptr) {
             const double vt = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0;
             double im = 0.0;
             for (int k = 0; k < oc.n; ++k) {
                 const size_t ix = (size_t)oc.sspec[k] * n_tris_sim + tri_map[t];
                 const double nopen = intg ? intg[ix] / span : (double)tri_counts[ix];
-                im += nopen * oc.g[k] * (vt * 1e-3 - oc.erev[k]);
+                const double erev = oc_erev ? oc_erev[(size_t)k * n_tris + t] : oc.erev[k];
+                im += nopen * oc.g[k] * (vt * 1e-3 - erev);
             }
             c -= im * 1e12;
         }
@@ -619,9 +626,16 @@ __global__ void k_tri_v(int nt, const int *tri_v, const double *V, const int *tr
     if (t < nt) out[tri_map[t]] = (V[tri_v[3 * t]] + V[tri_v[3 * t + 1]] + V[tri_v[3 * t + 2]]) / 3.0 * 1e-3;
 }
 
-__global__ void k_zero_charge(int nt, const int *tri_map, int *charge) {
+// Move the GHK charges of each current into last ([n_ghk][nt], when given) and
+// clear them for the next step (Tri::resetECharge, tri.cpp:449-468).
+__global__ void k_take_charge(int nt, const int *tri_map, int n_ghk, int stride, int *charge, int *last) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nt) charge[tri_map[t]] = 0;
+    if (t >= nt) return;
+    for (int g = 0; g < n_ghk; ++g) {
+        int *c = charge + (size_t)g * stride + tri_map[t];
+        if (last) last[(size_t)g * nt + t] = *c;
+        *c = 0;
+    }
 }
 
 // restart the time integrals of the channel species on this EField's triangles
@@ -700,11 +714,15 @@ struct AmgDev {
     }
 };
 
+constexpr int kEfMaxGhk = 4;  // GHK currents a simulation can carry
+
 struct stg_efield {
     int n = 0, n_tris = 0, nnz = 0, width = 0;
     std::vector<int> tri_v;                 // [n_tris*3]
     std::vector<double> vert_area, tri_area, cap_h, gext_h;
     std::vector<double> vert_clamp_h, tri_clamp_h;
+    std::vector<double> tri_cm_h;           // [n_tris] specific capacitance of each triangle
+    std::vector<double> oc_erev_h;          // [8][n_tris] per-triangle reversal potentials, empty = oc.erev
     std::vector<unsigned char> clamped_h;
     double sigma = 1.0 / 1.0e-3;            // nS/um, 1 ohm.m
     double vext = 0.0;
@@ -728,6 +746,9 @@ struct stg_efield {
     double *intg = nullptr;                 // attached simulation's channel-count integrals
     uint32_t intg_mask = 0;
     int32_t *charge = nullptr;              // attached simulation's GHK charges (this realization)
+    int32_t n_ghk = 0, charge_stride = 0;   // GHK currents; distance between their charge rows
+    int32_t *ghk_last = nullptr;            // [n_ghk][n_tris] charges of the last step
+    double ghk_dt = 0.0;                    // its length in seconds, 0 before the first step
     double intg_t0 = 0.0;                   // simulation time the integrals started from
     std::string err;
     hipStream_t stream = nullptr;
@@ -735,7 +756,7 @@ struct stg_efield {
     int *col = nullptr, *vt_ptr = nullptr, *vt_tri = nullptr, *tri_vd = nullptr;
     int *tri_map = nullptr;  // EField triangle -> the attached simulation's triangle slot
     double *w = nullptr, *wsum = nullptr, *d64 = nullptr, *V = nullptr, *I = nullptr, *b = nullptr;
-    double *cap = nullptr, *gext = nullptr, *vclamp = nullptr, *tclamp = nullptr;
+    double *cap = nullptr, *gext = nullptr, *vclamp = nullptr, *tclamp = nullptr, *oc_erev = nullptr;
     double *x64 = nullptr, *r64 = nullptr, *dv_prev = nullptr;
     float *a32 = nullptr, *d32 = nullptr, *m32 = nullptr;
     unsigned char *clamped = nullptr;
@@ -753,7 +774,7 @@ struct stg_efield {
         amg.release();
         void *ptrs[] = {col, vt_ptr, vt_tri, tri_vd, tri_map, w, wsum, d64, V, I, b, cap, gext, vclamp,
                         tclamp, x64, r64, dv_prev, a32, d32, m32, clamped, xs, rs, zs, ps, qs, part,
-                        scal, outs, cnt};
+                        scal, outs, cnt, oc_erev, ghk_last};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -767,7 +788,8 @@ extern "C" int stg_internal_tri_perm(const stg_sim *sim, int32_t real, const int
                                      int32_t *n);
 extern "C" int stg_internal_tri_v_device(stg_sim *sim, int32_t real, double **ptr);
 extern "C" int stg_internal_tri_intg_device(stg_sim *sim, uint32_t mask, double **ptr);
-extern "C" int stg_internal_tri_charge_device(stg_sim *sim, int32_t real, int32_t **ptr);
+extern "C" int stg_internal_tri_charge_device(stg_sim *sim, int32_t real, int32_t **ptr, int32_t *n_ghk,
+                                              int32_t *stride);
 
 namespace {
 
@@ -800,6 +822,10 @@ void upload_params(stg_efield *e) {
     ef_put(e->vclamp, e->vert_clamp_h);
     ef_put(e->tclamp, e->tri_clamp_h);
     ef_put(e->clamped, e->clamped_h);
+    if (!e->oc_erev_h.empty()) {
+        if (!e->oc_erev) e->oc_erev = ef_alloc<double>(e->oc_erev_h.size());
+        ef_put(e->oc_erev, e->oc_erev_h);
+    }
     e->a32_dirty = true;
 }
 
@@ -919,6 +945,7 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     }
     e->cap_h.resize(n);
     for (int i = 0; i < n; ++i) e->cap_h[i] = e->cm * e->vert_area[i];
+    e->tri_cm_h.assign(m->n_tris, e->cm);
     e->gext_h.assign(n, 0.0);
     e->vert_clamp_h.assign(n, 0.0);
     e->tri_clamp_h.assign(std::max(m->n_tris, 1), 0.0);
@@ -1548,14 +1575,17 @@ void start_integrals(stg_efield *e) {
     e->intg = nullptr;
     e->intg_mask = 0;
     e->charge = nullptr;
+    e->n_ghk = 0;
+    e->ghk_dt = 0.0;  // no step yet: every GHK current reads 0 (Tri::getGHKI)
     if (!e->sim) return;
     int rc0 = stg_get_time(e->sim, &e->intg_t0);
     if (rc0 != STG_OK) throw EfError{rc0, stg_last_error(e->sim)};
-    rc0 = stg_internal_tri_charge_device(e->sim, e->sim_real, &e->charge);
+    rc0 = stg_internal_tri_charge_device(e->sim, e->sim_real, &e->charge, &e->n_ghk, &e->charge_stride);
     if (rc0 != STG_OK) throw EfError{rc0, stg_last_error(e->sim)};
     if (e->charge && e->n_tris > 0) {
-        hipLaunchKernelGGL(k_zero_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
-                           e->tri_map, e->charge);
+        if (!e->ghk_last) e->ghk_last = ef_alloc<int32_t>((size_t)kEfMaxGhk * e->n_tris);
+        hipLaunchKernelGGL(k_take_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream, e->n_tris,
+                           e->tri_map, e->n_ghk, e->charge_stride, e->charge, nullptr);
         EF_HIP(hipGetLastError());
         EF_HIP(hipStreamSynchronize(e->stream));
     }
@@ -1600,15 +1630,20 @@ void advance_impl(stg_efield *e, double dt_s) {
         }
     }
     hipLaunchKernelGGL(k_vertex_current, dim3(nb), dim3(kB), 0, e->stream, n, e->vt_ptr, e->vt_tri,
-                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, intg, span, charge, e->tri_map,
-                       ntot, e->oc, e->I);
+                       e->tri_vd, e->V, e->tclamp, e->vclamp, tri_counts, intg, span, charge, e->n_ghk,
+                       e->charge_stride, e->tri_map, ntot, e->oc, e->oc_erev_h.empty() ? nullptr : e->oc_erev,
+                       e->n_tris, e->I);
     if (e->sim) {
         if (e->intg)
             hipLaunchKernelGGL(k_zero_intg, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream,
                                e->n_tris, e->tri_map, ntot, e->intg_mask, e->intg + tri0);
-        if (e->charge)
-            hipLaunchKernelGGL(k_zero_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream,
-                               e->n_tris, e->tri_map, e->charge);
+        if (e->charge && e->n_tris > 0) {
+            // the step's charges become the currents the getters report
+            hipLaunchKernelGGL(k_take_charge, dim3((e->n_tris + kB - 1) / kB), dim3(kB), 0, e->stream,
+                               e->n_tris, e->tri_map, e->n_ghk, e->charge_stride, e->charge,
+                               span > 0.0 ? e->ghk_last : nullptr);
+            if (span > 0.0) e->ghk_dt = span;
+        }
         e->intg_t0 = now;
     }
     const double oodt = 1.0 / (dt_s * 1e3);
@@ -1730,6 +1765,7 @@ int stg_efield_set_capac(stg_efield *e, double cm) {
         if (!(cm >= 0.0)) throw EfError{STG_ERR_ARG, "capacitance must be >= 0"};
         e->cm = cm;  // F/m^2 = pF/um^2 (efield.cpp:151-160)
         for (int i = 0; i < e->n; ++i) e->cap_h[i] = cm * e->vert_area[i];
+        std::fill(e->tri_cm_h.begin(), e->tri_cm_h.end(), cm);  // TetMesh::applySurfaceCapacitance
         upload_params(e);
     });
 }
@@ -1769,6 +1805,75 @@ int stg_efield_set_tri_iclamp(stg_efield *e, int32_t t, double amps) {
         check_t(e, t);
         e->tri_clamp_h[t] = amps * 1e12;
         upload_params(e);
+    });
+}
+
+int stg_efield_set_tri_capac(stg_efield *e, int32_t t, double cm) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_t(e, t);
+        if (!(cm >= 0.0)) throw EfError{STG_ERR_ARG, "capacitance must be >= 0"};
+        // TetMesh::applyTriCapacitance: each vertex's share of the area changes
+        // by (cm - previous cm) * area / 3
+        const double d = (cm - e->tri_cm_h[t]) * e->tri_area[t] / 3.0;
+        e->tri_cm_h[t] = cm;
+        for (int a = 0; a < 3; ++a) e->cap_h[e->tri_v[3 * t + a]] += d;
+        upload_params(e);
+    });
+}
+
+int stg_efield_get_tri_capac(stg_efield *e, int32_t t, double *cm) {
+    if (!e || !cm) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_t(e, t);
+        *cm = e->tri_cm_h[t];
+    });
+}
+
+int stg_efield_set_tri_ohmic_erev(stg_efield *e, int32_t k, int32_t t, double erev) {
+    if (!e) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_t(e, t);
+        if (k < 0 || k >= e->oc.n) throw EfError{STG_ERR_ARG, "ohmic current index out of range"};
+        if (e->oc_erev_h.empty()) {
+            e->oc_erev_h.resize((size_t)8 * e->n_tris);
+            for (int j = 0; j < 8; ++j)
+                std::fill(e->oc_erev_h.begin() + (size_t)j * e->n_tris,
+                          e->oc_erev_h.begin() + (size_t)(j + 1) * e->n_tris, j < e->oc.n ? e->oc.erev[j] : 0.0);
+        }
+        e->oc_erev_h[(size_t)k * e->n_tris + t] = erev;
+        upload_params(e);
+    });
+}
+
+int stg_efield_get_tri_ohmic_erev(stg_efield *e, int32_t k, int32_t t, double *erev) {
+    if (!e || !erev) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        check_t(e, t);
+        if (k < 0 || k >= e->oc.n) throw EfError{STG_ERR_ARG, "ohmic current index out of range"};
+        *erev = e->oc_erev_h.empty() ? e->oc.erev[k] : e->oc_erev_h[(size_t)k * e->n_tris + t];
+    });
+}
+
+int stg_efield_get_tri_ghk_currents(stg_efield *e, int32_t g, double *amps) {
+    if (!e || !amps) return STG_ERR_ARG;
+    return ef_guard(e, [&] {
+        if (g < -1 || g >= std::max(e->n_ghk, 1)) throw EfError{STG_ERR_ARG, "GHK current index out of range"};
+        const int nt = e->n_tris;
+        if (e->ghk_dt == 0.0 || e->n_ghk == 0 || !e->ghk_last) {
+            std::fill(amps, amps + nt, 0.0);
+            return;
+        }
+        std::vector<int32_t> q((size_t)e->n_ghk * nt);
+        EF_HIP(hipStreamSynchronize(e->stream));
+        EF_HIP(hipMemcpy(q.data(), e->ghk_last, q.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int t = 0; t < nt; ++t) {
+            int c = 0;  // Tri::getGHKI sums the charges before converting
+            if (g >= 0) c = q[(size_t)g * nt + t];
+            else
+                for (int k = 0; k < e->n_ghk; ++k) c += q[(size_t)k * nt + t];
+            amps[t] = ((double)c * 1.602176487e-19) / e->ghk_dt;
+        }
     });
 }
 
@@ -1816,6 +1921,7 @@ int stg_efield_set_ohmic(stg_efield *e, int32_t n, const int32_t *sspec, const d
     return ef_guard(e, [&] {
         if (n < 0 || n > 8) throw EfError{STG_ERR_ARG, "between 0 and 8 ohmic currents"};
         e->oc.n = n;
+        e->oc_erev_h.clear();  // every triangle starts from the current's own reversal potential
         for (int k = 0; k < n; ++k) {
             if (sspec[k] < 0 || sspec[k] >= 8) throw EfError{STG_ERR_ARG, "bad channel species index"};
             e->oc.sspec[k] = sspec[k];
@@ -1876,44 +1982,66 @@ int stg_efield_run(stg_efield *e, stg_sim *sim, double endtime, double efdt) {
 
 // ------------------------------------------------------------ checkpoint
 // Own format: header, vertex potentials and the previous step's increment
-// (the CG starting guess), the membrane and clamp parameters, and the time
-// the present channel-count integrals started from.
+// (the CG starting guess), the membrane and clamp parameters, the time the
+// present channel-count integrals started from, and the GHK charges of the
+// last step with its length (Tri::checkpoint keeps pECharge_last and its dt,
+// tri.cpp:133-157).
 namespace {
-constexpr uint64_t kEfStateMagic = 0x3153464547544ull;  // "TGEFS1"
+constexpr uint64_t kEfStateMagic = 0x3253464547544ull;  // "TGEFS2"
 struct EfStateHeader {
     uint64_t magic;
     int32_t n, n_tris;
     double intg_t0, sigma, vext, cm;
+    int32_t has_oc_erev, n_ghk;
+    double ghk_dt;
 };
-size_t ef_state_bytes(const stg_efield *e) {
-    return sizeof(EfStateHeader) + sizeof(double) * (5 * (size_t)e->n + (size_t)e->n_tris) + (size_t)e->n;
+std::vector<std::vector<double> *> ef_state_vectors(stg_efield *e) {
+    std::vector<std::vector<double> *> v{&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h, &e->tri_cm_h};
+    if (!e->oc_erev_h.empty()) v.push_back(&e->oc_erev_h);
+    return v;
 }
+size_t ef_state_bytes(stg_efield *e, bool oc_erev, int n_ghk) {
+    size_t b = sizeof(EfStateHeader) + 2 * sizeof(double) * (size_t)e->n;
+    for (auto *v : {&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h, &e->tri_cm_h})
+        b += v->size() * sizeof(double);
+    if (oc_erev) b += (size_t)8 * e->n_tris * sizeof(double);
+    return b + e->clamped_h.size() + (size_t)n_ghk * e->n_tris * sizeof(int32_t);
+}
+int ef_saved_ghk(const stg_efield *e) { return e->ghk_last ? e->n_ghk : 0; }
 }  // namespace
 
 int stg_efield_state_size(const stg_efield *e, size_t *bytes) {
     if (!e || !bytes) return STG_ERR_ARG;
-    *bytes = ef_state_bytes(e);
+    auto *m = const_cast<stg_efield *>(e);
+    *bytes = ef_state_bytes(m, !e->oc_erev_h.empty(), ef_saved_ghk(e));
     return STG_OK;
 }
 
 int stg_efield_checkpoint(stg_efield *e, void *buf, size_t bytes) {
     if (!e || !buf) return STG_ERR_ARG;
     return ef_guard(e, [&] {
-        if (bytes < ef_state_bytes(e)) throw EfError{STG_ERR_ARG, "checkpoint buffer too small"};
+        const int ng = ef_saved_ghk(e);
+        if (bytes < ef_state_bytes(e, !e->oc_erev_h.empty(), ng))
+            throw EfError{STG_ERR_ARG, "checkpoint buffer too small"};
         char *p = static_cast<char *>(buf);
-        const EfStateHeader h{kEfStateMagic, e->n, e->n_tris, e->intg_t0, e->sigma, e->vext, e->cm};
+        const EfStateHeader h{kEfStateMagic, e->n,  e->n_tris, e->intg_t0, e->sigma, e->vext, e->cm,
+                              e->oc_erev_h.empty() ? 0 : 1, ng, e->ghk_dt};
         std::memcpy(p, &h, sizeof(h));
         p += sizeof(h);
         const size_t nb = (size_t)e->n * sizeof(double);
+        EF_HIP(hipStreamSynchronize(e->stream));
         EF_HIP(hipMemcpy(p, e->V, nb, hipMemcpyDeviceToHost));
         p += nb;
         EF_HIP(hipMemcpy(p, e->dv_prev, nb, hipMemcpyDeviceToHost));
         p += nb;
-        for (const auto *v : {&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h}) {
+        for (const auto *v : ef_state_vectors(e)) {
             std::memcpy(p, v->data(), v->size() * sizeof(double));
             p += v->size() * sizeof(double);
         }
         std::memcpy(p, e->clamped_h.data(), e->clamped_h.size());
+        p += e->clamped_h.size();
+        if (ng > 0)
+            EF_HIP(hipMemcpy(p, e->ghk_last, (size_t)ng * e->n_tris * sizeof(int32_t), hipMemcpyDeviceToHost));
     });
 }
 
@@ -1927,18 +2055,27 @@ int stg_efield_restore(stg_efield *e, const void *buf, size_t bytes) {
         if (h.magic != kEfStateMagic) throw EfError{STG_ERR_ARG, "not an EField checkpoint"};
         if (h.n != e->n || h.n_tris != e->n_tris)
             throw EfError{STG_ERR_ARG, "checkpoint was written by an EField of another mesh"};
-        if (bytes < ef_state_bytes(e)) throw EfError{STG_ERR_ARG, "checkpoint too short"};
+        if (h.n_ghk != 0 && (h.n_ghk != e->n_ghk || !e->ghk_last))
+            throw EfError{STG_ERR_ARG, "checkpoint holds GHK currents the attached simulation lacks"};
+        if (bytes < ef_state_bytes(e, h.has_oc_erev != 0, h.n_ghk)) throw EfError{STG_ERR_ARG, "checkpoint too short"};
         p += sizeof(h);
         const size_t nb = (size_t)e->n * sizeof(double);
+        EF_HIP(hipStreamSynchronize(e->stream));
         EF_HIP(hipMemcpy(e->V, p, nb, hipMemcpyHostToDevice));
         p += nb;
         EF_HIP(hipMemcpy(e->dv_prev, p, nb, hipMemcpyHostToDevice));
         p += nb;
-        for (auto *v : {&e->cap_h, &e->gext_h, &e->vert_clamp_h, &e->tri_clamp_h}) {
+        if (h.has_oc_erev) e->oc_erev_h.resize((size_t)8 * e->n_tris);
+        else e->oc_erev_h.clear();
+        for (auto *v : ef_state_vectors(e)) {
             std::memcpy(v->data(), p, v->size() * sizeof(double));
             p += v->size() * sizeof(double);
         }
         std::memcpy(e->clamped_h.data(), p, e->clamped_h.size());
+        p += e->clamped_h.size();
+        if (h.n_ghk > 0)
+            EF_HIP(hipMemcpy(e->ghk_last, p, (size_t)h.n_ghk * e->n_tris * sizeof(int32_t), hipMemcpyHostToDevice));
+        e->ghk_dt = h.n_ghk > 0 ? h.ghk_dt : 0.0;
         e->intg_t0 = h.intg_t0;
         e->sigma = h.sigma;
         e->vext = h.vext;

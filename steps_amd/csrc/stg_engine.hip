@@ -153,7 +153,7 @@ struct stg_sim {
     unsigned long long *d_ext = nullptr;
     double *d_tri_v = nullptr;
     double *d_tri_intg = nullptr;  // [SS][NTRI] time integrals of channel counts, lazily allocated
-    int32_t *d_tri_charge = nullptr;  // [NTRI] GHK charge, when the model has GHK currents
+    int32_t *d_tri_charge = nullptr;  // [NG][NTRI] GHK charge per current, when the model has GHK currents
     uint32_t *d_tet_bdir = nullptr;   // [NT] when there are diffusion boundaries
     double *d_bnd_D = nullptr;        // [NBD][S]
     int4 *d_tri_nbr = nullptr;        // surface diffusion tables
@@ -852,8 +852,8 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     HIP_OK(hipMemset(s->d_tri_counts, 0, sizeof(int32_t) * (size_t)std::max(SS, 1) * std::max(NTRI, 1)));
     HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
     if (s->NG > 0) {
-        s->d_tri_charge = dalloc<int32_t>(std::max(NTRI, 1));
-        HIP_OK(hipMemset(s->d_tri_charge, 0, sizeof(int32_t) * std::max(NTRI, 1)));
+        s->d_tri_charge = dalloc<int32_t>((size_t)s->NG * std::max(NTRI, 1));
+        HIP_OK(hipMemset(s->d_tri_charge, 0, sizeof(int32_t) * (size_t)s->NG * std::max(NTRI, 1)));
     }
     s->d_tri_v = dalloc<double>(std::max(NTRI, 1));
     {
@@ -1409,7 +1409,7 @@ int stg_reset(stg_sim *s) {
         if (s->d_tri_intg)  // Tri::reset clears the ohmic-current integrals (tri.cpp:441)
             HIP_OK(hipMemsetAsync(s->d_tri_intg, 0, sizeof(double) * (size_t)s->SS * s->NTRI, s->stream));
         if (s->d_tri_charge)  // ... and the GHK charges (tri.cpp:435)
-            HIP_OK(hipMemsetAsync(s->d_tri_charge, 0, sizeof(int32_t) * (size_t)s->NTRI, s->stream));
+            HIP_OK(hipMemsetAsync(s->d_tri_charge, 0, sizeof(int32_t) * (size_t)s->NG * s->NTRI, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         s->time = 0.0;
         // the Philox period counters (rng_steps) keep running
@@ -1459,6 +1459,14 @@ int stg_get_nsteps(const stg_sim *s, int32_t real, uint64_t *n) {
     return guard(const_cast<stg_sim *>(s), [&] {
         check_real(s, real);
         *n = s->nsteps[real];
+    });
+}
+
+int stg_set_nsteps(stg_sim *s, int32_t real, uint64_t n) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        check_real(s, real);
+        s->nsteps[real] = n;
     });
 }
 
@@ -1959,7 +1967,7 @@ size_t state_bytes(const stg_sim *s) {
            sizeof(int32_t) * ((size_t)s->S * s->NT + (size_t)s->SS * s->NTRI) +
            sizeof(double) * (size_t)s->NTRI +
            (s->d_tri_intg ? sizeof(double) * (size_t)s->SS * s->NTRI : 0) +
-           (s->d_tri_charge ? sizeof(int32_t) * (size_t)s->NTRI : 0);
+           (s->d_tri_charge ? sizeof(int32_t) * (size_t)s->NG * s->NTRI : 0);
 }
 }  // namespace
 
@@ -2007,7 +2015,8 @@ int stg_checkpoint(stg_sim *s, void *buf, size_t bytes) {
             HIP_OK(hipMemcpy(p, s->d_tri_intg, (size_t)s->SS * s->NTRI * sizeof(double), hipMemcpyDeviceToHost));
             p += (size_t)s->SS * s->NTRI * sizeof(double);
         }
-        if (s->d_tri_charge) HIP_OK(hipMemcpy(p, s->d_tri_charge, (size_t)s->NTRI * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (s->d_tri_charge)
+            HIP_OK(hipMemcpy(p, s->d_tri_charge, (size_t)s->NG * s->NTRI * sizeof(int32_t), hipMemcpyDeviceToHost));
     });
 }
 
@@ -2066,7 +2075,8 @@ int stg_restore(stg_sim *s, const void *buf, size_t bytes) {
         } else if (s->d_tri_intg) {
             HIP_OK(hipMemset(s->d_tri_intg, 0, (size_t)s->SS * s->NTRI * sizeof(double)));
         }
-        if (want_charge) HIP_OK(hipMemcpy(s->d_tri_charge, p, (size_t)s->NTRI * sizeof(int32_t), hipMemcpyHostToDevice));
+        if (want_charge)
+            HIP_OK(hipMemcpy(s->d_tri_charge, p, (size_t)s->NG * s->NTRI * sizeof(int32_t), hipMemcpyHostToDevice));
         if (!ext.empty())
             HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
         HIP_OK(hipMemcpy(s->d_real_seed, s->seeds.data(), sizeof(uint64_t) * s->n_real, hipMemcpyHostToDevice));
@@ -2140,14 +2150,16 @@ int stg_get_temp(const stg_sim *s, double *kelvin) {
     return STG_OK;
 }
 
-// Device pointer of the GHK charge of each triangle of one realization
-// (internal order), null when the model has no GHK currents; for the EField
-// (not part of the public header).
-int stg_internal_tri_charge_device(stg_sim *s, int32_t real, int32_t **ptr) {
-    if (!s || !ptr) return STG_ERR_ARG;
+// Device pointer of the GHK charge of each current on each triangle of one
+// realization (internal order; current g at ptr[g * stride + i]), null when
+// the model has no GHK currents; for the EField (not part of the public header).
+int stg_internal_tri_charge_device(stg_sim *s, int32_t real, int32_t **ptr, int32_t *n_ghk, int32_t *stride) {
+    if (!s || !ptr || !n_ghk || !stride) return STG_ERR_ARG;
     return guard(s, [&] {
         check_real(s, real);
         *ptr = s->d_tri_charge ? s->d_tri_charge + s->tri_off[real] : nullptr;
+        *n_ghk = s->d_tri_charge ? s->NG : 0;
+        *stride = s->NTRI;
     });
 }
 

@@ -155,7 +155,7 @@ struct SysK {
     const int32_t *vdep_off;     // [n_vdep+1]
     const double *vdep_k;        // concatenated k(V) tables
     double *tri_intg;            // [SS][NTRI] time integral of counts (tri_intg_mask species)
-    int32_t *tri_charge;         // [NTRI] GHK charge moved out (elementary charges), or null
+    int32_t *tri_charge;         // [n_ghk][NTRI] GHK charge moved out per current (elementary charges), or null
     const uint32_t *tet_bdir;    // [NT] 4 x 8-bit diffusion-boundary direction per face
                                  // (0xff = none), or null
     const double *bnd_D;         // [directions][n_specs] diffusion constant across a boundary
@@ -507,7 +507,7 @@ __device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r,
                 L.c(ion) += dv;
                 if ((m.tracked_mask >> ion) & 1) L.o(ion) -= (double)dv * tcur;
             }
-            sys.tri_charge[tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
+            sys.tri_charge[(size_t)q * sys.n_tris + tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
         } else {
             const int i = (sel - R) / RT;
             const int qs = (sel - R) - i * RT;
@@ -756,7 +756,7 @@ __device__ long ssa_memb_lean(const ModelK &m, const SysK &sys, const Lds &L, in
                 L.c(ion) += dv;
                 if ((m.tracked_mask >> ion) & 1) L.o(ion) -= (double)dv * tcur;
             }
-            sys.tri_charge[tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
+            sys.tri_charge[(size_t)q * sys.n_tris + tp0 + i] += efflux ? m.g_z[q] : -m.g_z[q];
         } else {
             const int i = (sel - R) / RT;
             const int qs = (sel - R) - i * RT;
@@ -1184,7 +1184,7 @@ __device__ void group_period(const ModelK &m, const SysK &sys, int g, int launch
                         if ((m.tracked_mask >> ion) & 1) sys.gcorr[(size_t)ion * NT + o] += (double)dv * tcur;
                     }
                 }
-                sys.tri_charge[stri] += efflux ? m.g_z[gq] : -m.g_z[gq];
+                sys.tri_charge[(size_t)gq * sys.n_tris + stri] += efflux ? m.g_z[gq] : -m.g_z[gq];
             } else {
                 // surface reaction: triangle, inner and outer tet counts
                 // (SReac::apply, sreac.cpp:440-505)

@@ -85,6 +85,29 @@ class EField:
         self._chk(self.lib.stg_efield_set_ohmic(self.h, len(s), N.ptr(s, C.c_int32),
                                                 N.ptr(gg, C.c_double), N.ptr(e, C.c_double)))
 
+    def set_tri_ohmic_erev(self, k, t, erev):
+        self._chk(self.lib.stg_efield_set_tri_ohmic_erev(self.h, int(k), int(t), float(erev)))
+
+    def tri_ohmic_erev(self, k, t) -> float:
+        x = C.c_double()
+        self._chk(self.lib.stg_efield_get_tri_ohmic_erev(self.h, int(k), int(t), C.byref(x)))
+        return x.value
+
+    def set_tri_capac(self, t, cm):
+        self._chk(self.lib.stg_efield_set_tri_capac(self.h, int(t), float(cm)))
+
+    def tri_capac(self, t) -> float:
+        x = C.c_double()
+        self._chk(self.lib.stg_efield_get_tri_capac(self.h, int(t), C.byref(x)))
+        return x.value
+
+    def tri_ghk_currents(self, g=-1) -> np.ndarray:
+        """Per membrane triangle, the mean current (A, outward positive) GHK
+        current g (every current for -1) carried over the last step."""
+        a = np.empty(self.n_tris)
+        self._chk(self.lib.stg_efield_get_tri_ghk_currents(self.h, int(g), N.ptr(a, C.c_double)))
+        return a
+
     def attach(self, engine, real=0):
         self._chk(self.lib.stg_efield_attach(self.h, engine.h if engine else None, int(real)))
         self._engine = engine

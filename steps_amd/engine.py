@@ -154,6 +154,9 @@ class Engine:
         self._chk(self.lib.stg_get_nsteps(self.h, real, C.byref(n)))
         return n.value
 
+    def set_nsteps(self, n: int, real: int = 0):
+        self._chk(self.lib.stg_set_nsteps(self.h, real, int(n)))
+
     def update_period(self, real: int = 0) -> float:
         d = C.c_double()
         self._chk(self.lib.stg_get_update_period(self.h, real, C.byref(d)))

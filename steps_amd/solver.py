@@ -1,20 +1,31 @@
 """Solver classes with the interface of the reference's operator-split solvers.
 
 ``TetOpSplit`` mirrors ``_py_TetOpSplitP`` (pysteps/cysteps_mpi.pyx:74-1546,
-C++ TetOpSplitP in src/steps/mpi/tetopsplit/tetopsplit.hpp) and
+C++ TetOpSplitP in src/steps/mpi/tetopsplit/tetopsplit.hpp) together with the
+``_py_API`` / ``_py_TetAPI`` methods it inherits (pysteps/cysteps_solver.pyx);
 ``DistTetOpSplit`` mirrors ``_py_DistTetOpSplitP`` (pysteps/cysteps_dist.pyx:
 1574-2932). Both run on the HIP engine behind include/stg.h with the whole mesh
 resident on one GPU; the host partition arguments are accepted and ignored.
 Method names, argument meaning and error types follow the reference: invalid
-arguments raise ``ArgErr`` (a ValueError), unsupported features ``NotImplErr``.
+arguments raise ``ArgErr`` (a ValueError), unsupported features ``NotImplErr``;
+every reference method this engine does not provide exists and raises
+``NotImplErr`` (REFERENCE_METHODS below).
+
+Per-element constants, activation flags and clamps follow the reference's
+per-kproc state (steps_amd/elements.py) and reach the engine as per-element
+tables; until an element-level setter is used the engine runs on
+compartment-wide constants.
 """
 from __future__ import annotations
+
+import time
 
 import numpy as np
 
 from . import _native as N
+from .elements import TriElements, VolElements
 from .engine import Engine
-from .geom import Tetmesh, comps_mesh_tables, mesh_tables
+from .geom import ELEM_TET, ELEM_TRI, Tetmesh, comps_mesh_tables, mesh_tables
 from .model import Model, flatten, flatten_comps
 
 EF_NONE, EF_DEFAULT, EF_DV_BDSYS, EF_DV_PETSC = 0, 1, 2, 3  # src/steps/solver/api.hpp:61-66
@@ -57,6 +68,25 @@ def distribute_counts(n: float, vol: np.ndarray, rng: np.random.Generator) -> np
     return out.astype(np.int32)
 
 
+def _falling_h(counts: np.ndarray, lhs: np.ndarray) -> np.ndarray:
+    """Combinatorial factor per element: the product over reactants of
+    n (n-1) ... (n - lhs + 1), 0 when a reactant is short (Reac::rate,
+    src/steps/mpi/tetopsplit/reac.cpp; SReac::rate, sreac.cpp:305-430).
+    counts: [species][elements]."""
+    h = np.ones(counts.shape[1])
+    for s in np.nonzero(lhs)[0]:
+        n = counts[s].astype(np.float64)
+        for i in range(int(lhs[s])):
+            h = h * (n - i)
+        h = np.where(counts[s] < lhs[s], 0.0, h)
+    return h
+
+
+def _round_count(n: float, rng) -> int:
+    """Stochastic rounding of a molecule count (Tet::setCount callers)."""
+    return int(np.floor(n) + (1 if rng.random() < n - np.floor(n) else 0))
+
+
 class TetOpSplit:
     """Operator-split stochastic reaction-diffusion on one GPU (STEPS 'TetOpSplit')."""
 
@@ -72,8 +102,11 @@ class TetOpSplit:
             # every EField solver choice maps to the GPU preconditioned CG
             self._setup_efield(geom)
 
-    # --- construction ---
+    # ------------------------------------------------------------ construction
     def _setup(self, model, geom, rng):
+        """TetOpSplitP::_setup (tetopsplit.cpp:800-1075): compartments,
+        patches (each with its inner and optional outer compartment),
+        diffusion boundaries."""
         if not isinstance(geom, Tetmesh):
             raise _argerr("geometry must be a Tetmesh")
         self.model, self.geom, self.rng = model, geom, rng
@@ -81,18 +114,18 @@ class TetOpSplit:
         if not self.comps:
             raise _argerr("the geometry has no compartment")
         self._ci = {c.name: i for i, c in enumerate(self.comps)}
-        patches = [p for p in geom.patches.values() if p.surfsys]
-        if len(patches) > 1:
-            raise _notimpl("at most one patch with surface systems is supported")
-        self.patch = patches[0] if patches else None
-        if self.patch is not None and self.patch.icomp.name not in self._ci:
-            raise _argerr("the patch's inner compartment is not in the geometry")
-        self._preal = self._ci[self.patch.icomp.name] if self.patch is not None else 0
-        ptris = self.patch.tris if self.patch is not None else None
-        ssys = self.patch.surfsys if self.patch else ()
+        self.patches = [p for p in geom.patches.values() if p.surfsys]
+        self._pi = {p.name: i for i, p in enumerate(self.patches)}
+        for p in self.patches:
+            if p.icomp.name not in self._ci or (p.ocomp is not None and p.ocomp.name not in self._ci):
+                raise _argerr(f"a compartment of patch {p.name!r} is not in the geometry")
         nc = len(self.comps)
         self.diffbs = list(geom.diffbs.values())
-        if nc == 1:
+        self.patch = self.patches[0] if len(self.patches) == 1 else None
+        simple = nc == 1 and len(self.patches) <= 1 and all(p.ocomp is None for p in self.patches)
+        if simple:
+            ssys = self.patches[0].surfsys if self.patches else ()
+            ptris = self.patches[0].tris if self.patches else None
             self.fm = flatten(model, self.comps[0].volsys, ssys)
             self.mts = [mesh_tables(geom, self.comps[0], ptris)]
             self._bdirs = []
@@ -102,34 +135,73 @@ class TetOpSplit:
             self._ck = self.fm.reac_k.copy()[None, :]
             self._cd = self.fm.spec_dcst.copy()[None, :]
         else:
-            ct = flatten_comps(model, [c.volsys for c in self.comps], ssys)
+            pl = [(p.surfsys, self._ci[p.icomp.name], None if p.ocomp is None else self._ci[p.ocomp.name])
+                  for p in self.patches]
+            ct = flatten_comps(model, [c.volsys for c in self.comps], pl)
             self.fm = ct.fm
             self.mts, self._bdirs = comps_mesh_tables(
-                geom, self.comps, {self._preal: ptris} if ptris is not None else {}, self.diffbs)
+                geom, self.comps, [(p.tris, pl[i][1], pl[i][2]) for i, p in enumerate(self.patches)],
+                self.diffbs)
             self._comp_reacs, self._comp_diffs = ct.comp_reacs, ct.comp_diffs
             self._comp_specs = ct.comp_specs
             self._ck, self._cd = ct.comp_k.copy(), ct.comp_dcst.copy()
-        if self.patch is not None:
-            self._comp_specs[self._preal] |= {self.fm.specs[j] for j in
-                                              np.nonzero(np.abs(self.fm.sreac_lhs_i).sum(0)
-                                                         + np.abs(self.fm.sreac_upd_i).sum(0))[0]}
+        self._simple = simple
+        fm = self.fm
+        RS, NG = len(fm.sreacs), len(fm.ghk_chan)
+        # a patch's surface reactions and GHK currents define their volume
+        # species in the inner (and outer) compartment (Patchdef::setup_indices)
+        for pi, p in enumerate(self.patches):
+            inner, outer = set(), set()
+            for j in fm.patch_sreacs[pi]:
+                inner |= {fm.specs[x] for x in np.nonzero(fm.sreac_lhs_i[j] | fm.sreac_upd_i[j])[0]}
+                outer |= {fm.specs[x] for x in np.nonzero(fm.sreac_lhs_o[j] | fm.sreac_upd_o[j])[0]}
+            for g in fm.patch_ghk[pi]:
+                inner.add(fm.specs[fm.ghk_ion[g]])
+                if fm.ghk_voconc[g] < 0:
+                    outer.add(fm.specs[fm.ghk_ion[g]])
+            self._comp_specs[self._ci[p.icomp.name]] |= inner
+            if p.ocomp is not None:
+                self._comp_specs[self._ci[p.ocomp.name]] |= outer
         self.comp = self.comps[0]
-        self.mt = self.mts[self._preal]
         # global tet -> (compartment, local index)
         self._tet_real = -np.ones(geom.countTets(), dtype=np.int64)
         self._tet_local = -np.ones(geom.countTets(), dtype=np.int64)
         for ci, c in enumerate(self.comps):
             self._tet_real[c.tets] = ci
             self._tet_local[c.tets] = np.arange(len(c.tets))
+        # global triangle -> (compartment holding it, index there, patch); a
+        # compartment's triangles are those of its patches in patch order
+        self._tri_real = -np.ones(geom.countTris(), dtype=np.int64)
         self._tri_local = -np.ones(geom.countTris(), dtype=np.int64)
-        if self.patch is not None:
-            self._tri_local[self.patch.tris] = np.arange(len(self.patch.tris))
+        self._tri_patch = -np.ones(geom.countTris(), dtype=np.int64)
+        fill = np.zeros(nc, dtype=np.int64)
+        for pi, p in enumerate(self.patches):
+            r = self._ci[p.icomp.name]
+            self._tri_real[p.tris] = r
+            self._tri_local[p.tris] = fill[r] + np.arange(len(p.tris))
+            self._tri_patch[p.tris] = pi
+            fill[r] += len(p.tris)
+        self._rtri_patch = [self._tri_patch[self._real_tris(r)] for r in range(nc)]
+        self.mt = self.mts[self._ci[self.patch.icomp.name]] if self.patch is not None else self.mts[0]
+        # patch-level surface constants and switches
+        self._psk = np.zeros((len(self.patches), RS))
+        self._psact = np.ones((len(self.patches), RS + NG), dtype=bool)
+        for pi in range(len(self.patches)):
+            for j in fm.patch_sreacs[pi]:
+                self._psk[pi, j] = fm.sreac_k[j]
+        self._sd = fm.sspec_dcst.copy()
         self._seed = int(getattr(rng, "seed", 0))
-        if nc == 1:
-            self.eng = Engine(self.fm, [self.mt], [self._seed], mode=self._mode)
+        if simple:
+            self.eng = Engine(fm, [self.mts[0]], [self._seed], mode=self._mode)
         else:
-            self.eng = Engine(self.fm, self.mts, [self._seed] * nc, streams=np.arange(nc),
-                              reac_k=self._ck, spec_dcst=self._cd, mode=self._mode, comps=True)
+            sk = np.zeros((nc, RS))
+            for r in range(nc):
+                pats = np.unique(self._rtri_patch[r])
+                if len(pats) == 1:
+                    sk[r] = self._psk[pats[0]]
+            self.eng = Engine(fm, self.mts, [self._seed] * nc, streams=np.arange(nc),
+                              reac_k=self._ck, sreac_k=sk, spec_dcst=self._cd, mode=self._mode,
+                              comps=True)
         # Large meshes get the period kernel compiled for this model (identical
         # results, about a second to compile); without run-time compilation
         # the generic kernel stays.
@@ -138,11 +210,37 @@ class TetOpSplit:
                 self.eng.set_specialize(True)
             except N.StgError:
                 pass
+        # compartment-level switches (Reac / Diff active flags of every tet)
+        self._cract = np.ones((nc, len(fm.reacs)), dtype=bool)
+        self._cdact = np.ones((nc, len(fm.specs)), dtype=bool)
         # diffusion boundaries: open directions and directional constants
         self._db_active = set()        # (boundary, species, source comp)
         self._db_dcst = {}             # (boundary, species, source comp) -> constant
+        # per-element state, created on first use
+        self._vel = [None] * nc
+        self._tel = [None] * nc
+        for r in range(nc):
+            if self._needs_tri_table(r):
+                self._tri_elements(r)
+                self._push_tri(r)
         self._host_rng = np.random.default_rng([self._seed, 0x5EED])
         self.ef = None
+        self._efreal = None
+        self._rd_time = 0.0
+        self._ef_time = 0.0
+
+    def _real_tris(self, r):
+        """Global indices of compartment r's triangles in the engine's order."""
+        tris = np.nonzero(self._tri_real == r)[0]
+        return tris[np.argsort(self._tri_local[tris])]
+
+    def _needs_tri_table(self, r) -> bool:
+        """Triangles of several patches in one compartment, or a GHK current
+        its patch lacks, need per-triangle constants."""
+        pats = np.unique(self._rtri_patch[r])
+        if len(pats) > 1:
+            return True
+        return len(pats) == 1 and len(self.fm.patch_ghk[pats[0]]) < len(self.fm.ghk_chan)
 
     def _setup_efield(self, geom):
         """Membrane potential on the membrane's conduction volume
@@ -154,11 +252,22 @@ class TetOpSplit:
             raise _argerr("the membrane potential solve needs exactly one Memb (got %d)"
                           % len(membs))
         memb = self.memb = membs[0]
-        ptris = self.patch.tris if self.patch is not None else np.zeros(0, np.int64)
-        if len(ptris) and set(ptris.tolist()) != set(memb.tris.tolist()):
-            raise _notimpl("the membrane must consist of the patch that holds the surface systems")
-        # EField triangle k is the solver's patch triangle k
-        tris = ptris if len(ptris) else memb.tris
+        held = [p for p in memb.patches if p.surfsys]
+        tris = memb.tris
+        if held:
+            inner = {p.icomp.name for p in held}
+            if len(inner) != 1:
+                raise _notimpl("the patches of the membrane must share one inner compartment")
+            r = self._ci[inner.pop()]
+            rt = self._real_tris(r)
+            if set(rt.tolist()) != set(memb.tris.tolist()):
+                raise _notimpl("the membrane must consist of the patches that hold the surface "
+                               "systems of its inner compartment")
+            ohm = [frozenset(oc for s in p.surfsys for oc in self.model.surfsys[s].ohmiccurrs) for p in held]
+            if len(set(ohm)) > 1:
+                raise _notimpl("every patch of the membrane must hold the same ohmic currents")
+            tris = rt                   # EField triangle k is the compartment's triangle k
+            self._efreal = r
         tets = geom.tets[memb.tets]
         vids, local = np.unique(tets, return_inverse=True)
         self._ef_vert_local = -np.ones(geom.countVertices(), dtype=np.int64)
@@ -174,14 +283,15 @@ class TetOpSplit:
         self._vert_iclamp = np.zeros(len(vids))
         self._tri_iclamp = np.zeros(len(tris))
         self._vert_clamped = np.zeros(len(vids), dtype=bool)
+        self._memb_res = (0.0, 0.0)
         if len(self.fm.ohmic):
             if len(self.fm.ohmic) > 8:
                 raise _notimpl("at most 8 ohmic currents are supported")
             self.ef.set_ohmic(self.fm.ohmic_chan, self.fm.ohmic_g, self.fm.ohmic_erev)
-        if self.patch is not None:
-            self.ef.attach(self.eng, self._preal)
+        if self._efreal is not None:
+            self.ef.attach(self.eng, self._efreal)
 
-    # --- identity ---
+    # ---------------------------------------------------------------- identity
     def getSolverName(self):
         return self._name
 
@@ -194,14 +304,15 @@ class TetOpSplit:
     def getSolverEmail(self):
         return ""
 
-    # --- time stepping ---
+    # ----------------------------------------------------------- time stepping
     def reset(self):
         """TetOpSplitP::reset (tetopsplit.cpp:1436-1503): counts, time, step
         count and extents to zero and, with an EField, the membrane potential
         back to DEFAULT_MEMB_POT. The random streams keep running, as the
         reference's RNG does, so reset-and-run loops give independent runs;
         re-initialising the RNG with a new seed before reset() re-keys every
-        compartment and restarts the streams from that seed."""
+        compartment and restarts the streams from that seed. Constants,
+        activation flags and clamps stay as set."""
         seed = int(getattr(self.rng, "seed", self._seed))
         if seed != self._seed:
             self._seed = seed
@@ -211,14 +322,22 @@ class TetOpSplit:
         self.eng.reset()
         if self.ef is not None:
             self.ef.set_potential(DEFAULT_MEMB_POT)
-            if self.patch is not None:
-                self.ef.attach(self.eng, self._preal)   # channel integrals restart at t = 0
+            if self._efreal is not None:
+                self.ef.attach(self.eng, self._efreal)   # channel integrals restart at t = 0
+
+    def repartitionAndReset(self, tet_hosts=(), tri_hosts=None, wm_hosts=()):
+        """TetOpSplitP::repartitionAndReset: the whole mesh lives on one GPU, so
+        the partition is ignored and the simulation is reset."""
+        self.reset()
 
     def run(self, endtime: float):
+        tic = time.perf_counter()
         if self.ef is not None:
             self.ef.run(self.eng, endtime, self._efdt)   # _runWithEField
+            self._ef_time += time.perf_counter() - tic
         else:
             self.eng.run(endtime)
+            self._rd_time += time.perf_counter() - tic
 
     def advance(self, adv: float):
         if adv < 0.0:
@@ -237,6 +356,13 @@ class TetOpSplit:
     def getNSteps(self) -> int:
         return self.eng.nsteps()
 
+    def setNSteps(self, nsteps: int):
+        """TetOpSplitP::setNSteps (tetopsplit.cpp:1995)."""
+        if nsteps < 0:
+            raise _argerr("the step count must be >= 0")
+        for r in range(self.eng.n_real):
+            self.eng.set_nsteps(int(nsteps), r)
+
     getNIteration = getNSteps
 
     def getUpdPeriod(self) -> float:
@@ -248,10 +374,36 @@ class TetOpSplit:
     def getDiffExtent(self, local: bool = False) -> int:
         return sum(self.eng.extents(r)[1] for r in range(self.eng.n_real))
 
+    def getNReacExtent(self, local: bool = False) -> int:
+        return self.getReacExtent(local)
+
+    def getNDiffExtent(self, local: bool = False) -> int:
+        return self.getDiffExtent(local)
+
     def setDiffApplyThreshold(self, threshold: int):
         self.eng.set_diff_apply_threshold(threshold)
 
-    # --- lookups ---
+    # Wall-clock accounting (TetOpSplitP::getCompTime and the like). One GPU
+    # holds the whole mesh: there is no synchronisation, idle or exchange time.
+    def getCompTime(self) -> float:
+        return self._rd_time + self._ef_time
+
+    def getRDTime(self) -> float:
+        return self._rd_time
+
+    def getEFieldTime(self) -> float:
+        return self._ef_time
+
+    def getSyncTime(self) -> float:
+        return 0.0
+
+    def getIdleTime(self) -> float:
+        return 0.0
+
+    def getDataExchangeTime(self) -> float:
+        return 0.0
+
+    # ------------------------------------------------------------------ lookups
     def _spec(self, s: str) -> int:
         try:
             return self.fm.spec_index(s)
@@ -275,9 +427,16 @@ class TetOpSplit:
             raise _argerr(f"species {s!r} is undefined in compartment {self.comps[ci].name!r}")
         return sp
 
-    def _check_patch(self, p: str):
-        if self.patch is None or p != self.patch.name:
+    def _check_patch(self, p: str) -> int:
+        if p not in self._pi:
             raise _argerr(f"unknown patch {p!r}")
+        return self._pi[p]
+
+    def _patch_sspec(self, pi: int, s: str) -> int:
+        q = self._sspec(s)
+        if s not in self.fm.patch_sspecs[pi]:
+            raise _argerr(f"species {s!r} is undefined in patch {self.patches[pi].name!r}")
+        return q
 
     def _tet(self, t: int):
         """(compartment, local index) of a mesh tetrahedron."""
@@ -285,12 +444,133 @@ class TetOpSplit:
             raise _argerr(f"tetrahedron {t} is not in a compartment")
         return int(self._tet_real[t]), int(self._tet_local[t])
 
-    def _tri(self, t: int) -> int:
+    def _tri(self, t: int):
+        """(compartment holding it, local index there, patch) of a patch triangle."""
         if t < 0 or t >= len(self._tri_local) or self._tri_local[t] < 0:
-            raise _argerr(f"triangle {t} is not in the patch")
-        return int(self._tri_local[t])
+            raise _argerr(f"triangle {t} is not in a patch")
+        return int(self._tri_real[t]), int(self._tri_local[t]), int(self._tri_patch[t])
 
-    # --- compartments ---
+    def _tri_sspec(self, tri: int, s: str):
+        r, i, pi = self._tri(tri)
+        return r, i, pi, self._patch_sspec(pi, s)
+
+    # ---------------------------------------------------- per-element state
+    def _vol_elements(self, ci: int) -> VolElements:
+        if self._vel[ci] is None:
+            mt = self.mts[ci]
+            nb = np.asarray(mt.nbr)
+            same = nb >= 0
+            if mt.nbr_comp is not None:
+                same &= np.asarray(mt.nbr_comp) == ci
+            bdir = np.asarray(mt.bdir) if mt.bdir is not None else -np.ones_like(nb)
+            self._vel[ci] = VolElements(len(mt.vol), len(self.fm.reacs), len(self.fm.specs),
+                                        same & (bdir < 0), bdir)
+        return self._vel[ci]
+
+    def _reac_table_on(self, ci: int) -> VolElements:
+        vel = self._vol_elements(ci)
+        vel.reac_table_on(self._ck[ci], self._cract[ci])
+        return vel
+
+    def _push_reac(self, ci: int):
+        vel = self._vel[ci]
+        if vel is not None and vel.k is not None:
+            self.eng.set_tet_reac_table(vel.reac_table(), real=ci)
+        else:
+            for j in range(len(self.fm.reacs)):
+                self.eng.set_reac_k(j, self._ck[ci, j] if self._cract[ci, j] else 0.0, real=ci)
+
+    def _diff_table_on(self, ci: int) -> VolElements:
+        vel = self._vol_elements(ci)
+        if vel.dD is None:
+            dirs = self._bdirs
+
+            def act(d, s):
+                return (dirs[d][0], s, ci) in self._db_active
+
+            def dcst(d, s):
+                return self._db_dcst.get((dirs[d][0], s, ci))
+
+            vel.diff_table_on(self._cd[ci], self._cdact[ci], act, dcst)
+        return vel
+
+    def _push_diff(self, ci: int):
+        vel = self._vel[ci]
+        if vel is not None and vel.dD is not None:
+            self.eng.set_tet_diff_table(vel.diff_table(), real=ci)
+
+    def _push_clamp(self, ci: int):
+        vel = self._vel[ci]
+        m = None if vel is None or vel.clamp is None or not vel.clamp.any() else vel.clamp
+        self.eng.set_tet_clamp(m, real=ci)
+
+    def _tri_elements(self, r: int) -> TriElements:
+        if self._tel[r] is None:
+            pats = self._rtri_patch[r]
+            m = len(pats)
+            RS, NG = len(self.fm.sreacs), len(self.fm.ghk_chan)
+            held = np.zeros((RS + NG, m), dtype=bool)
+            k = np.zeros((RS, m))
+            for pi in np.unique(pats):
+                sel = pats == pi
+                for j in self.fm.patch_sreacs[pi]:
+                    held[j, sel] = True
+                for g in self.fm.patch_ghk[pi]:
+                    held[RS + g, sel] = True
+                k[:, sel] = self._psk[pi][:, None]
+            tel = TriElements(m, held, self.fm.sreac_vdep >= 0, k, len(self.fm.sspecs))
+            for pi in np.unique(pats):
+                tel.act[:, pats == pi] = self._psact[pi][:, None]
+            self._tel[r] = tel
+        return self._tel[r]
+
+    def _push_tri(self, r: int):
+        tel = self._tel[r]
+        if tel is not None:
+            self.eng.set_tri_sreac_table(tel.sreac_table(), real=r)
+        else:
+            pats = np.unique(self._rtri_patch[r])
+            if len(pats) == 1:
+                pi = pats[0]
+                for j in range(len(self.fm.sreacs)):
+                    self.eng.set_sreac_k(j, self._psk[pi, j] if self._psact[pi, j] else 0.0, real=r)
+
+    def _push_tri_clamp(self, r: int):
+        tel = self._tel[r]
+        m = None if tel is None or tel.clamp is None or not tel.clamp.any() else tel.clamp
+        self.eng.set_tri_clamp(m, real=r)
+
+    def _sdiff_table_on(self, r: int) -> TriElements:
+        tel = self._tri_elements(r)
+        if tel.eD is None:
+            g = self.mts[r].tri_edge_g
+            tel.eD = np.where(g[None, :, :] > 0, self._sd[:, None, None], 0.0)
+        return tel
+
+    def _push_sdiff(self, r: int):
+        tel = self._tel[r]
+        if tel is not None and tel.eD is not None:
+            self.eng.set_tri_sdiff_table(tel.eD, real=r)
+            self._push_tri(r)
+
+    # ------------------------------------------------------------- compartments
+    def getNComps(self) -> int:
+        return len(self.comps)
+
+    def getCompName(self, c_idx: int) -> str:
+        if not 0 <= c_idx < len(self.comps):
+            raise _argerr("compartment index out of range")
+        return self.comps[c_idx].name
+
+    def getNCompSpecs(self, c: str) -> int:
+        return len(self._comp_specs[self._check_comp(c)])
+
+    def getCompSpecName(self, c: str, s_idx: int) -> str:
+        names = [s for s in self.fm.specs if s in self._comp_specs[self._check_comp(c)]]
+        if not 0 <= s_idx < len(names):
+            raise _argerr("species index out of range")
+        return names[s_idx]
+
     def getCompVol(self, c: str) -> float:
         return float(self.mts[self._check_comp(c)].vol.sum())
 
@@ -300,8 +580,9 @@ class TetOpSplit:
 
     def setCompSpecCount(self, c: str, s: str, n: float):
         ci = self._check_comp(c)
-        self.eng.set_tet_counts(self._comp_spec(ci, s),
-                                distribute_counts(n, self.mts[ci].vol, self._host_rng), real=ci)
+        sp = self._comp_spec(ci, s)
+        new = distribute_counts(n, self.mts[ci].vol, self._host_rng)
+        self.eng.set_tet_counts(sp, new, real=ci)
 
     def getCompSpecAmount(self, c: str, s: str) -> float:
         return self.getCompSpecCount(c, s) / AVOGADRO
@@ -315,6 +596,22 @@ class TetOpSplit:
     def setCompSpecConc(self, c: str, s: str, conc: float):
         self.setCompSpecCount(c, s, conc * 1e3 * self.getCompVol(c) * AVOGADRO)
 
+    def getCompSpecClamped(self, c: str, s: str) -> bool:
+        """True when the species is clamped in every tet of the compartment
+        (_getCompSpecClamped)."""
+        ci = self._check_comp(c)
+        sp = self._comp_spec(ci, s)
+        vel = self._vel[ci]
+        return bool(vel is not None and vel.clamped(sp, np.arange(vel.n)).all())
+
+    def setCompSpecClamped(self, c: str, s: str, b: bool):
+        """_setCompSpecClamped (tetopsplit.cpp:2189): every tet of the compartment."""
+        ci = self._check_comp(c)
+        sp = self._comp_spec(ci, s)
+        vel = self._vol_elements(ci)
+        vel.set_clamped(sp, np.arange(vel.n), bool(b))
+        self._push_clamp(ci)
+
     def _reac(self, ci: int, r: str) -> int:
         if r not in self._comp_reacs[ci]:
             raise _argerr(f"unknown reaction {r!r} in compartment {self.comps[ci].name!r}")
@@ -325,10 +622,69 @@ class TetOpSplit:
         return float(self._ck[ci, self._reac(ci, r)])
 
     def setCompReacK(self, c: str, r: str, k: float):
+        """_setCompReacK: the constant of the reaction in every tet."""
         ci = self._check_comp(c)
         j = self._reac(ci, r)
-        self.eng.set_reac_k(j, k, real=ci)
+        if k < 0:
+            raise _argerr("rate constant must be >= 0")
         self._ck[ci, j] = k
+        vel = self._vel[ci]
+        if vel is not None and vel.k is not None:
+            vel.k[j, :] = k
+        self._push_reac(ci)
+
+    def getCompReacActive(self, c: str, r: str) -> bool:
+        """True unless the reaction is inactive in every tet (_getCompReacActive)."""
+        ci = self._check_comp(c)
+        j = self._reac(ci, r)
+        vel = self._vel[ci]
+        if vel is not None and vel.kact is not None:
+            return bool(vel.kact[j].any())
+        return bool(self._cract[ci, j])
+
+    def setCompReacActive(self, c: str, r: str, a: bool):
+        """_setCompReacActive (tetopsplit.cpp:2309): the flag of every tet."""
+        ci = self._check_comp(c)
+        j = self._reac(ci, r)
+        self._cract[ci, j] = bool(a)
+        vel = self._vel[ci]
+        if vel is not None and vel.kact is not None:
+            vel.kact[j, :] = bool(a)
+        self._push_reac(ci)
+
+    def _reac_hc(self, ci: int, j: int, tets=None):
+        """(h, c) per tet of compartment ci (or the given local tets) for
+        reaction j at the present counts (Reac::rate, Reac::resetCcst)."""
+        vel = self._vel[ci]
+        idx = np.arange(len(self.mts[ci].vol)) if tets is None else np.asarray(tets)
+        lhs = self.fm.reac_lhs[j]
+        counts = np.zeros((len(lhs), len(idx)), dtype=np.int64)
+        for s in np.nonzero(lhs)[0]:
+            counts[s] = self.eng.batch_tet_counts(int(s), idx, real=ci)
+        h = _falling_h(counts, lhs)
+        if vel is not None and vel.k is not None:
+            k = vel.reac_table()[j, idx]
+        else:
+            k = np.full(len(idx), self._ck[ci, j] if self._cract[ci, j] else 0.0)
+        order = int(lhs.sum())
+        vscale = 1e3 * self.mts[ci].vol[idx] * AVOGADRO
+        return h, k * vscale ** (1 - order)
+
+    def getCompReacH(self, c: str, r: str) -> float:
+        ci = self._check_comp(c)
+        return float(self._reac_hc(ci, self._reac(ci, r))[0].sum())
+
+    def getCompReacC(self, c: str, r: str) -> float:
+        """Volume-weighted mean of the tets' c (_getCompReacC)."""
+        ci = self._check_comp(c)
+        _, cc = self._reac_hc(ci, self._reac(ci, r))
+        v = self.mts[ci].vol
+        return float((cc * v).sum() / v.sum())
+
+    def getCompReacA(self, c: str, r: str) -> float:
+        ci = self._check_comp(c)
+        h, cc = self._reac_hc(ci, self._reac(ci, r))
+        return float((h * cc).sum())
 
     def _diff(self, ci: int, d: str) -> int:
         if d not in self._comp_diffs[ci]:
@@ -339,20 +695,45 @@ class TetOpSplit:
         ci = self._check_comp(c)
         return float(self._cd[ci, self._diff(ci, d)])
 
-    def setCompDiffD(self, c: str, d: str, dcst: float, direction_comp=None):
+    def setCompDiffD(self, c: str, d: str, dcst: float):
+        """_setCompDiffD: Diff::setDcst on every tet, which also clears the
+        rule's directional constants (diff.cpp:340-390)."""
         ci = self._check_comp(c)
         s = self._diff(ci, d)
         if dcst < 0:
             raise _argerr("diffusion constant must be >= 0")
-        self.eng.set_dcst(s, dcst, real=ci)
         self._cd[ci, s] = dcst
-        # Diff::setDcst also resets the rule's boundary directions to the new
-        # constant (diff.cpp:340-390)
         for key in [k for k in self._db_dcst if k[1] == s and k[2] == ci]:
             del self._db_dcst[key]
+        vel = self._vel[ci]
+        if vel is not None and vel.dD is not None:
+            vel.set_dcst(s, slice(None), dcst)
+            self._push_diff(ci)
+        self.eng.set_dcst(s, dcst if self._cdact[ci, s] else 0.0, real=ci)
         self._push_boundaries(s)
 
-    # --- diffusion boundaries (tetopsplit.cpp:2745-2931) ---
+    def getCompDiffActive(self, c: str, d: str) -> bool:
+        ci = self._check_comp(c)
+        s = self._diff(ci, d)
+        vel = self._vel[ci]
+        if vel is not None and vel.dAct is not None:
+            return bool(vel.dAct[s].any())
+        return bool(self._cdact[ci, s])
+
+    def setCompDiffActive(self, c: str, d: str, act: bool):
+        """_setCompDiffActive (tetopsplit.cpp:2437): the flag of every tet; an
+        inactive rule moves nothing, across diffusion boundaries either."""
+        ci = self._check_comp(c)
+        s = self._diff(ci, d)
+        self._cdact[ci, s] = bool(act)
+        vel = self._vel[ci]
+        if vel is not None and vel.dAct is not None:
+            vel.dAct[s, :] = bool(act)
+            self._push_diff(ci)
+        self.eng.set_dcst(s, self._cd[ci, s] if act else 0.0, real=ci)
+        self._push_boundaries(s)
+
+    # ----------------------------------------- diffusion boundaries (2745-2931)
     def _db(self, db: str) -> int:
         for i, b in enumerate(self.diffbs):
             if b.id == db:
@@ -368,27 +749,37 @@ class TetOpSplit:
         return sp
 
     def _push_boundaries(self, sp: int):
-        """Boundary constants of species sp from the activation state: a
-        direction from compartment c uses c's diffusion constant for the
-        species, or the directional constant set for it; 0 when inactive or
-        when c has no diffusion rule for it (no Diff kproc there)."""
+        """Realization-level boundary constants of species sp: a direction from
+        compartment c carries c's diffusion constant for the species, or the
+        directional constant set for it; 0 when the boundary is closed for it,
+        when c has no diffusion rule for it (no Diff kproc there) or the rule is
+        inactive. Compartments on per-face tables carry their own values."""
         for d, (bi, src, _dst) in enumerate(self._bdirs):
             has_rule = sp in self._comp_diffs[src].values()
-            on = (bi, sp, src) in self._db_active and has_rule
+            on = (bi, sp, src) in self._db_active and has_rule and self._cdact[src, sp]
             val = self._db_dcst.get((bi, sp, src), self._cd[src, sp]) if on else 0.0
             self.eng.set_boundary_dcst(d, sp, val)
+
+    def _db_dirs(self, bi: int, src: int):
+        return [d for d, (b, s_, _) in enumerate(self._bdirs) if b == bi and s_ == src]
 
     def setDiffBoundarySpecDiffusionActive(self, db: str, s: str, act: bool):
         bi = self._db(db)
         sp = self._db_spec(bi, s)
         b = self.diffbs[bi]
         for c in (b.compA, b.compB):
-            key = (bi, sp, self._ci[c.name])
+            ci = self._ci[c.name]
+            key = (bi, sp, ci)
             if act:
                 self._db_active.add(key)
             else:
                 self._db_active.discard(key)
             self._db_dcst.pop(key, None)    # setDiffBndActive -> setDcst clears overrides
+            vel = self._vel[ci]
+            if vel is not None and vel.dD is not None:
+                for d in self._db_dirs(bi, ci):
+                    vel.set_bnd_active(sp, d, bool(act))
+                self._push_diff(ci)
         self._push_boundaries(sp)
 
     def getDiffBoundarySpecDiffusionActive(self, db: str, s: str) -> bool:
@@ -399,30 +790,52 @@ class TetOpSplit:
         return all((bi, sp, self._ci[c.name]) in self._db_active for c in (b.compA, b.compB))
 
     def setDiffBoundarySpecDcst(self, db: str, s: str, dcst: float, direction_comp=None):
-        """Activate the boundary for s with constant dcst from both sides, or
-        only into direction_comp (tetopsplit.cpp:2860-2917)."""
+        """Open the boundary for s with constant dcst from both sides, or only
+        into direction_comp (tetopsplit.cpp:2860-2917)."""
         bi = self._db(db)
         sp = self._db_spec(bi, s)
         if dcst < 0:
             raise _argerr("diffusion constant must be >= 0")
         b = self.diffbs[bi]
         dest = None
-        if direction_comp is not None:
+        if direction_comp not in (None, ""):
             name = direction_comp if isinstance(direction_comp, str) else direction_comp.name
             if name not in (b.compA.name, b.compB.name):
                 raise _argerr("direction compartment is not on the diffusion boundary")
             dest = self._ci[name]
         for c in (b.compA, b.compB):
             src = self._ci[c.name]
-            if src != dest:
-                self._db_active.add((bi, sp, src))
-                self._db_dcst[(bi, sp, src)] = float(dcst)
+            if src == dest:
+                continue
+            self._db_active.add((bi, sp, src))
+            self._db_dcst[(bi, sp, src)] = float(dcst)
+            vel = self._vel[src]
+            if vel is not None and vel.dD is not None:
+                for d in self._db_dirs(bi, src):
+                    for t, f in zip(*np.nonzero(vel.bdir == d)):
+                        vel.set_direction_dcst(sp, t, f, float(dcst))
+                self._push_diff(src)
         self._push_boundaries(sp)
 
-    # --- tetrahedra ---
+    # deprecated spellings (cysteps_solver.pyx)
+    def setDiffBoundaryDiffusionActive(self, db: str, s: str, act: bool):
+        self.setDiffBoundarySpecDiffusionActive(db, s, act)
+
+    def getDiffBoundaryDiffusionActive(self, db: str, s: str) -> bool:
+        return self.getDiffBoundarySpecDiffusionActive(db, s)
+
+    def setDiffBoundaryDcst(self, db: str, s: str, dcst: float, direction_comp=""):
+        self.setDiffBoundarySpecDcst(db, s, dcst, direction_comp or None)
+
+    # -------------------------------------------------------------- tetrahedra
     def getTetVol(self, t: int) -> float:
         ci, i = self._tet(t)
         return float(self.mts[ci].vol[i])
+
+    def getTetSpecDefined(self, t: int, s: str) -> bool:
+        ci, _ = self._tet(t)
+        self._spec(s)
+        return s in self._comp_specs[ci]
 
     def getTetSpecCount(self, t: int, s: str) -> float:
         ci, i = self._tet(t)
@@ -434,8 +847,14 @@ class TetOpSplit:
         ci, i = self._tet(t)
         sp = self._comp_spec(ci, s)
         c = self.eng.get_tet_counts(sp, real=ci)
-        c[i] = int(np.floor(n) + (1 if self._host_rng.random() < n - np.floor(n) else 0))
+        c[i] = _round_count(n, self._host_rng)
         self.eng.set_tet_counts(sp, c, real=ci)
+
+    def getTetSpecAmount(self, t: int, s: str) -> float:
+        return self.getTetSpecCount(t, s) / AVOGADRO
+
+    def setTetSpecAmount(self, t: int, s: str, a: float):
+        self.setTetSpecCount(t, s, a * AVOGADRO)
 
     def getTetSpecConc(self, t: int, s: str) -> float:
         return self.getTetSpecCount(t, s) / (1e3 * self.getTetVol(t) * AVOGADRO)
@@ -443,38 +862,204 @@ class TetOpSplit:
     def setTetSpecConc(self, t: int, s: str, conc: float):
         self.setTetSpecCount(t, s, conc * 1e3 * self.getTetVol(t) * AVOGADRO)
 
+    def getTetSpecClamped(self, t: int, s: str) -> bool:
+        ci, i = self._tet(t)
+        sp = self._comp_spec(ci, s)
+        vel = self._vel[ci]
+        return bool(vel is not None and vel.clamped(sp, i))
+
+    def setTetSpecClamped(self, t: int, s: str, buf: bool):
+        """_setTetSpecClamped (tetopsplit.cpp:3951)."""
+        ci, i = self._tet(t)
+        sp = self._comp_spec(ci, s)
+        self._vol_elements(ci).set_clamped(sp, i, bool(buf))
+        self._push_clamp(ci)
+
+    def getTetReacK(self, t: int, r: str) -> float:
+        ci, i = self._tet(t)
+        j = self._reac(ci, r)
+        vel = self._vel[ci]
+        return float(vel.k[j, i]) if vel is not None and vel.k is not None else float(self._ck[ci, j])
+
+    def setTetReacK(self, t: int, r: str, kf: float):
+        """_setTetReacK (tetopsplit.cpp:3996)."""
+        ci, i = self._tet(t)
+        j = self._reac(ci, r)
+        if kf < 0:
+            raise _argerr("rate constant must be >= 0")
+        self._reac_table_on(ci).k[j, i] = kf
+        self._push_reac(ci)
+
+    def getTetReacActive(self, t: int, r: str) -> bool:
+        ci, i = self._tet(t)
+        j = self._reac(ci, r)
+        vel = self._vel[ci]
+        if vel is not None and vel.kact is not None:
+            return bool(vel.kact[j, i])
+        return bool(self._cract[ci, j])
+
+    def setTetReacActive(self, t: int, r: str, act: bool):
+        """_setTetReacActive (tetopsplit.cpp:4045)."""
+        ci, i = self._tet(t)
+        j = self._reac(ci, r)
+        self._reac_table_on(ci).kact[j, i] = bool(act)
+        self._push_reac(ci)
+
+    def _tet_face_toward(self, t: int, other: int) -> int:
+        f = np.nonzero(self.geom.tet_nbr[t] == other)[0]
+        if len(f) == 0:
+            raise _argerr(f"Tetrahedron {other} is not a neighbor of tetrahedron {t}.")
+        return int(f[0])
+
+    def getTetDiffD(self, t: int, d: str, direction_tet=None) -> float:
+        ci, i = self._tet(t)
+        s = self._diff(ci, d)
+        vel = self._vel[ci]
+        if vel is None or vel.dD is None:
+            if direction_tet is not None and direction_tet >= 0:
+                bi = self._tet_face_toward(t, int(direction_tet))
+                bd = self.mts[ci].bdir[i, bi] if self.mts[ci].bdir is not None else -1
+                if bd >= 0 and (self._bdirs[bd][0], s, ci) in self._db_dcst:
+                    return float(self._db_dcst[(self._bdirs[bd][0], s, ci)])
+            return float(self._cd[ci, s])
+        if direction_tet is None or direction_tet < 0:
+            return float(vel.dD[s, i])
+        v = vel.dDir[s, i, self._tet_face_toward(t, int(direction_tet))]
+        return float(vel.dD[s, i] if np.isnan(v) else v)
+
+    def setTetDiffD(self, t: int, d: str, dk: float, direction_tet=None):
+        """_setTetDiffD (tetopsplit.cpp:4103): the rule's constant in one tet,
+        or through the face toward direction_tet (Diff::setDirectionDcst)."""
+        ci, i = self._tet(t)
+        s = self._diff(ci, d)
+        if dk < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        vel = self._diff_table_on(ci)
+        if direction_tet is None or direction_tet < 0:
+            vel.set_dcst(s, i, float(dk))
+        else:
+            vel.set_direction_dcst(s, i, self._tet_face_toward(t, int(direction_tet)), float(dk))
+        self._push_diff(ci)
+
+    def getTetDiffActive(self, t: int, d: str) -> bool:
+        ci, i = self._tet(t)
+        s = self._diff(ci, d)
+        vel = self._vel[ci]
+        if vel is not None and vel.dAct is not None:
+            return bool(vel.dAct[s, i])
+        return bool(self._cdact[ci, s])
+
+    def setTetDiffActive(self, t: int, d: str, act: bool):
+        """_setTetDiffActive (tetopsplit.cpp:4166)."""
+        ci, i = self._tet(t)
+        s = self._diff(ci, d)
+        self._diff_table_on(ci).dAct[s, i] = bool(act)
+        self._push_diff(ci)
+
+    def getTetReacH(self, t: int, r: str) -> float:
+        ci, i = self._tet(t)
+        return float(self._reac_hc(ci, self._reac(ci, r), [i])[0][0])
+
+    def getTetReacC(self, t: int, r: str) -> float:
+        ci, i = self._tet(t)
+        return float(self._reac_hc(ci, self._reac(ci, r), [i])[1][0])
+
+    def getTetReacA(self, t: int, r: str) -> float:
+        ci, i = self._tet(t)
+        h, c = self._reac_hc(ci, self._reac(ci, r), [i])
+        return float(h[0] * c[0])
+
+    def getTetDiffA(self, t: int, d: str) -> float:
+        """Diff::rate (diff.cpp:458-470): the count times sum_f D_f A_f / (V d_f)."""
+        ci, i = self._tet(t)
+        s = self._diff(ci, d)
+        g = self.mts[ci].face_g[i]
+        vel = self._vel[ci]
+        if vel is not None and vel.dD is not None:
+            D = vel.diff_table()[s, i]
+        else:
+            D = np.full(4, self._cd[ci, s] if self._cdact[ci, s] else 0.0)
+            bd = self.mts[ci].bdir[i] if self.mts[ci].bdir is not None else -np.ones(4, dtype=int)
+            for f in range(4):
+                if bd[f] >= 0:
+                    D[f] = self.eng.boundary_dcst(int(bd[f]), s)
+        rate = float(((D[0] * g[0] + D[1] * g[1]) + D[2] * g[2]) + D[3] * g[3])
+        return rate * float(self.eng.batch_tet_counts(s, [i], real=ci)[0])
+
+    # batch access
+    def _tet_idx(self, index_array):
+        idx = np.asarray(index_array, dtype=np.int64)
+        if len(idx) and (np.any(idx < 0) or np.any(idx >= len(self._tet_local))
+                         or np.any(self._tet_local[idx] < 0)):
+            raise _argerr("tetrahedron index out of range")
+        return idx
+
     def getBatchTetSpecCounts(self, tets, s: str):
         c = np.zeros(len(tets))
         self.getBatchTetSpecCountsNP(np.asarray(tets, dtype=np.int64), s, c)
         return c.tolist()
 
     def getBatchTetSpecCountsNP(self, index_array, s: str, counts):
-        idx = np.asarray(index_array, dtype=np.int64)
+        idx = self._tet_idx(index_array)
         if len(counts) != len(idx):
             raise _argerr("counts and index arrays differ in length")
-        if len(idx) and (np.any(idx < 0) or np.any(idx >= len(self._tet_local))
-                         or np.any(self._tet_local[idx] < 0)):
-            raise _argerr("tetrahedron index out of range")
         sp = self._spec(s)
         real, loc = self._tet_real[idx], self._tet_local[idx]
         for ci in np.unique(real):
             m = real == ci
             counts[m] = self.eng.batch_tet_counts(sp, loc[m], real=int(ci))
 
+    def getBatchTetSpecConcs(self, tets, s: str):
+        c = np.zeros(len(tets))
+        self.getBatchTetSpecConcsNP(np.asarray(tets, dtype=np.int64), s, c)
+        return c.tolist()
+
     def getBatchTetSpecConcsNP(self, index_array, s: str, concs):
         self.getBatchTetSpecCountsNP(index_array, s, concs)
         idx = np.asarray(index_array, dtype=np.int64)
-        vol = np.array([self.mts[r].vol[i] for r, i in zip(self._tet_real[idx], self._tet_local[idx])])
+        vol = self.geom.tet_vol[idx]
         concs[:] = concs / (1e3 * vol * AVOGADRO)
 
-    def sumBatchTetCountsNP(self, tet_array, s: str) -> float:
+    def setBatchTetSpecCountsNP(self, index_array, s: str, counts):
+        """Counts of the listed tets, rounded stochastically
+        (TetOpSplitP::setBatchTetSpecCounts)."""
+        idx = self._tet_idx(index_array)
+        vals = np.asarray(counts, dtype=np.float64)
+        if len(vals) != len(idx):
+            raise _argerr("counts and index arrays differ in length")
+        if np.any(vals < 0):
+            raise _argerr("negative number of molecules")
+        sp = self._spec(s)
+        real, loc = self._tet_real[idx], self._tet_local[idx]
+        fl = np.floor(vals)
+        rounded = (fl + (self._host_rng.random(len(vals)) < vals - fl)).astype(np.int64)
+        for ci in np.unique(real):
+            m = real == ci
+            self._comp_spec(int(ci), s)
+            c = self.eng.get_tet_counts(sp, real=int(ci))
+            c[loc[m]] = rounded[m]
+            self.eng.set_tet_counts(sp, c, real=int(ci))
+
+    def setBatchTetSpecCounts(self, tets, s: str, counts):
+        self.setBatchTetSpecCountsNP(np.asarray(tets, dtype=np.int64), s, np.asarray(counts, dtype=float))
+
+    def setBatchTetSpecConcsNP(self, index_array, s: str, concs):
+        idx = self._tet_idx(index_array)
+        self.setBatchTetSpecCountsNP(idx, s, np.asarray(concs, dtype=float) * 1e3 * self.geom.tet_vol[idx]
+                                     * AVOGADRO)
+
+    def setBatchTetSpecConcs(self, tets, s: str, concs):
+        self.setBatchTetSpecConcsNP(np.asarray(tets, dtype=np.int64), s, concs)
+
+    def sumBatchTetSpecCountsNP(self, tet_array, s: str) -> float:
         c = np.zeros(len(tet_array))
         self.getBatchTetSpecCountsNP(tet_array, s, c)
         return float(c.sum())
 
-    # --- regions of interest (tetopsplit.cpp getROI*/setROI*) ---
+    sumBatchTetCountsNP = sumBatchTetSpecCountsNP
+
+    # -------------------------------------------------------- regions of interest
     def _roi(self, roi: str, want: int):
-        from .geom import ELEM_TET
         if roi not in self.geom.rois:
             raise _argerr(f"ROI {roi!r} does not exist")
         typ, idx = self.geom.rois[roi]
@@ -484,17 +1069,24 @@ class TetOpSplit:
         return idx
 
     def _roi_tets(self, roi: str):
-        from .geom import ELEM_TET
         idx = self._roi(roi, ELEM_TET)
         if np.any(self._tet_local[idx] < 0):
             raise _argerr(f"ROI {roi!r} holds a tetrahedron outside every compartment")
         return idx
 
+    def _roi_tris(self, roi: str):
+        idx = self._roi(roi, ELEM_TRI)
+        if np.any(self._tri_local[idx] < 0):
+            raise _argerr(f"ROI {roi!r} holds a triangle outside every patch")
+        return idx
+
+    def _roi_is_tri(self, roi: str) -> bool:
+        return roi in self.geom.rois and self.geom.rois[roi][0] == ELEM_TRI
+
     def getROIVol(self, roi: str) -> float:
         return float(self.geom.tet_vol[self._roi_tets(roi)].sum())
 
     def getROIArea(self, roi: str) -> float:
-        from .geom import ELEM_TRI
         return float(self.geom.tri_area[self._roi(roi, ELEM_TRI)].sum())
 
     def getROITetSpecCounts(self, roi: str, s: str):
@@ -507,21 +1099,17 @@ class TetOpSplit:
         self.getBatchTetSpecCountsNP(self._roi_tets(roi), s, counts)
 
     def getROITriSpecCounts(self, roi: str, s: str):
-        from .geom import ELEM_TRI
         idx = self._roi(roi, ELEM_TRI)
         c = np.zeros(len(idx))
         self.getBatchTriSpecCountsNP(idx, s, c)
         return c.tolist()
 
     def getROITriSpecCountsNP(self, roi: str, s: str, counts):
-        from .geom import ELEM_TRI
         self.getBatchTriSpecCountsNP(self._roi(roi, ELEM_TRI), s, counts)
 
     def getROISpecCount(self, roi: str, s: str) -> float:
         """Total over the ROI, reduced on the device per compartment."""
-        from .geom import ELEM_TET, ELEM_TRI
-        typ = self.geom.rois[roi][0] if roi in self.geom.rois else None
-        if typ == ELEM_TRI:
+        if self._roi_is_tri(roi):
             return float(sum(self.getROITriSpecCounts(roi, s)))
         idx = self._roi_tets(roi)
         sp = self._spec(s)
@@ -530,12 +1118,16 @@ class TetOpSplit:
                          for ci in np.unique(real)))
 
     def setROISpecCount(self, roi: str, s: str, n: float):
-        """Distribute n molecules over the ROI's tets by volume (setROISpecCount)."""
+        """Distribute n molecules over the ROI's tets by volume, or its
+        triangles by area (setROISpecCount)."""
+        if self._roi_is_tri(roi):
+            idx = self._roi_tris(roi)
+            self.setBatchTriSpecCountsNP(idx, s, distribute_counts(n, self.geom.tri_area[idx], self._host_rng))
+            return
         idx = self._roi_tets(roi)
         sp = self._spec(s)
         real, loc = self._tet_real[idx], self._tet_local[idx]
-        vol = self.geom.tet_vol[idx]
-        add = distribute_counts(n, vol, self._host_rng)
+        add = distribute_counts(n, self.geom.tet_vol[idx], self._host_rng)
         for ci in np.unique(real):
             m = real == ci
             c = self.eng.get_tet_counts(sp, real=int(ci))
@@ -554,25 +1146,296 @@ class TetOpSplit:
     def setROISpecConc(self, roi: str, s: str, conc: float):
         self.setROISpecCount(roi, s, conc * 1e3 * self.getROIVol(roi) * AVOGADRO)
 
-    # --- patches and triangles ---
+    def _roi_by_comp(self, roi: str):
+        idx = self._roi_tets(roi)
+        real, loc = self._tet_real[idx], self._tet_local[idx]
+        return [(int(ci), loc[real == ci]) for ci in np.unique(real)]
+
+    def setROISpecClamped(self, roi: str, s: str, clamped: bool):
+        """Clamp the species on every tet (or triangle) of the ROI."""
+        if self._roi_is_tri(roi):
+            for t in self._roi_tris(roi):
+                self.setTriSpecClamped(int(t), s, clamped)
+            return
+        for ci, loc in self._roi_by_comp(roi):
+            sp = self._comp_spec(ci, s)
+            self._vol_elements(ci).set_clamped(sp, loc, bool(clamped))
+            self._push_clamp(ci)
+
+    def setROIReacK(self, roi: str, r: str, kf: float):
+        if kf < 0:
+            raise _argerr("rate constant must be >= 0")
+        for ci, loc in self._roi_by_comp(roi):
+            j = self._reac(ci, r)
+            self._reac_table_on(ci).k[j, loc] = kf
+            self._push_reac(ci)
+
+    def setROIReacActive(self, roi: str, r: str, a: bool):
+        for ci, loc in self._roi_by_comp(roi):
+            j = self._reac(ci, r)
+            self._reac_table_on(ci).kact[j, loc] = bool(a)
+            self._push_reac(ci)
+
+    def setROIDiffD(self, roi: str, d: str, dcst: float):
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        for ci, loc in self._roi_by_comp(roi):
+            s = self._diff(ci, d)
+            self._diff_table_on(ci).set_dcst(s, loc, float(dcst))
+            self._push_diff(ci)
+
+    def setROIDiffActive(self, roi: str, d: str, act: bool):
+        for ci, loc in self._roi_by_comp(roi):
+            s = self._diff(ci, d)
+            self._diff_table_on(ci).dAct[s, loc] = bool(act)
+            self._push_diff(ci)
+
+    def setROISReacK(self, roi: str, sr: str, kf: float):
+        for t in self._roi_tris(roi):
+            self.setTriSReacK(int(t), sr, kf)
+
+    def setROISReacActive(self, roi: str, sr: str, a: bool):
+        for t in self._roi_tris(roi):
+            self.setTriSReacActive(int(t), sr, a)
+
+    def setROIVDepSReacActive(self, roi: str, vsr: str, a: bool):
+        for t in self._roi_tris(roi):
+            self.setTriVDepSReacActive(int(t), vsr, a)
+
+    # deprecated spellings (cysteps_solver.pyx)
+    def getROICount(self, roi: str, s: str) -> float:
+        return self.getROISpecCount(roi, s)
+
+    def setROICount(self, roi: str, s: str, n: float):
+        self.setROISpecCount(roi, s, n)
+
+    def getROIAmount(self, roi: str, s: str) -> float:
+        return self.getROISpecAmount(roi, s)
+
+    def setROIAmount(self, roi: str, s: str, a: float):
+        self.setROISpecAmount(roi, s, a)
+
+    def getROIConc(self, roi: str, s: str) -> float:
+        return self.getROISpecConc(roi, s)
+
+    def setROIConc(self, roi: str, s: str, c: float):
+        self.setROISpecConc(roi, s, c)
+
+    def setROIClamped(self, roi: str, s: str, b: bool):
+        self.setROISpecClamped(roi, s, b)
+
+    def getROITetCounts(self, roi: str, s: str):
+        return self.getROITetSpecCounts(roi, s)
+
+    def getROITetCountsNP(self, roi: str, s: str, counts):
+        self.getROITetSpecCountsNP(roi, s, counts)
+
+    def getROITriCounts(self, roi: str, s: str):
+        return self.getROITriSpecCounts(roi, s)
+
+    def getROITriCountsNP(self, roi: str, s: str, counts):
+        self.getROITriSpecCountsNP(roi, s, counts)
+
+    # ------------------------------------------------------ patches and triangles
+    def getNPatches(self) -> int:
+        return len(self.patches)
+
+    def getPatchName(self, p_idx: int) -> str:
+        if not 0 <= p_idx < len(self.patches):
+            raise _argerr("patch index out of range")
+        return self.patches[p_idx].name
+
+    def getNPatchSpecs(self, p: str) -> int:
+        return len(self.fm.patch_sspecs[self._check_patch(p)])
+
+    def getPatchSpecName(self, p: str, s_idx: int) -> str:
+        pi = self._check_patch(p)
+        names = [s for s in self.fm.sspecs if s in self.fm.patch_sspecs[pi]]
+        if not 0 <= s_idx < len(names):
+            raise _argerr("species index out of range")
+        return names[s_idx]
+
+    def getPatchArea(self, p: str) -> float:
+        return float(self.geom.tri_area[self.patches[self._check_patch(p)].tris].sum())
+
+    def _patch_local(self, pi: int):
+        """(compartment holding the patch's triangles, their local indices)."""
+        p = self.patches[pi]
+        return self._ci[p.icomp.name], self._tri_local[p.tris]
+
     def getPatchSpecCount(self, p: str, s: str) -> float:
-        self._check_patch(p)
-        return float(self.eng.sum_tri_counts(self._sspec(s), real=self._preal))
+        pi = self._check_patch(p)
+        q = self._patch_sspec(pi, s)
+        r, loc = self._patch_local(pi)
+        if len(self._rtri_patch[r]) == len(loc):
+            return float(self.eng.sum_tri_counts(q, real=r))
+        return float(self.eng.get_tri_counts(q, real=r)[loc].sum())
 
     def setPatchSpecCount(self, p: str, s: str, n: float):
-        self._check_patch(p)
-        self.eng.set_tri_counts(self._sspec(s),
-                                distribute_counts(n, self.mt.tri_area, self._host_rng), real=self._preal)
+        pi = self._check_patch(p)
+        q = self._patch_sspec(pi, s)
+        r, loc = self._patch_local(pi)
+        c = self.eng.get_tri_counts(q, real=r)
+        c[loc] = distribute_counts(n, self.geom.tri_area[self.patches[pi].tris], self._host_rng)
+        self.eng.set_tri_counts(q, c, real=r)
+
+    def getPatchSpecAmount(self, p: str, s: str) -> float:
+        return self.getPatchSpecCount(p, s) / AVOGADRO
+
+    def setPatchSpecAmount(self, p: str, s: str, a: float):
+        self.setPatchSpecCount(p, s, a * AVOGADRO)
+
+    def getPatchSpecClamped(self, p: str, s: str) -> bool:
+        pi = self._check_patch(p)
+        q = self._patch_sspec(pi, s)
+        r, loc = self._patch_local(pi)
+        tel = self._tel[r]
+        return bool(tel is not None and tel.clamped(q, loc).all())
+
+    def setPatchSpecClamped(self, p: str, s: str, b: bool):
+        """_setPatchSpecClamped: every triangle of the patch."""
+        pi = self._check_patch(p)
+        q = self._patch_sspec(pi, s)
+        r, loc = self._patch_local(pi)
+        self._tri_elements(r).set_clamped(q, loc, bool(b))
+        self._push_tri(r)
+        self._push_tri_clamp(r)
+
+    def _sreac(self, r, pi=None, vdep=None) -> int:
+        try:
+            j = self.fm.sreacs.index(r)
+        except ValueError:
+            raise _argerr(f"unknown surface reaction {r!r}") from None
+        if pi is not None and j not in self.fm.patch_sreacs[pi]:
+            raise _argerr(f"surface reaction {r!r} is undefined in patch {self.patches[pi].name!r}")
+        if vdep is not None and bool(self.fm.sreac_vdep[j] >= 0) != vdep:
+            raise _argerr(f"{r!r} is {'not ' if vdep else ''}a voltage-dependent surface reaction")
+        return j
 
     def getPatchSReacK(self, p: str, r: str) -> float:
-        self._check_patch(p)
-        return float(self.fm.sreac_k[self._sreac(r)])
+        pi = self._check_patch(p)
+        return float(self._psk[pi, self._sreac(r, pi, vdep=False)])
 
     def setPatchSReacK(self, p: str, r: str, k: float):
-        self._check_patch(p)
-        j = self._sreac(r)
-        self.eng.set_sreac_k(j, k, real=self._preal)
-        self.fm.sreac_k[j] = k
+        """_setPatchSReacK: the constant of the reaction on every triangle of the patch."""
+        pi = self._check_patch(p)
+        j = self._sreac(r, pi, vdep=False)
+        if k < 0:
+            raise _argerr("rate constant must be >= 0")
+        self._psk[pi, j] = k
+        rr, loc = self._patch_local(pi)
+        if self._tel[rr] is not None:
+            self._tel[rr].k[j, loc] = k
+        self._push_tri(rr)
+
+    def _set_patch_active(self, p: str, r: str, a: bool, vdep: bool):
+        pi = self._check_patch(p)
+        j = self._sreac(r, pi, vdep=vdep)
+        self._psact[pi, j] = bool(a)
+        rr, loc = self._patch_local(pi)
+        self._tri_elements(rr).act[j, loc] = bool(a)
+        self._push_tri(rr)
+
+    def _patch_active(self, p: str, r: str, vdep: bool) -> bool:
+        pi = self._check_patch(p)
+        j = self._sreac(r, pi, vdep=vdep)
+        rr, loc = self._patch_local(pi)
+        tel = self._tel[rr]
+        return bool(tel.act[j, loc].any()) if tel is not None else bool(self._psact[pi, j])
+
+    def getPatchSReacActive(self, p: str, r: str) -> bool:
+        return self._patch_active(p, r, False)
+
+    def setPatchSReacActive(self, p: str, r: str, a: bool):
+        self._set_patch_active(p, r, a, False)
+
+    def getPatchVDepSReacActive(self, p: str, r: str) -> bool:
+        return self._patch_active(p, r, True)
+
+    def setPatchVDepSReacActive(self, p: str, r: str, a: bool):
+        self._set_patch_active(p, r, a, True)
+
+    def _tri_v_of(self, r, loc):
+        """Potential (V) of triangles loc of compartment r, as reactions read it."""
+        if self.ef is not None and self._efreal == r:
+            return self.ef.tris_v()[loc]
+        return self.eng.get_tri_v(r)[loc]
+
+    def _vdep_k(self, j: int, v: np.ndarray) -> np.ndarray:
+        """k(V) by linear interpolation in the reaction's table
+        (VDepSReacdef::getVDepK); outside the table is an error, as there."""
+        t = int(self.fm.sreac_vdep[j])
+        tab = self.fm.vdep_k[self.fm.vdep_off[t]:self.fm.vdep_off[t + 1]]
+        x = (np.asarray(v, dtype=np.float64) - self.fm.vdep_vmin[t]) / self.fm.vdep_dv[t]
+        if np.any(~((x >= 0.0) & (x <= len(tab) - 1))):
+            raise N.StgError(N.STG_ERR_PROG, "triangle potential outside a voltage-dependent rate table")
+        lo, hi = np.floor(x).astype(int), np.ceil(x).astype(int)
+        fr = x - np.floor(x)
+        return (1.0 - fr) * tab[lo] + fr * tab[hi]
+
+    def _sreac_hc(self, pi: int, j: int, sel=None):
+        """(h, c) per triangle of patch pi (or its triangles sel, positions in
+        the patch) for surface reaction j at the present counts (SReac::rate,
+        sreac.cpp:305-430; constants as SReac::resetCcst, sreac.cpp:80-100:
+        scaled by the inner or outer tet's volume when volume species react,
+        else by the triangle's area)."""
+        fm = self.fm
+        p = self.patches[pi]
+        r, loc_all = self._patch_local(pi)
+        pos = np.arange(len(p.tris)) if sel is None else np.asarray(sel)
+        tris, loc = p.tris[pos], loc_all[pos]
+        h = np.ones(len(pos))
+        if len(fm.sspecs):
+            lhs = fm.sreac_lhs_s[j]
+            cnt = np.zeros((len(lhs), len(pos)), dtype=np.int64)
+            for q in np.nonzero(lhs)[0]:
+                cnt[q] = self.eng.get_tri_counts(int(q), real=r)[loc]
+            h = h * _falling_h(cnt, lhs)
+        tt = self.geom.tri_tets[tris]
+        inner = np.where(self._tet_real[tt[:, 0]] == r, tt[:, 0], tt[:, 1])
+        outer = np.where(tt[:, 0] == inner, tt[:, 1], tt[:, 0])
+        side = None
+        for lhs, tets, on in ((fm.sreac_lhs_i[j], inner, True), (fm.sreac_lhs_o[j], outer, False)):
+            if not lhs.any():
+                continue
+            side = tets
+            rc = self._tet_real[tets]
+            cnt = np.zeros((len(lhs), len(pos)), dtype=np.int64)
+            for s in np.nonzero(lhs)[0]:
+                for cc in np.unique(rc):
+                    m = rc == cc
+                    cnt[s, m] = self.eng.batch_tet_counts(int(s), self._tet_local[tets[m]], real=int(cc))
+            h = h * _falling_h(cnt, lhs)
+        tel = self._tel[r]
+        vd = fm.sreac_vdep[j] >= 0
+        if tel is not None:
+            k = tel.sreac_table()[j, loc]
+        else:
+            k = np.full(len(pos), (1.0 if vd else self._psk[pi, j]) if self._psact[pi, j] else 0.0)
+        if vd:
+            k = k * self._vdep_k(j, self._tri_v_of(r, loc))
+        order = int(fm.sreac_lhs_s[j].sum() + fm.sreac_lhs_i[j].sum() + fm.sreac_lhs_o[j].sum())
+        if side is None:
+            scale = self.geom.tri_area[tris] * AVOGADRO
+        else:
+            scale = 1e3 * self.geom.tet_vol[side] * AVOGADRO
+        return h, k * scale ** (1 - order)
+
+    def getPatchSReacH(self, p: str, r: str) -> float:
+        pi = self._check_patch(p)
+        return float(self._sreac_hc(pi, self._sreac(r, pi))[0].sum())
+
+    def getPatchSReacC(self, p: str, r: str) -> float:
+        """Area-weighted mean of the triangles' c (_getPatchSReacC)."""
+        pi = self._check_patch(p)
+        _, c = self._sreac_hc(pi, self._sreac(r, pi))
+        area = self.geom.tri_area[self.patches[pi].tris]
+        return float((c * area).sum() / area.sum())
+
+    def getPatchSReacA(self, p: str, r: str) -> float:
+        pi = self._check_patch(p)
+        h, c = self._sreac_hc(pi, self._sreac(r, pi))
+        return float((h * c).sum())
 
     def _sdiff(self, d: str) -> int:
         if d not in self.fm.sdiffs:
@@ -581,51 +1444,216 @@ class TetOpSplit:
 
     def getPatchSDiffD(self, p: str, d: str) -> float:
         self._check_patch(p)
-        return float(self.fm.sspec_dcst[self._sdiff(d)])
+        return float(self._sd[self._sdiff(d)])
 
     def setPatchSDiffD(self, p: str, d: str, dcst: float, direction_patch=None):
-        """Surface diffusion constant of a rule on the patch (setPatchSDiffD;
-        directional constants need surface diffusion boundaries, which this
-        solver does not model)."""
-        self._check_patch(p)
-        if direction_patch is not None:
+        """Surface diffusion constant of a rule on every triangle of the patch
+        (_setPatchSDiffD; directional constants need surface diffusion
+        boundaries, which this engine does not model)."""
+        pi = self._check_patch(p)
+        if direction_patch not in (None, ""):
             raise _notimpl("surface diffusion boundaries are not supported")
         if dcst < 0:
             raise _argerr("diffusion constant must be >= 0")
         q = self._sdiff(d)
-        self.eng.set_sdcst(q, dcst, real=self._preal)
-        self.fm.sspec_dcst[q] = dcst
+        r, loc = self._patch_local(pi)
+        tel = self._tel[r]
+        if tel is not None and tel.eD is not None or len(np.unique(self._rtri_patch[r])) > 1:
+            tel = self._sdiff_table_on(r)
+            tel.eD[q, loc, :] = np.where(self.mts[r].tri_edge_g[loc] > 0, dcst, 0.0)
+            self._push_sdiff(r)
+        else:
+            self.eng.set_sdcst(q, dcst, real=r)
+            self._sd[q] = dcst
 
-    def _sreac(self, r):
-        try:
-            return self.fm.sreacs.index(r)
-        except ValueError:
-            raise _argerr(f"unknown surface reaction {r!r}") from None
+    def getTriArea(self, tri: int) -> float:
+        if not 0 <= tri < self.geom.countTris():
+            raise _argerr(f"triangle {tri} out of range")
+        return float(self.geom.tri_area[tri])
+
+    def getTriSpecDefined(self, tri: int, s: str) -> bool:
+        r, i, pi = self._tri(tri)
+        self._sspec(s)
+        return s in self.fm.patch_sspecs[pi]
 
     def getTriSpecCount(self, tri: int, s: str) -> float:
-        return float(self.eng.get_tri_counts(self._sspec(s), real=self._preal)[self._tri(tri)])
+        r, i, pi, q = self._tri_sspec(tri, s)
+        return float(self.eng.get_tri_counts(q, real=r)[i])
 
     def setTriSpecCount(self, tri: int, s: str, n: float):
         if n < 0:
             raise _argerr("negative number of molecules")
-        sp = self._sspec(s)
-        c = self.eng.get_tri_counts(sp, real=self._preal)
-        c[self._tri(tri)] = int(round(n))
-        self.eng.set_tri_counts(sp, c, real=self._preal)
+        r, i, pi, q = self._tri_sspec(tri, s)
+        c = self.eng.get_tri_counts(q, real=r)
+        c[i] = _round_count(n, self._host_rng)
+        self.eng.set_tri_counts(q, c, real=r)
+
+    def getTriSpecAmount(self, tri: int, s: str) -> float:
+        return self.getTriSpecCount(tri, s) / AVOGADRO
+
+    def setTriSpecAmount(self, tri: int, s: str, a: float):
+        self.setTriSpecCount(tri, s, a * AVOGADRO)
+
+    def getTriSpecClamped(self, tri: int, s: str) -> bool:
+        r, i, pi, q = self._tri_sspec(tri, s)
+        tel = self._tel[r]
+        return bool(tel is not None and tel.clamped(q, i))
+
+    def setTriSpecClamped(self, tri: int, s: str, buf: bool):
+        """_setTriSpecClamped."""
+        r, i, pi, q = self._tri_sspec(tri, s)
+        self._tri_elements(r).set_clamped(q, i, bool(buf))
+        self._push_tri(r)
+        self._push_tri_clamp(r)
+
+    def getTriSReacK(self, tri: int, r: str) -> float:
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi, vdep=False)
+        tel = self._tel[rr]
+        return float(tel.k[j, i]) if tel is not None else float(self._psk[pi, j])
+
+    def setTriSReacK(self, tri: int, r: str, kf: float):
+        """_setTriSReacK (tetopsplit.cpp:4464)."""
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi, vdep=False)
+        if kf < 0:
+            raise _argerr("rate constant must be >= 0")
+        self._tri_elements(rr).k[j, i] = kf
+        self._push_tri(rr)
+
+    def _tri_active(self, tri: int, r: str, vdep: bool) -> bool:
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi, vdep=vdep)
+        tel = self._tel[rr]
+        return bool(tel.act[j, i]) if tel is not None else bool(self._psact[pi, j])
+
+    def _set_tri_active(self, tri: int, r: str, act: bool, vdep: bool):
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi, vdep=vdep)
+        self._tri_elements(rr).act[j, i] = bool(act)
+        self._push_tri(rr)
+
+    def getTriSReacActive(self, tri: int, r: str) -> bool:
+        return self._tri_active(tri, r, False)
+
+    def setTriSReacActive(self, tri: int, r: str, act: bool):
+        """_setTriSReacActive."""
+        self._set_tri_active(tri, r, act, False)
+
+    def getTriVDepSReacActive(self, tri: int, r: str) -> bool:
+        return self._tri_active(tri, r, True)
+
+    def setTriVDepSReacActive(self, tri: int, r: str, act: bool):
+        """_setTriVDepSReacActive."""
+        self._set_tri_active(tri, r, act, True)
+
+    def getTriVDepSReacK(self, tri: int, r: str) -> float:
+        """k(V) at the triangle's present potential (_getTriVDepSReacK)."""
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi, vdep=True)
+        return float(self._vdep_k(j, self._tri_v_of(rr, np.array([i])))[0])
+
+    def _tri_sreac_hc(self, tri: int, r: str):
+        rr, i, pi = self._tri(tri)
+        j = self._sreac(r, pi)
+        pos = int(np.nonzero(self.patches[pi].tris == tri)[0][0])
+        h, c = self._sreac_hc(pi, j, [pos])
+        return float(h[0]), float(c[0])
+
+    def getTriSReacH(self, tri: int, r: str) -> float:
+        return self._tri_sreac_hc(tri, r)[0]
+
+    def getTriSReacC(self, tri: int, r: str) -> float:
+        return self._tri_sreac_hc(tri, r)[1]
+
+    def getTriSReacA(self, tri: int, r: str) -> float:
+        h, c = self._tri_sreac_hc(tri, r)
+        return h * c
+
+    def _tri_edge_toward(self, tri: int, other: int) -> int:
+        rr, i, pi = self._tri(tri)
+        ro, io, po = self._tri(other) if 0 <= other < len(self._tri_local) and self._tri_local[other] >= 0 \
+            else (-1, -1, -1)
+        e = np.nonzero((self.mts[rr].tri_nbr[i] == io) & (ro == rr))[0] if io >= 0 else []
+        if len(e) == 0:
+            raise _argerr(f"Triangle {other} is not a neighbor of triangle {tri}.")
+        return int(e[0])
+
+    def getTriSDiffD(self, tri: int, d: str, direction_tri=None) -> float:
+        rr, i, pi = self._tri(tri)
+        q = self._sdiff(d)
+        tel = self._tel[rr]
+        if tel is None or tel.eD is None:
+            return float(self._sd[q])
+        if direction_tri is None or direction_tri < 0:
+            e = np.nonzero(self.mts[rr].tri_edge_g[i] > 0)[0]
+            return float(tel.eD[q, i, e[0]]) if len(e) else float(self._sd[q])
+        return float(tel.eD[q, i, self._tri_edge_toward(tri, int(direction_tri))])
+
+    def setTriSDiffD(self, tri: int, d: str, dcst: float, direction_tri=None):
+        """_setTriSDiffD: the rule's constant on one triangle, or through the
+        edge toward direction_tri (SDiff::setDcst / setDirectionDcst)."""
+        rr, i, pi = self._tri(tri)
+        q = self._sdiff(d)
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        tel = self._sdiff_table_on(rr)
+        if direction_tri is None or direction_tri < 0:
+            tel.eD[q, i, :] = np.where(self.mts[rr].tri_edge_g[i] > 0, dcst, 0.0)
+        else:
+            tel.eD[q, i, self._tri_edge_toward(tri, int(direction_tri))] = dcst
+        self._push_sdiff(rr)
+
+    # batch access to triangles
+    def _tri_idx(self, index_array):
+        idx = np.asarray(index_array, dtype=np.int64)
+        for t in idx:
+            self._tri(int(t))
+        return idx
 
     def getBatchTriSpecCounts(self, tris, s: str):
-        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
-        return [float(c[self._tri(int(t))]) for t in tris]
-
-    def sumBatchTriCountsNP(self, tri_array, s: str) -> float:
-        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
-        return float(c[[self._tri(int(t)) for t in tri_array]].sum())
+        c = np.zeros(len(tris))
+        self.getBatchTriSpecCountsNP(np.asarray(tris, dtype=np.int64), s, c)
+        return c.tolist()
 
     def getBatchTriSpecCountsNP(self, index_array, s: str, counts):
-        c = self.eng.get_tri_counts(self._sspec(s), real=self._preal)
-        counts[:] = c[[self._tri(int(t)) for t in index_array]]
+        idx = self._tri_idx(index_array)
+        if len(counts) != len(idx):
+            raise _argerr("counts and index arrays differ in length")
+        q = self._sspec(s)
+        real, loc = self._tri_real[idx], self._tri_local[idx]
+        for r in np.unique(real):
+            m = real == r
+            counts[m] = self.eng.get_tri_counts(q, real=int(r))[loc[m]]
 
-    # --- membrane potential (TetOpSplitP EField methods, tetopsplit.cpp:4751-5000) ---
+    def setBatchTriSpecCountsNP(self, index_array, s: str, counts):
+        idx = self._tri_idx(index_array)
+        vals = np.asarray(counts, dtype=np.float64)
+        if len(vals) != len(idx):
+            raise _argerr("counts and index arrays differ in length")
+        if np.any(vals < 0):
+            raise _argerr("negative number of molecules")
+        q = self._sspec(s)
+        real, loc = self._tri_real[idx], self._tri_local[idx]
+        fl = np.floor(vals)
+        rounded = (fl + (self._host_rng.random(len(vals)) < vals - fl)).astype(np.int64)
+        for r in np.unique(real):
+            m = real == r
+            c = self.eng.get_tri_counts(q, real=int(r))
+            c[loc[m]] = rounded[m]
+            self.eng.set_tri_counts(q, c, real=int(r))
+
+    def setBatchTriSpecCounts(self, tris, s: str, counts):
+        self.setBatchTriSpecCountsNP(np.asarray(tris, dtype=np.int64), s, np.asarray(counts, dtype=float))
+
+    def sumBatchTriSpecCountsNP(self, tri_array, s: str) -> float:
+        c = np.zeros(len(tri_array))
+        self.getBatchTriSpecCountsNP(tri_array, s, c)
+        return float(c.sum())
+
+    sumBatchTriCountsNP = sumBatchTriSpecCountsNP
+
+    # --------------------------------------------- membrane potential (EField)
     def _need_ef(self):
         if self.ef is None:
             raise _argerr("Method not available: EField calculation not included in simulation.")
@@ -686,6 +1714,13 @@ class TetOpSplit:
         if ro <= 0.0:
             raise _argerr("Resistivity must be greater than zero")
         self.ef.set_memb_res(ro, vrev)
+        self._memb_res = (float(ro), float(vrev))
+
+    def getMembRes(self, m: str):
+        """(resistivity, reversal potential) of the membrane's leak
+        (TetOpSplitP::_getMembRes, tetopsplit.cpp:5305)."""
+        self._memb(m)
+        return self._memb_res
 
     def getVertV(self, v: int) -> float:
         return float(self.ef.verts_v()[self._vert(v)])
@@ -733,10 +1768,12 @@ class TetOpSplit:
             self.setVertVClamped(int(x), cl)
 
     def getTriV(self, tri: int) -> float:
-        """Potential (V) of a membrane triangle, the mean of its vertices (EField::getTriV,
-        efield.cpp:290-302). Without an EField, the value voltage-dependent reactions read."""
+        """Potential (V) of a membrane triangle, the mean of its vertices
+        (EField::getTriV, efield.cpp:290-302). Without an EField, the value
+        voltage-dependent reactions read."""
         if self.ef is None:
-            return float(self.eng.get_tri_v(self._preal)[self._tri(tri)])
+            r, i, _ = self._tri(tri)
+            return float(self.eng.get_tri_v(r)[i])
         return float(self.ef.tris_v()[self._eftri(tri)])
 
     def setTriV(self, tri: int, v: float):
@@ -745,9 +1782,10 @@ class TetOpSplit:
         tetopsplit.cpp:4871-4893). Without one, only the value voltage-dependent
         reactions read (no counterpart in the reference, which raises)."""
         if self.ef is None:
-            vv = self.eng.get_tri_v(self._preal)
-            vv[self._tri(tri)] = v
-            self.eng.set_tri_v(vv, self._preal)
+            r, i, _ = self._tri(tri)
+            vv = self.eng.get_tri_v(r)
+            vv[i] = v
+            self.eng.set_tri_v(vv, r)
             return
         self._eftri(tri)
         vv = self.ef.verts_v()
@@ -771,22 +1809,129 @@ class TetOpSplit:
         self.ef.set_tri_iclamp(k, i)
         self._tri_iclamp[k] = i
 
-    def getTriOhmicI(self, tri: int, oc: str) -> float:
-        """Present ohmic current (A) of one current on a triangle: open channels
-        times g times (V - erev) (Tri::getOhmicI, tri.cpp:709-718)."""
+    def setTriCapac(self, tri: int, cm: float):
+        """_setTriCapac (tetopsplit.cpp:5187): specific capacitance (F/m^2) of
+        one membrane triangle."""
         k = self._eftri(tri)
+        if cm < 0.0:
+            raise _argerr("Capacitance must be non-negative")
+        self.ef.set_tri_capac(k, cm)
+
+    def _ohmic(self, oc: str) -> int:
         try:
-            j = self.fm.ohmic.index(oc)
+            return self.fm.ohmic.index(oc)
         except ValueError:
             raise _argerr(f"unknown ohmic current {oc!r}") from None
-        n = self.eng.get_tri_counts(int(self.fm.ohmic_chan[j]), real=self._preal)[self._tri(tri)]
+
+    def getTriOhmicErev(self, tri: int, oc: str) -> float:
+        """_getTriOhmicErev (tetopsplit.cpp:4963)."""
+        k = self._eftri(tri)
+        return self.ef.tri_ohmic_erev(self._ohmic(oc), k)
+
+    def setTriOhmicErev(self, tri: int, oc: str, erev: float):
+        """_setTriOhmicErev (tetopsplit.cpp:4933, Tri::setOCerev)."""
+        k = self._eftri(tri)
+        self.ef.set_tri_ohmic_erev(self._ohmic(oc), k, erev)
+
+    def getTriOhmicI(self, tri: int, oc: str = "") -> float:
+        """Present ohmic current (A) of one current on a triangle, or of all of
+        them: open channels times g times (V - erev) (Tri::getOhmicI,
+        tri.cpp:697-718)."""
+        k = self._eftri(tri)
+        r, i, _ = self._tri(tri)
         v = self.ef.tris_v()[k]
-        return float(n * self.fm.ohmic_g[j] * (v - self.fm.ohmic_erev[j]))
+        js = [self._ohmic(oc)] if oc else range(len(self.fm.ohmic))
+        tot = 0.0
+        for j in js:
+            n = float(self.eng.get_tri_counts(int(self.fm.ohmic_chan[j]), real=r)[i])
+            tot += (n * float(self.fm.ohmic_g[j])) * (v - self.ef.tri_ohmic_erev(j, k))
+        return float(tot)
+
+    def _ghk(self, ghk: str) -> int:
+        try:
+            return self.fm.ghk.index(ghk)
+        except ValueError:
+            raise _argerr(f"unknown GHK current {ghk!r}") from None
+
+    def getTriGHKI(self, tri: int, ghk: str = "") -> float:
+        """GHK current (A, outward positive) of one current on a triangle, or
+        of all of them: the charge its ions carried over the last EField step
+        over the step (Tri::getGHKI, tri.cpp:634-660)."""
+        k = self._eftri(tri)
+        g = -1
+        if ghk:
+            g = self._ghk(ghk)
+            if g not in self.fm.patch_ghk[self._tri(tri)[2]]:
+                raise _argerr("GHK current undefined in triangle.")
+        return float(self.ef.tri_ghk_currents(g)[k])
+
+    def getTriI(self, tri: int) -> float:
+        """_getTriI (tetopsplit.cpp:5101): GHK plus ohmic current."""
+        return self.getTriGHKI(tri) + self.getTriOhmicI(tri)
+
+    def _batch_ef(self, index_array, n_out, counts):
+        """EField positions of the listed triangles (-1 for triangles outside
+        every patch, which read 0 as in the reference)."""
+        self._need_ef()
+        idx = np.asarray(index_array, dtype=np.int64)
+        if len(counts) != n_out:
+            raise _argerr("Error: output array (counts) size does not match the input")
+        if np.any(idx < 0) or np.any(idx >= self.geom.countTris()):
+            raise _argerr("Error (Index Overbound): there is no such triangle")
+        pos = np.where(self._tri_local[idx] >= 0, self._ef_tri_local[idx], -1)
+        if np.any((self._tri_local[idx] >= 0) & (pos < 0)):
+            raise _argerr("a triangle is not assigned to the membrane")
+        return idx, pos
+
+    def getBatchTriGHKIsNP(self, index_array, ghk: str, counts):
+        """TetOpSplitP::getBatchTriGHKIsNP (tetopsplit.cpp:6244-6305)."""
+        idx, pos = self._batch_ef(index_array, len(index_array), counts)
+        g = self._ghk(ghk)
+        cur = self.ef.tri_ghk_currents(g)
+        held = np.array([pos[k] >= 0 and g in self.fm.patch_ghk[self._tri_patch[idx[k]]]
+                         for k in range(len(idx))], dtype=bool)
+        counts[:] = np.where(held, cur[np.maximum(pos, 0)], 0.0)
+
+    def sumBatchTriGHKIsNP(self, tri_array, ghk: str) -> float:
+        c = np.zeros(len(tri_array))
+        self.getBatchTriGHKIsNP(tri_array, ghk, c)
+        return float(c.sum())
+
+    def getBatchTriBatchGHKIsNP(self, index_array, ghks, counts):
+        """counts[t * len(ghks) + k] = current ghks[k] on triangle index_array[t]
+        (TetOpSplitP::getBatchTriBatchGHKIsNP, tetopsplit.cpp:6481-6550)."""
+        n = len(ghks)
+        idx, pos = self._batch_ef(index_array, len(index_array) * n, counts)
+        for k, ghk in enumerate(ghks):
+            col = np.zeros(len(idx))
+            self.getBatchTriGHKIsNP(idx, ghk, col)
+            counts[k::n] = col
+
+    def getBatchTriOhmicIsNP(self, index_array, oc: str, counts):
+        idx, pos = self._batch_ef(index_array, len(index_array), counts)
+        j = self._ohmic(oc)
+        for k, t in enumerate(idx):
+            counts[k] = self.getTriOhmicI(int(t), oc) if pos[k] >= 0 else 0.0
+
+    def sumBatchTriOhmicIsNP(self, tri_array, oc: str) -> float:
+        c = np.zeros(len(tri_array))
+        self.getBatchTriOhmicIsNP(tri_array, oc, c)
+        return float(c.sum())
+
+    def getBatchTriBatchOhmicIsNP(self, index_array, ocs, counts):
+        """counts[t * len(ocs) + k] = current ocs[k] on triangle index_array[t]
+        (TetOpSplitP::getBatchTriBatchOhmicIsNP, tetopsplit.cpp:6406-6478)."""
+        n = len(ocs)
+        self._batch_ef(index_array, len(index_array) * n, counts)
+        for k, oc in enumerate(ocs):
+            col = np.zeros(len(index_array))
+            self.getBatchTriOhmicIsNP(index_array, oc, col)
+            counts[k::n] = col
 
     def getBatchTriVsNP(self, index_array, vs):
         if self.ef is None:
-            vv = self.eng.get_tri_v(self._preal)
-            vs[:] = vv[[self._tri(int(t)) for t in index_array]]
+            for k, t in enumerate(index_array):
+                vs[k] = self.getTriV(int(t))
         else:
             vv = self.ef.tris_v()
             vs[:] = vv[[self._eftri(int(t)) for t in index_array]]
@@ -797,13 +1942,6 @@ class TetOpSplit:
         for k, t in enumerate(index_array):
             vs[k] = vv[[self._vert(int(x)) for x in self.geom.tets[int(t)]]].mean()
 
-    def getBatchTriOhmicIsNP(self, index_array, oc: str, cur):
-        for k, t in enumerate(index_array):
-            cur[k] = self.getTriOhmicI(int(t), oc)
-
-    def sumBatchTriOhmicIsNP(self, tri_array, oc: str) -> float:
-        return float(sum(self.getTriOhmicI(int(t), oc) for t in tri_array))
-
     def getBatchVertVsNP(self, index_array, vs):
         self._need_ef()
         vv = self.ef.verts_v()
@@ -813,7 +1951,7 @@ class TetOpSplit:
         for t, v in zip(index_array, vs):
             self.setTriV(int(t), float(v))
 
-    # --- checkpointing (own binary format: stg_checkpoint and stg_efield_checkpoint in include/stg.h) ---
+    # ----------------------- checkpointing (stg_checkpoint / stg_efield_checkpoint)
     _CP_MAGIC = b"STGCKPT1"
 
     def checkpoint(self, file_name: str):
@@ -841,6 +1979,77 @@ class TetOpSplit:
         self.eng.restore(data[24:24 + n_eng])
         if self.ef is not None:
             self.ef.restore(data[24 + n_eng:24 + n_eng + n_ef])
+
+    # --------------------------------------------------- deprecated spellings
+    # The reference keeps these older names beside the *Spec* methods
+    # (cysteps_solver.pyx, marked DEPRECATED there).
+    getCompCount, setCompCount = getCompSpecCount, setCompSpecCount
+    getCompAmount, setCompAmount = getCompSpecAmount, setCompSpecAmount
+    getCompConc, setCompConc = getCompSpecConc, setCompSpecConc
+    getCompClamped, setCompClamped = getCompSpecClamped, setCompSpecClamped
+    getPatchCount, setPatchCount = getPatchSpecCount, setPatchSpecCount
+    getPatchAmount, setPatchAmount = getPatchSpecAmount, setPatchSpecAmount
+    getPatchClamped, setPatchClamped = getPatchSpecClamped, setPatchSpecClamped
+    getTetCount, setTetCount = getTetSpecCount, setTetSpecCount
+    getTetAmount, setTetAmount = getTetSpecAmount, setTetSpecAmount
+    getTetConc, setTetConc = getTetSpecConc, setTetSpecConc
+    getTetClamped, setTetClamped = getTetSpecClamped, setTetSpecClamped
+    getTriCount, setTriCount = getTriSpecCount, setTriSpecCount
+    getTriAmount, setTriAmount = getTriSpecAmount, setTriSpecAmount
+    getTriClamped, setTriClamped = getTriSpecClamped, setTriSpecClamped
+    getBatchTetCounts, getBatchTetCountsNP = getBatchTetSpecCounts, getBatchTetSpecCountsNP
+    setBatchTetCounts, setBatchTetCountsNP = setBatchTetSpecCounts, setBatchTetSpecCountsNP
+    getBatchTetConcs, getBatchTetConcsNP = getBatchTetSpecConcs, getBatchTetSpecConcsNP
+    setBatchTetConcs, setBatchTetConcsNP = setBatchTetSpecConcs, setBatchTetSpecConcsNP
+    getBatchTriCounts, getBatchTriCountsNP = getBatchTriSpecCounts, getBatchTriSpecCountsNP
+    setBatchTriCounts, setBatchTriCountsNP = setBatchTriSpecCounts, setBatchTriSpecCountsNP
+
+
+# Reference methods this engine does not provide, each with the reason. They
+# raise NotImplErr, as the reference's API base does for methods a solver
+# lacks (src/steps/solver/api_*.cpp: NotImplErrLog).
+NOT_PROVIDED = {
+    # complexes (multi-state species) belong to the reference's newer solvers only
+    **{m: "complexes are not part of the operator-split path" for m in (
+        "getCompComplexAmount", "getCompComplexConc", "getCompComplexCount", "getCompComplexReacExtent",
+        "getCompComplexSUSAmount", "getCompComplexSUSConc", "getCompComplexSUSCount",
+        "setCompComplexAmount", "setCompComplexConc", "setCompComplexCount",
+        "getPatchComplexAmount", "getPatchComplexCount", "getPatchComplexSReacExtent",
+        "getPatchComplexSUSCount", "setPatchComplexAmount", "setPatchComplexCount")},
+    # geometry is fixed by the mesh (TetOpSplitP::_setCompVol and
+    # _setPatchArea throw NotImplErr too)
+    **{m: "volumes and areas are fixed by the mesh" for m in ("setCompVol", "setPatchArea")},
+    # per-reaction extents: the engine counts events per realization
+    **{m: "per-reaction extents are not counted; getReacExtent gives the total" for m in (
+        "getCompReacExtent", "resetCompReacExtent", "getPatchSReacExtent", "resetPatchSReacExtent")},
+    # ROI extents (TetOpSplitP throws NotImplErr for these, tetopsplit.cpp:7605-7879)
+    **{m: "ROI extents are not implemented for this solver" for m in (
+        "getROIReacExtent", "resetROIReacExtent", "getROISReacExtent", "resetROISReacExtent",
+        "getROIDiffExtent", "resetROIDiffExtent")},
+    # surface diffusion boundaries
+    **{m: "surface diffusion boundaries are not supported" for m in (
+        "setSDiffBoundaryDiffusionActive", "getSDiffBoundaryDiffusionActive", "setSDiffBoundaryDcst",
+        "setSDiffBoundarySpecDiffusionActive", "getSDiffBoundarySpecDiffusionActive",
+        "setSDiffBoundarySpecDcst")},
+    # the whole-state total propensity (API::getA0; TetOpSplitP does not provide it)
+    "getA0": "the total propensity of the state is not provided by operator splitting",
+    # DistTetOpSplit's Gibson-Bruck dependency graph and its mesh-bandwidth file
+    "dumpDepGraphToFile": "there is no global dependency graph: each tet runs its own SSA",
+    "saveMembOpt": "the GPU EField is a sparse CG solve: there is no bandwidth ordering to save",
+}
+
+
+def _make_notimpl(name, why):
+    def method(self, *args, **kwargs):
+        raise _notimpl(f"{name}: {why}")
+    method.__name__ = name
+    method.__doc__ = f"Not provided by this engine ({why}); raises NotImplErr."
+    return method
+
+
+for _n, _why in NOT_PROVIDED.items():
+    if not hasattr(TetOpSplit, _n):
+        setattr(TetOpSplit, _n, _make_notimpl(_n, _why))
 
 
 class DistTetOpSplit(TetOpSplit):

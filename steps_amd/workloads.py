@@ -265,3 +265,52 @@ def two_comp_membrane(n=4, side=4e-6, k_in=2e9, k_out=1e9, n_a=4000, n_t=1, bind
         g = M.GHKcurr("gca", ss, Ch, Ca, computeflux=True)
         g.setPInfo(20e-12, -65e-3, 293.15, 2e-3, 1e-7)
     return mdl, mesh, {"inner": ci, "outer": co}, patch
+
+
+def two_patch_membrane(n=4, side=4e-6, k_in=2e9, k_out=1e9, k_leak=3e9, dA=1e-11, dT2=2e-13):
+    """The cut of two_comp_membrane split into two patches with different
+    surface systems, both between 'inner' and 'outer': patch 'pa' (cut
+    triangles with barycentre y < side/2, surface system 'ssa') moves A in and
+    out through transporter T (T + A(outer) -> T + A(inner) at k_in, the
+    reverse at k_out) and passes the GHK calcium current of channel Ch; patch
+    'pb' (surface system 'ssb') leaks A outward through T2 (T2 + A(inner) ->
+    T2 + A(outer) at k_leak), flips T2 <-> T3 (5 /s, 2 /s) and lets T2
+    diffuse at dT2. The inner compartment holds A + B <-> C (1e6 /M/s, 1 /s).
+    Returns (model, mesh, comps {'inner', 'outer'}, patches {'pa', 'pb'})."""
+    mesh = geom.kuhn_cube(n, side)
+    left = np.nonzero(mesh.tet_bary[:, 0] < side / 2)[0]
+    right = np.nonzero(mesh.tet_bary[:, 0] >= side / 2)[0]
+    ci = geom.TmComp("inner", mesh, left)
+    co = geom.TmComp("outer", mesh, right)
+    ci.addVolsys("vin")
+    co.addVolsys("vout")
+    tt = mesh.tri_tets
+    side_of = np.zeros(mesh.countTets(), dtype=np.int64)
+    side_of[right] = 1
+    both = tt[:, 1] >= 0
+    cut = np.nonzero(both & (side_of[tt[:, 0]] != side_of[np.maximum(tt[:, 1], 0)]))[0]
+    low = mesh.tri_bary[cut, 1] < side / 2
+    pa = geom.TmPatch("pa", mesh, cut[low], ci, co)
+    pb = geom.TmPatch("pb", mesh, cut[~low], ci, co)
+    pa.addSurfsys("ssa")
+    pb.addSurfsys("ssb")
+    mdl = M.Model()
+    A, B, C = M.Spec("A", mdl), M.Spec("B", mdl), M.Spec("C", mdl)
+    Ca = M.Spec("Ca", mdl, valence=2)
+    T, T2, T3, Ch = M.Spec("T", mdl), M.Spec("T2", mdl), M.Spec("T3", mdl), M.Spec("Ch", mdl)
+    vin, vout = M.Volsys("vin", mdl), M.Volsys("vout", mdl)
+    for vs in (vin, vout):
+        M.Diff("dA", vs, A, dA)
+        M.Diff("dCa", vs, Ca, dA)
+    M.Reac("bind", vin, lhs=[A, B], rhs=[C], kcst=1e6)
+    M.Reac("unbind", vin, lhs=[C], rhs=[A, B], kcst=1.0)
+    ssa, ssb = M.Surfsys("ssa", mdl), M.Surfsys("ssb", mdl)
+    M.SReac("in", ssa, slhs=[T], olhs=[A], srhs=[T], irhs=[A], kcst=k_in)
+    M.SReac("out", ssa, slhs=[T], ilhs=[A], srhs=[T], orhs=[A], kcst=k_out)
+    g = M.GHKcurr("gca", ssa, Ch, Ca, computeflux=True)
+    g.setPInfo(20e-12, -65e-3, 293.15, 2e-3, 1e-7)
+    M.SReac("leak", ssb, slhs=[T2], ilhs=[A], srhs=[T2], orhs=[A], kcst=k_leak)
+    M.SReac("flip", ssb, slhs=[T2], srhs=[T3], kcst=5.0)
+    M.SReac("flop", ssb, slhs=[T3], srhs=[T2], kcst=2.0)
+    M.Diff("dT2", ssb, T2, dT2)
+    return mdl, mesh, {"inner": ci, "outer": co}, {"pa": pa, "pb": pb}

@@ -223,6 +223,7 @@ def test_ghk_charge_drives_the_potential():
         obj.set_capac(0.01)
         obj.set_volres(1.0)
     ef.attach(eng)
+    assert np.all(ef.tri_ghk_currents(-1) == 0.0)   # no step yet
     sim = bind.OracleSim(fm, mt, seed=9)
     sim.tri_counts[qo] = n_open
     sim.counts[sca] = ca0
@@ -234,8 +235,11 @@ def test_ghk_charge_drives_the_potential():
         sim.run(step * T, 0)
         assert np.array_equal(eng.get_tet_counts(sca), sim.counts[sca]), step
         i_tri = sim.tri_charge[:len(memb)] * 1.602176487e-19 / T
-        sim.tri_charge[:] = 0
+        sim.clear_charge()
         assert i_tri.sum() < 0   # calcium flows in
+        # Tri::getGHKI: the last step's charge over the step, per current and in total
+        assert np.allclose(ef.tri_ghk_currents(0), i_tri, rtol=1e-12, atol=0)
+        assert np.allclose(ef.tri_ghk_currents(-1), i_tri, rtol=1e-12, atol=0)
         dv = ora.advance(T, i_tri) * 1e-3
         got = ef.verts_v() - v_prev
         err = np.abs(got - dv).max() / np.abs(dv).max()
