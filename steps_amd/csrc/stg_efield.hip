@@ -1986,6 +1986,8 @@ int stg_efield_run(stg_efield *e, stg_sim *sim, double endtime, double efdt) {
 // present channel-count integrals started from, and the GHK charges of the
 // last step with its length (Tri::checkpoint keeps pECharge_last and its dt,
 // tri.cpp:133-157).
+}  // extern "C"
+
 namespace {
 constexpr uint64_t kEfStateMagic = 0x3253464547544ull;  // "TGEFS2"
 struct EfStateHeader {
@@ -2009,6 +2011,8 @@ size_t ef_state_bytes(stg_efield *e, bool oc_erev, int n_ghk) {
 }
 int ef_saved_ghk(const stg_efield *e) { return e->ghk_last ? e->n_ghk : 0; }
 }  // namespace
+
+extern "C" {
 
 int stg_efield_state_size(const stg_efield *e, size_t *bytes) {
     if (!e || !bytes) return STG_ERR_ARG;

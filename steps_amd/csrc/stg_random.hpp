@@ -207,22 +207,21 @@ __device__ __forceinline__ double ipow(double base, int32_t n) {
 // n*q = 30 that costs fewer instructions than the rejection sampler, whose
 // wavefront runs to its slowest lane's trial count and exact test.
 // The search stops at x = min(n, 150); x and n - x are carried as doubles
-// (exact integers), which gives the same operands as converting them each step.
+// (exact integers), which gives the same operands as converting them each
+// step, and x itself is the loop counter (no integer copy per step).
 __device__ __forceinline__ int32_t binom_inversion(Philox& g, int32_t n, double q) {
     double r = ipow(1.0 - q, n);
     const double s = q / (1.0 - q);
     double u = g.unif();
-    const int32_t lim = n < 150 ? n : 150;
-    int32_t x = 0;
+    const double lim = (double)(n < 150 ? n : 150);
     double xd = 0.0, nx = (double)n;
-    while (!(u <= r) && x < lim) {
-        ++x;
+    while (!(u <= r) && xd < lim) {
         xd = xd + 1.0;
         u = (u - r) * xd;
         r = r * (nx * s);
         nx = nx - 1.0;
     }
-    return x;
+    return (int32_t)xd;
 }
 
 // Binomial by transformed rejection with squeeze (Hoermann 1993, BTRS);
