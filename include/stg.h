@@ -464,16 +464,21 @@ int stg_efield_sizes(const stg_efield *ef, int32_t *n_verts, int32_t *n_tris, in
 
 /* ------------------------------------------------------------- mesh ingest
  * Face topology and measures of a tetrahedral mesh built on the device, the
- * work of the Tetmesh constructor (src/steps/geom/tetmesh.cpp:78-233) that
- * the mesh loaders feed (pysteps/steps/API_2/geom.py:1466-1592). Triangles are the
- * distinct faces, numbered by their sorted vertex triple; tri_tets lists the
- * first tet having the face and the neighbour across it (-1 on the boundary).
- * Coordinates in m; outputs in SI. Errors: STG_ERR_ARG for an index out of
- * range, a degenerate tet or a triangle shared by more than two tets. */
+ * work of the Tetmesh constructor Tetmesh(verts, tets, tris)
+ * (src/steps/geom/tetmesh.cpp:78-233) that the mesh loaders feed
+ * (pysteps/steps/API_2/geom.py:1466-1592). Triangles are numbered as the
+ * reference numbers them: the n_tris supplied triangles first (tris may be
+ * NULL when n_tris is 0; repeats dropped), then every other tet face in order
+ * of first appearance, tet by tet, faces (v0,v1,v2), (v0,v1,v3), (v0,v2,v3),
+ * (v1,v2,v3); a triangle's vertices are stored sorted; tri_tets lists the
+ * lowest-numbered tet having the face and the neighbour across it (-1 on the
+ * boundary). Coordinates in m; outputs in SI. Errors: STG_ERR_ARG for an index
+ * out of range, a degenerate tet, a triangle shared by more than two tets or
+ * a supplied triangle that is no tet's face. */
 typedef struct stg_meshbuild stg_meshbuild;
 
 int stg_mesh_build(const double *verts, int32_t n_verts, const int32_t *tets, int32_t n_tets,
-                   stg_meshbuild **out);
+                   const int32_t *tris, int32_t n_tris, stg_meshbuild **out);
 int stg_mesh_build_sizes(const stg_meshbuild *mb, int32_t *n_tets, int32_t *n_tris);
 /* Any output pointer may be NULL. Sizes: tet_vol [n_tets], tet_bary
  * [n_tets*3], tet_tri / tet_nbr / tet_nbr_face / tet_face_area /

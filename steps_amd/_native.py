@@ -194,7 +194,7 @@ SIGNATURES = {
     "stg_efield_state_size": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "stg_efield_checkpoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "stg_efield_restore": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
-    "stg_mesh_build": (C.c_int, [_dp, C.c_int32, _ip, C.c_int32, C.POINTER(C.c_void_p)]),
+    "stg_mesh_build": (C.c_int, [_dp, C.c_int32, _ip, C.c_int32, _ip, C.c_int32, C.POINTER(C.c_void_p)]),
     "stg_mesh_build_sizes": (C.c_int, [C.c_void_p, _ip, _ip]),
     "stg_mesh_build_fetch": (C.c_int, [C.c_void_p, _dp, _dp, _ip, _ip, _ip, _dp, _dp, _ip, _ip, _dp, _dp]),
     "stg_mesh_build_destroy": (C.c_int, [C.c_void_p]),

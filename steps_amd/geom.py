@@ -37,7 +37,56 @@ def _face_keys(faces: np.ndarray, nverts: int) -> np.ndarray:
 ELEM_VERTEX, ELEM_TRI, ELEM_TET, ELEM_UNDEFINED = 0, 1, 2, 99   # src/steps/geom/tetmesh.hpp:46
 
 
-def _device_topology(verts: np.ndarray, tets: np.ndarray) -> dict:
+# The faces of tet t in the order the reference enumerates them when it
+# numbers triangles (tetmesh.cpp:118-139): (v0,v1,v2), (v0,v1,v3), (v0,v2,v3),
+# (v1,v2,v3). The j-th of these is the face opposite vertex 3 - j.
+_REF_FACES = np.array([[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3]])
+
+
+def _host_topology(verts: np.ndarray, tets: np.ndarray, tris: np.ndarray) -> dict:
+    """Triangle numbering and tet/triangle adjacency of Tetmesh::Tetmesh
+    (src/steps/geom/tetmesh.cpp:78-190): the supplied triangles first, in their
+    order (repeats dropped), then every other tet face in order of first
+    appearance, tet by tet; a triangle's vertices sorted; its first tet the
+    lowest-numbered tet having it."""
+    nt, nv = len(tets), len(verts)
+    ns = len(tris)
+    faces = np.concatenate([tris.reshape(-1, 3), tets[:, _REF_FACES].reshape(-1, 3)]).astype(np.int64)
+    keys = _face_keys(faces, nv)
+    _, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+    inv = inv.reshape(-1)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty(len(first), dtype=np.int64)
+    rank[order] = np.arange(len(first))
+    tri_of = rank[inv]
+    ntri = len(first)
+    out = dict(tris=np.sort(faces[first[order]], axis=1))
+    ft = tri_of[ns:]                          # triangle of tet face 4 t + j
+    out["tet_tri"] = ft.reshape(nt, 4)[:, ::-1].copy()
+    counts = np.bincount(ft, minlength=ntri)
+    if counts.max() > 2:
+        raise ValueError("non-manifold mesh: a triangle is shared by more than two tets")
+    if np.any(counts == 0):
+        raise ValueError("a supplied triangle is not a face of any tetrahedron")
+    occ = np.argsort(ft, kind="stable")
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    a = occ[starts]
+    b = np.where(counts == 2, occ[np.minimum(starts + 1, len(occ) - 1)], -1)
+    out["tri_tets"] = np.stack([a // 4, np.where(b >= 0, b // 4, -1)], axis=1)
+    nbr = -np.ones((nt, 4), dtype=np.int64)
+    nbr_face = -np.ones((nt, 4), dtype=np.int64)
+    two = b >= 0
+    ta, fa = a[two] // 4, 3 - a[two] % 4
+    tb, fb = b[two] // 4, 3 - b[two] % 4
+    nbr[ta, fa] = tb
+    nbr_face[ta, fa] = fb
+    nbr[tb, fb] = ta
+    nbr_face[tb, fb] = fa
+    out["tet_nbr"], out["tet_nbr_face"] = nbr, nbr_face
+    return out
+
+
+def _device_topology(verts: np.ndarray, tets: np.ndarray, tris: np.ndarray) -> dict:
     """The face topology and measures of Tetmesh, built by the engine library on
     the GPU (stg_mesh_build): same arrays, same triangle numbering."""
     import ctypes as C
@@ -47,8 +96,10 @@ def _device_topology(verts: np.ndarray, tets: np.ndarray) -> dict:
     lib = N.load()
     v = np.ascontiguousarray(verts, dtype=np.float64)
     t = np.ascontiguousarray(tets, dtype=np.int32)
+    s = np.ascontiguousarray(np.asarray(tris).reshape(-1, 3), dtype=np.int32)
     h = C.c_void_p()
-    rc = lib.stg_mesh_build(N.ptr(v, C.c_double), len(v), N.ptr(t, C.c_int32), len(t), C.byref(h))
+    rc = lib.stg_mesh_build(N.ptr(v, C.c_double), len(v), N.ptr(t, C.c_int32), len(t), N.ptr(s, C.c_int32),
+                            len(s), C.byref(h))
     if rc != N.STG_OK:
         msg = lib.stg_mesh_build_last_error()
         if rc == N.STG_ERR_ARG:
@@ -80,13 +131,17 @@ def _device_topology(verts: np.ndarray, tets: np.ndarray) -> dict:
 
 
 class Tetmesh:
-    """Unstructured tetrahedral mesh with its face topology and measures (SI units).
+    """Unstructured tetrahedral mesh with its face topology and measures (SI units),
+    Tetmesh(verts, tets, tris) of the reference (src/steps/geom/tetmesh.cpp:78-233):
+    the optional triangles take the first triangle indices, in their order, and
+    the remaining tet faces follow in order of first appearance, so triangle
+    indices agree with the reference's for the same input.
 
     device=True builds the topology and measures on the GPU (the engine
     library's stg_mesh_build) instead of with numpy on the host: the same
     arrays, for meshes of millions of tets."""
 
-    def __init__(self, verts, tets, device: bool = False):
+    def __init__(self, verts, tets, tris=None, device: bool = False):
         self.verts = np.ascontiguousarray(np.asarray(verts, dtype=np.float64))
         self.tets = np.ascontiguousarray(np.asarray(tets, dtype=np.int64))
         if self.verts.ndim != 2 or self.verts.shape[1] != 3:
@@ -99,13 +154,17 @@ class Tetmesh:
             raise ValueError("mesh has no tetrahedra")
         if self.tets.min() < 0 or self.tets.max() >= nv:
             raise ValueError("tet vertex index out of range")
+        sup = np.zeros((0, 3), dtype=np.int64) if tris is None else np.asarray(tris, dtype=np.int64).reshape(-1, 3)
+        if len(sup) and (sup.min() < 0 or sup.max() >= nv):
+            raise ValueError("triangle vertex index out of range")
+        self.supplied_tris = sup
         self.comps: dict[str, TmComp] = {}
         self.patches: dict[str, TmPatch] = {}
         self.membs: dict[str, Memb] = {}
         self.diffbs: dict[str, DiffBoundary] = {}
         self.rois: dict[str, tuple[int, np.ndarray]] = {}
         if device:
-            for k, v in _device_topology(self.verts, self.tets).items():
+            for k, v in _device_topology(self.verts, self.tets, sup).items():
                 setattr(self, k, v)
             return
         p = self.verts[self.tets]  # (nt, 4, 3)
@@ -114,33 +173,8 @@ class Tetmesh:
         if np.any(self.tet_vol <= 0):
             raise ValueError("degenerate tetrahedron")
         self.tet_bary = p.mean(axis=1)
-        # faces opposite vertex f
-        opp = np.array([[1, 2, 3], [0, 2, 3], [0, 1, 3], [0, 1, 2]])
-        faces = self.tets[:, opp].reshape(-1, 3)  # (nt*4, 3)
-        keys = _face_keys(faces, nv)
-        uniq, first, inv = np.unique(keys, return_index=True, return_inverse=True)
-        inv = inv.reshape(-1)
-        self.tris = np.sort(faces[first], axis=1)  # unique triangles
-        ntri = len(self.tris)
-        self.tet_tri = inv.reshape(nt, 4)
-        # the (tet, face) occurrences of each triangle
-        occ = np.argsort(inv, kind="stable")
-        counts = np.bincount(inv, minlength=ntri)
-        if counts.max() > 2:
-            raise ValueError("non-manifold mesh: a triangle is shared by more than two tets")
-        starts = np.concatenate([[0], np.cumsum(counts)[:-1]])
-        a = occ[starts]
-        b = np.where(counts == 2, occ[np.minimum(starts + 1, len(occ) - 1)], -1)
-        self.tri_tets = np.stack([a // 4, np.where(b >= 0, b // 4, -1)], axis=1)
-        nbr = -np.ones(nt * 4, dtype=np.int64)
-        nbr_face = -np.ones(nt * 4, dtype=np.int64)
-        two = b >= 0
-        nbr[a[two]] = b[two] // 4
-        nbr_face[a[two]] = b[two] % 4
-        nbr[b[two]] = a[two] // 4
-        nbr_face[b[two]] = a[two] % 4
-        self.tet_nbr = nbr.reshape(nt, 4)
-        self.tet_nbr_face = nbr_face.reshape(nt, 4)
+        for k, v in _host_topology(self.verts, self.tets, sup).items():
+            setattr(self, k, v)
         tp = self.verts[self.tris]
         self.tri_area = 0.5 * np.linalg.norm(np.cross(tp[:, 1] - tp[:, 0], tp[:, 2] - tp[:, 0]), axis=1)
         self.tri_bary = tp.mean(axis=1)
@@ -150,6 +184,61 @@ class Tetmesh:
         ti = np.nonzero(has)
         d[ti] = np.linalg.norm(self.tet_bary[ti[0]] - self.tet_bary[self.tet_nbr[ti]], axis=1)
         self.tet_face_dist = d
+
+    # --- loaders (TetMesh.Load / LoadAbaqus / LoadGmsh / LoadVTK / LoadTetGen,
+    # pysteps/steps/API_2/geom.py:1466-1600): the mesh, with the file's blocks
+    # and groups as vertGroups / triGroups / tetGroups {name: index array} ---
+    @classmethod
+    def _with_groups(cls, loaded):
+        from .meshio import element_groups
+
+        mesh, nodes, tets, tris = loaded
+        mesh.vertGroups = element_groups(nodes)
+        mesh.triGroups = element_groups(tris)
+        mesh.tetGroups = element_groups(tets)
+        return mesh
+
+    @classmethod
+    def LoadAbaqus(cls, filename, scale=1.0, ebs=None, device: bool = False):
+        """An Abaqus file, or a (tetrahedron file, triangle file) pair."""
+        from . import meshio
+
+        if isinstance(filename, tuple) and len(filename) == 2:
+            return cls._with_groups(meshio.importAbaqus2(filename[0], filename[1], scale, device=device))
+        if not isinstance(filename, str):
+            raise TypeError(f"expected a string or a 2-tuple of strings, got {filename!r}")
+        return cls._with_groups(meshio.importAbaqus(filename, scale, ebs=ebs, device=device))
+
+    @classmethod
+    def LoadGmsh(cls, filename, scale=1.0, device: bool = False):
+        from . import meshio
+
+        return cls._with_groups(meshio.importGmsh(filename, scale, device=device))
+
+    @classmethod
+    def LoadVTK(cls, filename, scale=1.0, device: bool = False):
+        from . import meshio
+
+        return cls._with_groups(meshio.importVTK(filename, scale, device=device))
+
+    @classmethod
+    def LoadTetGen(cls, pathroot, scale, device: bool = False):
+        from . import meshio
+
+        return cls._with_groups(meshio.importTetGen(pathroot, scale, device=device))
+
+    @classmethod
+    def Load(cls, path, scale=1.0, strict: bool = False, device: bool = False):
+        """A mesh saved with Save (or the reference's TetMesh.Save / saveMesh),
+        its compartments and patches included."""
+        from . import meshio
+
+        return meshio.loadMesh(path, scale, strict, device=device)[0]
+
+    def Save(self, path) -> None:
+        from . import meshio
+
+        meshio.saveMesh(path, self)
 
     # --- regions of interest (Tetmesh::addROI, src/steps/geom/tetmesh.cpp) ---
     def addROI(self, id: str, type: int, indices) -> None:
@@ -370,53 +459,20 @@ def kuhn_cube(n: int, side: float, origin=(0.0, 0.0, 0.0), device: bool = False)
     return Tetmesh(verts, tets.reshape(-1, 4), device=device)
 
 
-def load_gmsh(path, scale: float = 1.0):
+def load_gmsh(path, scale: float = 1.0, device: bool = False):
     """Read a Gmsh 2.2 ASCII mesh (the format of the reference's test/mesh fixtures,
     read there by TetMesh.LoadGmsh, pysteps/steps/API_2/geom.py). Returns the
     Tetmesh and {physical name: (dim, element indices)}: for dim 3 the indices
-    are tets, for dim 2 they are the mesh's triangle indices."""
-    lines = iter(open(path).read().splitlines())
-    names, nodes, tets, tet_tag, tris, tri_tag = {}, {}, [], [], [], []
-    for ln in lines:
-        ln = ln.strip()
-        if ln == "$PhysicalNames":
-            for _ in range(int(next(lines))):
-                d, tag, name = next(lines).split(maxsplit=2)
-                names[int(tag)] = (int(d), name.strip('"'))
-        elif ln == "$Nodes":
-            for _ in range(int(next(lines))):
-                p = next(lines).split()
-                nodes[int(p[0])] = [float(x) * scale for x in p[1:4]]
-        elif ln == "$Elements":
-            for _ in range(int(next(lines))):
-                p = [int(x) for x in next(lines).split()]
-                etype, ntag = p[1], p[2]
-                tag = p[3] if ntag > 0 else 0
-                vs = p[3 + ntag:]
-                if etype == 4:
-                    tets.append(vs)
-                    tet_tag.append(tag)
-                elif etype == 2:
-                    tris.append(vs)
-                    tri_tag.append(tag)
-    ids = sorted(nodes)
-    index = {v: i for i, v in enumerate(ids)}
-    verts = np.array([nodes[v] for v in ids])
-    mesh = Tetmesh(verts, np.array([[index[v] for v in t] for t in tets]))
+    are tets, for dim 2 they are the mesh's triangle indices (the file's
+    triangles are the mesh's first triangles, in file order)."""
+    from .meshio import importGmsh
+
+    mesh, _, tets, tris = importGmsh(path, scale, device=device)
     groups: dict[str, tuple[int, np.ndarray]] = {}
-    tet_tag = np.array(tet_tag)
-    for tag, (dim, name) in names.items():
-        if dim == 3:
-            groups[name] = (3, np.nonzero(tet_tag == tag)[0])
-    if tris:
-        keys = _face_keys(np.array([[index[v] for v in t] for t in tris]), len(verts))
-        mesh_keys = _face_keys(mesh.tris, len(verts))
-        order = np.argsort(mesh_keys)
-        pos = order[np.searchsorted(mesh_keys[order], keys)]
-        tri_tag = np.array(tri_tag)
-        for tag, (dim, name) in names.items():
-            if dim == 2:
-                groups[name] = (2, pos[tri_tag == tag])
+    for proxy, dim in ((tets, 3), (tris, 2)):
+        for key, ids in proxy.getGroups().items():
+            if key[0] == 0 and isinstance(key[1], str):
+                groups[key[1]] = (dim, np.asarray(ids, dtype=np.int64))
     return mesh, groups
 
 

@@ -35,8 +35,10 @@ def _same(host, dev):
 def test_gmsh_fixtures(name):
     print(f"case mesh build {name}", flush=True)
     host, _ = geom.load_gmsh(GOLDEN / name, 1e-6)
-    dev = geom.Tetmesh(host.verts, host.tets, device=True)
+    dev = geom.Tetmesh(host.verts, host.tets, host.supplied_tris, device=True)
     _same(host, dev)
+    dev2, _ = geom.load_gmsh(GOLDEN / name, 1e-6, device=True)
+    _same(host, dev2)
 
 
 def test_jittered_cube():
