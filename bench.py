@@ -114,7 +114,7 @@ def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False,
                 specialized=eng.specialized)
 
 
-def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True):
+def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True, specialize=False):
     fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=n_real, seed=seed,
                                                              end_time=end_time)
     eng = Engine(fm, meshes, [seed + 1] * n_real, reac_k=reac_k)
@@ -122,6 +122,8 @@ def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True):
         for s in range(3):
             eng.set_tet_counts(s, counts[r][s], real=r)
     eng.set_launch(block or 256, graph)
+    if block and specialize:
+        eng.set_specialize(True)
     # launch shape from the autotuner (block = 0), timed on this ensemble's own state
     tuned = autotune(eng, trial_periods=3, blocks=(64, 128, 256)) if not block else None
     # warm-up pass (captures and instantiates the graph), then the timed pass
@@ -148,9 +150,9 @@ def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True):
                 max_periods=max(eng.nsteps(r) for r in range(n_real)))
 
 
-def time_membrane(w, steps=300, warmup=5):
+def time_membrane(w, steps=300, warmup=5, block=0, specialize=False):
     """Calcium buffering with membrane pumps and influx on the 100^3 x 6 cube."""
-    p = time_plain(w, steps, warmup)
+    p = time_plain(w, steps, warmup, block=block, specialize=specialize)
     btet = model_bytes(w)
     achieved = btet * w.n_tets / (p["ms_launch"] * 1e-3) / 1e9
     traffic = measured_traffic(MEMBRANE_TRAFFIC)
@@ -363,6 +365,9 @@ def main():
                     help="skip the 6,000,000-tet membrane cube and its EField")
     ap.add_argument("--no-config0", action="store_true",
                     help="skip the 3,072-tet run-every-1-ms protocol")
+    ap.add_argument("--part", choices=("all", "plain", "batched", "membrane"), default="all",
+                    help="run one section only (counter profiling); with --block the section runs "
+                         "that fixed schedule instead of the autotuned one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -381,6 +386,27 @@ def main():
         print("bench.py measures one GPU: the whole mesh is resident on a single MI355X; "
               "multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if args.part != "all":
+        out = {"part": args.part}
+        if args.part == "plain":
+            w = W.four_species_cube(n=40, side=40e-6, seed=0, device=True)
+            p = time_plain(w, args.steps, args.warmup, block=args.block, graph=args.graph,
+                           specialize=args.specialize, spec_waves=args.spec_waves)
+            out.update(tet_steps_per_s=p["tet_steps"] / p["wall"], periods=p["periods"],
+                       kernel_ms_per_launch=p["ms_launch"], launches=p["launches"])
+        elif args.part == "batched":
+            ens = time_ensemble(block=args.block, graph=args.graph if args.block else True,
+                                specialize=args.specialize)
+            out.update(tet_steps_per_s=ens["tet_steps"] / ens["wall"], tet_steps=ens["tet_steps"],
+                       kernel_ms_per_launch=ens["ms_launch"], launches=ens["launches"],
+                       max_periods=ens["max_periods"], conserved=ens["conserved"])
+        else:
+            wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0, device=True)
+            out.update(time_membrane(wm, args.steps, args.warmup, block=args.block,
+                                     specialize=args.specialize))
+        print(json.dumps(out))
+        return
 
     # mesh topology built on the device (stg_mesh_build), as a large-mesh user would
     w = W.four_species_cube(n=40, side=40e-6, seed=0, device=True)

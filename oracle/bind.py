@@ -240,8 +240,9 @@ class OracleSim:
         extra = {}
         if tn:
             extra = dict(
-                tri_nbr=np.concatenate([np.where(m.tri_nbr >= 0, m.tri_nbr + toff[c], -1)
-                                        for c, m in enumerate(comp_meshes) if len(m.tri_tet)]),
+                tri_nbr=np.concatenate([np.where(m.tri_nbr >= 0, m.tri_nbr + toff[
+                    m.tri_nbr_comp if getattr(m, "tri_nbr_comp", None) is not None else c], -1)
+                    for c, m in enumerate(comp_meshes) if len(m.tri_tet)]),
                 tri_nbr_edge=np.concatenate([m.tri_nbr_edge for m in comp_meshes if len(m.tri_tet)]),
                 tri_edge_g=np.concatenate([m.tri_edge_g for m in comp_meshes if len(m.tri_tet)]))
         if any(m.tri_otet is not None for m in comp_meshes):

@@ -124,16 +124,20 @@ class TriElements:
     triangle i (its patch's surface systems hold it); vdep[q]: surface
     reaction q is voltage dependent (its table value is a factor on k(V))."""
 
-    def __init__(self, m: int, held: np.ndarray, vdep: np.ndarray, k_patch: np.ndarray, SS: int,
-                 sd_patch: np.ndarray | None = None):
+    def __init__(self, m: int, held: np.ndarray, vdep: np.ndarray, k_patch: np.ndarray, SS: int):
         self.m, self.SS = m, SS
         self.held = np.asarray(held, dtype=bool)          # [RS+NG][m]
         self.vdep = np.asarray(vdep, dtype=bool)          # [RS]
         self.k = np.asarray(k_patch, dtype=np.float64).copy()     # [RS][m] surface constants
         self.act = np.ones_like(self.held)                 # [RS+NG][m]
         self.clamp = None                                  # [m] uint32
-        self.eD = None                                     # [SS][m][3] per-edge constant
-        self.sd = None if sd_patch is None else np.asarray(sd_patch, dtype=np.float64).copy()  # [SS][m]
+        # surface diffusion (diffusion table in use when sD is not None)
+        self.sHeld = None    # [SS][m] the triangle's patch has a rule for the species (an SDiff)
+        self.sD = None       # [SS][m] the rule's constant
+        self.sDir = None     # [SS][m][3] per-edge constant, NaN = none
+        self.sBnd = None     # [SS][m][3] boundary edge active
+        self.edge_same = None    # [m][3] edge leads to a triangle of the same patch
+        self.ebnd = None         # [m][3] surface diffusion boundary of the edge, -1 none
 
     def sreac_table(self) -> np.ndarray:
         """[RS+NG][m]: the constant (or 1 for a k(V) reaction and a GHK current)
@@ -143,6 +147,57 @@ class TriElements:
         val[:RS] = np.where(self.vdep[:, None], 1.0, self.k)
         return np.where(self.held & self.act, val, 0.0)
 
+    # --- surface diffusion rules (SDiff, sdiff.cpp:42-90, 295-455, 599-616) ---
+    def sdiff_table_on(self, s_held, s_dcst, edge_same, ebnd) -> None:
+        """Start from the patches' state: s_held[q, i] the triangle's patch has
+        a diffusion rule for q, s_dcst[q, i] its constant; edge_same[i, e] the
+        edge leads to a triangle of the same patch; ebnd[i, e] the surface
+        diffusion boundary on the edge or -1. Boundary edges start closed."""
+        if self.sD is not None:
+            return
+        SS, m = self.SS, self.m
+        self.sHeld = np.asarray(s_held, dtype=bool).reshape(SS, m).copy()
+        self.sD = np.asarray(s_dcst, dtype=np.float64).reshape(SS, m).copy()
+        self.sDir = np.full((SS, m, 3), np.nan)
+        self.sBnd = np.zeros((SS, m, 3), dtype=bool)
+        self.edge_same = np.asarray(edge_same, dtype=bool).reshape(m, 3)
+        self.ebnd = np.asarray(ebnd).reshape(m, 3)
+
+    def set_sdcst(self, q: int, tris, dcst: float) -> None:
+        """SDiff::setDcst: the rule's constant, every directional constant cleared."""
+        self.sD[q, tris] = dcst
+        self.sDir[q, tris, :] = np.nan
+
+    def set_sdirection_dcst(self, q: int, i: int, e: int, dcst: float) -> None:
+        """SDiff::setDirectionDcst: one edge's constant; a boundary edge opens."""
+        self.sDir[q, i, e] = dcst
+        if self.ebnd[i, e] >= 0:
+            self.sBnd[q, i, e] = True
+
+    def set_sbnd_active(self, q: int, i: int, e: int, act: bool) -> None:
+        """SDiff::setSDiffBndActive on a boundary edge: a change of the flag
+        resets the rule with setDcst(its constant), which clears the
+        triangle's directional constants."""
+        if self.sBnd[q, i, e] != act:
+            self.sBnd[q, i, e] = act
+            self.sDir[q, i, :] = np.nan
+
+    def sdcst(self, q: int, i: int, e: int | None = None) -> float:
+        """SDiff::dcst(direction): the edge's directional constant if set,
+        else the rule's."""
+        if e is not None and not np.isnan(self.sDir[q, i, e]):
+            return float(self.sDir[q, i, e])
+        return float(self.sD[q, i])
+
+    def sdiff_table(self) -> np.ndarray:
+        """[SS][m][3] surface diffusion constant through each edge: the
+        directional constant or the rule's, on edges to the same patch and on
+        active boundary edges of triangles that hold the rule; else 0."""
+        base = np.where(np.isnan(self.sDir), self.sD[:, :, None], self.sDir)
+        open_ = self.edge_same[None, :, :] | ((self.ebnd >= 0)[None, :, :] & self.sBnd)
+        return np.where(open_ & self.sHeld[:, :, None], base, 0.0)
+
+    # --- clamps (Tri::setClamped) ---
     def set_clamped(self, q: int, tris, on: bool) -> None:
         if self.clamp is None:
             self.clamp = np.zeros(self.m, dtype=np.uint32)

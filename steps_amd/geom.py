@@ -162,6 +162,7 @@ class Tetmesh:
         self.patches: dict[str, TmPatch] = {}
         self.membs: dict[str, Memb] = {}
         self.diffbs: dict[str, DiffBoundary] = {}
+        self.sdiffbs: dict[str, SDiffBoundary] = {}
         self.rois: dict[str, tuple[int, np.ndarray]] = {}
         if device:
             for k, v in _device_topology(self.verts, self.tets, sup).items():
@@ -317,6 +318,64 @@ class Tetmesh:
         idx = np.nonzero(inside)[0]
         return int(idx[0]) if len(idx) else -1
 
+    # --- bars (Tetmesh::buildBarData, src/steps/geom/tetmesh.cpp:233-257): the
+    # edges of the triangles, numbered in order of first appearance over the
+    # triangles' (v0,v1), (v0,v2), (v1,v2), each stored with sorted vertices ---
+    def _bar_data(self):
+        if getattr(self, "_bars", None) is None:
+            t = self.tris
+            pairs = np.stack([t[:, [0, 1]], t[:, [0, 2]], t[:, [1, 2]]], axis=1).reshape(-1, 2)
+            keys = pairs[:, 0] * np.int64(len(self.verts)) + pairs[:, 1]
+            _, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+            order = np.argsort(first, kind="stable")
+            rank = np.empty(len(first), dtype=np.int64)
+            rank[order] = np.arange(len(first))
+            self._bars = pairs[first[order]]
+            self._tri_bars = rank[inv.reshape(-1)].reshape(-1, 3)
+        return self._bars, self._tri_bars
+
+    @property
+    def bars(self) -> np.ndarray:
+        return self._bar_data()[0]
+
+    @property
+    def tri_bars(self) -> np.ndarray:
+        return self._bar_data()[1]
+
+    def countBars(self) -> int:
+        return len(self.bars)
+
+    def getBar(self, b: int):
+        if not 0 <= b < self.countBars():
+            raise ValueError("Bar index is out of range.")
+        return [int(x) for x in self.bars[b]]
+
+    def getTriBars(self, t: int):
+        if not 0 <= t < self.countTris():
+            raise ValueError("Triangle index is out of range.")
+        return [int(x) for x in self.tri_bars[t]]
+
+    def getBarTriNeighbs(self, b: int):
+        """Triangles having bar b, in increasing order (Tetmesh::getBarTriNeighbs)."""
+        if not 0 <= b < self.countBars():
+            raise ValueError("Bar index is out of range.")
+        return np.nonzero(np.any(self.tri_bars == b, axis=1))[0].tolist()
+
+    def tri_edge_of_bar(self, t: int, b: int) -> int:
+        """Edge slot of bar b in triangle t in the engine's edge order, edge e
+        joining the triangle's vertices e and (e + 1) mod 3; -1 if t lacks b.
+        The reference's bar j of a triangle is the engine's edge (0, 2, 1)[j]."""
+        j = np.nonzero(self.tri_bars[t] == b)[0]
+        return int((0, 2, 1)[j[0]]) if len(j) else -1
+
+    def _tri_patch_index(self):
+        """(patch of each triangle as an index into the patch list or -1, the list)."""
+        pats = list(self.patches.values())
+        out = -np.ones(self.countTris(), dtype=np.int64)
+        for i, p in enumerate(pats):
+            out[p.tris] = i
+        return out, pats
+
 
 class TmComp:
     """Compartment: a set of tets with volume systems (src/steps/geom/tmcomp.hpp)."""
@@ -389,6 +448,70 @@ class DiffBoundary:
 
     def getComps(self):
         return [self.compA, self.compB]
+
+
+class SDiffBoundary:
+    """Bars between two patches across which chosen surface species may
+    diffuse once activated (src/steps/geom/sdiffboundary.cpp:37-115; API_1
+    steps.geom.SDiffBoundary(id, container, bars, patches)). Every bar must
+    carry exactly one triangle of each patch; repeated bars count once.
+    bar_tris[i] = (triangle of patches[0], triangle of patches[1]) on bars[i]."""
+
+    def __init__(self, id: str, container: Tetmesh, bars, patches):
+        bars = [int(b) for b in bars]
+        if not bars:
+            raise ValueError("No triangles provided to Surface Diffusion Boundary initializer function.")
+        patches = list(patches)
+        if len(patches) != 2:
+            raise ValueError("The number of patches provided to Surface Diffusion initializer "
+                             "function must be length 2.")
+        if id in container.sdiffbs:
+            raise ValueError(f"surface diffusion boundary {id!r} already exists")
+        pa, pb = patches
+        nbars = container.countBars()
+        owned = {b for s in container.sdiffbs.values() for b in s.bars.tolist()}
+        tri_patch, plist = container._tri_patch_index()
+        ia = next((i for i, p in enumerate(plist) if p is pa), -2)
+        ib = next((i for i, p in enumerate(plist) if p is pb), -2)
+        keep, pairs, seen = [], [], set()
+        for b in bars:
+            if b in seen:
+                continue
+            seen.add(b)
+            if not 0 <= b < nbars:
+                raise ValueError(f"Invalid bar index {b}.")
+            if b in owned:
+                raise ValueError(f"Bar with index {b} already belongs to a surface diffusion boundary.")
+            tris = container.getBarTriNeighbs(b)
+            side = []
+            for idx, p in ((ia, pa), (ib, pb)):
+                mine = [t for t in tris if tri_patch[t] == idx]
+                if len(mine) > 1:
+                    raise ValueError(f"Duplicate copy of patch{p.name} in connected triangles to bar {b}")
+                if not mine:
+                    raise ValueError(f"Patch{p.name} is not connected to bar {b}")
+                side.append(mine[0])
+            keep.append(b)
+            pairs.append(side)
+        self.id = id
+        self.mesh = container
+        self.bars = np.asarray(keep, dtype=np.int64)
+        self.bar_tris = np.asarray(pairs, dtype=np.int64).reshape(-1, 2)
+        self.patchA, self.patchB = pa, pb
+        container.sdiffbs[id] = self
+
+    def getID(self) -> str:
+        return self.id
+
+    def getAllBarIndices(self) -> np.ndarray:
+        return self.bars.copy()
+
+    def getPatches(self):
+        return [self.patchA, self.patchB]
+
+    def isBarInside(self, bars):
+        s = set(self.bars.tolist())
+        return [int(b) in s for b in bars]
 
 
 class Memb:
@@ -494,6 +617,7 @@ class MeshTables:
     tri_otet: np.ndarray | None = None   # (m,) int32 outer tet (local in its compartment), -1 none
     tri_ocomp: np.ndarray | None = None  # (m,) int32 compartment of the outer tet
     tri_nbr_comp: np.ndarray | None = None   # (m,3) int32 compartment of each edge neighbour
+    tri_bnd: np.ndarray | None = None    # (m,3) int32 surface diffusion boundary of each edge, -1 none
 
 
 def patch_tri_tables(mesh: Tetmesh, tris):
@@ -528,7 +652,7 @@ def patch_tri_tables(mesh: Tetmesh, tris):
     return nbr, nbe, g
 
 
-def comps_mesh_tables(mesh: Tetmesh, comps, patches, diffbs):
+def comps_mesh_tables(mesh: Tetmesh, comps, patches, diffbs, sdiffbs=()):
     """Tables of several compartments of one mesh (one MeshTables per
     compartment, local tet numbering in mesh order). A face between two
     compartments stays open only on a diffusion boundary: it then keeps its
@@ -540,7 +664,10 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patches, diffbs):
     index or None), or {inner compartment index: triangles}. A compartment's
     triangles are those of its patches in patch order; each triangle gets its
     outer tet when the patch has an outer compartment; surface diffusion joins
-    triangles of the same patch only."""
+    triangles of the same patch, and across each bar of a surface diffusion
+    boundary in ``sdiffbs`` the triangles of its two patches (in either
+    compartment: tri_nbr_comp names the neighbour's), the edge carrying
+    tri_bnd = the boundary's index (tetopsplit.cpp:540-628, 945-1018)."""
     if isinstance(patches, dict):
         patches = [(t, ci, None) for ci, t in patches.items()]
     ncomp = len(comps)
@@ -618,7 +745,47 @@ def comps_mesh_tables(mesh: Tetmesh, comps, patches, diffbs):
             bdir=np.ascontiguousarray(bdir, dtype=np.int32),
             tri_nbr=tn[0], tri_nbr_edge=tn[1], tri_edge_g=tn[2],
             tri_otet=tri_otet, tri_ocomp=tri_ocomp))
+    if sdiffbs:
+        _link_sdiff_boundaries(mesh, out, patches, sdiffbs)
     return out, dirs
+
+
+def _link_sdiff_boundaries(mesh: Tetmesh, out, patches, sdiffbs) -> None:
+    """Edge links across surface diffusion boundary bars, both ways: the
+    neighbour, its edge slot and compartment, and l / (A d) with l the bar
+    length and d the distance between the two barycentres (SDiff::SDiff,
+    sdiff.cpp:42-90). The triangle of a boundary patch holds no other link
+    across such a bar (one triangle of each patch per bar)."""
+    pos_c = -np.ones(mesh.countTris(), dtype=np.int64)
+    pos_l = -np.ones(mesh.countTris(), dtype=np.int64)
+    fill = {}
+    for t, ic, _ in patches:
+        t = np.asarray(t, dtype=np.int64)
+        pos_c[t] = ic
+        pos_l[t] = fill.get(ic, 0) + np.arange(len(t))
+        fill[ic] = fill.get(ic, 0) + len(t)
+    for ci, mt in enumerate(out):
+        if len(mt.tri_tet):
+            mt.tri_nbr_comp = np.full((len(mt.tri_tet), 3), ci, dtype=np.int32)
+            mt.tri_bnd = -np.ones((len(mt.tri_tet), 3), dtype=np.int32)
+    for bi, sdb in enumerate(sdiffbs):
+        for b, (ta, tb) in zip(sdb.bars, sdb.bar_tris):
+            if pos_c[ta] < 0 or pos_c[tb] < 0:
+                raise ValueError("a surface diffusion boundary triangle is not in a simulated patch")
+            ea, eb = mesh.tri_edge_of_bar(ta, b), mesh.tri_edge_of_bar(tb, b)
+            v = mesh.bars[b]
+            length = float(np.linalg.norm(mesh.verts[v[0]] - mesh.verts[v[1]]))
+            d = float(np.linalg.norm(mesh.tri_bary[ta] - mesh.tri_bary[tb]))
+            for (t, e, c, lo), (e2, c2, lo2) in (((ta, ea, pos_c[ta], pos_l[ta]), (eb, pos_c[tb], pos_l[tb])),
+                                                 ((tb, eb, pos_c[tb], pos_l[tb]), (ea, pos_c[ta], pos_l[ta]))):
+                mt = out[c]
+                mt.tri_bnd[lo, e] = bi
+                if d <= 0.0:
+                    continue
+                mt.tri_nbr[lo, e] = lo2
+                mt.tri_nbr_edge[lo, e] = e2
+                mt.tri_nbr_comp[lo, e] = c2
+                mt.tri_edge_g[lo, e] = length / (mesh.tri_area[t] * d)
 
 
 def mesh_tables(mesh: Tetmesh, comp: TmComp, patch_tris=None) -> MeshTables:

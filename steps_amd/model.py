@@ -318,6 +318,8 @@ class FlatModel:
     patch_sreacs: list = field(default_factory=list)
     patch_ghk: list = field(default_factory=list)
     patch_sspecs: list = field(default_factory=list)
+    # per patch its surface diffusion rules: rule id -> (surface species, constant)
+    patch_sdiffs: list = field(default_factory=list)
 
     def outer(self) -> bool:
         """The model reaches the outer side of a membrane."""
@@ -343,6 +345,23 @@ def _count(lst, names):
     for s in lst:
         v[names.index(s.id)] += 1
     return v
+
+
+def _patch_sdiffs(ssys, sspecs) -> dict:
+    """Surface diffusion rules of one patch (its surface systems): rule id ->
+    (surface species index, constant). Each rule is one SDiff kproc per
+    triangle (Patchdef, src/steps/solver/patchdef.cpp); one species with two
+    rules in a patch would diffuse at the sum of their rates, which this
+    engine does not model."""
+    out, seen = {}, set()
+    for ss in ssys:
+        for d in ss.diffs.values():
+            q = sspecs.index(d.lig.id)
+            if q in seen:
+                raise _notimpl(f"species {d.lig.id!r} has more than one surface diffusion rule in a patch")
+            seen.add(q)
+            out[d.id] = (q, float(d.dcst))
+    return out
 
 
 def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
@@ -430,8 +449,9 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
     fm.sspec_dcst = np.zeros(SS)
     for ss in ssys:
         for d in ss.diffs.values():
-            fm.sspec_dcst[sspecs.index(d.lig.id)] = d.dcst
-            fm.sdiffs[d.id] = sspecs.index(d.lig.id)
+            q = sspecs.index(d.lig.id)
+            fm.sspec_dcst[q] = max(fm.sspec_dcst[q], d.dcst)   # mobile if any patch's rule moves it
+            fm.sdiffs[d.id] = q
     ohm = [c for ss in ssys for c in ss.ohmiccurrs.values()]
     fm.ohmic = [c.id for c in ohm]
     fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
@@ -443,6 +463,7 @@ def flatten(model: Model, volsys_ids, surfsys_ids=()) -> FlatModel:
         fm.patch_sreacs = [set(range(n))]
         fm.patch_ghk = [set(range(len(ghk)))]
         fm.patch_sspecs = [set(sspecs)]
+        fm.patch_sdiffs = [_patch_sdiffs(ssys, sspecs)]
     return fm
 
 
@@ -575,8 +596,9 @@ def flatten_comps(model: Model, comps_volsys, patches=()) -> CompTables:
     fm.sspec_dcst = np.zeros(SS)
     for ss in ssys:
         for d in ss.diffs.values():
-            fm.sspec_dcst[sspecs.index(d.lig.id)] = d.dcst
-            fm.sdiffs[d.id] = sspecs.index(d.lig.id)
+            q = sspecs.index(d.lig.id)
+            fm.sspec_dcst[q] = max(fm.sspec_dcst[q], d.dcst)   # mobile if any patch's rule moves it
+            fm.sdiffs[d.id] = q
     ohm = [c for ss in ssys for c in ss.ohmiccurrs.values()]
     fm.ohmic = [c.id for c in ohm]
     fm.ohmic_chan = np.array([sspecs.index(c.chanstate.id) for c in ohm], dtype=np.int32)
@@ -594,5 +616,6 @@ def flatten_comps(model: Model, comps_volsys, patches=()) -> CompTables:
             ps.update(c.chanstate.id for c in ss.ghkcurrs.values())
             ps.update(d.lig.id for d in ss.diffs.values())
         fm.patch_sspecs.append(ps)
+        fm.patch_sdiffs.append(_patch_sdiffs(held, sspecs))
     comp_specs = [set(f.specs) for f in per]
     return CompTables(fm, comp_reacs, comp_diffs, comp_specs, comp_k, comp_dcst)

@@ -121,8 +121,14 @@ class TetOpSplit:
                 raise _argerr(f"a compartment of patch {p.name!r} is not in the geometry")
         nc = len(self.comps)
         self.diffbs = list(geom.diffbs.values())
+        self.sdiffbs = list(geom.sdiffbs.values())
+        self._sdbi = {b.id: i for i, b in enumerate(self.sdiffbs)}
+        for b in self.sdiffbs:
+            if b.patchA.name not in self._pi or b.patchB.name not in self._pi:
+                raise _argerr(f"a patch of surface diffusion boundary {b.id!r} has no surface system")
         self.patch = self.patches[0] if len(self.patches) == 1 else None
-        simple = nc == 1 and len(self.patches) <= 1 and all(p.ocomp is None for p in self.patches)
+        simple = nc == 1 and len(self.patches) <= 1 and all(p.ocomp is None for p in self.patches) \
+            and not self.sdiffbs
         if simple:
             ssys = self.patches[0].surfsys if self.patches else ()
             ptris = self.patches[0].tris if self.patches else None
@@ -141,7 +147,7 @@ class TetOpSplit:
             self.fm = ct.fm
             self.mts, self._bdirs = comps_mesh_tables(
                 geom, self.comps, [(p.tris, pl[i][1], pl[i][2]) for i, p in enumerate(self.patches)],
-                self.diffbs)
+                self.diffbs, self.sdiffbs)
             self._comp_reacs, self._comp_diffs = ct.comp_reacs, ct.comp_diffs
             self._comp_specs = ct.comp_specs
             self._ck, self._cd = ct.comp_k.copy(), ct.comp_dcst.copy()
@@ -183,13 +189,8 @@ class TetOpSplit:
             fill[r] += len(p.tris)
         self._rtri_patch = [self._tri_patch[self._real_tris(r)] for r in range(nc)]
         self.mt = self.mts[self._ci[self.patch.icomp.name]] if self.patch is not None else self.mts[0]
-        # patch-level surface constants and switches
-        self._psk = np.zeros((len(self.patches), RS))
-        self._psact = np.ones((len(self.patches), RS + NG), dtype=bool)
-        for pi in range(len(self.patches)):
-            for j in fm.patch_sreacs[pi]:
-                self._psk[pi, j] = fm.sreac_k[j]
-        self._sd = fm.sspec_dcst.copy()
+        self._ck0, self._cd0 = self._ck.copy(), self._cd.copy()
+        self._patch_defaults()
         self._seed = int(getattr(rng, "seed", 0))
         if simple:
             self.eng = Engine(fm, [self.mts[0]], [self._seed], mode=self._mode)
@@ -219,15 +220,65 @@ class TetOpSplit:
         # per-element state, created on first use
         self._vel = [None] * nc
         self._tel = [None] * nc
-        for r in range(nc):
-            if self._needs_tri_table(r):
-                self._tri_elements(r)
-                self._push_tri(r)
+        self._init_tri_state()
         self._host_rng = np.random.default_rng([self._seed, 0x5EED])
         self.ef = None
         self._efreal = None
         self._rd_time = 0.0
         self._ef_time = 0.0
+
+    def _patch_defaults(self):
+        """Patch-level surface constants and switches from the model: surface
+        reaction constants and activation, diffusion rule constants and, per
+        species, whether the patch has a rule for it (Patchdef::reset,
+        src/steps/solver/patchdef.cpp:1065-1100)."""
+        fm = self.fm
+        RS, NG, SS, NP = len(fm.sreacs), len(fm.ghk_chan), len(fm.sspecs), len(self.patches)
+        self._psk = np.zeros((NP, RS))
+        self._psact = np.ones((NP, RS + NG), dtype=bool)
+        self._psd = np.zeros((NP, SS))
+        self._psdh = np.zeros((NP, SS), dtype=bool)
+        for pi in range(NP):
+            for j in fm.patch_sreacs[pi]:
+                self._psk[pi, j] = fm.sreac_k[j]
+            for q, dcst in fm.patch_sdiffs[pi].values():
+                self._psd[pi, q], self._psdh[pi, q] = dcst, True
+
+    def _restore_defaults(self):
+        """The kproc resets of TetOpSplitP::reset (tetopsplit.cpp:1436-1503):
+        rate and diffusion constants back to the model's (Compdef::reset,
+        compdef.cpp:479-504; Patchdef::reset), every reaction, diffusion rule
+        and current active (Reac::reset, reac.cpp:96-115; Diff::reset,
+        diff.cpp:284-303; SDiff::reset, sdiff.cpp:295-315), directional
+        constants cleared, diffusion boundaries closed and clamps released
+        (Tet::reset and Tri::reset clear the pool flags)."""
+        nc = len(self.comps)
+        self._ck[:], self._cd[:] = self._ck0, self._cd0
+        self._cract[:] = True
+        self._cdact[:] = True
+        self._db_active.clear()
+        self._db_dcst.clear()
+        self._patch_defaults()
+        for ci in range(nc):
+            if self._vel[ci] is not None:
+                self.eng.set_tet_reac_table(None, real=ci)
+                self.eng.set_tet_diff_table(None, real=ci)
+                self.eng.set_tet_clamp(None, real=ci)
+                self._vel[ci] = None
+            self._push_reac(ci)
+            for sp in range(len(self.fm.specs)):
+                self.eng.set_dcst(sp, self._cd[ci, sp], real=ci)
+        for sp in range(len(self.fm.specs)):
+            self._push_boundaries(sp)
+        for r in range(nc):
+            if self._tel[r] is not None:
+                self.eng.set_tri_sreac_table(None, real=r)
+                self.eng.set_tri_sdiff_table(None, real=r)
+                self.eng.set_tri_clamp(None, real=r)
+                self._tel[r] = None
+            if len(self._rtri_patch[r]):
+                self._push_tri(r)
+        self._init_tri_state()
 
     def _real_tris(self, r):
         """Global indices of compartment r's triangles in the engine's order."""
@@ -241,6 +292,43 @@ class TetOpSplit:
         if len(pats) > 1:
             return True
         return len(pats) == 1 and len(self.fm.patch_ghk[pats[0]]) < len(self.fm.ghk_chan)
+
+    def _needs_sdiff_table(self, r) -> bool:
+        """Surface diffusion needs per-edge constants where a compartment's
+        triangles come from patches with different diffusion rules, or where
+        a surface diffusion boundary runs (its edges open per species)."""
+        pats = np.unique(self._rtri_patch[r])
+        if len(pats) == 0 or not self._psdh[pats].any():
+            return False
+        if self.mts[r].tri_bnd is not None and np.any(self.mts[r].tri_bnd >= 0):
+            return True
+        return len(pats) > 1 and (np.any(self._psdh[pats] != self._psdh[pats[0]])
+                                  or np.any(self._psd[pats] != self._psd[pats[0]]))
+
+    def _real_sd(self, r) -> np.ndarray:
+        """Realization-wide surface diffusion constants (no per-edge table):
+        its patch's rule constants, 0 for species without a rule."""
+        pats = np.unique(self._rtri_patch[r])
+        out = np.zeros(len(self.fm.sspecs))
+        for pi in pats:
+            out = np.where(self._psdh[pi], self._psd[pi], out)
+        return out
+
+    def _init_tri_state(self):
+        """Patch-level surface constants into the engine, and the per-triangle
+        tables of the compartments that need them from the start."""
+        for r in range(len(self.comps)):
+            if len(self._rtri_patch[r]) == 0:
+                continue
+            sd = self._real_sd(r)
+            for q in range(len(self.fm.sspecs)):
+                self.eng.set_sdcst(q, sd[q], real=r)
+            if self._needs_tri_table(r):
+                self._tri_elements(r)
+                self._push_tri(r)
+            if self._needs_sdiff_table(r):
+                self._sdiff_table_on(r)
+                self._push_sdiff(r)
 
     def _setup_efield(self, geom):
         """Membrane potential on the membrane's conduction volume
@@ -311,8 +399,11 @@ class TetOpSplit:
         back to DEFAULT_MEMB_POT. The random streams keep running, as the
         reference's RNG does, so reset-and-run loops give independent runs;
         re-initialising the RNG with a new seed before reset() re-keys every
-        compartment and restarts the streams from that seed. Constants,
-        activation flags and clamps stay as set."""
+        compartment and restarts the streams from that seed. Rate and
+        diffusion constants return to the model's values, every reaction,
+        diffusion rule and current is active again, diffusion boundaries
+        close and clamps are released, as the reference's kproc resets do."""
+        self._restore_defaults()
         seed = int(getattr(self.rng, "seed", self._seed))
         if seed != self._seed:
             self._seed = seed
@@ -542,16 +633,31 @@ class TetOpSplit:
 
     def _sdiff_table_on(self, r: int) -> TriElements:
         tel = self._tri_elements(r)
-        if tel.eD is None:
-            g = self.mts[r].tri_edge_g
-            tel.eD = np.where(g[None, :, :] > 0, self._sd[:, None, None], 0.0)
+        if tel.sD is None:
+            mt = self.mts[r]
+            pats = self._rtri_patch[r]
+            g = np.asarray(mt.tri_edge_g)
+            bnd = np.asarray(mt.tri_bnd) if mt.tri_bnd is not None else -np.ones_like(g, dtype=np.int64)
+            tel.sdiff_table_on(self._psdh[pats].T, self._psd[pats].T, (g > 0) & (bnd < 0),
+                               np.where(g > 0, bnd, -1))
         return tel
 
     def _push_sdiff(self, r: int):
         tel = self._tel[r]
-        if tel is not None and tel.eD is not None:
-            self.eng.set_tri_sdiff_table(tel.eD, real=r)
+        if tel is not None and tel.sD is not None:
+            self.eng.set_tri_sdiff_table(tel.sdiff_table(), real=r)
             self._push_tri(r)
+
+    def _tri_eD(self, r: int) -> np.ndarray:
+        """[SS][m][3] surface diffusion constant through each edge of
+        compartment r's triangles, as the engine runs it."""
+        tel = self._tel[r]
+        if tel is not None and tel.sD is not None:
+            return tel.sdiff_table()
+        if self.mts[r].tri_edge_g is None:
+            return np.zeros((len(self.fm.sspecs), len(self.mts[r].tri_tet), 3))
+        g = np.asarray(self.mts[r].tri_edge_g)
+        return np.where(g[None] > 0, self._real_sd(r)[:, None, None], 0.0)
 
     # ------------------------------------------------------------- compartments
     def getNComps(self) -> int:
@@ -816,6 +922,90 @@ class TetOpSplit:
                         vel.set_direction_dcst(sp, t, f, float(dcst))
                 self._push_diff(src)
         self._push_boundaries(sp)
+
+    # ----------------------------- surface diffusion boundaries (3023-3200)
+    def _sdb(self, sdb: str) -> int:
+        if sdb not in self._sdbi:
+            raise _argerr(f"unknown surface diffusion boundary {sdb!r}")
+        return self._sdbi[sdb]
+
+    def _sdb_spec(self, bi: int, s: str) -> int:
+        b = self.sdiffbs[bi]
+        for p in (b.patchA, b.patchB):
+            if s not in self.fm.patch_sspecs[self._pi[p.name]]:
+                raise _argerr("Species undefined in patches connected by surface diffusion boundary.")
+        return self.fm.sspecs.index(s)
+
+    def _sdb_edges(self, bi: int):
+        """(global triangle, compartment, index there, patch, edge slot) of
+        every boundary triangle, per bar the patchA triangle then the patchB
+        one (SDiffBoundary::getTris / getTriDirection order)."""
+        b = self.sdiffbs[bi]
+        for bar, pair in zip(b.bars, b.bar_tris):
+            for t in pair:
+                r, i, pi = self._tri(int(t))
+                yield int(t), r, i, pi, self.geom.tri_edge_of_bar(int(t), int(bar))
+
+    def setSDiffBoundarySpecDiffusionActive(self, sdb: str, s: str, act: bool):
+        """_setSDiffBoundarySpecDiffusionActive (tetopsplit.cpp:3023-3078):
+        SDiff::setSDiffBndActive on the boundary edge of every boundary
+        triangle whose patch has a diffusion rule for s."""
+        bi = self._sdb(sdb)
+        q = self._sdb_spec(bi, s)
+        touched = set()
+        for _, r, i, pi, e in self._sdb_edges(bi):
+            if not self._psdh[pi, q]:
+                continue
+            self._sdiff_table_on(r).set_sbnd_active(q, i, e, bool(act))
+            touched.add(r)
+        for r in touched:
+            self._push_sdiff(r)
+
+    def getSDiffBoundarySpecDiffusionActive(self, sdb: str, s: str) -> bool:
+        """_getSDiffBoundarySpecDiffusionActive (tetopsplit.cpp:3081-3140): the
+        flag of the last boundary triangle (in boundary order) whose patch has
+        a diffusion rule for s; true when there is none."""
+        bi = self._sdb(sdb)
+        q = self._sdb_spec(bi, s)
+        active = True
+        for _, r, i, pi, e in self._sdb_edges(bi):
+            if not self._psdh[pi, q]:
+                continue
+            tel = self._tel[r]
+            active = bool(tel is not None and tel.sBnd is not None and tel.sBnd[q, i, e])
+        return active
+
+    def setSDiffBoundarySpecDcst(self, sdb: str, s: str, dcst: float, direction_patch=""):
+        """_setSDiffBoundarySpecDcst (tetopsplit.cpp:3143-3200):
+        SDiff::setDirectionDcst on the boundary edge of every boundary
+        triangle, which opens it; with direction_patch, only on the triangles
+        of the other patch (diffusion toward direction_patch)."""
+        bi = self._sdb(sdb)
+        q = self._sdb_spec(bi, s)
+        if dcst < 0:
+            raise _argerr("diffusion constant must be >= 0")
+        skip = -1
+        if direction_patch not in (None, ""):
+            name = direction_patch if isinstance(direction_patch, str) else direction_patch.name
+            skip = self._check_patch(name)
+        touched = set()
+        for _, r, i, pi, e in self._sdb_edges(bi):
+            if pi == skip or not self._psdh[pi, q]:
+                continue
+            self._sdiff_table_on(r).set_sdirection_dcst(q, i, e, float(dcst))
+            touched.add(r)
+        for r in touched:
+            self._push_sdiff(r)
+
+    # deprecated names (src/steps/solver/api_deprecated.cpp:560-590)
+    def setSDiffBoundaryDiffusionActive(self, sdb: str, s: str, act: bool):
+        self.setSDiffBoundarySpecDiffusionActive(sdb, s, act)
+
+    def getSDiffBoundaryDiffusionActive(self, sdb: str, s: str) -> bool:
+        return self.getSDiffBoundarySpecDiffusionActive(sdb, s)
+
+    def setSDiffBoundaryDcst(self, sdb: str, s: str, dcst: float, direction_patch=""):
+        self.setSDiffBoundarySpecDcst(sdb, s, dcst, direction_patch)
 
     # deprecated spellings (cysteps_solver.pyx)
     def setDiffBoundaryDiffusionActive(self, db: str, s: str, act: bool):
@@ -1437,34 +1627,55 @@ class TetOpSplit:
         h, c = self._sreac_hc(pi, self._sreac(r, pi))
         return float((h * c).sum())
 
-    def _sdiff(self, d: str) -> int:
+    def _sdiff(self, d: str, pi: int | None = None) -> int:
+        """Surface species of diffusion rule d; with pi, the rule must be one of
+        patch pi's (Patchdef::surfdiffG2L)."""
         if d not in self.fm.sdiffs:
             raise _argerr(f"unknown surface diffusion rule {d!r}")
+        if pi is not None and d not in self.fm.patch_sdiffs[pi]:
+            raise _argerr(f"Diffusion rule {d!r} undefined in patch {self.patches[pi].name!r}.")
         return self.fm.sdiffs[d]
 
+    def _tri_edge_nbr_patch(self, r: int, i: int, e: int) -> int:
+        """Patch of the triangle across edge e of compartment r's triangle i, or -1."""
+        mt = self.mts[r]
+        if mt.tri_nbr is None or mt.tri_nbr[i, e] < 0:
+            return -1
+        rn = int(mt.tri_nbr_comp[i, e]) if mt.tri_nbr_comp is not None else r
+        return int(self._rtri_patch[rn][mt.tri_nbr[i, e]])
+
     def getPatchSDiffD(self, p: str, d: str) -> float:
-        self._check_patch(p)
-        return float(self._sd[self._sdiff(d)])
+        """The rule's constant in the patch: its model value or the last
+        setPatchSDiffD."""
+        pi = self._check_patch(p)
+        return float(self._psd[pi, self._sdiff(d, pi)])
 
     def setPatchSDiffD(self, p: str, d: str, dcst: float, direction_patch=None):
-        """Surface diffusion constant of a rule on every triangle of the patch
-        (_setPatchSDiffD; directional constants need surface diffusion
-        boundaries, which this engine does not model)."""
+        """SDiff::setDcst on every triangle of the patch; with direction_patch,
+        SDiff::setDirectionDcst on the patch's edges toward that patch's
+        triangles (which opens surface diffusion boundary edges)."""
         pi = self._check_patch(p)
-        if direction_patch not in (None, ""):
-            raise _notimpl("surface diffusion boundaries are not supported")
+        q = self._sdiff(d, pi)
         if dcst < 0:
             raise _argerr("diffusion constant must be >= 0")
-        q = self._sdiff(d)
         r, loc = self._patch_local(pi)
-        tel = self._tel[r]
-        if tel is not None and tel.eD is not None or len(np.unique(self._rtri_patch[r])) > 1:
-            tel = self._sdiff_table_on(r)
-            tel.eD[q, loc, :] = np.where(self.mts[r].tri_edge_g[loc] > 0, dcst, 0.0)
-            self._push_sdiff(r)
-        else:
-            self.eng.set_sdcst(q, dcst, real=r)
-            self._sd[q] = dcst
+        if direction_patch in (None, ""):
+            self._psd[pi, q] = dcst
+            tel = self._tel[r]
+            if (tel is not None and tel.sD is not None) or self._needs_sdiff_table(r):
+                tel = self._sdiff_table_on(r)
+                tel.set_sdcst(q, loc, dcst)
+                self._push_sdiff(r)
+            else:
+                self.eng.set_sdcst(q, dcst, real=r)
+            return
+        dp = self._check_patch(direction_patch if isinstance(direction_patch, str) else direction_patch.name)
+        tel = self._sdiff_table_on(r)
+        for i in loc:
+            for e in range(3):
+                if self._tri_edge_nbr_patch(r, int(i), e) == dp:
+                    tel.set_sdirection_dcst(q, int(i), e, float(dcst))
+        self._push_sdiff(r)
 
     def getTriArea(self, tri: int) -> float:
         if not 0 <= tri < self.geom.countTris():
@@ -1571,37 +1782,43 @@ class TetOpSplit:
         return h * c
 
     def _tri_edge_toward(self, tri: int, other: int) -> int:
-        rr, i, pi = self._tri(tri)
-        ro, io, po = self._tri(other) if 0 <= other < len(self._tri_local) and self._tri_local[other] >= 0 \
-            else (-1, -1, -1)
-        e = np.nonzero((self.mts[rr].tri_nbr[i] == io) & (ro == rr))[0] if io >= 0 else []
-        if len(e) == 0:
+        """Edge slot of triangle tri toward patch triangle other, the two
+        sharing a bar (Tri::getTriDirection over the neighbours found through
+        the bars, tetopsplit.cpp:575-600)."""
+        self._tri(tri)
+        g = self.geom
+        ok = 0 <= other < g.countTris() and other != tri and self._tri_patch[other] >= 0
+        shared = set(g.tri_bars[tri].tolist()) & set(g.tri_bars[other].tolist()) if ok else set()
+        if not shared:
             raise _argerr(f"Triangle {other} is not a neighbor of triangle {tri}.")
-        return int(e[0])
+        return g.tri_edge_of_bar(tri, shared.pop())
 
     def getTriSDiffD(self, tri: int, d: str, direction_tri=None) -> float:
+        """_getTriSDiffD (tetopsplit.cpp:4534-4566): the rule's constant on the
+        triangle, or the directional constant toward direction_tri if one is
+        set (SDiff::dcst)."""
         rr, i, pi = self._tri(tri)
-        q = self._sdiff(d)
+        q = self._sdiff(d, pi)
+        e = None if direction_tri is None or direction_tri < 0 else self._tri_edge_toward(tri, int(direction_tri))
         tel = self._tel[rr]
-        if tel is None or tel.eD is None:
-            return float(self._sd[q])
-        if direction_tri is None or direction_tri < 0:
-            e = np.nonzero(self.mts[rr].tri_edge_g[i] > 0)[0]
-            return float(tel.eD[q, i, e[0]]) if len(e) else float(self._sd[q])
-        return float(tel.eD[q, i, self._tri_edge_toward(tri, int(direction_tri))])
+        if tel is None or tel.sD is None:
+            return float(self._psd[pi, q])
+        return tel.sdcst(q, i, e)
 
     def setTriSDiffD(self, tri: int, d: str, dcst: float, direction_tri=None):
-        """_setTriSDiffD: the rule's constant on one triangle, or through the
-        edge toward direction_tri (SDiff::setDcst / setDirectionDcst)."""
+        """_setTriSDiffD (tetopsplit.cpp:4570-4600): the rule's constant on one
+        triangle (directional constants cleared), or through the edge toward
+        direction_tri (a boundary edge opens)."""
         rr, i, pi = self._tri(tri)
-        q = self._sdiff(d)
+        q = self._sdiff(d, pi)
         if dcst < 0:
             raise _argerr("diffusion constant must be >= 0")
+        e = None if direction_tri is None or direction_tri < 0 else self._tri_edge_toward(tri, int(direction_tri))
         tel = self._sdiff_table_on(rr)
-        if direction_tri is None or direction_tri < 0:
-            tel.eD[q, i, :] = np.where(self.mts[rr].tri_edge_g[i] > 0, dcst, 0.0)
+        if e is None:
+            tel.set_sdcst(q, [i], float(dcst))
         else:
-            tel.eD[q, i, self._tri_edge_toward(tri, int(direction_tri))] = dcst
+            tel.set_sdirection_dcst(q, i, e, float(dcst))
         self._push_sdiff(rr)
 
     # batch access to triangles
@@ -2026,11 +2243,6 @@ NOT_PROVIDED = {
     **{m: "ROI extents are not implemented for this solver" for m in (
         "getROIReacExtent", "resetROIReacExtent", "getROISReacExtent", "resetROISReacExtent",
         "getROIDiffExtent", "resetROIDiffExtent")},
-    # surface diffusion boundaries
-    **{m: "surface diffusion boundaries are not supported" for m in (
-        "setSDiffBoundaryDiffusionActive", "getSDiffBoundaryDiffusionActive", "setSDiffBoundaryDcst",
-        "setSDiffBoundarySpecDiffusionActive", "getSDiffBoundarySpecDiffusionActive",
-        "setSDiffBoundarySpecDcst")},
     # the whole-state total propensity (API::getA0; TetOpSplitP does not provide it)
     "getA0": "the total propensity of the state is not provided by operator splitting",
     # DistTetOpSplit's Gibson-Bruck dependency graph and its mesh-bandwidth file

@@ -314,3 +314,44 @@ def two_patch_membrane(n=4, side=4e-6, k_in=2e9, k_out=1e9, k_leak=3e9, dA=1e-11
     M.SReac("flop", ssb, slhs=[T3], srhs=[T2], kcst=2.0)
     M.Diff("dT2", ssb, T2, dT2)
     return mdl, mesh, {"inner": ci, "outer": co}, {"pa": pa, "pb": pb}
+
+
+def sdiff_boundary_patches(n=4, side=4e-6, dX=1e-12, split=False):
+    """The top face (z = side) of a Kuhn cube split at x = side/2 into patch
+    'patchA' (x > side/2, surface system 'ssys1': A -> A at 1e5 /s and X
+    diffusing by rule 'diffX1') and patch 'patchB' (surface system 'ssys2': X
+    diffusing by rule 'diffX2'), both at dX, joined by the surface diffusion
+    boundary 'sdiffb' on the bars they share: the set-up of the reference's
+    test_sdiff_bugfix.py on a cube. split=False: one compartment 'cyto' holds
+    both patches; split=True: compartments 'left' and 'right' (tets by
+    barycentre x) hold patchB and patchA, so the boundary joins triangles of
+    two compartments. The compartments hold a volume species V diffusing at
+    1e-12. Returns (model, mesh, comps, patches, sdiffb)."""
+    mesh = geom.kuhn_cube(n, side)
+    if split:
+        right = np.nonzero(mesh.tet_bary[:, 0] >= side / 2)[0]
+        left = np.nonzero(mesh.tet_bary[:, 0] < side / 2)[0]
+        comps = {"left": geom.TmComp("left", mesh, left), "right": geom.TmComp("right", mesh, right)}
+        ca, cb = comps["right"], comps["left"]
+    else:
+        comps = {"cyto": geom.TmComp("cyto", mesh)}
+        ca = cb = comps["cyto"]
+    for c in comps.values():
+        c.addVolsys("vsys")
+    top = np.nonzero(np.all(np.isclose(mesh.verts[mesh.tris][:, :, 2], side), axis=1))[0]
+    xa = mesh.tri_bary[top, 0] > side / 2
+    pa = geom.TmPatch("patchA", mesh, top[xa], ca)
+    pb = geom.TmPatch("patchB", mesh, top[~xa], cb)
+    pa.addSurfsys("ssys1")
+    pb.addSurfsys("ssys2")
+    bars = sorted(set(mesh.tri_bars[pa.tris].ravel().tolist()) & set(mesh.tri_bars[pb.tris].ravel().tolist()))
+    sdb = geom.SDiffBoundary("sdiffb", mesh, bars, [pa, pb])
+    mdl = M.Model()
+    A, X, V = M.Spec("A", mdl), M.Spec("X", mdl), M.Spec("V", mdl)
+    vsys = M.Volsys("vsys", mdl)
+    M.Diff("dV", vsys, V, 1e-12)
+    s1, s2 = M.Surfsys("ssys1", mdl), M.Surfsys("ssys2", mdl)
+    M.SReac("sreac", s1, slhs=[A], srhs=[A], kcst=1e5)
+    M.Diff("diffX1", s1, X, dX)
+    M.Diff("diffX2", s2, X, dX)
+    return mdl, mesh, comps, {"patchA": pa, "patchB": pb}, sdb

@@ -72,8 +72,7 @@ def _oracle(sim, seed):
         if m:
             assert tel is not None      # several patches on one compartment: table mode
             sk.append(tel.sreac_table())
-            eD.append(tel.eD if tel.eD is not None
-                      else np.where(mts[c].tri_edge_g[None] > 0, sim._sd[:, None, None], 0.0))
+            eD.append(sim._tri_eD(c))
             tclamp.append(tel.clamp if tel.clamp is not None else np.zeros(m, np.uint32))
     o.set_tables(tet_k=np.concatenate(k, axis=1), tet_fD=np.concatenate(fD, axis=1),
                  tet_clamp=np.concatenate(clamp), tri_sk=np.concatenate(sk, axis=1),
@@ -285,3 +284,76 @@ def test_per_triangle_erev_and_capacitance_match_membrane_wide_values():
         runs.append(v.copy())
     assert np.allclose(runs[0], runs[1], rtol=1e-9, atol=1e-15)
     assert np.any(runs[0] != -65e-3)
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_sdiff_boundary_bit_exact(split):
+    """The set-up of the reference's test_sdiff_bugfix.py on a cube (one
+    compartment, or the two patches on two compartments): the boundary opened
+    for X, 0.1 x the constant toward patchA; counts equal the oracle's, X is
+    conserved, patchA never holds A, and closing the boundary again stops the
+    crossing."""
+    mdl, mesh, comps, patches, sdb = W.sdiff_boundary_patches(split=split)
+    sim = TetOpSplit(mdl, mesh, _rng(11))
+    print(f"case sdiff boundary split={split}", flush=True)
+    pa, pb = patches["patchA"].tris, patches["patchB"].tris
+    sim.setPatchSpecCount("patchB", "X", 40 * len(pb))
+    assert not sim.getSDiffBoundarySpecDiffusionActive("sdiffb", "X")
+    sim.setSDiffBoundarySpecDiffusionActive("sdiffb", "X", True)
+    assert sim.getSDiffBoundarySpecDiffusionActive("sdiffb", "X")
+    sim.setSDiffBoundarySpecDcst("sdiffb", "X", 1e-13, "patchA")
+    ta, tb = (int(t) for t in sdb.bar_tris[0])
+    assert sim.getTriSDiffD(tb, "diffX2", ta) == 1e-13 and sim.getTriSDiffD(tb, "diffX2") == 1e-12
+    assert sim.getTriSDiffD(ta, "diffX1", tb) == 1e-12
+    with pytest.raises(ValueError):
+        sim.getTriSDiffD(ta, "diffX2")                      # the rule of the other patch
+    with pytest.raises(ValueError):
+        sim.setSDiffBoundarySpecDiffusionActive("sdiffb", "V", True)   # not a species of the patches
+    o, off, toff = _oracle(sim, 11)
+    for t in (0.5, 2.0, 5.0):
+        _check(sim, o, off, toff, t)
+    n = 40 * len(pb)
+    assert sim.getPatchSpecCount("patchA", "X") + sim.getPatchSpecCount("patchB", "X") == n
+    assert sim.getPatchSpecCount("patchA", "X") > 0 and sim.getPatchSpecCount("patchA", "A") == 0
+    # the deprecated spelling closes it again; nothing crosses afterwards
+    sim.setSDiffBoundaryDiffusionActive("sdiffb", "X", False)
+    assert not sim.getSDiffBoundaryDiffusionActive("sdiffb", "X")
+    na = sim.getPatchSpecCount("patchA", "X")
+    sim.run(8.0)
+    assert sim.getPatchSpecCount("patchA", "X") == na
+
+
+def test_reset_restores_the_model_constants():
+    """TetOpSplitP::reset runs every kproc's reset: constants back to the
+    model's, reactions and diffusion active, clamps released, boundaries
+    closed. After reset the run equals an oracle built from the model."""
+    sim, mesh, comps, patches = _two_patch_sim(13)
+    inner = comps["inner"].tets
+    pb = patches["pb"].tris
+    sim.setCompReacK("inner", "bind", 3e6)
+    sim.setTetReacActive(int(inner[0]), "unbind", False)
+    sim.setCompDiffD("outer", "dA", 4e-11)
+    sim.setTetSpecClamped(int(inner[1]), "Ca", True)
+    sim.setPatchSReacK("pb", "leak", 1e9)
+    sim.setTriSDiffD(int(pb[0]), "dT2", 5e-13)
+    sim.setPatchSpecClamped("pa", "Ch", True)
+    print("case reset restores constants", flush=True)
+    sim.reset()
+    assert sim.getCompReacK("inner", "bind") == 1e6
+    assert sim.getTetReacActive(int(inner[0]), "unbind")
+    assert sim.getCompDiffD("outer", "dA") == 1e-11
+    assert not sim.getTetSpecClamped(int(inner[1]), "Ca")
+    assert sim.getPatchSReacK("pb", "leak") == 3e9
+    assert sim.getTriSDiffD(int(pb[0]), "dT2") == 2e-13
+    assert not sim.getPatchSpecClamped("pa", "Ch")
+    sim.setCompSpecCount("outer", "A", 3000)
+    sim.setCompSpecCount("inner", "B", 800)
+    sim.setPatchSpecCount("pb", "T2", 3 * len(pb))
+    sim.setPatchSpecCount("pa", "T", 2 * len(patches["pa"].tris))
+    o, off, toff = _oracle(sim, 13)
+    # the oracle's tables are the model's
+    ref, *_ = _two_patch_sim(13)
+    assert np.array_equal(sim._ck, ref._ck) and np.array_equal(sim._cd, ref._cd)
+    assert all(np.array_equal(sim._tri_eD(c), ref._tri_eD(c)) for c in range(len(sim.mts)))
+    for t in (0.01, 0.03):
+        _check(sim, o, off, toff, t)
