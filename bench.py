@@ -111,7 +111,7 @@ def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False,
     return dict(eng=eng, wall=wall, periods=periods, tet_steps=periods * w.n_tets,
                 ms_launch=ms_launch, launches=launches, dt=dt, tuned=tuned,
                 block=tuned["block"] if tuned else block, graph=tuned["graph"] if tuned else graph,
-                specialized=eng.specialized)
+                specialized=eng.specialized, lanes=eng.lanes)
 
 
 def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True, specialize=False):
@@ -165,7 +165,8 @@ def time_membrane(w, steps=300, warmup=5, block=0, specialize=False):
         achieved_gbs=achieved, frac=achieved / HBM_PEAK_GBS,
         traffic=traffic,
         traffic_frac=(traffic / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
-        schedule={"block": p["block"], "graph": bool(p["graph"]), "specialized_kernel": bool(p["specialized"])})
+        schedule={"block": p["block"], "graph": bool(p["graph"]), "specialized_kernel": bool(p["specialized"]),
+                  "lanes_per_tet": p["lanes"]})
 
 
 def time_efield(w, ef_steps=10, ef_dt=1e-5):
@@ -291,9 +292,13 @@ def config0_protocol_oracle(n_seeds, budget_s):
 def config0_protocol_gpu(n_seeds=100):
     """The same protocol on the GPU: one engine per seed run one after another
     (plain), and all seeds as one batched engine (each seed a realization with its
-    own Philox key); totals reduced on the device at every 1 ms save."""
+    own Philox key); totals reduced on the device at every 1 ms save. Each engine's
+    launch shape is autotuned on its initial state first (not timed)."""
     ws = [W.ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=sd) for sd in range(n_seeds)]
     eng = Engine(ws[0].fm, [ws[0].mt], [0])
+    for s_ in range(3):
+        eng.set_tet_counts(s_, ws[0].counts[s_])
+    tuned_plain = autotune(eng, trial_periods=10, blocks=(64, 128, 256))
     tic = time.perf_counter()
     steps = 0
     for sd, w in enumerate(ws[:10]):
@@ -310,14 +315,18 @@ def config0_protocol_gpu(n_seeds=100):
     for r, w in enumerate(ws):
         for s_ in range(3):
             beng.set_tet_counts(s_, w.counts[s_], real=r)
+    tuned_batch = autotune(beng, trial_periods=10, blocks=(64, 128, 256))
     tic = time.perf_counter()
     for k in range(1, 101):
         beng.run(k * 1e-3)
         [beng.sum_tet_counts(s_, r) for r in range(n_seeds) for s_ in range(3)]
     wall = time.perf_counter() - tic
     steps = sum(beng.nsteps(r) * w.n_tets for r, w in enumerate(ws))
+    shape = lambda t: {"block": t["block"], "graph": t["graph"], "specialized_kernel": t["specialized"],
+                       "lanes_per_tet": t["lanes"]}
     return dict(plain_one_seed_at_a_time_tet_steps_per_s=plain, plain_seeds=10,
-                batched_tet_steps_per_s=steps / wall, batched_seeds=n_seeds, batched_wall_s=wall)
+                batched_tet_steps_per_s=steps / wall, batched_seeds=n_seeds, batched_wall_s=wall,
+                plain_schedule=shape(tuned_plain), batched_schedule=shape(tuned_batch))
 
 
 def cpu_baseline(w, budget_s=15.0, seed=1, membrane=None, n_seeds0=100):
@@ -439,6 +448,7 @@ def main():
             "block": p["block"],
             "graph": bool(p["graph"]),
             "specialized_kernel": bool(p["specialized"]),
+            "lanes_per_tet": p["lanes"],
             "spec_min_waves": p["tuned"].get("spec_min_waves") if p["tuned"] else (
                 args.spec_waves if args.specialize else None),
             "schedule": "autotuned" if p["tuned"] else "fixed",
@@ -483,9 +493,12 @@ def main():
             "plain_same_model": "ab_binding_cube n=13 (13,182 tets), one realization",
             "schedule": {"block": sched.get("block"), "graph": sched.get("graph"),
                          "specialized_kernel": sched.get("specialized"),
+                         "lanes_per_tet": sched.get("lanes"),
                          "spec_min_waves": sched.get("spec_min_waves"),
                          "autotune_wall_ms_per_launch": sched.get("timings")},
-            "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"])},
+            "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"]),
+                                          "specialized_kernel": bool(ps["specialized"]),
+                                          "lanes_per_tet": ps["lanes"]},
         }
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])

@@ -301,6 +301,19 @@ int stg_set_tri_clamp(stg_sim *sim, int32_t real, const uint32_t *mask);
  * No reference counterpart (device tuning). */
 int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
 
+/* Lanes per tet of the period kernel (no reference counterpart; device
+ * tuning): 1 (one lane runs a tet's whole period) or 2, 4, 8 (the tile
+ * schedule, for volume-only models on meshes too small to fill the GPU: the
+ * lanes of a tet split its species for the entering gather, the leaving pass
+ * and the stores, and one of them runs its SSA). A block then holds
+ * block / lanes tets. Results are identical for every value. A model that
+ * needs the membrane variant (surface reactions, GHK currents, channel
+ * integrals, diffusion boundaries, per-element tables) returns
+ * STG_ERR_NOTIMPL for lanes > 1, and a run goes back to one lane per tet when
+ * the model comes to need it. */
+int stg_set_lanes(stg_sim *sim, int32_t lanes);
+int stg_get_lanes(const stg_sim *sim, int32_t *lanes);
+
 /* Model-specialised kernel (no reference counterpart; device tuning): on = 1
  * compiles, at run time with hipRTC (about a second), a period kernel with
  * this model's reaction network as compile-time constants and uses it for

@@ -94,6 +94,7 @@ struct stg_sim {
     int n_real = 0;
     int S = 0, R = 0, SS = 0, RS = 0, ND = 0;
     int block = 256;
+    int lanes = 1;  // lanes per tet (tile schedule for volume models when > 1)
     bool use_graph = false;
     bool xcd_remap = true;
     int diff_threshold = 10;  // diffusion_operator.hpp:99, tetopsplit.hpp:995
@@ -180,13 +181,14 @@ struct stg_sim {
     double *d_gather = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    // model-specialised volume kernel (blocks of up to 256), compiled at run time
+    // model-specialised volume kernel (blocks of up to 256), compiled at run time;
     // one compiled module per (membrane variant, waves-per-SIMD bound; 0 = the
-    // compiler's choice)
-    hipModule_t rtc_mods[2][9] = {};
-    hipFunction_t rtc_fns[2][9] = {};
+    // compiler's choice, lanes per tet 1/2/4/8)
+    hipModule_t rtc_mods[2][9][4] = {};
+    hipFunction_t rtc_fns[2][9][4] = {};
     hipFunction_t rtc_fn = nullptr;  // the one in use, or null for the generic kernel
     bool rtc_memb = false;           // whether rtc_fn is the membrane variant
+    int rtc_waves = 0;               // its waves-per-SIMD bound
     hipGraphExec_t graph_exec = nullptr;
     hipGraph_t graph = nullptr;
     int graph_launches = -1;
@@ -283,9 +285,10 @@ struct stg_sim {
 
     ~stg_sim() {
         free_graph();
-        for (auto &row : rtc_mods)
-            for (hipModule_t m : row)
-                if (m) (void)hipModuleUnload(m);
+        for (auto &plane : rtc_mods)
+            for (auto &row : plane)
+                for (hipModule_t m : row)
+                    if (m) (void)hipModuleUnload(m);
         void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
                         d_out[0],    d_out[1],      d_wide[0],     d_wide[1],     d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_desc, d_real_tet0,  d_real_k,      d_real_sk,
@@ -1014,12 +1017,23 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
 // hold a surface reaction, a GHK current or an integrated channel count
 bool needs_memb(const stg_sim *s) { return s->needs_memb(); }
 
+// index of a lanes-per-tet value (1, 2, 4, 8) in the compiled variants
+int lanes_index(int lanes) { return lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3; }
+
+// tets per block of the current launch shape
+int tets_per_block(const stg_sim *s) { return s->block / s->lanes; }
+
 using PeriodKernel = void (*)(stg::ModelK, stg::SysK, int);
 
 PeriodKernel period_kernel(const stg_sim *s, int B) {
     const bool memb = needs_memb(s);
-    if (B <= 256) return memb ? stg::k_period<256, true> : stg::k_period<256, false>;
-    return memb ? stg::k_period<1024, true> : stg::k_period<1024, false>;
+    if (memb) return B <= 256 ? stg::k_period<256, true> : stg::k_period<1024, true>;
+    static const PeriodKernel vol[2][4] = {
+        {stg::k_period<256, false, stg::DynTables, 1>, stg::k_period<256, false, stg::DynTables, 2>,
+         stg::k_period<256, false, stg::DynTables, 4>, stg::k_period<256, false, stg::DynTables, 8>},
+        {stg::k_period<1024, false, stg::DynTables, 1>, stg::k_period<1024, false, stg::DynTables, 2>,
+         stg::k_period<1024, false, stg::DynTables, 4>, stg::k_period<1024, false, stg::DynTables, 8>}};
+    return vol[B <= 256 ? 0 : 1][lanes_index(s->lanes)];
 }
 
 // ------------------------------------------------------------- specialisation
@@ -1084,16 +1098,21 @@ std::string spec_tables_source(const stg_sim *s) {
 void specialize(stg_sim *s, int min_waves) {
     if (min_waves < 0 || min_waves > 8) throw StgError{STG_ERR_ARG, "waves per SIMD bound must be in [0, 8]"};
     const int memb = s->needs_memb() ? 1 : 0;
-    if (s->rtc_fns[memb][min_waves]) {
-        if (s->rtc_fn != s->rtc_fns[memb][min_waves]) s->free_graph();
-        s->rtc_fn = s->rtc_fns[memb][min_waves];
+    // the membrane variant has one lane per tet (a run that needs it goes back to one)
+    const int lanes = memb ? 1 : s->lanes;
+    const int li = lanes_index(lanes);
+    if (s->rtc_fns[memb][min_waves][li]) {
+        if (s->rtc_fn != s->rtc_fns[memb][min_waves][li]) s->free_graph();
+        s->rtc_fn = s->rtc_fns[memb][min_waves][li];
         s->rtc_memb = memb != 0;
+        s->rtc_waves = min_waves;
         return;
     }
     const std::string kern = rtc_join(kRtcKernels, sizeof(kRtcKernels) / sizeof(kRtcKernels[0]));
     const std::string rnd = rtc_join(kRtcRandom, sizeof(kRtcRandom) / sizeof(kRtcRandom[0]));
     const std::string lfact = rtc_join(kRtcLfact, sizeof(kRtcLfact) / sizeof(kRtcLfact[0]));
-    const std::string name = std::string("stg::k_period<256, ") + (memb ? "true" : "false") + ", StgModelTables>";
+    const std::string name = std::string("stg::k_period<256, ") + (memb ? "true" : "false") + ", StgModelTables, " +
+                             std::to_string(lanes) + ">";
     std::string src =
         "using __hip_internal::int8_t; using __hip_internal::uint8_t;\n"
         "using __hip_internal::int16_t; using __hip_internal::uint16_t;\n"
@@ -1161,10 +1180,11 @@ void specialize(stg_sim *s, int min_waves) {
         (void)hipModuleUnload(mod);
         throw StgError{STG_ERR_DEVICE, "specialised kernel not found in its module"};
     }
-    s->rtc_mods[memb][min_waves] = mod;
-    s->rtc_fns[memb][min_waves] = fn;
+    s->rtc_mods[memb][min_waves][li] = mod;
+    s->rtc_fns[memb][min_waves][li] = fn;
     s->rtc_fn = fn;
     s->rtc_memb = memb != 0;
+    s->rtc_waves = min_waves;
     s->free_graph();
 }
 
@@ -1177,12 +1197,12 @@ void unspecialize(stg_sim *s) {
 // LDS bytes of the period kernel that launch_all would launch: dynamic rows
 // plus the block-reduction array (static)
 size_t launch_smem(const stg_sim *s, bool *spec_out) {
-    const int B = s->block;
+    const int B = s->block, BT = tets_per_block(s);
     // per-element tables and SSA groups are read by the generic membrane kernel only
     const bool spec = s->rtc_fn != nullptr && B <= 256 && (s->rtc_memb || !needs_memb(s)) && !s->has_tabs() &&
                       !s->has_groups();
     if (spec_out) *spec_out = spec;
-    return spec ? s->smem_bytes_lean(B, s->rtc_memb) : s->smem_bytes(B);
+    return spec ? s->smem_bytes_lean(BT, s->rtc_memb) : s->smem_bytes(BT);
 }
 
 // The periods of a run: launches 0 .. L-1 of the period kernel (launch l
@@ -1218,9 +1238,46 @@ void launch_final_enter(stg_sim *s, int L) {
     if (nb == 0) return;
     stg::SysK k = make_sys(s, L & 1);
     k.n_blocks = nb;
-    if (needs_memb(s)) hipLaunchKernelGGL(stg::k_enter<true>, dim3(nb), dim3(s->block), 0, s->stream, s->mk, k);
-    else hipLaunchKernelGGL(stg::k_enter<false>, dim3(nb), dim3(s->block), 0, s->stream, s->mk, k);
+    // one thread per tet of the block
+    const int BT = tets_per_block(s);
+    if (needs_memb(s)) hipLaunchKernelGGL(stg::k_enter<true>, dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+    else hipLaunchKernelGGL(stg::k_enter<false>, dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
     HIP_OK(hipGetLastError());
+}
+
+// A new launch shape (threads per block, lanes per tet): the blocks are laid
+// out again for the new tets per block, and the per-block extents are folded
+// into realization totals kept in each realization's first block.
+void relayout(stg_sim *s, int block, int lanes) {
+    if (block == s->block && lanes == s->lanes && !s->block_real.empty()) return;
+    std::vector<unsigned long long> keep;
+    const std::vector<int32_t> old_real = s->block_real;
+    if (!old_real.empty()) {
+        keep.resize(2 * old_real.size());
+        HIP_OK(hipMemcpy(keep.data(), s->d_ext, sizeof(unsigned long long) * keep.size(), hipMemcpyDeviceToHost));
+    }
+    // a specialised volume kernel is compiled per lane count: switch to this
+    // count's variant, or to the generic kernel until it is compiled
+    if (s->rtc_fn && !s->rtc_memb && lanes != s->lanes) s->rtc_fn = s->rtc_fns[0][s->rtc_waves][lanes_index(lanes)];
+    s->block = block;
+    s->lanes = lanes;
+    build_blocks(s, tets_per_block(s));
+    if (keep.empty()) return;
+    std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
+    std::vector<unsigned long long> tot(2 * s->n_real, 0);
+    for (size_t b = 0; b < old_real.size(); ++b) {
+        tot[2 * old_real[b]] += keep[2 * b];
+        tot[2 * old_real[b] + 1] += keep[2 * b + 1];
+    }
+    std::vector<bool> done(s->n_real, false);
+    for (size_t b = 0; b < s->block_real.size(); ++b) {
+        const int r = s->block_real[b];
+        if (done[r]) continue;
+        done[r] = true;
+        ext[2 * b] = tot[2 * r];
+        ext[2 * b + 1] = tot[2 * r + 1];
+    }
+    HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
 }
 
 void run_impl(stg_sim *s, double endtime) {
@@ -1231,7 +1288,11 @@ void run_impl(stg_sim *s, double endtime) {
         s->last_launches = 0;
         return;
     }
-    if (s->block_real.empty()) build_blocks(s, s->block);
+    // the tile schedule serves volume-only models: a model that has come to
+    // need the membrane variant (an EField asking for channel integrals) goes
+    // back to one lane per tet
+    if (s->lanes > 1 && needs_memb(s)) relayout(s, s->block, 1);
+    if (s->block_real.empty()) build_blocks(s, tets_per_block(s));
     // per-realization schedules
     std::vector<std::vector<double>> dts(s->n_real);
     int L = 0;
@@ -1909,40 +1970,32 @@ int stg_set_launch(stg_sim *s, int32_t block, int32_t use_graph) {
     return guard(s, [&] {
         if (block < 64 || block > 1024 || block % 64 != 0)
             throw StgError{STG_ERR_ARG, "block must be a multiple of 64 in [64, 1024]"};
-        const size_t smem = s->smem_bytes(block);
-        if (smem > 160 * 1024) throw StgError{STG_ERR_ARG, "block too large for the model's LDS footprint"};
-        // extents are kept per block: fold them into realization totals first
-        std::vector<unsigned long long> keep;
-        std::vector<int32_t> old_real = s->block_real;
-        if (!s->block_real.empty()) {
-            keep.resize(2 * s->block_real.size());
-            HIP_OK(hipMemcpy(keep.data(), s->d_ext, sizeof(unsigned long long) * keep.size(), hipMemcpyDeviceToHost));
-        }
-        if (block != s->block || s->block_real.empty()) {
-            s->block = block;
-            build_blocks(s, block);
-            if (!keep.empty()) {
-                // carry totals into the first block of each realization
-                std::vector<unsigned long long> ext(2 * s->block_real.size(), 0);
-                std::vector<unsigned long long> tot(2 * s->n_real, 0);
-                for (size_t b = 0; b < old_real.size(); ++b) {
-                    tot[2 * old_real[b]] += keep[2 * b];
-                    tot[2 * old_real[b] + 1] += keep[2 * b + 1];
-                }
-                std::vector<bool> done(s->n_real, false);
-                for (size_t b = 0; b < s->block_real.size(); ++b) {
-                    const int r = s->block_real[b];
-                    if (done[r]) continue;
-                    done[r] = true;
-                    ext[2 * b] = tot[2 * r];
-                    ext[2 * b + 1] = tot[2 * r + 1];
-                }
-                HIP_OK(hipMemcpy(s->d_ext, ext.data(), sizeof(unsigned long long) * ext.size(), hipMemcpyHostToDevice));
-            }
-        }
+        if (s->smem_bytes(block / s->lanes) > 160 * 1024)
+            throw StgError{STG_ERR_ARG, "block too large for the model's LDS footprint"};
+        relayout(s, block, s->lanes);
         s->use_graph = use_graph != 0;
         s->free_graph();
     });
+}
+
+int stg_set_lanes(stg_sim *s, int32_t lanes) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+            throw StgError{STG_ERR_ARG, "lanes per tet must be 1, 2, 4 or 8"};
+        if (lanes > 1 && needs_memb(s))
+            throw StgError{STG_ERR_NOTIMPL, "several lanes per tet serve volume-only models"};
+        const bool spec_on = s->rtc_fn != nullptr;
+        relayout(s, s->block, lanes);
+        // specialisation stays on: compile this lane count's variant if needed
+        if (spec_on && !s->rtc_fn) specialize(s, s->rtc_waves);
+    });
+}
+
+int stg_get_lanes(const stg_sim *s, int32_t *lanes) {
+    if (!s || !lanes) return STG_ERR_ARG;
+    *lanes = s->lanes;
+    return STG_OK;
 }
 
 // ------------------------------------------------------------ checkpoint

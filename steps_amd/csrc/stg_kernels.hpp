@@ -1279,9 +1279,133 @@ __device__ void group_period(const ModelK &m, const SysK &sys, int g, int launch
 #define STG_VOL_WAVES_PER_EU 1
 #endif
 
-template <int MAXB, bool MEMB, class MT = DynTables>
+// Per-block extents (events, molecules moved): wavefront reduction, then one
+// slot per block, added by thread 0. Every thread of the block calls it.
+__device__ __forceinline__ void block_extents(const SysK &sys, int slot, long nev, int32_t moved) {
+    unsigned long long ev = (unsigned long long)nev, mv = (unsigned long long)moved;
+    for (int off = 32; off > 0; off >>= 1) {
+        ev += __shfl_xor(ev, off, 64);
+        mv += __shfl_xor(mv, off, 64);
+    }
+    __shared__ unsigned long long red[2][16];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    if (lane == 0) {
+        red[0][wave] = ev;
+        red[1][wave] = mv;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long se = 0, sm = 0;
+        for (int w = 0; w < ((int)blockDim.x + 63) / 64; ++w) {
+            se += red[0][w];
+            sm += red[1][w];
+        }
+        sys.ext[2 * (size_t)slot] += se;
+        sys.ext[2 * (size_t)slot + 1] += sm;
+    }
+}
+
+// Volume-only models with G > 1 lanes per tet (the tile schedule, for meshes
+// too small to fill the GPU, where a period takes one wavefront's critical
+// path): the G lanes of a tet gather, leave and store its species s with
+// s = lane mod G, and lane 0 runs the tet's SSA. Each species keeps its own
+// Philox channel, so the draws, and the results, are those of one lane per
+// tet; a wavefront holds 64 / G tets, which shortens the SSA's slowest lane
+// and splits the leaving pass over the species.
+template <int G, class MT>
+__device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *smem, const int4 bd, int r, int launch,
+                            bool do_enter, bool do_step, long &nev, int32_t &moved) {
+    const int B = blockDim.x;
+    const int BT = B / G;  // tets per block
+    const int tid = threadIdx.x;
+    const int slot = tid / G, sub = tid % G;
+    const int S = tab_S<MT>(m);
+    const int ND = tab_ndiff<MT>(m);
+    Lds L;
+    L.B = BT;
+    L.tid = slot;
+    L.cnt = reinterpret_cast<int32_t *>(smem);
+    L.corr = reinterpret_cast<double *>(smem + (((size_t)S * BT * 4 + 15) & ~(size_t)15));
+    L.kc = L.corr + (size_t)S * BT;
+    L.a = L.kc + (size_t)sys.k_max * BT;
+    L.gx = nullptr;
+    const int t = bd.z + slot;
+    const bool valid = t < bd.w;
+    const int NT = sys.n_tets;
+    if (valid) {
+        for (int s = sub; s < S; s += G) {
+            L.c(s) = sys.counts[(size_t)s * NT + t];
+            L.o(s) = 0.0;
+        }
+        if (do_enter) {
+            const int4 nb4 = sys.nbr[t];
+            const uint32_t nf = sys.nface[t];
+            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            size_t off[4];
+#pragma unroll
+            for (int f = 0; f < 4; ++f)
+                off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
+            for (int d = sub; d < ND; d += G) {
+                const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
+                int32_t v[4];
+#pragma unroll
+                for (int f = 0; f < 4; ++f) v[f] = op[off[f]];
+                int32_t add = 0;
+#pragma unroll
+                for (int f = 0; f < 4; ++f) {
+                    if (v[f] == kSlotEscape) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
+                    add += nbv[f] >= 0 ? v[f] : 0;
+                }
+                L.c(tab_dspec<MT>(m, d)) += add;
+            }
+        }
+    }
+    __syncthreads();  // every species of the tet has entered
+    if (do_step) {
+        const int tet0 = sys.real_tet0[r];
+        const uint32_t ltet = (uint32_t)(t - tet0);
+        const uint32_t period = uni(sys.real_step0[r]) + (uint32_t)launch;
+        const double dt = uni(sys.dt_tab[(size_t)launch * sys.n_real + r]);
+        const uint64_t seed = uni(sys.real_seed[r]);
+        const uint32_t stream = uni(sys.real_stream[r]);
+        if (valid && sub == 0) {
+            if constexpr (MT::kStatic) nev = ssa_lean<MT>(sys, L, r, t, ltet, period, dt, seed, stream);
+            else nev = ssa_group<false, MT>(m, sys, L, r, t, ltet, period, dt, seed, stream);
+            if (nev < 0) {
+                atomicOr(sys.err, 1);
+                nev = 0;
+            }
+        }
+        __syncthreads();  // the SSA's counts and occupancy corrections are in LDS
+        if (valid) {
+            const double *Dr = sys.real_D + (size_t)r * S;
+            for (int d = sub; d < ND; d += G) {
+                const int sp = tab_dspec<MT>(m, d);
+                const int tg = pin(t);
+                const double2 g01 = sys.g[2 * (size_t)tg];
+                const double2 g23 = sys.g[2 * (size_t)tg + 1];
+                const double r4[4] = {g01.x, g01.y, g23.x, g23.y};
+                int4 o;
+                const int32_t n = L.c(sp);
+                // the lanes of a wavefront hold different species: D is read
+                // per lane, not taken from the first lane
+                const int32_t nl = leave(sys, n, L.o(sp), sp, (tab_tracked<MT>(m) >> sp) & 1, Dr[sp], r4,
+                                         0xffffffffu, S, dt, seed, ltet, period, stream, o);
+                if (nl > 0) L.c(sp) = n - nl;
+                moved += nl;
+                put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
+            }
+        }
+        __syncthreads();  // a species may be stored by another lane than the one that moved it
+    }
+    if (valid)
+        for (int s = sub; s < S; s += G) sys.counts[(size_t)s * NT + t] = L.c(s);
+}
+
+template <int MAXB, bool MEMB, class MT = DynTables, int G = 1>
 __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL_WAVES_PER_EU)
     k_period(const ModelK m, const SysK sys, int launch) {
+    static_assert(G == 1 || (!MEMB && G <= 8 && (G & (G - 1)) == 0), "lanes per tet: 1, or 2/4/8 for volume models");
     extern __shared__ __align__(16) unsigned char smem[];
     const int B = blockDim.x;
     const int tid = threadIdx.x;
@@ -1303,6 +1427,13 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     const bool do_step = launch < np;
     if (!do_enter && !do_step) return;  // uniform over the block
 
+    long nev = 0;
+    int32_t moved = 0;
+    if constexpr (G > 1) {
+        period_tile<G, MT>(m, sys, smem, bd, r, launch, do_enter, do_step, nev, moved);
+        block_extents(sys, bd.x, nev, moved);
+        return;
+    }
     const int S = tab_S<MT>(m);
     Lds L;
     L.B = B;
@@ -1322,8 +1453,6 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     const int t = bd.z + tid;
     const int tend = bd.w;
     const bool valid = t < tend;
-    long nev = 0;
-    int32_t moved = 0;
     // per-element tables of this realization (uniform over the block); the
     // engine runs the generic kernel whenever a realization has one
     const int rtab = (MEMB && !MT::kStatic && sys.real_tabs != nullptr) ? sys.real_tabs[r] : 0;
@@ -1357,19 +1486,40 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
 #pragma unroll
             for (int f = 0; f < 4; ++f)
                 off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
-            for (int d = 0; d < tab_ndiff<MT>(m); ++d) {
-                const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
-                int32_t v[4];
+            // the slots of up to four diffusing species are loaded together and
+            // the (rare) escapes resolved once all of them have arrived: an
+            // escape test per species would wait on that species' loads before
+            // the next species' loads are issued
+            const int ND = tab_ndiff<MT>(m);
+            for (int d0 = 0; d0 < ND; d0 += 4) {
+                int32_t v[4][4];
 #pragma unroll
-                for (int f = 0; f < 4; ++f) v[f] = op[off[f]];
-                int32_t add = 0;
+                for (int j = 0; j < 4; ++j) {
+                    const int16_t *op = sys.out_prev + (size_t)(d0 + j) * NT * 4;
 #pragma unroll
-                for (int f = 0; f < 4; ++f) {
-                    if (v[f] == kSlotEscape) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
-                    add += nbv[f] >= 0 ? v[f] : 0;
+                    for (int f = 0; f < 4; ++f) v[j][f] = d0 + j < ND ? (int32_t)op[off[f]] : 0;
                 }
-                const int dsp = tab_dspec<MT>(m, d);
-                if (!MEMB || !((clampm >> dsp) & 1u)) L.c(dsp) += add;
+                bool esc = false;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) esc = esc || v[j][f] == kSlotEscape;
+                if (esc) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+#pragma unroll
+                        for (int f = 0; f < 4; ++f)
+                            if (v[j][f] == kSlotEscape) v[j][f] = sys.wide_prev[(size_t)(d0 + j) * NT * 4 + off[f]];
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (d0 + j >= ND) break;
+                    int32_t add = 0;
+#pragma unroll
+                    for (int f = 0; f < 4; ++f) add += nbv[f] >= 0 ? v[j][f] : 0;
+                    const int dsp = tab_dspec<MT>(m, d0 + j);
+                    if (!MEMB || !((clampm >> dsp) & 1u)) L.c(dsp) += add;
+                }
             }
         }
         int tp0 = 0, tp1 = 0;
@@ -1465,28 +1615,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
         const int tw = pin(t);
         for (int s = 0; s < S; ++s) sys.counts[(size_t)s * NT + tw] = L.c(s);
     }
-    // per-block extents: wavefront reduction, then one slot per block
-    unsigned long long ev = (unsigned long long)nev, mv = (unsigned long long)moved;
-    for (int off = 32; off > 0; off >>= 1) {
-        ev += __shfl_xor(ev, off, 64);
-        mv += __shfl_xor(mv, off, 64);
-    }
-    __shared__ unsigned long long red[2][16];
-    const int wave = tid >> 6, lane = tid & 63;
-    if (lane == 0) {
-        red[0][wave] = ev;
-        red[1][wave] = mv;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        unsigned long long se = 0, sm = 0;
-        for (int w = 0; w < (B + 63) / 64; ++w) {
-            se += red[0][w];
-            sm += red[1][w];
-        }
-        sys.ext[2 * (size_t)bd.x] += se;
-        sys.ext[2 * (size_t)bd.x + 1] += sm;
-    }
+    block_extents(sys, bd.x, nev, moved);
 }
 
 // The entering pass alone, for the launch after the last period of a run: the

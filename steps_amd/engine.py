@@ -225,6 +225,17 @@ class Engine:
     def set_launch(self, block: int = 256, use_graph: bool = False):
         self._chk(self.lib.stg_set_launch(self.h, block, 1 if use_graph else 0))
 
+    def set_lanes(self, lanes: int = 1):
+        """Lanes per tet of the period kernel: 1, or 2/4/8 (the tile schedule
+        for volume-only models on small meshes; identical results)."""
+        self._chk(self.lib.stg_set_lanes(self.h, int(lanes)))
+
+    @property
+    def lanes(self) -> int:
+        v = C.c_int32()
+        self._chk(self.lib.stg_get_lanes(self.h, C.byref(v)))
+        return v.value
+
     def set_specialize(self, on: bool = True, min_waves: int = 0):
         """Use a period kernel compiled at run time for this model's volume
         reaction network (identical results; blocks up to 256).

@@ -243,8 +243,12 @@ def test_autotune_keeps_results():
     ref.run(w.end_time)
     eng = _engine(w, 2)
     res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True), spec_waves=(0, 8))
-    # generic and model-specialised kernels (two register bounds), both blocks, stream and graph
-    assert res["block"] in (64, 256) and len(res["timings"]) == 12
+    # generic and model-specialised kernels (two register bounds), both blocks:
+    # stream and graph with one lane per tet (12 shapes), stream with 2 and 4
+    # lanes per tet (6 shapes each)
+    assert res["block"] in (64, 256) and res["lanes"] in (1, 2, 4) and len(res["timings"]) == 24
+    assert sum(k.endswith("/lanes4") for k in res["timings"]) == 6
+    assert eng.lanes == res["lanes"]
     assert eng.time() == 0.0
     eng.run(w.end_time)
     assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
