@@ -194,8 +194,11 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
     eng = load_engine(w, 3)
     ef.run(eng, ef_dt, ef_dt)                      # warm-up step
     t0 = eng.time()
+    iters = []
     tic = time.perf_counter()
-    ef.run(eng, t0 + ef_steps * ef_dt, ef_dt)
+    for k in range(1, ef_steps + 1):               # one coupled step per call, to read its iterations
+        ef.run(eng, t0 + k * ef_dt, ef_dt)
+        iters.append(ef.stats()["iterations"])
     wall = time.perf_counter() - tic
     stats = ef.stats()
     # precision: one more step from the same state, mixed vs FP64 CG
@@ -229,7 +232,7 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
         workload="EField on calcium_membrane_cube: 1,030,301 vertices, 120,000 membrane "
                  "triangles, 10 us steps coupled to the reaction-diffusion period",
         n_verts=nv, nnz=nnz, setup_s=setup_s, ms_per_ef_step=wall * 1e3 / ef_steps,
-        ef_steps=ef_steps, cg_iterations_last=stats["iterations"],
+        ef_steps=ef_steps, cg_iterations_per_step=iters, cg_iterations_last=stats["iterations"],
         refinements_last=stats["refinements"], rel_residual_last=stats["rel_residual"],
         mixed_vs_fp64_dv_rel_err=err, mixed_iterations=it_mixed["iterations"],
         mixed_step_ms=mixed_s * 1e3, fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3,
@@ -374,7 +377,7 @@ def main():
                     help="skip the 6,000,000-tet membrane cube and its EField")
     ap.add_argument("--no-config0", action="store_true",
                     help="skip the 3,072-tet run-every-1-ms protocol")
-    ap.add_argument("--part", choices=("all", "plain", "batched", "membrane"), default="all",
+    ap.add_argument("--part", choices=("all", "plain", "batched", "membrane", "efield"), default="all",
                     help="run one section only (counter profiling); with --block the section runs "
                          "that fixed schedule instead of the autotuned one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -410,10 +413,13 @@ def main():
             out.update(tet_steps_per_s=ens["tet_steps"] / ens["wall"], tet_steps=ens["tet_steps"],
                        kernel_ms_per_launch=ens["ms_launch"], launches=ens["launches"],
                        max_periods=ens["max_periods"], conserved=ens["conserved"])
-        else:
+        elif args.part == "membrane":
             wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0, device=True)
             out.update(time_membrane(wm, args.steps, args.warmup, block=args.block,
                                      specialize=args.specialize))
+        else:
+            wm = W.calcium_membrane_cube(n=100, side=20e-6, seed=0, device=True)
+            out.update(time_efield(wm))
         print(json.dumps(out))
         return
 
