@@ -323,7 +323,7 @@ __device__ __forceinline__ double surf_h(const ModelK &m, const SysK &sys, const
 // MEMB = false compiles a variant without triangles (no surface reactions,
 // GHK currents, k(V) tables or channel integrals) for volume-only models.
 template <bool MEMB, class MT>
-__device__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
+__device__ __forceinline__ long ssa_group(const ModelK &m, const SysK &sys, const Lds &L, int r, int t,
                           uint32_t ltet, uint32_t period, double dt, uint64_t seed, uint32_t stream,
                           int rtab = 0, uint32_t clampm = 0u) {
     const int R = tab_R<MT>(m), RS = m.n_sreacs, NG = m.n_ghk;
@@ -794,7 +794,13 @@ __device__ __forceinline__ bool almost_zero_dt(double dt) {
 // the 4 faces. Faces on a diffusion boundary (bdir byte != 0xff) take the
 // boundary's constant for this species instead of D (0 when it is closed); a
 // per-face table fD (non-null) replaces both.
-__device__ int32_t leave(const SysK &sys, int32_t n, double corr, int sp, bool tracked, double D,
+// This function, tri_leave, ssa_group and group_period are always inlined:
+// the membrane kernel calls them from more than one place, and a real call
+// takes the kernel-argument structures by reference, which makes every lane
+// copy them into its private (scratch) memory at each call (up to 3 KiB per
+// lane; the generic 6M-tet membrane period took 1.23 ms that way, 0.32 ms
+// inlined).
+__device__ __forceinline__ int32_t leave(const SysK &sys, int32_t n, double corr, int sp, bool tracked, double D,
                          const double r4in[4], uint32_t bdir, int S, double dt, uint64_t seed,
                          uint32_t ltet, uint32_t period, uint32_t stream, int4 &o,
                          const double *fD = nullptr) {
@@ -869,7 +875,7 @@ __device__ int32_t leave(const SysK &sys, int32_t n, double corr, int sp, bool t
 // of the volume with three edge directions (SDiff, sdiff.cpp:42-90, and the
 // SDiff block of tetopsplit.cpp:1716-1788). Writes three edge slots.
 // A per-edge table eD (non-null) replaces D; a clamped source keeps its count.
-__device__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int tri, double dt,
+__device__ __forceinline__ int32_t tri_leave(const SysK &sys, int q, bool tracked, double D, int tri, double dt,
                              uint64_t seed, uint32_t period, uint32_t stream, int4 &o,
                              const double *eD = nullptr, bool clamped = false) {
     o = make_int4(0, 0, 0, 0);
@@ -1047,7 +1053,7 @@ __device__ double grp_tri_prop(const ModelK &m, const SysK &sys, int tri, int t,
 }
 
 // One period of group g (see above); returns through nev / moved.
-__device__ void group_period(const ModelK &m, const SysK &sys, int g, int launch, bool do_enter, bool do_step,
+__device__ __forceinline__ void group_period(const ModelK &m, const SysK &sys, int g, int launch, bool do_enter, bool do_step,
                              long &nev_out, int32_t &moved_out) {
     const int NT = sys.n_tets, NTR = sys.n_tris, S = m.n_specs, R = m.n_reacs, RS = m.n_sreacs;
     const int RT = RS + m.n_ghk;
