@@ -295,8 +295,9 @@ def config0_protocol_oracle(n_seeds, budget_s):
 def config0_protocol_gpu(n_seeds=100):
     """The same protocol on the GPU: one engine per seed run one after another
     (plain), and all seeds as one batched engine (each seed a realization with its
-    own Philox key); totals reduced on the device at every 1 ms save. Each engine's
-    launch shape is autotuned on its initial state first (not timed)."""
+    own Philox key); totals reduced on the device at every 1 ms save (one call for every
+    species of every realization). Each engine's launch shape is autotuned on its initial
+    state first (not timed)."""
     ws = [W.ab_binding_cube(n=8, side=10e-6, n_a=10_000, n_b=10_000, seed=sd) for sd in range(n_seeds)]
     eng = Engine(ws[0].fm, [ws[0].mt], [0])
     for s_ in range(3):
@@ -311,7 +312,7 @@ def config0_protocol_gpu(n_seeds=100):
             eng.set_tet_counts(s_, w.counts[s_])
         for k in range(1, 101):
             eng.run(k * 1e-3)
-            [eng.sum_tet_counts(s_) for s_ in range(3)]
+            eng.sum_all_tet_counts()
         steps += eng.nsteps() * w.n_tets
     plain = steps / (time.perf_counter() - tic)
     beng = Engine(ws[0].fm, [w.mt for w in ws], list(range(n_seeds)), streams=np.zeros(n_seeds))
@@ -322,7 +323,7 @@ def config0_protocol_gpu(n_seeds=100):
     tic = time.perf_counter()
     for k in range(1, 101):
         beng.run(k * 1e-3)
-        [beng.sum_tet_counts(s_, r) for r in range(n_seeds) for s_ in range(3)]
+        beng.sum_all_tet_counts()
     wall = time.perf_counter() - tic
     steps = sum(beng.nsteps(r) * w.n_tets for r, w in enumerate(ws))
     shape = lambda t: {"block": t["block"], "graph": t["graph"], "specialized_kernel": t["specialized"],

@@ -246,6 +246,14 @@ int stg_sum_batch_tet_counts(stg_sim *sim, int32_t real, int32_t spec, const int
  * the reduction behind getCompSpecCount / getPatchSpecCount
  * (tetopsplit.cpp:2045-2110). */
 int stg_sum_tet_counts(stg_sim *sim, int32_t real, int32_t spec, int64_t *total);
+/* Totals of every species in every realization in one device reduction:
+ * totals[r * n_specs + s] (stg_sum_all_tri_counts: [r * n_sspecs + q]). Each
+ * entry is the reduction of stg_sum_tet_counts (getCompSpecCount,
+ * tetopsplit.cpp:2045-2110); the batch of all of them in one call has no
+ * reference counterpart (a batched ensemble reads every realization's totals
+ * at each save with one device round trip). */
+int stg_sum_all_tet_counts(stg_sim *sim, int64_t *totals);
+int stg_sum_all_tri_counts(stg_sim *sim, int64_t *totals);
 int stg_sum_tri_counts(stg_sim *sim, int32_t real, int32_t sspec, int64_t *total);
 
 /* setCompReacK / setPatchSReacK / setCompDiffD (api.hpp). A changed

@@ -133,6 +133,8 @@ SIGNATURES = {
                                            C.POINTER(C.c_int64), C.c_size_t, _dp]),
     "stg_sum_tet_counts": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
     "stg_sum_tri_counts": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_int64)]),
+    "stg_sum_all_tet_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "stg_sum_all_tri_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "stg_set_reac_k": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_sreac_k": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "stg_set_dcst": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),

@@ -177,6 +177,10 @@ struct stg_sim {
     int32_t *d_err = nullptr;
     unsigned long long *d_sum = nullptr;
     int64_t *d_idx = nullptr;
+    // totals of every (realization, species), and the triangle offsets they
+    // need, allocated on first use
+    long long *d_real_sums = nullptr;
+    int32_t *d_tri_off = nullptr;
     size_t idx_cap = 0;
     double *d_gather = nullptr;
     hipStream_t stream = nullptr;
@@ -300,7 +304,7 @@ struct stg_sim {
                         d_tout[0],   d_tout[1],     d_tri_corr,    d_real_tabs,   d_tet_k,
                         d_tet_fD,    d_tri_sk,      d_tri_eD,      d_tet_clamp,   d_tri_clamp,
                         d_tet_grp,   d_grp_ptr,     d_grp_mem,     d_grp_real,    d_tri_otet,
-                        d_tri_oreal, d_gcorr};
+                        d_tri_oreal, d_gcorr,       d_real_sums,   d_tri_off};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
@@ -1751,6 +1755,36 @@ int stg_sum_tet_counts(stg_sim *s, int32_t real, int32_t spec, int64_t *total) {
         if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
         sum_range(s, s->d_counts + (size_t)spec * s->NT + s->tet_off[real], s->tet_off[real + 1] - s->tet_off[real],
                   total);
+    });
+}
+
+// Totals of every (realization, species) in one reduction over a
+// species-major [S][n] count array with realization offsets off (device).
+static void sum_all(stg_sim *s, const int32_t *counts, int32_t n, int32_t S, const int32_t *off, int64_t *totals) {
+    const size_t nout = (size_t)s->n_real * S;
+    if (nout == 0) return;
+    if (!s->d_real_sums) s->d_real_sums = dalloc<long long>((size_t)s->n_real * std::max(std::max(s->S, s->SS), 1));
+    hipLaunchKernelGGL(stg::k_sum_real_counts, dim3((unsigned)nout), dim3(256), 0, s->stream, counts, n, S, off,
+                       s->d_real_sums);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(totals, s->d_real_sums, sizeof(long long) * nout, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipStreamSynchronize(s->stream));
+}
+
+int stg_sum_all_tet_counts(stg_sim *s, int64_t *totals) {
+    if (!s || !totals) return STG_ERR_ARG;
+    return guard(s, [&] { sum_all(s, s->d_counts, s->NT, s->S, s->d_real_tet0, totals); });
+}
+
+int stg_sum_all_tri_counts(stg_sim *s, int64_t *totals) {
+    if (!s || !totals) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (!s->d_tri_off) {
+            s->d_tri_off = dalloc<int32_t>(s->tri_off.size());
+            HIP_OK(hipMemcpy(s->d_tri_off, s->tri_off.data(), sizeof(int32_t) * s->tri_off.size(),
+                             hipMemcpyHostToDevice));
+        }
+        sum_all(s, s->d_tri_counts, s->NTRI, s->SS, s->d_tri_off, totals);
     });
 }
 

@@ -1697,6 +1697,27 @@ __global__ void k_sum_counts(const int32_t *__restrict__ v, int32_t n,
     }
 }
 
+// Totals of every (realization, species): block b = r * S + s reduces
+// species s over realization r's elements [off[r], off[r + 1]) of a
+// species-major [S][n] array; no atomics, so the sums are the same every call.
+__global__ void k_sum_real_counts(const int32_t *__restrict__ v, int32_t n, int32_t S,
+                                  const int32_t *__restrict__ off, long long *__restrict__ out) {
+    const int r = blockIdx.x / S, s = blockIdx.x % S;
+    const int32_t *p = v + (size_t)s * n;
+    long long acc = 0;
+    for (int i = off[r] + (int)threadIdx.x; i < off[r + 1]; i += blockDim.x) acc += p[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    __shared__ long long red[16];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tot = 0;
+        for (int w = 0; w < (int)(blockDim.x + 63) / 64; ++w) tot += red[w];
+        out[blockIdx.x] = tot;
+    }
+}
+
 // Sum of counts[base + idx[i]] over i < n into one int64 (an ROI total).
 __global__ void k_sum_indexed(const int32_t *__restrict__ counts, const int64_t *__restrict__ idx,
                               int64_t n, int32_t base, unsigned long long *__restrict__ total) {

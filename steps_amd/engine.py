@@ -110,6 +110,8 @@ class Engine:
         N.check(rc, None)
         self.h = h
         self.n_real = n
+        self.n_specs = len(fm.specs)
+        self.n_sspecs = len(fm.sspecs)
         self.n_tets = [len(m.vol) for m in meshes]
         self.n_tris = [len(m.tri_tet) for m in meshes]
         self._keep = []  # the engine copied everything
@@ -213,6 +215,19 @@ class Engine:
         self._chk(self.lib.stg_sum_tri_counts(self.h, real, sspec, C.byref(t)))
         return t.value
 
+
+    def sum_all_tet_counts(self) -> np.ndarray:
+        """Totals of every species in every realization, [n_real, n_specs], in one device call."""
+        out = np.zeros((self.n_real, self.n_specs), dtype=np.int64)
+        self._chk(self.lib.stg_sum_all_tet_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
+
+    def sum_all_tri_counts(self) -> np.ndarray:
+        """Totals of every surface species in every realization, [n_real, n_sspecs]."""
+        out = np.zeros((self.n_real, self.n_sspecs), dtype=np.int64)
+        if out.size:
+            self._chk(self.lib.stg_sum_all_tri_counts(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))))
+        return out
     def set_reac_k(self, reac: int, k: float, real: int = 0):
         self._chk(self.lib.stg_set_reac_k(self.h, real, reac, float(k)))
 

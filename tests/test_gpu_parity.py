@@ -435,3 +435,31 @@ def test_specialised_membrane_kernel_matches_generic(kind):
         assert np.array_equal(_gpu_counts(spec, w), _gpu_counts(gen, w)), f"t={t}"
         assert np.array_equal(_gpu_tri_counts(spec, w), _gpu_tri_counts(gen, w)), f"t={t}"
     assert spec.extents() == gen.extents() and spec.extents()[0] > 0
+
+
+def test_all_realization_totals_in_one_call():
+    """stg_sum_all_tet_counts / stg_sum_all_tri_counts equal the per-call
+    device totals and the host sums of the count arrays, for a batch of
+    unequal realizations (volume) and for a membrane model (surface)."""
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=7, n_min=3, n_max=8,
+                                                             side=4e-6, n_a=900, n_b=700, seed=5)
+    print("case totals of 7 realizations", flush=True)
+    eng = Engine(fm, meshes, [3] * 7, reac_k=reac_k)
+    for r in range(7):
+        for s in range(3):
+            eng.set_tet_counts(s, counts[r][s], real=r)
+    for t in (0.0, 0.05):
+        if t:
+            eng.run(t)
+        tot = eng.sum_all_tet_counts()
+        assert tot.shape == (7, 3)
+        for r in range(7):
+            for s in range(3):
+                assert tot[r, s] == eng.sum_tet_counts(s, r) == int(eng.get_tet_counts(s, r).sum())
+    w = W.calcium_membrane_cube(n=4, side=2e-6, seed=1, end_time=1e-4, ca_um=50.0)
+    meng = _engine(w, 5)
+    meng.run(w.end_time)
+    tt = meng.sum_all_tri_counts()
+    assert tt.shape == (1, len(w.fm.sspecs))
+    assert all(tt[0, q] == int(meng.get_tri_counts(q).sum()) for q in range(len(w.fm.sspecs)))
+    assert tt.sum() > 0
