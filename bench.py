@@ -169,7 +169,7 @@ def time_membrane(w, steps=300, warmup=5, block=0, specialize=False):
                   "lanes_per_tet": p["lanes"]})
 
 
-def time_efield(w, ef_steps=10, ef_dt=1e-5):
+def time_efield(w, ef_steps=20, ef_dt=1e-5):
     """EField on the membrane cube's 1,030,301 vertices, coupled to the reaction-diffusion
     step: passive leak 1 S/m^2 at -65 mV, 1e-2 F/m^2, 1 ohm.m, 1 nA on the x = 0 face.
     Mixed-precision solve timed over coupled steps (reaction-diffusion periods included); then
@@ -192,14 +192,19 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
     for t, a in zip(face, area):
         ef.set_tri_iclamp(int(t), 1e-9 * a / area.sum())
     eng = load_engine(w, 3)
+    # the launch shape the solver interface would use for this mesh, chosen by
+    # the autotuner on the reaction-diffusion state before the EField attaches
+    tuned_rd = autotune(eng, trial_periods=5, blocks=(64, 128, 256), graphs=(False,))
     ef.run(eng, ef_dt, ef_dt)                      # warm-up step
     t0 = eng.time()
-    iters = []
-    tic = time.perf_counter()
+    iters, walls = [], []
     for k in range(1, ef_steps + 1):               # one coupled step per call, to read its iterations
+        tic = time.perf_counter()
         ef.run(eng, t0 + k * ef_dt, ef_dt)
+        walls.append(time.perf_counter() - tic)
         iters.append(ef.stats()["iterations"])
-    wall = time.perf_counter() - tic
+    wall = sum(walls)
+    settled = walls[ef_steps // 2:]
     stats = ef.stats()
     # precision: one more step from the same state, mixed vs FP64 CG
     v0 = ef.verts_v()
@@ -232,7 +237,9 @@ def time_efield(w, ef_steps=10, ef_dt=1e-5):
         workload="EField on calcium_membrane_cube: 1,030,301 vertices, 120,000 membrane "
                  "triangles, 10 us steps coupled to the reaction-diffusion period",
         n_verts=nv, nnz=nnz, setup_s=setup_s, ms_per_ef_step=wall * 1e3 / ef_steps,
-        ef_steps=ef_steps, cg_iterations_per_step=iters, cg_iterations_last=stats["iterations"],
+        ms_per_ef_step_second_half=sum(settled) * 1e3 / len(settled),
+        ef_steps=ef_steps, cg_iterations_per_step=iters,
+        rd_schedule={"block": tuned_rd["block"], "specialized_kernel": eng.specialized, "lanes_per_tet": eng.lanes}, cg_iterations_last=stats["iterations"],
         refinements_last=stats["refinements"], rel_residual_last=stats["rel_residual"],
         mixed_vs_fp64_dv_rel_err=err, mixed_iterations=it_mixed["iterations"],
         mixed_step_ms=mixed_s * 1e3, fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3,
