@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(1024) k_reduce(const double *partial, int nb, 
         if (op == 0) {
             scal[0] = t;
             scal[1] = 0.0;
+            out[0] = t;  // kept: scal[0] moves on with the iterations
         } else if (op == 1) {
             scal[3] = t;
         } else if (op == 2) {
@@ -762,6 +763,7 @@ struct stg_efield {
     unsigned char *clamped = nullptr;
     void *xs = nullptr, *rs = nullptr, *zs = nullptr, *ps = nullptr, *qs = nullptr;  // T-typed work
     double *part = nullptr, *scal = nullptr, *outs = nullptr;
+    double *h_fetch = nullptr;  // pinned host words the convergence checks read back
     unsigned *cnt = nullptr;
     bool a32_dirty = true;
     int nb = 1;
@@ -777,6 +779,7 @@ struct stg_efield {
                         scal, outs, cnt, oc_erev, ghk_last};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        if (h_fetch) (void)hipHostFree(h_fetch);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -979,6 +982,7 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
     e->part = ef_alloc<double>(e->nb);
     e->scal = ef_alloc<double>(4);
     e->outs = ef_alloc<double>(8);
+    EF_HIP(hipHostMalloc((void **)&e->h_fetch, 2 * sizeof(double), hipHostMallocDefault));
     e->cnt = ef_alloc<unsigned>(8);
     e->vt_ptr = ef_alloc<int>(n + 1);
     e->vt_tri = ef_alloc<int>(std::max<size_t>(vt_tri.size(), 1));
@@ -1000,11 +1004,19 @@ void build(stg_efield *e, const stg_efield_mesh *m) {
 }
 
 // Jacobi PCG on A d = r (r given in r64), result added to x64. Returns iterations.
+// Device scalars read back by the host's convergence decisions, through
+// pinned memory (a copy to pageable memory goes through a staging buffer,
+// which adds to every round trip). fetch2 reads two adjacent words at once.
 double fetch(stg_efield *e, const double *d) {
-    double h = 0.0;
-    EF_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    EF_HIP(hipMemcpyAsync(e->h_fetch, d, sizeof(double), hipMemcpyDeviceToHost, e->stream));
     EF_HIP(hipStreamSynchronize(e->stream));
-    return h;
+    return e->h_fetch[0];
+}
+void fetch2(stg_efield *e, const double *d, double &a, double &b) {
+    EF_HIP(hipMemcpyAsync(e->h_fetch, d, 2 * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    EF_HIP(hipStreamSynchronize(e->stream));
+    a = e->h_fetch[0];
+    b = e->h_fetch[1];
 }
 
 constexpr int kSmallSystem = 16384;  // solved by one persistent workgroup
@@ -1510,7 +1522,6 @@ int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
     hipLaunchKernelGGL(k_cg_start<T>, dim3(nb), dim3(kB), 0, e->stream, n, e->r64, x, r);
     vcycle<T>(e, r, z, e->part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 0, e->scal, e->outs);
-    const double rz0 = fetch(e, e->scal);
     // iteration it: the search directions alternate between two buffers, and
     // the first iteration starts from p = z
     auto iteration = [&](int it) {
@@ -1527,9 +1538,13 @@ int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
     // host round trip). After the first pair, a pair (even it, odd it) is the
     // same launch sequence every time, so it is captured once as a graph and
     // replayed: about 20 launches per iteration cost their launch gaps once.
+    // r.z of the start is read back together with the first pair's (one
+    // round trip). It is positive whenever CG runs (the residual is nonzero and
+    // the V-cycle is SPD); were it not, the first pair's correction is dropped.
     hipGraphExec_t &pair = sizeof(T) == 4 ? e->amg.pair32 : e->amg.pair64;
     int it = 0;
-    if (rz0 > 0.0) {
+    double rz0 = 0.0;
+    {
         while (it < max_iter) {
             if (it > 0 && (it & 1) == 0 && it + 2 <= max_iter) {
                 if (!pair) {
@@ -1547,7 +1562,17 @@ int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
             } else {
                 for (int k = 0; k < 2 && it < max_iter; ++k, ++it) iteration(it);
             }
-            const double rz = fetch(e, e->scal);
+            double rz = 0.0;
+            if (it <= 2) {
+                EF_HIP(hipMemcpyAsync(e->h_fetch, e->outs, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+                EF_HIP(hipMemcpyAsync(e->h_fetch + 1, e->scal, sizeof(double), hipMemcpyDeviceToHost, e->stream));
+                EF_HIP(hipStreamSynchronize(e->stream));
+                rz0 = e->h_fetch[0];
+                rz = e->h_fetch[1];
+                if (!(rz0 > 0.0)) return 0;
+            } else {
+                rz = fetch(e, e->scal);
+            }
             if (!(rz > rel_tol * rel_tol * rz0)) break;
         }
     }
@@ -1665,19 +1690,27 @@ void advance_impl(stg_efield *e, double dt_s) {
     }
     // initial guess: the previous step's increment
     EF_HIP(hipMemcpyAsync(e->x64, e->dv_prev, n * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    // b.b and the first residual's r.r come back in one round trip
     hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(kB), 0, e->stream, n, e->b, e->part);
     hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal, e->outs + 3);
-    const double bb = fetch(e, e->outs + 3);
+    hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, e->b, e->x64, e->r64,
+                       e->part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal, e->outs + 4);
+    double bb = 0.0, rr0 = 0.0;
+    fetch2(e, e->outs + 3, bb, rr0);
     int iters = 0, refine = 0;
     double rel = 0.0, prev = INFINITY;
     bool ok = true;
     if (bb > 0.0) {
         for (refine = 0;; ++refine) {
-            hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, e->b, e->x64,
-                               e->r64, e->part);
-            hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal,
-                               e->outs + 4);
-            const double rr = fetch(e, e->outs + 4);
+            double rr = rr0;
+            if (refine > 0) {
+                hipLaunchKernelGGL(k_residual64, dim3(nb), dim3(kB), 0, e->stream, m, e->sigma, e->b, e->x64,
+                                   e->r64, e->part);
+                hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, e->stream, e->part, nb, 3, e->scal,
+                                   e->outs + 4);
+                rr = fetch(e, e->outs + 4);
+            }
             rel = std::sqrt(rr / bb);
             if (rel <= e->tol) break;
             // the FP64 residual has a rounding floor near eps * cond(A); stop
