@@ -180,11 +180,18 @@ struct stg_sim {
     // totals of every (realization, species), and the triangle offsets they
     // need, allocated on first use
     long long *d_real_sums = nullptr;
+    long long *h_real_sums = nullptr;  // pinned: the read-back of d_real_sums
+    unsigned long long *h_sum = nullptr;  // pinned: the read-back of d_sum
+    int32_t *h_err = nullptr;             // pinned: the read-back of d_err after a run
     int32_t *d_tri_off = nullptr;
     size_t idx_cap = 0;
     double *d_gather = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // pinned staging of a run's schedule tables (a copy from pageable memory
+    // is staged and blocks the host), grown on demand
+    unsigned char *h_stage = nullptr;
+    size_t h_stage_cap = 0;
     // model-specialised volume kernel (blocks of up to 256), compiled at run time;
     // one compiled module per (membrane variant, waves-per-SIMD bound; 0 = the
     // compiler's choice, lanes per tet 1/2/4/8)
@@ -307,6 +314,10 @@ struct stg_sim {
                         d_tri_oreal, d_gcorr,       d_real_sums,   d_tri_off};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
+        if (h_stage) (void)hipHostFree(h_stage);
+        if (h_real_sums) (void)hipHostFree(h_real_sums);
+        if (h_sum) (void)hipHostFree(h_sum);
+        if (h_err) (void)hipHostFree(h_err);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1321,9 +1332,24 @@ void run_impl(stg_sim *s, double endtime) {
             s->d_dt_tab = dalloc<double>(s->dt_tab_cap);
             s->free_graph();
         }
-        HIP_OK(hipMemcpyAsync(s->d_dt_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s->stream));
-        HIP_OK(hipMemcpyAsync(s->d_real_nper, nper.data(), sizeof(int32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
-        HIP_OK(hipMemcpyAsync(s->d_real_step0, step0.data(), sizeof(uint32_t) * s->n_real, hipMemcpyHostToDevice, s->stream));
+        // the three tables go through pinned staging, so the copies run
+        // asynchronously; the run ends with a stream synchronisation, so the
+        // staging is free again for the next run
+        const size_t b_tab = sizeof(double) * tab.size(), b_int = sizeof(int32_t) * (size_t)s->n_real;
+        const size_t need = b_tab + 2 * b_int;
+        if (need > s->h_stage_cap) {
+            if (s->h_stage) HIP_OK(hipHostFree(s->h_stage));
+            s->h_stage = nullptr;
+            s->h_stage_cap = 0;
+            HIP_OK(hipHostMalloc((void **)&s->h_stage, 2 * need, hipHostMallocDefault));
+            s->h_stage_cap = 2 * need;
+        }
+        std::memcpy(s->h_stage, tab.data(), b_tab);
+        std::memcpy(s->h_stage + b_tab, nper.data(), b_int);
+        std::memcpy(s->h_stage + b_tab + b_int, step0.data(), b_int);
+        HIP_OK(hipMemcpyAsync(s->d_dt_tab, s->h_stage, b_tab, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(s->d_real_nper, s->h_stage + b_tab, b_int, hipMemcpyHostToDevice, s->stream));
+        HIP_OK(hipMemcpyAsync(s->d_real_step0, s->h_stage + b_tab + b_int, b_int, hipMemcpyHostToDevice, s->stream));
         order_blocks(s, nper, L);
         // the LDS footprint can grow after stg_set_launch checked it (an
         // EField asking for channel integrals switches to the membrane variant)
@@ -1352,6 +1378,9 @@ void run_impl(stg_sim *s, double endtime) {
         }
         HIP_OK(hipEventRecord(s->ev1, s->stream));
         launch_final_enter(s, L);
+        // the device error flags come back with the run's synchronisation
+        if (!s->h_err) HIP_OK(hipHostMalloc((void **)&s->h_err, sizeof(int32_t), hipHostMallocDefault));
+        HIP_OK(hipMemcpyAsync(s->h_err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
         float ms = 0.f;
         HIP_OK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
@@ -1365,8 +1394,7 @@ void run_impl(stg_sim *s, double endtime) {
             s->rng_steps[r] += dts[r].size();
         }
         s->time = endtime;
-        int32_t err = 0;
-        HIP_OK(hipMemcpy(&err, s->d_err, sizeof(int32_t), hipMemcpyDeviceToHost));
+        const int32_t err = *s->h_err;
         if (err) {
             HIP_OK(hipMemset(s->d_err, 0, sizeof(int32_t)));
             if (err & 1) throw StgError{STG_ERR_PROG, "SSA exceeded 4194304 events in one tet and period"};
@@ -1711,8 +1739,8 @@ int stg_get_batch_tet_counts_noalloc(stg_sim *s, int32_t real, int32_t spec, con
 }
 
 static void sum_range(stg_sim *s, const int32_t *ptr, int32_t n, int64_t *total) {
-    unsigned long long zero = 0, res = 0;
-    HIP_OK(hipMemcpyAsync(s->d_sum, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+    if (!s->h_sum) HIP_OK(hipHostMalloc((void **)&s->h_sum, sizeof(unsigned long long), hipHostMallocDefault));
+    HIP_OK(hipMemsetAsync(s->d_sum, 0, sizeof(unsigned long long), s->stream));
     if (n > 0) {
         const int B = 256;
         int nb = (n + B - 1) / B;
@@ -1720,9 +1748,9 @@ static void sum_range(stg_sim *s, const int32_t *ptr, int32_t n, int64_t *total)
         hipLaunchKernelGGL(stg::k_sum_counts, dim3(nb), dim3(B), 0, s->stream, ptr, n, s->d_sum);
         HIP_OK(hipGetLastError());
     }
-    HIP_OK(hipMemcpyAsync(&res, s->d_sum, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(s->h_sum, s->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
-    *total = (int64_t)res;
+    *total = (int64_t)*s->h_sum;
 }
 
 int stg_sum_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64_t *tets, size_t n,
@@ -1732,8 +1760,8 @@ int stg_sum_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64
         check_real(s, real);
         if (spec < 0 || spec >= s->S) throw StgError{STG_ERR_ARG, "species index out of range"};
         stage_indices(s, real, tets, n);
-        unsigned long long zero = 0, res = 0;
-        HIP_OK(hipMemcpyAsync(s->d_sum, &zero, sizeof(zero), hipMemcpyHostToDevice, s->stream));
+        if (!s->h_sum) HIP_OK(hipHostMalloc((void **)&s->h_sum, sizeof(unsigned long long), hipHostMallocDefault));
+        HIP_OK(hipMemsetAsync(s->d_sum, 0, sizeof(unsigned long long), s->stream));
         if (n > 0) {
             const int B = 256;
             const int nb = (int)std::min<size_t>((n + B - 1) / B, 2048);
@@ -1742,9 +1770,9 @@ int stg_sum_batch_tet_counts(stg_sim *s, int32_t real, int32_t spec, const int64
                                s->d_sum);
             HIP_OK(hipGetLastError());
         }
-        HIP_OK(hipMemcpyAsync(&res, s->d_sum, sizeof(res), hipMemcpyDeviceToHost, s->stream));
+        HIP_OK(hipMemcpyAsync(s->h_sum, s->d_sum, sizeof(unsigned long long), hipMemcpyDeviceToHost, s->stream));
         HIP_OK(hipStreamSynchronize(s->stream));
-        *total = (int64_t)res;
+        *total = (int64_t)*s->h_sum;
     });
 }
 
@@ -1763,12 +1791,16 @@ int stg_sum_tet_counts(stg_sim *s, int32_t real, int32_t spec, int64_t *total) {
 static void sum_all(stg_sim *s, const int32_t *counts, int32_t n, int32_t S, const int32_t *off, int64_t *totals) {
     const size_t nout = (size_t)s->n_real * S;
     if (nout == 0) return;
-    if (!s->d_real_sums) s->d_real_sums = dalloc<long long>((size_t)s->n_real * std::max(std::max(s->S, s->SS), 1));
+    const size_t cap = (size_t)s->n_real * std::max(std::max(s->S, s->SS), 1);
+    if (!s->d_real_sums) s->d_real_sums = dalloc<long long>(cap);
+    if (!s->h_real_sums) HIP_OK(hipHostMalloc((void **)&s->h_real_sums, sizeof(long long) * cap, hipHostMallocDefault));
     hipLaunchKernelGGL(stg::k_sum_real_counts, dim3((unsigned)nout), dim3(256), 0, s->stream, counts, n, S, off,
                        s->d_real_sums);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(totals, s->d_real_sums, sizeof(long long) * nout, hipMemcpyDeviceToHost, s->stream));
+    HIP_OK(hipMemcpyAsync(s->h_real_sums, s->d_real_sums, sizeof(long long) * nout, hipMemcpyDeviceToHost,
+                          s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+    std::memcpy(totals, s->h_real_sums, sizeof(long long) * nout);
 }
 
 int stg_sum_all_tet_counts(stg_sim *s, int64_t *totals) {
