@@ -83,6 +83,10 @@ def measured_traffic(path=TRAFFIC_SUMMARY, key="traffic_bytes_per_launch"):
     return None
 
 
+# face slot width (bytes) of the fixed launch shapes (--block); the autotuner picks its own
+FIXED_SLOT_WIDTH = 2
+
+
 def load_engine(w, seed, block=256, graph=False, specialize=False, spec_waves=0):
     eng = Engine(w.fm, [w.mt], [seed])
     for s in range(len(w.fm.specs)):
@@ -90,6 +94,7 @@ def load_engine(w, seed, block=256, graph=False, specialize=False, spec_waves=0)
     for s in range(len(w.fm.sspecs)):
         eng.set_tri_counts(s, w.tri_counts[s])
     eng.set_launch(block, graph)
+    eng.set_slot_width(FIXED_SLOT_WIDTH)
     if specialize:
         eng.set_specialize(True, spec_waves)
     return eng
@@ -111,7 +116,7 @@ def time_plain(w, steps, warmup, seed=1, block=0, graph=False, specialize=False,
     return dict(eng=eng, wall=wall, periods=periods, tet_steps=periods * w.n_tets,
                 ms_launch=ms_launch, launches=launches, dt=dt, tuned=tuned,
                 block=tuned["block"] if tuned else block, graph=tuned["graph"] if tuned else graph,
-                specialized=eng.specialized, lanes=eng.lanes)
+                specialized=eng.specialized, lanes=eng.lanes, slot_width=eng.slot_width)
 
 
 def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True, specialize=False):
@@ -122,6 +127,7 @@ def time_ensemble(n_real=1024, end_time=0.1, seed=0, block=0, graph=True, specia
         for s in range(3):
             eng.set_tet_counts(s, counts[r][s], real=r)
     eng.set_launch(block or 256, graph)
+    eng.set_slot_width(FIXED_SLOT_WIDTH)
     if block and specialize:
         eng.set_specialize(True)
     # launch shape from the autotuner (block = 0), timed on this ensemble's own state
@@ -166,7 +172,7 @@ def time_membrane(w, steps=300, warmup=5, block=0, specialize=False):
         traffic=traffic,
         traffic_frac=(traffic / (p["ms_launch"] * 1e-3) / 1e9 / HBM_PEAK_GBS) if traffic else None,
         schedule={"block": p["block"], "graph": bool(p["graph"]), "specialized_kernel": bool(p["specialized"]),
-                  "lanes_per_tet": p["lanes"]})
+                  "lanes_per_tet": p["lanes"], "slot_width": p["slot_width"]})
 
 
 def time_efield(w, ef_steps=20, ef_dt=1e-5):
@@ -239,7 +245,8 @@ def time_efield(w, ef_steps=20, ef_dt=1e-5):
         n_verts=nv, nnz=nnz, setup_s=setup_s, ms_per_ef_step=wall * 1e3 / ef_steps,
         ms_per_ef_step_second_half=sum(settled) * 1e3 / len(settled),
         ef_steps=ef_steps, cg_iterations_per_step=iters,
-        rd_schedule={"block": tuned_rd["block"], "specialized_kernel": eng.specialized, "lanes_per_tet": eng.lanes}, cg_iterations_last=stats["iterations"],
+        rd_schedule={"block": tuned_rd["block"], "specialized_kernel": eng.specialized, "lanes_per_tet": eng.lanes,
+                     "slot_width": eng.slot_width}, cg_iterations_last=stats["iterations"],
         refinements_last=stats["refinements"], rel_residual_last=stats["rel_residual"],
         mixed_vs_fp64_dv_rel_err=err, mixed_iterations=it_mixed["iterations"],
         mixed_step_ms=mixed_s * 1e3, fp64_iterations=it_64["iterations"], fp64_step_ms=fp64_s * 1e3,
@@ -334,7 +341,7 @@ def config0_protocol_gpu(n_seeds=100):
     wall = time.perf_counter() - tic
     steps = sum(beng.nsteps(r) * w.n_tets for r, w in enumerate(ws))
     shape = lambda t: {"block": t["block"], "graph": t["graph"], "specialized_kernel": t["specialized"],
-                       "lanes_per_tet": t["lanes"]}
+                       "lanes_per_tet": t["lanes"], "slot_width": t["slot_width"]}
     return dict(plain_one_seed_at_a_time_tet_steps_per_s=plain, plain_seeds=10,
                 batched_tet_steps_per_s=steps / wall, batched_seeds=n_seeds, batched_wall_s=wall,
                 plain_schedule=shape(tuned_plain), batched_schedule=shape(tuned_batch))
@@ -376,6 +383,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="with --block: replay launches as a graph")
     ap.add_argument("--specialize", action="store_true",
                     help="with --block: use the model-specialised kernel (compiled at run time)")
+    ap.add_argument("--slot-width", type=int, default=2, choices=(1, 2),
+                    help="with --block: face slot width in bytes")
     ap.add_argument("--spec-waves", type=int, default=0,
                     help="with --specialize: wavefronts per SIMD the compiled kernel must allow")
     ap.add_argument("--no-cpu", action="store_true")
@@ -391,6 +400,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    global FIXED_SLOT_WIDTH
+    FIXED_SLOT_WIDTH = args.slot_width
     if args.copy_calibration:
         import torch
 
@@ -463,6 +474,7 @@ def main():
             "graph": bool(p["graph"]),
             "specialized_kernel": bool(p["specialized"]),
             "lanes_per_tet": p["lanes"],
+            "slot_width": p["slot_width"],
             "spec_min_waves": p["tuned"].get("spec_min_waves") if p["tuned"] else (
                 args.spec_waves if args.specialize else None),
             "schedule": "autotuned" if p["tuned"] else "fixed",
@@ -508,11 +520,13 @@ def main():
             "schedule": {"block": sched.get("block"), "graph": sched.get("graph"),
                          "specialized_kernel": sched.get("specialized"),
                          "lanes_per_tet": sched.get("lanes"),
+                         "slot_width": sched.get("slot_width"),
                          "spec_min_waves": sched.get("spec_min_waves"),
                          "autotune_wall_ms_per_launch": sched.get("timings")},
             "plain_same_model_schedule": {"block": ps["block"], "graph": bool(ps["graph"]),
                                           "specialized_kernel": bool(ps["specialized"]),
-                                          "lanes_per_tet": ps["lanes"]},
+                                          "lanes_per_tet": ps["lanes"],
+                                          "slot_width": ps["slot_width"]},
         }
         out["batched"]["batched_over_plain"] = (out["batched"]["tet_steps_per_s"] /
                                                 out["batched"]["plain_same_model_tet_steps_per_s"])

@@ -322,6 +322,15 @@ int stg_set_launch(stg_sim *sim, int32_t block, int32_t use_graph);
 int stg_set_lanes(stg_sim *sim, int32_t lanes);
 int stg_get_lanes(const stg_sim *sim, int32_t *lanes);
 
+/* Width of the face slots that carry a period's leaving molecules to the
+ * neighbour's entering gather (no reference counterpart; device tuning): 2
+ * (16-bit, the default) or 1 (8-bit) bytes per face and diffusing species. A
+ * count above the slot's range (32,767 or 127) takes a 32-bit side entry, so
+ * results are identical for both widths; 8-bit slots move fewer bytes while
+ * few faces pass more than 127 molecules in one period. */
+int stg_set_slot_width(stg_sim *sim, int32_t bytes);
+int stg_get_slot_width(const stg_sim *sim, int32_t *bytes);
+
 /* Model-specialised kernel (no reference counterpart; device tuning): on = 1
  * compiles, at run time with hipRTC (about a second), a period kernel with
  * this model's reaction network as compile-time constants and uses it for

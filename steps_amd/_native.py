@@ -141,6 +141,8 @@ SIGNATURES = {
     "stg_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "stg_set_lanes": (C.c_int, [C.c_void_p, C.c_int32]),
     "stg_get_lanes": (C.c_int, [C.c_void_p, _ip]),
+    "stg_set_slot_width": (C.c_int, [C.c_void_p, C.c_int32]),
+    "stg_get_slot_width": (C.c_int, [C.c_void_p, _ip]),
     "stg_set_specialize": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "stg_get_specialize": (C.c_int, [C.c_void_p, _ip]),
     "stg_set_diff_apply_threshold": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -95,6 +95,7 @@ struct stg_sim {
     int S = 0, R = 0, SS = 0, RS = 0, ND = 0;
     int block = 256;
     int lanes = 1;  // lanes per tet (tile schedule for volume models when > 1)
+    int slot_bytes = 2;  // face slot width: 2 (16-bit) or 1 (8-bit)
     bool use_graph = false;
     bool xcd_remap = true;
     int diff_threshold = 10;  // diffusion_operator.hpp:99, tetopsplit.hpp:995
@@ -138,7 +139,7 @@ struct stg_sim {
     uint32_t *d_nface = nullptr;
     double2 *d_g = nullptr;
     int32_t *d_counts = nullptr;
-    int16_t *d_out[2] = {nullptr, nullptr};   // 16-bit face slots (ping-pong)
+    int16_t *d_out[2] = {nullptr, nullptr};   // face slots (ping-pong), sized for 16-bit ones
     int32_t *d_wide[2] = {nullptr, nullptr};  // escaped slot counts
     int32_t *d_tri_ptr = nullptr;
     int32_t *d_tri_counts = nullptr;
@@ -193,10 +194,10 @@ struct stg_sim {
     unsigned char *h_stage = nullptr;
     size_t h_stage_cap = 0;
     // model-specialised volume kernel (blocks of up to 256), compiled at run time;
-    // one compiled module per (membrane variant, waves-per-SIMD bound; 0 = the
-    // compiler's choice, lanes per tet 1/2/4/8)
-    hipModule_t rtc_mods[2][9][4] = {};
-    hipFunction_t rtc_fns[2][9][4] = {};
+    // one compiled module per (membrane variant, waves-per-SIMD bound (0 = the
+    // compiler's choice), lanes per tet 1/2/4/8 x face slot width 2/1)
+    hipModule_t rtc_mods[2][9][8] = {};
+    hipFunction_t rtc_fns[2][9][8] = {};
     hipFunction_t rtc_fn = nullptr;  // the one in use, or null for the generic kernel
     bool rtc_memb = false;           // whether rtc_fn is the membrane variant
     int rtc_waves = 0;               // its waves-per-SIMD bound
@@ -1035,20 +1036,29 @@ bool needs_memb(const stg_sim *s) { return s->needs_memb(); }
 // index of a lanes-per-tet value (1, 2, 4, 8) in the compiled variants
 int lanes_index(int lanes) { return lanes == 1 ? 0 : lanes == 2 ? 1 : lanes == 4 ? 2 : 3; }
 
+// index of a (lanes per tet, slot width) pair among the specialised variants
+int variant_index(int lanes, int slot_bytes) { return lanes_index(lanes) + (slot_bytes == 1 ? 4 : 0); }
+
 // tets per block of the current launch shape
 int tets_per_block(const stg_sim *s) { return s->block / s->lanes; }
 
 using PeriodKernel = void (*)(stg::ModelK, stg::SysK, int);
 
-PeriodKernel period_kernel(const stg_sim *s, int B) {
-    const bool memb = needs_memb(s);
-    if (memb) return B <= 256 ? stg::k_period<256, true> : stg::k_period<1024, true>;
+template <class SlotT>
+PeriodKernel period_kernel_w(const stg_sim *s, int B) {
+    using stg::DynTables;
+    if (needs_memb(s))
+        return B <= 256 ? stg::k_period<256, true, DynTables, 1, SlotT> : stg::k_period<1024, true, DynTables, 1, SlotT>;
     static const PeriodKernel vol[2][4] = {
-        {stg::k_period<256, false, stg::DynTables, 1>, stg::k_period<256, false, stg::DynTables, 2>,
-         stg::k_period<256, false, stg::DynTables, 4>, stg::k_period<256, false, stg::DynTables, 8>},
-        {stg::k_period<1024, false, stg::DynTables, 1>, stg::k_period<1024, false, stg::DynTables, 2>,
-         stg::k_period<1024, false, stg::DynTables, 4>, stg::k_period<1024, false, stg::DynTables, 8>}};
+        {stg::k_period<256, false, DynTables, 1, SlotT>, stg::k_period<256, false, DynTables, 2, SlotT>,
+         stg::k_period<256, false, DynTables, 4, SlotT>, stg::k_period<256, false, DynTables, 8, SlotT>},
+        {stg::k_period<1024, false, DynTables, 1, SlotT>, stg::k_period<1024, false, DynTables, 2, SlotT>,
+         stg::k_period<1024, false, DynTables, 4, SlotT>, stg::k_period<1024, false, DynTables, 8, SlotT>}};
     return vol[B <= 256 ? 0 : 1][lanes_index(s->lanes)];
+}
+
+PeriodKernel period_kernel(const stg_sim *s, int B) {
+    return s->slot_bytes == 1 ? period_kernel_w<int8_t>(s, B) : period_kernel_w<int16_t>(s, B);
 }
 
 // ------------------------------------------------------------- specialisation
@@ -1115,7 +1125,7 @@ void specialize(stg_sim *s, int min_waves) {
     const int memb = s->needs_memb() ? 1 : 0;
     // the membrane variant has one lane per tet (a run that needs it goes back to one)
     const int lanes = memb ? 1 : s->lanes;
-    const int li = lanes_index(lanes);
+    const int li = variant_index(lanes, s->slot_bytes);
     if (s->rtc_fns[memb][min_waves][li]) {
         if (s->rtc_fn != s->rtc_fns[memb][min_waves][li]) s->free_graph();
         s->rtc_fn = s->rtc_fns[memb][min_waves][li];
@@ -1127,7 +1137,7 @@ void specialize(stg_sim *s, int min_waves) {
     const std::string rnd = rtc_join(kRtcRandom, sizeof(kRtcRandom) / sizeof(kRtcRandom[0]));
     const std::string lfact = rtc_join(kRtcLfact, sizeof(kRtcLfact) / sizeof(kRtcLfact[0]));
     const std::string name = std::string("stg::k_period<256, ") + (memb ? "true" : "false") + ", StgModelTables, " +
-                             std::to_string(lanes) + ">";
+                             std::to_string(lanes) + (s->slot_bytes == 1 ? ", int8_t>" : ", int16_t>");
     std::string src =
         "using __hip_internal::int8_t; using __hip_internal::uint8_t;\n"
         "using __hip_internal::int16_t; using __hip_internal::uint16_t;\n"
@@ -1255,8 +1265,14 @@ void launch_final_enter(stg_sim *s, int L) {
     k.n_blocks = nb;
     // one thread per tet of the block
     const int BT = tets_per_block(s);
-    if (needs_memb(s)) hipLaunchKernelGGL(stg::k_enter<true>, dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
-    else hipLaunchKernelGGL(stg::k_enter<false>, dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+    const bool memb = needs_memb(s);
+    if (s->slot_bytes == 1) {
+        if (memb) hipLaunchKernelGGL((stg::k_enter<true, int8_t>), dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+        else hipLaunchKernelGGL((stg::k_enter<false, int8_t>), dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+    } else {
+        if (memb) hipLaunchKernelGGL((stg::k_enter<true, int16_t>), dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+        else hipLaunchKernelGGL((stg::k_enter<false, int16_t>), dim3(nb), dim3(BT), 0, s->stream, s->mk, k);
+    }
     HIP_OK(hipGetLastError());
 }
 
@@ -1273,7 +1289,8 @@ void relayout(stg_sim *s, int block, int lanes) {
     }
     // a specialised volume kernel is compiled per lane count: switch to this
     // count's variant, or to the generic kernel until it is compiled
-    if (s->rtc_fn && !s->rtc_memb && lanes != s->lanes) s->rtc_fn = s->rtc_fns[0][s->rtc_waves][lanes_index(lanes)];
+    if (s->rtc_fn && !s->rtc_memb && lanes != s->lanes)
+        s->rtc_fn = s->rtc_fns[0][s->rtc_waves][variant_index(lanes, s->slot_bytes)];
     s->block = block;
     s->lanes = lanes;
     build_blocks(s, tets_per_block(s));
@@ -2061,6 +2078,27 @@ int stg_set_lanes(stg_sim *s, int32_t lanes) {
 int stg_get_lanes(const stg_sim *s, int32_t *lanes) {
     if (!s || !lanes) return STG_ERR_ARG;
     *lanes = s->lanes;
+    return STG_OK;
+}
+
+int stg_set_slot_width(stg_sim *s, int32_t bytes) {
+    if (!s) return STG_ERR_ARG;
+    return guard(s, [&] {
+        if (bytes != 1 && bytes != 2) throw StgError{STG_ERR_ARG, "face slot width must be 1 or 2 bytes"};
+        if (bytes == s->slot_bytes) return;
+        // slots live only within a run (its trailing entering pass empties
+        // them), so the width can change between runs
+        const bool spec_on = s->rtc_fn != nullptr;
+        s->slot_bytes = bytes;
+        s->rtc_fn = nullptr;
+        s->free_graph();
+        if (spec_on) specialize(s, s->rtc_waves);
+    });
+}
+
+int stg_get_slot_width(const stg_sim *s, int32_t *bytes) {
+    if (!s || !bytes) return STG_ERR_ARG;
+    *bytes = s->slot_bytes;
     return STG_OK;
 }
 

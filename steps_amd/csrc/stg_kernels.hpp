@@ -128,8 +128,9 @@ struct SysK {
     const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
     const double2 *g;            // [NT*2] face factors (f0,f1),(f2,f3)
     int32_t *counts;             // [S][NT]
-    const int16_t *out_prev;     // [ND][NT][4] molecules leaving through each face; a count
-    int16_t *out_cur;            // above kSlotMax is kSlotEscape here and lives in wide_*
+    const void *out_prev;        // [ND][NT][4] molecules leaving through each face, 8- or 16-bit
+    void *out_cur;               // (the kernel's SlotT); a count above the slot's maximum is
+                                 // its escape value here and lives in wide_*
     const int32_t *wide_prev;    // [ND][NT][4] escaped face counts (touched only on escape)
     int32_t *wide_cur;
     const int32_t *tri_ptr;      // [NT+1] or null when there are no triangles
@@ -198,27 +199,40 @@ constexpr int32_t kTabReac = 1, kTabDiff = 2, kTabClamp = 4, kTabTriReac = 8, kT
 // both structures travel in the 4 KiB kernel-argument segment
 static_assert(sizeof(ModelK) + sizeof(SysK) + 16 <= 4096, "kernel arguments exceed 4 KiB");
 
-// Face slots are 16-bit: a period's molecules through one face rarely exceed a
-// few hundred, and halving the slots halves the bytes of the only per-face
-// traffic (written by the owner, gathered by the neighbour). A count above
-// kSlotMax is stored as kSlotEscape with the count itself in a 32-bit side
-// table at the same index, which is read only when the escape is seen.
-constexpr int32_t kSlotMax = 32767;
-constexpr int32_t kSlotEscape = -32768;
+// Face slots are 8- or 16-bit (SlotT = int8_t or int16_t, chosen per engine):
+// a period's molecules through one face are mostly a few, and narrow slots cut
+// the only per-face traffic (written by the owner, gathered by the neighbour).
+// A count above slot_max<SlotT> is stored as slot_escape<SlotT> with the count
+// itself in a 32-bit side table at the same index, which is read only when the
+// escape is seen. 8-bit slots escape above 127 molecules per face and period,
+// 16-bit ones above 32,767: the narrower the slots, the fewer bytes while
+// escapes are rare.
+template <class SlotT>
+constexpr int32_t slot_max = (1 << (8 * sizeof(SlotT) - 1)) - 1;
+template <class SlotT>
+constexpr int32_t slot_escape = -(1 << (8 * sizeof(SlotT) - 1));
 
-__device__ __forceinline__ void put_slots(int16_t *slots, int32_t *wide, size_t base, int4 o) {
+// the four slots of one (tet, diffusing species), one store
+template <class SlotT>
+__device__ __forceinline__ void put_slots(void *slots, int32_t *wide, size_t base, int4 o) {
+    constexpr int bits = 8 * (int)sizeof(SlotT);
+    constexpr uint32_t mask = (1u << bits) - 1u;
     const int32_t v[4] = {o.x, o.y, o.z, o.w};
     uint32_t h[4];
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
-        if (v[f] > kSlotMax) {
+        if (v[f] > slot_max<SlotT>) {
             wide[base + f] = v[f];
-            h[f] = (uint32_t)(uint16_t)(int16_t)kSlotEscape;
+            h[f] = (uint32_t)slot_escape<SlotT> & mask;
         } else {
-            h[f] = (uint32_t)(uint16_t)(int16_t)v[f];
+            h[f] = (uint32_t)v[f] & mask;
         }
     }
-    *reinterpret_cast<uint2 *>(slots + base) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    SlotT *p = static_cast<SlotT *>(slots) + base;
+    if constexpr (sizeof(SlotT) == 2)
+        *reinterpret_cast<uint2 *>(p) = make_uint2(h[0] | (h[1] << 16), h[2] | (h[3] << 16));
+    else
+        *reinterpret_cast<uint32_t *>(p) = h[0] | (h[1] << 8) | (h[2] << 16) | (h[3] << 24);
 }
 
 // c = k scale^-(order-1) (comp_ccst, src/steps/mpi/tetopsplit/reac.cpp:44-55),
@@ -1053,6 +1067,7 @@ __device__ double grp_tri_prop(const ModelK &m, const SysK &sys, int tri, int t,
 }
 
 // One period of group g (see above); returns through nev / moved.
+template <class SlotT>
 __device__ __forceinline__ void group_period(const ModelK &m, const SysK &sys, int g, int launch, bool do_enter, bool do_step,
                              long &nev_out, int32_t &moved_out) {
     const int NT = sys.n_tets, NTR = sys.n_tris, S = m.n_specs, R = m.n_reacs, RS = m.n_sreacs;
@@ -1072,8 +1087,8 @@ __device__ __forceinline__ void group_period(const ModelK &m, const SysK &sys, i
                 for (int f = 0; f < 4; ++f) {
                     if (nbv[f] < 0) continue;
                     const size_t ix = (size_t)d * NT * 4 + (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff);
-                    int32_t v = sys.out_prev[ix];
-                    if (v == kSlotEscape) v = sys.wide_prev[ix];
+                    int32_t v = static_cast<const SlotT *>(sys.out_prev)[ix];
+                    if (v == slot_escape<SlotT>) v = sys.wide_prev[ix];
                     add += v;
                 }
                 if (!((clampm >> sp) & 1u)) sys.counts[(size_t)sp * NT + t] += add;
@@ -1258,7 +1273,7 @@ __device__ __forceinline__ void group_period(const ModelK &m, const SysK &sys, i
                                      sys.real_D[(size_t)r * S + sp], r4, bdir, S, dtr, seed, ltet, per, stream, o, fD);
             if (nl > 0 && !((clampm >> sp) & 1u)) sys.counts[ci] = n - nl;
             moved += nl;
-            put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + t) * 4, o);
+            put_slots<SlotT>(sys.out_cur, sys.wide_cur, ((size_t)d * NT + t) * 4, o);
         }
         if (m.n_sdiff > 0) {
             int4 *toc = reinterpret_cast<int4 *>(sys.tout_cur);
@@ -1318,7 +1333,7 @@ __device__ __forceinline__ void block_extents(const SysK &sys, int slot, long ne
 // Philox channel, so the draws, and the results, are those of one lane per
 // tet; a wavefront holds 64 / G tets, which shortens the SSA's slowest lane
 // and splits the leaving pass over the species.
-template <int G, class MT>
+template <int G, class MT, class SlotT>
 __device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *smem, const int4 bd, int r, int launch,
                             bool do_enter, bool do_step, long &nev, int32_t &moved) {
     const int B = blockDim.x;
@@ -1352,14 +1367,14 @@ __device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *sme
             for (int f = 0; f < 4; ++f)
                 off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
             for (int d = sub; d < ND; d += G) {
-                const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
+                const SlotT *op = static_cast<const SlotT *>(sys.out_prev) + (size_t)d * NT * 4;
                 int32_t v[4];
 #pragma unroll
                 for (int f = 0; f < 4; ++f) v[f] = op[off[f]];
                 int32_t add = 0;
 #pragma unroll
                 for (int f = 0; f < 4; ++f) {
-                    if (v[f] == kSlotEscape) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
+                    if (v[f] == slot_escape<SlotT>) v[f] = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
                     add += nbv[f] >= 0 ? v[f] : 0;
                 }
                 L.c(tab_dspec<MT>(m, d)) += add;
@@ -1399,7 +1414,7 @@ __device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *sme
                                          0xffffffffu, S, dt, seed, ltet, period, stream, o);
                 if (nl > 0) L.c(sp) = n - nl;
                 moved += nl;
-                put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
+                put_slots<SlotT>(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
             }
         }
         __syncthreads();  // a species may be stored by another lane than the one that moved it
@@ -1408,7 +1423,7 @@ __device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *sme
         for (int s = sub; s < S; s += G) sys.counts[(size_t)s * NT + t] = L.c(s);
 }
 
-template <int MAXB, bool MEMB, class MT = DynTables, int G = 1>
+template <int MAXB, bool MEMB, class MT = DynTables, int G = 1, class SlotT = int16_t>
 __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL_WAVES_PER_EU)
     k_period(const ModelK m, const SysK sys, int launch) {
     static_assert(G == 1 || (!MEMB && G <= 8 && (G & (G - 1)) == 0), "lanes per tet: 1, or 2/4/8 for volume models");
@@ -1436,7 +1451,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
     long nev = 0;
     int32_t moved = 0;
     if constexpr (G > 1) {
-        period_tile<G, MT>(m, sys, smem, bd, r, launch, do_enter, do_step, nev, moved);
+        period_tile<G, MT, SlotT>(m, sys, smem, bd, r, launch, do_enter, do_step, nev, moved);
         block_extents(sys, bd.x, nev, moved);
         return;
     }
@@ -1469,7 +1484,8 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             const int grp = sys.tet_grp[t];
             if (grp >= 0) {
                 in_group = true;
-                if (sys.grp_mem[sys.grp_ptr[grp]] == t) group_period(m, sys, grp, launch, do_enter, do_step, nev, moved);
+                if (sys.grp_mem[sys.grp_ptr[grp]] == t)
+                    group_period<SlotT>(m, sys, grp, launch, do_enter, do_step, nev, moved);
             }
         }
     }
@@ -1501,7 +1517,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                 int32_t v[4][4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    const int16_t *op = sys.out_prev + (size_t)(d0 + j) * NT * 4;
+                    const SlotT *op = static_cast<const SlotT *>(sys.out_prev) + (size_t)(d0 + j) * NT * 4;
 #pragma unroll
                     for (int f = 0; f < 4; ++f) v[j][f] = d0 + j < ND ? (int32_t)op[off[f]] : 0;
                 }
@@ -1509,13 +1525,13 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
 #pragma unroll
                 for (int j = 0; j < 4; ++j)
 #pragma unroll
-                    for (int f = 0; f < 4; ++f) esc = esc || v[j][f] == kSlotEscape;
+                    for (int f = 0; f < 4; ++f) esc = esc || v[j][f] == slot_escape<SlotT>;
                 if (esc) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j)
 #pragma unroll
                         for (int f = 0; f < 4; ++f)
-                            if (v[j][f] == kSlotEscape) v[j][f] = sys.wide_prev[(size_t)(d0 + j) * NT * 4 + off[f]];
+                            if (v[j][f] == slot_escape<SlotT>) v[j][f] = sys.wide_prev[(size_t)(d0 + j) * NT * 4 + off[f]];
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -1598,7 +1614,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
                                              bdir, S, dt, seed, ltet, period, stream, o, fD);
                     if (nl > 0 && !(MEMB && ((clampm >> sp) & 1u))) L.c(sp) = n - nl;
                     moved += nl;
-                    put_slots(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
+                    put_slots<SlotT>(sys.out_cur, sys.wide_cur, ((size_t)d * NT + pin(t)) * 4, o);
                 }
             }
             if (MEMB && m.n_sdiff > 0) {
@@ -1629,7 +1645,7 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
 // last period was the previous launch (the first launch_blocks[L] block
 // descriptors). A kernel of its own so that every k_period dispatch of a
 // single-realization run is a full period.
-template <bool MEMB>
+template <bool MEMB, class SlotT>
 __global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) {
     const int4 bd = sys.block_desc[blockIdx.x];
     const int t = bd.z + (int)threadIdx.x;
@@ -1644,12 +1660,12 @@ __global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) 
 #pragma unroll
     for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
     for (int d = 0; d < m.n_diff; ++d) {
-        const int16_t *op = sys.out_prev + (size_t)d * NT * 4;
+        const SlotT *op = static_cast<const SlotT *>(sys.out_prev) + (size_t)d * NT * 4;
         int32_t add = 0;
 #pragma unroll
         for (int f = 0; f < 4; ++f) {
             int32_t v = op[off[f]];
-            if (v == kSlotEscape) v = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
+            if (v == slot_escape<SlotT>) v = sys.wide_prev[(size_t)d * NT * 4 + off[f]];
             add += nbv[f] >= 0 ? v : 0;
         }
         if (!((clampm >> m.dspec[d]) & 1u)) sys.counts[(size_t)m.dspec[d] * NT + t] += add;

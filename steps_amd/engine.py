@@ -251,6 +251,16 @@ class Engine:
         self._chk(self.lib.stg_get_lanes(self.h, C.byref(v)))
         return v.value
 
+    def set_slot_width(self, nbytes: int = 2):
+        """Face slot width: 2 (16-bit) or 1 (8-bit) bytes; identical results."""
+        self._chk(self.lib.stg_set_slot_width(self.h, int(nbytes)))
+
+    @property
+    def slot_width(self) -> int:
+        v = C.c_int32()
+        self._chk(self.lib.stg_get_slot_width(self.h, C.byref(v)))
+        return v.value
+
     def set_specialize(self, on: bool = True, min_waves: int = 0):
         """Use a period kernel compiled at run time for this model's volume
         reaction network (identical results; blocks up to 256).

@@ -252,3 +252,16 @@ def test_membrane_groups_with_tables_and_graph():
         eng.run(t)
         o.run(t, 0)
         assert np.array_equal(_gpu_counts(eng, S, 2), o.counts), t
+
+
+def test_membrane_between_compartments_with_8bit_slots():
+    """The SSA-group path with 8-bit face slots: identical to the oracle."""
+    eng, o, ct, S, SS, T = _membrane_engine_and_oracle(7, 0)
+    eng.set_slot_width(1)
+    print("case membrane between compartments, 8-bit slots", flush=True)
+    for t in (0.01, 0.04):
+        eng.run(t)
+        o.run(t, 0)
+        assert np.array_equal(_gpu_counts(eng, S, 2), o.counts), t
+        assert np.array_equal(np.stack([eng.get_tri_counts(q, 0) for q in range(SS)]),
+                              o.tri_counts[:SS, :T]), t

@@ -245,10 +245,11 @@ def test_autotune_keeps_results():
     res = autotune(eng, trial_periods=3, blocks=(64, 256), graphs=(False, True), spec_waves=(0, 8))
     # generic and model-specialised kernels (two register bounds), both blocks:
     # stream and graph with one lane per tet (12 shapes), stream with 2 and 4
-    # lanes per tet (6 shapes each)
-    assert res["block"] in (64, 256) and res["lanes"] in (1, 2, 4) and len(res["timings"]) == 24
+    # lanes per tet (6 shapes each), each with 16- and 8-bit face slots
+    assert res["block"] in (64, 256) and res["lanes"] in (1, 2, 4) and len(res["timings"]) == 48
     assert sum(k.endswith("/lanes4") for k in res["timings"]) == 6
-    assert eng.lanes == res["lanes"]
+    assert sum(k.endswith("/slot8") for k in res["timings"]) == 24
+    assert eng.lanes == res["lanes"] and eng.slot_width == res["slot_width"]
     assert eng.time() == 0.0
     eng.run(w.end_time)
     assert np.array_equal(_gpu_counts(eng, w), _gpu_counts(ref, w))
