@@ -135,8 +135,7 @@ struct stg_sim {
     double time = 0.0;
     // device
     double *d_vol = nullptr;
-    int4 *d_nbr = nullptr;
-    uint32_t *d_nface = nullptr;
+    int4 *d_nbr = nullptr;  // per face: neighbour's global tet x 4 + its face slot, -1 = none
     double2 *d_g = nullptr;
     int32_t *d_counts = nullptr;
     int16_t *d_out[2] = {nullptr, nullptr};   // face slots (ping-pong), sized for 16-bit ones
@@ -301,7 +300,7 @@ struct stg_sim {
             for (auto &row : plane)
                 for (hipModule_t m : row)
                     if (m) (void)hipModuleUnload(m);
-        void *ptrs[] = {d_vol,       d_nbr,         d_nface,       d_g,           d_counts,
+        void *ptrs[] = {d_vol,       d_nbr,         d_g,           d_counts,
                         d_out[0],    d_out[1],      d_wide[0],     d_wide[1],     d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_desc, d_real_tet0,  d_real_k,      d_real_sk,
                         d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
@@ -586,7 +585,6 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     // host staging of the concatenated mesh
     std::vector<double> vol(NT);
     std::vector<int4> nbr(NT);
-    std::vector<uint32_t> nface(NT);
     s->h_face_g.assign((size_t)NT * 4, 0.0);
     std::vector<int32_t> tri_ptr(NT + 1, 0);
     std::vector<double> tri_area(std::max(NTRI, 1));
@@ -605,7 +603,6 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
         for (int t = 0; t < m.n_tets; ++t) {
             vol[t0 + t] = m.tet_vol[t];
             int nb[4];
-            uint32_t nf = 0;
             for (int f = 0; f < 4; ++f) {
                 const int32_t j = m.tet_nbr[4 * t + f];
                 const int32_t jf = m.tet_nbr_face[4 * t + f];
@@ -629,12 +626,11 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
                     s->h_tet_bdir[t0 + t] &= ~(0xffu << (8 * f));
                     s->h_tet_bdir[t0 + t] |= (uint32_t)bd << (8 * f);
                 }
-                nb[f] = j >= 0 ? j + s->tet_off[jr] : -1;
-                nf |= (uint32_t)(j >= 0 ? jf : 0) << (8 * f);
+                // the entering gather reads the neighbour's slot for this face directly
+                nb[f] = j >= 0 ? 4 * (j + s->tet_off[jr]) + jf : -1;
                 s->h_face_g[4 * (size_t)(t0 + t) + f] = g;
             }
             nbr[t0 + t] = make_int4(nb[0], nb[1], nb[2], nb[3]);
-            nface[t0 + t] = nf;
         }
         // triangles sorted (stably) by inner tet
         std::vector<int32_t> order(m.n_tris);
@@ -816,7 +812,6 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     HIP_OK(hipEventCreate(&s->ev1));
     s->d_vol = dalloc<double>(NT);
     s->d_nbr = dalloc<int4>(NT);
-    s->d_nface = dalloc<uint32_t>(NT);
     s->d_g = dalloc<double2>((size_t)NT * 2);
     s->d_counts = dalloc<int32_t>((size_t)std::max(S, 1) * NT);
     s->d_out[0] = dalloc<int16_t>((size_t)std::max(s->ND, 1) * NT * 4);
@@ -860,7 +855,6 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     s->d_sum = dalloc<unsigned long long>(1);
     HIP_OK(hipMemcpy(s->d_vol, vol.data(), sizeof(double) * NT, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(s->d_nbr, nbr.data(), sizeof(int4) * NT, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(s->d_nface, nface.data(), sizeof(uint32_t) * NT, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(s->d_g, s->h_face_g.data(), sizeof(double) * 4 * (size_t)NT, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(s->d_tri_ptr, tri_ptr.data(), sizeof(int32_t) * (NT + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(s->d_tri_area, tri_area.data(), sizeof(double) * std::max(NTRI, 1), hipMemcpyHostToDevice));
@@ -969,7 +963,6 @@ stg::SysK make_sys(const stg_sim *s, int launch_parity) {
     k.k_max = std::max(1, s->R + s->max_tris * (s->RS + s->NG));
     k.vol = s->d_vol;
     k.nbr = s->d_nbr;
-    k.nface = s->d_nface;
     k.g = s->d_g;
     k.counts = s->d_counts;
     k.out_prev = s->d_out[launch_parity ^ 1];

@@ -124,8 +124,8 @@ struct SysK {
     int32_t diff_threshold;      // leaving counts up to this are split per molecule
     int32_t k_max;               // most reactions of one tet group (LDS rows per thread)
     const double *vol;           // [NT]
-    const int4 *nbr;             // [NT] global neighbour index, -1 = none
-    const uint32_t *nface;       // [NT] 4 x 8-bit face slot in the neighbour
+    const int4 *nbr;             // [NT] gather index of each face: the neighbour's global tet
+                                 // x 4 + this face's slot in it, -1 = no neighbour
     const double2 *g;            // [NT*2] face factors (f0,f1),(f2,f3)
     int32_t *counts;             // [S][NT]
     const void *out_prev;        // [ND][NT][4] molecules leaving through each face, 8- or 16-bit
@@ -1078,15 +1078,14 @@ __device__ __forceinline__ void group_period(const ModelK &m, const SysK &sys, i
             const int t = sys.grp_mem[k], r = sys.grp_real[k];
             const int rtab = real_tab(sys, r);
             const uint32_t clampm = tet_clamp_of(sys, t, rtab);
-            const int4 nb4 = sys.nbr[t];
-            const uint32_t nf = sys.nface[t];
-            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            const int4 gi4 = sys.nbr[t];
+            const int gi[4] = {gi4.x, gi4.y, gi4.z, gi4.w};
             for (int d = 0; d < m.n_diff; ++d) {
                 const int sp = m.dspec[d];
                 int32_t add = 0;
                 for (int f = 0; f < 4; ++f) {
-                    if (nbv[f] < 0) continue;
-                    const size_t ix = (size_t)d * NT * 4 + (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff);
+                    if (gi[f] < 0) continue;
+                    const size_t ix = (size_t)d * NT * 4 + (size_t)gi[f];
                     int32_t v = static_cast<const SlotT *>(sys.out_prev)[ix];
                     if (v == slot_escape<SlotT>) v = sys.wide_prev[ix];
                     add += v;
@@ -1359,13 +1358,11 @@ __device__ void period_tile(const ModelK &m, const SysK &sys, unsigned char *sme
             L.o(s) = 0.0;
         }
         if (do_enter) {
-            const int4 nb4 = sys.nbr[t];
-            const uint32_t nf = sys.nface[t];
-            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            const int4 gi4 = sys.nbr[t];
+            const int nbv[4] = {gi4.x, gi4.y, gi4.z, gi4.w};
             size_t off[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f)
-                off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
+            for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] : 0;
             for (int d = sub; d < ND; d += G) {
                 const SlotT *op = static_cast<const SlotT *>(sys.out_prev) + (size_t)d * NT * 4;
                 int32_t v[4];
@@ -1498,16 +1495,14 @@ __global__ void __launch_bounds__(MAXB, MEMB ? STG_PERIOD_WAVES_PER_EU : STG_VOL
             L.c(s) = sys.counts[(size_t)s * NT + t];
             L.o(s) = 0.0;
         }
-        const int4 nb4 = sys.nbr[t];
-        const uint32_t nf = sys.nface[t];
+        const int4 gi4 = sys.nbr[t];
         if (do_enter) {
             // every slot is read unconditionally (a missing neighbour reads
             // slot 0 and is masked) so the four loads are in flight together
-            const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+            const int nbv[4] = {gi4.x, gi4.y, gi4.z, gi4.w};
             size_t off[4];
 #pragma unroll
-            for (int f = 0; f < 4; ++f)
-                off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
+            for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] : 0;
             // the slots of up to four diffusing species are loaded together and
             // the (rare) escapes resolved once all of them have arrived: an
             // escape test per species would wait on that species' loads before
@@ -1653,12 +1648,11 @@ __global__ void __launch_bounds__(1024) k_enter(const ModelK m, const SysK sys) 
     const int NT = sys.n_tets;
     const int rtab = (MEMB && sys.real_tabs != nullptr) ? sys.real_tabs[bd.y] : 0;
     const uint32_t clampm = (rtab & kTabClamp) ? sys.tet_clamp[t] : 0u;
-    const int4 nb4 = sys.nbr[t];
-    const uint32_t nf = sys.nface[t];
-    const int nbv[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const int4 gi4 = sys.nbr[t];
+    const int nbv[4] = {gi4.x, gi4.y, gi4.z, gi4.w};
     size_t off[4];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] * 4 + ((nf >> (8 * f)) & 0xff) : 0;
+    for (int f = 0; f < 4; ++f) off[f] = nbv[f] >= 0 ? (size_t)nbv[f] : 0;
     for (int d = 0; d < m.n_diff; ++d) {
         const SlotT *op = static_cast<const SlotT *>(sys.out_prev) + (size_t)d * NT * 4;
         int32_t add = 0;
