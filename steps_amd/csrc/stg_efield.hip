@@ -83,7 +83,7 @@ __device__ __forceinline__ void block_partial(double v, double *partial) {
 // One workgroup sums the nb partials in a fixed order and applies `op`:
 //   0: scal[0] = sum, scal[1] = 0            (CG start: rz, beta)
 //   1: scal[3] = sum                          (p.q of this iteration)
-//   2: scal[1] = sum / scal[0], scal[0] = sum (new rz and beta)
+//   2: scal[1] = sum / scal[0], scal[0] = sum (new rz and beta), out[1] = sum
 //   3: out[0] = sum                           (a plain reduction)
 __global__ void __launch_bounds__(1024) k_reduce(const double *partial, int nb, int op, double *scal,
                                                  double *out) {
@@ -106,6 +106,7 @@ __global__ void __launch_bounds__(1024) k_reduce(const double *partial, int nb, 
         } else if (op == 2) {
             scal[1] = scal[0] > 0.0 ? t / scal[0] : 0.0;
             scal[0] = t;
+            out[1] = t;  // beside the start's rz in out[0]: one read-back for both
         } else {
             out[0] = t;
         }
@@ -1564,11 +1565,7 @@ int pcg_amg(stg_efield *e, double rel_tol, int max_iter) {
             }
             double rz = 0.0;
             if (it <= 2) {
-                EF_HIP(hipMemcpyAsync(e->h_fetch, e->outs, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-                EF_HIP(hipMemcpyAsync(e->h_fetch + 1, e->scal, sizeof(double), hipMemcpyDeviceToHost, e->stream));
-                EF_HIP(hipStreamSynchronize(e->stream));
-                rz0 = e->h_fetch[0];
-                rz = e->h_fetch[1];
+                fetch2(e, e->outs, rz0, rz);
                 if (!(rz0 > 0.0)) return 0;
             } else {
                 rz = fetch(e, e->scal);

@@ -147,7 +147,11 @@ struct stg_sim {
     int32_t *d_real_tet0 = nullptr;
     double *d_real_k = nullptr, *d_real_sk = nullptr, *d_real_D = nullptr;
     uint64_t *d_real_seed = nullptr;
-    uint32_t *d_real_stream = nullptr, *d_real_step0 = nullptr;
+    uint32_t *d_real_stream = nullptr;
+    // the run's tables in one allocation, uploaded with one copy:
+    // [periods per realization, int32][first Philox period, uint32][dt table, double]
+    double *d_run_tab = nullptr;
+    uint32_t *d_real_step0 = nullptr;  // views into d_run_tab
     int32_t *d_real_nper = nullptr;
     double *d_dt_tab = nullptr;
     size_t dt_tab_cap = 0;
@@ -208,6 +212,7 @@ struct stg_sim {
     // Launch order of the blocks: by periods in the current run, most first, so
     // the blocks still active at launch l are a prefix of launch_blocks[l].
     std::vector<int32_t> desc_np;        // periods per block the uploaded order was made for
+    std::vector<int32_t> desc_nper;      // periods per realization the order and launch_blocks were made for
     std::vector<int32_t> launch_blocks;  // [launches] blocks dispatched
     double last_ms = 0.0;
     int32_t last_launches = 0;
@@ -303,8 +308,8 @@ struct stg_sim {
         void *ptrs[] = {d_vol,       d_nbr,         d_g,           d_counts,
                         d_out[0],    d_out[1],      d_wide[0],     d_wide[1],     d_tri_ptr,     d_tri_counts,  d_tri_area,
                         d_block_desc, d_real_tet0,  d_real_k,      d_real_sk,
-                        d_real_D,    d_real_seed,   d_real_stream, d_real_step0,  d_real_nper,
-                        d_dt_tab,    d_ext,         d_err,         d_sum,         d_idx,
+                        d_real_D,    d_real_seed,   d_real_stream, d_run_tab,
+                        d_ext,       d_err,         d_sum,         d_idx,
                         d_gather,    d_tri_v,       d_vdep_vmin,   d_vdep_dv,     d_vdep_k,
                         d_vdep_off,  d_tri_intg,    d_tri_charge,  d_tet_bdir,    d_bnd_D,
                         d_tri_nbr,   d_tri_nface,   d_tri_g,       d_tri_user,    d_real_sD,
@@ -547,6 +552,22 @@ void upload_real_tables(stg_sim *s) {
         HIP_OK(hipMemcpyAsync(s->d_real_D, s->h_D.data(), sizeof(double) * s->h_D.size(),
                               hipMemcpyHostToDevice, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));
+}
+
+// Room in the run tables for a dt table of tab_len entries (periods x
+// realizations). A new allocation changes the tables' addresses, so a
+// captured launch sequence is dropped; no kernel is in flight between runs.
+void reserve_run_tab(stg_sim *s, size_t tab_len) {
+    if (s->d_run_tab && tab_len <= s->dt_tab_cap) return;
+    const size_t cap = std::max(tab_len * 2, (size_t)16 * s->n_real);
+    double *p = dalloc<double>((size_t)s->n_real + cap);  // 8 B per realization for the two int tables
+    if (s->d_run_tab) (void)hipFree(s->d_run_tab);
+    s->d_run_tab = p;
+    s->d_real_nper = reinterpret_cast<int32_t *>(p);
+    s->d_real_step0 = reinterpret_cast<uint32_t *>(s->d_real_nper + s->n_real);
+    s->d_dt_tab = p + s->n_real;
+    s->dt_tab_cap = cap;
+    s->free_graph();
 }
 
 void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_mesh *meshes,
@@ -849,8 +870,7 @@ void create_impl(stg_sim *s, const stg_model *model, int32_t n_real, const stg_m
     }
     s->d_real_seed = dalloc<uint64_t>(n_real);
     s->d_real_stream = dalloc<uint32_t>(n_real);
-    s->d_real_step0 = dalloc<uint32_t>(n_real);
-    s->d_real_nper = dalloc<int32_t>(n_real);
+    reserve_run_tab(s, (size_t)16 * n_real);
     s->d_err = dalloc<int32_t>(1);
     s->d_sum = dalloc<unsigned long long>(1);
     HIP_OK(hipMemcpy(s->d_vol, vol.data(), sizeof(double) * NT, hipMemcpyHostToDevice));
@@ -931,6 +951,11 @@ void build_blocks(stg_sim *s, int B) {
 // participants of every launch a prefix: launch l dispatches only those,
 // rather than every block of the ensemble with most returning at once.
 void order_blocks(stg_sim *s, const std::vector<int32_t> &nper, int L) {
+    // the order and the blocks of each launch follow from the blocks'
+    // realizations (fixed until the blocks are rebuilt, which clears desc_np)
+    // and the periods of each realization: runs of equal length reuse them
+    if (!s->desc_np.empty() && nper == s->desc_nper) return;
+    s->desc_nper.clear();  // set again once the order is in place
     const size_t nb = s->block_real.size();
     std::vector<int32_t> np(nb);
     for (size_t b = 0; b < nb; ++b) np[b] = nper[s->block_real[b]];
@@ -939,7 +964,10 @@ void order_blocks(stg_sim *s, const std::vector<int32_t> &nper, int L) {
     for (int v = L - 1; v >= 0; --v) at_least[v] += at_least[v + 1];
     s->launch_blocks.assign((size_t)L + 1, 0);
     for (int l = 0; l <= L; ++l) s->launch_blocks[l] = at_least[std::max(l, 1)];
-    if (np == s->desc_np) return;
+    if (np == s->desc_np) {
+        s->desc_nper = nper;
+        return;
+    }
     std::vector<int32_t> pos((size_t)L + 2, 0);  // first slot of each np value
     for (int v = L - 1; v >= 0; --v) pos[v] = at_least[v + 1];
     std::vector<int4> desc(nb);
@@ -950,6 +978,7 @@ void order_blocks(stg_sim *s, const std::vector<int32_t> &nper, int L) {
     HIP_OK(hipMemcpyAsync(s->d_block_desc, desc.data(), sizeof(int4) * nb, hipMemcpyHostToDevice, s->stream));
     HIP_OK(hipStreamSynchronize(s->stream));  // the host vector goes out of scope
     s->desc_np.swap(np);
+    s->desc_nper = nper;
 }
 
 stg::SysK make_sys(const stg_sim *s, int launch_parity) {
@@ -1335,16 +1364,10 @@ void run_impl(stg_sim *s, double endtime) {
             step0[r] = (uint32_t)s->rng_steps[r];
             for (size_t i = 0; i < dts[r].size(); ++i) tab[i * s->n_real + r] = dts[r][i];
         }
-        if (tab.size() > s->dt_tab_cap) {
-            if (s->d_dt_tab) (void)hipFree(s->d_dt_tab);
-            s->d_dt_tab = nullptr;
-            s->dt_tab_cap = tab.size() * 2;
-            s->d_dt_tab = dalloc<double>(s->dt_tab_cap);
-            s->free_graph();
-        }
-        // the three tables go through pinned staging, so the copies run
-        // asynchronously; the run ends with a stream synchronisation, so the
-        // staging is free again for the next run
+        reserve_run_tab(s, tab.size());
+        // the three tables go through pinned staging in the device layout, so
+        // one asynchronous copy moves them; the run ends with a stream
+        // synchronisation, so the staging is free again for the next run
         const size_t b_tab = sizeof(double) * tab.size(), b_int = sizeof(int32_t) * (size_t)s->n_real;
         const size_t need = b_tab + 2 * b_int;
         if (need > s->h_stage_cap) {
@@ -1354,12 +1377,10 @@ void run_impl(stg_sim *s, double endtime) {
             HIP_OK(hipHostMalloc((void **)&s->h_stage, 2 * need, hipHostMallocDefault));
             s->h_stage_cap = 2 * need;
         }
-        std::memcpy(s->h_stage, tab.data(), b_tab);
-        std::memcpy(s->h_stage + b_tab, nper.data(), b_int);
-        std::memcpy(s->h_stage + b_tab + b_int, step0.data(), b_int);
-        HIP_OK(hipMemcpyAsync(s->d_dt_tab, s->h_stage, b_tab, hipMemcpyHostToDevice, s->stream));
-        HIP_OK(hipMemcpyAsync(s->d_real_nper, s->h_stage + b_tab, b_int, hipMemcpyHostToDevice, s->stream));
-        HIP_OK(hipMemcpyAsync(s->d_real_step0, s->h_stage + b_tab + b_int, b_int, hipMemcpyHostToDevice, s->stream));
+        std::memcpy(s->h_stage, nper.data(), b_int);
+        std::memcpy(s->h_stage + b_int, step0.data(), b_int);
+        std::memcpy(s->h_stage + 2 * b_int, tab.data(), b_tab);
+        HIP_OK(hipMemcpyAsync(s->d_run_tab, s->h_stage, need, hipMemcpyHostToDevice, s->stream));
         order_blocks(s, nper, L);
         // the LDS footprint can grow after stg_set_launch checked it (an
         // EField asking for channel integrals switches to the membrane variant)
