@@ -139,7 +139,9 @@ typedef struct stg_realization {
 
 /* Engine for one realization: replaces the TetOpSplitP constructor
  * (src/steps/mpi/tetopsplit/tetopsplit.cpp:83-118, pysteps/cysteps_mpi.pyx:80-109)
- * and the DistTetOpSplit constructor (pysteps/cysteps_dist.pyx:1581-1633). */
+ * and the DistTetOpSplit constructor (pysteps/cysteps_dist.pyx:1581-1633).
+ * At most 536,870,911 tets over all realizations of a handle (each face's
+ * gather index, tet x 4 + face, is an int32); more returns STG_ERR_ARG. */
 int stg_create(const stg_model *model, const stg_mesh *mesh, uint64_t seed, int32_t mode,
                stg_sim **out);
 
