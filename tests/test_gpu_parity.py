@@ -156,6 +156,34 @@ def test_batch_matches_independent_runs():
         assert eng.nsteps(r) == o.nsteps
 
 
+def test_batch_repeated_and_changing_run_lengths():
+    """Runs of equal length reuse the uploaded block order and the blocks of
+    each launch; a run of another length rebuilds them; a long run grows the
+    run tables (dt per period and realization). Every realization stays
+    bit-exact against its own oracle run through the same saves."""
+    fm, meshes, counts, reac_k, info = W.ab_binding_ensemble(n_real=4, n_min=3, n_max=6,
+                                                             side=4e-6, n_a=800, n_b=600, seed=5)
+    print("case batch of 4, saves of changing length", flush=True)
+    eng = Engine(fm, meshes, [23] * 4, reac_k=reac_k)
+    oracles = []
+    for r in range(4):
+        for s in range(3):
+            eng.set_tet_counts(s, counts[r][s], real=r)
+        o = bind.OracleSim(fm, meshes[r], seed=23, stream=r, reac_k=reac_k[r])
+        o.counts[:] = counts[r]
+        oracles.append(o)
+    # one period per save up to 12 ms (periods of 13-53 ms are clipped at the
+    # save), 1-4 periods to 60 ms, and up to 26 periods to 400 ms, beyond the
+    # run tables' first allocation (16 periods per realization)
+    for t in (0.001, 0.002, 0.003, 0.010, 0.011, 0.012, 0.060, 0.061, 0.400, 0.401):
+        eng.run(t)
+        for r, o in enumerate(oracles):
+            o.run(t, 0)
+            got = np.stack([eng.get_tet_counts(s, r) for s in range(3)])
+            assert np.array_equal(got, o.counts), f"t={t} realization {r}"
+            assert eng.nsteps(r) == o.nsteps, f"t={t} realization {r}"
+
+
 @pytest.mark.parametrize("block,graph", [(64, False), (1024, False), (256, True)])
 def test_launch_shape_invariance(block, graph):
     w = W.four_species_cube(n=5, side=5e-6, seed=8, end_time=0.03)
